@@ -1,0 +1,182 @@
+#!/usr/bin/env python3
+"""Benchmark of the TLFEA solve path on one MI355X.
+
+Workload (single large mesh): cantilever 1.0 x 0.1 x 0.1 m, 200x20x20 hex cells x 6 =
+480,000 TET10 elements, 674,081 nodes, 2,022,243 DOF; E = 2.1e11 Pa, nu = 0.3,
+rho = 7850; nodes at x = 0 fixed; 1e5 N in -z spread over the x = L nodes.
+
+One step = one linear-static solve to a relative residual of 1e-8: tangent stiffness
+assembly at the current positions, Dirichlet elimination, CG with the reference's
+sqrt(rr/rr0) test, position update (the reference's LinearStaticSolver::Solve).  Inputs are
+resident in HBM when the timed region starts.  `value` = solves/s.
+
+Kernel rooflines are measured live with HIP events on the launch stream (the null stream,
+which torch.cuda.Event records on):
+  * roofline      -- the step's dominant kernel, the block SpMV of CG,
+                     bytes/launch = 76*node_nnz + 52*n_nodes;
+  * force_kernel  -- the fused FP64 P + f_int kernel,
+                     bytes/launch = 1280*n_elem + 48*n_nodes (the reference byte model).
+cpu_baseline: the FP64 oracle on a bounded sample (a slice of the element assembly and
+10 CG iterations on the full matrix), extrapolated to one solve with the GPU's iteration
+count.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cells", type=str, default="200,20,20")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from tlfea_amd import capi, mesh as M
+    from tlfea_amd.fea import Body, LinearStatic
+
+    capi.lib()
+    nx, ny, nz = (int(v) for v in args.cells.split(","))
+    E, nu, rho = 2.1e11, 0.3, 7850.0
+    t0 = time.time()
+    m = M.clamp_and_tip_load(M.kuhn_beam(nx, ny, nz), -1.0e5)
+    log(f"mesh {m.n_elem} el / {m.n_nodes} nodes in {time.time() - t0:.1f}s")
+    body = Body(m, E, nu, rho)
+    node_nnz, dof_nnz, nb = body.pattern_sizes()
+    ls = LinearStatic(body, tol=args.tol, max_iter=200000, check_every=64)
+    X0 = [t.clone() for t in body.x]
+    torch.cuda.synchronize()
+
+    def step():
+        for d in range(3):
+            body.x[d].copy_(X0[d])
+        rc = ls.solve()
+        if rc != 0:
+            raise RuntimeError(f"solve did not converge: rc={rc} iters={ls.iters} res={ls.res}")
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    iters = ls.iters
+    ms_per_step = dt / args.steps * 1e3
+    log(f"solve: {ms_per_step:.1f} ms, {iters} CG iterations, rel res {ls.res:.3e}")
+
+    # ---- kernel rooflines (HIP events on the null stream) ---------------------------
+    R = args.kernel_reps
+    p = torch.randn(3 * m.n_nodes, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(p)
+    for _ in range(3):
+        body.spmv(ls.k, p, y)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(R):
+        body.spmv(ls.k, p, y)
+    e1.record()
+    torch.cuda.synchronize()
+    spmv_s = e0.elapsed_time(e1) / 1e3 / R
+    # algorithmic bytes: 3x3 block values (72 B) + column index (4 B) per stored block, x read
+    # and y written once (48 B per node), row id per node (4 B)
+    spmv_bytes = 76 * node_nnz + 52 * m.n_nodes
+    spmv_gbs = spmv_bytes / spmv_s / 1e9
+
+    f = torch.zeros(3 * m.n_nodes, dtype=torch.float64, device="cuda")
+    for _ in range(3):
+        body.internal_force(f=f, accumulate=True)
+    e0.record()
+    for _ in range(R):
+        body.internal_force(f=f, accumulate=True)
+    e1.record()
+    torch.cuda.synchronize()
+    force_s = e0.elapsed_time(e1) / 1e3 / R
+    force_bytes = 1280 * m.n_elem + 48 * m.n_nodes
+    force_gbs = force_bytes / force_s / 1e9
+    log(f"spmv {spmv_s * 1e6:.1f} us {spmv_gbs:.0f} GB/s; force {force_s * 1e6:.1f} us {force_gbs:.0f} GB/s")
+
+    # ---- CPU baseline: oracle on a bounded sample -------------------------------------
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        ncores = min(os.cpu_count() or 1, 16)
+        O.set_num_threads(ncores)
+        go, djo = O.grad(m)
+        lam, mu = E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
+        # assembly sample: a slice of the elements on a per-slice pattern
+        frac_el = max(1, m.n_elem // 40)
+        sub = M.TetMesh(m.x, m.y, m.z, m.elems[:frac_el])
+        soff, scol = O.dof_pattern(sub)
+        sg, sdj = go[: frac_el * 150], djo[: frac_el * 5]
+        ta = time.perf_counter()
+        O.stiffness_values(sub, sg, sdj, lam, mu, soff, scol)
+        t_asm = (time.perf_counter() - ta) * (m.n_elem / frac_el)
+        # CG iteration sample on the full matrix (values from the GPU's CSR copy)
+        doff_t, dcol_t = body.dof_csr()
+        vals = body.stiffness_csr_values(ls.k).cpu().numpy()
+        doff_n, dcol_n = doff_t.cpu().numpy(), dcol_t.cpu().numpy()
+        b = np.random.default_rng(0).standard_normal(3 * m.n_nodes)
+        tc = time.perf_counter()
+        O.cg(doff_n, dcol_n, vals, b, tol=0.0, max_iter=10)
+        t_it = (time.perf_counter() - tc) / 10
+        cpu_solve_s = t_asm + iters * t_it
+        cpu = {"value": 1.0 / cpu_solve_s, "unit": "solves/s", "cores": ncores, "kind": "port",
+               "sample": f"oracle stiffness assembly of {frac_el} of {m.n_elem} elements (1 thread) + 10 CG "
+                         f"iterations on the full {dof_nnz}-nnz CSR ({ncores} OpenMP threads), extrapolated to "
+                         f"{iters} iterations: {t_asm:.1f} s assembly + {t_it * 1e3:.1f} ms/iteration"}
+        log(f"cpu baseline: {cpu_solve_s:.1f} s per solve")
+
+    out = {
+        "metric": "TET10 solves/s at 1e-8 rel. residual (learned+FP32 vs plain); force-kernel GB/s",
+        "value": args.steps / dt,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"plain FP64 linear-static CG solve, cantilever {nx}x{ny}x{nz} cells x 6 TET10 "
+                               f"({m.n_elem} el, {m.n_nodes} nodes, {3 * m.n_nodes} DOF), tol {args.tol:g}",
+                   "n_elem": m.n_elem, "n_dof": 3 * m.n_nodes, "cg_iterations": iters,
+                   "final_rel_residual": ls.res, "K_nnz": dof_nnz},
+        "roofline": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": None, "kernel": "k_cg_spmv/k_spmv_block",
+                     "avg_us": spmv_s * 1e6, "bytes_per_launch": spmv_bytes},
+        "force_kernel": {"bound": "hbm", "achieved": force_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": force_gbs / HBM_PEAK_GBS, "kernel": "k_force<10,5,false>",
+                         "avg_us": force_s * 1e6, "bytes_per_launch": force_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,177 @@
+/*
+ * tlfea_hip.h -- C-ABI of the MI355X (gfx950) TLFEA solve-path library (libtlfea_hip.so).
+ *
+ * Plain C: POD structs, device pointers, sizes, hipStream_t (passed as void*), int status.
+ * The C++ drop-in classes (include/tlfea/) are the only intended callers; tests and the
+ * benchmark also bind it through ctypes.  Nothing here allocates inside a hot call: the
+ * calls that need persistent device state (patterns, CG workspaces) return opaque handles
+ * from setup calls.
+ *
+ * Device data layouts (all FP64 unless noted):
+ *   conn      int32, column-major conn[e + a*n_elem]         (the reference's MatrixXi Map)
+ *   x, y, z   SoA node positions
+ *   f, v, u   AoS per DOF [3*node + d]
+ *   grad_t    AoSoA tiles of 64 elements: grad_t[((tile*nq + q)*nn*3 + a + nn*d)*64 + lane],
+ *             element e = tile*64 + lane, entry = dN_a/dX_d at quadrature point q
+ *   detJ_t    detJ_t[(tile*nq + q)*64 + lane]
+ *   qp tensors (F, P, Fdot, P_vis): qp_t[((tile*nq + q)*9 + i + 3*j)*64 + lane]
+ * Padded lanes (e >= n_elem) hold zeros.  tlfea_padded_elems() gives the padded count.
+ */
+#ifndef TLFEA_HIP_H
+#define TLFEA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TLFEA_OK 0
+#define TLFEA_EINVAL (-1)    /* bad argument (size, null pointer, unsupported element) */
+#define TLFEA_EHIP (-2)      /* HIP runtime error */
+#define TLFEA_ENOTPD (-3)    /* CG met p.Kp <= 0 (LinearStaticSolver.cu:404-407)      */
+#define TLFEA_ECAPACITY (-4) /* a setup limit (row length, index range) was exceeded */
+#define TLFEA_ENOCONV (-5)   /* iteration limit reached without convergence          */
+
+#define TLFEA_TILE 64 /* elements per AoSoA tile = one wavefront */
+
+/* Quadrature rule, passed by value into kernels.  n = 5 (TET10 Keast) or 1 (TET4). */
+typedef struct tlfea_quad {
+    int n;
+    double x[5], y[5], z[5], w[5];
+} tlfea_quad;
+
+/* Material: SVK (lambda, mu) plus Kelvin-Voigt damping (eta, lambda_d). */
+typedef struct tlfea_material {
+    double lambda, mu, eta_damp, lambda_damp;
+} tlfea_material;
+
+/* ---- general ---------------------------------------------------------------------- */
+const char* tlfea_version(void);
+const char* tlfea_last_error(void);
+int tlfea_device_synchronize(void);
+int64_t tlfea_padded_elems(int n_elem);
+/* Replaces the reference's Quadrature::tet5pt_* / tet1pt_* tables
+ * (src/utils/quadrature_utils.h:125-179). */
+int tlfea_quadrature(int nn, tlfea_quad* out);
+
+/* ---- element precompute (setup) ---------------------------------------------------- */
+/* Replaces dn_du_pre_kernel / GPU_FEAT10_Data::CalcDnDuPre (src/elements/FEAT10Data.cu:92-198,
+ * 277-285) and dn_du_pre_feat4_kernel (src/elements/FEAT4Data.cu:98-190).  nn = 10 or 4. */
+int tlfea_precompute_grad(int nn, int n_elem, const int* conn, const double* x, const double* y, const double* z,
+                          const tlfea_quad* quad, double* grad_t, double* detJ_t, void* stream);
+
+/* ---- stress and internal force ---------------------------------------------------- */
+/* Replaces calc_p_kernel / compute_p (src/elements/FEAT10Data.cu:287-305,
+ * src/elements/FEAT10DataFunc.cuh:86-291; TET4 src/elements/FEAT4DataFunc.cuh:33-200):
+ * stores F, P = P_svk + P_vis, Fdot, P_vis at every quadrature point (any output may be
+ * NULL).  v == NULL (or zero damping) stores Fdot = P_vis = 0. */
+int tlfea_calc_p(int nn, int n_elem, const int* conn, const double* x, const double* y, const double* z,
+                 const double* grad_t, const tlfea_material* mat, const double* v, double* F_t, double* P_t,
+                 double* Fdot_t, double* Pvis_t, void* stream);
+
+/* Replaces compute_internal_force_kernel / CalcInternalForce
+ * (src/elements/FEAT10Data.cu:307-323, src/elements/FEAT10DataFunc.cuh:404-460):
+ * f[3*node+d] += sum_q P_q g_a(q) detJ_q w_q.  Accumulates onto f, as the reference does. */
+int tlfea_fint_from_p(int nn, int n_elem, const int* conn, const double* P_t, const double* grad_t,
+                      const double* detJ_t, const tlfea_quad* quad, double* f, void* stream);
+
+/* Fused P + f_int (the solver loops' force evaluation, src/solvers/SyncedNesterov.cu:180-207):
+ * computes the SVK (+ Kelvin-Voigt when v != NULL) stress per quadrature point in registers
+ * and scatters the element nodal forces; nothing per quadrature point is stored.
+ * accumulate == 0 zeroes f first. */
+int tlfea_internal_force(int nn, int n_elem, int n_nodes, const int* conn, const double* x, const double* y,
+                         const double* z, const double* grad_t, const double* detJ_t, const tlfea_quad* quad,
+                         const tlfea_material* mat, const double* v, double* f, int accumulate, void* stream);
+
+/* FP32 variant of the fused force kernel, displacement form F = I + sum_a u_a (x) g_a:
+ * u = x - X is passed as three float arrays, grad32_t / dV32_t are float copies of grad_t
+ * and detJ_t*w (tlfea_grad_to_f32).  f is float. */
+int tlfea_grad_to_f32(int nn, int n_elem, const double* grad_t, const double* detJ_t, const tlfea_quad* quad,
+                      float* grad32_t, float* dV32_t, void* stream);
+int tlfea_internal_force_f32(int nn, int n_elem, int n_nodes, const int* conn, const float* ux, const float* uy,
+                             const float* uz, const float* grad32_t, const float* dV32_t, const tlfea_material* mat,
+                             float* f, int accumulate, void* stream);
+
+/* Layout converters for the drop-in Retrieve*ToCPU calls: AoSoA -> the reference's layouts
+ * grad[(e*nq+q)*nn*3 + a + nn*d], detJ[e*nq+q], qp[(e*nq+q)*9 + i + 3j]. */
+int tlfea_grad_to_ref(int nn, int n_elem, const double* grad_t, const double* detJ_t, double* grad_ref,
+                      double* detJ_ref, void* stream);
+int tlfea_qp_to_ref(int nn, int n_elem, const double* qp_t, double* qp_ref, void* stream);
+
+/* ---- sparsity patterns (setup; opaque handle owns device memory) ------------------ */
+typedef struct tlfea_pattern tlfea_pattern;
+/* Builds, on the GPU from element->node incidence (no key sort), the node-level pattern
+ * of src/elements/FEAT10Data.cu:34-74,361-417 and the DOF-level pattern of
+ * src/solvers/LinearStaticSolver.cu:75-122,240-294 (= node pattern expanded 3x3), plus the
+ * element -> slot maps used for scatter-free-of-search assembly and the sliced (64-row)
+ * block layout used by the SpMV. */
+int tlfea_pattern_create(int nn, int n_elem, int n_nodes, const int* conn, tlfea_pattern** out, void* stream);
+int tlfea_pattern_destroy(tlfea_pattern* p);
+/* node_nnz = mass-matrix nnz, dof_nnz = 9*node_nnz = stiffness nnz, sell_blocks = padded
+ * block count of the sliced layout (values per matrix: 9*sell_blocks for K, sell_blocks
+ * for M). */
+int tlfea_pattern_sizes(const tlfea_pattern* p, int64_t* node_nnz, int64_t* dof_nnz, int64_t* sell_blocks);
+/* Exact reference CSR patterns (device outputs, int32). */
+int tlfea_pattern_node_csr(const tlfea_pattern* p, int* offsets, int* cols, void* stream);
+int tlfea_pattern_dof_csr(const tlfea_pattern* p, int* offsets, int* cols, void* stream);
+
+/* ---- mass and stiffness assembly ------------------------------------------------- */
+/* Replaces mass_matrix_qp_kernel / CalcMassMatrix (src/elements/FEAT10Data.cu:200-271,
+ * 341-359; TET4 src/elements/FEAT4Data.cu:197-243).  Writes the sliced-layout values
+ * m_sell[sell_blocks] (zeroed first). */
+int tlfea_mass_assemble(const tlfea_pattern* p, int nn, int n_elem, const int* conn, const double* detJ_t,
+                        const tlfea_quad* quad, double rho, double* m_sell, void* stream);
+/* Sliced-layout node matrix -> values in the node CSR order of tlfea_pattern_node_csr. */
+int tlfea_mass_to_csr(const tlfea_pattern* p, const double* m_sell, double* vals_csr, void* stream);
+/* Replaces assemble_stiffness_kernel / compute_hessian_assemble_csr
+ * (src/solvers/LinearStaticSolver.cu:128-141,300-310; src/elements/FEAT10DataFunc.cuh:512-768;
+ * src/materials/SVK.cuh:41-67): K = h * K_svk(x) + C_kv(x), written into the sliced 3x3
+ * block layout k_sell[9*sell_blocks] (zeroed first). */
+int tlfea_stiffness_assemble(const tlfea_pattern* p, int nn, int n_elem, const int* conn, const double* x,
+                             const double* y, const double* z, const double* grad_t, const double* detJ_t,
+                             const tlfea_quad* quad, const tlfea_material* mat, double h, double* k_sell,
+                             void* stream);
+/* Sliced block matrix -> values in the DOF CSR order of tlfea_pattern_dof_csr. */
+int tlfea_stiffness_to_csr(const tlfea_pattern* p, const double* k_sell, double* vals_csr, void* stream);
+/* Replaces mark_fixed_dofs_kernel + apply_bcs_kernel (src/solvers/LinearStaticSolver.cu:
+ * 147-184,316-342): fixed DOF rows become identity rows with f = 0, free rows get the fixed
+ * columns zeroed.  fixed_nodes: device int32[n_fixed]. */
+int tlfea_apply_dirichlet(const tlfea_pattern* p, int n_fixed, const int* fixed_nodes, double* k_sell, double* f,
+                          void* stream);
+
+/* ---- sparse / dense kernels -------------------------------------------------------- */
+/* y = K x on the sliced 3x3 block layout (replaces cusparseSpMV, LinearStaticSolver.cu:398). */
+int tlfea_spmv_block(const tlfea_pattern* p, const double* k_sell, const double* x, double* y, void* stream);
+/* y = M x on the node matrix applied per component (x, y AoS 3 per node). */
+int tlfea_spmv_node3(const tlfea_pattern* p, const double* m_sell, const double* x, double* y, void* stream);
+/* Plain scalar CSR SpMV (the reference's matrix format), y = A x. */
+int tlfea_spmv_csr(int n_rows, const int* offsets, const int* cols, const double* vals, const double* x, double* y,
+                   void* stream);
+/* BLAS-1 with device-resident results (replaces cublasDdot/Daxpy/Dscal/Dnrm2,
+ * LinearStaticSolver.cu:379-428, SyncedAdamWNocoop.cu:34).  Reductions are two-level and
+ * deterministic.  `scratch` must hold tlfea_reduce_scratch_size(n) doubles. */
+int64_t tlfea_reduce_scratch_size(int64_t n);
+int tlfea_dot(int64_t n, const double* a, const double* b, double* out, double* scratch, void* stream);
+int tlfea_nrm2(int64_t n, const double* a, double* out, double* scratch, void* stream);
+int tlfea_axpy(int64_t n, double alpha, const double* x, double* y, void* stream);
+int tlfea_scal(int64_t n, double alpha, double* x, void* stream);
+
+/* ---- CG (replaces LinearStaticSolver::SolveLinearSystemCG, LinearStaticSolver.cu:348-443) */
+typedef struct tlfea_cg tlfea_cg;
+int tlfea_cg_create(const tlfea_pattern* p, tlfea_cg** out);
+int tlfea_cg_destroy(tlfea_cg* c);
+/* Unpreconditioned CG from u = 0, stop when sqrt(rr/rr0) < tol (the reference's rule),
+ * TLFEA_ENOTPD on p.Kp <= 0.  Scalars stay on the device; the host reads the convergence
+ * flag every `check_every` iterations only. */
+int tlfea_cg_solve(tlfea_cg* c, const double* k_sell, const double* f, double* u, double tol, int max_iter,
+                   int check_every, int* iters_out, double* rel_res_out, void* stream);
+
+/* ---- position update (LinearStaticSolver.cu:189-196,449-463) ----------------------- */
+int tlfea_add_displacement(int n_nodes, const double* u, double* x, double* y, double* z, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TLFEA_HIP_H */

@@ -1,0 +1,149 @@
+"""GPU parity of the element kernels against the FP64 oracle.
+
+Tolerances: 1e-12 relative (norm-wise) for gradients, det J, F, P and internal forces
+(the FP64 path's stated bar); 1e-5 relative for the FP32 displacement-form kernel.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import E_STEEL, NU_STEEL, RHO_STEEL, deformed, lame, load_mesh, rel_err
+
+pytestmark = pytest.mark.gpu
+
+MESHES = ["cube", "beam_vtu", "C1", "beam_highres_t10", "beam_highres_t4", "kuhn_t4"]
+
+
+def _body(m, E=E_STEEL, nu=NU_STEEL, rho=RHO_STEEL, eta=0.0, lambda_d=0.0):
+    from tlfea_amd.fea import Body
+    return Body(m, E, nu, rho, eta, lambda_d)
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.mark.parametrize("name", MESHES)
+def test_precompute_grad(gpu, name):
+    print(f"[precompute] {name}", flush=True)
+    m = load_mesh(name)
+    b = _body(m)
+    g, dj = b.grad_ref()
+    go, djo = O.grad(m)
+    assert rel_err(_np(dj), djo) < 1e-13
+    assert rel_err(_np(g), go) < 1e-12
+
+
+@pytest.mark.parametrize("name", MESHES)
+def test_calc_p_and_fint_from_p(gpu, name):
+    import torch
+    print(f"[calc_p] {name}", flush=True)
+    m = load_mesh(name)
+    b = _body(m)
+    x, y, z = deformed(m, seed=1)
+    for i, a in enumerate((x, y, z)):
+        b.x[i].copy_(torch.as_tensor(a))
+    F, P, Fd, Pv = b.calc_p()
+    go, djo = O.grad(m)
+    lam, mu = lame(E_STEEL, NU_STEEL)
+    Fo, Po, _, _ = O.calc_p(m, go, lam, mu, x, y, z)
+    assert rel_err(_np(b.qp_ref(F)), Fo) < 1e-12
+    assert rel_err(_np(b.qp_ref(P)), Po) < 1e-12
+    assert np.all(_np(b.qp_ref(Fd)) == 0) and np.all(_np(b.qp_ref(Pv)) == 0)
+    # CalcInternalForce accumulates (FEAT10Data.cu:318-323 never clears f_int)
+    b.f_int.zero_()
+    b.fint_from_p(P)
+    f1 = _np(b.f_int).copy()
+    fo = O.fint_from_p(m, Po, go, djo)
+    assert rel_err(f1, fo) < 1e-12
+    b.fint_from_p(P)
+    assert rel_err(_np(b.f_int), 2 * fo) < 1e-12
+
+
+@pytest.mark.parametrize("name", MESHES)
+@pytest.mark.parametrize("damp", [False, True])
+def test_fused_internal_force(gpu, name, damp):
+    import torch
+    print(f"[fused force] {name} damp={damp}", flush=True)
+    m = load_mesh(name)
+    eta, ld = (1e6, 1e5) if damp else (0.0, 0.0)
+    b = _body(m, eta=eta, lambda_d=ld)
+    x, y, z = deformed(m, seed=2)
+    xt = [torch.as_tensor(a).cuda() for a in (x, y, z)]
+    v = np.random.default_rng(3).standard_normal(3 * m.n_nodes) * 1e-2
+    vt = torch.as_tensor(v).cuda()
+    f = b.internal_force(x=xt, v=vt if damp else None)
+    go, djo = O.grad(m)
+    lam, mu = lame(E_STEEL, NU_STEEL)
+    fo = O.internal_force(m, go, djo, lam, mu, x, y, z, v=v if damp else None, eta=eta, lambda_d=ld)
+    assert rel_err(_np(f), fo) < 1e-12
+    # accumulate flag adds onto the existing vector
+    b.internal_force(x=xt, v=vt if damp else None, f=f, accumulate=True)
+    assert rel_err(_np(f), 2 * fo) < 1e-12
+
+
+def test_fused_force_rest_state_is_zero(gpu):
+    m = load_mesh("C1")
+    b = _body(m)
+    f = _np(b.internal_force())
+    # at the reference configuration P = 0 up to rounding: |f| << E * h^2 * 1e-12 scale
+    assert np.abs(f).max() < E_STEEL * 0.05 ** 2 * 1e-12
+
+
+def test_fused_force_c2_full_size(gpu):
+    """Full benchmark mesh (480,000 TET10): fused kernel against the oracle."""
+    import torch
+    print("[fused force] C2", flush=True)
+    m = load_mesh("C2")
+    b = _body(m)
+    x, y, z = deformed(m, seed=4, amp=2e-4)
+    xt = [torch.as_tensor(a).cuda() for a in (x, y, z)]
+    f = _np(b.internal_force(x=xt))
+    go, djo = O.grad(m)
+    lam, mu = lame(E_STEEL, NU_STEEL)
+    fo = O.internal_force(m, go, djo, lam, mu, x, y, z)
+    assert rel_err(f, fo) < 1e-12
+    # equilibrium of internal forces: sum over nodes vanishes (translation invariance)
+    assert np.abs(f.reshape(-1, 3).sum(0)).max() < 1e-9 * np.abs(f).max() * np.sqrt(m.n_nodes)
+
+
+@pytest.mark.parametrize("name", ["C1", "beam_highres_t10", "kuhn_t4"])
+def test_fp32_displacement_form(gpu, name):
+    """FP32 kernel in displacement form against the FP64 oracle on a small-strain state."""
+    import torch
+    from tlfea_amd import capi
+    from tlfea_amd.capi import ptr
+    import ctypes as C
+    m = load_mesh(name)
+    b = _body(m)
+    rng = np.random.default_rng(5)
+    L = np.ptp(m.xyz(), axis=0).max()
+    u = 1e-4 * L * rng.standard_normal((m.n_nodes, 3))
+    x, y, z = m.x + u[:, 0], m.y + u[:, 1], m.z + u[:, 2]
+    g32 = torch.zeros(b.grad_t.numel(), dtype=torch.float32, device="cuda")
+    dv32 = torch.zeros(b.detJ_t.numel(), dtype=torch.float32, device="cuda")
+    capi.call("tlfea_grad_to_f32", b.nn, b.n_elem, ptr(b.grad_t), ptr(b.detJ_t), C.byref(b.quad), ptr(g32),
+              ptr(dv32), None)
+    ut = [torch.as_tensor(u[:, d].astype(np.float32)).cuda() for d in range(3)]
+    f32 = torch.zeros(3 * m.n_nodes, dtype=torch.float32, device="cuda")
+    capi.call("tlfea_internal_force_f32", b.nn, b.n_elem, b.n_nodes, ptr(b.conn), ptr(ut[0]), ptr(ut[1]),
+              ptr(ut[2]), ptr(g32), ptr(dv32), C.byref(b.mat), ptr(f32), 0, None)
+    go, djo = O.grad(m)
+    lam, mu = lame(E_STEEL, NU_STEEL)
+    fo = O.internal_force(m, go, djo, lam, mu, x, y, z)
+    assert rel_err(_np(f32).astype(np.float64), fo) < 1e-5
+
+
+def test_invalid_arguments_raise(gpu):
+    from tlfea_amd import capi
+    import ctypes as C
+    q = capi.quadrature(10)
+    with pytest.raises(capi.TlfeaError):
+        capi.call("tlfea_precompute_grad", 7, 10, None, None, None, None, C.byref(q), None, None, None)
+    with pytest.raises(capi.TlfeaError):
+        capi.call("tlfea_precompute_grad", 10, 10, None, None, None, None, C.byref(q), None, None, None)
+    q4 = capi.quadrature(4)
+    with pytest.raises(capi.TlfeaError):  # quadrature/element mismatch
+        capi.call("tlfea_precompute_grad", 10, 0, None, None, None, None, C.byref(q4), None, None, None)

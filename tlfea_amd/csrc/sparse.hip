@@ -1,0 +1,1109 @@
+// Sparsity patterns, mass / stiffness assembly, Dirichlet elimination, SpMV, BLAS-1 and CG.
+//
+// Storage.  The node-level pattern (row i = every node sharing an element with node i,
+// ascending, diagonal included) is the reference's mass pattern; the DOF-level stiffness
+// pattern is that pattern expanded 3x3, so its CSR is derived and never sorted.  Matrix
+// values live in a sliced block layout: node rows are sorted by length (descending)
+// inside windows of WINDOW rows and cut into slices of 64 rows; slice s holds
+// len_s = max row length in the slice block-columns, stored column-major so that lane l
+// of a wave reads block k of its row at [base_s + k*64 + l] -- every value load of the
+// SpMV is one contiguous 512-byte segment.  K stores 9 values per block
+// (k_sell[9*base_s + (k*9 + 3*d + c)*64 + l]); M stores one.  Padding blocks hold zeros
+// and point at the row itself.  Each element (a,b) pair maps to its block through a
+// precomputed slot index, so assembly needs no search.
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "common.h"
+
+namespace tlfea_hip {
+
+constexpr int WINDOW = 8192;  // rows sorted by length inside this window
+constexpr int MAXROW = 256;   // max distinct nodes per pattern row (setup limit)
+constexpr int RED_BLOCKS = 512;
+constexpr int RED_THREADS = 256;
+
+}  // namespace tlfea_hip
+
+struct tlfea_pattern {
+    int nn = 0, n_elem = 0, n_nodes = 0;
+    int64_t nnz = 0;          // node-level nnz
+    int64_t n_slices = 0;     // 64-row slices
+    int64_t sell_blocks = 0;  // padded blocks
+    int* d_off = nullptr;     // node CSR offsets [n_nodes+1]
+    int* d_col = nullptr;     // node CSR cols [nnz]
+    int* d_slot = nullptr;    // per (e,a,b): block index in the sliced layout [n_elem*nn*nn], (e-major)
+    int* d_sell_row = nullptr;    // node id per slice lane [n_slices*64], -1 = padding lane
+    int* d_sell_col = nullptr;    // block column node [sell_blocks]
+    int64_t* d_slice_base = nullptr;  // [n_slices+1]
+    int* d_slice_len = nullptr;       // [n_slices]
+    int* d_row_pos = nullptr;         // node -> permuted position [n_nodes]
+    unsigned char* d_mask = nullptr;  // DOF mask scratch for Dirichlet [3*n_nodes]
+};
+
+struct tlfea_cg {
+    const tlfea_pattern* p = nullptr;
+    int64_t n = 0;
+    double* d_r = nullptr;
+    double* d_p = nullptr;
+    double* d_Kp = nullptr;
+    double* d_part = nullptr;  // [RED_BLOCKS]
+    double* d_scal = nullptr;  // [8]: rr0, rr, alpha, beta, rel, pKp
+    int* d_flags = nullptr;    // [4]: 0 done, 1 notpd, 2 iters, 3 block counter
+};
+
+namespace tlfea_hip {
+
+// ------------------------------------------------------------------------------------
+// Pattern construction
+// ------------------------------------------------------------------------------------
+template <int NN>
+__global__ void k_count_incidence(int n_elem, const int* __restrict__ conn, int* __restrict__ cnt) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n_elem * NN) return;
+    atomicAdd(&cnt[conn[t]], 1);  // conn is column-major: t = e + a*n_elem
+}
+
+template <int NN>
+__global__ void k_fill_incidence(int n_elem, const int* __restrict__ conn, int* __restrict__ cursor,
+                                 int* __restrict__ inc) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n_elem * NN) return;
+    const int e = (int)(t % n_elem);
+    const int pos = atomicAdd(&cursor[conn[t]], 1);
+    inc[pos] = e;
+}
+
+// One lane per node: sorted, de-duplicated union of the nodes of its incident elements,
+// kept in the lane's LDS segment by insertion.  mode 0 writes the row length, mode 1 the
+// columns.
+template <int NN>
+__global__ __launch_bounds__(64) void k_build_rows(int n_nodes, int n_elem, const int* __restrict__ conn,
+                                                   const int* __restrict__ inc_off, const int* __restrict__ inc,
+                                                   int mode, int* __restrict__ rowlen, const int* __restrict__ off,
+                                                   int* __restrict__ cols, int* __restrict__ overflow) {
+    __shared__ int buf[64 * MAXROW];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_nodes) return;
+    int* row = buf + threadIdx.x;  // strided by 64 to spread LDS banks
+    int len = 0;
+    bool over = false;
+    for (int k = inc_off[i]; k < inc_off[i + 1]; k++) {
+        const int e = inc[k];
+        for (int a = 0; a < NN; a++) {
+            const int c = conn[e + (int64_t)a * n_elem];
+            // binary search for insertion point
+            int lo = 0, hi = len;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (row[mid * 64] < c)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            if (lo < len && row[lo * 64] == c) continue;
+            if (len == MAXROW) {
+                over = true;
+                continue;
+            }
+            for (int m = len; m > lo; m--) row[m * 64] = row[(m - 1) * 64];
+            row[lo * 64] = c;
+            len++;
+        }
+    }
+    if (over) atomicOr(overflow, 1);
+    if (mode == 0) {
+        rowlen[i] = len;
+    } else {
+        const int o = off[i];
+        for (int m = 0; m < len; m++) cols[o + m] = row[m * 64];
+    }
+}
+
+// Slot map: block index in the sliced layout for every (e, a, b).
+template <int NN>
+__global__ void k_slot_map(int n_elem, const int* __restrict__ conn, const int* __restrict__ off,
+                           const int* __restrict__ cols, const int* __restrict__ row_pos,
+                           const int64_t* __restrict__ slice_base, int* __restrict__ slot, int* __restrict__ err) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n_elem * NN * NN) return;
+    const int e = (int)(t / (NN * NN));
+    const int ab = (int)(t % (NN * NN));
+    const int a = ab / NN, b = ab % NN;
+    const int r = conn[e + (int64_t)a * n_elem];
+    const int c = conn[e + (int64_t)b * n_elem];
+    int lo = off[r], hi = off[r + 1];
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cols[mid] < c)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    if (lo >= off[r + 1] || cols[lo] != c) {
+        atomicOr(err, 2);
+        slot[t] = 0;
+        return;
+    }
+    const int s = lo - off[r];
+    const int pos = row_pos[r];
+    const int64_t blk = slice_base[pos >> 6] + (int64_t)s * TILE + (pos & (TILE - 1));
+    slot[t] = (int)blk;
+}
+
+__global__ void k_fill_sell_cols(int64_t n_slices, const int* __restrict__ sell_row, const int* __restrict__ off,
+                                 const int* __restrict__ cols, const int64_t* __restrict__ slice_base,
+                                 const int* __restrict__ slice_len, int* __restrict__ sell_col) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_slices * TILE) return;
+    const int64_t sl = t >> 6;
+    const int lane = (int)(t & (TILE - 1));
+    const int row = sell_row[t];
+    const int L = slice_len[sl];
+    const int64_t base = slice_base[sl];
+    const int o = row >= 0 ? off[row] : 0;
+    const int len = row >= 0 ? off[row + 1] - o : 0;
+    const int self = row >= 0 ? row : 0;
+    for (int k = 0; k < L; k++) sell_col[base + (int64_t)k * TILE + lane] = (k < len) ? cols[o + k] : self;
+}
+
+__global__ void k_dof_csr(int n_nodes, const int* __restrict__ off, const int* __restrict__ cols,
+                          int* __restrict__ doff, int* __restrict__ dcol) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * (int64_t)n_nodes) return;
+    const int i = (int)(t / 3), d = (int)(t % 3);
+    const int o = off[i], len = off[i + 1] - o;
+    const int64_t ro = 9 * (int64_t)o + 3 * (int64_t)d * len;
+    doff[t] = (int)ro;
+    if (t == 3 * (int64_t)n_nodes - 1) doff[t + 1] = 9 * off[n_nodes];
+    for (int m = 0; m < len; m++) {
+        const int c = cols[o + m];
+        dcol[ro + 3 * m + 0] = 3 * c;
+        dcol[ro + 3 * m + 1] = 3 * c + 1;
+        dcol[ro + 3 * m + 2] = 3 * c + 2;
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// Mass: one lane per (e, a, b), M_ab = sum_q rho N_a N_b detJ_q w_q (FEAT10Data.cu:200-271).
+// ------------------------------------------------------------------------------------
+template <int NN, int NQ>
+__global__ void k_mass(int n_elem, const int* __restrict__ slot, const double* __restrict__ detJ_t, tlfea_quad quad,
+                       double rho, double* __restrict__ m_sell) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n_elem * NN * NN) return;
+    const int e = (int)(t / (NN * NN));
+    const int ab = (int)(t % (NN * NN));
+    const int a = ab / NN, b = ab % NN;
+    const int tile = e >> 6, lane = e & (TILE - 1);
+    double m = 0.0;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const double L[4] = {1.0 - quad.x[q] - quad.y[q] - quad.z[q], quad.x[q], quad.y[q], quad.z[q]};
+        double Na, Nb;
+        if constexpr (NN == 10) {
+            constexpr int E[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
+            Na = a < 4 ? L[a] * (2.0 * L[a] - 1.0) : 4.0 * L[E[a - 4][0]] * L[E[a - 4][1]];
+            Nb = b < 4 ? L[b] * (2.0 * L[b] - 1.0) : 4.0 * L[E[b - 4][0]] * L[E[b - 4][1]];
+        } else {
+            Na = L[a];
+            Nb = L[b];
+        }
+        m += rho * Na * Nb * detJ_t[((int64_t)tile * NQ + q) * TILE + lane] * quad.w[q];
+    }
+    unsafeAtomicAdd(&m_sell[slot[t]], m);
+}
+
+// ------------------------------------------------------------------------------------
+// Tangent stiffness.  A block of 64 lanes handles one element tile at one quadrature point
+// per pass: each lane computes F, F F^T, tr E and Fh_a = F g_a of its element into LDS,
+// then the 64 lanes sweep the element tile's (a,b) pairs, each producing the 3x3 block
+//   K_ab = (lambda Fh_a (x) Fh_b + lambda trE (g_a.g_b) I + mu (Fh_b.Fh_a) I
+//           + mu Fh_b (x) Fh_a + mu (g_a.g_b) F F^T - mu (g_a.g_b) I) dV
+// (src/materials/SVK.cuh:41-67 at src/elements/FEAT10DataFunc.cuh:512-684) plus the
+// Kelvin-Voigt block
+//   C_ab = (eta Fh_b (x) Fh_a + eta (g_a.g_b) F F^T + lambda_d Fh_a (x) Fh_b) dV
+// (FEAT10DataFunc.cuh:686-767), scattered with atomics through the slot map.
+// ------------------------------------------------------------------------------------
+template <int NN, int NQ>
+__global__ __launch_bounds__(64) void k_stiffness(int n_elem, const int* __restrict__ conn,
+                                                  const double* __restrict__ x, const double* __restrict__ y,
+                                                  const double* __restrict__ z, const double* __restrict__ grad_t,
+                                                  const double* __restrict__ detJ_t, tlfea_quad quad,
+                                                  tlfea_material mat, double h, const int* __restrict__ slot,
+                                                  double* __restrict__ k_sell) {
+    __shared__ double sg[NN][3][TILE];
+    __shared__ double sFh[NN][3][TILE];
+    __shared__ double sFFt[6][TILE];
+    __shared__ double strE[TILE], sdV[TILE];
+    const int tile = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int e = tile * TILE + lane;
+    const int n_valid = min(TILE, n_elem - tile * TILE);
+    const bool damp = (mat.eta_damp != 0.0 || mat.lambda_damp != 0.0);
+    for (int q = 0; q < NQ; q++) {
+        if (lane < n_valid) {
+            const double* G = grad_t + ((int64_t)tile * NQ + q) * (NN * 3 * TILE) + lane;
+            double F[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+            double g[NN][3];
+#pragma unroll
+            for (int a = 0; a < NN; a++) {
+                const int n = conn[e + (int64_t)a * n_elem];
+                const double xa[3] = {x[n], y[n], z[n]};
+#pragma unroll
+                for (int d = 0; d < 3; d++) g[a][d] = G[(a + NN * d) * TILE];
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++) F[i][j] += xa[i] * g[a][j];
+            }
+            double C00 = 0, C11 = 0, C22 = 0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                C00 += F[k][0] * F[k][0];
+                C11 += F[k][1] * F[k][1];
+                C22 += F[k][2] * F[k][2];
+            }
+            strE[lane] = 0.5 * (C00 + C11 + C22 - 3.0);
+            int idx = 0;
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = i; j < 3; j++) sFFt[idx++][lane] = F[i][0] * F[j][0] + F[i][1] * F[j][1] + F[i][2] * F[j][2];
+#pragma unroll
+            for (int a = 0; a < NN; a++)
+#pragma unroll
+                for (int r = 0; r < 3; r++) {
+                    sg[a][r][lane] = g[a][r];
+                    sFh[a][r][lane] = F[r][0] * g[a][0] + F[r][1] * g[a][1] + F[r][2] * g[a][2];
+                }
+            sdV[lane] = detJ_t[((int64_t)tile * NQ + q) * TILE + lane] * quad.w[q];
+        }
+        __syncthreads();
+        // sweep (element, a, b) triples of the tile
+        const int total = n_valid * NN * NN;
+        for (int t = lane; t < total; t += TILE) {
+            const int el = t % n_valid;  // consecutive lanes -> different elements (LDS banks)
+            const int ab = t / n_valid;
+            const int a = ab / NN, b = ab % NN;
+            const double ga[3] = {sg[a][0][el], sg[a][1][el], sg[a][2][el]};
+            const double gb[3] = {sg[b][0][el], sg[b][1][el], sg[b][2][el]};
+            const double Fa[3] = {sFh[a][0][el], sFh[a][1][el], sFh[a][2][el]};
+            const double Fb[3] = {sFh[b][0][el], sFh[b][1][el], sFh[b][2][el]};
+            const double ff[3][3] = {{sFFt[0][el], sFFt[1][el], sFFt[2][el]},
+                                     {sFFt[1][el], sFFt[3][el], sFFt[4][el]},
+                                     {sFFt[2][el], sFFt[4][el], sFFt[5][el]}};
+            const double hab = gb[0] * ga[0] + gb[1] * ga[1] + gb[2] * ga[2];
+            const double FbFa = Fb[0] * Fa[0] + Fb[1] * Fa[1] + Fb[2] * Fa[2];
+            const double trE = strE[el], dV = sdV[el];
+            const int64_t blk = slot[((int64_t)(tile * TILE + el)) * NN * NN + ab];
+            const int64_t kb = 9 * (blk & ~(int64_t)(TILE - 1)) + (blk & (TILE - 1));
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const double dl = (d == c) ? 1.0 : 0.0;
+                    double K = (mat.lambda * Fa[d] * Fb[c] + mat.lambda * trE * hab * dl + mat.mu * FbFa * dl +
+                                mat.mu * Fb[d] * Fa[c] + mat.mu * hab * ff[d][c] - mat.mu * hab * dl) *
+                               dV;
+                    K *= h;
+                    if (damp)
+                        K += (mat.eta_damp * (Fb[d] * Fa[c]) + mat.eta_damp * ff[d][c] * hab +
+                              mat.lambda_damp * (Fa[d] * Fb[c])) *
+                             dV;
+                    unsafeAtomicAdd(&k_sell[kb + (3 * d + c) * TILE], K);
+                }
+        }
+        __syncthreads();
+    }
+}
+
+// Sliced -> CSR value orders.
+__global__ void k_sell_to_csr(int64_t n_slices, int nvals, const int* __restrict__ sell_row,
+                              const int64_t* __restrict__ slice_base, const int* __restrict__ off,
+                              const double* __restrict__ sell, double* __restrict__ csr) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_slices * TILE) return;
+    const int row = sell_row[t];
+    if (row < 0) return;
+    const int64_t sl = t >> 6;
+    const int lane = (int)(t & (TILE - 1));
+    const int64_t base = slice_base[sl];
+    const int o = off[row], len = off[row + 1] - o;
+    if (nvals == 1) {
+        for (int k = 0; k < len; k++) csr[o + k] = sell[base + (int64_t)k * TILE + lane];
+    } else {
+        // DOF row 3*row+d holds [block k][c] at 9*o + 3*d*len + 3*k + c
+        for (int k = 0; k < len; k++)
+            for (int d = 0; d < 3; d++)
+                for (int c = 0; c < 3; c++)
+                    csr[9 * (int64_t)o + 3 * (int64_t)d * len + 3 * k + c] =
+                        sell[9 * base + ((int64_t)k * 9 + 3 * d + c) * TILE + lane];
+    }
+}
+
+__global__ void k_mark_fixed(int n_fixed, const int* __restrict__ fixed, unsigned char* __restrict__ mask) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_fixed) return;
+    const int n = fixed[t];
+    mask[3 * n] = mask[3 * n + 1] = mask[3 * n + 2] = 1;
+}
+
+__global__ void k_dirichlet(int64_t n_slices, const int* __restrict__ sell_row, const int64_t* __restrict__ slice_base,
+                            const int* __restrict__ slice_len, const int* __restrict__ sell_col,
+                            const unsigned char* __restrict__ mask, double* __restrict__ k_sell,
+                            double* __restrict__ f) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_slices * TILE) return;
+    const int row = sell_row[t];
+    if (row < 0) return;
+    const int64_t sl = t >> 6;
+    const int lane = (int)(t & (TILE - 1));
+    const int64_t base = slice_base[sl];
+    const int L = slice_len[sl];
+    bool rf[3];
+    for (int d = 0; d < 3; d++) {
+        rf[d] = mask[3 * row + d] != 0;
+        if (rf[d] && f) f[3 * row + d] = 0.0;
+    }
+    for (int k = 0; k < L; k++) {
+        const int col = sell_col[base + (int64_t)k * TILE + lane];
+        for (int d = 0; d < 3; d++)
+            for (int c = 0; c < 3; c++) {
+                double* v = &k_sell[9 * base + ((int64_t)k * 9 + 3 * d + c) * TILE + lane];
+                if (rf[d]) {
+                    // identity row (only the true diagonal entry; padding blocks are zero and
+                    // also point at the row, so restrict to k's real diagonal below)
+                    *v = 0.0;
+                } else if (mask[3 * col + c]) {
+                    *v = 0.0;
+                }
+            }
+    }
+    // set the diagonal of fixed rows: the diagonal block is the real block whose column is
+    // the row itself; padding blocks (k >= true length) also carry col == row, so locate
+    // the first one, which is the real diagonal (real blocks precede padding).
+    if (rf[0] || rf[1] || rf[2]) {
+        for (int k = 0; k < L; k++) {
+            if (sell_col[base + (int64_t)k * TILE + lane] == row) {
+                for (int d = 0; d < 3; d++)
+                    if (rf[d]) k_sell[9 * base + ((int64_t)k * 9 + 4 * d) * TILE + lane] = 1.0;
+                break;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// SpMV on the sliced block layout: one lane per node row, 3 outputs.
+// ------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_spmv_block(int64_t n_slices, const int* __restrict__ sell_row,
+                                                    const int64_t* __restrict__ slice_base,
+                                                    const int* __restrict__ slice_len,
+                                                    const int* __restrict__ sell_col,
+                                                    const double* __restrict__ K, const double* __restrict__ xv,
+                                                    double* __restrict__ yv) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_slices * TILE) return;
+    const int row = sell_row[t];
+    const int64_t sl = t >> 6;
+    const int lane = (int)(t & (TILE - 1));
+    const int64_t base = slice_base[sl];
+    const int L = slice_len[sl];
+    const double* Kp = K + 9 * base + lane;
+    const int* cp = sell_col + base + lane;
+    double y0 = 0, y1 = 0, y2 = 0;
+    for (int k = 0; k < L; k++) {
+        const int col = cp[(int64_t)k * TILE];
+        const double x0 = xv[3 * col], x1 = xv[3 * col + 1], x2 = xv[3 * col + 2];
+        const double* v = Kp + (int64_t)k * 9 * TILE;
+        y0 += v[0 * TILE] * x0 + v[1 * TILE] * x1 + v[2 * TILE] * x2;
+        y1 += v[3 * TILE] * x0 + v[4 * TILE] * x1 + v[5 * TILE] * x2;
+        y2 += v[6 * TILE] * x0 + v[7 * TILE] * x1 + v[8 * TILE] * x2;
+    }
+    if (row >= 0) {
+        yv[3 * row] = y0;
+        yv[3 * row + 1] = y1;
+        yv[3 * row + 2] = y2;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_spmv_node3(int64_t n_slices, const int* __restrict__ sell_row,
+                                                    const int64_t* __restrict__ slice_base,
+                                                    const int* __restrict__ slice_len,
+                                                    const int* __restrict__ sell_col,
+                                                    const double* __restrict__ M, const double* __restrict__ xv,
+                                                    double* __restrict__ yv) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_slices * TILE) return;
+    const int row = sell_row[t];
+    const int64_t sl = t >> 6;
+    const int lane = (int)(t & (TILE - 1));
+    const int64_t base = slice_base[sl];
+    const int L = slice_len[sl];
+    double y0 = 0, y1 = 0, y2 = 0;
+    for (int k = 0; k < L; k++) {
+        const int col = sell_col[base + (int64_t)k * TILE + lane];
+        const double m = M[base + (int64_t)k * TILE + lane];
+        y0 += m * xv[3 * col];
+        y1 += m * xv[3 * col + 1];
+        y2 += m * xv[3 * col + 2];
+    }
+    if (row >= 0) {
+        yv[3 * row] = y0;
+        yv[3 * row + 1] = y1;
+        yv[3 * row + 2] = y2;
+    }
+}
+
+__global__ void k_spmv_csr(int n_rows, const int* __restrict__ off, const int* __restrict__ cols,
+                           const double* __restrict__ vals, const double* __restrict__ xv, double* __restrict__ yv) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_rows) return;
+    double s = 0.0;
+    for (int k = off[r]; k < off[r + 1]; k++) s += vals[k] * xv[cols[k]];
+    yv[r] = s;
+}
+
+// ------------------------------------------------------------------------------------
+// Reductions: block partials (wave shuffles + LDS), then one block combines them in a
+// fixed order -> deterministic.
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double red[RED_THREADS / 64];
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < (int)(blockDim.x >> 6); k++) s += red[k];
+    __syncthreads();
+    return s;  // valid in thread 0
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_dot_partial(int64_t n, const double* __restrict__ a,
+                                                             const double* __restrict__ b,
+                                                             double* __restrict__ part) {
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        s += a[i] * b[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_final(int nparts, const double* __restrict__ part,
+                                                              double* __restrict__ out, int take_sqrt) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += part[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) *out = take_sqrt ? sqrt(s) : s;
+}
+
+__global__ void k_axpy(int64_t n, double alpha, const double* __restrict__ xv, double* __restrict__ yv) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) yv[i] += alpha * xv[i];
+}
+
+__global__ void k_scal(int64_t n, double alpha, double* __restrict__ xv) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) xv[i] *= alpha;
+}
+
+__global__ void k_add_disp(int n, const double* __restrict__ u, double* __restrict__ x, double* __restrict__ y,
+                           double* __restrict__ z) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x[i] += u[3 * i];
+    y[i] += u[3 * i + 1];
+    z[i] += u[3 * i + 2];
+}
+
+// ------------------------------------------------------------------------------------
+// CG kernels with device-resident scalars.  The last block of each reduction kernel
+// (found with an atomic ticket after a release fence) combines the partials in block
+// order and computes the step scalars, so an iteration is three launches.
+// scal: 0 rr0, 1 rr, 2 alpha, 3 beta, 4 rel, 5 pKp;  flags: 0 done, 1 notpd, 2 iters, 3 ticket
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ bool last_block(int* ticket) {
+    __shared__ bool is_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int t = atomicAdd(ticket, 1);
+        is_last = (t == (int)gridDim.x - 1);
+    }
+    __syncthreads();
+    if (is_last) __threadfence();
+    return is_last;
+}
+
+__device__ __forceinline__ double sum_parts(const double* part, int n) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += __builtin_nontemporal_load(part + i);
+    return block_sum(s);
+}
+
+// r = p = f, u = 0, rr0 = rr = f.f
+__global__ __launch_bounds__(RED_THREADS) void k_cg_init(int64_t n, const double* __restrict__ f,
+                                                         double* __restrict__ u, double* __restrict__ r,
+                                                         double* __restrict__ p, double* __restrict__ part,
+                                                         double* __restrict__ scal, int* __restrict__ flags) {
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double fi = f[i];
+        u[i] = 0.0;
+        r[i] = fi;
+        p[i] = fi;
+        s += fi * fi;
+    }
+    s = block_sum(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+    if (last_block(&flags[3])) {
+        const double rr = sum_parts(part, gridDim.x);
+        if (threadIdx.x == 0) {
+            scal[0] = rr;
+            scal[1] = rr;
+            scal[4] = (rr == 0.0) ? 0.0 : 1.0;
+            flags[0] = (rr == 0.0) ? 1 : 0;
+            flags[1] = 0;
+            flags[2] = 0;
+            flags[3] = 0;
+        }
+    }
+}
+
+// Kp = K p on the sliced layout and p.Kp partials; last block: alpha = rr / pKp.
+__global__ __launch_bounds__(256) void k_cg_spmv(int64_t n_slices, const int* __restrict__ sell_row,
+                                                 const int64_t* __restrict__ slice_base,
+                                                 const int* __restrict__ slice_len, const int* __restrict__ sell_col,
+                                                 const double* __restrict__ K, const double* __restrict__ p,
+                                                 double* __restrict__ Kp, double* __restrict__ part,
+                                                 double* __restrict__ scal, int* __restrict__ flags) {
+    if (flags[0]) return;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0.0;
+    if (t < n_slices * TILE) {
+        const int row = sell_row[t];
+        const int64_t sl = t >> 6;
+        const int lane = (int)(t & (TILE - 1));
+        const int64_t base = slice_base[sl];
+        const int L = slice_len[sl];
+        const double* Kv = K + 9 * base + lane;
+        const int* cp = sell_col + base + lane;
+        double y0 = 0, y1 = 0, y2 = 0;
+        for (int k = 0; k < L; k++) {
+            const int col = cp[(int64_t)k * TILE];
+            const double x0 = p[3 * col], x1 = p[3 * col + 1], x2 = p[3 * col + 2];
+            const double* v = Kv + (int64_t)k * 9 * TILE;
+            y0 += v[0 * TILE] * x0 + v[1 * TILE] * x1 + v[2 * TILE] * x2;
+            y1 += v[3 * TILE] * x0 + v[4 * TILE] * x1 + v[5 * TILE] * x2;
+            y2 += v[6 * TILE] * x0 + v[7 * TILE] * x1 + v[8 * TILE] * x2;
+        }
+        if (row >= 0) {
+            Kp[3 * row] = y0;
+            Kp[3 * row + 1] = y1;
+            Kp[3 * row + 2] = y2;
+            s = p[3 * row] * y0 + p[3 * row + 1] * y1 + p[3 * row + 2] * y2;
+        }
+    }
+    s = block_sum(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+    if (last_block(&flags[3])) {
+        const double pKp = sum_parts(part, gridDim.x);
+        if (threadIdx.x == 0) {
+            scal[5] = pKp;
+            if (pKp <= 0.0) {
+                flags[1] = 1;
+                flags[0] = 1;
+            } else {
+                scal[2] = scal[1] / pKp;
+            }
+            flags[3] = 0;
+        }
+    }
+}
+
+// u += alpha p, r -= alpha Kp, rr' partials; last block: convergence test and beta.
+__global__ __launch_bounds__(RED_THREADS) void k_cg_update(int64_t n, double* __restrict__ u,
+                                                           double* __restrict__ r, const double* __restrict__ p,
+                                                           const double* __restrict__ Kp, double* __restrict__ part,
+                                                           double* __restrict__ scal, int* __restrict__ flags,
+                                                           double tol) {
+    if (flags[0]) return;
+    const double alpha = scal[2];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        u[i] += alpha * p[i];
+        const double ri = r[i] - alpha * Kp[i];
+        r[i] = ri;
+        s += ri * ri;
+    }
+    s = block_sum(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+    if (last_block(&flags[3])) {
+        const double rn = sum_parts(part, gridDim.x);
+        if (threadIdx.x == 0) {
+            const double rr0 = scal[0];
+            const double rel = rr0 > 0.0 ? sqrt(rn / rr0) : sqrt(rn);
+            scal[4] = rel;
+            flags[2] += 1;
+            if (rel < tol) {
+                flags[0] = 1;
+            } else {
+                scal[3] = rn / scal[1];
+                scal[1] = rn;
+            }
+            flags[3] = 0;
+        }
+    }
+}
+
+// p = beta p + r
+__global__ void k_cg_dir(int64_t n, double* __restrict__ p, const double* __restrict__ r,
+                         const double* __restrict__ scal, const int* __restrict__ flags) {
+    if (flags[0]) return;
+    const double beta = scal[3];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = beta * p[i] + r[i];
+}
+
+}  // namespace tlfea_hip
+
+using namespace tlfea_hip;
+
+// ======================================================================================
+// C-ABI
+// ======================================================================================
+namespace {
+template <typename T>
+int dmalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    TLFEA_HIP_TRY(hipMalloc((void**)p, sizeof(T) * count));
+    return TLFEA_OK;
+}
+
+void free_pattern(tlfea_pattern* p) {
+    if (!p) return;
+    hipFree(p->d_off);
+    hipFree(p->d_col);
+    hipFree(p->d_slot);
+    hipFree(p->d_sell_row);
+    hipFree(p->d_sell_col);
+    hipFree(p->d_slice_base);
+    hipFree(p->d_slice_len);
+    hipFree(p->d_row_pos);
+    hipFree(p->d_mask);
+    delete p;
+}
+
+template <int NN>
+int build_pattern(tlfea_pattern* P, const int* conn, hipStream_t s) {
+    const int n_elem = P->n_elem, n_nodes = P->n_nodes;
+    const int64_t ne = (int64_t)n_elem * NN;
+    int *d_cnt = nullptr, *d_inc = nullptr, *d_inc_off = nullptr, *d_rowlen = nullptr, *d_flag = nullptr;
+    int rc = TLFEA_OK;
+    auto cleanup = [&]() {
+        hipFree(d_cnt);
+        hipFree(d_inc);
+        hipFree(d_inc_off);
+        hipFree(d_rowlen);
+        hipFree(d_flag);
+    };
+#define TRY(x)                 \
+    do {                       \
+        rc = (x);              \
+        if (rc) {              \
+            cleanup();         \
+            return rc;         \
+        }                      \
+    } while (0)
+    TRY(dmalloc(&d_cnt, n_nodes));
+    TRY(dmalloc(&d_inc_off, n_nodes + 1));
+    TRY(dmalloc(&d_inc, ne));
+    TRY(dmalloc(&d_rowlen, n_nodes));
+    TRY(dmalloc(&d_flag, 4));
+    TLFEA_HIP_TRY(hipMemsetAsync(d_cnt, 0, sizeof(int) * n_nodes, s));
+    TLFEA_HIP_TRY(hipMemsetAsync(d_flag, 0, sizeof(int) * 4, s));
+    k_count_incidence<NN><<<grid_for(ne, 256), 256, 0, s>>>(n_elem, conn, d_cnt);
+    TLFEA_LAUNCH_CHECK();
+    std::vector<int> h_cnt(n_nodes), h_off(n_nodes + 1);
+    TLFEA_HIP_TRY(hipMemcpyAsync(h_cnt.data(), d_cnt, sizeof(int) * n_nodes, hipMemcpyDeviceToHost, s));
+    TLFEA_HIP_TRY(hipStreamSynchronize(s));
+    h_off[0] = 0;
+    for (int i = 0; i < n_nodes; i++) h_off[i + 1] = h_off[i] + h_cnt[i];
+    TLFEA_HIP_TRY(hipMemcpyAsync(d_inc_off, h_off.data(), sizeof(int) * (n_nodes + 1), hipMemcpyHostToDevice, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(d_cnt, h_off.data(), sizeof(int) * n_nodes, hipMemcpyHostToDevice, s));
+    k_fill_incidence<NN><<<grid_for(ne, 256), 256, 0, s>>>(n_elem, conn, d_cnt, d_inc);
+    TLFEA_LAUNCH_CHECK();
+    k_build_rows<NN><<<grid_for(n_nodes, 64), 64, 0, s>>>(n_nodes, n_elem, conn, d_inc_off, d_inc, 0, d_rowlen,
+                                                            nullptr, nullptr, d_flag);
+    TLFEA_LAUNCH_CHECK();
+    std::vector<int> h_len(n_nodes);
+    int h_flag[4];
+    TLFEA_HIP_TRY(hipMemcpyAsync(h_len.data(), d_rowlen, sizeof(int) * n_nodes, hipMemcpyDeviceToHost, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(h_flag, d_flag, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
+    TLFEA_HIP_TRY(hipStreamSynchronize(s));
+    if (h_flag[0]) {
+        cleanup();
+        set_error("pattern row longer than %d nodes", MAXROW);
+        return TLFEA_ECAPACITY;
+    }
+    std::vector<int64_t> off64(n_nodes + 1, 0);
+    for (int i = 0; i < n_nodes; i++) off64[i + 1] = off64[i] + h_len[i];
+    if (9 * off64[n_nodes] > INT32_MAX) {
+        cleanup();
+        set_error("stiffness nnz %lld exceeds the int32 index range", (long long)(9 * off64[n_nodes]));
+        return TLFEA_ECAPACITY;
+    }
+    P->nnz = off64[n_nodes];
+    std::vector<int> h_noff(n_nodes + 1);
+    for (int i = 0; i <= n_nodes; i++) h_noff[i] = (int)off64[i];
+    TRY(dmalloc(&P->d_off, n_nodes + 1));
+    TRY(dmalloc(&P->d_col, P->nnz));
+    TLFEA_HIP_TRY(hipMemcpyAsync(P->d_off, h_noff.data(), sizeof(int) * (n_nodes + 1), hipMemcpyHostToDevice, s));
+    k_build_rows<NN><<<grid_for(n_nodes, 64), 64, 0, s>>>(n_nodes, n_elem, conn, d_inc_off, d_inc, 1, nullptr,
+                                                            P->d_off, P->d_col, d_flag);
+    TLFEA_LAUNCH_CHECK();
+
+    // sliced layout: sort rows by length (descending, stable) inside each window
+    std::vector<int> perm(n_nodes);
+    std::iota(perm.begin(), perm.end(), 0);
+    for (int w0 = 0; w0 < n_nodes; w0 += WINDOW) {
+        const int w1 = std::min(n_nodes, w0 + WINDOW);
+        std::stable_sort(perm.begin() + w0, perm.begin() + w1,
+                         [&](int a, int b) { return h_len[a] > h_len[b]; });
+    }
+    // slices never straddle windows: pad each window to a multiple of 64 lanes
+    std::vector<int> sell_row;
+    sell_row.reserve(n_nodes + WINDOW);
+    for (int w0 = 0; w0 < n_nodes; w0 += WINDOW) {
+        const int w1 = std::min(n_nodes, w0 + WINDOW);
+        for (int k = w0; k < w1; k++) sell_row.push_back(perm[k]);
+        while (sell_row.size() % TILE) sell_row.push_back(-1);
+    }
+    const int64_t n_slices = (int64_t)sell_row.size() / TILE;
+    std::vector<int64_t> base(n_slices + 1, 0);
+    std::vector<int> slen(n_slices, 0);
+    std::vector<int> row_pos(n_nodes, 0);
+    for (int64_t sl = 0; sl < n_slices; sl++) {
+        int L = 0;
+        for (int l = 0; l < TILE; l++) {
+            const int r = sell_row[sl * TILE + l];
+            if (r >= 0) {
+                L = std::max(L, h_len[r]);
+                row_pos[r] = (int)(sl * TILE + l);
+            }
+        }
+        slen[sl] = L;
+        base[sl + 1] = base[sl] + (int64_t)L * TILE;
+    }
+    P->n_slices = n_slices;
+    P->sell_blocks = base[n_slices];
+    if (9 * P->sell_blocks > INT32_MAX) {
+        cleanup();
+        set_error("sliced block count %lld exceeds the int32 index range", (long long)P->sell_blocks);
+        return TLFEA_ECAPACITY;
+    }
+    TRY(dmalloc(&P->d_sell_row, n_slices * TILE));
+    TRY(dmalloc(&P->d_slice_base, n_slices + 1));
+    TRY(dmalloc(&P->d_slice_len, n_slices));
+    TRY(dmalloc(&P->d_row_pos, n_nodes));
+    TRY(dmalloc(&P->d_sell_col, P->sell_blocks));
+    TRY(dmalloc(&P->d_slot, (int64_t)n_elem * NN * NN));
+    TRY(dmalloc(&P->d_mask, 3 * (int64_t)n_nodes));
+    TLFEA_HIP_TRY(hipMemcpyAsync(P->d_sell_row, sell_row.data(), sizeof(int) * sell_row.size(), hipMemcpyHostToDevice, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(P->d_slice_base, base.data(), sizeof(int64_t) * (n_slices + 1), hipMemcpyHostToDevice, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(P->d_slice_len, slen.data(), sizeof(int) * n_slices, hipMemcpyHostToDevice, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(P->d_row_pos, row_pos.data(), sizeof(int) * n_nodes, hipMemcpyHostToDevice, s));
+    k_fill_sell_cols<<<grid_for(n_slices * TILE, 256), 256, 0, s>>>(n_slices, P->d_sell_row, P->d_off, P->d_col,
+                                                                    P->d_slice_base, P->d_slice_len, P->d_sell_col);
+    TLFEA_LAUNCH_CHECK();
+    const int64_t nslot = (int64_t)n_elem * NN * NN;
+    k_slot_map<NN><<<grid_for(nslot, 256), 256, 0, s>>>(n_elem, conn, P->d_off, P->d_col, P->d_row_pos,
+                                                          P->d_slice_base, P->d_slot, d_flag);
+    TLFEA_LAUNCH_CHECK();
+    TLFEA_HIP_TRY(hipMemcpyAsync(h_flag, d_flag, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
+    TLFEA_HIP_TRY(hipStreamSynchronize(s));
+    cleanup();
+    if (h_flag[0] & 2) {
+        set_error("internal error: element pair missing from its pattern row");
+        return TLFEA_EINVAL;
+    }
+    return TLFEA_OK;
+#undef TRY
+}
+}  // namespace
+
+extern "C" int tlfea_pattern_create(int nn, int n_elem, int n_nodes, const int* conn, tlfea_pattern** out,
+                                    void* stream) {
+    TLFEA_REQUIRE(out != nullptr, "null output handle");
+    *out = nullptr;
+    TLFEA_REQUIRE(nn == 10 || nn == 4, "unsupported element: %d nodes", nn);
+    TLFEA_REQUIRE(n_elem > 0 && n_nodes > 0 && conn, "empty mesh or null connectivity");
+    TLFEA_REQUIRE((int64_t)n_elem * nn * nn < INT32_MAX, "mesh too large for the slot map");
+    auto* P = new tlfea_pattern();
+    P->nn = nn;
+    P->n_elem = n_elem;
+    P->n_nodes = n_nodes;
+    hipStream_t s = as_stream(stream);
+    const int rc = nn == 10 ? build_pattern<10>(P, conn, s) : build_pattern<4>(P, conn, s);
+    if (rc != TLFEA_OK) {
+        free_pattern(P);
+        return rc;
+    }
+    *out = P;
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_pattern_destroy(tlfea_pattern* p) {
+    free_pattern(p);
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_pattern_sizes(const tlfea_pattern* p, int64_t* node_nnz, int64_t* dof_nnz,
+                                   int64_t* sell_blocks) {
+    TLFEA_REQUIRE(p != nullptr, "null pattern");
+    if (node_nnz) *node_nnz = p->nnz;
+    if (dof_nnz) *dof_nnz = 9 * p->nnz;
+    if (sell_blocks) *sell_blocks = p->sell_blocks;
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_pattern_node_csr(const tlfea_pattern* p, int* offsets, int* cols, void* stream) {
+    TLFEA_REQUIRE(p && offsets && cols, "null argument");
+    hipStream_t s = as_stream(stream);
+    TLFEA_HIP_TRY(hipMemcpyAsync(offsets, p->d_off, sizeof(int) * (p->n_nodes + 1), hipMemcpyDeviceToDevice, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(cols, p->d_col, sizeof(int) * p->nnz, hipMemcpyDeviceToDevice, s));
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_pattern_dof_csr(const tlfea_pattern* p, int* offsets, int* cols, void* stream) {
+    TLFEA_REQUIRE(p && offsets && cols, "null argument");
+    k_dof_csr<<<grid_for(3 * (int64_t)p->n_nodes, 256), 256, 0, as_stream(stream)>>>(p->n_nodes, p->d_off, p->d_col,
+                                                                                      offsets, cols);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_mass_assemble(const tlfea_pattern* p, int nn, int n_elem, const int* conn,
+                                   const double* detJ_t, const tlfea_quad* quad, double rho, double* m_sell,
+                                   void* stream) {
+    (void)conn;
+    TLFEA_REQUIRE(p && detJ_t && quad && m_sell, "null argument");
+    TLFEA_REQUIRE(nn == p->nn && n_elem == p->n_elem, "element data does not match the pattern");
+    hipStream_t s = as_stream(stream);
+    TLFEA_HIP_TRY(hipMemsetAsync(m_sell, 0, sizeof(double) * p->sell_blocks, s));
+    const int64_t nt = (int64_t)n_elem * nn * nn;
+    if (nn == 10)
+        k_mass<10, 5><<<grid_for(nt, 256), 256, 0, s>>>(n_elem, p->d_slot, detJ_t, *quad, rho, m_sell);
+    else
+        k_mass<4, 1><<<grid_for(nt, 256), 256, 0, s>>>(n_elem, p->d_slot, detJ_t, *quad, rho, m_sell);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_mass_to_csr(const tlfea_pattern* p, const double* m_sell, double* vals_csr, void* stream) {
+    TLFEA_REQUIRE(p && m_sell && vals_csr, "null argument");
+    k_sell_to_csr<<<grid_for(p->n_slices * TILE, 256), 256, 0, as_stream(stream)>>>(
+        p->n_slices, 1, p->d_sell_row, p->d_slice_base, p->d_off, m_sell, vals_csr);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_stiffness_assemble(const tlfea_pattern* p, int nn, int n_elem, const int* conn, const double* x,
+                                        const double* y, const double* z, const double* grad_t, const double* detJ_t,
+                                        const tlfea_quad* quad, const tlfea_material* mat, double h, double* k_sell,
+                                        void* stream) {
+    TLFEA_REQUIRE(p && conn && x && y && z && grad_t && detJ_t && quad && mat && k_sell, "null argument");
+    TLFEA_REQUIRE(nn == p->nn && n_elem == p->n_elem, "element data does not match the pattern");
+    hipStream_t s = as_stream(stream);
+    TLFEA_HIP_TRY(hipMemsetAsync(k_sell, 0, sizeof(double) * 9 * p->sell_blocks, s));
+    const unsigned tiles = (unsigned)(padded(n_elem) / TILE);
+    if (nn == 10)
+        k_stiffness<10, 5><<<tiles, TILE, 0, s>>>(n_elem, conn, x, y, z, grad_t, detJ_t, *quad, *mat, h, p->d_slot,
+                                                  k_sell);
+    else
+        k_stiffness<4, 1><<<tiles, TILE, 0, s>>>(n_elem, conn, x, y, z, grad_t, detJ_t, *quad, *mat, h, p->d_slot,
+                                                 k_sell);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_stiffness_to_csr(const tlfea_pattern* p, const double* k_sell, double* vals_csr, void* stream) {
+    TLFEA_REQUIRE(p && k_sell && vals_csr, "null argument");
+    k_sell_to_csr<<<grid_for(p->n_slices * TILE, 256), 256, 0, as_stream(stream)>>>(
+        p->n_slices, 9, p->d_sell_row, p->d_slice_base, p->d_off, k_sell, vals_csr);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_apply_dirichlet(const tlfea_pattern* p, int n_fixed, const int* fixed_nodes, double* k_sell,
+                                     double* f, void* stream) {
+    TLFEA_REQUIRE(p && k_sell, "null argument");
+    if (n_fixed <= 0) return TLFEA_OK;
+    TLFEA_REQUIRE(fixed_nodes != nullptr, "null fixed-node list");
+    hipStream_t s = as_stream(stream);
+    TLFEA_HIP_TRY(hipMemsetAsync(p->d_mask, 0, 3 * (size_t)p->n_nodes, s));
+    k_mark_fixed<<<grid_for(n_fixed, 256), 256, 0, s>>>(n_fixed, fixed_nodes, p->d_mask);
+    TLFEA_LAUNCH_CHECK();
+    k_dirichlet<<<grid_for(p->n_slices * TILE, 256), 256, 0, s>>>(p->n_slices, p->d_sell_row, p->d_slice_base,
+                                                                  p->d_slice_len, p->d_sell_col, p->d_mask, k_sell, f);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_spmv_block(const tlfea_pattern* p, const double* k_sell, const double* x, double* y,
+                                void* stream) {
+    TLFEA_REQUIRE(p && k_sell && x && y, "null argument");
+    k_spmv_block<<<grid_for(p->n_slices * TILE, 256), 256, 0, as_stream(stream)>>>(
+        p->n_slices, p->d_sell_row, p->d_slice_base, p->d_slice_len, p->d_sell_col, k_sell, x, y);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_spmv_node3(const tlfea_pattern* p, const double* m_sell, const double* x, double* y,
+                                void* stream) {
+    TLFEA_REQUIRE(p && m_sell && x && y, "null argument");
+    k_spmv_node3<<<grid_for(p->n_slices * TILE, 256), 256, 0, as_stream(stream)>>>(
+        p->n_slices, p->d_sell_row, p->d_slice_base, p->d_slice_len, p->d_sell_col, m_sell, x, y);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_spmv_csr(int n_rows, const int* offsets, const int* cols, const double* vals, const double* x,
+                              double* y, void* stream) {
+    TLFEA_REQUIRE(n_rows >= 0, "negative row count");
+    if (n_rows == 0) return TLFEA_OK;
+    TLFEA_REQUIRE(offsets && cols && vals && x && y, "null argument");
+    k_spmv_csr<<<grid_for(n_rows, 256), 256, 0, as_stream(stream)>>>(n_rows, offsets, cols, vals, x, y);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
+extern "C" int64_t tlfea_reduce_scratch_size(int64_t n) {
+    (void)n;
+    return RED_BLOCKS;
+}
+
+extern "C" int tlfea_dot(int64_t n, const double* a, const double* b, double* out, double* scratch, void* stream) {
+    TLFEA_REQUIRE(a && b && out && scratch && n >= 0, "bad argument");
+    hipStream_t s = as_stream(stream);
+    k_dot_partial<<<RED_BLOCKS, RED_THREADS, 0, s>>>(n, a, b, scratch);
+    TLFEA_LAUNCH_CHECK();
+    k_reduce_final<<<1, RED_THREADS, 0, s>>>(RED_BLOCKS, scratch, out, 0);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_nrm2(int64_t n, const double* a, double* out, double* scratch, void* stream) {
+    TLFEA_REQUIRE(a && out && scratch && n >= 0, "bad argument");
+    hipStream_t s = as_stream(stream);
+    k_dot_partial<<<RED_BLOCKS, RED_THREADS, 0, s>>>(n, a, a, scratch);
+    TLFEA_LAUNCH_CHECK();
+    k_reduce_final<<<1, RED_THREADS, 0, s>>>(RED_BLOCKS, scratch, out, 1);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_axpy(int64_t n, double alpha, const double* x, double* y, void* stream) {
+    TLFEA_REQUIRE(x && y && n >= 0, "bad argument");
+    if (n == 0) return TLFEA_OK;
+    k_axpy<<<grid_for(n, 256), 256, 0, as_stream(stream)>>>(n, alpha, x, y);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_scal(int64_t n, double alpha, double* x, void* stream) {
+    TLFEA_REQUIRE(x && n >= 0, "bad argument");
+    if (n == 0) return TLFEA_OK;
+    k_scal<<<grid_for(n, 256), 256, 0, as_stream(stream)>>>(n, alpha, x);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_add_displacement(int n_nodes, const double* u, double* x, double* y, double* z, void* stream) {
+    TLFEA_REQUIRE(n_nodes >= 0 && u && x && y && z, "bad argument");
+    if (n_nodes == 0) return TLFEA_OK;
+    k_add_disp<<<grid_for(n_nodes, 256), 256, 0, as_stream(stream)>>>(n_nodes, u, x, y, z);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_cg_create(const tlfea_pattern* p, tlfea_cg** out) {
+    TLFEA_REQUIRE(p && out, "null argument");
+    auto* c = new tlfea_cg();
+    c->p = p;
+    c->n = 3 * (int64_t)p->n_nodes;
+    int rc = TLFEA_OK;
+    if ((rc = dmalloc(&c->d_r, c->n)) || (rc = dmalloc(&c->d_p, c->n)) || (rc = dmalloc(&c->d_Kp, c->n)) ||
+        (rc = dmalloc(&c->d_part, RED_BLOCKS + (int)(p->n_slices * TILE / 256) + 1)) ||
+        (rc = dmalloc(&c->d_scal, 8)) || (rc = dmalloc(&c->d_flags, 4))) {
+        tlfea_cg_destroy(c);
+        return rc;
+    }
+    TLFEA_HIP_TRY(hipMemset(c->d_flags, 0, sizeof(int) * 4));
+    TLFEA_HIP_TRY(hipMemset(c->d_p, 0, sizeof(double) * c->n));
+    *out = c;
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_cg_destroy(tlfea_cg* c) {
+    if (!c) return TLFEA_OK;
+    hipFree(c->d_r);
+    hipFree(c->d_p);
+    hipFree(c->d_Kp);
+    hipFree(c->d_part);
+    hipFree(c->d_scal);
+    hipFree(c->d_flags);
+    delete c;
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_cg_solve(tlfea_cg* c, const double* k_sell, const double* f, double* u, double tol, int max_iter,
+                              int check_every, int* iters_out, double* rel_res_out, void* stream) {
+    TLFEA_REQUIRE(c && k_sell && f && u, "null argument");
+    TLFEA_REQUIRE(max_iter >= 0, "negative iteration limit");
+    if (check_every <= 0) check_every = 16;
+    hipStream_t s = as_stream(stream);
+    const tlfea_pattern* p = c->p;
+    const int64_t n = c->n;
+    const unsigned gs = grid_for(p->n_slices * TILE, 256);
+    const unsigned gv = RED_BLOCKS;
+    // The SpMV grid can exceed RED_BLOCKS partials: d_part was sized for both.
+    k_cg_init<<<gv, RED_THREADS, 0, s>>>(n, f, u, c->d_r, c->d_p, c->d_part, c->d_scal, c->d_flags);
+    TLFEA_LAUNCH_CHECK();
+    int flags[4] = {0, 0, 0, 0};
+    double scal[8] = {0};
+    for (int it = 0; it < max_iter; it++) {
+        k_cg_spmv<<<gs, 256, 0, s>>>(p->n_slices, p->d_sell_row, p->d_slice_base, p->d_slice_len, p->d_sell_col,
+                                      k_sell, c->d_p, c->d_Kp, c->d_part, c->d_scal, c->d_flags);
+        k_cg_update<<<gv, RED_THREADS, 0, s>>>(n, u, c->d_r, c->d_p, c->d_Kp, c->d_part, c->d_scal, c->d_flags, tol);
+        k_cg_dir<<<gv, 256, 0, s>>>(n, c->d_p, c->d_r, c->d_scal, c->d_flags);
+        TLFEA_LAUNCH_CHECK();
+        if ((it + 1) % check_every == 0 || it + 1 == max_iter) {
+            TLFEA_HIP_TRY(hipMemcpyAsync(flags, c->d_flags, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
+            TLFEA_HIP_TRY(hipStreamSynchronize(s));
+            if (flags[0]) break;
+        }
+    }
+    TLFEA_HIP_TRY(hipMemcpyAsync(flags, c->d_flags, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(scal, c->d_scal, sizeof(double) * 8, hipMemcpyDeviceToHost, s));
+    TLFEA_HIP_TRY(hipStreamSynchronize(s));
+    if (iters_out) *iters_out = flags[2];
+    if (rel_res_out) *rel_res_out = scal[4];
+    if (flags[1]) {
+        set_error("CG: non-positive p.Kp (%g) detected; aborting", scal[5]);
+        return TLFEA_ENOTPD;
+    }
+    if (!flags[0] && max_iter > 0) return TLFEA_ENOCONV;
+    return TLFEA_OK;
+}

@@ -1,0 +1,230 @@
+"""Python host layer over the C-ABI, for tests and the benchmark.
+
+``Body`` holds one mesh's device state the way ``GPU_FEAT10_Data`` / ``GPU_FEAT4_Data``
+do in the reference (`src/elements/FEAT10Data.cuh:733-809`), with the device layouts
+of ``include/tlfea_hip.h``; ``LinearStatic`` follows ``LinearStaticSolver::Solve``
+(`src/solvers/LinearStaticSolver.cu:469-481`).  All compute goes through libtlfea_hip.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import capi
+from .capi import call, ptr
+
+DEV = "cuda"
+
+
+def _dev(a, dtype):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(DEV)
+
+
+class Body:
+    def __init__(self, mesh, E=0.0, nu=0.0, rho=0.0, eta=0.0, lambda_d=0.0, stream=None):
+        self.mesh = mesh
+        self.nn = mesh.nn
+        self.nq = 5 if self.nn == 10 else 1
+        self.n_elem = mesh.n_elem
+        self.n_nodes = mesh.n_nodes
+        self.stream = stream
+        self.quad = capi.quadrature(self.nn)
+        self.set_material(E, nu, rho, eta, lambda_d)
+        ce = np.asarray(mesh.elems)
+        if ce.size and (ce.min() < 0 or ce.max() >= self.n_nodes):
+            raise ValueError("connectivity references nodes outside [0, n_nodes)")
+        if len(mesh.fixed) and (np.min(mesh.fixed) < 0 or np.max(mesh.fixed) >= self.n_nodes):
+            raise ValueError("fixed-node list references nodes outside [0, n_nodes)")
+        self.conn = _dev(mesh.conn_colmajor(), torch.int32)
+        self.X0 = [_dev(a, torch.float64) for a in (mesh.x, mesh.y, mesh.z)]
+        self.x = [t.clone() for t in self.X0]
+        pe = capi.padded_elems(self.n_elem)
+        self.grad_t = torch.zeros(pe * self.nq * self.nn * 3, dtype=torch.float64, device=DEV)
+        self.detJ_t = torch.zeros(pe * self.nq, dtype=torch.float64, device=DEV)
+        self.f_int = torch.zeros(3 * self.n_nodes, dtype=torch.float64, device=DEV)
+        self.f_ext = _dev(mesh.f_ext if mesh.f_ext is not None else np.zeros(3 * self.n_nodes), torch.float64)
+        self.fixed = _dev(mesh.fixed, torch.int32)
+        self._pattern = None
+        self.precompute()
+
+    def set_material(self, E, nu, rho, eta=0.0, lambda_d=0.0):
+        self.E, self.nu, self.rho = E, nu, rho
+        self.mat = capi.material(E, nu, eta, lambda_d) if (E or nu) else capi.Material(0, 0, eta, lambda_d)
+
+    @property
+    def s(self):
+        return self.stream
+
+    def precompute(self):
+        call("tlfea_precompute_grad", self.nn, self.n_elem, ptr(self.conn), ptr(self.X0[0]), ptr(self.X0[1]),
+             ptr(self.X0[2]), C.byref(self.quad), ptr(self.grad_t), ptr(self.detJ_t), self.s)
+
+    def grad_ref(self):
+        g = torch.empty(self.n_elem * self.nq * self.nn * 3, dtype=torch.float64, device=DEV)
+        d = torch.empty(self.n_elem * self.nq, dtype=torch.float64, device=DEV)
+        call("tlfea_grad_to_ref", self.nn, self.n_elem, ptr(self.grad_t), ptr(self.detJ_t), ptr(g), ptr(d), self.s)
+        return g, d
+
+    def qp_tensor(self):
+        return torch.zeros(capi.padded_elems(self.n_elem) * self.nq * 9, dtype=torch.float64, device=DEV)
+
+    def qp_ref(self, t):
+        out = torch.empty(self.n_elem * self.nq * 9, dtype=torch.float64, device=DEV)
+        call("tlfea_qp_to_ref", self.nn, self.n_elem, ptr(t), ptr(out), self.s)
+        return out
+
+    def calc_p(self, v=None, x=None):
+        x = self.x if x is None else x
+        F, P, Fd, Pv = (self.qp_tensor() for _ in range(4))
+        call("tlfea_calc_p", self.nn, self.n_elem, ptr(self.conn), ptr(x[0]), ptr(x[1]), ptr(x[2]), ptr(self.grad_t),
+             C.byref(self.mat), ptr(v), ptr(F), ptr(P), ptr(Fd), ptr(Pv), self.s)
+        return F, P, Fd, Pv
+
+    def fint_from_p(self, P, f=None):
+        f = self.f_int if f is None else f
+        call("tlfea_fint_from_p", self.nn, self.n_elem, ptr(self.conn), ptr(P), ptr(self.grad_t), ptr(self.detJ_t),
+             C.byref(self.quad), ptr(f), self.s)
+        return f
+
+    def internal_force(self, x=None, v=None, f=None, accumulate=False):
+        x = self.x if x is None else x
+        f = torch.empty(3 * self.n_nodes, dtype=torch.float64, device=DEV) if f is None else f
+        call("tlfea_internal_force", self.nn, self.n_elem, self.n_nodes, ptr(self.conn), ptr(x[0]), ptr(x[1]),
+             ptr(x[2]), ptr(self.grad_t), ptr(self.detJ_t), C.byref(self.quad), C.byref(self.mat), ptr(v), ptr(f),
+             int(bool(accumulate)), self.s)
+        return f
+
+    # ---- sparse ------------------------------------------------------------------
+    @property
+    def pattern(self):
+        if self._pattern is None:
+            h = C.c_void_p()
+            call("tlfea_pattern_create", self.nn, self.n_elem, self.n_nodes, ptr(self.conn), C.byref(h), self.s)
+            self._pattern = h
+        return self._pattern
+
+    def pattern_sizes(self):
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        call("tlfea_pattern_sizes", self.pattern, C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
+
+    def node_csr(self):
+        nnz, _, _ = self.pattern_sizes()
+        off = torch.empty(self.n_nodes + 1, dtype=torch.int32, device=DEV)
+        col = torch.empty(nnz, dtype=torch.int32, device=DEV)
+        call("tlfea_pattern_node_csr", self.pattern, ptr(off), ptr(col), self.s)
+        return off, col
+
+    def dof_csr(self):
+        _, nnz, _ = self.pattern_sizes()
+        off = torch.empty(3 * self.n_nodes + 1, dtype=torch.int32, device=DEV)
+        col = torch.empty(nnz, dtype=torch.int32, device=DEV)
+        call("tlfea_pattern_dof_csr", self.pattern, ptr(off), ptr(col), self.s)
+        return off, col
+
+    def mass_sell(self, rho=None):
+        rho = self.rho if rho is None else rho
+        _, _, nb = self.pattern_sizes()
+        m = torch.empty(nb, dtype=torch.float64, device=DEV)
+        call("tlfea_mass_assemble", self.pattern, self.nn, self.n_elem, ptr(self.conn), ptr(self.detJ_t),
+             C.byref(self.quad), float(rho), ptr(m), self.s)
+        return m
+
+    def mass_csr_values(self, m_sell):
+        nnz, _, _ = self.pattern_sizes()
+        v = torch.empty(nnz, dtype=torch.float64, device=DEV)
+        call("tlfea_mass_to_csr", self.pattern, ptr(m_sell), ptr(v), self.s)
+        return v
+
+    def stiffness_sell(self, x=None, h=1.0):
+        x = self.x if x is None else x
+        _, _, nb = self.pattern_sizes()
+        k = torch.empty(9 * nb, dtype=torch.float64, device=DEV)
+        call("tlfea_stiffness_assemble", self.pattern, self.nn, self.n_elem, ptr(self.conn), ptr(x[0]), ptr(x[1]),
+             ptr(x[2]), ptr(self.grad_t), ptr(self.detJ_t), C.byref(self.quad), C.byref(self.mat), float(h), ptr(k),
+             self.s)
+        return k
+
+    def stiffness_csr_values(self, k_sell):
+        _, nnz, _ = self.pattern_sizes()
+        v = torch.empty(nnz, dtype=torch.float64, device=DEV)
+        call("tlfea_stiffness_to_csr", self.pattern, ptr(k_sell), ptr(v), self.s)
+        return v
+
+    def apply_dirichlet(self, k_sell, f):
+        call("tlfea_apply_dirichlet", self.pattern, int(self.fixed.numel()), ptr(self.fixed), ptr(k_sell), ptr(f),
+             self.s)
+
+    def spmv(self, k_sell, xv, y=None):
+        y = torch.empty(3 * self.n_nodes, dtype=torch.float64, device=DEV) if y is None else y
+        call("tlfea_spmv_block", self.pattern, ptr(k_sell), ptr(xv), ptr(y), self.s)
+        return y
+
+    def spmv_mass(self, m_sell, xv, y=None):
+        y = torch.empty(3 * self.n_nodes, dtype=torch.float64, device=DEV) if y is None else y
+        call("tlfea_spmv_node3", self.pattern, ptr(m_sell), ptr(xv), ptr(y), self.s)
+        return y
+
+    def close(self):
+        if self._pattern is not None:
+            capi.lib().tlfea_pattern_destroy(self._pattern)
+            self._pattern = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class LinearStatic:
+    """K(x) u = f_ext with Dirichlet elimination and device-scalar CG, then x += u
+    (`src/solvers/LinearStaticSolver.cu:469-481`)."""
+
+    def __init__(self, body: Body, tol=1e-10, max_iter=10000, check_every=32):
+        self.b = body
+        self.tol, self.max_iter, self.check_every = tol, max_iter, check_every
+        h = C.c_void_p()
+        call("tlfea_cg_create", body.pattern, C.byref(h))
+        self.cg = h
+        self.k = None
+        self.f = None
+        self.u = torch.zeros(3 * body.n_nodes, dtype=torch.float64, device=DEV)
+        self.iters = 0
+        self.res = 0.0
+
+    def assemble(self):
+        self.k = self.b.stiffness_sell()
+        self.f = self.b.f_ext.clone()
+        self.b.apply_dirichlet(self.k, self.f)
+
+    def cg_solve(self):
+        it = C.c_int()
+        res = C.c_double()
+        rc = capi.lib().tlfea_cg_solve(self.cg, ptr(self.k), ptr(self.f), ptr(self.u), float(self.tol),
+                                       int(self.max_iter), int(self.check_every), C.byref(it), C.byref(res),
+                                       self.b.s)
+        self.iters, self.res = it.value, res.value
+        if rc not in (capi.TLFEA_OK, capi.TLFEA_ENOCONV):
+            capi.check(rc)
+        return rc
+
+    def solve(self):
+        self.assemble()
+        rc = self.cg_solve()
+        x = self.b.x
+        call("tlfea_add_displacement", self.b.n_nodes, ptr(self.u), ptr(x[0]), ptr(x[1]), ptr(x[2]), self.b.s)
+        return rc
+
+    def close(self):
+        if self.cg is not None:
+            capi.lib().tlfea_cg_destroy(self.cg)
+            self.cg = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
