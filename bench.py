@@ -14,7 +14,7 @@ Kernel rooflines are measured live with HIP events on the launch stream (the nul
 which torch.cuda.Event records on):
   * roofline      -- the step's dominant kernel, the block SpMV of CG,
                      bytes/launch = 76*node_nnz + 52*n_nodes;
-  * force_kernel  -- the fused FP64 P + f_int kernel,
+  * force_kernel  -- the fused FP64 P + f_int (block-local plan: force + node gather),
                      bytes/launch = 1280*n_elem + 48*n_nodes (the reference byte model).
 cpu_baseline: the FP64 oracle on a bounded sample (a slice of the element assembly and
 10 CG iterations on the full matrix), extrapolated to one solve with the GPU's iteration
@@ -107,11 +107,12 @@ def main():
     spmv_gbs = spmv_bytes / spmv_s / 1e9
 
     f = torch.zeros(3 * m.n_nodes, dtype=torch.float64, device="cuda")
+    body.plan  # build the block-local force plan (setup)
     for _ in range(3):
-        body.internal_force(f=f, accumulate=True)
+        body.internal_force_planned(f=f)
     e0.record()
     for _ in range(R):
-        body.internal_force(f=f, accumulate=True)
+        body.internal_force_planned(f=f)
     e1.record()
     torch.cuda.synchronize()
     force_s = e0.elapsed_time(e1) / 1e3 / R
@@ -171,7 +172,7 @@ def main():
                      "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": None, "kernel": "k_cg_spmv/k_spmv_block",
                      "avg_us": spmv_s * 1e6, "bytes_per_launch": spmv_bytes},
         "force_kernel": {"bound": "hbm", "achieved": force_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": force_gbs / HBM_PEAK_GBS, "kernel": "k_force<10,5,false>",
+                         "frac": force_gbs / HBM_PEAK_GBS, "kernel": "k_force_planned<10,5,false> + k_gather_partials",
                          "avg_us": force_s * 1e6, "bytes_per_launch": force_bytes},
         "cpu_baseline": cpu,
     }

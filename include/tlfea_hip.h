@@ -85,6 +85,26 @@ int tlfea_internal_force(int nn, int n_elem, int n_nodes, const int* conn, const
                          const double* z, const double* grad_t, const double* detJ_t, const tlfea_quad* quad,
                          const tlfea_material* mat, const double* v, double* f, int accumulate, void* stream);
 
+/* ---- block-local force plan (setup; opaque handle owns device memory) ------------ */
+/* Elements are cut into blocks of up to 256 consecutive elements; each block gets the
+ * sorted list of the nodes it touches and, per (element, local node), the slot of that
+ * node in the list.  The planned force kernel stages the block's node positions (and
+ * velocities) in LDS, accumulates the element nodal forces in LDS and writes one partial
+ * per (block, node) with plain stores; a node-gather kernel then sums each node's
+ * partials in a fixed order.  No global atomics; the in-block LDS accumulation order
+ * follows wave scheduling, so repeated calls agree to rounding, not bit for bit. */
+typedef struct tlfea_force_plan tlfea_force_plan;
+int tlfea_force_plan_create(int nn, int n_elem, int n_nodes, const int* conn, tlfea_force_plan** out, void* stream);
+int tlfea_force_plan_destroy(tlfea_force_plan* p);
+/* elements per block, blocks, total partials (sum over blocks of unique nodes), max
+ * unique nodes in one block */
+int tlfea_force_plan_sizes(const tlfea_force_plan* p, int* block_elems, int* n_blocks, int64_t* n_partials,
+                           int* max_block_nodes);
+/* Same result as tlfea_internal_force (SVK + Kelvin-Voigt when v != NULL). */
+int tlfea_internal_force_planned(const tlfea_force_plan* plan, const double* x, const double* y, const double* z,
+                                 const double* grad_t, const double* detJ_t, const tlfea_quad* quad,
+                                 const tlfea_material* mat, const double* v, double* f, int accumulate, void* stream);
+
 /* FP32 variant of the fused force kernel, displacement form F = I + sum_a u_a (x) g_a:
  * u = x - X is passed as three float arrays, grad32_t / dV32_t are float copies of grad_t
  * and detJ_t*w (tlfea_grad_to_f32).  f is float. */
@@ -108,6 +128,10 @@ typedef struct tlfea_pattern tlfea_pattern;
  * element -> slot maps used for scatter-free-of-search assembly and the sliced (64-row)
  * block layout used by the SpMV. */
 int tlfea_pattern_create(int nn, int n_elem, int n_nodes, const int* conn, tlfea_pattern** out, void* stream);
+/* As tlfea_pattern_create with an explicit row-sort window (rows are sorted by length
+ * inside consecutive windows of `window` rows before slicing; 1 keeps the mesh order). */
+int tlfea_pattern_create_ex(int nn, int n_elem, int n_nodes, const int* conn, int window, tlfea_pattern** out,
+                            void* stream);
 int tlfea_pattern_destroy(tlfea_pattern* p);
 /* node_nnz = mass-matrix nnz, dof_nnz = 9*node_nnz = stiffness nnz, sell_blocks = padded
  * block count of the sliced layout (values per matrix: 9*sell_blocks for K, sell_blocks
@@ -170,6 +194,77 @@ int tlfea_cg_solve(tlfea_cg* c, const double* k_sell, const double* f, double* u
 
 /* ---- position update (LinearStaticSolver.cu:189-196,449-463) ----------------------- */
 int tlfea_add_displacement(int n_nodes, const double* u, double* x, double* y, double* z, void* stream);
+
+
+/* ---- implicit-Euler velocity solvers (SyncedNesterov / SyncedAdamW) -------------- */
+/* One element body on the device.  The solvers read the element data, update x/y/z in
+ * place (as the reference's kernels write d_data->x12() etc.), and use f_int and
+ * constraint as workspaces / outputs (their values after a step are those of the last
+ * evaluation, like the reference's d_f_int and d_constraint). */
+typedef struct tlfea_body {
+    int nn, n_elem, n_nodes;
+    const int* conn;
+    double *x, *y, *z;
+    const double *grad_t, *detJ_t;
+    tlfea_quad quad;
+    tlfea_material mat;
+    const tlfea_pattern* pattern; /* node pattern of this mesh */
+    const double* m_sell;         /* mass matrix, sliced layout (tlfea_mass_assemble) */
+    const double* f_ext;          /* [3*n_nodes] */
+    int n_fixed;                  /* constraints = 3 * n_fixed */
+    const int* fixed_nodes;       /* [n_fixed] */
+    const double *xref, *yref, *zref; /* constraint targets (positions at Setup) */
+    double* f_int;                /* [3*n_nodes] */
+    double* constraint;           /* [3*n_fixed] */
+    const tlfea_force_plan* force_plan; /* optional; NULL = atomic-scatter force kernel */
+} tlfea_body;
+
+/* Byte-compatible with the reference's SyncedNesterovParams (SyncedNesterov.cuh:31-35). */
+typedef struct tlfea_nesterov_params {
+    double alpha, rho, inner_tol, outer_tol;
+    int max_outer, max_inner;
+    double time_step;
+} tlfea_nesterov_params;
+
+/* Byte-compatible with the reference's SyncedAdamWParams (SyncedAdamW.cuh:32-39). */
+typedef struct tlfea_adamw_params {
+    double lr, beta1, beta2, eps, weight_decay, lr_decay;
+    double inner_tol, outer_tol, rho;
+    int max_outer, max_inner;
+    double time_step;
+    int convergence_check_interval;
+    double inner_rtol;
+} tlfea_adamw_params;
+
+#define TLFEA_SOLVER_NESTEROV 0     /* SyncedNesterovSolver */
+#define TLFEA_SOLVER_ADAMW 1        /* SyncedAdamWSolver (cooperative semantics) */
+#define TLFEA_SOLVER_ADAMW_NOCOOP 2 /* SyncedAdamWNocoopSolver */
+
+typedef struct tlfea_solver tlfea_solver;
+/* Allocates the solver state (v_guess, v_prev, lambda, x_prev, iterates, reduction
+ * scratch) for `body`, all zero (SyncedNesterovSolver::Setup, SyncedNesterov.cuh:157-179).
+ * The body descriptor is copied; its device arrays must outlive the solver. */
+int tlfea_solver_create(int kind, const tlfea_body* body, tlfea_solver** out);
+int tlfea_solver_destroy(tlfea_solver* s);
+/* SetParameters: stores the parameters and zeroes v_guess, v_prev and lambda
+ * (SyncedNesterov.cuh:139-155, SyncedAdamW.cuh:160-190).  params points at a
+ * tlfea_nesterov_params or tlfea_adamw_params according to the solver kind. */
+int tlfea_solver_set_parameters(tlfea_solver* s, const void* params);
+/* Optional relative stop, not in the reference: also end the inner loop once
+ * ||g_free|| <= rel_tol * ||g_free at the first inner iteration|| (fixed DOFs excluded).
+ * 0 (the default) disables it. */
+int tlfea_solver_set_rel_tol(tlfea_solver* s, double rel_tol);
+/* How often (in inner iterations) the host reads the device convergence flag to stop
+ * launching; results do not depend on it.  Default 16. */
+int tlfea_solver_set_poll_interval(tlfea_solver* s, int every);
+/* One Solve() = one implicit time step (SyncedNesterov.cu:97-360, SyncedAdamW.cu:99-344,
+ * SyncedAdamWNocoop.cu:255-458).  Synchronous, like the reference. */
+int tlfea_solver_solve(tlfea_solver* s, void* stream);
+/* Device pointers of the solver state: v_guess, v_prev [3*n_nodes], lambda [3*n_fixed]. */
+int tlfea_solver_state(tlfea_solver* s, double** v_guess, double** v_prev, double** lambda_guess);
+/* Statistics of the last Solve(): inner iterations executed (summed over outer
+ * iterations), outer iterations executed, last ||g||, last ||g_free||/||g_free,0||. */
+int tlfea_solver_stats(const tlfea_solver* s, int* inner_iters, int* outer_iters, double* norm_g, double* rel_g);
 
 #ifdef __cplusplus
 }

@@ -64,9 +64,10 @@ def test_calc_p_and_fint_from_p(gpu, name):
 
 @pytest.mark.parametrize("name", MESHES)
 @pytest.mark.parametrize("damp", [False, True])
-def test_fused_internal_force(gpu, name, damp):
+@pytest.mark.parametrize("planned", [False, True])
+def test_fused_internal_force(gpu, name, damp, planned):
     import torch
-    print(f"[fused force] {name} damp={damp}", flush=True)
+    print(f"[fused force] {name} damp={damp} planned={planned}", flush=True)
     m = load_mesh(name)
     eta, ld = (1e6, 1e5) if damp else (0.0, 0.0)
     b = _body(m, eta=eta, lambda_d=ld)
@@ -74,13 +75,17 @@ def test_fused_internal_force(gpu, name, damp):
     xt = [torch.as_tensor(a).cuda() for a in (x, y, z)]
     v = np.random.default_rng(3).standard_normal(3 * m.n_nodes) * 1e-2
     vt = torch.as_tensor(v).cuda()
-    f = b.internal_force(x=xt, v=vt if damp else None)
+    force = b.internal_force_planned if planned else b.internal_force
+    f = force(x=xt, v=vt if damp else None)
     go, djo = O.grad(m)
     lam, mu = lame(E_STEEL, NU_STEEL)
     fo = O.internal_force(m, go, djo, lam, mu, x, y, z, v=v if damp else None, eta=eta, lambda_d=ld)
     assert rel_err(_np(f), fo) < 1e-12
+    if planned:  # repeat calls agree to rounding (in-block LDS accumulation order varies)
+        f2 = force(x=xt, v=vt if damp else None)
+        assert rel_err(_np(f2), _np(f)) < 1e-14
     # accumulate flag adds onto the existing vector
-    b.internal_force(x=xt, v=vt if damp else None, f=f, accumulate=True)
+    force(x=xt, v=vt if damp else None, f=f, accumulate=True)
     assert rel_err(_np(f), 2 * fo) < 1e-12
 
 
@@ -101,10 +106,15 @@ def test_fused_force_c2_full_size(gpu):
     x, y, z = deformed(m, seed=4, amp=2e-4)
     xt = [torch.as_tensor(a).cuda() for a in (x, y, z)]
     f = _np(b.internal_force(x=xt))
+    fp = _np(b.internal_force_planned(x=xt))
     go, djo = O.grad(m)
     lam, mu = lame(E_STEEL, NU_STEEL)
     fo = O.internal_force(m, go, djo, lam, mu, x, y, z)
     assert rel_err(f, fo) < 1e-12
+    assert rel_err(fp, fo) < 1e-12
+    ps = b.plan_sizes()
+    print(f"[plan] {ps}", flush=True)
+    assert ps["n_blocks"] == (m.n_elem + 255) // 256
     # equilibrium of internal forces: sum over nodes vanishes (translation invariance)
     assert np.abs(f.reshape(-1, 3).sum(0)).max() < 1e-9 * np.abs(f).max() * np.sqrt(m.n_nodes)
 

@@ -1,0 +1,156 @@
+"""GPU parity of the SyncedNesterov / SyncedAdamW / SyncedAdamWNocoop loops against the
+oracle's restatement of the same loops.
+
+Trajectories run with zero tolerances (so both sides execute exactly max_outer x max_inner
+iterations) from a deformed start under a load on every DOF, and are compared after each
+Solve(): displacements, v_guess, v_prev and lambda.  Both sides do the same arithmetic and
+differ only in summation order, which the iterations amplify: Nesterov within 1e-8
+relative; AdamW within 1e-6, because its normalised step m/(sqrt(v)+eps) turns any
+gradient entry near rounding level into a full +-lr step (from the rest state, where
+f_int is pure rounding noise, AdamW trajectories of any two FP64 implementations
+diverge -- measured on the CPU by perturbing the oracle's inputs at 1e-14).  The stopping
+tests are checked separately with the reference's example tolerances.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import E_STEEL, NU_STEEL, RHO_STEEL, deformed, load_mesh, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(name, kind, E=E_STEEL, nu=NU_STEEL, rho=RHO_STEEL, eta=0.0, lambda_d=0.0, planned=True,
+           random_load=False):
+    from tlfea_amd.fea import Body, Solver
+    m = load_mesh(name)
+    if random_load:  # loads on every DOF keep every gradient entry far above rounding noise
+        m.f_ext = np.random.default_rng(21).standard_normal(3 * m.n_nodes) * 50.0
+    b = Body(m, E, nu, rho, eta, lambda_d)
+    s = Solver(b, kind, planned=planned)
+    prob = O.Problem(m, E, nu, rho, eta=eta, lambda_d=lambda_d)
+    return m, b, s, prob
+
+
+def _start_deformed(m, b, st, amp=2e-5):
+    """Same deformed start on both sides (x_prev of the first step); the constraint targets
+    stay at the rest positions."""
+    import torch
+    x, y, z = deformed(m, seed=22, amp=amp, rot=0.0)
+    for d, a in enumerate((x, y, z)):
+        b.x[d].copy_(torch.as_tensor(a))
+    st["x"][:], st["y"][:], st["z"][:] = x, y, z
+
+
+def _compare(m, b, s, prob, st, tol=1e-9):
+    x = np.stack([t.cpu().numpy() for t in b.x], 1)
+    xo = np.stack([st["x"], st["y"], st["z"]], 1)
+    X0 = m.xyz()
+    assert rel_err(x - X0, xo - X0) < tol
+    assert rel_err(s.v_guess().cpu().numpy(), st["v_guess"]) < tol
+    assert rel_err(s.v_prev().cpu().numpy(), st["v_prev"]) < tol
+    if len(m.fixed):
+        assert rel_err(s.lambda_guess().cpu().numpy(), st["lam"]) < tol
+
+
+@pytest.mark.parametrize("name,damp,planned", [("cube", False, True), ("C1", False, True), ("C1", True, True),
+                                               ("C1", False, False), ("kuhn_t4", False, True)])
+def test_nesterov_trajectory(gpu, name, damp, planned):
+    from tlfea_amd import capi
+    print(f"[nesterov] {name} damp={damp} planned={planned}", flush=True)
+    eta, ld = (2e5, 1e5) if damp else (0.0, 0.0)
+    m, b, s, prob = _setup(name, capi.SOLVER_NESTEROV, eta=eta, lambda_d=ld, planned=planned, random_load=True)
+    alpha = 2e-10 if name == "cube" else 5e-9  # stable step for steel on each mesh
+    p = capi.NesterovParams(alpha=alpha, rho=1e10, inner_tol=0.0, outer_tol=0.0, max_outer=2, max_inner=25,
+                            time_step=1e-3)
+    s.set_parameters(p)
+    st = prob.new_state()
+    _start_deformed(m, b, st)
+    for step in range(2):
+        s.solve()
+        n_inner = prob.nesterov_step(st, p.alpha, p.rho, p.inner_tol, p.outer_tol, p.max_outer, p.max_inner,
+                                     p.time_step)
+        assert s.stats()["inner"] == n_inner == 50
+        _compare(m, b, s, prob, st, tol=1e-8)
+
+
+def test_nesterov_stopping_rules(gpu):
+    """Reference tolerances on the cube (test_feat10_nesterov.cc:189: alpha 1e-8, rho 1e14,
+    tolerances 1e-6, 5 outer x 300 inner, dt 1e-3): same stopping decisions as the oracle."""
+    from tlfea_amd import capi
+    m, b, s, prob = _setup("cube", capi.SOLVER_NESTEROV, E=7e8, nu=0.33, rho=2700.0)
+    p = capi.NesterovParams(alpha=1e-8, rho=1e14, inner_tol=1e-6, outer_tol=1e-6, max_outer=5, max_inner=300,
+                            time_step=1e-3)
+    s.set_parameters(p)
+    st = prob.new_state()
+    for step in range(3):
+        s.solve()
+        n = prob.nesterov_step(st, p.alpha, p.rho, p.inner_tol, p.outer_tol, p.max_outer, p.max_inner, p.time_step)
+        assert abs(s.stats()["inner"] - n) <= 2, (s.stats(), n)
+        _compare(m, b, s, prob, st, tol=1e-6)
+
+
+@pytest.mark.parametrize("nocoop", [False, True])
+@pytest.mark.parametrize("name", ["cube", "C1"])
+def test_adamw_trajectory(gpu, nocoop, name):
+    from tlfea_amd import capi
+    print(f"[adamw] {name} nocoop={nocoop}", flush=True)
+    kind = capi.SOLVER_ADAMW_NOCOOP if nocoop else capi.SOLVER_ADAMW
+    m, b, s, prob = _setup(name, kind, eta=1e4, lambda_d=1e3, random_load=True)
+    p = capi.AdamWParams(lr=2e-4, beta1=0.9, beta2=0.999, eps=1e-8, weight_decay=1e-4, lr_decay=0.995,
+                         inner_tol=0.0, outer_tol=0.0, rho=1e10, max_outer=2, max_inner=30, time_step=1e-3,
+                         convergence_check_interval=10, inner_rtol=0.0)
+    s.set_parameters(p)
+    st = prob.new_state()
+    _start_deformed(m, b, st)
+    for step in range(2):
+        s.solve()
+        n = prob.adamw_step(st, p.lr, p.beta1, p.beta2, p.eps, p.weight_decay, p.lr_decay, p.inner_tol, p.outer_tol,
+                            p.rho, p.max_outer, p.max_inner, p.time_step, p.convergence_check_interval, p.inner_rtol,
+                            nocoop=nocoop)
+        assert s.stats()["inner"] == n
+        _compare(m, b, s, prob, st, tol=1e-6)
+
+
+@pytest.mark.parametrize("nocoop", [False, True])
+def test_adamw_stopping_rules(gpu, nocoop):
+    """beam_simulation.cc:226 parameters (tolerances on) on the cube: same decisions."""
+    from tlfea_amd import capi
+    kind = capi.SOLVER_ADAMW_NOCOOP if nocoop else capi.SOLVER_ADAMW
+    m, b, s, prob = _setup("cube", kind, E=7e8, nu=0.33, rho=2700.0, eta=0.0, lambda_d=0.1)
+    p = capi.AdamWParams(lr=2e-4, beta1=0.9, beta2=0.999, eps=1e-8, weight_decay=1e-4, lr_decay=0.995,
+                         inner_tol=1e-1, outer_tol=1e-6, rho=1e14, max_outer=5, max_inner=300, time_step=1e-3,
+                         convergence_check_interval=10, inner_rtol=0.0)
+    s.set_parameters(p)
+    st = prob.new_state()
+    for step in range(3):
+        s.solve()
+        n = prob.adamw_step(st, p.lr, p.beta1, p.beta2, p.eps, p.weight_decay, p.lr_decay, p.inner_tol, p.outer_tol,
+                            p.rho, p.max_outer, p.max_inner, p.time_step, p.convergence_check_interval, p.inner_rtol,
+                            nocoop=nocoop)
+        assert abs(s.stats()["inner"] - n) <= 10, (s.stats(), n)
+
+
+def test_nesterov_rel_tol_reaches_residual(gpu):
+    """The optional relative stop: the final step gradient (oracle, fixed DOFs excluded) is
+    at most 1e-8 of the initial one."""
+    from tlfea_amd import capi
+    m, b, s, prob = _setup("cube", capi.SOLVER_NESTEROV, E=7e8, nu=0.33, rho=2700.0)
+    p = capi.NesterovParams(alpha=1e-8, rho=1e14, inner_tol=0.0, outer_tol=0.0, max_outer=1, max_inner=200000,
+                            time_step=1e-3)
+    s.set_parameters(p)
+    s.set_rel_tol(1e-8)
+    s.set_poll_interval(64)
+    s.solve()
+    st = s.stats()
+    assert st["rel"] <= 1e-8 and st["inner"] < 200000
+    X0 = m.xyz()
+    free = np.ones(3 * m.n_nodes, bool)
+    for n in m.fixed:
+        free[3 * n:3 * n + 3] = False
+    z = np.zeros(3 * m.n_nodes)
+    g0 = prob.step_gradient(X0[:, 0], X0[:, 1], X0[:, 2], z, z, np.zeros(3 * len(m.fixed)), p.time_step, p.rho)
+    v = s.v_guess().cpu().numpy()
+    # v_prev after Solve() equals v_guess; evaluate g at the converged v with v_prev = 0
+    g1 = prob.step_gradient(X0[:, 0], X0[:, 1], X0[:, 2], v, z, np.zeros(3 * len(m.fixed)), p.time_step, p.rho)
+    assert np.linalg.norm(g1[free]) <= 2e-8 * np.linalg.norm(g0[free])

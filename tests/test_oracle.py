@@ -1,0 +1,478 @@
+"""Pins the FP64 CPU oracle (CPU only).
+
+The reference holds no golden vectors or known-answer tests, so every oracle function is
+checked against what the mathematics fixes -- closed forms, invariants, brute force and
+independent routes -- chosen so that a dropped term, a wrong sign or index or a transposed
+operand fails one of them.
+"""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import E_STEEL, NU_STEEL, RHO_STEEL, deformed, lame, load_mesh, rel_err
+from tlfea_amd import mesh as M
+
+SMALL = ["cube", "beam_vtu", "C1", "kuhn_t4"]
+
+
+def _tet_volume(X):
+    return np.linalg.det(np.stack([X[1] - X[0], X[2] - X[0], X[3] - X[0]])) / 6.0
+
+
+# ---------------------------------------------------------------------------------------
+def test_quadrature_constants_and_exactness():
+    qx, qy, qz, qw = O.quad_t10()
+    assert qw[0] == -4.0 / 5.0 / 6.0 and all(w == 9.0 / 20.0 / 6.0 for w in qw[1:])
+    assert abs(qw.sum() - 1.0 / 6.0) < 1e-17
+    # exact on x^a y^b z^c with a+b+c <= 3 over the unit tet: a! b! c! / (a+b+c+3)!
+    for a, b, c in itertools.product(range(4), repeat=3):
+        if a + b + c > 3:
+            continue
+        exact = math.factorial(a) * math.factorial(b) * math.factorial(c) / math.factorial(a + b + c + 3)
+        approx = float(np.sum(qw * qx ** a * qy ** b * qz ** c))
+        assert abs(approx - exact) < 1e-16, (a, b, c)
+    # degree 4 is NOT exact (the rule is degree 3): guards against a wrong table
+    exact4 = math.factorial(4) / math.factorial(7)
+    assert abs(float(np.sum(qw * qx ** 4)) - exact4) > 1e-6
+    tx, ty, tz, tw = O.quad_t4()
+    assert tx[0] == ty[0] == tz[0] == 0.25 and tw[0] == 1.0 / 6.0
+
+
+# ---------------------------------------------------------------------------------------
+@pytest.mark.parametrize("name", SMALL + ["beam_highres_t10", "beam_highres_t4"])
+def test_grad_invariants(name):
+    m = load_mesh(name)
+    g, dj = O.grad(m)
+    nn = m.nn
+    nq = 5 if nn == 10 else 1
+    G = g.reshape(m.n_elem, nq, 3, nn)  # [e, q, d, a]  (grad[a + nn*d])
+    # partition of unity: sum_a g_a = 0
+    assert np.abs(G.sum(axis=3)).max() < 1e-9 * np.abs(G).max()
+    # sum_a X_a (x) g_a = I at every quadrature point
+    X = m.xyz()[m.elems]  # [e, a, i]
+    ident = np.einsum("eai,eqda->eqid", X, G)
+    assert np.abs(ident - np.eye(3)).max() < 1e-10
+    # straight-sided tets: det J = 6 * volume at every quadrature point
+    V = np.array([_tet_volume(X[e, :4]) for e in range(m.n_elem)])
+    assert rel_err(dj.reshape(m.n_elem, nq), np.repeat(6 * V[:, None], nq, 1)) < 1e-12
+    assert np.all(dj > 0)
+
+
+def test_grad_affine_closed_form():
+    """For a straight-sided element g_a = J^{-T} dN_a/dxi with J = [X1-X0, X2-X0, X3-X0]
+    (an explicit numpy inverse, independent of the pivoting solve)."""
+    m = load_mesh("beam_vtu")
+    g, dj = O.grad(m)
+    qx, qy, qz, _ = O.quad_t10()
+    G = g.reshape(m.n_elem, 5, 3, 10)
+    for e in range(m.n_elem):
+        X = m.xyz()[m.elems[e]]
+        J = np.stack([X[1] - X[0], X[2] - X[0], X[3] - X[0]], 1)
+        JinvT = np.linalg.inv(J).T
+        for q in range(5):
+            L = np.array([1 - qx[q] - qy[q] - qz[q], qx[q], qy[q], qz[q]])
+            dL = np.array([[-1, -1, -1], [1, 0, 0], [0, 1, 0], [0, 0, 1.0]])
+            dN = np.zeros((10, 3))
+            for i in range(4):
+                dN[i] = (4 * L[i] - 1) * dL[i]
+            for k, (i, j) in enumerate(M.T10_EDGES):
+                dN[4 + k] = 4 * (L[i] * dL[j] + L[j] * dL[i])
+            ref = dN @ JinvT.T  # rows g_a
+            assert np.allclose(G[e, q].T, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+def test_beam_vtu_detj_value():
+    """beam.vtu: 24 straight TET10 of volume 10/24 -> det J = 2.5 everywhere."""
+    m = load_mesh("beam_vtu")
+    _, dj = O.grad(m)
+    assert np.allclose(dj, 2.5, rtol=1e-13)
+
+
+# ---------------------------------------------------------------------------------------
+def _single_elem_mesh(F=None):
+    m = M.kuhn_beam(1, 1, 1, 1.0, 1.0, 1.0)
+    m.elems = m.elems[:1].copy()
+    return m
+
+
+def _p_at(F, lam, mu):
+    """P of the oracle for a homogeneous deformation x = F X on one element."""
+    m = _single_elem_mesh()
+    g, _ = O.grad(m)
+    X = m.xyz()
+    x = X @ F.T
+    _, P, _, _ = O.calc_p(m, g, lam, mu, x[:, 0].copy(), x[:, 1].copy(), x[:, 2].copy())
+    return P.reshape(5, 3, 3).transpose(0, 2, 1)  # stored column-major
+
+
+def test_svk_closed_forms_and_energy_derivative():
+    lam, mu = lame(E_STEEL, NU_STEEL)
+    P = _p_at(np.eye(3), lam, mu)
+    assert np.abs(P).max() < 1e-3  # rounding of E * 1e-16
+    th = 0.7
+    R = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
+    assert np.abs(_p_at(R, lam, mu)).max() < 1e-2
+    s = 1.01
+    P = _p_at(np.diag([s, 1, 1]), lam, mu)[0]
+    e11 = 0.5 * (s * s - 1)
+    assert abs(P[0, 0] - s * (lam + 2 * mu) * e11) < 1e-12 * abs(P[0, 0])
+    assert abs(P[1, 1] - lam * e11) < 1e-12 * abs(P[1, 1])
+    assert abs(P[0, 1]) < 1e-3
+    # P = dW/dF with W = lambda/2 (tr E)^2 + mu tr(E^2), central differences
+    rng = np.random.default_rng(0)
+    F = np.eye(3) + 0.05 * rng.standard_normal((3, 3))
+
+    def W(F):
+        E = 0.5 * (F.T @ F - np.eye(3))
+        return 0.5 * lam * np.trace(E) ** 2 + mu * np.trace(E @ E)
+
+    P = _p_at(F, lam, mu)[0]
+    h = 1e-7
+    for i in range(3):
+        for j in range(3):
+            Fp, Fm = F.copy(), F.copy()
+            Fp[i, j] += h
+            Fm[i, j] -= h
+            assert abs((W(Fp) - W(Fm)) / (2 * h) - P[i, j]) < 1e-6 * np.abs(P).max()
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_internal_force_invariants(name):
+    m = load_mesh(name)
+    g, dj = O.grad(m)
+    lam, mu = lame(E_STEEL, NU_STEEL)
+    f0 = O.internal_force(m, g, dj, lam, mu)
+    x, y, z = deformed(m, seed=11)
+    f = O.internal_force(m, g, dj, lam, mu, x, y, z)
+    assert np.abs(f0).max() < 1e-10 * np.abs(f).max()
+    # translation invariance: sum over nodes vanishes
+    assert np.abs(f.reshape(-1, 3).sum(0)).max() < 1e-9 * np.abs(f).max()
+    # rotation invariance: moments of the internal forces vanish (sum x_a x f_a = 0)
+    xa = np.stack([x, y, z], 1)
+    mom = np.cross(xa, f.reshape(-1, 3)).sum(0)
+    assert np.abs(mom).max() < 1e-8 * np.abs(f).max() * np.abs(xa).max()
+
+
+def test_internal_force_is_energy_gradient():
+    m = load_mesh("cube")
+    g, dj = O.grad(m)
+    lam, mu = lame(1e9, 0.3)
+    x, y, z = deformed(m, seed=12, amp=1e-2)
+    f = O.internal_force(m, g, dj, lam, mu, x, y, z)
+    h = 1e-7
+    for node in (0, 6, 13, 26):
+        for d, arr in enumerate((x, y, z)):
+            a = arr.copy()
+            a[node] += h
+            args = [x, y, z]
+            args[d] = a
+            ep = O.energy(m, g, dj, lam, mu, *args)
+            a[node] -= 2 * h
+            em = O.energy(m, g, dj, lam, mu, *args)
+            fd = (ep - em) / (2 * h)
+            assert abs(fd - f[3 * node + d]) < 1e-6 * np.abs(f).max()
+
+
+@pytest.mark.parametrize("name", ["C1", "kuhn_t4"])
+def test_patch_test(name):
+    """Affine displacement -> constant P -> zero net force on every interior node."""
+    m = load_mesh(name)
+    g, dj = O.grad(m)
+    lam, mu = lame(E_STEEL, NU_STEEL)
+    A = np.array([[1e-3, 2e-4, 0], [0, -5e-4, 1e-4], [3e-4, 0, 2e-4]])
+    X = m.xyz()
+    x = X + X @ A.T
+    f = O.internal_force(m, g, dj, lam, mu, x[:, 0].copy(), x[:, 1].copy(), x[:, 2].copy()).reshape(-1, 3)
+    lo, hi = X.min(0), X.max(0)
+    tol = 1e-9
+    interior = np.all((X > lo + tol) & (X < hi - tol), axis=1)
+    assert interior.sum() > 0
+    assert np.abs(f[interior]).max() < 1e-9 * np.abs(f).max()
+    assert np.abs(f[~interior]).max() > 1e-3 * np.abs(f).max()
+
+
+# ---------------------------------------------------------------------------------------
+@pytest.mark.parametrize("name", SMALL)
+def test_patterns_against_numpy_unique(name):
+    m = load_mesh(name)
+    off, col = O.node_pattern(m)
+    E = m.elems.astype(np.int64)
+    n = m.n_nodes
+    keys = np.unique((E[:, :, None] * n + E[:, None, :]).reshape(-1))
+    rows, cols = keys // n, keys % n
+    assert np.array_equal(col, cols) and np.array_equal(off, np.searchsorted(rows, np.arange(n + 1)))
+    doff, dcol = O.dof_pattern(m)
+    assert len(dcol) == 9 * len(col)
+    assert doff[-1] == 9 * len(col)
+    # DOF row 3i+d = node row i expanded by 3
+    for i in (0, n // 2, n - 1):
+        r = np.arange(off[i], off[i + 1])
+        exp = (3 * col[r][:, None] + np.arange(3)[None, :]).reshape(-1)
+        for d in range(3):
+            assert np.array_equal(dcol[doff[3 * i + d]:doff[3 * i + d + 1]], exp)
+
+
+def test_pattern_sizes_of_benchmark_beam():
+    m = load_mesh("C1")
+    off, col = O.node_pattern(m)
+    assert (m.n_elem, m.n_nodes, len(col)) == (480, 1025, 22457)
+    doff, dcol = O.dof_pattern(m)
+    assert len(dcol) == 202113
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_mass_matrix(name):
+    m = load_mesh(name)
+    g, dj = O.grad(m)
+    off, col = O.node_pattern(m)
+    vals = O.mass_values(m, dj, RHO_STEEL, off, col)
+    import scipy.sparse as sp
+    Mm = sp.csr_matrix((vals, col, off)).toarray()
+    assert np.abs(Mm - Mm.T).max() < 1e-12 * np.abs(Mm).max()
+    X = m.xyz()[m.elems]
+    V = np.array([_tet_volume(X[e, :4]) for e in range(m.n_elem)])
+    assert abs(Mm.sum() - RHO_STEEL * V.sum()) < 1e-12 * RHO_STEEL * V.sum()
+    # row sums = rho * integral of N_i: corner -V/20, midside V/5 (TET10); V/4 (TET4)
+    rs = np.zeros(m.n_nodes)
+    for e in range(m.n_elem):
+        for a in range(m.nn):
+            w = (-1 / 20 if a < 4 else 1 / 5) if m.nn == 10 else 1 / 4
+            rs[m.elems[e, a]] += RHO_STEEL * V[e] * w
+    assert np.abs(Mm.sum(1) - rs).max() < 1e-12 * np.abs(rs).max()
+    if m.nn == 4:  # 1-point rule: every element entry is rho detJ / 96 = rho V / 16
+        e0 = m.elems[0]
+        assert abs(Mm[e0[0], e0[1]] - RHO_STEEL * V[0] / 16) < 1e-12 or Mm[e0[0], e0[1]] > RHO_STEEL * V[0] / 16
+
+
+# ---------------------------------------------------------------------------------------
+def _K(m, lam, mu, x=None, y=None, z=None, h=1.0, eta=0.0, ld=0.0):
+    g, dj = O.grad(m)
+    off, col = O.dof_pattern(m)
+    vals = O.stiffness_values(m, g, dj, lam, mu, off, col, x, y, z, h=h, eta=eta, lambda_d=ld)
+    import scipy.sparse as sp
+    return sp.csr_matrix((vals, col, off)).toarray(), off, col, vals
+
+
+@pytest.mark.parametrize("name", ["cube", "kuhn_t4"])
+def test_stiffness_symmetry_rigid_modes_and_fd(name):
+    m = load_mesh(name)
+    lam, mu = lame(1e9, 0.3)
+    K, *_ = _K(m, lam, mu)
+    assert np.abs(K - K.T).max() < 1e-12 * np.abs(K).max()
+    X = m.xyz()
+    modes = [np.tile(np.eye(3)[d], m.n_nodes) for d in range(3)]
+    for d in range(3):  # infinitesimal rotations
+        w = np.zeros(3)
+        w[d] = 1.0
+        modes.append(np.cross(w, X).reshape(-1))
+    for r in modes:
+        assert np.abs(K @ r).max() < 1e-10 * np.abs(K).max() * np.abs(r).max()
+    # tangent = derivative of f_int at a deformed state (central differences)
+    g, dj = O.grad(m)
+    x, y, z = deformed(m, seed=13, amp=5e-3)
+    Kx, *_ = _K(m, lam, mu, x, y, z)
+    assert np.abs(Kx - Kx.T).max() < 1e-10 * np.abs(Kx).max()
+    u = np.random.default_rng(14).standard_normal(3 * m.n_nodes)
+    eps = 1e-7 * np.abs(m.xyz()).max()
+    U = u.reshape(-1, 3)
+    fp = O.internal_force(m, g, dj, lam, mu, x + eps * U[:, 0], y + eps * U[:, 1], z + eps * U[:, 2])
+    fm = O.internal_force(m, g, dj, lam, mu, x - eps * U[:, 0], y - eps * U[:, 1], z - eps * U[:, 2])
+    assert rel_err(Kx @ u, (fp - fm) / (2 * eps)) < 1e-6
+
+
+def test_stiffness_linear_closed_form():
+    """At F = I: K_ab = sum_q (lambda g_a (x) g_b + mu (g_a . g_b) I + mu g_b (x) g_a) dV,
+    assembled independently in numpy."""
+    m = load_mesh("cube")
+    lam, mu = lame(1e9, 0.3)
+    K, *_ = _K(m, lam, mu, h=2.0)
+    g, dj = O.grad(m)
+    qw = O.quad_t10()[3]
+    G = g.reshape(m.n_elem, 5, 3, 10)
+    Kr = np.zeros_like(K)
+    for e in range(m.n_elem):
+        for q in range(5):
+            dV = dj[e * 5 + q] * qw[q]
+            for a in range(10):
+                for b in range(10):
+                    ga, gb = G[e, q, :, a], G[e, q, :, b]
+                    blk = lam * np.outer(ga, gb) + mu * (ga @ gb) * np.eye(3) + mu * np.outer(gb, ga)
+                    ia, ib = m.elems[e, a], m.elems[e, b]
+                    Kr[3 * ia:3 * ia + 3, 3 * ib:3 * ib + 3] += 2.0 * blk * dV
+    assert np.abs(K - Kr).max() < 1e-12 * np.abs(Kr).max()
+
+
+def test_stiffness_damping_block():
+    """The Kelvin-Voigt block equals the derivative of the viscous force w.r.t. v at
+    F = I: C_ab = (eta (g_a.g_b) I + eta g_b (x) g_a + lambda_d g_a (x) g_b) dV."""
+    m = load_mesh("cube")
+    lam, mu = lame(1e9, 0.3)
+    C, *_ = _K(m, 0.0, 0.0, eta=3.0, ld=5.0)  # elastic part off: the block alone
+    g, dj = O.grad(m)
+    v = np.random.default_rng(15).standard_normal(3 * m.n_nodes)
+    f0 = O.internal_force(m, g, dj, 0.0, 0.0, v=np.zeros_like(v), eta=3.0, lambda_d=5.0)
+    f1 = O.internal_force(m, g, dj, 0.0, 0.0, v=v, eta=3.0, lambda_d=5.0)
+    assert rel_err(C @ v, f1 - f0) < 1e-10
+
+
+# ---------------------------------------------------------------------------------------
+def test_dirichlet_against_dense_elimination():
+    m = load_mesh("cube")
+    lam, mu = lame(1e9, 0.3)
+    K, off, col, vals = _K(m, lam, mu)
+    v2, f2 = O.apply_dirichlet(off, col, vals, m.f_ext, m.fixed)
+    import scipy.sparse as sp
+    Kb = sp.csr_matrix((v2, col, off)).toarray()
+    fixed = np.zeros(3 * m.n_nodes, bool)
+    for n in m.fixed:
+        fixed[3 * n:3 * n + 3] = True
+    Kd = K.copy()
+    Kd[fixed, :] = 0
+    Kd[:, fixed] = 0
+    Kd[fixed, fixed] = 1
+    assert np.array_equal(Kb, Kd)
+    fd = m.f_ext.copy()
+    fd[fixed] = 0
+    assert np.array_equal(f2, fd)
+
+
+@pytest.mark.parametrize("name", ["cube", "beam_vtu", "C1"])
+def test_cg_and_cholesky_against_dense(name):
+    m = load_mesh(name)
+    lam, mu = lame(E_STEEL, NU_STEEL)
+    K, off, col, vals = _K(m, lam, mu)
+    v2, f2 = O.apply_dirichlet(off, col, vals, m.f_ext, m.fixed)
+    import scipy.sparse as sp
+    Kb = sp.csr_matrix((v2, col, off)).toarray()
+    ud = np.linalg.solve(Kb, f2)
+    uc = O.band_cholesky_solve(off, col, v2, f2)
+    assert rel_err(uc, ud) < 1e-10
+    u, it, res = O.cg(off, col, v2, f2, tol=1e-12, max_iter=100000)
+    assert it > 0 and res < 1e-12
+    assert rel_err(u, ud) < 1e-9
+    assert np.all(u[np.repeat(np.isin(np.arange(m.n_nodes), m.fixed), 3)] == 0)
+
+
+def test_euler_bernoulli_band():
+    """beam_linear_static.cc:223-234: the C1 tip deflection is within a few % of PL^3/(3EI)."""
+    m = load_mesh("C1")
+    lam, mu = lame(E_STEEL, NU_STEEL)
+    _, off, col, vals = _K(m, lam, mu)
+    v2, f2 = O.apply_dirichlet(off, col, vals, m.f_ext, m.fixed)
+    u = O.band_cholesky_solve(off, col, v2, f2)
+    dz = u[3 * m.meta["tip_nodes"] + 2].min()
+    eb = 1000.0 / (3 * E_STEEL * 0.1 ** 4 / 12)
+    assert 0.97 < abs(dz) / eb < 1.03
+
+
+def test_cg_not_positive_definite():
+    m = load_mesh("cube")
+    lam, mu = lame(-1e9, 0.3)
+    _, off, col, vals = _K(m, lam, mu)
+    v2, f2 = O.apply_dirichlet(off, col, vals, m.f_ext, m.fixed)
+    u, it, res = O.cg(off, col, v2, f2, tol=1e-10, max_iter=100)
+    assert it == -1
+
+
+# ---------------------------------------------------------------------------------------
+def _newton(m, prob, dt, mode, rho=0.0, iters=40):
+    """Direct (Newton, dense tangent) solutions the solvers must reach:
+    mode "step":   M v/dt + f_int(X + dt v) - f_ext + rho dt^2 v|_fixed = 0 on all DOFs
+                   (one implicit step with the augmented-Lagrangian penalty at lambda = 0);
+    mode "static": f_int(X + dt v) = f_ext on the free DOFs with v = 0 at the fixed nodes
+                   (the fixed point of several outer iterations, because the reference
+                   sets v_prev = v_guess after every outer iteration, SyncedNesterov.cu:294-296).
+    """
+    import scipy.sparse as sp
+    lam, mu = prob.lam, prob.mu
+    M3 = np.kron(sp.csr_matrix((prob.m_val, prob.m_col, prob.m_off)).toarray(), np.eye(3))
+    fixed = np.zeros(3 * m.n_nodes, bool)
+    for n in m.fixed:
+        fixed[3 * n:3 * n + 3] = True
+    free = ~fixed
+    X = m.xyz()
+    v = np.zeros(3 * m.n_nodes)
+    for _ in range(iters):
+        x = X + dt * v.reshape(-1, 3)
+        f = O.internal_force(m, prob.g, prob.detJ, lam, mu, x[:, 0].copy(), x[:, 1].copy(), x[:, 2].copy())
+        K, *_ = _K(m, lam, mu, x[:, 0].copy(), x[:, 1].copy(), x[:, 2].copy())
+        if mode == "step":
+            g = M3 @ v / dt + f - m.f_ext + rho * dt * dt * v * fixed
+            J = M3 / dt + dt * K + np.diag(rho * dt * dt * fixed)
+            dv = np.linalg.solve(J, -g)
+            v += dv
+        else:
+            g = f - m.f_ext
+            dv = np.linalg.solve(dt * K[np.ix_(free, free)], -g[free])
+            v[free] += dv
+        if np.linalg.norm(dv) < 1e-15 * max(1e-300, np.linalg.norm(v)):
+            break
+    return v, free
+
+
+def test_nesterov_single_outer_reaches_implicit_step():
+    """One outer iteration of the restated Nesterov loop, run long, lands on the direct
+    solution of its implicit step (penalty rho at lambda = 0)."""
+    m = load_mesh("cube")
+    prob = O.Problem(m, 7e8, 0.33, 2700.0)
+    dt, rho = 1e-3, 1e10
+    st = prob.new_state()
+    prob.nesterov_step(st, 1e-8, rho, 0.0, 0.0, 1, 100000, dt)
+    v_ref, free = _newton(m, prob, dt, "step", rho=rho)
+    assert rel_err(st["v_guess"], v_ref) < 1e-9
+
+
+def test_nesterov_outer_loop_approaches_static_equilibrium():
+    """With the reference's rho = 1e14 and many outer iterations the step tends to static
+    equilibrium (the augmented-Lagrangian fixed point converges slowly: a 5e-3 band)."""
+    m = load_mesh("cube")
+    prob = O.Problem(m, 7e8, 0.33, 2700.0)
+    dt = 1e-3
+    st = prob.new_state()
+    prob.nesterov_step(st, 1e-8, 1e14, 0.0, 0.0, 60, 5000, dt)
+    v_ref, free = _newton(m, prob, dt, "static")
+    assert rel_err(st["v_guess"][free], v_ref[free]) < 5e-3
+    assert np.abs(st["v_guess"][~free]).max() < 1e-2 * np.abs(v_ref).max()
+
+
+@pytest.mark.parametrize("nocoop", [False, True])
+def test_adamw_single_outer_approaches_implicit_step(nocoop):
+    """AdamW's iterates are pinned by restatement only ("parity unpinned" for the exact
+    sequence); its one-outer result is checked to approach the direct implicit-step
+    solution (fixed learning rate -> a loose band)."""
+    m = load_mesh("cube")
+    prob = O.Problem(m, 7e8, 0.33, 2700.0)
+    dt, rho = 1e-3, 1e10
+    st = prob.new_state()
+    prob.adamw_step(st, 1e-4, 0.9, 0.999, 1e-8, 0.0, 0.999, 0.0, 0.0, rho, 1, 4000, dt, 10, 0.0, nocoop=nocoop)
+    v_ref, free = _newton(m, prob, dt, "step", rho=rho)
+    assert rel_err(st["v_guess"], v_ref) < 5e-2
+
+
+def test_nesterov_trajectory_matches_python_restatement():
+    """Independent pure-Python restatement of one Nesterov inner loop (alpha, beta sequence,
+    look-ahead, gradient, update) on the cube for a handful of iterations."""
+    m = load_mesh("cube")
+    prob = O.Problem(m, 7e8, 0.33, 2700.0)
+    dt, alpha, rho = 1e-3, 1e-8, 1e10
+    st = prob.new_state()
+    prob.nesterov_step(st, alpha, rho, 0.0, 0.0, 1, 6, dt)
+    import scipy.sparse as sp
+    M3 = np.kron(sp.csr_matrix((prob.m_val, prob.m_col, prob.m_off)).toarray(), np.eye(3))
+    X = m.xyz()
+    vk = np.zeros(3 * m.n_nodes)
+    vkm1 = vk.copy()
+    t = 1.0
+    for _ in range(6):
+        tn = 0.5 * (1 + np.sqrt(1 + 4 * t * t))
+        y = vk + (t - 1) / tn * (vk - vkm1)
+        x = X + dt * y.reshape(-1, 3)
+        f = O.internal_force(m, prob.g, prob.detJ, prob.lam, prob.mu, x[:, 0].copy(), x[:, 1].copy(), x[:, 2].copy())
+        g = M3 @ y / dt + f - m.f_ext
+        for n in m.fixed:
+            g[3 * n:3 * n + 3] += dt * rho * (x[n] - X[n])
+        vkm1, vk, t = vk, y - alpha * g, tn
+    assert rel_err(st["v_guess"], vk) < 1e-12

@@ -36,6 +36,31 @@ class Material(C.Structure):
     _fields_ = [("lam", C.c_double), ("mu", C.c_double), ("eta_damp", C.c_double), ("lambda_damp", C.c_double)]
 
 
+class Body(C.Structure):
+    _fields_ = [("nn", C.c_int), ("n_elem", C.c_int), ("n_nodes", C.c_int), ("conn", C.c_void_p),
+                ("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("grad_t", C.c_void_p),
+                ("detJ_t", C.c_void_p), ("quad", Quad), ("mat", Material), ("pattern", C.c_void_p),
+                ("m_sell", C.c_void_p), ("f_ext", C.c_void_p), ("n_fixed", C.c_int), ("fixed_nodes", C.c_void_p),
+                ("xref", C.c_void_p), ("yref", C.c_void_p), ("zref", C.c_void_p), ("f_int", C.c_void_p),
+                ("constraint", C.c_void_p), ("force_plan", C.c_void_p)]
+
+
+class NesterovParams(C.Structure):
+    """Same layout as the reference's SyncedNesterovParams (SyncedNesterov.cuh:31-35)."""
+    _fields_ = [("alpha", C.c_double), ("rho", C.c_double), ("inner_tol", C.c_double), ("outer_tol", C.c_double),
+                ("max_outer", C.c_int), ("max_inner", C.c_int), ("time_step", C.c_double)]
+
+
+class AdamWParams(C.Structure):
+    """Same layout as the reference's SyncedAdamWParams (SyncedAdamW.cuh:32-39)."""
+    _fields_ = [("lr", C.c_double), ("beta1", C.c_double), ("beta2", C.c_double), ("eps", C.c_double),
+                ("weight_decay", C.c_double), ("lr_decay", C.c_double), ("inner_tol", C.c_double),
+                ("outer_tol", C.c_double), ("rho", C.c_double), ("max_outer", C.c_int), ("max_inner", C.c_int),
+                ("time_step", C.c_double), ("convergence_check_interval", C.c_int), ("inner_rtol", C.c_double)]
+
+
+SOLVER_NESTEROV, SOLVER_ADAMW, SOLVER_ADAMW_NOCOOP = 0, 1, 2
+
 vp = C.c_void_p
 i32 = C.c_int
 i64 = C.c_int64
@@ -53,11 +78,17 @@ _SIGS = {
     "tlfea_fint_from_p": (i32, [i32, i32, vp, vp, vp, vp, C.POINTER(Quad), vp, vp]),
     "tlfea_internal_force": (i32, [i32, i32, i32, vp, vp, vp, vp, vp, vp, C.POINTER(Quad), C.POINTER(Material), vp,
                                    vp, i32, vp]),
+    "tlfea_force_plan_create": (i32, [i32, i32, i32, vp, C.POINTER(vp), vp]),
+    "tlfea_force_plan_destroy": (i32, [vp]),
+    "tlfea_force_plan_sizes": (i32, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i64), C.POINTER(i32)]),
+    "tlfea_internal_force_planned": (i32, [vp, vp, vp, vp, vp, vp, C.POINTER(Quad), C.POINTER(Material), vp, vp, i32,
+                                           vp]),
     "tlfea_grad_to_f32": (i32, [i32, i32, vp, vp, C.POINTER(Quad), vp, vp, vp]),
     "tlfea_internal_force_f32": (i32, [i32, i32, i32, vp, vp, vp, vp, vp, vp, C.POINTER(Material), vp, i32, vp]),
     "tlfea_grad_to_ref": (i32, [i32, i32, vp, vp, vp, vp, vp]),
     "tlfea_qp_to_ref": (i32, [i32, i32, vp, vp, vp]),
     "tlfea_pattern_create": (i32, [i32, i32, i32, vp, C.POINTER(vp), vp]),
+    "tlfea_pattern_create_ex": (i32, [i32, i32, i32, vp, i32, C.POINTER(vp), vp]),
     "tlfea_pattern_destroy": (i32, [vp]),
     "tlfea_pattern_sizes": (i32, [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)]),
     "tlfea_pattern_node_csr": (i32, [vp, vp, vp, vp]),
@@ -80,6 +111,14 @@ _SIGS = {
     "tlfea_cg_destroy": (i32, [vp]),
     "tlfea_cg_solve": (i32, [vp, vp, vp, vp, f64, i32, i32, C.POINTER(i32), C.POINTER(f64), vp]),
     "tlfea_add_displacement": (i32, [i32, vp, vp, vp, vp, vp]),
+    "tlfea_solver_create": (i32, [i32, C.POINTER(Body), C.POINTER(vp)]),
+    "tlfea_solver_destroy": (i32, [vp]),
+    "tlfea_solver_set_parameters": (i32, [vp, vp]),
+    "tlfea_solver_set_rel_tol": (i32, [vp, f64]),
+    "tlfea_solver_set_poll_interval": (i32, [vp, i32]),
+    "tlfea_solver_solve": (i32, [vp, vp]),
+    "tlfea_solver_state": (i32, [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]),
+    "tlfea_solver_stats": (i32, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(f64), C.POINTER(f64)]),
 }
 
 _lib = None

@@ -6,7 +6,7 @@
 // element within the 64-element tile, so every array in the AoSoA layout is read and
 // written as one contiguous 512-byte segment per wave instruction.  Per-element kernels
 // run one element per lane and read grad_t the same way (q fixed per instruction).
-#include "common.h"
+#include "internal.h"
 
 namespace tlfea_hip {
 
@@ -283,7 +283,8 @@ __global__ __launch_bounds__(256) void k_force(int n_elem, const int* __restrict
                                                const double* __restrict__ y, const double* __restrict__ z,
                                                const double* __restrict__ grad_t, const double* __restrict__ detJ_t,
                                                tlfea_quad quad, tlfea_material mat, const double* __restrict__ v,
-                                               double* __restrict__ f) {
+                                               double* __restrict__ f, const int* __restrict__ flags) {
+    if (flags && (flags[0] | flags[1])) return;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n_elem) return;
     const int tile = e >> 6, lane = e & (TILE - 1);
@@ -466,6 +467,29 @@ __global__ void k_qp_to_ref(int n_elem, const double* __restrict__ qp_t, double*
 
 using namespace tlfea_hip;
 
+int tlfea_hip::launch_force(int nn, int n_elem, const int* conn, const double* x, const double* y, const double* z,
+                            const double* grad_t, const double* detJ_t, const tlfea_quad& quad,
+                            const tlfea_material& mat, const double* v, double* f, const int* flags, hipStream_t s) {
+    if (n_elem == 0) return TLFEA_OK;
+    const bool damp = (v != nullptr) && (mat.eta_damp != 0.0 || mat.lambda_damp != 0.0);
+    const unsigned g = grid_for(n_elem, 256);
+    if (nn == 10) {
+        if (damp)
+            k_force<10, 5, true><<<g, 256, 0, s>>>(n_elem, conn, x, y, z, grad_t, detJ_t, quad, mat, v, f, flags);
+        else
+            k_force<10, 5, false><<<g, 256, 0, s>>>(n_elem, conn, x, y, z, grad_t, detJ_t, quad, mat, nullptr, f,
+                                                    flags);
+    } else {
+        if (damp)
+            k_force<4, 1, true><<<g, 256, 0, s>>>(n_elem, conn, x, y, z, grad_t, detJ_t, quad, mat, v, f, flags);
+        else
+            k_force<4, 1, false><<<g, 256, 0, s>>>(n_elem, conn, x, y, z, grad_t, detJ_t, quad, mat, nullptr, f,
+                                                   flags);
+    }
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
 // ======================================================================================
 // C-ABI
 // ======================================================================================
@@ -566,21 +590,7 @@ extern "C" int tlfea_internal_force(int nn, int n_elem, int n_nodes, const int* 
     if (!accumulate) TLFEA_HIP_TRY(hipMemsetAsync(f, 0, sizeof(double) * 3 * (size_t)n_nodes, s));
     if (n_elem == 0) return TLFEA_OK;
     TLFEA_REQUIRE(conn && x && y && z && grad_t && detJ_t, "null pointer argument");
-    const bool damp = (v != nullptr) && (mat->eta_damp != 0.0 || mat->lambda_damp != 0.0);
-    const unsigned g = grid_for(n_elem, 256);
-    if (nn == 10) {
-        if (damp)
-            k_force<10, 5, true><<<g, 256, 0, s>>>(n_elem, conn, x, y, z, grad_t, detJ_t, *quad, *mat, v, f);
-        else
-            k_force<10, 5, false><<<g, 256, 0, s>>>(n_elem, conn, x, y, z, grad_t, detJ_t, *quad, *mat, nullptr, f);
-    } else {
-        if (damp)
-            k_force<4, 1, true><<<g, 256, 0, s>>>(n_elem, conn, x, y, z, grad_t, detJ_t, *quad, *mat, v, f);
-        else
-            k_force<4, 1, false><<<g, 256, 0, s>>>(n_elem, conn, x, y, z, grad_t, detJ_t, *quad, *mat, nullptr, f);
-    }
-    TLFEA_LAUNCH_CHECK();
-    return TLFEA_OK;
+    return launch_force(nn, n_elem, conn, x, y, z, grad_t, detJ_t, *quad, *mat, v, f, nullptr, s);
 }
 
 extern "C" int tlfea_grad_to_f32(int nn, int n_elem, const double* grad_t, const double* detJ_t,

@@ -1,0 +1,52 @@
+// Private declarations shared by the library's translation units.
+#pragma once
+
+#include "common.h"
+
+// Node pattern + sliced block layout (built by tlfea_pattern_create, sparse.hip).
+struct tlfea_pattern {
+    int nn = 0, n_elem = 0, n_nodes = 0;
+    int64_t nnz = 0;                  // node-level nnz
+    int64_t n_slices = 0;             // 64-row slices
+    int64_t sell_blocks = 0;          // padded blocks
+    int* d_off = nullptr;             // node CSR offsets [n_nodes+1]
+    int* d_col = nullptr;             // node CSR cols [nnz]
+    int* d_slot = nullptr;            // per (e,a,b): block index in the sliced layout [n_elem*nn*nn]
+    int* d_sell_row = nullptr;        // node id per slice lane [n_slices*64], -1 = padding lane
+    int* d_sell_col = nullptr;        // block column node [sell_blocks]
+    int64_t* d_slice_base = nullptr;  // [n_slices+1]
+    int* d_slice_len = nullptr;       // [n_slices]
+    int* d_row_pos = nullptr;         // node -> permuted position [n_nodes]
+    unsigned char* d_mask = nullptr;  // DOF mask scratch for Dirichlet [3*n_nodes]
+};
+
+// Block-local force plan (element.hip).
+struct tlfea_force_plan {
+    int nn = 0, n_elem = 0, n_nodes = 0;
+    int block_elems = 256;          // elements per block
+    int n_blocks = 0;
+    int max_u = 0;                  // max unique nodes of a block
+    int64_t n_partials = 0;
+    int64_t* d_blk_off = nullptr;   // [n_blocks+1] offsets into the unique-node list / partials
+    int* d_unodes = nullptr;        // [n_partials] global node id of each (block, slot)
+    unsigned short* d_slot = nullptr;  // AoSoA [tile][a][64]: slot of node a of element tile*64+lane
+    double* d_part = nullptr;       // [n_partials*3] block partial forces
+    int* d_g_off = nullptr;         // [n_nodes+1] node -> partial list
+    int64_t* d_g_idx = nullptr;     // partial positions, ascending per node
+};
+
+namespace tlfea_hip {
+
+// Planned fused force (element.hip): partials then gather into f (accumulate or overwrite).
+int launch_force_planned(const tlfea_force_plan* p, const double* x, const double* y, const double* z,
+                         const double* grad_t, const double* detJ_t, const tlfea_quad& quad,
+                         const tlfea_material& mat, const double* v, double* f, int accumulate, const int* flags,
+                         hipStream_t s);
+
+// Fused FP64 force launch (element.hip).  With `flags` non-null the kernel returns at once
+// when flags[0] | flags[1] is set (used inside the iterative solvers' queued iterations).
+int launch_force(int nn, int n_elem, const int* conn, const double* x, const double* y, const double* z,
+                 const double* grad_t, const double* detJ_t, const tlfea_quad& quad, const tlfea_material& mat,
+                 const double* v, double* f, const int* flags, hipStream_t s);
+
+}  // namespace tlfea_hip

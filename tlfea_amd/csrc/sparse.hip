@@ -15,32 +15,17 @@
 #include <numeric>
 #include <vector>
 
-#include "common.h"
+#include "internal.h"
 
 namespace tlfea_hip {
 
-constexpr int WINDOW = 8192;  // rows sorted by length inside this window
+constexpr int WINDOW = 65536;  // rows sorted by length inside this window (measured best on the C2 beam)
 constexpr int MAXROW = 256;   // max distinct nodes per pattern row (setup limit)
 constexpr int RED_BLOCKS = 512;
 constexpr int RED_THREADS = 256;
 
 }  // namespace tlfea_hip
 
-struct tlfea_pattern {
-    int nn = 0, n_elem = 0, n_nodes = 0;
-    int64_t nnz = 0;          // node-level nnz
-    int64_t n_slices = 0;     // 64-row slices
-    int64_t sell_blocks = 0;  // padded blocks
-    int* d_off = nullptr;     // node CSR offsets [n_nodes+1]
-    int* d_col = nullptr;     // node CSR cols [nnz]
-    int* d_slot = nullptr;    // per (e,a,b): block index in the sliced layout [n_elem*nn*nn], (e-major)
-    int* d_sell_row = nullptr;    // node id per slice lane [n_slices*64], -1 = padding lane
-    int* d_sell_col = nullptr;    // block column node [sell_blocks]
-    int64_t* d_slice_base = nullptr;  // [n_slices+1]
-    int* d_slice_len = nullptr;       // [n_slices]
-    int* d_row_pos = nullptr;         // node -> permuted position [n_nodes]
-    unsigned char* d_mask = nullptr;  // DOF mask scratch for Dirichlet [3*n_nodes]
-};
 
 struct tlfea_cg {
     const tlfea_pattern* p = nullptr;
@@ -50,7 +35,7 @@ struct tlfea_cg {
     double* d_Kp = nullptr;
     double* d_part = nullptr;  // [RED_BLOCKS]
     double* d_scal = nullptr;  // [8]: rr0, rr, alpha, beta, rel, pKp
-    int* d_flags = nullptr;    // [4]: 0 done, 1 notpd, 2 iters, 3 block counter
+    int* d_flags = nullptr;    // [4]: 0 done, 1 notpd, 2 iters
 };
 
 namespace tlfea_hip {
@@ -396,8 +381,44 @@ __global__ void k_dirichlet(int64_t n_slices, const int* __restrict__ sell_row, 
 }
 
 // ------------------------------------------------------------------------------------
-// SpMV on the sliced block layout: one lane per node row, 3 outputs.
+// SpMV on the sliced block layout: one lane per node row, 3 outputs.  The block loop is
+// unrolled by 4 so four column indices, their x gathers and 36 value loads are in flight
+// together.
 // ------------------------------------------------------------------------------------
+__device__ __forceinline__ void block_row_product(const double* __restrict__ Kv, const int* __restrict__ cp, int L,
+                                                  const double* __restrict__ xv, double& y0, double& y1,
+                                                  double& y2) {
+    y0 = y1 = y2 = 0.0;
+    int k = 0;
+    for (; k + 4 <= L; k += 4) {
+        int col[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) col[u] = cp[(int64_t)(k + u) * TILE];
+        double xg[4][3];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            xg[u][0] = xv[3 * col[u]];
+            xg[u][1] = xv[3 * col[u] + 1];
+            xg[u][2] = xv[3 * col[u] + 2];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const double* v = Kv + (int64_t)(k + u) * 9 * TILE;
+            y0 += v[0 * TILE] * xg[u][0] + v[1 * TILE] * xg[u][1] + v[2 * TILE] * xg[u][2];
+            y1 += v[3 * TILE] * xg[u][0] + v[4 * TILE] * xg[u][1] + v[5 * TILE] * xg[u][2];
+            y2 += v[6 * TILE] * xg[u][0] + v[7 * TILE] * xg[u][1] + v[8 * TILE] * xg[u][2];
+        }
+    }
+    for (; k < L; k++) {
+        const int col = cp[(int64_t)k * TILE];
+        const double x0 = xv[3 * col], x1 = xv[3 * col + 1], x2 = xv[3 * col + 2];
+        const double* v = Kv + (int64_t)k * 9 * TILE;
+        y0 += v[0 * TILE] * x0 + v[1 * TILE] * x1 + v[2 * TILE] * x2;
+        y1 += v[3 * TILE] * x0 + v[4 * TILE] * x1 + v[5 * TILE] * x2;
+        y2 += v[6 * TILE] * x0 + v[7 * TILE] * x1 + v[8 * TILE] * x2;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_spmv_block(int64_t n_slices, const int* __restrict__ sell_row,
                                                     const int64_t* __restrict__ slice_base,
                                                     const int* __restrict__ slice_len,
@@ -410,18 +431,8 @@ __global__ __launch_bounds__(256) void k_spmv_block(int64_t n_slices, const int*
     const int64_t sl = t >> 6;
     const int lane = (int)(t & (TILE - 1));
     const int64_t base = slice_base[sl];
-    const int L = slice_len[sl];
-    const double* Kp = K + 9 * base + lane;
-    const int* cp = sell_col + base + lane;
-    double y0 = 0, y1 = 0, y2 = 0;
-    for (int k = 0; k < L; k++) {
-        const int col = cp[(int64_t)k * TILE];
-        const double x0 = xv[3 * col], x1 = xv[3 * col + 1], x2 = xv[3 * col + 2];
-        const double* v = Kp + (int64_t)k * 9 * TILE;
-        y0 += v[0 * TILE] * x0 + v[1 * TILE] * x1 + v[2 * TILE] * x2;
-        y1 += v[3 * TILE] * x0 + v[4 * TILE] * x1 + v[5 * TILE] * x2;
-        y2 += v[6 * TILE] * x0 + v[7 * TILE] * x1 + v[8 * TILE] * x2;
-    }
+    double y0, y1, y2;
+    block_row_product(K + 9 * base + lane, sell_col + base + lane, slice_len[sl], xv, y0, y1, y2);
     if (row >= 0) {
         yv[3 * row] = y0;
         yv[3 * row + 1] = y1;
@@ -527,35 +538,22 @@ __global__ void k_add_disp(int n, const double* __restrict__ u, double* __restri
 }
 
 // ------------------------------------------------------------------------------------
-// CG kernels with device-resident scalars.  The last block of each reduction kernel
-// (found with an atomic ticket after a release fence) combines the partials in block
-// order and computes the step scalars, so an iteration is three launches.
-// scal: 0 rr0, 1 rr, 2 alpha, 3 beta, 4 rel, 5 pKp;  flags: 0 done, 1 notpd, 2 iters, 3 ticket
+// CG kernels with device-resident scalars.  Each reduction writes one partial per block;
+// a one-block kernel combines them in block order (deterministic) and computes the step
+// scalars.  The kernel boundary orders the partial writes before the combine, so no
+// device-wide fence (an L2 write-back on this chip) is needed inside the hot kernels.
+// scal: 0 rr0, 1 rr, 2 alpha, 3 beta, 4 rel, 5 pKp;  flags: 0 done, 1 notpd, 2 iters
 // ------------------------------------------------------------------------------------
-__device__ __forceinline__ bool last_block(int* ticket) {
-    __shared__ bool is_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int t = atomicAdd(ticket, 1);
-        is_last = (t == (int)gridDim.x - 1);
-    }
-    __syncthreads();
-    if (is_last) __threadfence();
-    return is_last;
-}
-
 __device__ __forceinline__ double sum_parts(const double* part, int n) {
     double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s += __builtin_nontemporal_load(part + i);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += part[i];
     return block_sum(s);
 }
 
-// r = p = f, u = 0, rr0 = rr = f.f
+// r = p = f, u = 0, partial sums of f.f
 __global__ __launch_bounds__(RED_THREADS) void k_cg_init(int64_t n, const double* __restrict__ f,
                                                          double* __restrict__ u, double* __restrict__ r,
-                                                         double* __restrict__ p, double* __restrict__ part,
-                                                         double* __restrict__ scal, int* __restrict__ flags) {
+                                                         double* __restrict__ p, double* __restrict__ part) {
     double s = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const double fi = f[i];
@@ -566,27 +564,28 @@ __global__ __launch_bounds__(RED_THREADS) void k_cg_init(int64_t n, const double
     }
     s = block_sum(s);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
-    if (last_block(&flags[3])) {
-        const double rr = sum_parts(part, gridDim.x);
-        if (threadIdx.x == 0) {
-            scal[0] = rr;
-            scal[1] = rr;
-            scal[4] = (rr == 0.0) ? 0.0 : 1.0;
-            flags[0] = (rr == 0.0) ? 1 : 0;
-            flags[1] = 0;
-            flags[2] = 0;
-            flags[3] = 0;
-        }
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_cg_init_fin(int nparts, const double* __restrict__ part,
+                                                             double* __restrict__ scal, int* __restrict__ flags) {
+    const double rr = sum_parts(part, nparts);
+    if (threadIdx.x == 0) {
+        scal[0] = rr;
+        scal[1] = rr;
+        scal[4] = (rr == 0.0) ? 0.0 : 1.0;
+        flags[0] = (rr == 0.0) ? 1 : 0;
+        flags[1] = 0;
+        flags[2] = 0;
     }
 }
 
-// Kp = K p on the sliced layout and p.Kp partials; last block: alpha = rr / pKp.
+// Kp = K p on the sliced layout and p.Kp partials.
 __global__ __launch_bounds__(256) void k_cg_spmv(int64_t n_slices, const int* __restrict__ sell_row,
                                                  const int64_t* __restrict__ slice_base,
                                                  const int* __restrict__ slice_len, const int* __restrict__ sell_col,
                                                  const double* __restrict__ K, const double* __restrict__ p,
                                                  double* __restrict__ Kp, double* __restrict__ part,
-                                                 double* __restrict__ scal, int* __restrict__ flags) {
+                                                 const int* __restrict__ flags) {
     if (flags[0]) return;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double s = 0.0;
@@ -596,17 +595,8 @@ __global__ __launch_bounds__(256) void k_cg_spmv(int64_t n_slices, const int* __
         const int lane = (int)(t & (TILE - 1));
         const int64_t base = slice_base[sl];
         const int L = slice_len[sl];
-        const double* Kv = K + 9 * base + lane;
-        const int* cp = sell_col + base + lane;
-        double y0 = 0, y1 = 0, y2 = 0;
-        for (int k = 0; k < L; k++) {
-            const int col = cp[(int64_t)k * TILE];
-            const double x0 = p[3 * col], x1 = p[3 * col + 1], x2 = p[3 * col + 2];
-            const double* v = Kv + (int64_t)k * 9 * TILE;
-            y0 += v[0 * TILE] * x0 + v[1 * TILE] * x1 + v[2 * TILE] * x2;
-            y1 += v[3 * TILE] * x0 + v[4 * TILE] * x1 + v[5 * TILE] * x2;
-            y2 += v[6 * TILE] * x0 + v[7 * TILE] * x1 + v[8 * TILE] * x2;
-        }
+        double y0, y1, y2;
+        block_row_product(K + 9 * base + lane, sell_col + base + lane, L, p, y0, y1, y2);
         if (row >= 0) {
             Kp[3 * row] = y0;
             Kp[3 * row + 1] = y1;
@@ -616,27 +606,30 @@ __global__ __launch_bounds__(256) void k_cg_spmv(int64_t n_slices, const int* __
     }
     s = block_sum(s);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
-    if (last_block(&flags[3])) {
-        const double pKp = sum_parts(part, gridDim.x);
-        if (threadIdx.x == 0) {
-            scal[5] = pKp;
-            if (pKp <= 0.0) {
-                flags[1] = 1;
-                flags[0] = 1;
-            } else {
-                scal[2] = scal[1] / pKp;
-            }
-            flags[3] = 0;
+}
+
+// alpha = rr / pKp, abort flag on pKp <= 0
+__global__ __launch_bounds__(RED_THREADS) void k_cg_alpha(int nparts, const double* __restrict__ part,
+                                                          double* __restrict__ scal, int* __restrict__ flags) {
+    if (flags[0]) return;
+    const double pKp = sum_parts(part, nparts);
+    if (threadIdx.x == 0) {
+        scal[5] = pKp;
+        if (pKp <= 0.0) {
+            flags[1] = 1;
+            flags[0] = 1;
+        } else {
+            scal[2] = scal[1] / pKp;
         }
     }
 }
 
-// u += alpha p, r -= alpha Kp, rr' partials; last block: convergence test and beta.
+// u += alpha p, r -= alpha Kp, partials of r.r
 __global__ __launch_bounds__(RED_THREADS) void k_cg_update(int64_t n, double* __restrict__ u,
                                                            double* __restrict__ r, const double* __restrict__ p,
                                                            const double* __restrict__ Kp, double* __restrict__ part,
-                                                           double* __restrict__ scal, int* __restrict__ flags,
-                                                           double tol) {
+                                                           const double* __restrict__ scal,
+                                                           const int* __restrict__ flags) {
     if (flags[0]) return;
     const double alpha = scal[2];
     double s = 0.0;
@@ -648,20 +641,24 @@ __global__ __launch_bounds__(RED_THREADS) void k_cg_update(int64_t n, double* __
     }
     s = block_sum(s);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
-    if (last_block(&flags[3])) {
-        const double rn = sum_parts(part, gridDim.x);
-        if (threadIdx.x == 0) {
-            const double rr0 = scal[0];
-            const double rel = rr0 > 0.0 ? sqrt(rn / rr0) : sqrt(rn);
-            scal[4] = rel;
-            flags[2] += 1;
-            if (rel < tol) {
-                flags[0] = 1;
-            } else {
-                scal[3] = rn / scal[1];
-                scal[1] = rn;
-            }
-            flags[3] = 0;
+}
+
+// convergence test sqrt(rr'/rr0) < tol (LinearStaticSolver.cu:418-423) and beta = rr'/rr
+__global__ __launch_bounds__(RED_THREADS) void k_cg_beta(int nparts, const double* __restrict__ part,
+                                                         double* __restrict__ scal, int* __restrict__ flags,
+                                                         double tol) {
+    if (flags[0]) return;
+    const double rn = sum_parts(part, nparts);
+    if (threadIdx.x == 0) {
+        const double rr0 = scal[0];
+        const double rel = rr0 > 0.0 ? sqrt(rn / rr0) : sqrt(rn);
+        scal[4] = rel;
+        flags[2] += 1;
+        if (rel < tol) {
+            flags[0] = 1;
+        } else {
+            scal[3] = rn / scal[1];
+            scal[1] = rn;
         }
     }
 }
@@ -706,7 +703,7 @@ void free_pattern(tlfea_pattern* p) {
 }
 
 template <int NN>
-int build_pattern(tlfea_pattern* P, const int* conn, hipStream_t s) {
+int build_pattern(tlfea_pattern* P, const int* conn, int window, hipStream_t s) {
     const int n_elem = P->n_elem, n_nodes = P->n_nodes;
     const int64_t ne = (int64_t)n_elem * NN;
     int *d_cnt = nullptr, *d_inc = nullptr, *d_inc_off = nullptr, *d_rowlen = nullptr, *d_flag = nullptr;
@@ -777,16 +774,16 @@ int build_pattern(tlfea_pattern* P, const int* conn, hipStream_t s) {
     // sliced layout: sort rows by length (descending, stable) inside each window
     std::vector<int> perm(n_nodes);
     std::iota(perm.begin(), perm.end(), 0);
-    for (int w0 = 0; w0 < n_nodes; w0 += WINDOW) {
-        const int w1 = std::min(n_nodes, w0 + WINDOW);
+    for (int w0 = 0; w0 < n_nodes; w0 += window) {
+        const int w1 = std::min(n_nodes, w0 + window);
         std::stable_sort(perm.begin() + w0, perm.begin() + w1,
                          [&](int a, int b) { return h_len[a] > h_len[b]; });
     }
     // slices never straddle windows: pad each window to a multiple of 64 lanes
     std::vector<int> sell_row;
     sell_row.reserve(n_nodes + WINDOW);
-    for (int w0 = 0; w0 < n_nodes; w0 += WINDOW) {
-        const int w1 = std::min(n_nodes, w0 + WINDOW);
+    for (int w0 = 0; w0 < n_nodes; w0 += window) {
+        const int w1 = std::min(n_nodes, w0 + window);
         for (int k = w0; k < w1; k++) sell_row.push_back(perm[k]);
         while (sell_row.size() % TILE) sell_row.push_back(-1);
     }
@@ -845,17 +842,24 @@ int build_pattern(tlfea_pattern* P, const int* conn, hipStream_t s) {
 
 extern "C" int tlfea_pattern_create(int nn, int n_elem, int n_nodes, const int* conn, tlfea_pattern** out,
                                     void* stream) {
+    return tlfea_pattern_create_ex(nn, n_elem, n_nodes, conn, WINDOW, out, stream);
+}
+
+extern "C" int tlfea_pattern_create_ex(int nn, int n_elem, int n_nodes, const int* conn, int window,
+                                       tlfea_pattern** out, void* stream) {
     TLFEA_REQUIRE(out != nullptr, "null output handle");
     *out = nullptr;
     TLFEA_REQUIRE(nn == 10 || nn == 4, "unsupported element: %d nodes", nn);
     TLFEA_REQUIRE(n_elem > 0 && n_nodes > 0 && conn, "empty mesh or null connectivity");
     TLFEA_REQUIRE((int64_t)n_elem * nn * nn < INT32_MAX, "mesh too large for the slot map");
+    TLFEA_REQUIRE(window >= 1, "row-sort window must be >= 1");
+    window = (window + TILE - 1) / TILE * TILE;  // windows hold whole slices
     auto* P = new tlfea_pattern();
     P->nn = nn;
     P->n_elem = n_elem;
     P->n_nodes = n_nodes;
     hipStream_t s = as_stream(stream);
-    const int rc = nn == 10 ? build_pattern<10>(P, conn, s) : build_pattern<4>(P, conn, s);
+    const int rc = nn == 10 ? build_pattern<10>(P, conn, window, s) : build_pattern<4>(P, conn, window, s);
     if (rc != TLFEA_OK) {
         free_pattern(P);
         return rc;
@@ -1079,14 +1083,17 @@ extern "C" int tlfea_cg_solve(tlfea_cg* c, const double* k_sell, const double* f
     const unsigned gs = grid_for(p->n_slices * TILE, 256);
     const unsigned gv = RED_BLOCKS;
     // The SpMV grid can exceed RED_BLOCKS partials: d_part was sized for both.
-    k_cg_init<<<gv, RED_THREADS, 0, s>>>(n, f, u, c->d_r, c->d_p, c->d_part, c->d_scal, c->d_flags);
+    k_cg_init<<<gv, RED_THREADS, 0, s>>>(n, f, u, c->d_r, c->d_p, c->d_part);
+    k_cg_init_fin<<<1, RED_THREADS, 0, s>>>(gv, c->d_part, c->d_scal, c->d_flags);
     TLFEA_LAUNCH_CHECK();
     int flags[4] = {0, 0, 0, 0};
     double scal[8] = {0};
     for (int it = 0; it < max_iter; it++) {
         k_cg_spmv<<<gs, 256, 0, s>>>(p->n_slices, p->d_sell_row, p->d_slice_base, p->d_slice_len, p->d_sell_col,
-                                      k_sell, c->d_p, c->d_Kp, c->d_part, c->d_scal, c->d_flags);
-        k_cg_update<<<gv, RED_THREADS, 0, s>>>(n, u, c->d_r, c->d_p, c->d_Kp, c->d_part, c->d_scal, c->d_flags, tol);
+                                      k_sell, c->d_p, c->d_Kp, c->d_part, c->d_flags);
+        k_cg_alpha<<<1, RED_THREADS, 0, s>>>(gs, c->d_part, c->d_scal, c->d_flags);
+        k_cg_update<<<gv, RED_THREADS, 0, s>>>(n, u, c->d_r, c->d_p, c->d_Kp, c->d_part, c->d_scal, c->d_flags);
+        k_cg_beta<<<1, RED_THREADS, 0, s>>>(gv, c->d_part, c->d_scal, c->d_flags, tol);
         k_cg_dir<<<gv, 256, 0, s>>>(n, c->d_p, c->d_r, c->d_scal, c->d_flags);
         TLFEA_LAUNCH_CHECK();
         if ((it + 1) % check_every == 0 || it + 1 == max_iter) {

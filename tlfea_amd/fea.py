@@ -23,8 +23,9 @@ def _dev(a, dtype):
 
 
 class Body:
-    def __init__(self, mesh, E=0.0, nu=0.0, rho=0.0, eta=0.0, lambda_d=0.0, stream=None):
+    def __init__(self, mesh, E=0.0, nu=0.0, rho=0.0, eta=0.0, lambda_d=0.0, stream=None, window=None):
         self.mesh = mesh
+        self.window = window
         self.nn = mesh.nn
         self.nq = 5 if self.nn == 10 else 1
         self.n_elem = mesh.n_elem
@@ -47,6 +48,7 @@ class Body:
         self.f_ext = _dev(mesh.f_ext if mesh.f_ext is not None else np.zeros(3 * self.n_nodes), torch.float64)
         self.fixed = _dev(mesh.fixed, torch.int32)
         self._pattern = None
+        self._plan = None
         self.precompute()
 
     def set_material(self, E, nu, rho, eta=0.0, lambda_d=0.0):
@@ -96,12 +98,36 @@ class Body:
              int(bool(accumulate)), self.s)
         return f
 
+    @property
+    def plan(self):
+        if self._plan is None:
+            h = C.c_void_p()
+            call("tlfea_force_plan_create", self.nn, self.n_elem, self.n_nodes, ptr(self.conn), C.byref(h), self.s)
+            self._plan = h
+        return self._plan
+
+    def plan_sizes(self):
+        a, b, c, d = C.c_int(), C.c_int(), C.c_int64(), C.c_int()
+        call("tlfea_force_plan_sizes", self.plan, C.byref(a), C.byref(b), C.byref(c), C.byref(d))
+        return dict(block_elems=a.value, n_blocks=b.value, n_partials=c.value, max_block_nodes=d.value)
+
+    def internal_force_planned(self, x=None, v=None, f=None, accumulate=False):
+        x = self.x if x is None else x
+        f = torch.empty(3 * self.n_nodes, dtype=torch.float64, device=DEV) if f is None else f
+        call("tlfea_internal_force_planned", self.plan, ptr(x[0]), ptr(x[1]), ptr(x[2]), ptr(self.grad_t),
+             ptr(self.detJ_t), C.byref(self.quad), C.byref(self.mat), ptr(v), ptr(f), int(bool(accumulate)), self.s)
+        return f
+
     # ---- sparse ------------------------------------------------------------------
     @property
     def pattern(self):
         if self._pattern is None:
             h = C.c_void_p()
-            call("tlfea_pattern_create", self.nn, self.n_elem, self.n_nodes, ptr(self.conn), C.byref(h), self.s)
+            if self.window is None:
+                call("tlfea_pattern_create", self.nn, self.n_elem, self.n_nodes, ptr(self.conn), C.byref(h), self.s)
+            else:
+                call("tlfea_pattern_create_ex", self.nn, self.n_elem, self.n_nodes, ptr(self.conn), int(self.window),
+                     C.byref(h), self.s)
             self._pattern = h
         return self._pattern
 
@@ -171,6 +197,9 @@ class Body:
         if self._pattern is not None:
             capi.lib().tlfea_pattern_destroy(self._pattern)
             self._pattern = None
+        if self._plan is not None:
+            capi.lib().tlfea_force_plan_destroy(self._plan)
+            self._plan = None
 
     def __del__(self):
         try:
@@ -207,7 +236,7 @@ class LinearStatic:
                                        int(self.max_iter), int(self.check_every), C.byref(it), C.byref(res),
                                        self.b.s)
         self.iters, self.res = it.value, res.value
-        if rc not in (capi.TLFEA_OK, capi.TLFEA_ENOCONV):
+        if rc not in (capi.TLFEA_OK, capi.TLFEA_ENOCONV, capi.TLFEA_ENOTPD):
             capi.check(rc)
         return rc
 
@@ -222,6 +251,81 @@ class LinearStatic:
         if self.cg is not None:
             capi.lib().tlfea_cg_destroy(self.cg)
             self.cg = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Solver:
+    """SyncedNesterov / SyncedAdamW / SyncedAdamWNocoop on one Body
+    (`src/solvers/SyncedNesterov.cuh:39-179`, `SyncedAdamW.cuh`, `SyncedAdamWNocoop.cuh`)."""
+
+    def __init__(self, body: Body, kind: int = capi.SOLVER_NESTEROV, planned: bool = True):
+        self.b = body
+        self.kind = kind
+        self.m_sell = body.mass_sell()
+        nf = int(body.fixed.numel())
+        self.constraint = torch.zeros(max(1, 3 * nf), dtype=torch.float64, device=DEV)
+        self.xref = [t.clone() for t in body.X0]
+        d = capi.Body(body.nn, body.n_elem, body.n_nodes, ptr(body.conn), ptr(body.x[0]), ptr(body.x[1]),
+                      ptr(body.x[2]), ptr(body.grad_t), ptr(body.detJ_t), body.quad, body.mat, body.pattern,
+                      ptr(self.m_sell), ptr(body.f_ext), nf, ptr(body.fixed) if nf else None, ptr(self.xref[0]),
+                      ptr(self.xref[1]), ptr(self.xref[2]), ptr(body.f_int), ptr(self.constraint),
+                      body.plan if planned else None)
+        self.desc = d
+        h = C.c_void_p()
+        call("tlfea_solver_create", kind, C.byref(d), C.byref(h))
+        self.h = h
+        vg, vp_, lam = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        call("tlfea_solver_state", h, C.byref(vg), C.byref(vp_), C.byref(lam))
+        self._state = (vg.value, vp_.value, lam.value)
+        self.params = None
+
+    def set_parameters(self, params):
+        self.params = params
+        call("tlfea_solver_set_parameters", self.h, C.cast(C.pointer(params), C.c_void_p))
+
+    def set_rel_tol(self, rel_tol):
+        call("tlfea_solver_set_rel_tol", self.h, float(rel_tol))
+
+    def set_poll_interval(self, every):
+        call("tlfea_solver_set_poll_interval", self.h, int(every))
+
+    def solve(self):
+        call("tlfea_solver_solve", self.h, self.b.s)
+
+    def stats(self):
+        a, b_, c, d = C.c_int(), C.c_int(), C.c_double(), C.c_double()
+        call("tlfea_solver_stats", self.h, C.byref(a), C.byref(b_), C.byref(c), C.byref(d))
+        return dict(inner=a.value, outer=b_.value, norm_g=c.value, rel=d.value)
+
+    def _vec(self, addr, n):
+        out = torch.empty(n, dtype=torch.float64, device=DEV)
+        torch.cuda.synchronize()
+        import ctypes
+        hip = ctypes.CDLL("libamdhip64.so")
+        hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        rc = hip.hipMemcpy(ptr(out), addr, 8 * n, 3)  # device to device
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy failed: {rc}")
+        return out
+
+    def v_guess(self):
+        return self._vec(self._state[0], 3 * self.b.n_nodes)
+
+    def v_prev(self):
+        return self._vec(self._state[1], 3 * self.b.n_nodes)
+
+    def lambda_guess(self):
+        return self._vec(self._state[2], 3 * int(self.b.fixed.numel()))
+
+    def close(self):
+        if getattr(self, "h", None) is not None:
+            capi.lib().tlfea_solver_destroy(self.h)
+            self.h = None
 
     def __del__(self):
         try:
