@@ -47,6 +47,9 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cells", type=str, default="200,20,20")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="one launch each of the planned force kernel, the block SpMV and a 512 MiB copy "
+                         "(the copy calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE), then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -60,6 +63,23 @@ def main():
     capi.lib()
     nx, ny, nz = (int(v) for v in args.cells.split(","))
     E, nu, rho = 2.1e11, 0.3, 7850.0
+    if args.pmc_probe:
+        m = M.clamp_and_tip_load(M.kuhn_beam(nx, ny, nz), -1.0e5)
+        b = Body(m, E, nu, rho)
+        f = torch.zeros(3 * m.n_nodes, dtype=torch.float64, device="cuda")
+        b.plan
+        k = b.stiffness_sell()
+        p = torch.randn(3 * m.n_nodes, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(p)
+        torch.cuda.synchronize()
+        b.internal_force_planned(f=f)
+        b.spmv(k, p, y)
+        src = torch.randn(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("pmc probe done")
+        return
     t0 = time.time()
     m = M.clamp_and_tip_load(M.kuhn_beam(nx, ny, nz), -1.0e5)
     log(f"mesh {m.n_elem} el / {m.n_nodes} nodes in {time.time() - t0:.1f}s")
@@ -151,6 +171,10 @@ def main():
                          f"{iters} iterations: {t_asm:.1f} s assembly + {t_it * 1e3:.1f} ms/iteration"}
         log(f"cpu baseline: {cpu_solve_s:.1f} s per solve")
 
+    # HBM bytes per SpMV launch on this workload from the two rocprofv3 PMC passes
+    # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, scales calibrated on a 512 MiB copy; see
+    # profiles/pmc_traffic_c2_force_spmv.md); only known for the default 200x20x20 beam
+    traffic = (890247 * 2.0 + 31606) * 1024 if (nx, ny, nz) == (200, 20, 20) else None
     out = {
         "metric": "TET10 solves/s at 1e-8 rel. residual (learned+FP32 vs plain); force-kernel GB/s",
         "value": args.steps / dt,
@@ -169,7 +193,7 @@ def main():
                    "n_elem": m.n_elem, "n_dof": 3 * m.n_nodes, "cg_iterations": iters,
                    "final_rel_residual": ls.res, "K_nnz": dof_nnz},
         "roofline": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": None, "kernel": "k_cg_spmv/k_spmv_block",
+                     "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_cg_spmv/k_spmv_block",
                      "avg_us": spmv_s * 1e6, "bytes_per_launch": spmv_bytes},
         "force_kernel": {"bound": "hbm", "achieved": force_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": force_gbs / HBM_PEAK_GBS, "kernel": "k_force_planned<10,5,false> + k_gather_partials",

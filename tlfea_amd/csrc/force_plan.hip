@@ -246,7 +246,8 @@ extern "C" int tlfea_force_plan_create(int nn, int n_elem, int n_nodes, const in
     }
     P->max_u = max_u;
     P->n_partials = blk_off[nb];
-    // LDS: up to 9 arrays of max_u doubles (positions, forces, velocities)
+    // LDS: up to 9 arrays of max_u doubles (positions, forces, velocities) within the
+    // 160 KiB of a gfx950 CU
     if ((size_t)max_u * 9 * sizeof(double) > 160 * 1024 || max_u > 65535) {
         free_plan(P);
         set_error("a block of %d elements touches %d nodes: exceeds the LDS budget", PLAN_BLOCK, max_u);

@@ -100,11 +100,18 @@ class Body:
 
     @property
     def plan(self):
+        """Block-local force plan, or None when the mesh order is too scattered for the LDS
+        budget (then the atomic-scatter kernel is used)."""
         if self._plan is None:
             h = C.c_void_p()
-            call("tlfea_force_plan_create", self.nn, self.n_elem, self.n_nodes, ptr(self.conn), C.byref(h), self.s)
-            self._plan = h
-        return self._plan
+            rc = capi.lib().tlfea_force_plan_create(self.nn, self.n_elem, self.n_nodes, ptr(self.conn), C.byref(h),
+                                                     self.s)
+            if rc == capi.TLFEA_ECAPACITY:
+                self._plan = False
+            else:
+                capi.check(rc)
+                self._plan = h
+        return self._plan or None
 
     def plan_sizes(self):
         a, b, c, d = C.c_int(), C.c_int(), C.c_int64(), C.c_int()
@@ -197,9 +204,9 @@ class Body:
         if self._pattern is not None:
             capi.lib().tlfea_pattern_destroy(self._pattern)
             self._pattern = None
-        if self._plan is not None:
+        if self._plan:
             capi.lib().tlfea_force_plan_destroy(self._plan)
-            self._plan = None
+        self._plan = None
 
     def __del__(self):
         try:
