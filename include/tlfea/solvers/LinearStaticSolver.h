@@ -1,0 +1,44 @@
+// tlfea/solvers/LinearStaticSolver.h -- linear static solve K(x) u = f_ext with Dirichlet
+// elimination and CG, then x += u (reference: src/solvers/LinearStaticSolver.cuh:38-122).
+#pragma once
+
+#include "../../tlfea_hip.h"
+#include "../elements/FEAT10Data.h"
+#include "../elements/FEAT4Data.h"
+#include "SolverBase.h"
+
+namespace tlfea {
+
+template <typename TData>
+class LinearStaticSolver : public SolverBase {
+  public:
+    // The caller has run Setup(), SetSVK(), CalcDnDuPre() and optionally SetNodalFixed() /
+    // SetExternalForce().  tol is the CG relative residual sqrt(r.r / r0.r0).
+    LinearStaticSolver(TData* data, Real tol = 1e-10, int max_iter = 10000);
+    ~LinearStaticSolver() override;
+    LinearStaticSolver(const LinearStaticSolver&) = delete;
+    LinearStaticSolver& operator=(const LinearStaticSolver&) = delete;
+
+    void Solve() override;
+    void SetParameters(void*) override {}
+    int GetLastIterCount() const { return last_iter_count_; }
+    Real GetLastResidual() const { return last_residual_; }
+
+  private:
+    TData* data_;
+    int n_dof_;
+    Real cg_tol_;
+    int cg_max_iter_;
+    int last_iter_count_ = 0;
+    Real last_residual_ = 0.0;
+    tlfea_cg* cg_ = nullptr;
+    double *k_ = nullptr, *f_ = nullptr, *u_ = nullptr;
+};
+
+using LinearStaticSolverT10 = LinearStaticSolver<GPU_FEAT10_Data>;
+using LinearStaticSolverT4 = LinearStaticSolver<GPU_FEAT4_Data>;
+
+extern template class LinearStaticSolver<GPU_FEAT10_Data>;
+extern template class LinearStaticSolver<GPU_FEAT4_Data>;
+
+}  // namespace tlfea

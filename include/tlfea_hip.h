@@ -192,6 +192,11 @@ int tlfea_cg_destroy(tlfea_cg* c);
 int tlfea_cg_solve(tlfea_cg* c, const double* k_sell, const double* f, double* u, double tol, int max_iter,
                    int check_every, int* iters_out, double* rel_res_out, void* stream);
 
+/* ---- constraints (FEAT10DataFunc.cuh:470-481, FEAT10Data.cu:325-339) -------------- */
+/* c[3k+d] = x_d(fixed[k]) - x_ref,d(fixed[k]) for k < n_fixed. */
+int tlfea_constraint_eval(int n_fixed, const int* fixed_nodes, const double* x, const double* y, const double* z,
+                          const double* xref, const double* yref, const double* zref, double* c, void* stream);
+
 /* ---- position update (LinearStaticSolver.cu:189-196,449-463) ----------------------- */
 int tlfea_add_displacement(int n_nodes, const double* u, double* x, double* y, double* z, void* stream);
 
@@ -246,6 +251,9 @@ typedef struct tlfea_solver tlfea_solver;
  * The body descriptor is copied; its device arrays must outlive the solver. */
 int tlfea_solver_create(int kind, const tlfea_body* body, tlfea_solver** out);
 int tlfea_solver_destroy(tlfea_solver* s);
+/* Replace the body descriptor (same mesh sizes and constraint count), e.g. after a material
+ * change; the solver state is kept. */
+int tlfea_solver_update_body(tlfea_solver* s, const tlfea_body* body);
 /* SetParameters: stores the parameters and zeroes v_guess, v_prev and lambda
  * (SyncedNesterov.cuh:139-155, SyncedAdamW.cuh:160-190).  params points at a
  * tlfea_nesterov_params or tlfea_adamw_params according to the solver kind. */

@@ -47,8 +47,12 @@ def _compile(src: str) -> str:
     return obj
 
 
+DRIVER = os.path.join(OUT, "bin", "tlfea_driver")
+
+
 def build(verbose: bool = False, jobs: int | None = None) -> str:
-    """Compile every source (always; no staleness check) and link the shared library."""
+    """Compile every source (always; no staleness check), link the shared library and the
+    drivers."""
     os.makedirs(OUT, exist_ok=True)
     hip, cpp = sources()
     srcs = hip + cpp
@@ -61,6 +65,17 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")
     os.replace(tmp, LIB)
+    # command-line drivers over the drop-in C++ API, linked against the library
+    bindir = os.path.join(OUT, "bin")
+    os.makedirs(bindir, exist_ok=True)
+    for src in sorted(glob.glob(os.path.join(CSRC, "drivers", "*.cpp"))):
+        exe = os.path.join(bindir, os.path.splitext(os.path.basename(src))[0])
+        cmd = [HIPCC, "-x", "hip", *COMMON_FLAGS, src, "-o", exe + ".tmp", "-L", OUT, "-ltlfea_hip",
+               "-Wl,-rpath,$ORIGIN/.."]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"driver build failed for {src}:\n{r.stderr[-6000:]}")
+        os.replace(exe + ".tmp", exe)
     if verbose:
         print(f"built {LIB} from {len(srcs)} sources", file=sys.stderr)
     return LIB

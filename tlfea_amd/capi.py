@@ -111,8 +111,10 @@ _SIGS = {
     "tlfea_cg_destroy": (i32, [vp]),
     "tlfea_cg_solve": (i32, [vp, vp, vp, vp, f64, i32, i32, C.POINTER(i32), C.POINTER(f64), vp]),
     "tlfea_add_displacement": (i32, [i32, vp, vp, vp, vp, vp]),
+    "tlfea_constraint_eval": (i32, [i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "tlfea_solver_create": (i32, [i32, C.POINTER(Body), C.POINTER(vp)]),
     "tlfea_solver_destroy": (i32, [vp]),
+    "tlfea_solver_update_body": (i32, [vp, C.POINTER(Body)]),
     "tlfea_solver_set_parameters": (i32, [vp, vp]),
     "tlfea_solver_set_rel_tol": (i32, [vp, f64]),
     "tlfea_solver_set_poll_interval": (i32, [vp, i32]),

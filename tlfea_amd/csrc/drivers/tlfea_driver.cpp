@@ -1,0 +1,209 @@
+// Command-line driver over the drop-in C++ API (FEASolver, GPU_FEAT10_Data / GPU_FEAT4_Data,
+// LinearStaticSolver, SyncedNesterovSolver, SyncedAdamW[Nocoop]Solver).  It runs the
+// reference's example scenarios (examples/beam_linear_static.cc, beam_linear_static_t4.cc,
+// test_feat10_nesterov.cc, test_feat10_adamw.cc) and writes the final nodal positions as
+// raw float64 (x block, y block, z block) to --out, plus a JSON summary on stdout.
+//
+//   tlfea_driver linear_static  --mesh kuhn:20,2,2 | kuhn4:12,3,2 | vtu:PATH | tetgen:PREFIX
+//                 [--E 2.1e11 --nu 0.3 --load -1000 --tol 1e-10 --max-iter 50000] --out FILE
+//   tlfea_driver nesterov|adamw|adamw_nocoop --mesh tetgen:PREFIX|kuhn:.. [--steps 3] [--E ..]
+//                 [--rho0 ..] [--fix z0|x0] [--node-load NODE,FX] --out FILE
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../../include/tlfea/FEASolver.h"
+#include "../../../include/tlfea/solvers/LinearStaticSolver.h"
+#include "../../../include/tlfea/solvers/SyncedAdamWNocoop.h"
+#include "../../../include/tlfea/utils/mesh_io.h"
+#include "../../../include/tlfea/utils/quadrature_utils.h"
+
+using namespace tlfea;
+
+namespace {
+std::map<std::string, std::string> parse(int argc, char** argv) {
+    std::map<std::string, std::string> a;
+    for (int i = 2; i + 1 < argc; i += 2) a[argv[i]] = argv[i + 1];
+    return a;
+}
+double num(std::map<std::string, std::string>& a, const char* k, double d) {
+    return a.count(k) ? std::atof(a[k].c_str()) : d;
+}
+
+bool load_mesh(const std::string& spec, MatrixXR& nodes, MatrixXi& elems) {
+    if (spec.rfind("kuhn", 0) == 0) {
+        const bool t4 = spec.rfind("kuhn4:", 0) == 0;
+        int nx, ny, nz;
+        if (std::sscanf(spec.c_str() + (t4 ? 6 : 5), "%d,%d,%d", &nx, &ny, &nz) != 3) return false;
+        GenerateKuhnBeam(nx, ny, nz, 1.0, 0.1, 0.1, !t4, nodes, elems);
+        return true;
+    }
+    if (spec.rfind("vtu:", 0) == 0) return LoadVtuTet(spec.substr(4), nodes, elems) > 0;
+    if (spec.rfind("tetgen:", 0) == 0) {
+        const std::string p = spec.substr(7);
+        return FEAT10_read_nodes(p + ".node", nodes) > 0 && FEAT10_read_elements(p + ".ele", elems) > 0;
+    }
+    return false;
+}
+
+void write_positions(const std::string& path, const VectorXR& x, const VectorXR& y, const VectorXR& z) {
+    std::ofstream f(path, std::ios::binary);
+    for (const VectorXR* v : {&x, &y, &z}) f.write(reinterpret_cast<const char*>(v->data()), sizeof(double) * v->size());
+}
+
+template <typename TData>
+int run(const std::string& mode, std::map<std::string, std::string>& a, MatrixXR& nodes, MatrixXi& elems) {
+    const int n_nodes = (int)nodes.rows(), n_el = (int)elems.rows();
+    VectorXR hx(n_nodes), hy(n_nodes), hz(n_nodes);
+    for (int i = 0; i < n_nodes; i++) {
+        hx(i) = nodes(i, 0);
+        hy(i) = nodes(i, 1);
+        hz(i) = nodes(i, 2);
+    }
+    TData body(n_el, n_nodes);
+    FEASolver fea;
+    fea.AddElement("body", &body);
+    auto* b = static_cast<TData*>(fea.GetElement("body"));
+    b->Initialize();
+    const bool t10 = TData::N_NODES_PER_ELEM == 10;
+    // boundary conditions and loads, as the reference drivers set them
+    std::vector<int> fixed;
+    VectorXR f(3 * n_nodes);
+    f.setZero();
+    const std::string fix = a.count("--fix") ? a["--fix"] : (mode == "linear_static" ? "x0" : "z0");
+    const Real xmin = hx.minCoeff(), xmax = hx.maxCoeff(), L = xmax - xmin;
+    for (int i = 0; i < n_nodes; i++) {
+        if (fix == "z0" && std::abs(hz(i)) < 1e-8) fixed.push_back(i);
+        if (fix == "x0" && std::abs(hx(i) - xmin) < 5e-4 * L) fixed.push_back(i);
+    }
+    if (a.count("--node-load")) {
+        int node;
+        double fx;
+        std::sscanf(a["--node-load"].c_str(), "%d,%lf", &node, &fx);
+        f(3 * node + 0) = fx;
+    } else {
+        std::vector<int> tip;
+        for (int i = 0; i < n_nodes; i++)
+            if (std::abs(hx(i) - xmax) < 5e-4 * L) tip.push_back(i);
+        const Real load = num(a, "--load", -1000.0);
+        for (int i : tip) f(3 * i + 2) = load / (Real)tip.size();
+    }
+    VectorXi hf((std::ptrdiff_t)fixed.size());
+    for (size_t k = 0; k < fixed.size(); k++) hf((std::ptrdiff_t)k) = fixed[k];
+    b->SetNodalFixed(hf);
+    b->SetExternalForce(f);
+    if (t10)
+        b->Setup(Quadrature::tet5pt_x, Quadrature::tet5pt_y, Quadrature::tet5pt_z, Quadrature::tet5pt_weights, hx, hy,
+                 hz, elems);
+    else
+        b->Setup(Quadrature::tet1pt_x, Quadrature::tet1pt_y, Quadrature::tet1pt_z, Quadrature::tet1pt_weights, hx, hy,
+                 hz, elems);
+    b->SetDensity(num(a, "--rho0", 7850.0));
+    b->SetDamping(num(a, "--eta", 0.0), num(a, "--lambda-d", 0.0));
+    b->SetSVK(num(a, "--E", 2.1e11), num(a, "--nu", 0.3));
+    b->CalcDnDuPre();
+    auto t0 = std::chrono::steady_clock::now();
+    std::string extra;
+    if (mode == "linear_static") {
+        LinearStaticSolver<TData> solver(b, num(a, "--tol", 1e-10), (int)num(a, "--max-iter", 50000));
+        fea.AddSolver("linear_static", &solver);
+        fea.GetSolver("linear_static")->Solve();
+        extra = ", \"iters\": " + std::to_string(solver.GetLastIterCount()) +
+                ", \"residual\": " + std::to_string(solver.GetLastResidual());
+    } else {
+        b->CalcMassMatrix();
+        b->CalcConstraintData();
+        b->ConvertToCSR_ConstraintJacT();
+        const int steps = (int)num(a, "--steps", 3);
+        std::string iters;
+        if (mode == "nesterov") {
+            SyncedNesterovSolver solver(b, b->get_n_constraint());
+            solver.Setup();
+            SyncedNesterovParams p = {num(a, "--alpha", 1e-8), num(a, "--rho", 1e14), num(a, "--inner-tol", 1e-6),
+                                      num(a, "--outer-tol", 1e-6), (int)num(a, "--max-outer", 5),
+                                      (int)num(a, "--max-inner", 300), num(a, "--dt", 1e-3)};
+            solver.SetParameters(&p);
+            fea.AddSolver("nesterov", &solver);
+            for (int k = 0; k < steps; k++) {
+                fea.GetSolver("nesterov")->Solve();
+                iters += (k ? "," : "") + std::to_string(solver.GetLastInnerIterations());
+            }
+        } else {
+            SyncedAdamWParams p = {num(a, "--lr", 2e-4), 0.9, 0.999, 1e-8, num(a, "--wd", 1e-4),
+                                   num(a, "--lr-decay", 0.995), num(a, "--inner-tol", 1e-1),
+                                   num(a, "--outer-tol", 1e-6), num(a, "--rho", 1e14), (int)num(a, "--max-outer", 5),
+                                   (int)num(a, "--max-inner", 300), num(a, "--dt", 1e-3), 10, 0.0};
+            if (mode == "adamw") {
+                SyncedAdamWSolver solver(b, b->get_n_constraint());
+                solver.Setup();
+                solver.SetParameters(&p);
+                fea.AddSolver("adamw", &solver);
+                for (int k = 0; k < steps; k++) {
+                    fea.GetSolver("adamw")->Solve();
+                    iters += (k ? "," : "") + std::to_string(solver.GetLastInnerIterations());
+                }
+            } else {
+                SyncedAdamWNocoopSolver solver(b, b->get_n_constraint());
+                solver.Setup();
+                solver.SetParameters(&p);
+                fea.AddSolver("adamw_nocoop", &solver);
+                for (int k = 0; k < steps; k++) {
+                    fea.GetSolver("adamw_nocoop")->Solve();
+                    iters += (k ? "," : "") + std::to_string(solver.GetLastInnerIterations());
+                }
+            }
+        }
+        extra = ", \"inner_iters\": [" + iters + "]";
+    }
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    VectorXR x, y, z;
+    b->RetrievePositionToCPU(x, y, z);
+    if (a.count("--out")) write_positions(a["--out"], x, y, z);
+    if (a.count("--vtk")) b->WriteOutputVTK(a["--vtk"]);
+    Real tip = 0.0;
+    for (int i = 0; i < n_nodes; i++)
+        if (std::abs(hx(i) - xmax) < 5e-4 * L) tip = std::min(tip, z(i) - hz(i));
+    std::printf("{\"mode\": \"%s\", \"n_elem\": %d, \"n_nodes\": %d, \"n_fixed\": %zu, \"seconds\": %.6f, "
+                "\"tip_uz\": %.12e%s}\n",
+                mode.c_str(), n_el, n_nodes, fixed.size(), secs, tip, extra.c_str());
+    b->Destroy();
+    return 0;
+}
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 2) {
+        std::fprintf(stderr, "usage: tlfea_driver linear_static|nesterov|adamw|adamw_nocoop --mesh SPEC [...]\n");
+        return 2;
+    }
+    const std::string mode = argv[1];
+    auto a = parse(argc, argv);
+    MatrixXR nodes;
+    MatrixXi elems;
+    if (!a.count("--mesh") || !load_mesh(a["--mesh"], nodes, elems)) {
+        std::fprintf(stderr, "could not load mesh\n");
+        return 2;
+    }
+    if (mode == "mesh_info") {  // no device work: dumps the mesh as raw float64 nodes + int32 elements
+        if (a.count("--out")) {
+            std::ofstream f(a["--out"], std::ios::binary);
+            f.write(reinterpret_cast<const char*>(nodes.data()), sizeof(double) * nodes.size());
+            f.write(reinterpret_cast<const char*>(elems.data()), sizeof(int) * elems.size());
+        }
+        std::printf("{\"n_nodes\": %td, \"n_elem\": %td, \"nn\": %td}\n", nodes.rows(), elems.rows(), elems.cols());
+        return 0;
+    }
+    try {
+        if (elems.cols() == 10) return run<GPU_FEAT10_Data>(mode, a, nodes, elems);
+        return run<GPU_FEAT4_Data>(mode, a, nodes, elems);
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+}

@@ -1,0 +1,162 @@
+// Drop-in solver classes over the C-ABI (tlfea_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "../../../include/tlfea/solvers/LinearStaticSolver.h"
+#include "../../../include/tlfea/solvers/SyncedAdamWNocoop.h"
+
+namespace tlfea {
+
+namespace {
+void lib_check(int rc, const char* what) {
+    if (rc != TLFEA_OK) throw std::runtime_error(std::string(what) + ": " + tlfea_last_error());
+}
+
+detail::TetElementData* as_tet(ElementBase* data) {
+    if (data == nullptr) return nullptr;
+    if (data->type != TYPE_T10 && data->type != TYPE_T4) return nullptr;
+    return static_cast<detail::TetElementData*>(data);
+}
+
+static_assert(sizeof(SyncedNesterovParams) == sizeof(tlfea_nesterov_params), "param layout");
+static_assert(sizeof(SyncedAdamWParams) == sizeof(tlfea_adamw_params), "param layout");
+}  // namespace
+
+namespace detail {
+
+// SyncedNesterov.cuh:39-110: the reference dispatches on the element type and rejects
+// unknown types; ANCF elements are outside this build.
+VelocitySolverBase::VelocitySolverBase(ElementBase* data, int n_constraints, int kind)
+    : data_(data), kind_(kind), n_coef_(data ? data->get_n_coef() : 0), n_constraints_(n_constraints) {
+    std::memset(params_, 0, sizeof(params_));
+    if (!as_tet(data))
+        std::fprintf(stderr, "[TLFEA ERROR] unsupported element type for this solver (TET10 and TET4 only)\n");
+}
+
+VelocitySolverBase::~VelocitySolverBase() {
+    if (h_) tlfea_solver_destroy(h_);
+}
+
+// SyncedNesterov.cuh:157-179: (re)allocates zeroed state.  The element's mass matrix must
+// have been computed (CalcMassMatrix), as the reference's gradient reads it.
+void VelocitySolverBase::Setup() {
+    TetElementData* e = as_tet(data_);
+    if (!e) return;
+    if (h_) {
+        tlfea_solver_destroy(h_);
+        h_ = nullptr;
+    }
+    tlfea_body b = e->BodyDescriptor();
+    if (3 * b.n_fixed != n_constraints_)
+        std::fprintf(stderr,
+                     "[TLFEA ERROR] solver built for %d constraints but the element has %d; using the element's\n",
+                     n_constraints_, 3 * b.n_fixed);
+    lib_check(tlfea_solver_create(kind_, &b, &h_), "solver Setup");
+    if (have_params_) lib_check(tlfea_solver_set_parameters(h_, params_), "solver Setup");
+    lib_check(tlfea_solver_set_rel_tol(h_, rel_tol_), "solver Setup");
+}
+
+void VelocitySolverBase::set_params(const void* p) {
+    if (!p) return;
+    std::memcpy(params_, p, kind_ == TLFEA_SOLVER_NESTEROV ? sizeof(tlfea_nesterov_params) : sizeof(tlfea_adamw_params));
+    have_params_ = true;
+    if (!h_) Setup();
+    if (h_) lib_check(tlfea_solver_set_parameters(h_, params_), "SetParameters");
+}
+
+void VelocitySolverBase::SetRelativeTolerance(Real rel_tol) {
+    rel_tol_ = rel_tol;
+    if (h_) lib_check(tlfea_solver_set_rel_tol(h_, rel_tol), "SetRelativeTolerance");
+}
+
+void VelocitySolverBase::Solve() {
+    TetElementData* e = as_tet(data_);
+    if (!e) return;
+    if (!h_) Setup();
+    if (!have_params_) {
+        std::fprintf(stderr, "[TLFEA ERROR] SetParameters must be called before Solve\n");
+        return;
+    }
+    tlfea_body b = e->BodyDescriptor();  // picks up material or load changes
+    lib_check(tlfea_solver_update_body(h_, &b), "Solve");
+    lib_check(tlfea_solver_solve(h_, nullptr), "Solve");
+    double ng = 0.0;
+    lib_check(tlfea_solver_stats(h_, &last_inner_, &last_outer_, &ng, &last_rel_), "Solve");
+}
+
+Real* VelocitySolverBase::GetVelocityDevicePtr() {
+    double* p = nullptr;
+    if (h_) tlfea_solver_state(h_, &p, nullptr, nullptr);
+    return p;
+}
+Real* VelocitySolverBase::GetPrevVelocityDevicePtr() {
+    double* p = nullptr;
+    if (h_) tlfea_solver_state(h_, nullptr, &p, nullptr);
+    return p;
+}
+Real* VelocitySolverBase::GetLambdaDevicePtr() {
+    double* p = nullptr;
+    if (h_) tlfea_solver_state(h_, nullptr, nullptr, &p);
+    return p;
+}
+
+}  // namespace detail
+
+// ---------------------------------------------------------------------------------------
+// LinearStaticSolver (LinearStaticSolver.cu:206-481)
+// ---------------------------------------------------------------------------------------
+template <typename TData>
+LinearStaticSolver<TData>::LinearStaticSolver(TData* data, Real tol, int max_iter)
+    : data_(data), n_dof_(3 * data->get_n_coef()), cg_tol_(tol), cg_max_iter_(max_iter) {}
+
+template <typename TData>
+LinearStaticSolver<TData>::~LinearStaticSolver() {
+    if (cg_) tlfea_cg_destroy(cg_);
+    if (k_) hipFree(k_);
+    if (f_) hipFree(f_);
+    if (u_) hipFree(u_);
+}
+
+template <typename TData>
+void LinearStaticSolver<TData>::Solve() {
+    tlfea_body b = data_->BodyDescriptor();
+    int64_t nnz = 0, dof = 0, nb = 0;
+    lib_check(tlfea_pattern_sizes(b.pattern, &nnz, &dof, &nb), "LinearStaticSolver");
+    if (!cg_) {
+        lib_check(tlfea_cg_create(b.pattern, &cg_), "LinearStaticSolver");
+        if (hipMalloc((void**)&k_, sizeof(double) * 9 * (size_t)nb) != hipSuccess ||
+            hipMalloc((void**)&f_, sizeof(double) * (size_t)n_dof_) != hipSuccess ||
+            hipMalloc((void**)&u_, sizeof(double) * (size_t)n_dof_) != hipSuccess)
+            throw std::runtime_error("LinearStaticSolver: device allocation failed");
+    }
+    // assemble the tangent at the current positions (h = 1), copy f_ext, eliminate fixed DOFs
+    lib_check(tlfea_stiffness_assemble(b.pattern, b.nn, b.n_elem, b.conn, b.x, b.y, b.z, b.grad_t, b.detJ_t, &b.quad,
+                                       &b.mat, 1.0, k_, nullptr),
+              "AssembleLinearStiffness");
+    if (hipMemcpy(f_, b.f_ext, sizeof(double) * n_dof_, hipMemcpyDeviceToDevice) != hipSuccess)
+        throw std::runtime_error("LinearStaticSolver: copy of f_ext failed");
+    if (b.n_fixed > 0)
+        lib_check(tlfea_apply_dirichlet(b.pattern, b.n_fixed, b.fixed_nodes, k_, f_, nullptr), "ApplyDirichletBCs");
+    int iters = 0;
+    double res = 0.0;
+    const int rc = tlfea_cg_solve(cg_, k_, f_, u_, cg_tol_, cg_max_iter_, 32, &iters, &res, nullptr);
+    if (rc == TLFEA_ENOTPD) {
+        std::fprintf(stderr, "[TLFEA ERROR] LinearStaticSolver CG: non-positive p*Kp detected; aborting.\n");
+    } else if (rc != TLFEA_OK && rc != TLFEA_ENOCONV) {
+        lib_check(rc, "SolveLinearSystemCG");
+    }
+    // the reference reports max_iter and the last residual when CG does not converge
+    last_iter_count_ = (rc == TLFEA_ENOCONV) ? cg_max_iter_ : iters;
+    last_residual_ = res;
+    lib_check(tlfea_add_displacement(b.n_nodes, u_, b.x, b.y, b.z, nullptr), "UpdatePositions");
+    if (hipDeviceSynchronize() != hipSuccess) throw std::runtime_error("LinearStaticSolver: device error");
+}
+
+template class LinearStaticSolver<GPU_FEAT10_Data>;
+template class LinearStaticSolver<GPU_FEAT4_Data>;
+
+}  // namespace tlfea

@@ -373,6 +373,18 @@ __global__ __launch_bounds__(S_RED) void k_dual(Body b, double rho, double dt, d
     }
 }
 
+__global__ void k_constraint_eval(int n_fixed, const int* __restrict__ fixed, const double* __restrict__ x,
+                                  const double* __restrict__ y, const double* __restrict__ z,
+                                  const double* __restrict__ xr, const double* __restrict__ yr,
+                                  const double* __restrict__ zr, double* __restrict__ c) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_fixed) return;
+    const int n = fixed[k];
+    c[3 * k] = x[n] - xr[n];
+    c[3 * k + 1] = y[n] - yr[n];
+    c[3 * k + 2] = z[n] - zr[n];
+}
+
 __global__ void k_reset_inner(int* __restrict__ flags) {
     if (threadIdx.x == 0 && blockIdx.x == 0) flags[0] = 0;
 }
@@ -472,6 +484,16 @@ extern "C" int tlfea_solver_create(int kind, const tlfea_body* body, tlfea_solve
 
 extern "C" int tlfea_solver_destroy(tlfea_solver* s) {
     free_solver(s);
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_solver_update_body(tlfea_solver* s, const tlfea_body* body) {
+    TLFEA_REQUIRE(s && body, "null argument");
+    TLFEA_REQUIRE(body->nn == s->b.nn && body->n_elem == s->b.n_elem && body->n_nodes == s->b.n_nodes &&
+                      body->n_fixed == s->b.n_fixed,
+                  "body sizes differ from the solver's");
+    TLFEA_REQUIRE(body->pattern && body->m_sell && body->f_ext && body->f_int, "body is missing device arrays");
+    s->b = *body;
     return TLFEA_OK;
 }
 
@@ -640,5 +662,17 @@ extern "C" int tlfea_solver_solve(tlfea_solver* S, void* stream) {
     S->last_outer = hflags[3];
     S->last_norm_g = hs[1];
     S->last_rel = hs[7];
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_constraint_eval(int n_fixed, const int* fixed_nodes, const double* x, const double* y,
+                                     const double* z, const double* xref, const double* yref, const double* zref,
+                                     double* c, void* stream) {
+    TLFEA_REQUIRE(n_fixed >= 0, "negative constraint count");
+    if (n_fixed == 0) return TLFEA_OK;
+    TLFEA_REQUIRE(fixed_nodes && x && y && z && xref && yref && zref && c, "null argument");
+    k_constraint_eval<<<grid_for(n_fixed, 256), 256, 0, as_stream(stream)>>>(n_fixed, fixed_nodes, x, y, z, xref, yref,
+                                                                           zref, c);
+    TLFEA_LAUNCH_CHECK();
     return TLFEA_OK;
 }
