@@ -6,14 +6,18 @@ Workload (single large mesh): cantilever 1.0 x 0.1 x 0.1 m, 200x20x20 hex cells 
 rho = 7850; nodes at x = 0 fixed; 1e5 N in -z spread over the x = L nodes.
 
 One step = one linear-static solve to a relative residual of 1e-8: tangent stiffness
-assembly at the current positions, Dirichlet elimination, CG with the reference's
-sqrt(rr/rr0) test, position update (the reference's LinearStaticSolver::Solve).  Inputs are
-resident in HBM when the timed region starts.  `value` = solves/s.
+assembly at the current positions, Dirichlet elimination, CG, position update (the
+reference's LinearStaticSolver::Solve).  `value` = solves/s of the FP32+FP64 path (CG with
+the Krylov recursion in FP32 and FP64 reliable updates, converged on the true FP64
+residual ||f - K u||/||f|| < 1e-8); the plain FP64 path (the drop-in, with the reference's
+sqrt(rr/rr0) test) is timed on the same problem and reported beside it.  Inputs are
+resident in HBM when the timed region starts.
 
 Kernel rooflines are measured live with HIP events on the launch stream (the null stream,
 which torch.cuda.Event records on):
-  * roofline      -- the step's dominant kernel, the block SpMV of CG,
-                     bytes/launch = 76*node_nnz + 52*n_nodes;
+  * roofline      -- the step's dominant kernel, the FP32 block SpMV of the mixed CG,
+                     bytes/launch = 40*node_nnz + 28*n_nodes;
+  * spmv_fp64     -- the FP64 block SpMV (plain path), bytes/launch = 76*node_nnz + 52*n_nodes;
   * force_kernel  -- the fused FP64 P + f_int (block-local plan: force + node gather),
                      bytes/launch = 1280*n_elem + 48*n_nodes (the reference byte model).
 cpu_baseline: the FP64 oracle on a bounded sample (a slice of the element assembly and
@@ -74,6 +78,12 @@ def main():
         torch.cuda.synchronize()
         b.internal_force_planned(f=f)
         b.spmv(k, p, y)
+        from tlfea_amd.capi import call as _call, ptr as _ptr
+        k32 = torch.empty(k.numel(), dtype=torch.float32, device="cuda")
+        _call("tlfea_matrix_to_f32", b.pattern, _ptr(k), _ptr(k32), None)
+        p32, y32 = p.float(), torch.empty(3 * m.n_nodes, dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        _call("tlfea_spmv_block_f32", b.pattern, _ptr(k32), _ptr(p32), _ptr(y32), None)
         src = torch.randn(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
@@ -85,28 +95,36 @@ def main():
     log(f"mesh {m.n_elem} el / {m.n_nodes} nodes in {time.time() - t0:.1f}s")
     body = Body(m, E, nu, rho)
     node_nnz, dof_nnz, nb = body.pattern_sizes()
-    ls = LinearStatic(body, tol=args.tol, max_iter=200000, check_every=64)
     X0 = [t.clone() for t in body.x]
-    torch.cuda.synchronize()
 
-    def step():
-        for d in range(3):
-            body.x[d].copy_(X0[d])
-        rc = ls.solve()
-        if rc != 0:
-            raise RuntimeError(f"solve did not converge: rc={rc} iters={ls.iters} res={ls.res}")
+    def timed_solves(ls):
+        def step():
+            for d in range(3):
+                body.x[d].copy_(X0[d])
+            rc = ls.solve()
+            if rc != 0:
+                raise RuntimeError(f"solve did not converge: rc={rc} iters={ls.iters} res={ls.res}")
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t
+    # plain FP64 path (the drop-in): reference CG rule sqrt(rr/rr0) < tol
+    ls64 = LinearStatic(body, tol=args.tol, max_iter=200000, check_every=64)
+    dt64 = timed_solves(ls64)
+    log(f"plain FP64 solve: {dt64 / args.steps * 1e3:.1f} ms, {ls64.iters} CG iterations, rel res {ls64.res:.3e}")
+    # FP32 Krylov recursion + FP64 reliable updates: true FP64 residual < tol
+    ls = LinearStatic(body, tol=args.tol, max_iter=200000, check_every=64, mixed=True)
+    dt = timed_solves(ls)
     iters = ls.iters
     ms_per_step = dt / args.steps * 1e3
-    log(f"solve: {ms_per_step:.1f} ms, {iters} CG iterations, rel res {ls.res:.3e}")
+    log(f"mixed solve: {ms_per_step:.1f} ms, {iters} CG iterations, {ls.refreshes} FP64 refreshes, "
+        f"true rel res {ls.res:.3e}")
+    ls64.close()
 
     # ---- kernel rooflines (HIP events on the null stream) ---------------------------
     R = args.kernel_reps
@@ -125,6 +143,21 @@ def main():
     # and y written once (48 B per node), row id per node (4 B)
     spmv_bytes = 76 * node_nnz + 52 * m.n_nodes
     spmv_gbs = spmv_bytes / spmv_s / 1e9
+    # FP32 block SpMV of the mixed CG (the step's dominant kernel): 36 B values + 4 B column
+    # per block, x read + y write 24 B per node, row id 4 B
+    p32 = p.float()
+    y32 = torch.empty_like(p32)
+    from tlfea_amd.capi import call as _call, ptr as _ptr
+    for _ in range(3):
+        _call("tlfea_spmv_block_f32", body.pattern, _ptr(ls.k32), _ptr(p32), _ptr(y32), None)
+    e0.record()
+    for _ in range(R):
+        _call("tlfea_spmv_block_f32", body.pattern, _ptr(ls.k32), _ptr(p32), _ptr(y32), None)
+    e1.record()
+    torch.cuda.synchronize()
+    spmv32_s = e0.elapsed_time(e1) / 1e3 / R
+    spmv32_bytes = 40 * node_nnz + 28 * m.n_nodes
+    spmv32_gbs = spmv32_bytes / spmv32_s / 1e9
 
     f = torch.zeros(3 * m.n_nodes, dtype=torch.float64, device="cuda")
     body.plan  # build the block-local force plan (setup)
@@ -138,7 +171,8 @@ def main():
     force_s = e0.elapsed_time(e1) / 1e3 / R
     force_bytes = 1280 * m.n_elem + 48 * m.n_nodes
     force_gbs = force_bytes / force_s / 1e9
-    log(f"spmv {spmv_s * 1e6:.1f} us {spmv_gbs:.0f} GB/s; force {force_s * 1e6:.1f} us {force_gbs:.0f} GB/s")
+    log(f"spmv64 {spmv_s * 1e6:.1f} us {spmv_gbs:.0f} GB/s; spmv32 {spmv32_s * 1e6:.1f} us {spmv32_gbs:.0f} GB/s; "
+        f"force {force_s * 1e6:.1f} us {force_gbs:.0f} GB/s")
 
     # ---- CPU baseline: oracle on a bounded sample -------------------------------------
     cpu = None
@@ -157,6 +191,7 @@ def main():
         O.stiffness_values(sub, sg, sdj, lam, mu, soff, scol)
         t_asm = (time.perf_counter() - ta) * (m.n_elem / frac_el)
         # CG iteration sample on the full matrix (values from the GPU's CSR copy)
+        iters = ls64.iters
         doff_t, dcol_t = body.dof_csr()
         vals = body.stiffness_csr_values(ls.k).cpu().numpy()
         doff_n, dcol_n = doff_t.cpu().numpy(), dcol_t.cpu().numpy()
@@ -166,15 +201,17 @@ def main():
         t_it = (time.perf_counter() - tc) / 10
         cpu_solve_s = t_asm + iters * t_it
         cpu = {"value": 1.0 / cpu_solve_s, "unit": "solves/s", "cores": ncores, "kind": "port",
-               "sample": f"oracle stiffness assembly of {frac_el} of {m.n_elem} elements (1 thread) + 10 CG "
-                         f"iterations on the full {dof_nnz}-nnz CSR ({ncores} OpenMP threads), extrapolated to "
-                         f"{iters} iterations: {t_asm:.1f} s assembly + {t_it * 1e3:.1f} ms/iteration"}
+               "sample": f"oracle FP64 stiffness assembly of {frac_el} of {m.n_elem} elements (1 thread) + 10 CG "
+                         f"iterations on the full {dof_nnz}-nnz CSR ({ncores} OpenMP threads), extrapolated to the "
+                         f"{iters} iterations of the plain FP64 GPU CG: {t_asm:.1f} s assembly + "
+                         f"{t_it * 1e3:.1f} ms/iteration"}
         log(f"cpu baseline: {cpu_solve_s:.1f} s per solve")
 
     # HBM bytes per SpMV launch on this workload from the two rocprofv3 PMC passes
     # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, scales calibrated on a 512 MiB copy; see
     # profiles/pmc_traffic_c2_force_spmv.md); only known for the default 200x20x20 beam
     traffic = (890247 * 2.0 + 31606) * 1024 if (nx, ny, nz) == (200, 20, 20) else None
+    traffic32 = None  # FP32 SpMV: no PMC pass yet
     out = {
         "metric": "TET10 solves/s at 1e-8 rel. residual (learned+FP32 vs plain); force-kernel GB/s",
         "value": args.steps / dt,
@@ -186,15 +223,24 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"plain FP64 linear-static CG solve, cantilever {nx}x{ny}x{nz} cells x 6 TET10 "
-                               f"({m.n_elem} el, {m.n_nodes} nodes, {3 * m.n_nodes} DOF), tol {args.tol:g}",
-                   "n_elem": m.n_elem, "n_dof": 3 * m.n_nodes, "cg_iterations": iters,
-                   "final_rel_residual": ls.res, "K_nnz": dof_nnz},
-        "roofline": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_cg_spmv/k_spmv_block",
-                     "avg_us": spmv_s * 1e6, "bytes_per_launch": spmv_bytes},
+        "config": {"workload": f"linear-static solve to a true FP64 relative residual of {args.tol:g} "
+                               f"(stiffness assembly + Dirichlet + CG with FP32 recursion and FP64 reliable updates "
+                               f"+ position update), cantilever {nx}x{ny}x{nz} cells x 6 TET10 "
+                               f"({m.n_elem} el, {m.n_nodes} nodes, {3 * m.n_nodes} DOF)",
+                   "n_elem": m.n_elem, "n_dof": 3 * m.n_nodes, "cg_iterations": iters, "fp64_refreshes": ls.refreshes,
+                   "final_rel_residual": ls.res, "K_nnz": dof_nnz,
+                   "plain_fp64": {"solves_per_s": args.steps / dt64, "ms_per_solve": dt64 / args.steps * 1e3,
+                                  "cg_iterations": ls64.iters, "final_rel_residual": ls64.res},
+                   "speedup_vs_plain_fp64": dt64 / dt},
+        "roofline": {"bound": "hbm", "achieved": spmv32_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": spmv32_gbs / HBM_PEAK_GBS, "traffic": traffic32,
+                     "kernel": "k_mcg_spmv/k_spmv_block32 (FP32 block SpMV)", "avg_us": spmv32_s * 1e6,
+                     "bytes_per_launch": spmv32_bytes},
+        "spmv_fp64": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_cg_spmv/k_spmv_block",
+                      "avg_us": spmv_s * 1e6, "bytes_per_launch": spmv_bytes},
         "force_kernel": {"bound": "hbm", "achieved": force_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": force_gbs / HBM_PEAK_GBS, "kernel": "k_force_planned<10,5,false> + k_gather_partials",
                          "avg_us": force_s * 1e6, "bytes_per_launch": force_bytes},

@@ -168,6 +168,8 @@ int tlfea_apply_dirichlet(const tlfea_pattern* p, int n_fixed, const int* fixed_
 /* ---- sparse / dense kernels -------------------------------------------------------- */
 /* y = K x on the sliced 3x3 block layout (replaces cusparseSpMV, LinearStaticSolver.cu:398). */
 int tlfea_spmv_block(const tlfea_pattern* p, const double* k_sell, const double* x, double* y, void* stream);
+/* FP32 y = K32 x on the sliced layout (the mixed-precision CG's operator). */
+int tlfea_spmv_block_f32(const tlfea_pattern* p, const float* k32, const float* x, float* y, void* stream);
 /* y = M x on the node matrix applied per component (x, y AoS 3 per node). */
 int tlfea_spmv_node3(const tlfea_pattern* p, const double* m_sell, const double* x, double* y, void* stream);
 /* Plain scalar CSR SpMV (the reference's matrix format), y = A x. */
@@ -196,6 +198,20 @@ int tlfea_cg_solve(tlfea_cg* c, const double* k_sell, const double* f, double* u
 /* c[3k+d] = x_d(fixed[k]) - x_ref,d(fixed[k]) for k < n_fixed. */
 int tlfea_constraint_eval(int n_fixed, const int* fixed_nodes, const double* x, const double* y, const double* z,
                           const double* xref, const double* yref, const double* zref, double* c, void* stream);
+
+/* ---- mixed precision CG (not in the reference) -------------------------------------- */
+/* k32[9*sell_blocks] = (float) k_sell. */
+int tlfea_matrix_to_f32(const tlfea_pattern* p, const double* k_sell, float* k32, void* stream);
+/* CG on K u = f with the Krylov recursion in FP32 (FP32 block SpMV on k32, FP32 r and p)
+ * and FP64 where accuracy is decided: u is accumulated in FP64, dot products and norms are
+ * accumulated in FP64, and the true residual f - K u is recomputed with the FP64 matrix
+ * (reliable update) whenever the recursive residual has dropped by `refresh` (e.g. 1e-2)
+ * since the last refresh or its estimate reaches tol.  Converged when
+ * ||f - K u|| / ||f|| < tol in FP64.  rel_res_out is that FP64 ratio; refreshes_out counts
+ * the FP64 residual evaluations. */
+int tlfea_cg_solve_mixed(tlfea_cg* c, const double* k_sell, const float* k32, const double* f, double* u,
+                         double tol, double refresh, int max_iter, int check_every, int* iters_out,
+                         double* rel_res_out, int* refreshes_out, void* stream);
 
 /* ---- position update (LinearStaticSolver.cu:189-196,449-463) ----------------------- */
 int tlfea_add_displacement(int n_nodes, const double* u, double* x, double* y, double* z, void* stream);

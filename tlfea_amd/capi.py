@@ -101,6 +101,7 @@ _SIGS = {
     "tlfea_apply_dirichlet": (i32, [vp, i32, vp, vp, vp, vp]),
     "tlfea_spmv_block": (i32, [vp, vp, vp, vp, vp]),
     "tlfea_spmv_node3": (i32, [vp, vp, vp, vp, vp]),
+    "tlfea_spmv_block_f32": (i32, [vp, vp, vp, vp, vp]),
     "tlfea_spmv_csr": (i32, [i32, vp, vp, vp, vp, vp, vp]),
     "tlfea_reduce_scratch_size": (i64, [i64]),
     "tlfea_dot": (i32, [i64, vp, vp, vp, vp, vp]),
@@ -111,6 +112,9 @@ _SIGS = {
     "tlfea_cg_destroy": (i32, [vp]),
     "tlfea_cg_solve": (i32, [vp, vp, vp, vp, f64, i32, i32, C.POINTER(i32), C.POINTER(f64), vp]),
     "tlfea_add_displacement": (i32, [i32, vp, vp, vp, vp, vp]),
+    "tlfea_matrix_to_f32": (i32, [vp, vp, vp, vp]),
+    "tlfea_cg_solve_mixed": (i32, [vp, vp, vp, vp, vp, f64, f64, i32, i32, C.POINTER(i32), C.POINTER(f64),
+                                   C.POINTER(i32), vp]),
     "tlfea_constraint_eval": (i32, [i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "tlfea_solver_create": (i32, [i32, C.POINTER(Body), C.POINTER(vp)]),
     "tlfea_solver_destroy": (i32, [vp]),

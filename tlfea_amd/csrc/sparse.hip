@@ -33,6 +33,9 @@ struct tlfea_cg {
     double* d_r = nullptr;
     double* d_p = nullptr;
     double* d_Kp = nullptr;
+    float* d_r32 = nullptr;   // mixed-precision CG: FP32 recursive residual
+    float* d_p32 = nullptr;   // FP32 direction
+    float* d_q32 = nullptr;   // FP32 K p
     double* d_part = nullptr;  // [RED_BLOCKS]
     double* d_scal = nullptr;  // [8]: rr0, rr, alpha, beta, rel, pKp
     int* d_flags = nullptr;    // [4]: 0 done, 1 notpd, 2 iters
@@ -672,6 +675,241 @@ __global__ void k_cg_dir(int64_t n, double* __restrict__ p, const double* __rest
         p[i] = beta * p[i] + r[i];
 }
 
+
+// ------------------------------------------------------------------------------------
+// Mixed-precision CG (FP32 recursion, FP64 accumulation / residual refresh).
+// scal: 0 ||f||^2, 1 rho = r.r, 2 alpha, 3 beta, 4 rel (true), 5 pq, 6 rho at last refresh,
+//       7 rel estimate, 8 rho of the previous iteration;  flags: 0 done, 1 notpd, 2 iters, 3 refresh requested, 4 refreshes
+// ------------------------------------------------------------------------------------
+__global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
+__device__ __forceinline__ void block_row_product32(const float* __restrict__ Kv, const int* __restrict__ cp, int L,
+                                                    const float* __restrict__ xv, float& y0, float& y1, float& y2) {
+    y0 = y1 = y2 = 0.0f;
+    int k = 0;
+    for (; k + 4 <= L; k += 4) {
+        int col[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) col[u] = cp[(int64_t)(k + u) * TILE];
+        float xg[4][3];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            xg[u][0] = xv[3 * col[u]];
+            xg[u][1] = xv[3 * col[u] + 1];
+            xg[u][2] = xv[3 * col[u] + 2];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const float* v = Kv + (int64_t)(k + u) * 9 * TILE;
+            y0 += v[0 * TILE] * xg[u][0] + v[1 * TILE] * xg[u][1] + v[2 * TILE] * xg[u][2];
+            y1 += v[3 * TILE] * xg[u][0] + v[4 * TILE] * xg[u][1] + v[5 * TILE] * xg[u][2];
+            y2 += v[6 * TILE] * xg[u][0] + v[7 * TILE] * xg[u][1] + v[8 * TILE] * xg[u][2];
+        }
+    }
+    for (; k < L; k++) {
+        const int col = cp[(int64_t)k * TILE];
+        const float x0 = xv[3 * col], x1 = xv[3 * col + 1], x2 = xv[3 * col + 2];
+        const float* v = Kv + (int64_t)k * 9 * TILE;
+        y0 += v[0 * TILE] * x0 + v[1 * TILE] * x1 + v[2 * TILE] * x2;
+        y1 += v[3 * TILE] * x0 + v[4 * TILE] * x1 + v[5 * TILE] * x2;
+        y2 += v[6 * TILE] * x0 + v[7 * TILE] * x1 + v[8 * TILE] * x2;
+    }
+}
+
+// u = 0, r32 = p32 = f, partials of f.f
+__global__ __launch_bounds__(RED_THREADS) void k_mcg_init(int64_t n, const double* __restrict__ f,
+                                                          double* __restrict__ u, float* __restrict__ r,
+                                                          float* __restrict__ p, double* __restrict__ part) {
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double fi = f[i];
+        u[i] = 0.0;
+        r[i] = (float)fi;
+        p[i] = (float)fi;
+        s += fi * fi;
+    }
+    s = block_sum(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_mcg_init_fin(int nparts, const double* __restrict__ part,
+                                                              double* __restrict__ scal, int* __restrict__ flags) {
+    const double ff = sum_parts(part, nparts);
+    if (threadIdx.x == 0) {
+        scal[0] = ff;
+        scal[1] = ff;
+        scal[6] = ff;
+        scal[4] = ff == 0.0 ? 0.0 : 1.0;
+        scal[7] = scal[4];
+        for (int k = 0; k < 5; k++) flags[k] = 0;
+        if (ff == 0.0) flags[0] = 1;
+    }
+}
+
+// q = K32 p; FP64 partials of p.q
+__global__ __launch_bounds__(256) void k_mcg_spmv(int64_t n_slices, const int* __restrict__ sell_row,
+                                                  const int64_t* __restrict__ slice_base,
+                                                  const int* __restrict__ slice_len, const int* __restrict__ sell_col,
+                                                  const float* __restrict__ K, const float* __restrict__ p,
+                                                  float* __restrict__ q, double* __restrict__ part,
+                                                  const int* __restrict__ flags) {
+    if (flags[0]) return;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0.0;
+    if (t < n_slices * TILE) {
+        const int row = sell_row[t];
+        const int64_t sl = t >> 6;
+        const int lane = (int)(t & (TILE - 1));
+        const int64_t base = slice_base[sl];
+        float y0, y1, y2;
+        block_row_product32(K + 9 * base + lane, sell_col + base + lane, slice_len[sl], p, y0, y1, y2);
+        if (row >= 0) {
+            q[3 * row] = y0;
+            q[3 * row + 1] = y1;
+            q[3 * row + 2] = y2;
+            s = (double)p[3 * row] * y0 + (double)p[3 * row + 1] * y1 + (double)p[3 * row + 2] * y2;
+        }
+    }
+    s = block_sum(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_mcg_alpha(int nparts, const double* __restrict__ part,
+                                                           double* __restrict__ scal, int* __restrict__ flags) {
+    if (flags[0]) return;
+    const double pq = sum_parts(part, nparts);
+    if (threadIdx.x == 0) {
+        scal[5] = pq;
+        if (pq <= 0.0) {
+            flags[1] = 1;
+            flags[0] = 1;
+        } else {
+            scal[2] = scal[1] / pq;
+        }
+    }
+}
+
+// u += alpha p (FP64), r -= alpha q (FP32), FP64 partials of r.r
+__global__ __launch_bounds__(RED_THREADS) void k_mcg_update(int64_t n, double* __restrict__ u,
+                                                            float* __restrict__ r, const float* __restrict__ p,
+                                                            const float* __restrict__ q, double* __restrict__ part,
+                                                            const double* __restrict__ scal,
+                                                            const int* __restrict__ flags) {
+    if (flags[0]) return;
+    const double alpha = scal[2];
+    const float a32 = (float)alpha;
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        u[i] += alpha * (double)p[i];
+        const float ri = r[i] - a32 * q[i];
+        r[i] = ri;
+        s += (double)ri * ri;
+    }
+    s = block_sum(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// rho' = r.r; request a refresh when rho' < refresh^2 * rho_at_last_refresh or the estimate
+// reaches tol; otherwise beta = rho'/rho.
+__global__ __launch_bounds__(RED_THREADS) void k_mcg_beta(int nparts, const double* __restrict__ part,
+                                                          double* __restrict__ scal, int* __restrict__ flags,
+                                                          double tol, double refresh) {
+    if (flags[0]) return;
+    const double rn = sum_parts(part, nparts);
+    if (threadIdx.x == 0) {
+        flags[2] += 1;
+        const double est = scal[0] > 0.0 ? sqrt(rn / scal[0]) : 0.0;
+        scal[7] = est;
+        flags[3] = (rn < refresh * refresh * scal[6] || est < tol) ? 1 : 0;
+        scal[8] = scal[1];  // rho before this iteration (for a refreshed beta)
+        scal[3] = rn / scal[1];
+        scal[1] = rn;
+    }
+}
+
+// reliable update: r64 = f - K u (FP64 matrix), r32 = (float) r64, partials of r64.r64
+__global__ __launch_bounds__(256) void k_mcg_true_res(int64_t n_slices, const int* __restrict__ sell_row,
+                                                      const int64_t* __restrict__ slice_base,
+                                                      const int* __restrict__ slice_len,
+                                                      const int* __restrict__ sell_col, const double* __restrict__ K,
+                                                      const double* __restrict__ u, const double* __restrict__ f,
+                                                      float* __restrict__ r32, double* __restrict__ part,
+                                                      const int* __restrict__ flags) {
+    if (flags[0] || !flags[3]) return;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0.0;
+    if (t < n_slices * TILE) {
+        const int row = sell_row[t];
+        const int64_t sl = t >> 6;
+        const int lane = (int)(t & (TILE - 1));
+        const int64_t base = slice_base[sl];
+        double y0, y1, y2;
+        block_row_product(K + 9 * base + lane, sell_col + base + lane, slice_len[sl], u, y0, y1, y2);
+        if (row >= 0) {
+            const double r0 = f[3 * row] - y0, r1 = f[3 * row + 1] - y1, r2 = f[3 * row + 2] - y2;
+            r32[3 * row] = (float)r0;
+            r32[3 * row + 1] = (float)r1;
+            r32[3 * row + 2] = (float)r2;
+            s = r0 * r0 + r1 * r1 + r2 * r2;
+        }
+    }
+    s = block_sum(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// after a refresh: true relative residual, convergence, and beta from the refreshed r
+__global__ __launch_bounds__(RED_THREADS) void k_mcg_refresh_fin(int nparts, const double* __restrict__ part,
+                                                                 double* __restrict__ scal, int* __restrict__ flags,
+                                                                 double tol) {
+    if (flags[0] || !flags[3]) return;
+    const double rt = sum_parts(part, nparts);
+    if (threadIdx.x == 0) {
+        const double rel = scal[0] > 0.0 ? sqrt(rt / scal[0]) : 0.0;
+        scal[4] = rel;
+        flags[4] += 1;
+        flags[3] = 0;
+        if (rel < tol) {
+            flags[0] = 1;
+        } else {
+            // beta was rho'/rho with the recursive rho'; use the true residual instead
+            scal[3] = rt / scal[8];
+            scal[1] = rt;
+            scal[6] = rt;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_spmv_block32(int64_t n_slices, const int* __restrict__ sell_row,
+                                                      const int64_t* __restrict__ slice_base,
+                                                      const int* __restrict__ slice_len,
+                                                      const int* __restrict__ sell_col, const float* __restrict__ K,
+                                                      const float* __restrict__ xv, float* __restrict__ yv) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_slices * TILE) return;
+    const int row = sell_row[t];
+    const int64_t sl = t >> 6;
+    const int lane = (int)(t & (TILE - 1));
+    const int64_t base = slice_base[sl];
+    float y0, y1, y2;
+    block_row_product32(K + 9 * base + lane, sell_col + base + lane, slice_len[sl], xv, y0, y1, y2);
+    if (row >= 0) {
+        yv[3 * row] = y0;
+        yv[3 * row + 1] = y1;
+        yv[3 * row + 2] = y2;
+    }
+}
+
+__global__ void k_mcg_dir(int64_t n, float* __restrict__ p, const float* __restrict__ r,
+                          const double* __restrict__ scal, const int* __restrict__ flags) {
+    if (flags[0]) return;
+    const float beta = (float)scal[3];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = beta * p[i] + r[i];
+}
+
 }  // namespace tlfea_hip
 
 using namespace tlfea_hip;
@@ -1050,11 +1288,12 @@ extern "C" int tlfea_cg_create(const tlfea_pattern* p, tlfea_cg** out) {
     int rc = TLFEA_OK;
     if ((rc = dmalloc(&c->d_r, c->n)) || (rc = dmalloc(&c->d_p, c->n)) || (rc = dmalloc(&c->d_Kp, c->n)) ||
         (rc = dmalloc(&c->d_part, RED_BLOCKS + (int)(p->n_slices * TILE / 256) + 1)) ||
-        (rc = dmalloc(&c->d_scal, 8)) || (rc = dmalloc(&c->d_flags, 4))) {
+        (rc = dmalloc(&c->d_scal, 16)) || (rc = dmalloc(&c->d_flags, 8)) || (rc = dmalloc(&c->d_r32, c->n)) ||
+        (rc = dmalloc(&c->d_p32, c->n)) || (rc = dmalloc(&c->d_q32, c->n))) {
         tlfea_cg_destroy(c);
         return rc;
     }
-    TLFEA_HIP_TRY(hipMemset(c->d_flags, 0, sizeof(int) * 4));
+    TLFEA_HIP_TRY(hipMemset(c->d_flags, 0, sizeof(int) * 8));
     TLFEA_HIP_TRY(hipMemset(c->d_p, 0, sizeof(double) * c->n));
     *out = c;
     return TLFEA_OK;
@@ -1068,6 +1307,9 @@ extern "C" int tlfea_cg_destroy(tlfea_cg* c) {
     hipFree(c->d_part);
     hipFree(c->d_scal);
     hipFree(c->d_flags);
+    hipFree(c->d_r32);
+    hipFree(c->d_p32);
+    hipFree(c->d_q32);
     delete c;
     return TLFEA_OK;
 }
@@ -1112,5 +1354,68 @@ extern "C" int tlfea_cg_solve(tlfea_cg* c, const double* k_sell, const double* f
         return TLFEA_ENOTPD;
     }
     if (!flags[0] && max_iter > 0) return TLFEA_ENOCONV;
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_matrix_to_f32(const tlfea_pattern* p, const double* k_sell, float* k32, void* stream) {
+    TLFEA_REQUIRE(p && k_sell && k32, "null argument");
+    const int64_t n = 9 * p->sell_blocks;
+    k_to_f32<<<grid_for(n, 256), 256, 0, as_stream(stream)>>>(n, k_sell, k32);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_cg_solve_mixed(tlfea_cg* c, const double* k_sell, const float* k32, const double* f, double* u,
+                                    double tol, double refresh, int max_iter, int check_every, int* iters_out,
+                                    double* rel_res_out, int* refreshes_out, void* stream) {
+    TLFEA_REQUIRE(c && k_sell && k32 && f && u, "null argument");
+    TLFEA_REQUIRE(max_iter >= 0 && refresh > 0.0 && refresh < 1.0, "bad iteration limit or refresh factor");
+    if (check_every <= 0) check_every = 16;
+    hipStream_t s = as_stream(stream);
+    const tlfea_pattern* p = c->p;
+    const int64_t n = c->n;
+    const unsigned gs = grid_for(p->n_slices * TILE, 256);
+    const unsigned gv = RED_BLOCKS;
+    k_mcg_init<<<gv, RED_THREADS, 0, s>>>(n, f, u, c->d_r32, c->d_p32, c->d_part);
+    k_mcg_init_fin<<<1, RED_THREADS, 0, s>>>(gv, c->d_part, c->d_scal, c->d_flags);
+    TLFEA_LAUNCH_CHECK();
+    int flags[8] = {0};
+    double scal[8] = {0};
+    for (int it = 0; it < max_iter; it++) {
+        k_mcg_spmv<<<gs, 256, 0, s>>>(p->n_slices, p->d_sell_row, p->d_slice_base, p->d_slice_len, p->d_sell_col, k32,
+                                      c->d_p32, c->d_q32, c->d_part, c->d_flags);
+        k_mcg_alpha<<<1, RED_THREADS, 0, s>>>(gs, c->d_part, c->d_scal, c->d_flags);
+        k_mcg_update<<<gv, RED_THREADS, 0, s>>>(n, u, c->d_r32, c->d_p32, c->d_q32, c->d_part, c->d_scal, c->d_flags);
+        k_mcg_beta<<<1, RED_THREADS, 0, s>>>(gv, c->d_part, c->d_scal, c->d_flags, tol, refresh);
+        k_mcg_true_res<<<gs, 256, 0, s>>>(p->n_slices, p->d_sell_row, p->d_slice_base, p->d_slice_len, p->d_sell_col,
+                                          k_sell, u, f, c->d_r32, c->d_part, c->d_flags);
+        k_mcg_refresh_fin<<<1, RED_THREADS, 0, s>>>(gs, c->d_part, c->d_scal, c->d_flags, tol);
+        k_mcg_dir<<<gv, 256, 0, s>>>(n, c->d_p32, c->d_r32, c->d_scal, c->d_flags);
+        TLFEA_LAUNCH_CHECK();
+        if ((it + 1) % check_every == 0 || it + 1 == max_iter) {
+            TLFEA_HIP_TRY(hipMemcpyAsync(flags, c->d_flags, sizeof(int) * 8, hipMemcpyDeviceToHost, s));
+            TLFEA_HIP_TRY(hipStreamSynchronize(s));
+            if (flags[0]) break;
+        }
+    }
+    TLFEA_HIP_TRY(hipMemcpyAsync(flags, c->d_flags, sizeof(int) * 8, hipMemcpyDeviceToHost, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(scal, c->d_scal, sizeof(double) * 8, hipMemcpyDeviceToHost, s));
+    TLFEA_HIP_TRY(hipStreamSynchronize(s));
+    if (iters_out) *iters_out = flags[2];
+    if (rel_res_out) *rel_res_out = scal[4];
+    if (refreshes_out) *refreshes_out = flags[4];
+    if (flags[1]) {
+        set_error("mixed CG: non-positive p.Kp (%g) detected; aborting", scal[5]);
+        return TLFEA_ENOTPD;
+    }
+    if (!flags[0] && max_iter > 0) return TLFEA_ENOCONV;
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_spmv_block_f32(const tlfea_pattern* p, const float* k32, const float* x, float* y, void* stream) {
+    TLFEA_REQUIRE(p && k32 && x && y, "null argument");
+    k_spmv_block32<<<grid_for(p->n_slices * TILE, 256), 256, 0, as_stream(stream)>>>(
+        p->n_slices, p->d_sell_row, p->d_slice_base, p->d_slice_len, p->d_sell_col, k32, x, y);
+    TLFEA_LAUNCH_CHECK();
     return TLFEA_OK;
 }

@@ -171,10 +171,10 @@ class Body:
         call("tlfea_mass_to_csr", self.pattern, ptr(m_sell), ptr(v), self.s)
         return v
 
-    def stiffness_sell(self, x=None, h=1.0):
+    def stiffness_sell(self, x=None, h=1.0, out=None):
         x = self.x if x is None else x
         _, _, nb = self.pattern_sizes()
-        k = torch.empty(9 * nb, dtype=torch.float64, device=DEV)
+        k = torch.empty(9 * nb, dtype=torch.float64, device=DEV) if out is None else out
         call("tlfea_stiffness_assemble", self.pattern, self.nn, self.n_elem, ptr(self.conn), ptr(x[0]), ptr(x[1]),
              ptr(x[2]), ptr(self.grad_t), ptr(self.detJ_t), C.byref(self.quad), C.byref(self.mat), float(h), ptr(k),
              self.s)
@@ -219,9 +219,14 @@ class LinearStatic:
     """K(x) u = f_ext with Dirichlet elimination and device-scalar CG, then x += u
     (`src/solvers/LinearStaticSolver.cu:469-481`)."""
 
-    def __init__(self, body: Body, tol=1e-10, max_iter=10000, check_every=32):
+    def __init__(self, body: Body, tol=1e-10, max_iter=10000, check_every=32, mixed=False, refresh=1e-2):
+        """mixed=True: FP32 Krylov recursion with FP64 accumulation and reliable updates
+        (tlfea_cg_solve_mixed); tol is then the true FP64 relative residual."""
         self.b = body
         self.tol, self.max_iter, self.check_every = tol, max_iter, check_every
+        self.mixed, self.refresh = mixed, refresh
+        self.k32 = None
+        self.refreshes = 0
         h = C.c_void_p()
         call("tlfea_cg_create", body.pattern, C.byref(h))
         self.cg = h
@@ -232,16 +237,28 @@ class LinearStatic:
         self.res = 0.0
 
     def assemble(self):
-        self.k = self.b.stiffness_sell()
+        self.k = self.b.stiffness_sell(out=self.k)
         self.f = self.b.f_ext.clone()
         self.b.apply_dirichlet(self.k, self.f)
+        if self.mixed:
+            if self.k32 is None:
+                self.k32 = torch.empty(self.k.numel(), dtype=torch.float32, device=DEV)
+            call("tlfea_matrix_to_f32", self.b.pattern, ptr(self.k), ptr(self.k32), self.b.s)
 
     def cg_solve(self):
         it = C.c_int()
         res = C.c_double()
-        rc = capi.lib().tlfea_cg_solve(self.cg, ptr(self.k), ptr(self.f), ptr(self.u), float(self.tol),
-                                       int(self.max_iter), int(self.check_every), C.byref(it), C.byref(res),
-                                       self.b.s)
+        if self.mixed:
+            nref = C.c_int()
+            rc = capi.lib().tlfea_cg_solve_mixed(self.cg, ptr(self.k), ptr(self.k32), ptr(self.f), ptr(self.u),
+                                                 float(self.tol), float(self.refresh), int(self.max_iter),
+                                                 int(self.check_every), C.byref(it), C.byref(res), C.byref(nref),
+                                                 self.b.s)
+            self.refreshes = nref.value
+        else:
+            rc = capi.lib().tlfea_cg_solve(self.cg, ptr(self.k), ptr(self.f), ptr(self.u), float(self.tol),
+                                           int(self.max_iter), int(self.check_every), C.byref(it), C.byref(res),
+                                           self.b.s)
         self.iters, self.res = it.value, res.value
         if rc not in (capi.TLFEA_OK, capi.TLFEA_ENOCONV, capi.TLFEA_ENOTPD):
             capi.check(rc)
