@@ -5,24 +5,28 @@ Workload (single large mesh): cantilever 1.0 x 0.1 x 0.1 m, 200x20x20 hex cells 
 480,000 TET10 elements, 674,081 nodes, 2,022,243 DOF; E = 2.1e11 Pa, nu = 0.3,
 rho = 7850; nodes at x = 0 fixed; 1e5 N in -z spread over the x = L nodes.
 
-One step = one linear-static solve to a relative residual of 1e-8: tangent stiffness
-assembly at the current positions, Dirichlet elimination, CG, position update (the
-reference's LinearStaticSolver::Solve).  `value` = solves/s of the FP32+FP64 path (CG with
-the Krylov recursion in FP32 and FP64 reliable updates, converged on the true FP64
-residual ||f - K u||/||f|| < 1e-8); the plain FP64 path (the drop-in, with the reference's
-sqrt(rr/rr0) test) is timed on the same problem and reported beside it.  Inputs are
-resident in HBM when the timed region starts.
+One step = one linear-static solve to a relative residual of 1e-8 (the reference's
+LinearStaticSolver::Solve: tangent K(x) at the current positions, Dirichlet elimination,
+CG with the stopping rule sqrt(rr/rr0) < tol, position update).  `value` = solves/s of the
+matrix-free FP64 path: CG whose operator K(x) p is applied element by element from the
+shape-function gradients (nothing assembled); the plain FP64 path (the drop-in: block
+assembly + Dirichlet + CG on the assembled matrix) is timed on the same problem and
+reported beside it.  Inputs are resident in HBM when the timed region starts.
 
 Kernel rooflines are measured live with HIP events on the launch stream (the null stream,
 which torch.cuda.Event records on):
-  * roofline      -- the step's dominant kernel, the FP32 block SpMV of the mixed CG,
-                     bytes/launch = 40*node_nnz + 28*n_nodes;
-  * spmv_fp64     -- the FP64 block SpMV (plain path), bytes/launch = 76*node_nnz + 52*n_nodes;
+  * roofline      -- the step's dominant kernel pair, the matrix-free tangent
+                     (k_tangent_planned + k_gather_tangent), bytes/launch =
+                     440*n_elem + 96*n_nodes (inverse Jacobians 360 B, detJ 40 B and
+                     connectivity 40 B per element; x, X0 and p read and y written, 96 B
+                     per node);
+  * spmv_fp64     -- the FP64 block SpMV of the plain path, bytes/launch =
+                     76*node_nnz + 52*n_nodes;
   * force_kernel  -- the fused FP64 P + f_int (block-local plan: force + node gather),
-                     bytes/launch = 1280*n_elem + 48*n_nodes (the reference byte model).
+                     bytes/launch = 1280*n_elem + 48*n_nodes.
 cpu_baseline: the FP64 oracle on a bounded sample (a slice of the element assembly and
-10 CG iterations on the full matrix), extrapolated to one solve with the GPU's iteration
-count.
+10 CG iterations on the full matrix), extrapolated to one solve with the plain GPU CG's
+iteration count.
 """
 from __future__ import annotations
 
@@ -36,6 +40,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes of one matrix-free tangent launch pair on the 200x20x20 beam (rocprofv3 PMC)
+PMC_TANGENT_BYTES = None
 
 
 def log(*a):
@@ -52,7 +58,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cells", type=str, default="200,20,20")
     ap.add_argument("--pmc-probe", action="store_true",
-                    help="one launch each of the planned force kernel, the block SpMV and a 512 MiB copy "
+                    help="one launch each of the planned force kernel, the block SpMV, the matrix-free "
+                         "tangent and a 512 MiB copy "
                          "(the copy calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE), then exit")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -78,12 +85,7 @@ def main():
         torch.cuda.synchronize()
         b.internal_force_planned(f=f)
         b.spmv(k, p, y)
-        from tlfea_amd.capi import call as _call, ptr as _ptr
-        k32 = torch.empty(k.numel(), dtype=torch.float32, device="cuda")
-        _call("tlfea_matrix_to_f32", b.pattern, _ptr(k), _ptr(k32), None)
-        p32, y32 = p.float(), torch.empty(3 * m.n_nodes, dtype=torch.float32, device="cuda")
-        torch.cuda.synchronize()
-        _call("tlfea_spmv_block_f32", b.pattern, _ptr(k32), _ptr(p32), _ptr(y32), None)
+        b.tangent_apply(p, out=y)
         src = torch.randn(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
@@ -113,65 +115,53 @@ def main():
         torch.cuda.synchronize()
         return time.perf_counter() - t
 
-    # plain FP64 path (the drop-in): reference CG rule sqrt(rr/rr0) < tol
+    # plain FP64 path (the drop-in): assembled K, reference CG rule sqrt(rr/rr0) < tol
     ls64 = LinearStatic(body, tol=args.tol, max_iter=200000, check_every=64)
     dt64 = timed_solves(ls64)
     log(f"plain FP64 solve: {dt64 / args.steps * 1e3:.1f} ms, {ls64.iters} CG iterations, rel res {ls64.res:.3e}")
-    # FP32 Krylov recursion + FP64 reliable updates: true FP64 residual < tol
-    ls = LinearStatic(body, tol=args.tol, max_iter=200000, check_every=64, mixed=True)
+    # matrix-free FP64 path: same CG, K(x) p applied per element
+    ls = LinearStatic(body, tol=args.tol, max_iter=200000, check_every=64, matfree=True)
     dt = timed_solves(ls)
     iters = ls.iters
     ms_per_step = dt / args.steps * 1e3
-    log(f"mixed solve: {ms_per_step:.1f} ms, {iters} CG iterations, {ls.refreshes} FP64 refreshes, "
-        f"true rel res {ls.res:.3e}")
-    ls64.close()
+    log(f"matrix-free FP64 solve: {ms_per_step:.1f} ms, {iters} CG iterations, rel res {ls.res:.3e}")
+    # the two solutions agree (same Krylov sequence, different summation order)
+    u_diff = (torch.linalg.norm(ls.u - ls64.u) / torch.linalg.norm(ls64.u)).item()
+    log(f"matrix-free vs assembled displacement rel diff {u_diff:.3e}")
 
     # ---- kernel rooflines (HIP events on the null stream) ---------------------------
     R = args.kernel_reps
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def event_time(fn):
+        for _ in range(3):
+            fn()
+        e0.record()
+        for _ in range(R):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / R
+
+    for d in range(3):
+        body.x[d].copy_(X0[d])
     p = torch.randn(3 * m.n_nodes, dtype=torch.float64, device="cuda")
     y = torch.empty_like(p)
-    for _ in range(3):
-        body.spmv(ls.k, p, y)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(R):
-        body.spmv(ls.k, p, y)
-    e1.record()
-    torch.cuda.synchronize()
-    spmv_s = e0.elapsed_time(e1) / 1e3 / R
-    # algorithmic bytes: 3x3 block values (72 B) + column index (4 B) per stored block, x read
+    # matrix-free tangent: inverse Jacobians (360 B) + detJ (40 B) + connectivity (40 B) per
+    # element, x, X0 and p read and y written once (96 B) per node
+    tan_s = event_time(lambda: body.tangent_apply(p, out=y))
+    tan_bytes = 440 * m.n_elem + 96 * m.n_nodes
+    tan_gbs = tan_bytes / tan_s / 1e9
+    # FP64 block SpMV: 3x3 block values (72 B) + column index (4 B) per stored block, x read
     # and y written once (48 B per node), row id per node (4 B)
+    spmv_s = event_time(lambda: body.spmv(ls64.k, p, y))
     spmv_bytes = 76 * node_nnz + 52 * m.n_nodes
     spmv_gbs = spmv_bytes / spmv_s / 1e9
-    # FP32 block SpMV of the mixed CG (the step's dominant kernel): 36 B values + 4 B column
-    # per block, x read + y write 24 B per node, row id 4 B
-    p32 = p.float()
-    y32 = torch.empty_like(p32)
-    from tlfea_amd.capi import call as _call, ptr as _ptr
-    for _ in range(3):
-        _call("tlfea_spmv_block_f32", body.pattern, _ptr(ls.k32), _ptr(p32), _ptr(y32), None)
-    e0.record()
-    for _ in range(R):
-        _call("tlfea_spmv_block_f32", body.pattern, _ptr(ls.k32), _ptr(p32), _ptr(y32), None)
-    e1.record()
-    torch.cuda.synchronize()
-    spmv32_s = e0.elapsed_time(e1) / 1e3 / R
-    spmv32_bytes = 40 * node_nnz + 28 * m.n_nodes
-    spmv32_gbs = spmv32_bytes / spmv32_s / 1e9
-
     f = torch.zeros(3 * m.n_nodes, dtype=torch.float64, device="cuda")
-    body.plan  # build the block-local force plan (setup)
-    for _ in range(3):
-        body.internal_force_planned(f=f)
-    e0.record()
-    for _ in range(R):
-        body.internal_force_planned(f=f)
-    e1.record()
-    torch.cuda.synchronize()
-    force_s = e0.elapsed_time(e1) / 1e3 / R
+    force_s = event_time(lambda: body.internal_force_planned(f=f))
     force_bytes = 1280 * m.n_elem + 48 * m.n_nodes
     force_gbs = force_bytes / force_s / 1e9
-    log(f"spmv64 {spmv_s * 1e6:.1f} us {spmv_gbs:.0f} GB/s; spmv32 {spmv32_s * 1e6:.1f} us {spmv32_gbs:.0f} GB/s; "
+    log(f"tangent {tan_s * 1e6:.1f} us {tan_gbs:.0f} GB/s; spmv64 {spmv_s * 1e6:.1f} us {spmv_gbs:.0f} GB/s; "
         f"force {force_s * 1e6:.1f} us {force_gbs:.0f} GB/s")
 
     # ---- CPU baseline: oracle on a bounded sample -------------------------------------
@@ -191,27 +181,28 @@ def main():
         O.stiffness_values(sub, sg, sdj, lam, mu, soff, scol)
         t_asm = (time.perf_counter() - ta) * (m.n_elem / frac_el)
         # CG iteration sample on the full matrix (values from the GPU's CSR copy)
-        iters = ls64.iters
+        iters64 = ls64.iters
         doff_t, dcol_t = body.dof_csr()
-        vals = body.stiffness_csr_values(ls.k).cpu().numpy()
+        vals = body.stiffness_csr_values(ls64.k).cpu().numpy()
         doff_n, dcol_n = doff_t.cpu().numpy(), dcol_t.cpu().numpy()
         b = np.random.default_rng(0).standard_normal(3 * m.n_nodes)
         tc = time.perf_counter()
         O.cg(doff_n, dcol_n, vals, b, tol=0.0, max_iter=10)
         t_it = (time.perf_counter() - tc) / 10
-        cpu_solve_s = t_asm + iters * t_it
+        cpu_solve_s = t_asm + iters64 * t_it
         cpu = {"value": 1.0 / cpu_solve_s, "unit": "solves/s", "cores": ncores, "kind": "port",
                "sample": f"oracle FP64 stiffness assembly of {frac_el} of {m.n_elem} elements (1 thread) + 10 CG "
                          f"iterations on the full {dof_nnz}-nnz CSR ({ncores} OpenMP threads), extrapolated to the "
-                         f"{iters} iterations of the plain FP64 GPU CG: {t_asm:.1f} s assembly + "
+                         f"{iters64} iterations of the plain FP64 GPU CG: {t_asm:.1f} s assembly + "
                          f"{t_it * 1e3:.1f} ms/iteration"}
         log(f"cpu baseline: {cpu_solve_s:.1f} s per solve")
 
-    # HBM bytes per SpMV launch on this workload from the two rocprofv3 PMC passes
-    # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, scales calibrated on a 512 MiB copy; see
-    # profiles/pmc_traffic_c2_force_spmv.md); only known for the default 200x20x20 beam
-    traffic = (890247 * 2.0 + 31606) * 1024 if (nx, ny, nz) == (200, 20, 20) else None
-    traffic32 = None  # FP32 SpMV: no PMC pass yet
+    # HBM bytes per launch on this workload from two rocprofv3 PMC passes (FETCH_SIZE x 2.0
+    # + WRITE_SIZE x 1.0, scales calibrated on a 512 MiB copy); known for the default
+    # 200x20x20 beam only
+    default = (nx, ny, nz) == (200, 20, 20)
+    traffic = (890247 * 2.0 + 31606) * 1024 if default else None
+    traffic_tan = PMC_TANGENT_BYTES if default else None
     out = {
         "metric": "TET10 solves/s at 1e-8 rel. residual (learned+FP32 vs plain); force-kernel GB/s",
         "value": args.steps / dt,
@@ -223,21 +214,20 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64",
+        "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"linear-static solve to a true FP64 relative residual of {args.tol:g} "
-                               f"(stiffness assembly + Dirichlet + CG with FP32 recursion and FP64 reliable updates "
-                               f"+ position update), cantilever {nx}x{ny}x{nz} cells x 6 TET10 "
-                               f"({m.n_elem} el, {m.n_nodes} nodes, {3 * m.n_nodes} DOF)",
-                   "n_elem": m.n_elem, "n_dof": 3 * m.n_nodes, "cg_iterations": iters, "fp64_refreshes": ls.refreshes,
-                   "final_rel_residual": ls.res, "K_nnz": dof_nnz,
+        "config": {"workload": f"linear-static solve to a relative residual of {args.tol:g} (matrix-free FP64 CG "
+                               f"on K(x) + position update), cantilever {nx}x{ny}x{nz} cells x 6 TET10 "
+                               f"({m.n_elem} el, {m.n_nodes} nodes, {3 * m.n_nodes} DOF), 1e5 N tip load",
+                   "n_elem": m.n_elem, "n_dof": 3 * m.n_nodes, "cg_iterations": iters,
+                   "final_rel_residual": ls.res, "K_nnz": dof_nnz, "u_rel_diff_vs_assembled": u_diff,
                    "plain_fp64": {"solves_per_s": args.steps / dt64, "ms_per_solve": dt64 / args.steps * 1e3,
                                   "cg_iterations": ls64.iters, "final_rel_residual": ls64.res},
                    "speedup_vs_plain_fp64": dt64 / dt},
-        "roofline": {"bound": "hbm", "achieved": spmv32_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": spmv32_gbs / HBM_PEAK_GBS, "traffic": traffic32,
-                     "kernel": "k_mcg_spmv/k_spmv_block32 (FP32 block SpMV)", "avg_us": spmv32_s * 1e6,
-                     "bytes_per_launch": spmv32_bytes},
+        "roofline": {"bound": "hbm", "achieved": tan_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": tan_gbs / HBM_PEAK_GBS, "traffic": traffic_tan,
+                     "kernel": "k_tangent_planned<10,5> + k_gather_tangent (matrix-free K(x) p)",
+                     "avg_us": tan_s * 1e6, "bytes_per_launch": tan_bytes},
         "spmv_fp64": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                       "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_cg_spmv/k_spmv_block",
                       "avg_us": spmv_s * 1e6, "bytes_per_launch": spmv_bytes},

@@ -61,6 +61,12 @@ int tlfea_quadrature(int nn, tlfea_quad* out);
  * 277-285) and dn_du_pre_feat4_kernel (src/elements/FEAT4Data.cu:98-190).  nn = 10 or 4. */
 int tlfea_precompute_grad(int nn, int n_elem, const int* conn, const double* x, const double* y, const double* z,
                           const tlfea_quad* quad, double* grad_t, double* detJ_t, void* stream);
+/* Inverse-transposed Jacobians M = J^-T per (element, q) from the same J and pivoting, so
+ * that g_a = M dN_a/dxi (a pivot below 1e-14 gives M = 0).  Layout
+ * jinv_t[((tile*nq + q)*9 + 3i + k)*64 + lane] = M[i][k], padded_elems(n_elem)*nq*9
+ * doubles.  Not in the reference: the compact input of the matrix-free tangent. */
+int tlfea_precompute_jinv(int nn, int n_elem, const int* conn, const double* x, const double* y, const double* z,
+                          const tlfea_quad* quad, double* jinv_t, void* stream);
 
 /* ---- stress and internal force ---------------------------------------------------- */
 /* Replaces calc_p_kernel / compute_p (src/elements/FEAT10Data.cu:287-305,
@@ -104,6 +110,22 @@ int tlfea_force_plan_sizes(const tlfea_force_plan* p, int* block_elems, int* n_b
 int tlfea_internal_force_planned(const tlfea_force_plan* plan, const double* x, const double* y, const double* z,
                                  const double* grad_t, const double* detJ_t, const tlfea_quad* quad,
                                  const tlfea_material* mat, const double* v, double* f, int accumulate, void* stream);
+
+/* Matrix-free tangent action y = K(x) p, the operator that compute_hessian_assemble_csr +
+ * SolveLinearSystemCG's SpMV apply (src/elements/FEAT10DataFunc.cuh:512-768,
+ * src/materials/SVK.cuh:41-67, src/solvers/LinearStaticSolver.cu:360-371) without
+ * assembling it: per quadrature point F = I + sum_a u_a (x) g_a (u = x - X0),
+ * dF = sum_a p_a (x) g_a, dE = (dF^T F + F^T dF)/2, dP = dF S + F (lambda tr(dE) I + 2 mu dE),
+ * y_a = sum_q dP g_a dV, with g_a rebuilt from jinv_t (tlfea_precompute_jinv).  Equal to
+ * the assembled K(x) p up to rounding; reads ~420 B per TET10 element (M, detJ, slots)
+ * instead of ~3 kB of assembled 3x3 blocks.  With a DOF mask (uint8 [3*n_nodes],
+ * 1 = fixed) the masked rows return y = p (the Dirichlet identity rows of
+ * tlfea_apply_dirichlet) and the masked columns are ignored (p read as 0); NULL = no
+ * mask.  Kelvin-Voigt damping is not part of this operator (mat damping must be 0). */
+int tlfea_tangent_apply_planned(const tlfea_force_plan* plan, const double* x, const double* y, const double* z,
+                                const double* X0, const double* Y0, const double* Z0, const double* jinv_t,
+                                const double* detJ_t, const tlfea_quad* quad, const tlfea_material* mat,
+                                const unsigned char* fixed_dof_mask, const double* p, double* out, void* stream);
 
 /* FP32 variant of the fused force kernel, displacement form F = I + sum_a u_a (x) g_a:
  * u = x - X is passed as three float arrays, grad32_t / dV32_t are float copies of grad_t
@@ -198,6 +220,19 @@ int tlfea_cg_solve(tlfea_cg* c, const double* k_sell, const double* f, double* u
 /* c[3k+d] = x_d(fixed[k]) - x_ref,d(fixed[k]) for k < n_fixed. */
 int tlfea_constraint_eval(int n_fixed, const int* fixed_nodes, const double* x, const double* y, const double* z,
                           const double* xref, const double* yref, const double* zref, double* c, void* stream);
+
+/* ---- matrix-free CG (not in the reference) ----------------------------------------- */
+/* The reference's CG (same sqrt(rr/rr0) < tol rule, TLFEA_ENOTPD on p.Kp <= 0) with the
+ * operator K(x) applied by tlfea_tangent_apply_planned instead of an assembled matrix:
+ * the linear-static system of LinearStaticSolver::Solve after Dirichlet elimination. */
+int tlfea_cg_solve_matfree(tlfea_cg* c, const tlfea_force_plan* plan, const double* x, const double* y,
+                           const double* z, const double* X0, const double* Y0, const double* Z0,
+                           const double* jinv_t, const double* detJ_t, const tlfea_quad* quad,
+                           const tlfea_material* mat, const unsigned char* fixed_dof_mask, const double* f, double* u,
+                           double tol, int max_iter, int check_every, int* iters_out, double* rel_res_out,
+                           void* stream);
+/* DOF mask (uint8 [3*n_nodes], 1 = DOF of a fixed node) from a fixed-node list. */
+int tlfea_fixed_dof_mask(int n_nodes, int n_fixed, const int* fixed_nodes, unsigned char* mask, void* stream);
 
 /* ---- mixed precision CG (not in the reference) -------------------------------------- */
 /* k32[9*sell_blocks] = (float) k_sell. */

@@ -1,0 +1,205 @@
+"""GPU parity of the matrix-free tangent action y = K(x) p and of the CG built on it.
+
+Bars: K(x) p 1e-12 relative against the oracle's assembled tangent times p (a different
+summation order only); symmetry p.Kq = q.Kp to 1e-12; the central difference of the
+internal force (an independent route through the force kernel) to 1e-7; matrix-free CG
+displacements against the banded Cholesky solve with the same bars as the assembled CG.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import E_STEEL, NU_STEEL, RHO_STEEL, deformed, lame, load_mesh, rel_err
+
+pytestmark = pytest.mark.gpu
+
+MESHES = ["cube", "beam_vtu", "C1", "kuhn_t4"]
+
+
+def _body(m, E=E_STEEL, nu=NU_STEEL, eta=0.0):
+    from tlfea_amd.fea import Body
+    return Body(m, E, nu, RHO_STEEL, eta, 0.0)
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _set_positions(b, xyz):
+    import torch
+    for d in range(3):
+        b.x[d].copy_(torch.as_tensor(xyz[d]))
+
+
+def _dn_table(nn, quad):
+    """dN/dxi per (q, node, k) from the shape functions (FEAT10Data.cu:92-198)."""
+    dL = np.array([[-1.0, -1, -1], [1, 0, 0], [0, 1, 0], [0, 0, 1]])
+    out = []
+    for q in range(quad.n):
+        if nn == 4:
+            out.append(dL.copy())
+            continue
+        L = np.array([1 - quad.x[q] - quad.y[q] - quad.z[q], quad.x[q], quad.y[q], quad.z[q]])
+        rows = [(4 * L[a] - 1) * dL[a] for a in range(4)]
+        for i, j in ((0, 1), (1, 2), (0, 2), (0, 3), (1, 3), (2, 3)):
+            rows.append(4 * (L[i] * dL[j] + L[j] * dL[i]))
+        out.append(np.array(rows))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("name", ["cube", "kuhn_t4", "beam_highres_t10"])
+def test_jinv_rebuilds_gradients(gpu, name):
+    """g_a = J^-T dN_a/dxi from the stored inverse Jacobians equals the stored gradients."""
+    m = load_mesh(name)
+    b = _body(m)
+    nq = b.nq
+    pe = b.grad_t.numel() // (nq * b.nn * 3)
+    M = _np(b.jinv_t).reshape(pe // 64, nq, 9, 64).transpose(0, 3, 1, 2).reshape(pe, nq, 3, 3)[: m.n_elem]
+    dn = _dn_table(b.nn, b.quad)                                  # [q, a, k]
+    g = np.einsum("eqik,qak->eqai", M, dn)                        # [e, q, a, i]
+    gr, _ = b.grad_ref()
+    gr = _np(gr).reshape(m.n_elem, nq, 3, b.nn).transpose(0, 1, 3, 2)  # ref layout a + nn*d
+    assert rel_err(g, gr) < 1e-14
+
+
+@pytest.mark.parametrize("name", MESHES)
+@pytest.mark.parametrize("masked", [False, True])
+def test_tangent_matches_oracle_matrix(gpu, name, masked):
+    import torch
+    print(f"[tangent] {name} masked={masked}", flush=True)
+    m = load_mesh(name)
+    b = _body(m)
+    xd = deformed(m, seed=3)
+    _set_positions(b, xd)
+    p = np.random.default_rng(11).standard_normal(3 * m.n_nodes)
+    y = _np(b.tangent_apply(torch.as_tensor(p).cuda(), masked=masked))
+    go, djo = O.grad(m)
+    lam, mu = lame(E_STEEL, NU_STEEL)
+    doff, dcol = O.dof_pattern(m)
+    vo = O.stiffness_values(m, go, djo, lam, mu, doff, dcol, *xd)
+    if masked:
+        vo, _ = O.apply_dirichlet(doff, dcol, vo, m.f_ext, m.fixed)
+    ref = O.spmv(doff, dcol, vo, p)
+    assert rel_err(y, ref) < 1e-12
+    if masked and len(m.fixed):
+        fd = (3 * np.asarray(m.fixed)[:, None] + np.arange(3)).reshape(-1)
+        assert np.array_equal(y[fd], p[fd])
+
+
+@pytest.mark.parametrize("name", ["beam_vtu", "kuhn_t4"])
+def test_tangent_symmetric(gpu, name):
+    import torch
+    m = load_mesh(name)
+    b = _body(m)
+    _set_positions(b, deformed(m, seed=4))
+    rng = np.random.default_rng(12)
+    p = torch.as_tensor(rng.standard_normal(3 * m.n_nodes)).cuda()
+    q = torch.as_tensor(rng.standard_normal(3 * m.n_nodes)).cuda()
+    kp = b.tangent_apply(p, masked=False)
+    kq = b.tangent_apply(q, masked=False)
+    a, c = torch.dot(q, kp).item(), torch.dot(p, kq).item()
+    scale = (torch.linalg.norm(q) * torch.linalg.norm(kp)).item()
+    assert abs(a - c) < 1e-12 * scale
+
+
+@pytest.mark.parametrize("name", ["cube", "kuhn_t4"])
+def test_tangent_is_force_derivative(gpu, name):
+    """K(x) p against (f_int(x + h p) - f_int(x - h p)) / 2h from the force kernel."""
+    import torch
+    m = load_mesh(name)
+    b = _body(m)
+    xd = deformed(m, seed=5, amp=2e-2)
+    _set_positions(b, xd)
+    p = np.random.default_rng(13).standard_normal(3 * m.n_nodes)
+    kp = _np(b.tangent_apply(torch.as_tensor(p).cuda(), masked=False))
+    L = np.ptp(m.xyz(), axis=0).max()
+    h = 1e-6 * L
+    fs = []
+    for sgn in (1.0, -1.0):
+        xs = [torch.as_tensor(xd[d] + sgn * h * p[d::3]).cuda() for d in range(3)]
+        fs.append(_np(b.internal_force(x=xs)))
+    fd = (fs[0] - fs[1]) / (2 * h)
+    assert rel_err(kp, fd) < 1e-7
+
+
+def test_tangent_rejects_damping(gpu):
+    import ctypes as C
+
+    import torch
+    from tlfea_amd import capi
+    from tlfea_amd.capi import ptr
+    m = load_mesh("cube")
+    b = _body(m, eta=1e3)
+    p = torch.zeros(3 * m.n_nodes, dtype=torch.float64, device="cuda")
+    out = torch.empty_like(p)
+    X0 = b.X0
+    rc = capi.lib().tlfea_tangent_apply_planned(b.plan, ptr(b.x[0]), ptr(b.x[1]), ptr(b.x[2]), ptr(X0[0]),
+                                                ptr(X0[1]), ptr(X0[2]), ptr(b.jinv_t), ptr(b.detJ_t),
+                                                C.byref(b.quad), C.byref(b.mat), None, ptr(p), ptr(out), None)
+    assert rc == capi.TLFEA_EINVAL
+
+
+CG_DISP_BAR = {"cube": 1e-10, "beam_vtu": 1e-10, "C1": 1e-10, "beam_highres_t4": 20 * 3.2e7 * 2.2e-16}
+# relative iteration-count bar at tol 1e-8 (measured spread 0-8% on the larger meshes)
+ITER_BAR = {"cube": 0.03, "beam_vtu": 0.12, "C1": 0.03, "beam_highres_t4": 0.12}
+
+
+@pytest.mark.parametrize("name", ["cube", "beam_vtu", "C1", "beam_highres_t4"])
+def test_matfree_cg_against_direct(gpu, name):
+    from tlfea_amd.fea import LinearStatic
+    print(f"[matfree cg] {name}", flush=True)
+    m = load_mesh(name)
+    b = _body(m)
+    ls = LinearStatic(b, tol=1e-13, max_iter=200000, matfree=True)
+    ls.assemble()
+    rc = ls.cg_solve()
+    assert rc == 0, (ls.iters, ls.res)
+    go, djo = O.grad(m)
+    lam, mu = lame(E_STEEL, NU_STEEL)
+    doff, dcol = O.dof_pattern(m)
+    vo = O.stiffness_values(m, go, djo, lam, mu, doff, dcol)
+    vo, fo = O.apply_dirichlet(doff, dcol, vo, m.f_ext, m.fixed)
+    ud = O.band_cholesky_solve(doff, dcol, vo, fo)
+    assert rel_err(_np(ls.u), ud) < CG_DISP_BAR[name]
+    # the same Krylov sequence as the oracle's CG: the iterates after 10 steps agree to
+    # rounding (a 1e-15 relative perturbation of K moves them by <= 3e-14 at 10 steps on
+    # these meshes; by 30 steps finite-precision CG has amplified it to 1e-3 on the cube
+    # and 5e-5 on beam_highres_t4, measured with the oracle), and at a moderate tolerance the iteration counts agree to
+    # within ITER_BAR (finite-precision CG counts move with the summation order, the more
+    # so the larger kappa)
+    lsk = LinearStatic(b, tol=0.0, max_iter=10, matfree=True)
+    lsk.assemble()
+    lsk.cg_solve()
+    uk, _, _ = O.cg(doff, dcol, vo, fo, tol=0.0, max_iter=10)
+    assert lsk.iters == 10 and rel_err(_np(lsk.u), uk) < 1e-11
+    ls8 = LinearStatic(b, tol=1e-8, max_iter=200000, matfree=True)
+    ls8.assemble()
+    assert ls8.cg_solve() == 0
+    uo, ito, _ = O.cg(doff, dcol, vo, fo, tol=1e-8, max_iter=200000)
+    assert abs(ls8.iters - ito) <= max(3, ITER_BAR[name] * ito), (ls8.iters, ito)
+    assert rel_err(_np(ls8.u), uo) < 1e-6
+
+
+def test_matfree_solve_moves_positions(gpu):
+    """LinearStaticSolver's beam scenario (tol 1e-10) through the matrix-free operator: the
+    tip deflection sits in the Euler-Bernoulli band (beam_linear_static.cc:223-234)."""
+    from tlfea_amd.fea import LinearStatic
+    m = load_mesh("C1")
+    b = _body(m)
+    ls = LinearStatic(b, tol=1e-10, max_iter=50000, matfree=True)
+    assert ls.solve() == 0
+    z = _np(b.x[2])
+    tip = m.meta["tip_nodes"]
+    dz = (z - m.z)[tip].min()
+    eb = 1000.0 / (3 * E_STEEL * 0.1 ** 4 / 12)
+    assert 0.95 * eb < abs(dz) < 1.05 * eb
+
+
+def test_matfree_cg_not_positive_definite(gpu):
+    from tlfea_amd import capi
+    from tlfea_amd.fea import LinearStatic
+    m = load_mesh("cube")
+    b = _body(m, E=-1e9)
+    ls = LinearStatic(b, tol=1e-10, max_iter=100, matfree=True)
+    ls.assemble()
+    assert ls.cg_solve() == capi.TLFEA_ENOTPD

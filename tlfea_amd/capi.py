@@ -83,6 +83,10 @@ _SIGS = {
     "tlfea_force_plan_sizes": (i32, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i64), C.POINTER(i32)]),
     "tlfea_internal_force_planned": (i32, [vp, vp, vp, vp, vp, vp, C.POINTER(Quad), C.POINTER(Material), vp, vp, i32,
                                            vp]),
+    "tlfea_tangent_apply_planned": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(Quad), C.POINTER(Material),
+                                          vp, vp, vp, vp]),
+    "tlfea_precompute_jinv": (i32, [i32, i32, vp, vp, vp, vp, C.POINTER(Quad), vp, vp]),
+    "tlfea_fixed_dof_mask": (i32, [i32, i32, vp, vp, vp]),
     "tlfea_grad_to_f32": (i32, [i32, i32, vp, vp, C.POINTER(Quad), vp, vp, vp]),
     "tlfea_internal_force_f32": (i32, [i32, i32, i32, vp, vp, vp, vp, vp, vp, C.POINTER(Material), vp, i32, vp]),
     "tlfea_grad_to_ref": (i32, [i32, i32, vp, vp, vp, vp, vp]),
@@ -111,6 +115,8 @@ _SIGS = {
     "tlfea_cg_create": (i32, [vp, C.POINTER(vp)]),
     "tlfea_cg_destroy": (i32, [vp]),
     "tlfea_cg_solve": (i32, [vp, vp, vp, vp, f64, i32, i32, C.POINTER(i32), C.POINTER(f64), vp]),
+    "tlfea_cg_solve_matfree": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(Quad), C.POINTER(Material),
+                                     vp, vp, vp, f64, i32, i32, C.POINTER(i32), C.POINTER(f64), vp]),
     "tlfea_add_displacement": (i32, [i32, vp, vp, vp, vp, vp]),
     "tlfea_matrix_to_f32": (i32, [vp, vp, vp, vp]),
     "tlfea_cg_solve_mixed": (i32, [vp, vp, vp, vp, vp, f64, f64, i32, i32, C.POINTER(i32), C.POINTER(f64),

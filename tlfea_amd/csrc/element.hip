@@ -93,7 +93,8 @@ template <int NN, int NQ>
 __global__ __launch_bounds__(256) void k_precompute(int n_elem, int n_tiles, const int* __restrict__ conn,
                                                     const double* __restrict__ x, const double* __restrict__ y,
                                                     const double* __restrict__ z, tlfea_quad quad,
-                                                    double* __restrict__ grad_t, double* __restrict__ detJ_t) {
+                                                    double* __restrict__ grad_t, double* __restrict__ detJ_t,
+                                                    double* __restrict__ jinv_t) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = (int)(gid & (TILE - 1));
     const int64_t tq = gid >> 6;  // tile*NQ + q
@@ -101,11 +102,16 @@ __global__ __launch_bounds__(256) void k_precompute(int n_elem, int n_tiles, con
     const int q = (int)(tq % NQ);
     const int tile = (int)(tq / NQ);
     const int e = tile * TILE + lane;
-    double* G = grad_t + tq * (NN * 3 * TILE) + lane;
+    double* G = grad_t ? grad_t + tq * (NN * 3 * TILE) + lane : nullptr;
+    double* Mi = jinv_t ? jinv_t + tq * (9 * TILE) + lane : nullptr;
     if (e >= n_elem) {
+        if (G)
 #pragma unroll
-        for (int c = 0; c < NN * 3; c++) G[c * TILE] = 0.0;
-        detJ_t[tq * TILE + lane] = 0.0;
+            for (int c = 0; c < NN * 3; c++) G[c * TILE] = 0.0;
+        if (Mi)
+#pragma unroll
+            for (int c = 0; c < 9; c++) Mi[c * TILE] = 0.0;
+        if (detJ_t) detJ_t[tq * TILE + lane] = 0.0;
         return;
     }
     double X[NN][3];
@@ -135,7 +141,7 @@ __global__ __launch_bounds__(256) void k_precompute(int n_elem, int n_tiles, con
         for (int i = 0; i < 3; i++)
 #pragma unroll
             for (int j = 0; j < 3; j++) J[i][j] += X[a][i] * dN[a][j];
-    detJ_t[tq * TILE + lane] = det3(J);
+    if (detJ_t) detJ_t[tq * TILE + lane] = det3(J);
     double JT[3][3];
 #pragma unroll
     for (int i = 0; i < 3; i++)
@@ -143,13 +149,25 @@ __global__ __launch_bounds__(256) void k_precompute(int n_elem, int n_tiles, con
         for (int j = 0; j < 3; j++) JT[i][j] = J[j][i];
     if constexpr (NN == 10) {
         // J^T g_a = dN_a/dxi by partial pivoting (FEAT10Data.cu:180-188)
+        if (G)
 #pragma unroll
-        for (int a = 0; a < NN; a++) {
-            double g[3];
-            solve3_pivot(JT, dN[a], g);
+            for (int a = 0; a < NN; a++) {
+                double g[3];
+                solve3_pivot(JT, dN[a], g);
 #pragma unroll
-            for (int d = 0; d < 3; d++) G[(a + NN * d) * TILE] = g[d];
-        }
+                for (int d = 0; d < 3; d++) G[(a + NN * d) * TILE] = g[d];
+            }
+        // M = J^-T column by column with the same pivoting (g_a = M dN_a/dxi; a pivot below
+        // 1e-14 gives M = 0, hence g = 0, as above)
+        if (Mi)
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const double ek[3] = {k == 0 ? 1.0 : 0.0, k == 1 ? 1.0 : 0.0, k == 2 ? 1.0 : 0.0};
+                double col[3];
+                solve3_pivot(JT, ek, col);
+#pragma unroll
+                for (int i = 0; i < 3; i++) Mi[(3 * i + k) * TILE] = col[i];
+            }
     } else {
         // explicit cofactor inverse of J^T, 1/det or 0 (FEAT4Data.cu:153-178)
         const double dt = det3(JT);
@@ -164,11 +182,17 @@ __global__ __launch_bounds__(256) void k_precompute(int n_elem, int n_tiles, con
         Ti[2][0] = (JT[1][0] * JT[2][1] - JT[1][1] * JT[2][0]) * id;
         Ti[2][1] = -(JT[0][0] * JT[2][1] - JT[0][1] * JT[2][0]) * id;
         Ti[2][2] = (JT[0][0] * JT[1][1] - JT[0][1] * JT[1][0]) * id;
+        if (G)
 #pragma unroll
-        for (int a = 0; a < NN; a++)
+            for (int a = 0; a < NN; a++)
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+                    G[(a + NN * i) * TILE] = Ti[i][0] * dN[a][0] + Ti[i][1] * dN[a][1] + Ti[i][2] * dN[a][2];
+        if (Mi)
 #pragma unroll
             for (int i = 0; i < 3; i++)
-                G[(a + NN * i) * TILE] = Ti[i][0] * dN[a][0] + Ti[i][1] * dN[a][1] + Ti[i][2] * dN[a][2];
+#pragma unroll
+                for (int k = 0; k < 3; k++) Mi[(3 * i + k) * TILE] = Ti[i][k];
     }
 }
 
@@ -538,10 +562,27 @@ extern "C" int tlfea_precompute_grad(int nn, int n_elem, const int* conn, const 
     const int64_t threads = (int64_t)n_tiles * nq * TILE;
     if (nn == 10)
         k_precompute<10, 5><<<grid_for(threads, 256), 256, 0, as_stream(stream)>>>(n_elem, n_tiles, conn, x, y, z,
-                                                                                   *quad, grad_t, detJ_t);
+                                                                                   *quad, grad_t, detJ_t, nullptr);
     else
         k_precompute<4, 1><<<grid_for(threads, 256), 256, 0, as_stream(stream)>>>(n_elem, n_tiles, conn, x, y, z,
-                                                                                  *quad, grad_t, detJ_t);
+                                                                                  *quad, grad_t, detJ_t, nullptr);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_precompute_jinv(int nn, int n_elem, const int* conn, const double* x, const double* y,
+                                     const double* z, const tlfea_quad* quad, double* jinv_t, void* stream) {
+    if (int rc = check_elem(nn, n_elem, quad)) return rc;
+    if (n_elem == 0) return TLFEA_OK;
+    TLFEA_REQUIRE(conn && x && y && z && jinv_t, "null pointer argument");
+    const int n_tiles = (int)(padded(n_elem) / TILE);
+    const int64_t threads = (int64_t)n_tiles * quad->n * TILE;
+    if (nn == 10)
+        k_precompute<10, 5><<<grid_for(threads, 256), 256, 0, as_stream(stream)>>>(n_elem, n_tiles, conn, x, y, z,
+                                                                                   *quad, nullptr, nullptr, jinv_t);
+    else
+        k_precompute<4, 1><<<grid_for(threads, 256), 256, 0, as_stream(stream)>>>(n_elem, n_tiles, conn, x, y, z,
+                                                                                  *quad, nullptr, nullptr, jinv_t);
     TLFEA_LAUNCH_CHECK();
     return TLFEA_OK;
 }

@@ -43,6 +43,21 @@ int launch_force_planned(const tlfea_force_plan* p, const double* x, const doubl
                          const tlfea_material& mat, const double* v, double* f, int accumulate, const int* flags,
                          hipStream_t s);
 
+// dN/dxi of every node at every quadrature point (a kernel argument).
+struct DnTable {
+    double d[5][10][3];
+};
+DnTable make_dn_table(int nn, const tlfea_quad& quad);
+
+// Matrix-free tangent action (force_plan.hip) at positions x (rest X0, inverse
+// Jacobians jinv_t); dot_part (optional) receives per-block partials of p . out (blocks of
+// 256 nodes).
+int launch_tangent_planned(const tlfea_force_plan* pl, const double* x, const double* y, const double* z,
+                           const double* X0, const double* Y0, const double* Z0, const double* jinv_t,
+                           const double* detJ_t, const tlfea_quad& quad, const tlfea_material& mat,
+                           const unsigned char* mask, const double* p, double* out, double* dot_part,
+                           const int* flags, hipStream_t s);
+
 // Fused FP64 force launch (element.hip).  With `flags` non-null the kernel returns at once
 // when flags[0] | flags[1] is set (used inside the iterative solvers' queued iterations).
 int launch_force(int nn, int n_elem, const int* conn, const double* x, const double* y, const double* z,

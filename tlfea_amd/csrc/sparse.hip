@@ -1357,6 +1357,74 @@ extern "C" int tlfea_cg_solve(tlfea_cg* c, const double* k_sell, const double* f
     return TLFEA_OK;
 }
 
+// r = p = f with the masked (fixed) DOFs zeroed, u = 0, partial sums of r.r
+__global__ __launch_bounds__(RED_THREADS) void k_cg_init_masked(int64_t n, const double* __restrict__ f,
+                                                                const unsigned char* __restrict__ mask,
+                                                                double* __restrict__ u, double* __restrict__ r,
+                                                                double* __restrict__ p, double* __restrict__ part) {
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double fi = (mask && mask[i]) ? 0.0 : f[i];
+        u[i] = 0.0;
+        r[i] = fi;
+        p[i] = fi;
+        s += fi * fi;
+    }
+    s = block_sum(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+extern "C" int tlfea_cg_solve_matfree(tlfea_cg* c, const tlfea_force_plan* plan, const double* x, const double* y,
+                                      const double* z, const double* X0, const double* Y0, const double* Z0,
+                                      const double* jinv_t, const double* detJ_t, const tlfea_quad* quad,
+                                      const tlfea_material* mat, const unsigned char* fixed_dof_mask, const double* f,
+                                      double* u, double tol, int max_iter, int check_every, int* iters_out,
+                                      double* rel_res_out, void* stream) {
+    TLFEA_REQUIRE(c && plan && x && y && z && X0 && Y0 && Z0 && jinv_t && detJ_t && quad && mat && f && u,
+                  "null argument");
+    TLFEA_REQUIRE(quad->n == (plan->nn == 10 ? 5 : 1), "quadrature size does not match the element type");
+    TLFEA_REQUIRE(max_iter >= 0, "negative iteration limit");
+    TLFEA_REQUIRE(plan->n_nodes == c->p->n_nodes, "plan and CG workspace sizes differ");
+    TLFEA_REQUIRE(mat->eta_damp == 0.0 && mat->lambda_damp == 0.0, "the matrix-free tangent excludes damping");
+    if (check_every <= 0) check_every = 16;
+    hipStream_t s = as_stream(stream);
+    const int64_t n = c->n;
+    const unsigned gv = RED_BLOCKS;
+    // the gather's dot partials (one per 256 nodes) fit d_part: it holds n_slices*64/256+1 more
+    const int gn = (int)grid_for(plan->n_nodes, 256);
+    k_cg_init_masked<<<gv, RED_THREADS, 0, s>>>(n, f, fixed_dof_mask, u, c->d_r, c->d_p, c->d_part);
+    k_cg_init_fin<<<1, RED_THREADS, 0, s>>>(gv, c->d_part, c->d_scal, c->d_flags);
+    TLFEA_LAUNCH_CHECK();
+    int flags[4] = {0, 0, 0, 0};
+    double scal[8] = {0};
+    for (int it = 0; it < max_iter; it++) {
+        int rc = launch_tangent_planned(plan, x, y, z, X0, Y0, Z0, jinv_t, detJ_t, *quad, *mat, fixed_dof_mask,
+                                        c->d_p, c->d_Kp, c->d_part, c->d_flags, s);
+        if (rc != TLFEA_OK) return rc;
+        k_cg_alpha<<<1, RED_THREADS, 0, s>>>(gn, c->d_part, c->d_scal, c->d_flags);
+        k_cg_update<<<gv, RED_THREADS, 0, s>>>(n, u, c->d_r, c->d_p, c->d_Kp, c->d_part, c->d_scal, c->d_flags);
+        k_cg_beta<<<1, RED_THREADS, 0, s>>>(gv, c->d_part, c->d_scal, c->d_flags, tol);
+        k_cg_dir<<<gv, 256, 0, s>>>(n, c->d_p, c->d_r, c->d_scal, c->d_flags);
+        TLFEA_LAUNCH_CHECK();
+        if ((it + 1) % check_every == 0 || it + 1 == max_iter) {
+            TLFEA_HIP_TRY(hipMemcpyAsync(flags, c->d_flags, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
+            TLFEA_HIP_TRY(hipStreamSynchronize(s));
+            if (flags[0]) break;
+        }
+    }
+    TLFEA_HIP_TRY(hipMemcpyAsync(flags, c->d_flags, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(scal, c->d_scal, sizeof(double) * 8, hipMemcpyDeviceToHost, s));
+    TLFEA_HIP_TRY(hipStreamSynchronize(s));
+    if (iters_out) *iters_out = flags[2];
+    if (rel_res_out) *rel_res_out = scal[4];
+    if (flags[1]) {
+        set_error("CG: non-positive p.Kp (%g) detected; aborting", scal[5]);
+        return TLFEA_ENOTPD;
+    }
+    if (!flags[0] && max_iter > 0) return TLFEA_ENOCONV;
+    return TLFEA_OK;
+}
+
 extern "C" int tlfea_matrix_to_f32(const tlfea_pattern* p, const double* k_sell, float* k32, void* stream) {
     TLFEA_REQUIRE(p && k_sell && k32, "null argument");
     const int64_t n = 9 * p->sell_blocks;

@@ -49,6 +49,8 @@ class Body:
         self.fixed = _dev(mesh.fixed, torch.int32)
         self._pattern = None
         self._plan = None
+        self._dof_mask = None
+        self._jinv = None
         self.precompute()
 
     def set_material(self, E, nu, rho, eta=0.0, lambda_d=0.0):
@@ -124,6 +126,43 @@ class Body:
         call("tlfea_internal_force_planned", self.plan, ptr(x[0]), ptr(x[1]), ptr(x[2]), ptr(self.grad_t),
              ptr(self.detJ_t), C.byref(self.quad), C.byref(self.mat), ptr(v), ptr(f), int(bool(accumulate)), self.s)
         return f
+
+    def elastic_material(self):
+        """The material without Kelvin-Voigt damping (the tangent of the elastic force)."""
+        return capi.Material(self.mat.lam, self.mat.mu, 0.0, 0.0)
+
+    def fixed_dof_mask(self):
+        """uint8 [3*n_nodes], 1 on the DOFs of fixed nodes."""
+        if self._dof_mask is None:
+            m = torch.empty(3 * self.n_nodes, dtype=torch.uint8, device=DEV)
+            nf = int(self.fixed.numel())
+            call("tlfea_fixed_dof_mask", self.n_nodes, nf, ptr(self.fixed) if nf else None, ptr(m), self.s)
+            self._dof_mask = m
+        return self._dof_mask
+
+    @property
+    def jinv_t(self):
+        """Inverse-transposed Jacobians per (element, q), the matrix-free tangent's input."""
+        if self._jinv is None:
+            pe = capi.padded_elems(self.n_elem)
+            self._jinv = torch.zeros(pe * self.nq * 9, dtype=torch.float64, device=DEV)
+            call("tlfea_precompute_jinv", self.nn, self.n_elem, ptr(self.conn), ptr(self.X0[0]), ptr(self.X0[1]),
+                 ptr(self.X0[2]), C.byref(self.quad), ptr(self._jinv), self.s)
+        return self._jinv
+
+    def tangent_apply(self, p, x=None, out=None, masked=True):
+        """y = K(x) p without assembly; with masked=True fixed rows return p and fixed
+        columns are dropped (the Dirichlet-eliminated operator)."""
+        if self.plan is None:
+            raise RuntimeError("matrix-free tangent needs a force plan (mesh too scattered for the LDS budget)")
+        x = self.x if x is None else x
+        out = torch.empty(3 * self.n_nodes, dtype=torch.float64, device=DEV) if out is None else out
+        mat = self.elastic_material()
+        X0 = self.X0
+        call("tlfea_tangent_apply_planned", self.plan, ptr(x[0]), ptr(x[1]), ptr(x[2]), ptr(X0[0]), ptr(X0[1]),
+             ptr(X0[2]), ptr(self.jinv_t), ptr(self.detJ_t), C.byref(self.quad), C.byref(mat),
+             ptr(self.fixed_dof_mask()) if masked else None, ptr(p), ptr(out), self.s)
+        return out
 
     # ---- sparse ------------------------------------------------------------------
     @property
@@ -219,12 +258,17 @@ class LinearStatic:
     """K(x) u = f_ext with Dirichlet elimination and device-scalar CG, then x += u
     (`src/solvers/LinearStaticSolver.cu:469-481`)."""
 
-    def __init__(self, body: Body, tol=1e-10, max_iter=10000, check_every=32, mixed=False, refresh=1e-2):
+    def __init__(self, body: Body, tol=1e-10, max_iter=10000, check_every=32, mixed=False, refresh=1e-2,
+                 matfree=False):
         """mixed=True: FP32 Krylov recursion with FP64 accumulation and reliable updates
-        (tlfea_cg_solve_mixed); tol is then the true FP64 relative residual."""
+        (tlfea_cg_solve_mixed); tol is then the true FP64 relative residual.
+        matfree=True: FP64 CG with K(x) p applied element by element
+        (tlfea_cg_solve_matfree); nothing is assembled."""
+        if mixed and matfree:
+            raise ValueError("mixed and matfree are exclusive")
         self.b = body
         self.tol, self.max_iter, self.check_every = tol, max_iter, check_every
-        self.mixed, self.refresh = mixed, refresh
+        self.mixed, self.refresh, self.matfree = mixed, refresh, matfree
         self.k32 = None
         self.refreshes = 0
         h = C.c_void_p()
@@ -237,6 +281,13 @@ class LinearStatic:
         self.res = 0.0
 
     def assemble(self):
+        if self.matfree:
+            if self.b.plan is None:
+                raise RuntimeError("matrix-free CG needs a force plan")
+            self.f = self.b.f_ext  # fixed DOFs are zeroed inside the solve
+            self.b.fixed_dof_mask()
+            self.b.jinv_t
+            return
         self.k = self.b.stiffness_sell(out=self.k)
         self.f = self.b.f_ext.clone()
         self.b.apply_dirichlet(self.k, self.f)
@@ -255,6 +306,16 @@ class LinearStatic:
                                                  int(self.check_every), C.byref(it), C.byref(res), C.byref(nref),
                                                  self.b.s)
             self.refreshes = nref.value
+        elif self.matfree:
+            b = self.b
+            mat = b.elastic_material()
+            X0 = b.X0
+            rc = capi.lib().tlfea_cg_solve_matfree(self.cg, b.plan, ptr(b.x[0]), ptr(b.x[1]), ptr(b.x[2]),
+                                                   ptr(X0[0]), ptr(X0[1]), ptr(X0[2]), ptr(b.jinv_t),
+                                                   ptr(b.detJ_t), C.byref(b.quad), C.byref(mat),
+                                                   ptr(b.fixed_dof_mask()), ptr(self.f), ptr(self.u),
+                                                   float(self.tol), int(self.max_iter), int(self.check_every),
+                                                   C.byref(it), C.byref(res), b.s)
         else:
             rc = capi.lib().tlfea_cg_solve(self.cg, ptr(self.k), ptr(self.f), ptr(self.u), float(self.tol),
                                            int(self.max_iter), int(self.check_every), C.byref(it), C.byref(res),
