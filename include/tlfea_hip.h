@@ -234,6 +234,48 @@ int tlfea_cg_solve_matfree(tlfea_cg* c, const tlfea_force_plan* plan, const doub
 /* DOF mask (uint8 [3*n_nodes], 1 = DOF of a fixed node) from a fixed-node list. */
 int tlfea_fixed_dof_mask(int n_nodes, int n_fixed, const int* fixed_nodes, unsigned char* mask, void* stream);
 
+/* ---- batches of independent meshes (not in the reference) ---------------------------- */
+/* Instances are concatenated: instance i owns elements elem_off[i]..elem_off[i+1]-1 and
+ * nodes node_off[i]..node_off[i+1]-1 (host arrays of n_inst+1, starting at 0); conn
+ * (device, column-major [nn][n_elem]) holds concatenated node ids; x/y/z are the rest
+ * positions; lambda/mu (host, per instance) the Lame constants; fixed_dof_mask (device,
+ * uint8 [3*n_nodes]) marks fixed DOFs.  Inputs are copied; the handle owns its memory. */
+typedef struct tlfea_batch tlfea_batch;
+int tlfea_batch_create(int nn, int n_inst, const int* elem_off, const int* node_off, const int* conn,
+                       const double* x, const double* y, const double* z, const double* lambda, const double* mu,
+                       const unsigned char* fixed_dof_mask, tlfea_batch** out, void* stream);
+int tlfea_batch_destroy(tlfea_batch* b);
+int tlfea_batch_sizes(const tlfea_batch* b, int* n_inst, int* n_elem, int* n_nodes, int* n_chunks);
+/* Nodes per reduction chunk (>= 64): the granularity of the per-instance dot products. */
+int tlfea_batch_set_chunk(tlfea_batch* b, int chunk_nodes);
+/* Finished instances are dropped from the launches once the unfinished ones hold less
+ * than `ratio` of the launched elements (default 0.6; 0 disables). */
+int tlfea_batch_set_compaction(tlfea_batch* b, double ratio);
+/* out = K p for every instance at its rest configuration, Dirichlet rows/columns masked
+ * (the operator of tlfea_tangent_apply_planned per instance). */
+int tlfea_batch_tangent_apply(tlfea_batch* b, const double* p, double* out, void* stream);
+/* FP32 Jacobi-Chebyshev preconditioner for tlfea_batch_cg_solve(precond = 1): the
+ * diagonal D of K (FP64, matrix-free), lambda_max(D^-1 K) per instance by `power_iters`
+ * FP32 power iterations (scaled by `safety` >= 1), and `degree` Chebyshev steps on
+ * [lambda_max / ratio, lambda_max] applied with the FP32 operator.  degree = 0 removes it.
+ * lambda_max (host [n_inst], may be NULL) receives the unscaled estimates. */
+int tlfea_batch_precond_setup(tlfea_batch* b, int degree, double ratio, int power_iters, double safety,
+                              double* lambda_max, void* stream);
+/* Linear-static CG for every instance at once: K_i u_i = f_i (fixed DOFs eliminated),
+ * the reference's CG per instance with its own alpha/beta, stopping when
+ * ||f_i - K_i u_i|| / ||f_i|| < tol on the free DOFs.  use_guess != 0 starts from u
+ * (else from 0).  Finished instances stop updating; the host polls every `poll`
+ * iterations, which are replayed from a captured hipGraph when use_graph != 0.  With
+ * precond = 1 the CG is the flexible (Polak-Ribiere) variant preconditioned by the FP32
+ * Chebyshev stage of tlfea_batch_precond_setup; u, r, alpha, beta and the stopping test
+ * stay FP64.
+ * Per-instance results go to the host arrays iters / rel_res / status (TLFEA_OK,
+ * TLFEA_ENOCONV, TLFEA_ENOTPD); total_iters = iterations run for the batch.  Returns
+ * TLFEA_ENOCONV if any instance failed. */
+int tlfea_batch_cg_solve(tlfea_batch* b, const double* f, double* u, int use_guess, double tol, int max_iter,
+                         int poll, int use_graph, int precond, int* iters, double* rel_res, int* status,
+                         int* total_iters, void* stream);
+
 /* ---- mixed precision CG (not in the reference) -------------------------------------- */
 /* k32[9*sell_blocks] = (float) k_sell. */
 int tlfea_matrix_to_f32(const tlfea_pattern* p, const double* k_sell, float* k32, void* stream);

@@ -1,0 +1,202 @@
+"""GPU parity of the batched linear-static path against single-mesh solves and the oracle.
+
+Bars: the batched operator equals each instance's own matrix-free tangent to 1e-13 and the
+oracle's assembled K p to 1e-12.  CG stops on its recursive residual (the reference's
+rule); the true residual ||f - K u|| / ||f|| of the result can only be trusted down to the
+FP64 floor eps * ||K|| ||u|| / ||f||, which on these slender beams (kappa 1e6-8e8) is
+1e-10..2e-8 -- even the oracle's direct solve sits there.  So the true residual is checked
+against tol + 20 * eps * ||K||_1 ||u|| / ||f|| per instance, and displacements against the
+banded Cholesky solve to 1e-6.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import lame, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _items(seed=0, n=5, big=False):
+    """Small uneven cantilevers with random materials and tip loads."""
+    from tlfea_amd import mesh as M
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        nx = int(rng.integers(3, 12 if not big else 40))
+        ny = int(rng.integers(1, 4))
+        nz = int(rng.integers(1, 4))
+        L = float(rng.uniform(0.5, 2.0))
+        a = float(rng.uniform(0.05, 0.2))
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        m = M.clamp_and_tip_load(M.kuhn_beam(nx, ny, nz, L, a, a), d * float(10 ** rng.uniform(0, 4)))
+        out.append((m, dict(E=float(10 ** rng.uniform(6, 10)), nu=float(rng.uniform(0.2, 0.45)))))
+    return out
+
+
+def _oracle_system(m, E, nu):
+    go, djo = O.grad(m)
+    lam, mu = lame(E, nu)
+    doff, dcol = O.dof_pattern(m)
+    vo = O.stiffness_values(m, go, djo, lam, mu, doff, dcol)
+    vo, fo = O.apply_dirichlet(doff, dcol, vo, m.f_ext, m.fixed)
+    return doff, dcol, vo, fo
+
+
+def _k_norm1(doff, vo):
+    """max row sum of |K| (= ||K||_1 for the symmetric K), an upper bound of ||K||_2."""
+    return max(np.abs(vo[doff[i]:doff[i + 1]]).sum() for i in range(len(doff) - 1))
+
+
+def _residual_bar(tol, doff, vo, u, f):
+    return tol + 20 * 2.2e-16 * _k_norm1(doff, vo) * np.linalg.norm(u) / np.linalg.norm(f)
+
+
+def test_batch_operator_matches_single_and_oracle(gpu):
+    import torch
+    from tlfea_amd.batch import Batch
+    from tlfea_amd.fea import Body
+    items = _items(1, 4)
+    b = Batch(items)
+    rng = np.random.default_rng(2)
+    p = rng.standard_normal(3 * b.n_nodes)
+    y = _np(b.tangent_apply(torch.as_tensor(p).cuda()))
+    for i, (m, mat) in enumerate(items):
+        sl = b.inst_slice(i)
+        body = Body(m, mat["E"], mat["nu"], 1000.0)
+        yi = _np(body.tangent_apply(torch.as_tensor(p[sl]).cuda()))
+        assert rel_err(y[sl], yi) < 1e-13
+        doff, dcol, vo, _ = _oracle_system(m, mat["E"], mat["nu"])
+        assert rel_err(y[sl], O.spmv(doff, dcol, vo, p[sl])) < 1e-12
+
+
+def test_batch_cg_against_direct(gpu):
+    from tlfea_amd.batch import Batch
+    items = _items(3, 5)
+    b = Batch(items)
+    u = _np(b.solve(tol=1e-12, max_iter=100000))
+    assert b.rc == 0 and np.all(b.status == 0), (b.status, b.rel)
+    assert np.all(b.rel < 1e-12)
+    for i, (m, mat) in enumerate(items):
+        doff, dcol, vo, fo = _oracle_system(m, mat["E"], mat["nu"])
+        ud = O.band_cholesky_solve(doff, dcol, vo, fo)
+        ui = u[b.inst_slice(i)]
+        r = fo - O.spmv(doff, dcol, vo, ui)
+        assert np.linalg.norm(r) / np.linalg.norm(fo) < _residual_bar(1e-12, doff, vo, ui, fo), i
+        assert rel_err(ui, ud) < 1e-6, i
+        # same Krylov sequence as the oracle's CG: the first 10 iterates agree
+        if i == 0:
+            uk, _, _ = O.cg(doff, dcol, vo, fo, tol=0.0, max_iter=10)
+            b10 = Batch([items[0]])
+            u10 = _np(b10.solve(tol=0.0, max_iter=10, graph=False))
+            assert b10.iters[0] == 10 and rel_err(u10, uk) < 1e-11
+
+
+def test_batch_true_residual_and_sequential(gpu):
+    """tol 1e-8: the reported residual is the true one (recomputed with the FP64 operator)
+    and every instance agrees with its own single-mesh matrix-free solve."""
+    import torch
+    from tlfea_amd.batch import Batch
+    from tlfea_amd.fea import Body, LinearStatic
+    items = _items(4, 6, big=True)
+    b = Batch(items)
+    u = b.solve(tol=1e-8)
+    assert b.rc == 0, (b.status, b.rel)
+    r = b.f - b.tangent_apply(u)
+    r[b.mask.bool()] = 0.0
+    f = b.f.clone()
+    f[b.mask.bool()] = 0.0
+    for i, (m, mat) in enumerate(items):
+        sl = b.inst_slice(i)
+        rel = (torch.linalg.norm(r[sl]) / torch.linalg.norm(f[sl])).item()
+        doff, dcol, vo, fo = _oracle_system(m, mat["E"], mat["nu"])
+        assert rel < _residual_bar(1e-8, doff, vo, _np(u[sl]), fo), (i, rel)
+        body = Body(m, mat["E"], mat["nu"], 1000.0)
+        ls = LinearStatic(body, tol=1e-8, max_iter=200000, matfree=True)
+        ls.assemble()
+        assert ls.cg_solve() == 0
+        # two 1e-8 solutions of the same system differ by at most ~kappa * 1e-8
+        assert rel_err(_np(u[sl]), _np(ls.u)) < 1e-4, i
+        assert abs(int(b.iters[i]) - ls.iters) <= max(3, 0.25 * ls.iters), (i, b.iters[i], ls.iters)
+
+
+def test_batch_graph_matches_eager(gpu):
+    """The captured iteration graph is replayed many times per solve, with host polls in
+    between; three graph solves in a row agree with the eager-launch solve."""
+    from tlfea_amd.batch import Batch
+    items = _items(5, 5)
+    b = Batch(items)
+    ue = _np(b.solve(tol=1e-11, graph=False, poll=8))
+    it_e = b.iters.copy()
+    for _ in range(3):
+        ug = _np(b.solve(tol=1e-11, graph=True, poll=8))
+        assert b.rc == 0
+        assert rel_err(ug, ue) < 1e-8
+        assert np.all(np.abs(b.iters - it_e) <= np.maximum(3, 0.05 * it_e))
+
+
+def test_batch_guess_zero_load_and_limits(gpu):
+    import torch
+    from tlfea_amd import capi
+    from tlfea_amd.batch import Batch
+    items = _items(6, 4)
+    items[2][0].f_ext = np.zeros(3 * items[2][0].n_nodes)  # an unloaded instance
+    b = Batch(items)
+    u = b.solve(tol=1e-10)
+    assert b.rc == 0
+    assert b.iters[2] == 0 and np.all(_np(u[b.inst_slice(2)]) == 0)
+    # from the converged solution every instance stops at once (tolerance above the true
+    # residual of the converged u)
+    r = b.f - b.tangent_apply(u)
+    r[b.mask.bool()] = 0.0
+    f = b.f.clone()
+    f[b.mask.bool()] = 0.0
+    true = [(torch.linalg.norm(r[b.inst_slice(i)]) / max(torch.linalg.norm(f[b.inst_slice(i)]).item(), 1e-300)).item()
+            for i in range(len(items)) if i != 2]
+    b.solve(tol=2 * max(true), u0=u.clone())
+    assert np.all(b.iters == 0), b.iters
+    # an iteration limit below convergence reports the unfinished instances
+    b.solve(tol=1e-12, max_iter=5)
+    assert b.rc == capi.TLFEA_ENOCONV
+    assert b.status[2] == 0 and np.all(b.status[[0, 1, 3]] == capi.TLFEA_ENOCONV)
+    # the chunk size changes the reduction granularity, not the answer
+    b.set_chunk(64)
+    assert b.n_chunks() > len(items)
+    u64 = b.solve(tol=1e-10)
+    assert rel_err(_np(u64), _np(u)) < 1e-7
+
+
+def test_batch_fp32_chebyshev_preconditioner(gpu):
+    """Flexible FP64 CG with the FP32 Jacobi-Chebyshev stage: same solution as the plain
+    batched CG (both at tol 1e-10), true FP64 residual within the floor bar, and several
+    times fewer outer iterations (2.9-4.9x measured on these beams); compaction of finished instances changes nothing."""
+    import torch
+    from tlfea_amd.batch import Batch
+    items = _items(7, 6, big=True)
+    b = Batch(items)
+    u0 = _np(b.solve(tol=1e-10))
+    it0 = b.iters.copy()
+    lm = b.precond_setup(degree=6, ratio=30.0, power_iters=20)
+    assert np.all(lm > 0) and np.all(np.isfinite(lm))
+    # lambda_max(D^-1 K) of a stiffness matrix lies in (1, 2 * max nodes per element]
+    assert np.all(lm > 1.0) and np.all(lm < 20.0), lm
+    u1 = _np(b.solve(tol=1e-10, precond=True))
+    assert b.rc == 0, (b.status, b.rel)
+    assert np.all(b.iters * 2 < it0), (b.iters, it0)
+    r = b.f - b.tangent_apply(torch.as_tensor(u1).cuda())
+    r[b.mask.bool()] = 0.0
+    for i, (m, mat) in enumerate(items):
+        sl = b.inst_slice(i)
+        doff, dcol, vo, fo = _oracle_system(m, mat["E"], mat["nu"])
+        rel = np.linalg.norm(_np(r[sl])) / np.linalg.norm(fo)
+        assert rel < _residual_bar(1e-10, doff, vo, u1[sl], fo), (i, rel)
+        assert rel_err(u1[sl], u0[sl]) < 1e-6, i
+    # without compaction, and without the graph: same answers
+    b.set_compaction(0.0)
+    u2 = _np(b.solve(tol=1e-10, precond=True, graph=False))
+    assert rel_err(u2, u1) < 1e-7

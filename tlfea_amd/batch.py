@@ -1,0 +1,130 @@
+"""Host layer for batches of independent meshes (``tlfea_batch_*`` in include/tlfea_hip.h).
+
+``Batch`` concatenates a list of meshes with their materials into one device numbering
+and solves the linear-static problem K_i u_i = f_i of every instance at once; the
+per-instance loop it replaces is LinearStaticSolver::Solve called once per mesh
+(`src/solvers/LinearStaticSolver.cu:469-481`).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import capi
+from .capi import call, ptr
+
+DEV = "cuda"
+
+
+def lame(E, nu):
+    return E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
+
+
+class Batch:
+    """items: list of (TetMesh, dict(E=..., nu=...)); every mesh has the same element type."""
+
+    def __init__(self, items, stream=None):
+        if not items:
+            raise ValueError("empty batch")
+        nn = items[0][0].nn
+        if any(m.nn != nn for m, _ in items):
+            raise ValueError("all instances must have the same element type")
+        self.items = items
+        self.nn = nn
+        self.n_inst = len(items)
+        self.stream = stream
+        ne = np.array([m.n_elem for m, _ in items], np.int64)
+        nv = np.array([m.n_nodes for m, _ in items], np.int64)
+        self.elem_off = np.concatenate([[0], np.cumsum(ne)]).astype(np.int32)
+        self.node_off = np.concatenate([[0], np.cumsum(nv)]).astype(np.int32)
+        self.n_elem = int(self.elem_off[-1])
+        self.n_nodes = int(self.node_off[-1])
+        elems = np.concatenate([np.asarray(m.elems, np.int64) + int(o) for (m, _), o in zip(items, self.node_off[:-1])])
+        if elems.min() < 0 or elems.max() >= self.n_nodes:
+            raise ValueError("connectivity references nodes outside the batch")
+        conn = np.ascontiguousarray(elems.T.reshape(-1)).astype(np.int32)  # column-major [nn][n_elem]
+        xyz = np.concatenate([m.xyz() for m, _ in items])
+        mask = np.zeros(3 * self.n_nodes, np.uint8)
+        f = np.zeros(3 * self.n_nodes)
+        for (m, _), o in zip(items, self.node_off[:-1]):
+            fx = np.asarray(m.fixed, np.int64) + int(o)
+            mask[(3 * fx[:, None] + np.arange(3)).reshape(-1)] = 1
+            if m.f_ext is not None:
+                f[3 * int(o): 3 * (int(o) + m.n_nodes)] = m.f_ext
+        lam_mu = np.array([lame(mat["E"], mat["nu"]) for _, mat in items])
+        self.lam = np.ascontiguousarray(lam_mu[:, 0])
+        self.mu = np.ascontiguousarray(lam_mu[:, 1])
+        self.conn = torch.as_tensor(conn).to(DEV)
+        self.X = [torch.as_tensor(np.ascontiguousarray(xyz[:, d])).to(DEV) for d in range(3)]
+        self.mask = torch.as_tensor(mask).to(DEV)
+        self.f = torch.as_tensor(f).to(DEV)
+        h = C.c_void_p()
+        eo, no = self.elem_off, self.node_off
+        call("tlfea_batch_create", nn, self.n_inst, eo.ctypes.data, no.ctypes.data, ptr(self.conn), ptr(self.X[0]),
+             ptr(self.X[1]), ptr(self.X[2]), self.lam.ctypes.data, self.mu.ctypes.data, ptr(self.mask), C.byref(h),
+             self.stream)
+        self.h = h
+        self.iters = np.zeros(self.n_inst, np.int32)
+        self.rel = np.zeros(self.n_inst)
+        self.status = np.zeros(self.n_inst, np.int32)
+        self.total_iters = 0
+
+    def inst_slice(self, i):
+        """DOF slice of instance i."""
+        return slice(3 * int(self.node_off[i]), 3 * int(self.node_off[i + 1]))
+
+    def set_chunk(self, nodes):
+        call("tlfea_batch_set_chunk", self.h, int(nodes))
+
+    def set_compaction(self, ratio):
+        call("tlfea_batch_set_compaction", self.h, float(ratio))
+
+    def n_chunks(self):
+        a = C.c_int()
+        call("tlfea_batch_sizes", self.h, None, None, None, C.byref(a))
+        return a.value
+
+    def tangent_apply(self, p, out=None):
+        out = torch.empty(3 * self.n_nodes, dtype=torch.float64, device=DEV) if out is None else out
+        call("tlfea_batch_tangent_apply", self.h, ptr(p), ptr(out), self.stream)
+        return out
+
+    def precond_setup(self, degree=6, ratio=30.0, power_iters=20, safety=1.1):
+        """FP32 Jacobi-Chebyshev preconditioner; returns the lambda_max(D^-1 K) estimates."""
+        lm = np.zeros(self.n_inst)
+        call("tlfea_batch_precond_setup", self.h, int(degree), float(ratio), int(power_iters), float(safety),
+             lm.ctypes.data, self.stream)
+        self.precond_degree = degree
+        return lm
+
+    def solve(self, tol=1e-8, max_iter=100000, u0=None, poll=32, graph=True, f=None, out=None, precond=False):
+        """Returns u (device, all instances); per-instance iterations, relative residuals
+        and status land in self.iters / self.rel / self.status.  precond=True uses the
+        FP32 Chebyshev stage set up by precond_setup() inside a flexible FP64 CG."""
+        f = self.f if f is None else f
+        u = out if out is not None else torch.empty(3 * self.n_nodes, dtype=torch.float64, device=DEV)
+        if u0 is not None:
+            u.copy_(u0)
+        tot = C.c_int()
+        rc = capi.lib().tlfea_batch_cg_solve(self.h, ptr(f), ptr(u), int(u0 is not None), float(tol), int(max_iter),
+                                             int(poll), int(bool(graph)), int(bool(precond)), self.iters.ctypes.data,
+                                             self.rel.ctypes.data, self.status.ctypes.data, C.byref(tot),
+                                             self.stream)
+        self.total_iters = tot.value
+        if rc not in (capi.TLFEA_OK, capi.TLFEA_ENOCONV):
+            capi.check(rc)
+        self.rc = rc
+        return u
+
+    def close(self):
+        if getattr(self, "h", None) is not None:
+            capi.lib().tlfea_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

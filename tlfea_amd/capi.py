@@ -117,6 +117,14 @@ _SIGS = {
     "tlfea_cg_solve": (i32, [vp, vp, vp, vp, f64, i32, i32, C.POINTER(i32), C.POINTER(f64), vp]),
     "tlfea_cg_solve_matfree": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(Quad), C.POINTER(Material),
                                      vp, vp, vp, f64, i32, i32, C.POINTER(i32), C.POINTER(f64), vp]),
+    "tlfea_batch_create": (i32, [i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(vp), vp]),
+    "tlfea_batch_destroy": (i32, [vp]),
+    "tlfea_batch_sizes": (i32, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
+    "tlfea_batch_set_chunk": (i32, [vp, i32]),
+    "tlfea_batch_tangent_apply": (i32, [vp, vp, vp, vp]),
+    "tlfea_batch_cg_solve": (i32, [vp, vp, vp, i32, f64, i32, i32, i32, i32, vp, vp, vp, C.POINTER(i32), vp]),
+    "tlfea_batch_set_compaction": (i32, [vp, f64]),
+    "tlfea_batch_precond_setup": (i32, [vp, i32, f64, i32, f64, vp, vp]),
     "tlfea_add_displacement": (i32, [i32, vp, vp, vp, vp, vp]),
     "tlfea_matrix_to_f32": (i32, [vp, vp, vp, vp]),
     "tlfea_cg_solve_mixed": (i32, [vp, vp, vp, vp, vp, f64, f64, i32, i32, C.POINTER(i32), C.POINTER(f64),

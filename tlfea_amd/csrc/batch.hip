@@ -1,0 +1,992 @@
+// Many independent meshes solved together (not in the reference).
+//
+// The instances are concatenated into one element/node numbering (instance i owns the
+// contiguous ranges elem_off[i]..elem_off[i+1] and node_off[i]..node_off[i+1]), so one
+// launch of each kernel covers the whole batch.  Element kernels use the force plan of
+// the concatenated mesh (a block may straddle two instances: nodes are never shared
+// between instances, so a node's partials still come from its own instance only) and look
+// up the instance's material per element.  Vector kernels run over "chunks": node ranges
+// of at most chunk_nodes nodes inside one instance, one workgroup each, so that every
+// dot product is reduced per chunk and then, in a fixed order, per instance.
+//
+// The linear-static CG is the reference's SolveLinearSystemCG
+// (src/solvers/LinearStaticSolver.cu:348-443) run for every instance at once, with its
+// own alpha/beta and stopping test per instance; finished instances stop updating and
+// their elements are skipped.  The stopping test is ||f - K u|| / ||f|| < tol on the free
+// DOFs (the reference's sqrt(rr/rr0) when the initial guess is zero).  An optional FP32
+// Chebyshev preconditioner applies the same operator in single precision.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "internal.h"
+
+struct tlfea_batch {
+    int nn = 0, n_inst = 0, n_elem = 0, n_nodes = 0;
+    tlfea_quad quad{};
+    tlfea_force_plan* plan = nullptr;
+    std::vector<int> h_elem_off, h_node_off;
+    int* d_conn = nullptr;
+    double *d_X = nullptr, *d_Y = nullptr, *d_Z = nullptr;
+    unsigned char* d_mask = nullptr;
+    int* d_elem_inst = nullptr;
+    double *d_jinv = nullptr, *d_detJ = nullptr;
+    float *d_jinv32 = nullptr, *d_detJ32 = nullptr;
+    double *d_lam = nullptr, *d_mu = nullptr;
+    float *d_lam32 = nullptr, *d_mu32 = nullptr;
+    // chunk tables
+    int chunk_nodes = 0, n_chunks = 0;
+    int *d_chunk_inst = nullptr, *d_chunk_beg = nullptr, *d_chunk_end = nullptr, *d_inst_chunk_off = nullptr;
+    // solver workspace
+    double *d_r = nullptr, *d_p = nullptr, *d_q = nullptr, *d_z = nullptr;
+    float *d_r32 = nullptr, *d_z32 = nullptr, *d_d32 = nullptr, *d_q32 = nullptr;
+    double* d_part = nullptr;  // [n_chunks * 2]
+    double* d_s = nullptr;     // [n_inst * NS] per-instance scalars
+    int* d_i = nullptr;        // [n_inst * NI] per-instance flags
+    int* d_all = nullptr;      // [4]: 0 active instances
+    double* d_cheb = nullptr;  // [n_inst * 4]: lambda_max estimate, theta, delta, sigma1
+    double* d_rold = nullptr;  // previous residual (flexible CG)
+    float* d_dinv32 = nullptr;  // inverse diagonal of K (1 on fixed DOFs)
+    int cheb_degree = 0;        // 0 = no preconditioner set up
+    // active-work lists (used once finished instances are compacted out of the launches)
+    std::vector<int> h_inst_chunk_off;
+    int* d_blk_list = nullptr;
+    int* d_chunk_list = nullptr;
+    int* d_inst_list = nullptr;
+    bool compact = false;
+    int n_blk_list = 0, n_chunk_list = 0, n_inst_list = 0;
+    int64_t launched_elems = 0;
+    double compact_ratio = 0.6;  // compact when active elements fall below this share
+    hipStream_t s_own = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int graph_iters = 0, graph_precond = -1;
+};
+
+namespace tlfea_hip {
+
+constexpr int BT = 256;  // threads per chunk workgroup
+// per-instance scalars
+enum { S_FF = 0, S_RZ, S_RR, S_ALPHA, S_BETA, S_PQ, S_REL, S_RZ_OLD, NS };
+// Chebyshev parameters per instance
+enum { C_LMAX = 0, C_THETA, C_DELTA, C_SIGMA, NC };
+// per-instance flags
+enum { I_DONE = 0, I_ITERS, I_NOTPD, I_ZERO_F, NI };
+
+__device__ __forceinline__ double chunk_sum(double v) {
+    __shared__ double red[BT / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < BT / 64; w++) t += red[w];
+    return t;
+}
+
+struct Chunks {
+    const int* inst;
+    const int* beg;
+    const int* end;
+    const int* list;  // launched chunk ids (null = all)
+};
+__device__ __forceinline__ int chunk_id(const Chunks& ch) { return ch.list ? ch.list[blockIdx.x] : blockIdx.x; }
+
+// instance ids handled by the per-instance reduction kernels (null list = 0..n-1)
+struct Insts {
+    int n;
+    const int* list;
+};
+__device__ __forceinline__ int inst_at(const Insts& in, int j) { return in.list ? in.list[j] : j; }
+
+// q = sum of partials per node (or p on masked DOFs); chunk partial of p.q
+template <typename T>
+__global__ __launch_bounds__(BT) void k_bgather(Chunks ch, const int* __restrict__ g_off,
+                                                const int64_t* __restrict__ g_idx, const T* __restrict__ part,
+                                                const unsigned char* __restrict__ mask, const T* __restrict__ p,
+                                                T* __restrict__ q, double* __restrict__ cpart,
+                                                const int* __restrict__ iflags) {
+    const int c = chunk_id(ch);
+    const int inst = ch.inst[c];
+    if (iflags && iflags[inst * NI + I_DONE]) {
+        if (threadIdx.x == 0 && cpart) cpart[c] = 0.0;
+        return;
+    }
+    double dot = 0.0;
+    for (int i = ch.beg[c] + threadIdx.x; i < ch.end[c]; i += BT) {
+        T s0 = T(0), s1 = T(0), s2 = T(0);
+        for (int k = g_off[i]; k < g_off[i + 1]; k++) {
+            const T* pp = part + 3 * g_idx[k];
+            s0 += pp[0];
+            s1 += pp[1];
+            s2 += pp[2];
+        }
+        const T sv[3] = {s0, s1, s2};
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const T v = mask[3 * i + d] ? p[3 * i + d] : sv[d];
+            q[3 * i + d] = v;
+            dot += (double)p[3 * i + d] * (double)v;
+        }
+    }
+    dot = chunk_sum(dot);
+    if (threadIdx.x == 0 && cpart) cpart[c] = dot;
+}
+
+// r = f - q on free DOFs (0 on masked); chunk partials of f.f and r.r
+__global__ __launch_bounds__(BT) void k_binit(Chunks ch, const double* __restrict__ f, const double* __restrict__ q,
+                                              const unsigned char* __restrict__ mask, double* __restrict__ r,
+                                              double* __restrict__ cpart) {
+    const int c = chunk_id(ch);
+    double ff = 0.0, rr = 0.0;
+    for (int64_t k = 3 * (int64_t)ch.beg[c] + threadIdx.x; k < 3 * (int64_t)ch.end[c]; k += BT) {
+        const bool fx = mask[k] != 0;
+        const double fk = fx ? 0.0 : f[k];
+        const double rk = fx ? 0.0 : fk - q[k];
+        r[k] = rk;
+        ff += fk * fk;
+        rr += rk * rk;
+    }
+    ff = chunk_sum(ff);
+    rr = chunk_sum(rr);
+    if (threadIdx.x == 0) {
+        cpart[2 * c] = ff;
+        cpart[2 * c + 1] = rr;
+    }
+}
+
+// per instance: ||f||^2, r.r, stopping test at the initial guess
+__global__ __launch_bounds__(BT) void k_breduce_init(int n_inst, const int* __restrict__ ico,
+                                                     const double* __restrict__ cpart, double* __restrict__ S,
+                                                     int* __restrict__ I, int* __restrict__ all, double tol) {
+    int active = 0;
+    for (int i = threadIdx.x; i < n_inst; i += BT) {
+        double ff = 0.0, rr = 0.0;
+        for (int c = ico[i]; c < ico[i + 1]; c++) {
+            ff += cpart[2 * c];
+            rr += cpart[2 * c + 1];
+        }
+        double* s = S + (int64_t)i * NS;
+        int* fl = I + (int64_t)i * NI;
+        s[S_FF] = ff;
+        s[S_RR] = rr;
+        s[S_RZ] = rr;
+        s[S_BETA] = 0.0;
+        s[S_REL] = ff > 0.0 ? sqrt(rr / ff) : 0.0;
+        fl[I_ITERS] = 0;
+        fl[I_NOTPD] = 0;
+        fl[I_ZERO_F] = ff == 0.0;
+        fl[I_DONE] = (ff == 0.0 || rr <= tol * tol * ff) ? 1 : 0;
+        active += !fl[I_DONE];
+    }
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    atomicAdd(&cnt, active);
+    __syncthreads();
+    if (threadIdx.x == 0) all[0] = cnt;
+}
+
+// u = 0 for instances without load (the reference's CG returns 0 for f = 0)
+__global__ __launch_bounds__(BT) void k_bzero_u(Chunks ch, const int* __restrict__ I, double* __restrict__ u) {
+    const int c = chunk_id(ch);
+    if (!I[ch.inst[c] * NI + I_ZERO_F]) return;
+    for (int64_t k = 3 * (int64_t)ch.beg[c] + threadIdx.x; k < 3 * (int64_t)ch.end[c]; k += BT) u[k] = 0.0;
+}
+
+// per instance: alpha = rz / p.q, abort on p.q <= 0 (LinearStaticSolver.cu:404-407)
+__global__ __launch_bounds__(BT) void k_breduce_alpha(Insts in, const int* __restrict__ ico,
+                                                      const double* __restrict__ cpart, double* __restrict__ S,
+                                                      int* __restrict__ I) {
+    for (int j = threadIdx.x; j < in.n; j += BT) {
+        const int i = inst_at(in, j);
+        int* fl = I + (int64_t)i * NI;
+        if (fl[I_DONE]) continue;
+        double pq = 0.0;
+        for (int c = ico[i]; c < ico[i + 1]; c++) pq += cpart[c];
+        double* s = S + (int64_t)i * NS;
+        s[S_PQ] = pq;
+        if (pq <= 0.0) {
+            fl[I_NOTPD] = 1;
+            fl[I_DONE] = 1;
+        } else {
+            s[S_ALPHA] = s[S_RZ] / pq;
+        }
+    }
+}
+
+// u += alpha p, r -= alpha q; chunk partial of r.r
+__global__ __launch_bounds__(BT) void k_bupdate(Chunks ch, double* __restrict__ u, double* __restrict__ r,
+                                                const double* __restrict__ p, const double* __restrict__ q,
+                                                const double* __restrict__ S, const int* __restrict__ I,
+                                                double* __restrict__ cpart) {
+    const int c = chunk_id(ch);
+    const int inst = ch.inst[c];
+    if (I[inst * NI + I_DONE]) {
+        if (threadIdx.x == 0) cpart[c] = 0.0;
+        return;
+    }
+    const double alpha = S[(int64_t)inst * NS + S_ALPHA];
+    double rr = 0.0;
+    for (int64_t k = 3 * (int64_t)ch.beg[c] + threadIdx.x; k < 3 * (int64_t)ch.end[c]; k += BT) {
+        u[k] += alpha * p[k];
+        const double rk = r[k] - alpha * q[k];
+        r[k] = rk;
+        rr += rk * rk;
+    }
+    rr = chunk_sum(rr);
+    if (threadIdx.x == 0) cpart[c] = rr;
+}
+
+// per instance: stopping test on ||r|| / ||f|| and beta = rr' / rr (no preconditioner)
+__global__ __launch_bounds__(BT) void k_breduce_beta(Insts in, const int* __restrict__ ico,
+                                                     const double* __restrict__ cpart, double* __restrict__ S,
+                                                     int* __restrict__ I, int* __restrict__ all, double tol) {
+    int active = 0;
+    for (int j = threadIdx.x; j < in.n; j += BT) {
+        const int i = inst_at(in, j);
+        int* fl = I + (int64_t)i * NI;
+        if (fl[I_DONE]) continue;
+        double rr = 0.0;
+        for (int c = ico[i]; c < ico[i + 1]; c++) rr += cpart[c];
+        double* s = S + (int64_t)i * NS;
+        fl[I_ITERS] += 1;
+        s[S_RR] = rr;
+        s[S_REL] = sqrt(rr / s[S_FF]);
+        if (rr < tol * tol * s[S_FF]) {
+            fl[I_DONE] = 1;
+        } else {
+            s[S_BETA] = rr / s[S_RZ];
+            s[S_RZ] = rr;
+            active++;
+        }
+    }
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    atomicAdd(&cnt, active);
+    __syncthreads();
+    if (threadIdx.x == 0) all[0] = cnt;
+}
+
+// p = z + beta p
+__global__ __launch_bounds__(BT) void k_bdir(Chunks ch, double* __restrict__ p, const double* __restrict__ z,
+                                             const double* __restrict__ S, const int* __restrict__ I) {
+    const int c = chunk_id(ch);
+    const int inst = ch.inst[c];
+    if (I[inst * NI + I_DONE]) return;
+    const double beta = S[(int64_t)inst * NS + S_BETA];
+    for (int64_t k = 3 * (int64_t)ch.beg[c] + threadIdx.x; k < 3 * (int64_t)ch.end[c]; k += BT)
+        p[k] = beta != 0.0 ? z[k] + beta * p[k] : z[k];
+}
+
+// ---- FP32 Jacobi-Chebyshev preconditioner ----------------------------------------------
+// z ~= K^-1 r by `degree` Chebyshev steps on D^-1 K over [lmax/ratio, lmax] (Saad,
+// Iterative Methods for Sparse Linear Systems, 2nd ed., Alg. 12.1), all in FP32; the
+// polynomial is positive on (0, lmax], so the preconditioner is SPD up to rounding and
+// the outer FP64 CG uses the flexible (Polak-Ribiere) beta.
+
+// rho_k of the three-term recursion: rho_0 = 1/sigma, rho_k = 1/(2 sigma - rho_{k-1})
+__device__ __forceinline__ float cheb_rho(float sigma, int k) {
+    float rho = 1.0f / sigma;
+    for (int j = 0; j < k; j++) rho = 1.0f / (2.0f * sigma - rho);
+    return rho;
+}
+
+// res = r, d = D^-1 res / theta, z = d
+__global__ __launch_bounds__(BT) void k_cheb_init(Chunks ch, const double* __restrict__ r,
+                                                  const float* __restrict__ dinv, const double* __restrict__ C,
+                                                  const int* __restrict__ I, float* __restrict__ res,
+                                                  float* __restrict__ d, float* __restrict__ zc) {
+    const int c = chunk_id(ch);
+    const int inst = ch.inst[c];
+    if (I[inst * NI + I_DONE]) return;
+    const float ith = (float)(1.0 / C[(int64_t)inst * NC + C_THETA]);
+    for (int64_t k = 3 * (int64_t)ch.beg[c] + threadIdx.x; k < 3 * (int64_t)ch.end[c]; k += BT) {
+        const float rk = (float)r[k];
+        const float dk = dinv[k] * rk * ith;
+        res[k] = rk;
+        d[k] = dk;
+        zc[k] = dk;
+    }
+}
+
+// step k >= 1: res -= q (q = K d), d = rho_k rho_{k-1} d + (2 rho_k / delta) D^-1 res, z += d
+__global__ __launch_bounds__(BT) void k_cheb_step(Chunks ch, int k, const float* __restrict__ q,
+                                                  const float* __restrict__ dinv, const double* __restrict__ C,
+                                                  const int* __restrict__ I, float* __restrict__ res,
+                                                  float* __restrict__ d, float* __restrict__ zc) {
+    const int c = chunk_id(ch);
+    const int inst = ch.inst[c];
+    if (I[inst * NI + I_DONE]) return;
+    const float sigma = (float)C[(int64_t)inst * NC + C_SIGMA];
+    const float delta = (float)C[(int64_t)inst * NC + C_DELTA];
+    const float rp = cheb_rho(sigma, k - 1), rk = cheb_rho(sigma, k);
+    const float a = rk * rp, bcoef = 2.0f * rk / delta;
+    for (int64_t j = 3 * (int64_t)ch.beg[c] + threadIdx.x; j < 3 * (int64_t)ch.end[c]; j += BT) {
+        const float rj = res[j] - q[j];
+        res[j] = rj;
+        const float dj = a * d[j] + bcoef * dinv[j] * rj;
+        d[j] = dj;
+        zc[j] += dj;
+    }
+}
+
+// z = z32 in FP64; chunk partials of z.r and z.r_old
+__global__ __launch_bounds__(BT) void k_cheb_fin(Chunks ch, const float* __restrict__ zc,
+                                                 const double* __restrict__ r, const double* __restrict__ rold,
+                                                 const int* __restrict__ I, double* __restrict__ z,
+                                                 double* __restrict__ cpart) {
+    const int c = chunk_id(ch);
+    const int inst = ch.inst[c];
+    double zr = 0.0, zro = 0.0;
+    if (!I[inst * NI + I_DONE]) {
+        for (int64_t k = 3 * (int64_t)ch.beg[c] + threadIdx.x; k < 3 * (int64_t)ch.end[c]; k += BT) {
+            const double zk = (double)zc[k];
+            z[k] = zk;
+            zr += zk * r[k];
+            zro += zk * rold[k];
+        }
+    }
+    zr = chunk_sum(zr);
+    zro = chunk_sum(zro);
+    if (threadIdx.x == 0) {
+        cpart[2 * c] = zr;
+        cpart[2 * c + 1] = zro;
+    }
+}
+
+// per instance after the first preconditioning: rz = z.r, p = z (beta = 0)
+__global__ __launch_bounds__(BT) void k_breduce_rz0(Insts in, const int* __restrict__ ico,
+                                                    const double* __restrict__ cpart, double* __restrict__ S,
+                                                    const int* __restrict__ I) {
+    for (int j = threadIdx.x; j < in.n; j += BT) {
+        const int i = inst_at(in, j);
+        if (I[(int64_t)i * NI + I_DONE]) continue;
+        double zr = 0.0;
+        for (int c = ico[i]; c < ico[i + 1]; c++) zr += cpart[2 * c];
+        S[(int64_t)i * NS + S_RZ] = zr;
+        S[(int64_t)i * NS + S_BETA] = 0.0;
+    }
+}
+
+// u += alpha p, r_old = r, r -= alpha q; chunk partial of r.r
+__global__ __launch_bounds__(BT) void k_bupdate_keep(Chunks ch, double* __restrict__ u, double* __restrict__ r,
+                                                     double* __restrict__ rold, const double* __restrict__ p,
+                                                     const double* __restrict__ q, const double* __restrict__ S,
+                                                     const int* __restrict__ I, double* __restrict__ cpart) {
+    const int c = chunk_id(ch);
+    const int inst = ch.inst[c];
+    if (I[inst * NI + I_DONE]) {
+        if (threadIdx.x == 0) cpart[c] = 0.0;
+        return;
+    }
+    const double alpha = S[(int64_t)inst * NS + S_ALPHA];
+    double rr = 0.0;
+    for (int64_t k = 3 * (int64_t)ch.beg[c] + threadIdx.x; k < 3 * (int64_t)ch.end[c]; k += BT) {
+        u[k] += alpha * p[k];
+        const double r0 = r[k];
+        rold[k] = r0;
+        const double rk = r0 - alpha * q[k];
+        r[k] = rk;
+        rr += rk * rk;
+    }
+    rr = chunk_sum(rr);
+    if (threadIdx.x == 0) cpart[c] = rr;
+}
+
+// per instance: stopping test only (preconditioned path)
+__global__ __launch_bounds__(BT) void k_breduce_check(Insts in, const int* __restrict__ ico,
+                                                      const double* __restrict__ cpart, double* __restrict__ S,
+                                                      int* __restrict__ I, int* __restrict__ all, double tol) {
+    int active = 0;
+    for (int j = threadIdx.x; j < in.n; j += BT) {
+        const int i = inst_at(in, j);
+        int* fl = I + (int64_t)i * NI;
+        if (fl[I_DONE]) continue;
+        double rr = 0.0;
+        for (int c = ico[i]; c < ico[i + 1]; c++) rr += cpart[c];
+        double* s = S + (int64_t)i * NS;
+        fl[I_ITERS] += 1;
+        s[S_RR] = rr;
+        s[S_REL] = sqrt(rr / s[S_FF]);
+        if (rr < tol * tol * s[S_FF])
+            fl[I_DONE] = 1;
+        else
+            active++;
+    }
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    atomicAdd(&cnt, active);
+    __syncthreads();
+    if (threadIdx.x == 0) all[0] = cnt;
+}
+
+// per instance: flexible beta = z.(r - r_old) / rz_old
+__global__ __launch_bounds__(BT) void k_breduce_fbeta(Insts in, const int* __restrict__ ico,
+                                                      const double* __restrict__ cpart, double* __restrict__ S,
+                                                      const int* __restrict__ I) {
+    for (int j = threadIdx.x; j < in.n; j += BT) {
+        const int i = inst_at(in, j);
+        if (I[(int64_t)i * NI + I_DONE]) continue;
+        double zr = 0.0, zro = 0.0;
+        for (int c = ico[i]; c < ico[i + 1]; c++) {
+            zr += cpart[2 * c];
+            zro += cpart[2 * c + 1];
+        }
+        double* s = S + (int64_t)i * NS;
+        s[S_BETA] = (zr - zro) / s[S_RZ];
+        s[S_RZ] = zr;
+    }
+}
+
+// ---- power iteration for lambda_max(D^-1 K) per instance (FP32) -----------------------
+__global__ __launch_bounds__(BT) void k_pow_start(Chunks ch, const unsigned char* __restrict__ mask,
+                                                  float* __restrict__ v) {
+    const int c = chunk_id(ch);
+    for (int64_t k = 3 * (int64_t)ch.beg[c] + threadIdx.x; k < 3 * (int64_t)ch.end[c]; k += BT) {
+        // deterministic pseudo-random start in [0.5, 1.5)
+        uint32_t h = (uint32_t)k * 2654435761u;
+        h ^= h >> 15;
+        h *= 2246822519u;
+        h ^= h >> 13;
+        v[k] = mask[k] ? 0.0f : 0.5f + (float)(h & 0xffffu) / 65536.0f;
+    }
+}
+
+// y = D^-1 w on free DOFs; chunk partial of |y|^2
+__global__ __launch_bounds__(BT) void k_pow_scale(Chunks ch, const float* __restrict__ w,
+                                                  const float* __restrict__ dinv,
+                                                  const unsigned char* __restrict__ mask, float* __restrict__ y,
+                                                  double* __restrict__ cpart) {
+    const int c = chunk_id(ch);
+    double yy = 0.0;
+    for (int64_t k = 3 * (int64_t)ch.beg[c] + threadIdx.x; k < 3 * (int64_t)ch.end[c]; k += BT) {
+        const float yk = mask[k] ? 0.0f : dinv[k] * w[k];
+        y[k] = yk;
+        yy += (double)yk * yk;
+    }
+    yy = chunk_sum(yy);
+    if (threadIdx.x == 0) cpart[c] = yy;
+}
+
+// per instance: lambda = |y| / |v_prev| (|v_prev| = 1 after the first pass), stored in C
+__global__ __launch_bounds__(BT) void k_pow_reduce(int n_inst, const int* __restrict__ ico,
+                                                   const double* __restrict__ cpart, double* __restrict__ C) {
+    for (int i = threadIdx.x; i < n_inst; i += BT) {
+        double yy = 0.0;
+        for (int c = ico[i]; c < ico[i + 1]; c++) yy += cpart[c];
+        C[(int64_t)i * NC + C_LMAX] = sqrt(yy);
+    }
+}
+
+// v = y / |y| per instance (the previous pass's norm)
+__global__ __launch_bounds__(BT) void k_pow_norm(Chunks ch, const float* __restrict__ y, const double* __restrict__ C,
+                                                 float* __restrict__ v) {
+    const int c = chunk_id(ch);
+    const double nrm = C[(int64_t)ch.inst[c] * NC + C_LMAX];
+    const float s = nrm > 0.0 ? (float)(1.0 / nrm) : 0.0f;
+    for (int64_t k = 3 * (int64_t)ch.beg[c] + threadIdx.x; k < 3 * (int64_t)ch.end[c]; k += BT) v[k] = y[k] * s;
+}
+
+// Chebyshev interval [b/ratio, b] with b = safety * lambda estimate
+__global__ void k_cheb_params(int n_inst, double safety, double ratio, double* __restrict__ C) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_inst) return;
+    double* c = C + (int64_t)i * NC;
+    const double bmax = safety * c[C_LMAX];
+    const double amin = bmax / ratio;
+    c[C_THETA] = 0.5 * (bmax + amin);
+    c[C_DELTA] = 0.5 * (bmax - amin);
+    c[C_SIGMA] = c[C_THETA] / c[C_DELTA];
+}
+
+// D^-1 in FP32 from the gathered diagonal (1 on fixed DOFs, where the gather returns p = 1)
+__global__ void k_inv_diag(int64_t n, const double* __restrict__ dg, float* __restrict__ dinv) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dinv[i] = dg[i] > 0.0 ? (float)(1.0 / dg[i]) : 1.0f;
+}
+
+__global__ void k_fill(int64_t n, double v, double* __restrict__ a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+__global__ void k_elem_inst(int n_inst, const int* __restrict__ elem_off, int* __restrict__ elem_inst) {
+    const int i = blockIdx.x;
+    if (i >= n_inst) return;
+    for (int e = elem_off[i] + threadIdx.x; e < elem_off[i + 1]; e += blockDim.x) elem_inst[e] = i;
+}
+
+__global__ void k_d2f(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
+}  // namespace tlfea_hip
+
+using namespace tlfea_hip;
+
+namespace {
+
+template <typename T>
+int dalloc(T** p, size_t n) {
+    if (n == 0) n = 1;
+    if (hipMalloc((void**)p, sizeof(T) * n) != hipSuccess) {
+        set_error("device allocation of %zu bytes failed", sizeof(T) * n);
+        return TLFEA_EHIP;
+    }
+    return TLFEA_OK;
+}
+
+void free_batch(tlfea_batch* b) {
+    if (!b) return;
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    if (b->plan) tlfea_force_plan_destroy(b->plan);
+    void* ptrs[] = {b->d_conn, b->d_X, b->d_Y, b->d_Z, b->d_mask, b->d_elem_inst, b->d_jinv, b->d_detJ,
+                    b->d_jinv32, b->d_detJ32, b->d_lam, b->d_mu, b->d_lam32, b->d_mu32, b->d_chunk_inst,
+                    b->d_chunk_beg, b->d_chunk_end, b->d_inst_chunk_off, b->d_r, b->d_p, b->d_q, b->d_z,
+                    b->d_r32, b->d_z32, b->d_d32, b->d_q32, b->d_part, b->d_s, b->d_i, b->d_all, b->d_cheb,
+                    b->d_rold, b->d_dinv32, b->d_blk_list, b->d_chunk_list, b->d_inst_list};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (b->s_own) (void)hipStreamDestroy(b->s_own);
+    delete b;
+}
+
+Chunks chunks(const tlfea_batch* b) {
+    return Chunks{b->d_chunk_inst, b->d_chunk_beg, b->d_chunk_end, b->compact ? b->d_chunk_list : nullptr};
+}
+int n_chunk_launch(const tlfea_batch* b) { return b->compact ? b->n_chunk_list : b->n_chunks; }
+Insts insts(const tlfea_batch* b) {
+    return b->compact ? Insts{b->n_inst_list, b->d_inst_list} : Insts{b->n_inst, nullptr};
+}
+
+void drop_graph(tlfea_batch* b) {
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+}
+
+// Restricts the launches to the instances not yet done (host flags `done`).
+int compact_lists(tlfea_batch* b, const std::vector<int>& done) {
+    std::vector<int> il, cl, bl;
+    int64_t elems = 0;
+    for (int i = 0; i < b->n_inst; i++) {
+        if (done[i]) continue;
+        il.push_back(i);
+        for (int c = b->h_inst_chunk_off[i]; c < b->h_inst_chunk_off[i + 1]; c++) cl.push_back(c);
+        const int b0 = b->h_elem_off[i] / PLAN_BLOCK, b1 = (b->h_elem_off[i + 1] - 1) / PLAN_BLOCK;
+        for (int k = b0; k <= b1; k++)
+            if (bl.empty() || bl.back() < k) bl.push_back(k);
+        elems += b->h_elem_off[i + 1] - b->h_elem_off[i];
+    }
+    if (il.empty()) return TLFEA_OK;
+    TLFEA_HIP_TRY(hipMemcpy(b->d_inst_list, il.data(), sizeof(int) * il.size(), hipMemcpyHostToDevice));
+    TLFEA_HIP_TRY(hipMemcpy(b->d_chunk_list, cl.data(), sizeof(int) * cl.size(), hipMemcpyHostToDevice));
+    TLFEA_HIP_TRY(hipMemcpy(b->d_blk_list, bl.data(), sizeof(int) * bl.size(), hipMemcpyHostToDevice));
+    b->n_inst_list = (int)il.size();
+    b->n_chunk_list = (int)cl.size();
+    b->n_blk_list = (int)bl.size();
+    b->compact = true;
+    b->launched_elems = elems;
+    drop_graph(b);
+    return TLFEA_OK;
+}
+
+void full_lists(tlfea_batch* b) {
+    if (b->compact) drop_graph(b);
+    b->compact = false;
+    b->launched_elems = b->n_elem;
+}
+
+// (Re)builds the chunk tables for a chunk size.
+int build_chunks(tlfea_batch* b, int chunk_nodes) {
+    std::vector<int> ci, cb, ce, ico(b->n_inst + 1, 0);
+    for (int i = 0; i < b->n_inst; i++) {
+        ico[i] = (int)ci.size();
+        for (int n0 = b->h_node_off[i]; n0 < b->h_node_off[i + 1]; n0 += chunk_nodes) {
+            ci.push_back(i);
+            cb.push_back(n0);
+            ce.push_back(std::min(n0 + chunk_nodes, b->h_node_off[i + 1]));
+        }
+    }
+    ico[b->n_inst] = (int)ci.size();
+    if (ci.empty()) {  // keep one empty chunk so launches are never empty
+        ci.push_back(0);
+        cb.push_back(0);
+        ce.push_back(0);
+    }
+    for (int* p : {b->d_chunk_inst, b->d_chunk_beg, b->d_chunk_end, b->d_inst_chunk_off})
+        if (p) (void)hipFree(p);
+    if (b->d_part) (void)hipFree(b->d_part);
+    b->d_chunk_inst = b->d_chunk_beg = b->d_chunk_end = b->d_inst_chunk_off = nullptr;
+    b->d_part = nullptr;
+    int rc;
+    if ((rc = dalloc(&b->d_chunk_inst, ci.size())) || (rc = dalloc(&b->d_chunk_beg, cb.size())) ||
+        (rc = dalloc(&b->d_chunk_end, ce.size())) || (rc = dalloc(&b->d_inst_chunk_off, ico.size())) ||
+        (rc = dalloc(&b->d_part, 2 * ci.size())))
+        return rc;
+    TLFEA_HIP_TRY(hipMemcpy(b->d_chunk_inst, ci.data(), sizeof(int) * ci.size(), hipMemcpyHostToDevice));
+    TLFEA_HIP_TRY(hipMemcpy(b->d_chunk_beg, cb.data(), sizeof(int) * cb.size(), hipMemcpyHostToDevice));
+    TLFEA_HIP_TRY(hipMemcpy(b->d_chunk_end, ce.data(), sizeof(int) * ce.size(), hipMemcpyHostToDevice));
+    TLFEA_HIP_TRY(hipMemcpy(b->d_inst_chunk_off, ico.data(), sizeof(int) * ico.size(), hipMemcpyHostToDevice));
+    if (b->d_chunk_list) (void)hipFree(b->d_chunk_list);
+    b->d_chunk_list = nullptr;
+    if ((rc = dalloc(&b->d_chunk_list, ci.size()))) return rc;
+    b->h_inst_chunk_off = ico;
+    b->compact = false;
+    b->chunk_nodes = chunk_nodes;
+    b->n_chunks = (int)ci.size();
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    return TLFEA_OK;
+}
+
+// FP64 batched operator q = K p (masked), chunk partials of p.q into d_part.
+int apply64(tlfea_batch* b, const double* p, double* q, const int* iflags, hipStream_t s) {
+    const TanMat<double> tm{0.0, 0.0, b->d_elem_inst, b->d_lam, b->d_mu, iflags, NI};
+    if (int rc = launch_tangent_partials<double>(b->plan, b->d_X, b->d_Y, b->d_Z, b->d_X, b->d_Y, b->d_Z, b->d_jinv,
+                                                 b->d_detJ, b->quad, tm, b->d_mask, p, nullptr, s,
+                                                 b->compact ? b->d_blk_list : nullptr, b->n_blk_list))
+        return rc;
+    k_bgather<double><<<n_chunk_launch(b), BT, 0, s>>>(chunks(b), b->plan->d_g_off, b->plan->d_g_idx,
+                                                       b->plan->d_part, b->d_mask, p, q, b->d_part, iflags);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
+// FP32 batched operator q = K p (masked), no dot partials.
+int apply32(tlfea_batch* b, const float* p, float* q, const int* iflags, hipStream_t s) {
+    const TanMat<float> tm{0.0f, 0.0f, b->d_elem_inst, b->d_lam32, b->d_mu32, iflags, NI};
+    if (int rc = launch_tangent_partials<float>(b->plan, b->d_X, b->d_Y, b->d_Z, b->d_X, b->d_Y, b->d_Z, b->d_jinv32,
+                                                b->d_detJ32, b->quad, tm, b->d_mask, p, nullptr, s,
+                                                b->compact ? b->d_blk_list : nullptr, b->n_blk_list))
+        return rc;
+    k_bgather<float><<<n_chunk_launch(b), BT, 0, s>>>(chunks(b), b->plan->d_g_off, b->plan->d_g_idx,
+                                                reinterpret_cast<const float*>(b->plan->d_part), b->d_mask, p, q,
+                                                nullptr, iflags);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
+// z = Chebyshev(r) for the active instances, then z.r / z.r_old chunk partials.
+int precondition(tlfea_batch* b, hipStream_t s) {
+    const Chunks ch = chunks(b);
+    k_cheb_init<<<n_chunk_launch(b), BT, 0, s>>>(ch, b->d_r, b->d_dinv32, b->d_cheb, b->d_i, b->d_r32, b->d_d32, b->d_z32);
+    for (int k = 1; k < b->cheb_degree; k++) {
+        if (int rc = apply32(b, b->d_d32, b->d_q32, b->d_i, s)) return rc;
+        k_cheb_step<<<n_chunk_launch(b), BT, 0, s>>>(ch, k, b->d_q32, b->d_dinv32, b->d_cheb, b->d_i, b->d_r32, b->d_d32,
+                                                b->d_z32);
+    }
+    k_cheb_fin<<<n_chunk_launch(b), BT, 0, s>>>(ch, b->d_z32, b->d_r, b->d_rold, b->d_i, b->d_z, b->d_part);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
+// One flexible preconditioned CG iteration for every active instance.
+int pcg_iteration(tlfea_batch* b, double* u, double tol, hipStream_t s) {
+    const Chunks ch = chunks(b);
+    if (int rc = apply64(b, b->d_p, b->d_q, b->d_i, s)) return rc;
+    k_breduce_alpha<<<1, BT, 0, s>>>(insts(b), b->d_inst_chunk_off, b->d_part, b->d_s, b->d_i);
+    k_bupdate_keep<<<n_chunk_launch(b), BT, 0, s>>>(ch, u, b->d_r, b->d_rold, b->d_p, b->d_q, b->d_s, b->d_i, b->d_part);
+    k_breduce_check<<<1, BT, 0, s>>>(insts(b), b->d_inst_chunk_off, b->d_part, b->d_s, b->d_i, b->d_all, tol);
+    if (int rc = precondition(b, s)) return rc;
+    k_breduce_fbeta<<<1, BT, 0, s>>>(insts(b), b->d_inst_chunk_off, b->d_part, b->d_s, b->d_i);
+    k_bdir<<<n_chunk_launch(b), BT, 0, s>>>(ch, b->d_p, b->d_z, b->d_s, b->d_i);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
+// One CG iteration for every active instance (6 launches, no host synchronisation).
+int cg_iteration(tlfea_batch* b, double* u, double tol, hipStream_t s) {
+    const Chunks ch = chunks(b);
+    if (int rc = apply64(b, b->d_p, b->d_q, b->d_i, s)) return rc;
+    k_breduce_alpha<<<1, BT, 0, s>>>(insts(b), b->d_inst_chunk_off, b->d_part, b->d_s, b->d_i);
+    k_bupdate<<<n_chunk_launch(b), BT, 0, s>>>(ch, u, b->d_r, b->d_p, b->d_q, b->d_s, b->d_i, b->d_part);
+    k_breduce_beta<<<1, BT, 0, s>>>(insts(b), b->d_inst_chunk_off, b->d_part, b->d_s, b->d_i, b->d_all, tol);
+    k_bdir<<<n_chunk_launch(b), BT, 0, s>>>(ch, b->d_p, b->d_r, b->d_s, b->d_i);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
+}  // namespace
+
+extern "C" int tlfea_batch_create(int nn, int n_inst, const int* elem_off, const int* node_off, const int* conn,
+                                  const double* x, const double* y, const double* z, const double* lambda,
+                                  const double* mu, const unsigned char* fixed_dof_mask, tlfea_batch** out,
+                                  void* stream) {
+    TLFEA_REQUIRE(out != nullptr, "null output handle");
+    *out = nullptr;
+    TLFEA_REQUIRE(nn == 10 || nn == 4, "nn must be 10 or 4");
+    TLFEA_REQUIRE(n_inst > 0 && elem_off && node_off && conn && x && y && z && lambda && mu && fixed_dof_mask,
+                  "bad argument");
+    TLFEA_REQUIRE(elem_off[0] == 0 && node_off[0] == 0, "offsets must start at 0");
+    for (int i = 0; i < n_inst; i++)
+        TLFEA_REQUIRE(elem_off[i + 1] > elem_off[i] && node_off[i + 1] > node_off[i],
+                      "every instance needs at least one element and one node");
+    const int n_elem = elem_off[n_inst], n_nodes = node_off[n_inst];
+    TLFEA_REQUIRE(n_elem > 0 && n_nodes > 0, "empty batch");
+    hipStream_t s = as_stream(stream);
+    auto* b = new tlfea_batch();
+    b->nn = nn;
+    b->n_inst = n_inst;
+    b->n_elem = n_elem;
+    b->n_nodes = n_nodes;
+    b->h_elem_off.assign(elem_off, elem_off + n_inst + 1);
+    b->h_node_off.assign(node_off, node_off + n_inst + 1);
+    int rc = tlfea_quadrature(nn, &b->quad);
+    const int nq = b->quad.n;
+    const int64_t pe = padded(n_elem);
+    const size_t nd = 3 * (size_t)n_nodes;
+    auto fail = [&](int code) {
+        free_batch(b);
+        return code;
+    };
+    if (rc) return fail(rc);
+    if ((rc = dalloc(&b->d_conn, (size_t)n_elem * nn)) || (rc = dalloc(&b->d_X, n_nodes)) ||
+        (rc = dalloc(&b->d_Y, n_nodes)) || (rc = dalloc(&b->d_Z, n_nodes)) || (rc = dalloc(&b->d_mask, nd)) ||
+        (rc = dalloc(&b->d_elem_inst, n_elem)) || (rc = dalloc(&b->d_jinv, (size_t)pe * nq * 9)) ||
+        (rc = dalloc(&b->d_detJ, (size_t)pe * nq)) || (rc = dalloc(&b->d_jinv32, (size_t)pe * nq * 9)) ||
+        (rc = dalloc(&b->d_detJ32, (size_t)pe * nq)) || (rc = dalloc(&b->d_lam, n_inst)) ||
+        (rc = dalloc(&b->d_mu, n_inst)) || (rc = dalloc(&b->d_lam32, n_inst)) || (rc = dalloc(&b->d_mu32, n_inst)) ||
+        (rc = dalloc(&b->d_r, nd)) || (rc = dalloc(&b->d_p, nd)) || (rc = dalloc(&b->d_q, nd)) ||
+        (rc = dalloc(&b->d_z, nd)) || (rc = dalloc(&b->d_r32, nd)) || (rc = dalloc(&b->d_z32, nd)) ||
+        (rc = dalloc(&b->d_d32, nd)) || (rc = dalloc(&b->d_q32, nd)) || (rc = dalloc(&b->d_s, (size_t)n_inst * NS)) ||
+        (rc = dalloc(&b->d_i, (size_t)n_inst * NI)) || (rc = dalloc(&b->d_all, 4)) ||
+        (rc = dalloc(&b->d_cheb, (size_t)n_inst * NC)) || (rc = dalloc(&b->d_rold, nd)) ||
+        (rc = dalloc(&b->d_dinv32, nd)))
+        return fail(rc);
+    if (hipMemcpyAsync(b->d_conn, conn, sizeof(int) * (size_t)n_elem * nn, hipMemcpyDeviceToDevice, s) ||
+        hipMemcpyAsync(b->d_X, x, sizeof(double) * n_nodes, hipMemcpyDeviceToDevice, s) ||
+        hipMemcpyAsync(b->d_Y, y, sizeof(double) * n_nodes, hipMemcpyDeviceToDevice, s) ||
+        hipMemcpyAsync(b->d_Z, z, sizeof(double) * n_nodes, hipMemcpyDeviceToDevice, s) ||
+        hipMemcpyAsync(b->d_mask, fixed_dof_mask, nd, hipMemcpyDeviceToDevice, s) ||
+        hipMemcpyAsync(b->d_lam, lambda, sizeof(double) * n_inst, hipMemcpyHostToDevice, s) ||
+        hipMemcpyAsync(b->d_mu, mu, sizeof(double) * n_inst, hipMemcpyHostToDevice, s)) {
+        set_error("batch input copy failed");
+        return fail(TLFEA_EHIP);
+    }
+    // element -> instance map (offsets staged through the r workspace)
+    int* d_eoff = reinterpret_cast<int*>(b->d_r);
+    if (hipMemcpyAsync(d_eoff, elem_off, sizeof(int) * (n_inst + 1), hipMemcpyHostToDevice, s)) {
+        set_error("batch offset copy failed");
+        return fail(TLFEA_EHIP);
+    }
+    k_elem_inst<<<n_inst, 256, 0, s>>>(n_inst, d_eoff, b->d_elem_inst);
+    if ((rc = launch_precompute(nn, n_elem, b->d_conn, b->d_X, b->d_Y, b->d_Z, b->quad, nullptr, b->d_detJ, b->d_jinv,
+                                s)))
+        return fail(rc);
+    k_d2f<<<grid_for((int64_t)pe * nq * 9, 256), 256, 0, s>>>((int64_t)pe * nq * 9, b->d_jinv, b->d_jinv32);
+    k_d2f<<<grid_for((int64_t)pe * nq, 256), 256, 0, s>>>((int64_t)pe * nq, b->d_detJ, b->d_detJ32);
+    k_d2f<<<grid_for(n_inst, 256), 256, 0, s>>>(n_inst, b->d_lam, b->d_lam32);
+    k_d2f<<<grid_for(n_inst, 256), 256, 0, s>>>(n_inst, b->d_mu, b->d_mu32);
+    if (hipGetLastError() != hipSuccess) {
+        set_error("batch setup kernel launch failed");
+        return fail(TLFEA_EHIP);
+    }
+    if ((rc = tlfea_force_plan_create(nn, n_elem, n_nodes, b->d_conn, &b->plan, stream))) return fail(rc);
+    if ((rc = dalloc(&b->d_blk_list, b->plan->n_blocks)) || (rc = dalloc(&b->d_inst_list, n_inst))) return fail(rc);
+    b->launched_elems = n_elem;
+    if ((rc = build_chunks(b, 1024))) return fail(rc);
+    if (hipStreamCreateWithFlags(&b->s_own, hipStreamNonBlocking) != hipSuccess) {
+        set_error("stream creation failed");
+        return fail(TLFEA_EHIP);
+    }
+    TLFEA_HIP_TRY(hipStreamSynchronize(s));
+    *out = b;
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_batch_destroy(tlfea_batch* b) {
+    free_batch(b);
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_batch_sizes(const tlfea_batch* b, int* n_inst, int* n_elem, int* n_nodes, int* n_chunks) {
+    TLFEA_REQUIRE(b != nullptr, "null batch");
+    if (n_inst) *n_inst = b->n_inst;
+    if (n_elem) *n_elem = b->n_elem;
+    if (n_nodes) *n_nodes = b->n_nodes;
+    if (n_chunks) *n_chunks = b->n_chunks;
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_batch_set_chunk(tlfea_batch* b, int chunk_nodes) {
+    TLFEA_REQUIRE(b != nullptr, "null batch");
+    TLFEA_REQUIRE(chunk_nodes >= 64 && chunk_nodes <= (1 << 20), "chunk size out of range [64, 2^20]");
+    return build_chunks(b, chunk_nodes);
+}
+
+extern "C" int tlfea_batch_precond_setup(tlfea_batch* b, int degree, double ratio, int power_iters,
+                                         double safety, double* lambda_max, void* stream) {
+    TLFEA_REQUIRE(b != nullptr, "null batch");
+    TLFEA_REQUIRE(degree >= 0 && degree <= 16, "Chebyshev degree out of range [0, 16]");
+    TLFEA_REQUIRE(degree == 0 || (ratio > 1.0 && power_iters >= 1 && safety >= 1.0), "bad Chebyshev parameters");
+    b->cheb_degree = 0;
+    drop_graph(b);
+    full_lists(b);
+    if (degree == 0) return TLFEA_OK;
+    hipStream_t s = as_stream(stream);
+    const Chunks ch = chunks(b);
+    const int64_t nd = 3 * (int64_t)b->n_nodes;
+    // diagonal of K at the rest configuration: partials, then the gather with p = 1
+    const TanMat<double> tm{0.0, 0.0, b->d_elem_inst, b->d_lam, b->d_mu, nullptr, NI};
+    if (int rc = launch_tangent_diag_partials(b->plan, b->d_X, b->d_Y, b->d_Z, b->d_X, b->d_Y, b->d_Z, b->d_jinv,
+                                              b->d_detJ, b->quad, tm, s))
+        return rc;
+    k_fill<<<grid_for(nd, 256), 256, 0, s>>>(nd, 1.0, b->d_z);
+    k_bgather<double><<<b->n_chunks, BT, 0, s>>>(ch, b->plan->d_g_off, b->plan->d_g_idx, b->plan->d_part, b->d_mask,
+                                                 b->d_z, b->d_q, nullptr, nullptr);
+    k_inv_diag<<<grid_for(nd, 256), 256, 0, s>>>(nd, b->d_q, b->d_dinv32);
+    // power iteration on D^-1 K
+    k_pow_start<<<b->n_chunks, BT, 0, s>>>(ch, b->d_mask, b->d_d32);
+    for (int it = 0; it < power_iters; it++) {
+        if (int rc = apply32(b, b->d_d32, b->d_q32, nullptr, s)) return rc;
+        k_pow_scale<<<b->n_chunks, BT, 0, s>>>(ch, b->d_q32, b->d_dinv32, b->d_mask, b->d_z32, b->d_part);
+        k_pow_reduce<<<1, BT, 0, s>>>(b->n_inst, b->d_inst_chunk_off, b->d_part, b->d_cheb);
+        k_pow_norm<<<b->n_chunks, BT, 0, s>>>(ch, b->d_z32, b->d_cheb, b->d_d32);
+    }
+    k_cheb_params<<<grid_for(b->n_inst, 256), 256, 0, s>>>(b->n_inst, safety, ratio, b->d_cheb);
+    TLFEA_LAUNCH_CHECK();
+    if (lambda_max) {
+        std::vector<double> C((size_t)b->n_inst * NC);
+        TLFEA_HIP_TRY(hipMemcpyAsync(C.data(), b->d_cheb, sizeof(double) * C.size(), hipMemcpyDeviceToHost, s));
+        TLFEA_HIP_TRY(hipStreamSynchronize(s));
+        for (int i = 0; i < b->n_inst; i++) lambda_max[i] = C[(size_t)i * NC + C_LMAX];
+    }
+    TLFEA_HIP_TRY(hipStreamSynchronize(s));
+    b->cheb_degree = degree;
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_batch_tangent_apply(tlfea_batch* b, const double* p, double* out, void* stream) {
+    TLFEA_REQUIRE(b && p && out, "null argument");
+    full_lists(b);
+    return apply64(b, p, out, nullptr, as_stream(stream));
+}
+
+extern "C" int tlfea_batch_set_compaction(tlfea_batch* b, double ratio) {
+    TLFEA_REQUIRE(b != nullptr, "null batch");
+    TLFEA_REQUIRE(ratio >= 0.0 && ratio <= 1.0, "compaction ratio must be in [0, 1]");
+    b->compact_ratio = ratio;
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_batch_cg_solve(tlfea_batch* b, const double* f, double* u, int use_guess, double tol,
+                                    int max_iter, int poll, int use_graph, int precond, int* iters, double* rel_res,
+                                    int* status, int* total_iters, void* stream) {
+    TLFEA_REQUIRE(b && f && u, "null argument");
+    TLFEA_REQUIRE(precond == 0 || b->cheb_degree > 0, "preconditioner requested but not set up");
+    TLFEA_REQUIRE(max_iter >= 0 && tol >= 0.0, "bad tolerance or iteration limit");
+    if (poll <= 0) poll = 16;
+    hipStream_t caller = as_stream(stream);
+    hipStream_t s = b->s_own;
+    const double compact_ratio = b->compact_ratio;
+    // order the solve after the caller's pending work
+    TLFEA_HIP_TRY(hipStreamSynchronize(caller));
+    full_lists(b);
+    const Chunks ch = chunks(b);
+    const size_t nd = 3 * (size_t)b->n_nodes;
+    if (!use_guess) TLFEA_HIP_TRY(hipMemsetAsync(u, 0, sizeof(double) * nd, s));
+    // r = f - K u0
+    if (int rc = apply64(b, u, b->d_q, nullptr, s)) return rc;
+    k_binit<<<b->n_chunks, BT, 0, s>>>(ch, f, b->d_q, b->d_mask, b->d_r, b->d_part);
+    k_breduce_init<<<1, BT, 0, s>>>(b->n_inst, b->d_inst_chunk_off, b->d_part, b->d_s, b->d_i, b->d_all, tol);
+    k_bzero_u<<<b->n_chunks, BT, 0, s>>>(ch, b->d_i, u);
+    if (precond) {
+        // z0 = M^-1 r0, rz = r0.z0, p = z0 (r_old = r0 makes the z.r_old partial harmless)
+        TLFEA_HIP_TRY(hipMemcpyAsync(b->d_rold, b->d_r, sizeof(double) * nd, hipMemcpyDeviceToDevice, s));
+        if (int rc = precondition(b, s)) return rc;
+        k_breduce_rz0<<<1, BT, 0, s>>>(insts(b), b->d_inst_chunk_off, b->d_part, b->d_s, b->d_i);
+        k_bdir<<<n_chunk_launch(b), BT, 0, s>>>(ch, b->d_p, b->d_z, b->d_s, b->d_i);
+    } else {
+        TLFEA_HIP_TRY(hipMemcpyAsync(b->d_p, b->d_r, sizeof(double) * nd, hipMemcpyDeviceToDevice, s));
+    }
+    TLFEA_LAUNCH_CHECK();
+    int active = 0;
+    TLFEA_HIP_TRY(hipMemcpyAsync(&active, b->d_all, sizeof(int), hipMemcpyDeviceToHost, s));
+    TLFEA_HIP_TRY(hipStreamSynchronize(s));
+    int it = 0;
+    std::vector<int> hflags((size_t)b->n_inst * NI), done(b->n_inst, 0);
+    auto iteration = [&](hipStream_t st) { return precond ? pcg_iteration(b, u, tol, st) : cg_iteration(b, u, tol, st); };
+    if (use_graph && (b->graph == nullptr || b->graph_iters != poll || b->graph_precond != precond)) {
+        if (b->graph) (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    while (active > 0 && it < max_iter) {
+        const int n = std::min(poll, max_iter - it);
+        if (use_graph && n == poll) {
+            if (!b->graph) {
+                hipGraph_t g;
+                TLFEA_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                int rc = TLFEA_OK;
+                for (int k = 0; k < poll && rc == TLFEA_OK; k++) rc = iteration(s);
+                const hipError_t ec = hipStreamEndCapture(s, &g);
+                if (rc) return rc;
+                if (ec != hipSuccess) {
+                    set_error("graph capture failed: %s", hipGetErrorString(ec));
+                    return TLFEA_EHIP;
+                }
+                const hipError_t ei = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (ei != hipSuccess) {
+                    set_error("graph instantiation failed: %s", hipGetErrorString(ei));
+                    b->graph = nullptr;
+                    return TLFEA_EHIP;
+                }
+                b->graph_iters = poll;
+                b->graph_precond = precond;
+            }
+            TLFEA_HIP_TRY(hipGraphLaunch(b->graph, s));
+        } else {
+            for (int k = 0; k < n; k++)
+                if (int rc = iteration(s)) return rc;
+        }
+        it += n;
+        TLFEA_HIP_TRY(hipMemcpyAsync(&active, b->d_all, sizeof(int), hipMemcpyDeviceToHost, s));
+        TLFEA_HIP_TRY(hipStreamSynchronize(s));
+        if (active > 0 && compact_ratio > 0.0) {
+            // drop finished instances from the launches once they are a large enough share
+            TLFEA_HIP_TRY(hipMemcpy(hflags.data(), b->d_i, sizeof(int) * hflags.size(), hipMemcpyDeviceToHost));
+            int64_t act_elems = 0;
+            for (int i = 0; i < b->n_inst; i++) {
+                done[i] = hflags[(size_t)i * NI + I_DONE];
+                if (!done[i]) act_elems += b->h_elem_off[i + 1] - b->h_elem_off[i];
+            }
+            if ((double)act_elems < compact_ratio * (double)b->launched_elems)
+                if (int rc = compact_lists(b, done)) return rc;
+        }
+    }
+    full_lists(b);
+    // the graph bakes in u: a later solve with another output buffer must re-capture
+    if (use_graph && b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    std::vector<double> S((size_t)b->n_inst * NS);
+    std::vector<int> I((size_t)b->n_inst * NI);
+    TLFEA_HIP_TRY(hipMemcpy(S.data(), b->d_s, sizeof(double) * S.size(), hipMemcpyDeviceToHost));
+    TLFEA_HIP_TRY(hipMemcpy(I.data(), b->d_i, sizeof(int) * I.size(), hipMemcpyDeviceToHost));
+    int bad = 0;
+    for (int i = 0; i < b->n_inst; i++) {
+        if (iters) iters[i] = I[(size_t)i * NI + I_ITERS];
+        if (rel_res) rel_res[i] = S[(size_t)i * NS + S_REL];
+        int st = TLFEA_OK;
+        if (I[(size_t)i * NI + I_NOTPD])
+            st = TLFEA_ENOTPD;
+        else if (!I[(size_t)i * NI + I_DONE])
+            st = TLFEA_ENOCONV;
+        if (status) status[i] = st;
+        bad += st != TLFEA_OK;
+    }
+    if (total_iters) *total_iters = it;
+    if (bad) {
+        set_error("%d of %d instances did not converge", bad, b->n_inst);
+        return TLFEA_ENOCONV;
+    }
+    return TLFEA_OK;
+}

@@ -551,23 +551,28 @@ extern "C" int tlfea_quadrature(int nn, tlfea_quad* out) {
     return TLFEA_OK;
 }
 
+int tlfea_hip::launch_precompute(int nn, int n_elem, const int* conn, const double* x, const double* y,
+                                 const double* z, const tlfea_quad& quad, double* grad_t, double* detJ_t,
+                                 double* jinv_t, hipStream_t s) {
+    const int n_tiles = (int)(padded(n_elem) / TILE);
+    const int64_t threads = (int64_t)n_tiles * quad.n * TILE;
+    if (nn == 10)
+        k_precompute<10, 5><<<grid_for(threads, 256), 256, 0, s>>>(n_elem, n_tiles, conn, x, y, z, quad, grad_t,
+                                                                   detJ_t, jinv_t);
+    else
+        k_precompute<4, 1><<<grid_for(threads, 256), 256, 0, s>>>(n_elem, n_tiles, conn, x, y, z, quad, grad_t,
+                                                                  detJ_t, jinv_t);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
 extern "C" int tlfea_precompute_grad(int nn, int n_elem, const int* conn, const double* x, const double* y,
                                      const double* z, const tlfea_quad* quad, double* grad_t, double* detJ_t,
                                      void* stream) {
     if (int rc = check_elem(nn, n_elem, quad)) return rc;
     if (n_elem == 0) return TLFEA_OK;
     TLFEA_REQUIRE(conn && x && y && z && grad_t && detJ_t, "null pointer argument");
-    const int n_tiles = (int)(padded(n_elem) / TILE);
-    const int nq = quad->n;
-    const int64_t threads = (int64_t)n_tiles * nq * TILE;
-    if (nn == 10)
-        k_precompute<10, 5><<<grid_for(threads, 256), 256, 0, as_stream(stream)>>>(n_elem, n_tiles, conn, x, y, z,
-                                                                                   *quad, grad_t, detJ_t, nullptr);
-    else
-        k_precompute<4, 1><<<grid_for(threads, 256), 256, 0, as_stream(stream)>>>(n_elem, n_tiles, conn, x, y, z,
-                                                                                  *quad, grad_t, detJ_t, nullptr);
-    TLFEA_LAUNCH_CHECK();
-    return TLFEA_OK;
+    return launch_precompute(nn, n_elem, conn, x, y, z, *quad, grad_t, detJ_t, nullptr, as_stream(stream));
 }
 
 extern "C" int tlfea_precompute_jinv(int nn, int n_elem, const int* conn, const double* x, const double* y,
@@ -575,16 +580,7 @@ extern "C" int tlfea_precompute_jinv(int nn, int n_elem, const int* conn, const 
     if (int rc = check_elem(nn, n_elem, quad)) return rc;
     if (n_elem == 0) return TLFEA_OK;
     TLFEA_REQUIRE(conn && x && y && z && jinv_t, "null pointer argument");
-    const int n_tiles = (int)(padded(n_elem) / TILE);
-    const int64_t threads = (int64_t)n_tiles * quad->n * TILE;
-    if (nn == 10)
-        k_precompute<10, 5><<<grid_for(threads, 256), 256, 0, as_stream(stream)>>>(n_elem, n_tiles, conn, x, y, z,
-                                                                                   *quad, nullptr, nullptr, jinv_t);
-    else
-        k_precompute<4, 1><<<grid_for(threads, 256), 256, 0, as_stream(stream)>>>(n_elem, n_tiles, conn, x, y, z,
-                                                                                  *quad, nullptr, nullptr, jinv_t);
-    TLFEA_LAUNCH_CHECK();
-    return TLFEA_OK;
+    return launch_precompute(nn, n_elem, conn, x, y, z, *quad, nullptr, nullptr, jinv_t, as_stream(stream));
 }
 
 extern "C" int tlfea_calc_p(int nn, int n_elem, const int* conn, const double* x, const double* y, const double* z,

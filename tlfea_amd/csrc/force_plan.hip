@@ -18,7 +18,6 @@
 
 namespace tlfea_hip {
 
-constexpr int PLAN_BLOCK = 256;
 
 template <int NN, int NQ, bool DAMP>
 __global__ __launch_bounds__(PLAN_BLOCK) void k_force_planned(
@@ -175,240 +174,6 @@ int launch_force_planned(const tlfea_force_plan* p, const double* x, const doubl
 }
 
 
-// ------------------------------------------------------------------------------------
-// Matrix-free tangent action: block partials of y_a = sum_q dP(p) g_a dV.
-//
-// The shape-function gradients are not read: per quadrature point the kernel reads
-// M = J^-T (9 values) and rebuilds g_a = M dN_a/dxi from the dN/dxi table (kernel
-// argument, the same for every element).  With Ju = sum_a u_a (x) dN_a/dxi (u = x - X)
-// and Jp likewise:  F = I + Ju M^T,  dF = Jp M^T,  and the output
-// y_a = sum_q Q_q dN_a/dxi(q) with Q_q = dV dP M.
-// ------------------------------------------------------------------------------------
-template <int NN, int NQ>
-__global__ __launch_bounds__(PLAN_BLOCK) void k_tangent_planned(
-    int n_elem, const int64_t* __restrict__ blk_off, const int* __restrict__ unodes,
-    const unsigned short* __restrict__ slot_t, const double* __restrict__ x, const double* __restrict__ y,
-    const double* __restrict__ z, const double* __restrict__ X0, const double* __restrict__ Y0,
-    const double* __restrict__ Z0, const double* __restrict__ p, const unsigned char* __restrict__ mask,
-    const double* __restrict__ jinv_t, const double* __restrict__ detJ_t, DnTable dn, tlfea_quad quad,
-    tlfea_material mat, double* __restrict__ part, int max_u, const int* __restrict__ flags) {
-    if (flags && flags[0]) return;
-    extern __shared__ double lds[];
-    double* sU = lds;              // [3][max_u] displacements
-    double* sF = lds + 3 * max_u;  // [3][max_u] accumulated output
-    double* sP = lds + 6 * max_u;  // [3][max_u] direction (masked)
-    const int b = blockIdx.x;
-    const int64_t u0 = blk_off[b];
-    const int U = (int)(blk_off[b + 1] - u0);
-    const int e = b * PLAN_BLOCK + threadIdx.x;
-    const bool active = e < n_elem;
-    const int tile = e >> 6, lane = e & (TILE - 1);
-    // slots and the first quadrature point's M / detJ are requested before the staging
-    // loop so that their latency overlaps it
-    int sl[NN];
-    double Mn[9], dJn = 0.0;
-    if (active) {
-#pragma unroll
-        for (int a = 0; a < NN; a++) sl[a] = slot_t[((int64_t)tile * NN + a) * TILE + lane];
-        const double* Mp = jinv_t + (int64_t)tile * NQ * (9 * TILE) + lane;
-#pragma unroll
-        for (int c = 0; c < 9; c++) Mn[c] = Mp[c * TILE];
-        dJn = detJ_t[(int64_t)tile * NQ * TILE + lane];
-    }
-    for (int u = threadIdx.x; u < U; u += PLAN_BLOCK) {
-        const int n = unodes[u0 + u];
-        sU[u] = x[n] - X0[n];
-        sU[max_u + u] = y[n] - Y0[n];
-        sU[2 * max_u + u] = z[n] - Z0[n];
-        sF[u] = sF[max_u + u] = sF[2 * max_u + u] = 0.0;
-#pragma unroll
-        for (int d = 0; d < 3; d++) sP[d * max_u + u] = (mask && mask[3 * n + d]) ? 0.0 : p[3 * n + d];
-    }
-    __syncthreads();
-    if (active) {
-        double fa[NN][3];
-#pragma unroll
-        for (int a = 0; a < NN; a++) fa[a][0] = fa[a][1] = fa[a][2] = 0.0;
-        // M and detJ of the next quadrature point are loaded one iteration ahead
-#pragma unroll 1
-        for (int q = 0; q < NQ; q++) {
-            const int64_t tq = (int64_t)tile * NQ + q;
-            double M[3][3];
-#pragma unroll
-            for (int c = 0; c < 9; c++) M[c / 3][c % 3] = Mn[c];
-            const double dV = dJn * quad.w[q];
-            if (q + 1 < NQ) {
-                const double* Mp = jinv_t + (tq + 1) * (9 * TILE) + lane;
-#pragma unroll
-                for (int c = 0; c < 9; c++) Mn[c] = Mp[c * TILE];
-                dJn = detJ_t[(tq + 1) * TILE + lane];
-            }
-            double Ju[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, Jp[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-#pragma unroll
-            for (int a = 0; a < NN; a++) {
-                const int s = sl[a];
-                const double ua[3] = {sU[s], sU[max_u + s], sU[2 * max_u + s]};
-                const double pa[3] = {sP[s], sP[max_u + s], sP[2 * max_u + s]};
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    const double c = dn.d[q][a][k];
-#pragma unroll
-                    for (int i = 0; i < 3; i++) {
-                        Ju[i][k] += ua[i] * c;
-                        Jp[i][k] += pa[i] * c;
-                    }
-                }
-            }
-            double F[3][3], dF[3][3];
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    F[i][j] = (i == j ? 1.0 : 0.0) + Ju[i][0] * M[j][0] + Ju[i][1] * M[j][1] + Ju[i][2] * M[j][2];
-                    dF[i][j] = Jp[i][0] * M[j][0] + Jp[i][1] * M[j][1] + Jp[i][2] * M[j][2];
-                }
-            // E = (F^T F - I)/2 from H = F - I without cancellation: (H + H^T + H^T H)/2
-            double Em[3][3], dE[3][3];
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    const double hij = F[i][j] - (i == j ? 1.0 : 0.0), hji = F[j][i] - (i == j ? 1.0 : 0.0);
-                    double hh = 0.0;
-#pragma unroll
-                    for (int r = 0; r < 3; r++) hh += (F[r][i] - (r == i ? 1.0 : 0.0)) * (F[r][j] - (r == j ? 1.0 : 0.0));
-                    Em[i][j] = 0.5 * (hij + hji + hh);
-                    dE[i][j] = 0.5 * (dF[0][i] * F[0][j] + dF[1][i] * F[1][j] + dF[2][i] * F[2][j] +
-                                      F[0][i] * dF[0][j] + F[1][i] * dF[1][j] + F[2][i] * dF[2][j]);
-                }
-            const double trE = Em[0][0] + Em[1][1] + Em[2][2], trdE = dE[0][0] + dE[1][1] + dE[2][2];
-            double S[3][3], dS[3][3];
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    S[i][j] = 2.0 * mat.mu * Em[i][j] + (i == j ? mat.lambda * trE : 0.0);
-                    dS[i][j] = 2.0 * mat.mu * dE[i][j] + (i == j ? mat.lambda * trdE : 0.0);
-                }
-            // Q = dV (dF S + F dS) M
-            double dP[3][3];
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int j = 0; j < 3; j++)
-                    dP[i][j] = (dF[i][0] * S[0][j] + dF[i][1] * S[1][j] + dF[i][2] * S[2][j] + F[i][0] * dS[0][j] +
-                                F[i][1] * dS[1][j] + F[i][2] * dS[2][j]) *
-                               dV;
-            double Q[3][3];
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int k = 0; k < 3; k++) Q[i][k] = dP[i][0] * M[0][k] + dP[i][1] * M[1][k] + dP[i][2] * M[2][k];
-#pragma unroll
-            for (int a = 0; a < NN; a++) {
-                const double c0 = dn.d[q][a][0], c1 = dn.d[q][a][1], c2 = dn.d[q][a][2];
-#pragma unroll
-                for (int i = 0; i < 3; i++) fa[a][i] += Q[i][0] * c0 + Q[i][1] * c1 + Q[i][2] * c2;
-            }
-        }
-#pragma unroll
-        for (int a = 0; a < NN; a++)
-#pragma unroll
-            for (int i = 0; i < 3; i++) atomicAdd(&sF[i * max_u + sl[a]], fa[a][i]);
-    }
-    __syncthreads();
-    for (int u = threadIdx.x; u < U; u += PLAN_BLOCK) {
-        double* o = part + 3 * (u0 + u);
-        o[0] = sF[u];
-        o[1] = sF[max_u + u];
-        o[2] = sF[2 * max_u + u];
-    }
-}
-
-// out[3i+d] = sum of node i's partials, or p for masked DOFs; optional block partials of p.out
-__global__ __launch_bounds__(256) void k_gather_tangent(int n_nodes, const int* __restrict__ g_off,
-                                                        const int64_t* __restrict__ g_idx,
-                                                        const double* __restrict__ part,
-                                                        const unsigned char* __restrict__ mask,
-                                                        const double* __restrict__ p, double* __restrict__ out,
-                                                        double* __restrict__ dot_part, const int* __restrict__ flags) {
-    if (flags && flags[0]) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double dotv = 0.0;
-    if (i < n_nodes) {
-        double s[3] = {0.0, 0.0, 0.0};
-        for (int k = g_off[i]; k < g_off[i + 1]; k++) {
-            const double* pp = part + 3 * g_idx[k];
-            s[0] += pp[0];
-            s[1] += pp[1];
-            s[2] += pp[2];
-        }
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            const double v = (mask && mask[3 * i + d]) ? p[3 * i + d] : s[d];
-            out[3 * i + d] = v;
-            dotv += p[3 * i + d] * v;
-        }
-    }
-    if (dot_part) {
-        __shared__ double red[4];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) dotv += __shfl_xor(dotv, o, 64);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dotv;
-        __syncthreads();
-        if (threadIdx.x == 0) dot_part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
-    }
-}
-
-// dN/dxi at the quadrature points of `quad`: TET10 corners (4 L_i - 1) grad L_i, edge k
-// (i,j) of {(0,1),(1,2),(0,2),(0,3),(1,3),(2,3)}: 4 (L_i grad L_j + L_j grad L_i); TET4
-// the constant gradients of the barycentric coordinates (FEAT10Data.cu:92-198).
-DnTable make_dn_table(int nn, const tlfea_quad& quad) {
-    DnTable t;
-    std::memset(&t, 0, sizeof(t));
-    const double dL[4][3] = {{-1, -1, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-    for (int q = 0; q < quad.n && q < 5; q++) {
-        const double L[4] = {1.0 - quad.x[q] - quad.y[q] - quad.z[q], quad.x[q], quad.y[q], quad.z[q]};
-        if (nn == 4) {
-            for (int a = 0; a < 4; a++)
-                for (int k = 0; k < 3; k++) t.d[q][a][k] = dL[a][k];
-            continue;
-        }
-        for (int a = 0; a < 4; a++)
-            for (int k = 0; k < 3; k++) t.d[q][a][k] = (4.0 * L[a] - 1.0) * dL[a][k];
-        const int E[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
-        for (int m = 0; m < 6; m++) {
-            const int i = E[m][0], j = E[m][1];
-            for (int k = 0; k < 3; k++) t.d[q][4 + m][k] = 4.0 * (L[i] * dL[j][k] + L[j] * dL[i][k]);
-        }
-    }
-    return t;
-}
-
-int launch_tangent_planned(const tlfea_force_plan* pl, const double* x, const double* y, const double* z,
-                           const double* X0, const double* Y0, const double* Z0, const double* jinv_t,
-                           const double* detJ_t, const tlfea_quad& quad, const tlfea_material& mat,
-                           const unsigned char* mask, const double* p, double* out, double* dot_part,
-                           const int* flags, hipStream_t s) {
-    const size_t lds = sizeof(double) * (size_t)pl->max_u * 9;
-    const DnTable dn = make_dn_table(pl->nn, quad);
-    if (pl->n_blocks > 0) {
-        if (pl->nn == 10)
-            k_tangent_planned<10, 5><<<pl->n_blocks, PLAN_BLOCK, lds, s>>>(
-                pl->n_elem, pl->d_blk_off, pl->d_unodes, pl->d_slot, x, y, z, X0, Y0, Z0, p, mask, jinv_t, detJ_t, dn,
-                quad, mat, pl->d_part, pl->max_u, flags);
-        else
-            k_tangent_planned<4, 1><<<pl->n_blocks, PLAN_BLOCK, lds, s>>>(
-                pl->n_elem, pl->d_blk_off, pl->d_unodes, pl->d_slot, x, y, z, X0, Y0, Z0, p, mask, jinv_t, detJ_t, dn,
-                quad, mat, pl->d_part, pl->max_u, flags);
-        TLFEA_LAUNCH_CHECK();
-    }
-    k_gather_tangent<<<grid_for(pl->n_nodes, 256), 256, 0, s>>>(pl->n_nodes, pl->d_g_off, pl->d_g_idx, pl->d_part,
-                                                                mask, p, out, dot_part, flags);
-    TLFEA_LAUNCH_CHECK();
-    return TLFEA_OK;
-}
-
 __global__ void k_fixed_mask(int n_fixed, const int* __restrict__ fixed, unsigned char* __restrict__ mask) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_fixed) return;
@@ -543,19 +308,6 @@ extern "C" int tlfea_internal_force_planned(const tlfea_force_plan* plan, const 
     TLFEA_REQUIRE(quad->n == (plan->nn == 10 ? 5 : 1), "quadrature does not match the element");
     return launch_force_planned(plan, x, y, z, grad_t, detJ_t, *quad, *mat, v, f, accumulate, nullptr,
                                 as_stream(stream));
-}
-
-extern "C" int tlfea_tangent_apply_planned(const tlfea_force_plan* plan, const double* x, const double* y,
-                                           const double* z, const double* X0, const double* Y0, const double* Z0,
-                                           const double* jinv_t, const double* detJ_t, const tlfea_quad* quad,
-                                           const tlfea_material* mat, const unsigned char* fixed_dof_mask,
-                                           const double* p, double* out, void* stream) {
-    TLFEA_REQUIRE(plan && x && y && z && X0 && Y0 && Z0 && jinv_t && detJ_t && quad && mat && p && out,
-                  "null argument");
-    TLFEA_REQUIRE(quad->n == (plan->nn == 10 ? 5 : 1), "quadrature size does not match the element type");
-    TLFEA_REQUIRE(mat->eta_damp == 0.0 && mat->lambda_damp == 0.0, "the matrix-free tangent excludes damping");
-    return launch_tangent_planned(plan, x, y, z, X0, Y0, Z0, jinv_t, detJ_t, *quad, *mat, fixed_dof_mask, p, out,
-                                  nullptr, nullptr, as_stream(stream));
 }
 
 extern "C" int tlfea_fixed_dof_mask(int n_nodes, int n_fixed, const int* fixed_nodes, unsigned char* mask,

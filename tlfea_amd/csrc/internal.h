@@ -37,6 +37,8 @@ struct tlfea_force_plan {
 
 namespace tlfea_hip {
 
+constexpr int PLAN_BLOCK = 256;  // elements per force-plan block (one thread each)
+
 // Planned fused force (element.hip): partials then gather into f (accumulate or overwrite).
 int launch_force_planned(const tlfea_force_plan* p, const double* x, const double* y, const double* z,
                          const double* grad_t, const double* detJ_t, const tlfea_quad& quad,
@@ -44,19 +46,50 @@ int launch_force_planned(const tlfea_force_plan* p, const double* x, const doubl
                          hipStream_t s);
 
 // dN/dxi of every node at every quadrature point (a kernel argument).
+template <typename T>
 struct DnTable {
-    double d[5][10][3];
+    T d[5][10][3];
 };
-DnTable make_dn_table(int nn, const tlfea_quad& quad);
+DnTable<double> make_dn_table(int nn, const tlfea_quad& quad);
 
-// Matrix-free tangent action (force_plan.hip) at positions x (rest X0, inverse
-// Jacobians jinv_t); dot_part (optional) receives per-block partials of p . out (blocks of
-// 256 nodes).
+// Material of the tangent: one (lambda, mu) for all elements, or per instance through
+// elem_inst (then inst_done[inst * done_stride], when set, skips the elements of finished
+// instances).
+template <typename T>
+struct TanMat {
+    T lam, mu;
+    const int* elem_inst;
+    const T* inst_lam;
+    const T* inst_mu;
+    const int* inst_done;
+    int done_stride;
+};
+
+// Block partials of the matrix-free tangent (tangent.hip) into plan->d_part (read as T),
+// for all plan blocks or, with blk_list, for the n_list blocks it names.
+template <typename T>
+int launch_tangent_partials(const tlfea_force_plan* pl, const double* x, const double* y, const double* z,
+                            const double* X0, const double* Y0, const double* Z0, const T* jinv_t, const T* detJ_t,
+                            const tlfea_quad& quad, const TanMat<T>& mat, const unsigned char* mask, const T* p,
+                            const int* flags, hipStream_t s, const int* blk_list, int n_list);
+
+// Block partials of the tangent's diagonal (FP64), gathered like the tangent's.
+int launch_tangent_diag_partials(const tlfea_force_plan* pl, const double* x, const double* y, const double* z,
+                                 const double* X0, const double* Y0, const double* Z0, const double* jinv_t,
+                                 const double* detJ_t, const tlfea_quad& quad, const TanMat<double>& mat,
+                                 hipStream_t s);
+
+// Matrix-free tangent action (FP64, one body) at positions x (rest X0, inverse Jacobians
+// jinv_t); dot_part (optional) receives per-block partials of p . out (blocks of 256 nodes).
 int launch_tangent_planned(const tlfea_force_plan* pl, const double* x, const double* y, const double* z,
                            const double* X0, const double* Y0, const double* Z0, const double* jinv_t,
                            const double* detJ_t, const tlfea_quad& quad, const tlfea_material& mat,
                            const unsigned char* mask, const double* p, double* out, double* dot_part,
                            const int* flags, hipStream_t s);
+
+// Element precompute (element.hip): any of grad_t / detJ_t / jinv_t may be null.
+int launch_precompute(int nn, int n_elem, const int* conn, const double* x, const double* y, const double* z,
+                      const tlfea_quad& quad, double* grad_t, double* detJ_t, double* jinv_t, hipStream_t s);
 
 // Fused FP64 force launch (element.hip).  With `flags` non-null the kernel returns at once
 // when flags[0] | flags[1] is set (used inside the iterative solvers' queued iterations).

@@ -1,0 +1,477 @@
+// Matrix-free SVK tangent action y = K(x) p, FP64 or FP32, one body or a batch.
+//
+// The operator is the one compute_hessian_assemble_csr builds element by element and
+// SolveLinearSystemCG applies through its SpMV (src/elements/FEAT10DataFunc.cuh:512-768,
+// src/materials/SVK.cuh:41-67, src/solvers/LinearStaticSolver.cu:360-371), applied here
+// without assembly.  The shape-function gradients are not read: per quadrature point the
+// kernel reads M = J^-T (9 values) and rebuilds g_a = M dN_a/dxi from the dN/dxi table (a
+// kernel argument shared by all elements).  With Ju = sum_a u_a (x) dN_a/dxi (u = x - X)
+// and Jp likewise:
+//     F = I + Ju M^T,  dF = Jp M^T,  E = (H + H^T + H^T H)/2 with H = F - I,
+//     dE = (dF^T F + F^T dF)/2,  S = lambda tr(E) I + 2 mu E,  dS likewise from dE,
+//     dP = dF S + F dS,  y_a = sum_q Q_q dN_a/dxi(q)  with  Q_q = w_q detJ_q dP M.
+// Elements are processed in the force plan's 256-element blocks: the block's nodes are
+// staged in LDS, element outputs are summed there, and each block writes one partial per
+// node it touches; a gather kernel sums a node's partials in a fixed order.
+#include <cstring>
+
+#include "internal.h"
+
+namespace tlfea_hip {
+
+template <typename T>
+__device__ __forceinline__ T tan_lam(const TanMat<T>& m, int inst) {
+    return m.elem_inst ? m.inst_lam[inst] : m.lam;
+}
+template <typename T>
+__device__ __forceinline__ T tan_mu(const TanMat<T>& m, int inst) {
+    return m.elem_inst ? m.inst_mu[inst] : m.mu;
+}
+
+// dN/dxi of node a at quadrature point q of the standard rules (TET10: the 5-point rule of
+// tlfea_quadrature; TET4: constant), evaluated with the same operations as
+// make_dn_table so that the folded constants equal the table's entries.
+template <int NN>
+__host__ __device__ constexpr double std_dn(int q, int a, int k) {
+    const double dLk[4] = {-1.0, k == 0 ? 1.0 : 0.0, k == 1 ? 1.0 : 0.0, k == 2 ? 1.0 : 0.0};
+    if (NN == 4) return dLk[a];
+    const double s = 1.0 / 6.0;
+    const double px = q == 0 ? 0.25 : (q == 2 ? 0.5 : s);
+    const double py = q == 0 ? 0.25 : (q == 3 ? 0.5 : s);
+    const double pz = q == 0 ? 0.25 : (q == 4 ? 0.5 : s);
+    const double L[4] = {1.0 - px - py - pz, px, py, pz};
+    if (a < 4) return (4.0 * L[a] - 1.0) * dLk[a];
+    const int E[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
+    const int i = E[a - 4][0], j = E[a - 4][1];
+    return 4.0 * (L[i] * dLk[j] + L[j] * dLk[i]);
+}
+
+// One quadrature point of the tangent kernel: accumulates Q_q dN_a/dxi into fa.  With
+// STD the dN/dxi values are compile-time constants (q must then be a constant after
+// unrolling), otherwise they come from the kernel-argument table.
+template <int NN, int NQ, typename T, bool STD>
+__device__ __forceinline__ void tangent_qp(int q, int64_t tile, int lane, const int (&sl)[NN], const T* sU,
+                                           const T* sP, int max_u, const T* __restrict__ jinv_t,
+                                           const T* __restrict__ detJ_t, const DnTable<T>& dn, const tlfea_quad& quad,
+                                           T lam, T mu, T (&Mn)[9], T& dJn, T (&fa)[NN][3]) {
+    // keep the node values in LDS rather than registers across quadrature points (frees
+    // ~120 VGPRs in FP64, i.e. one more wave per SIMD)
+    __asm__ volatile("" ::: "memory");
+    const int64_t tq = tile * NQ + q;
+    T M[3][3];
+#pragma unroll
+    for (int c = 0; c < 9; c++) M[c / 3][c % 3] = Mn[c];
+    const T dV = dJn * (T)quad.w[q];
+    // M and detJ of the next quadrature point are loaded one iteration ahead
+    if (q + 1 < NQ) {
+        const T* Mp = jinv_t + (tq + 1) * (9 * TILE) + lane;
+#pragma unroll
+        for (int c = 0; c < 9; c++) Mn[c] = Mp[c * TILE];
+        dJn = detJ_t[(tq + 1) * TILE + lane];
+    }
+    T Ju[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, Jp[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+    for (int a = 0; a < NN; a++) {
+        const int s = sl[a];
+        const T ua[3] = {sU[s], sU[max_u + s], sU[2 * max_u + s]};
+        const T pa[3] = {sP[s], sP[max_u + s], sP[2 * max_u + s]};
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const T c = STD ? (T)std_dn<NN>(q, a, k) : dn.d[q][a][k];
+            if (STD && c == T(0)) continue;
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                Ju[i][k] += ua[i] * c;
+                Jp[i][k] += pa[i] * c;
+            }
+        }
+    }
+    // H = F - I and dF
+    T H[3][3], dF[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            H[i][j] = Ju[i][0] * M[j][0] + Ju[i][1] * M[j][1] + Ju[i][2] * M[j][2];
+            dF[i][j] = Jp[i][0] * M[j][0] + Jp[i][1] * M[j][1] + Jp[i][2] * M[j][2];
+        }
+    T Em[3][3], dE[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const T hh = H[0][i] * H[0][j] + H[1][i] * H[1][j] + H[2][i] * H[2][j];
+            Em[i][j] = T(0.5) * (H[i][j] + H[j][i] + hh);
+            // dF^T F + F^T dF with F = I + H
+            const T s2 = dF[i][j] + dF[j][i] + dF[0][i] * H[0][j] + dF[1][i] * H[1][j] + dF[2][i] * H[2][j] +
+                         H[0][i] * dF[0][j] + H[1][i] * dF[1][j] + H[2][i] * dF[2][j];
+            dE[i][j] = T(0.5) * s2;
+        }
+    const T trE = Em[0][0] + Em[1][1] + Em[2][2], trdE = dE[0][0] + dE[1][1] + dE[2][2];
+    T S[3][3], dS[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            S[i][j] = T(2) * mu * Em[i][j] + (i == j ? lam * trE : T(0));
+            dS[i][j] = T(2) * mu * dE[i][j] + (i == j ? lam * trdE : T(0));
+        }
+    // dP = dF S + (I + H) dS, scaled by dV
+    T dP[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            dP[i][j] = (dF[i][0] * S[0][j] + dF[i][1] * S[1][j] + dF[i][2] * S[2][j] + dS[i][j] + H[i][0] * dS[0][j] +
+                        H[i][1] * dS[1][j] + H[i][2] * dS[2][j]) *
+                       dV;
+    T Q[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) Q[i][k] = dP[i][0] * M[0][k] + dP[i][1] * M[1][k] + dP[i][2] * M[2][k];
+#pragma unroll
+    for (int a = 0; a < NN; a++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const T c = STD ? (T)std_dn<NN>(q, a, k) : dn.d[q][a][k];
+            if (STD && c == T(0)) continue;
+#pragma unroll
+            for (int i = 0; i < 3; i++) fa[a][i] += Q[i][k] * c;
+        }
+}
+
+template <int NN, int NQ, typename T, bool STD>
+__global__ __launch_bounds__(PLAN_BLOCK) void k_tangent(
+    int n_elem, const int64_t* __restrict__ blk_off, const int* __restrict__ unodes,
+    const unsigned short* __restrict__ slot_t, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const double* __restrict__ X0, const double* __restrict__ Y0,
+    const double* __restrict__ Z0, const T* __restrict__ p, const unsigned char* __restrict__ mask,
+    const T* __restrict__ jinv_t, const T* __restrict__ detJ_t, DnTable<T> dn, tlfea_quad quad, TanMat<T> mat,
+    T* __restrict__ part, int max_u, const int* __restrict__ flags, const int* __restrict__ blk_list) {
+    if (flags && flags[0]) return;
+    extern __shared__ unsigned char lds_raw[];
+    T* sU = reinterpret_cast<T*>(lds_raw);  // [3][max_u] displacements
+    T* sF = sU + 3 * max_u;                 // [3][max_u] accumulated output
+    T* sP = sU + 6 * max_u;                 // [3][max_u] direction (masked)
+    const int b = blk_list ? blk_list[blockIdx.x] : blockIdx.x;
+    const int64_t u0 = blk_off[b];
+    const int U = (int)(blk_off[b + 1] - u0);
+    const int e = b * PLAN_BLOCK + threadIdx.x;
+    int inst = 0;
+    bool active = e < n_elem;
+    if (active && mat.elem_inst) {
+        inst = mat.elem_inst[e];
+        if (mat.inst_done && mat.inst_done[inst * mat.done_stride]) active = false;
+    }
+    const int tile = e >> 6, lane = e & (TILE - 1);
+    // slots and the first quadrature point's M / detJ are requested before the staging
+    // loop so that their latency overlaps it
+    int sl[NN];
+    T Mn[9], dJn = T(0);
+    if (active) {
+#pragma unroll
+        for (int a = 0; a < NN; a++) sl[a] = slot_t[((int64_t)tile * NN + a) * TILE + lane];
+        const T* Mp = jinv_t + (int64_t)tile * NQ * (9 * TILE) + lane;
+#pragma unroll
+        for (int c = 0; c < 9; c++) Mn[c] = Mp[c * TILE];
+        dJn = detJ_t[(int64_t)tile * NQ * TILE + lane];
+    }
+    for (int u = threadIdx.x; u < U; u += PLAN_BLOCK) {
+        const int n = unodes[u0 + u];
+        sU[u] = (T)(x[n] - X0[n]);
+        sU[max_u + u] = (T)(y[n] - Y0[n]);
+        sU[2 * max_u + u] = (T)(z[n] - Z0[n]);
+        sF[u] = sF[max_u + u] = sF[2 * max_u + u] = T(0);
+#pragma unroll
+        for (int d = 0; d < 3; d++) sP[d * max_u + u] = (mask && mask[3 * n + d]) ? T(0) : p[3 * n + d];
+    }
+    __syncthreads();
+    if (active) {
+        const T lam = tan_lam(mat, inst), mu = tan_mu(mat, inst);
+        T fa[NN][3];
+#pragma unroll
+        for (int a = 0; a < NN; a++) fa[a][0] = fa[a][1] = fa[a][2] = T(0);
+        if constexpr (STD) {
+#pragma unroll
+            for (int q = 0; q < NQ; q++)
+                tangent_qp<NN, NQ, T, true>(q, tile, lane, sl, sU, sP, max_u, jinv_t, detJ_t, dn, quad, lam, mu, Mn,
+                                            dJn, fa);
+        } else {
+#pragma unroll 1
+            for (int q = 0; q < NQ; q++)
+                tangent_qp<NN, NQ, T, false>(q, tile, lane, sl, sU, sP, max_u, jinv_t, detJ_t, dn, quad, lam, mu, Mn,
+                                             dJn, fa);
+        }
+#pragma unroll
+        for (int a = 0; a < NN; a++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) atomicAdd(&sF[i * max_u + sl[a]], fa[a][i]);
+    }
+    __syncthreads();
+    for (int u = threadIdx.x; u < U; u += PLAN_BLOCK) {
+        T* o = part + 3 * (u0 + u);
+        o[0] = sF[u];
+        o[1] = sF[max_u + u];
+        o[2] = sF[2 * max_u + u];
+    }
+}
+
+// out[3i+d] = sum of node i's partials, or p for masked DOFs; optional block partials of p.out
+__global__ __launch_bounds__(256) void k_gather_tangent(int n_nodes, const int* __restrict__ g_off,
+                                                        const int64_t* __restrict__ g_idx,
+                                                        const double* __restrict__ part,
+                                                        const unsigned char* __restrict__ mask,
+                                                        const double* __restrict__ p, double* __restrict__ out,
+                                                        double* __restrict__ dot_part, const int* __restrict__ flags) {
+    if (flags && flags[0]) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double dotv = 0.0;
+    if (i < n_nodes) {
+        double s[3] = {0.0, 0.0, 0.0};
+        for (int k = g_off[i]; k < g_off[i + 1]; k++) {
+            const double* pp = part + 3 * g_idx[k];
+            s[0] += pp[0];
+            s[1] += pp[1];
+            s[2] += pp[2];
+        }
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double v = (mask && mask[3 * i + d]) ? p[3 * i + d] : s[d];
+            out[3 * i + d] = v;
+            dotv += p[3 * i + d] * v;
+        }
+    }
+    if (dot_part) {
+        __shared__ double red[4];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) dotv += __shfl_xor(dotv, o, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dotv;
+        __syncthreads();
+        if (threadIdx.x == 0) dot_part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+    }
+}
+
+// dN/dxi at the quadrature points of `quad`: TET10 corners (4 L_i - 1) grad L_i, edge k
+// (i,j) of {(0,1),(1,2),(0,2),(0,3),(1,3),(2,3)}: 4 (L_i grad L_j + L_j grad L_i); TET4
+// the constant gradients of the barycentric coordinates (FEAT10Data.cu:92-198).
+DnTable<double> make_dn_table(int nn, const tlfea_quad& quad) {
+    DnTable<double> t;
+    std::memset(&t, 0, sizeof(t));
+    const double dL[4][3] = {{-1, -1, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int q = 0; q < quad.n && q < 5; q++) {
+        const double L[4] = {1.0 - quad.x[q] - quad.y[q] - quad.z[q], quad.x[q], quad.y[q], quad.z[q]};
+        if (nn == 4) {
+            for (int a = 0; a < 4; a++)
+                for (int k = 0; k < 3; k++) t.d[q][a][k] = dL[a][k];
+            continue;
+        }
+        for (int a = 0; a < 4; a++)
+            for (int k = 0; k < 3; k++) t.d[q][a][k] = (4.0 * L[a] - 1.0) * dL[a][k];
+        const int E[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
+        for (int m = 0; m < 6; m++) {
+            const int i = E[m][0], j = E[m][1];
+            for (int k = 0; k < 3; k++) t.d[q][4 + m][k] = 4.0 * (L[i] * dL[j][k] + L[j] * dL[i][k]);
+        }
+    }
+    return t;
+}
+
+template <typename T>
+int launch_tangent_partials(const tlfea_force_plan* pl, const double* x, const double* y, const double* z,
+                            const double* X0, const double* Y0, const double* Z0, const T* jinv_t, const T* detJ_t,
+                            const tlfea_quad& quad, const TanMat<T>& mat, const unsigned char* mask, const T* p,
+                            const int* flags, hipStream_t s, const int* blk_list, int n_list) {
+    const int nb = blk_list ? n_list : pl->n_blocks;
+    if (nb == 0) return TLFEA_OK;
+    const DnTable<double> dn64 = make_dn_table(pl->nn, quad);
+    DnTable<T> dn;
+    for (int q = 0; q < 5; q++)
+        for (int a = 0; a < 10; a++)
+            for (int k = 0; k < 3; k++) dn.d[q][a][k] = (T)dn64.d[q][a][k];
+    const size_t lds = sizeof(T) * (size_t)pl->max_u * 9;
+    T* part = reinterpret_cast<T*>(pl->d_part);  // d_part holds 3*n_partials doubles
+    if (pl->nn == 10) {
+        // the constant-folded kernel when the rule is exactly the standard 5-point one
+        bool std_rule = quad.n == 5;
+        for (int q = 0; q < 5 && std_rule; q++)
+            for (int a = 0; a < 10; a++)
+                for (int k = 0; k < 3; k++) std_rule = std_rule && dn64.d[q][a][k] == std_dn<10>(q, a, k);
+        if (std_rule)
+            k_tangent<10, 5, T, true><<<nb, PLAN_BLOCK, lds, s>>>(pl->n_elem, pl->d_blk_off, pl->d_unodes, pl->d_slot,
+                                                                 x, y, z, X0, Y0, Z0, p, mask, jinv_t, detJ_t, dn,
+                                                                 quad, mat, part, pl->max_u, flags, blk_list);
+        else
+            k_tangent<10, 5, T, false><<<nb, PLAN_BLOCK, lds, s>>>(pl->n_elem, pl->d_blk_off, pl->d_unodes,
+                                                                  pl->d_slot, x, y, z, X0, Y0, Z0, p, mask, jinv_t,
+                                                                  detJ_t, dn, quad, mat, part, pl->max_u, flags,
+                                                                  blk_list);
+    } else {
+        k_tangent<4, 1, T, true><<<nb, PLAN_BLOCK, lds, s>>>(pl->n_elem, pl->d_blk_off, pl->d_unodes, pl->d_slot, x,
+                                                            y, z, X0, Y0, Z0, p, mask, jinv_t, detJ_t, dn, quad, mat,
+                                                            part, pl->max_u, flags, blk_list);
+    }
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
+template int launch_tangent_partials<double>(const tlfea_force_plan*, const double*, const double*, const double*,
+                                             const double*, const double*, const double*, const double*,
+                                             const double*, const tlfea_quad&, const TanMat<double>&,
+                                             const unsigned char*, const double*, const int*, hipStream_t,
+                                             const int*, int);
+template int launch_tangent_partials<float>(const tlfea_force_plan*, const double*, const double*, const double*,
+                                            const double*, const double*, const double*, const float*, const float*,
+                                            const tlfea_quad&, const TanMat<float>&, const unsigned char*,
+                                            const float*, const int*, hipStream_t, const int*, int);
+
+int launch_tangent_planned(const tlfea_force_plan* pl, const double* x, const double* y, const double* z,
+                           const double* X0, const double* Y0, const double* Z0, const double* jinv_t,
+                           const double* detJ_t, const tlfea_quad& quad, const tlfea_material& mat,
+                           const unsigned char* mask, const double* p, double* out, double* dot_part,
+                           const int* flags, hipStream_t s) {
+    const TanMat<double> tm{mat.lambda, mat.mu, nullptr, nullptr, nullptr, nullptr, 0};
+    if (int rc = launch_tangent_partials<double>(pl, x, y, z, X0, Y0, Z0, jinv_t, detJ_t, quad, tm, mask, p, flags,
+                                                 s, nullptr, 0))
+        return rc;
+    k_gather_tangent<<<grid_for(pl->n_nodes, 256), 256, 0, s>>>(pl->n_nodes, pl->d_g_off, pl->d_g_idx, pl->d_part,
+                                                                mask, p, out, dot_part, flags);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
+
+// ------------------------------------------------------------------------------------
+// Diagonal of the tangent, d[(a,i)] = K_(a,i),(a,i): the diagonal terms of the element
+// blocks of compute_hessian_assemble_csr (FEAT10DataFunc.cuh:512-768) at the positions x,
+//   dV [ (lambda + mu) (F g_a)_i^2 + |g_a|^2 (lambda tr E + mu ((F F^T)_ii - 1)) + mu |F g_a|^2 ],
+// written as block partials like the tangent (gathered by the same kernels).
+// ------------------------------------------------------------------------------------
+template <int NN, int NQ>
+__global__ __launch_bounds__(PLAN_BLOCK) void k_tangent_diag(
+    int n_elem, const int64_t* __restrict__ blk_off, const int* __restrict__ unodes,
+    const unsigned short* __restrict__ slot_t, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const double* __restrict__ X0, const double* __restrict__ Y0,
+    const double* __restrict__ Z0, const double* __restrict__ jinv_t, const double* __restrict__ detJ_t,
+    DnTable<double> dn, tlfea_quad quad, TanMat<double> mat, double* __restrict__ part, int max_u) {
+    extern __shared__ unsigned char lds_raw[];
+    double* sU = reinterpret_cast<double*>(lds_raw);  // [3][max_u]
+    double* sF = sU + 3 * max_u;                      // [3][max_u]
+    const int b = blockIdx.x;
+    const int64_t u0 = blk_off[b];
+    const int U = (int)(blk_off[b + 1] - u0);
+    for (int u = threadIdx.x; u < U; u += PLAN_BLOCK) {
+        const int n = unodes[u0 + u];
+        sU[u] = x[n] - X0[n];
+        sU[max_u + u] = y[n] - Y0[n];
+        sU[2 * max_u + u] = z[n] - Z0[n];
+        sF[u] = sF[max_u + u] = sF[2 * max_u + u] = 0.0;
+    }
+    __syncthreads();
+    const int e = b * PLAN_BLOCK + threadIdx.x;
+    if (e < n_elem) {
+        const int inst = mat.elem_inst ? mat.elem_inst[e] : 0;
+        const double lam = tan_lam(mat, inst), mu = tan_mu(mat, inst);
+        const int tile = e >> 6, lane = e & (TILE - 1);
+        int sl[NN];
+#pragma unroll
+        for (int a = 0; a < NN; a++) sl[a] = slot_t[((int64_t)tile * NN + a) * TILE + lane];
+        double da[NN][3];
+#pragma unroll
+        for (int a = 0; a < NN; a++) da[a][0] = da[a][1] = da[a][2] = 0.0;
+#pragma unroll 1
+        for (int q = 0; q < NQ; q++) {
+            const int64_t tq = (int64_t)tile * NQ + q;
+            double M[3][3];
+#pragma unroll
+            for (int c = 0; c < 9; c++) M[c / 3][c % 3] = jinv_t[(tq * 9 + c) * TILE + lane];
+            const double dV = detJ_t[tq * TILE + lane] * quad.w[q];
+            double Ju[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+            for (int a = 0; a < NN; a++) {
+                const int s = sl[a];
+                const double ua[3] = {sU[s], sU[max_u + s], sU[2 * max_u + s]};
+#pragma unroll
+                for (int k = 0; k < 3; k++)
+#pragma unroll
+                    for (int i = 0; i < 3; i++) Ju[i][k] += ua[i] * dn.d[q][a][k];
+            }
+            double F[3][3];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+                    F[i][j] = (i == j ? 1.0 : 0.0) + Ju[i][0] * M[j][0] + Ju[i][1] * M[j][1] + Ju[i][2] * M[j][2];
+            double trE = 0.0, FFd[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                FFd[i] = F[i][0] * F[i][0] + F[i][1] * F[i][1] + F[i][2] * F[i][2];
+                // tr(F^T F) = sum of the squared entries
+                trE += FFd[i];
+            }
+            trE = 0.5 * (trE - 3.0);
+#pragma unroll
+            for (int a = 0; a < NN; a++) {
+                double g[3];
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+                    g[i] = M[i][0] * dn.d[q][a][0] + M[i][1] * dn.d[q][a][1] + M[i][2] * dn.d[q][a][2];
+                const double gg = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+                double Fh[3];
+#pragma unroll
+                for (int i = 0; i < 3; i++) Fh[i] = F[i][0] * g[0] + F[i][1] * g[1] + F[i][2] * g[2];
+                const double FhFh = Fh[0] * Fh[0] + Fh[1] * Fh[1] + Fh[2] * Fh[2];
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+                    da[a][i] += dV * ((lam + mu) * Fh[i] * Fh[i] + gg * (lam * trE + mu * (FFd[i] - 1.0)) + mu * FhFh);
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < NN; a++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) atomicAdd(&sF[i * max_u + sl[a]], da[a][i]);
+    }
+    __syncthreads();
+    for (int u = threadIdx.x; u < U; u += PLAN_BLOCK) {
+        double* o = part + 3 * (u0 + u);
+        o[0] = sF[u];
+        o[1] = sF[max_u + u];
+        o[2] = sF[2 * max_u + u];
+    }
+}
+
+int launch_tangent_diag_partials(const tlfea_force_plan* pl, const double* x, const double* y, const double* z,
+                                 const double* X0, const double* Y0, const double* Z0, const double* jinv_t,
+                                 const double* detJ_t, const tlfea_quad& quad, const TanMat<double>& mat,
+                                 hipStream_t s) {
+    if (pl->n_blocks == 0) return TLFEA_OK;
+    const DnTable<double> dn = make_dn_table(pl->nn, quad);
+    const size_t lds = sizeof(double) * (size_t)pl->max_u * 6;
+    if (pl->nn == 10)
+        k_tangent_diag<10, 5><<<pl->n_blocks, PLAN_BLOCK, lds, s>>>(pl->n_elem, pl->d_blk_off, pl->d_unodes,
+                                                                    pl->d_slot, x, y, z, X0, Y0, Z0, jinv_t, detJ_t,
+                                                                    dn, quad, mat, pl->d_part, pl->max_u);
+    else
+        k_tangent_diag<4, 1><<<pl->n_blocks, PLAN_BLOCK, lds, s>>>(pl->n_elem, pl->d_blk_off, pl->d_unodes,
+                                                                   pl->d_slot, x, y, z, X0, Y0, Z0, jinv_t, detJ_t,
+                                                                   dn, quad, mat, pl->d_part, pl->max_u);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
+}  // namespace tlfea_hip
+
+using namespace tlfea_hip;
+
+extern "C" int tlfea_tangent_apply_planned(const tlfea_force_plan* plan, const double* x, const double* y,
+                                           const double* z, const double* X0, const double* Y0, const double* Z0,
+                                           const double* jinv_t, const double* detJ_t, const tlfea_quad* quad,
+                                           const tlfea_material* mat, const unsigned char* fixed_dof_mask,
+                                           const double* p, double* out, void* stream) {
+    TLFEA_REQUIRE(plan && x && y && z && X0 && Y0 && Z0 && jinv_t && detJ_t && quad && mat && p && out,
+                  "null argument");
+    TLFEA_REQUIRE(quad->n == (plan->nn == 10 ? 5 : 1), "quadrature size does not match the element type");
+    TLFEA_REQUIRE(mat->eta_damp == 0.0 && mat->lambda_damp == 0.0, "the matrix-free tangent excludes damping");
+    return launch_tangent_planned(plan, x, y, z, X0, Y0, Z0, jinv_t, detJ_t, *quad, *mat, fixed_dof_mask, p, out,
+                                  nullptr, nullptr, as_stream(stream));
+}
