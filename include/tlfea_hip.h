@@ -261,6 +261,10 @@ int tlfea_batch_tangent_apply(tlfea_batch* b, const double* p, double* out, void
  * lambda_max (host [n_inst], may be NULL) receives the unscaled estimates. */
 int tlfea_batch_precond_setup(tlfea_batch* b, int degree, double ratio, int power_iters, double safety,
                               double* lambda_max, void* stream);
+/* u += Chebyshev(f - K u), `sweeps` times: the FP32 stage of tlfea_batch_precond_setup used
+ * as a smoother of an initial guess (it damps the error components of the upper part of
+ * the spectrum of D^-1 K). */
+int tlfea_batch_smooth(tlfea_batch* b, const double* f, double* u, int sweeps, void* stream);
 /* Linear-static CG for every instance at once: K_i u_i = f_i (fixed DOFs eliminated),
  * the reference's CG per instance with its own alpha/beta, stopping when
  * ||f_i - K_i u_i|| / ||f_i|| < tol on the free DOFs.  use_guess != 0 starts from u
@@ -275,6 +279,49 @@ int tlfea_batch_precond_setup(tlfea_batch* b, int degree, double ratio, int powe
 int tlfea_batch_cg_solve(tlfea_batch* b, const double* f, double* u, int use_guess, double tol, int max_iter,
                          int poll, int use_graph, int precond, int* iters, double* rel_res, int* status,
                          int* total_iters, void* stream);
+
+/* ---- learned warm start (not in the reference) -------------------------------------- */
+/* A message-passing network on the mesh node graph (CSR off/col of tlfea_pattern_node_csr,
+ * diagonal included): h0 = relu(x W_in^T + b_in); per layer m = mean_{j in N(i)} h_j,
+ * h = h + relu([h, m] W_l^T + b_l); out = h W_out^T + b_out.  Width 64, 16 input features,
+ * 3 outputs (padded to 16).  BF16 weights and activations on the matrix cores with FP32
+ * accumulation; FP32 master weights and Adam.  Parameters (FP32, n_params) are packed as
+ * W_in [64x16], b_in [64], per layer W_l [64x128], b_l [64], W_out [16x64], b_out [16]
+ * (output rows 3..15 stay 0).  feat: bf16 [n x 16]; out: FP32 [n x 16]. */
+typedef struct tlfea_mpnn tlfea_mpnn;
+int tlfea_mpnn_create(int layers, tlfea_mpnn** out);
+int tlfea_mpnn_destroy(tlfea_mpnn* net);
+int tlfea_mpnn_num_params(const tlfea_mpnn* net, int* n_params);
+/* host FP32 parameters in, Adam state reset */
+int tlfea_mpnn_set_params(tlfea_mpnn* net, const float* host);
+int tlfea_mpnn_get_params(const tlfea_mpnn* net, float* host);
+int tlfea_mpnn_get_grads(const tlfea_mpnn* net, float* host);
+int tlfea_mpnn_zero_grads(tlfea_mpnn* net, void* stream);
+int tlfea_mpnn_forward(tlfea_mpnn* net, int n, const unsigned short* feat, const int* off, const int* col,
+                       float* out, void* stream);
+/* forward + loss sum_i w_i |out_i[:3] - target_i|^2 / sum_i w_i + backward; gradients are
+ * added to the gradient buffer; *loss (host) receives the loss. */
+int tlfea_mpnn_loss_grad(tlfea_mpnn* net, int n, const unsigned short* feat, const int* off, const int* col,
+                         const float* target, const float* weight, double* loss, void* stream);
+int tlfea_mpnn_adam(tlfea_mpnn* net, double lr, double beta1, double beta2, double eps, void* stream);
+/* Backward of a caller-supplied output gradient dout (FP32 [n x 3]) through the network
+ * state of the last tlfea_mpnn_forward on the same n nodes; gradients are added. */
+int tlfea_mpnn_backward(tlfea_mpnn* net, int n, const int* off, const int* col, const float* dout, void* stream);
+/* Output gradient of the energy loss sum_inst w_inst e^T K e / E_inst with
+ * e = scale * out - u*: dout[i, d] = 2 w_over_E[inst] scale[3 inst + d] Ke[3i + d]. */
+int tlfea_mpnn_energy_grad(int n, const int* node_inst, const double* scale, const double* Ke,
+                           const double* w_over_E, float* dout, void* stream);
+/* Features of the nodes of a batch of cantilevers: inst_par[i*12] = bbox low corner (3),
+ * extents (3), total load (3), E, nu, unused; fixed_dof_mask and f as in tlfea_batch.
+ * Per node: normalised rest coordinates (3), fixed flag, nodal load / |F| (3), F / |F| (3),
+ * (log10 E - 8)/2, (nu - 0.325)/0.125, log aspect ratios (2), xi^2, (3 xi^2 - xi^3)/2. */
+int tlfea_mpnn_features(int n, const int* node_inst, const double* x, const double* y, const double* z,
+                        const unsigned char* fixed_dof_mask, const double* f, const double* inst_par,
+                        unsigned short* feat, void* stream);
+/* u[3i+d] = scale[3 inst(i) + d] * out[i, d] on free DOFs (0 on fixed ones), the
+ * displacement guess (per-instance, per-component scales). */
+int tlfea_mpnn_scale_output(int n, const int* node_inst, const float* out, const double* scale,
+                            const unsigned char* fixed_dof_mask, double* u, void* stream);
 
 /* ---- mixed precision CG (not in the reference) -------------------------------------- */
 /* k32[9*sell_blocks] = (float) k_sell. */

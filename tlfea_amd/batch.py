@@ -99,6 +99,12 @@ class Batch:
         self.precond_degree = degree
         return lm
 
+    def smooth(self, u, sweeps=2, f=None):
+        """u += Chebyshev(f - K u), in place (needs precond_setup)."""
+        f = self.f if f is None else f
+        call("tlfea_batch_smooth", self.h, ptr(f), ptr(u), int(sweeps), self.stream)
+        return u
+
     def solve(self, tol=1e-8, max_iter=100000, u0=None, poll=32, graph=True, f=None, out=None, precond=False):
         """Returns u (device, all instances); per-instance iterations, relative residuals
         and status land in self.iters / self.rel / self.status.  precond=True uses the

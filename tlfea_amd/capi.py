@@ -124,7 +124,22 @@ _SIGS = {
     "tlfea_batch_tangent_apply": (i32, [vp, vp, vp, vp]),
     "tlfea_batch_cg_solve": (i32, [vp, vp, vp, i32, f64, i32, i32, i32, i32, vp, vp, vp, C.POINTER(i32), vp]),
     "tlfea_batch_set_compaction": (i32, [vp, f64]),
+    "tlfea_batch_smooth": (i32, [vp, vp, vp, i32, vp]),
     "tlfea_batch_precond_setup": (i32, [vp, i32, f64, i32, f64, vp, vp]),
+    "tlfea_mpnn_create": (i32, [i32, C.POINTER(vp)]),
+    "tlfea_mpnn_destroy": (i32, [vp]),
+    "tlfea_mpnn_num_params": (i32, [vp, C.POINTER(i32)]),
+    "tlfea_mpnn_set_params": (i32, [vp, vp]),
+    "tlfea_mpnn_get_params": (i32, [vp, vp]),
+    "tlfea_mpnn_get_grads": (i32, [vp, vp]),
+    "tlfea_mpnn_zero_grads": (i32, [vp, vp]),
+    "tlfea_mpnn_forward": (i32, [vp, i32, vp, vp, vp, vp, vp]),
+    "tlfea_mpnn_loss_grad": (i32, [vp, i32, vp, vp, vp, vp, vp, C.POINTER(f64), vp]),
+    "tlfea_mpnn_adam": (i32, [vp, f64, f64, f64, f64, vp]),
+    "tlfea_mpnn_backward": (i32, [vp, i32, vp, vp, vp, vp]),
+    "tlfea_mpnn_energy_grad": (i32, [i32, vp, vp, vp, vp, vp, vp]),
+    "tlfea_mpnn_features": (i32, [i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "tlfea_mpnn_scale_output": (i32, [i32, vp, vp, vp, vp, vp, vp]),
     "tlfea_add_displacement": (i32, [i32, vp, vp, vp, vp, vp]),
     "tlfea_matrix_to_f32": (i32, [vp, vp, vp, vp]),
     "tlfea_cg_solve_mixed": (i32, [vp, vp, vp, vp, vp, f64, f64, i32, i32, C.POINTER(i32), C.POINTER(f64),

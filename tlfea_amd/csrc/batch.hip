@@ -188,6 +188,11 @@ __global__ __launch_bounds__(BT) void k_breduce_init(int n_inst, const int* __re
     if (threadIdx.x == 0) all[0] = cnt;
 }
 
+__global__ void k_bclear_flags(int n_inst, int* __restrict__ I) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_inst) I[(int64_t)i * NI + I_DONE] = 0;
+}
+
 // u = 0 for instances without load (the reference's CG returns 0 for f = 0)
 __global__ __launch_bounds__(BT) void k_bzero_u(Chunks ch, const int* __restrict__ I, double* __restrict__ u) {
     const int c = chunk_id(ch);
@@ -860,6 +865,33 @@ extern "C" int tlfea_batch_precond_setup(tlfea_batch* b, int degree, double rati
     }
     TLFEA_HIP_TRY(hipStreamSynchronize(s));
     b->cheb_degree = degree;
+    return TLFEA_OK;
+}
+
+// u += z, FP64 (z from the Chebyshev stage)
+__global__ __launch_bounds__(BT) void k_badd(Chunks ch, const double* __restrict__ z, double* __restrict__ u) {
+    const int c = chunk_id(ch);
+    for (int64_t k = 3 * (int64_t)ch.beg[c] + threadIdx.x; k < 3 * (int64_t)ch.end[c]; k += BT) u[k] += z[k];
+}
+
+extern "C" int tlfea_batch_smooth(tlfea_batch* b, const double* f, double* u, int sweeps, void* stream) {
+    TLFEA_REQUIRE(b && f && u, "null argument");
+    TLFEA_REQUIRE(b->cheb_degree > 0, "the Chebyshev stage is not set up");
+    TLFEA_REQUIRE(sweeps >= 0, "negative sweep count");
+    hipStream_t s = as_stream(stream);
+    full_lists(b);
+    const Chunks ch = chunks(b);
+    // every instance takes part: clear the done flags the Chebyshev kernels test
+    k_bclear_flags<<<grid_for(b->n_inst, 256), 256, 0, s>>>(b->n_inst, b->d_i);
+    for (int k = 0; k < sweeps; k++) {
+        if (int rc = apply64(b, u, b->d_q, nullptr, s)) return rc;
+        k_binit<<<b->n_chunks, BT, 0, s>>>(ch, f, b->d_q, b->d_mask, b->d_r, b->d_part);
+        TLFEA_HIP_TRY(hipMemcpyAsync(b->d_rold, b->d_r, sizeof(double) * 3 * (size_t)b->n_nodes,
+                                     hipMemcpyDeviceToDevice, s));
+        if (int rc = precondition(b, s)) return rc;
+        k_badd<<<b->n_chunks, BT, 0, s>>>(ch, b->d_z, u);
+        TLFEA_LAUNCH_CHECK();
+    }
     return TLFEA_OK;
 }
 
