@@ -16,7 +16,7 @@ reported beside it.  Inputs are resident in HBM when the timed region starts.
 Kernel rooflines are measured live with HIP events on the launch stream (the null stream,
 which torch.cuda.Event records on):
   * roofline      -- the step's dominant kernel pair, the matrix-free tangent
-                     (k_tangent_planned + k_gather_tangent), bytes/launch =
+                     (k_tangent<10,5,double> + k_gather_tangent), bytes/launch =
                      440*n_elem + 96*n_nodes (inverse Jacobians 360 B, detJ 40 B and
                      connectivity 40 B per element; x, X0 and p read and y written, 96 B
                      per node);
@@ -27,6 +27,11 @@ which torch.cuda.Event records on):
 cpu_baseline: the FP64 oracle on a bounded sample (a slice of the element assembly and
 10 CG iterations on the full matrix), extrapolated to one solve with the plain GPU CG's
 iteration count.
+
+`batched` (not part of `value`): the batch of 256 uneven cantilevers of the beam sampler
+(seed 2026) solved to 1e-8 -- all at once by the batched FP64 CG (finished instances
+compacted out), with the FP32 Jacobi-Chebyshev stage, from the learned guess, and one
+after another by the plain FP64 drop-in path -- in solves/s.
 """
 from __future__ import annotations
 
@@ -40,12 +45,73 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes of one matrix-free tangent launch pair on the 200x20x20 beam (rocprofv3 PMC)
-PMC_TANGENT_BYTES = None
+# HBM bytes of one matrix-free tangent launch pair (k_tangent + k_gather_tangent) on the
+# 200x20x20 beam: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (KiB), scales calibrated
+# on a 512 MiB copy
+PMC_TANGENT_BYTES = (152015 * 2.0 + 29109 + 30817 * 2.0 + 21561) * 1024
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def batched_beams(n, tol, log):
+    """Solves/s on the sampler's batch of uneven beams: batched paths and the plain
+    per-instance drop-in path on the same instances."""
+    import numpy as np
+    import torch
+    from tlfea_amd import mesh as M
+    from tlfea_amd.batch import Batch
+    from tlfea_amd.fea import Body, LinearStatic
+    from tlfea_amd.learned import BatchGraph, WarmStart
+    items = M.random_beam_batch(n, 2026)
+    b = Batch(items)
+    res = {"workload": f"{n} cantilevers of the beam sampler (seed 2026): {b.n_elem} TET10 elements, "
+                       f"{3 * b.n_nodes} DOF in total; linear-static solves to {tol:g}"}
+
+    def timed(tag, fn):
+        fn()  # warm-up
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        ok = int((b.status == 0).sum())
+        res[tag] = {"solves_per_s": ok / dt, "s": dt, "converged": ok, "iters_median": float(np.median(b.iters)),
+                    "iters_max": int(b.iters.max())}
+        log(f"batched {tag}: {dt:.3f} s, {ok}/{n} converged, median {np.median(b.iters):.0f} iterations")
+        return out
+
+    u64 = timed("fp64", lambda: b.solve(tol=tol))
+    b.precond_setup(degree=6, ratio=30.0, power_iters=20)
+    u32 = timed("fp32_chebyshev", lambda: b.solve(tol=tol, precond=True))
+    net = WarmStart.load()
+    g = BatchGraph(b)
+    uw = timed("fp64_learned_start", lambda: b.solve(tol=tol, u0=net.guess(b, graph=g)))
+    rd = []
+    for u in (u32, uw):
+        rd.append(max(float(torch.linalg.norm(u[b.inst_slice(i)] - u64[b.inst_slice(i)])
+                            / torch.linalg.norm(u64[b.inst_slice(i)])) for i in range(n)))
+    res["max_rel_diff_vs_fp64"] = {"fp32_chebyshev": rd[0], "fp64_learned_start": rd[1]}
+    # the plain drop-in path, one instance after another
+    t_seq, ok = 0.0, 0
+    for m, mat in items:
+        body = Body(m, mat["E"], mat["nu"], mat["rho"])
+        ls = LinearStatic(body, tol=tol, max_iter=200000, check_every=32)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        rc = ls.solve()
+        torch.cuda.synchronize()
+        t_seq += time.perf_counter() - t
+        ok += rc == 0
+        ls.close()
+        body.close()
+    res["plain_fp64_sequential"] = {"solves_per_s": ok / t_seq, "s": t_seq, "converged": ok}
+    log(f"plain sequential: {t_seq:.1f} s for {n} solves")
+    for k in ("fp64", "fp32_chebyshev", "fp64_learned_start"):
+        res[k]["speedup_vs_plain_sequential"] = res[k]["solves_per_s"] / res["plain_fp64_sequential"]["solves_per_s"]
+    b.close()
+    return res
 
 
 def main():
@@ -57,6 +123,9 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cells", type=str, default="200,20,20")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-beams measurement")
+    ap.add_argument("--batch-n", type=int, default=256)
+    ap.add_argument("--batch-only", action="store_true", help="only the batched-beams measurement")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one launch each of the planned force kernel, the block SpMV, the matrix-free "
                          "tangent and a 512 MiB copy "
@@ -91,6 +160,9 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         log("pmc probe done")
+        return
+    if args.batch_only:
+        print(json.dumps({"batched": batched_beams(args.batch_n, args.tol, log)}))
         return
     t0 = time.time()
     m = M.clamp_and_tip_load(M.kuhn_beam(nx, ny, nz), -1.0e5)
@@ -197,6 +269,8 @@ def main():
                          f"{t_it * 1e3:.1f} ms/iteration"}
         log(f"cpu baseline: {cpu_solve_s:.1f} s per solve")
 
+    batched = None if args.no_batch else batched_beams(args.batch_n, args.tol, log)
+
     # HBM bytes per launch on this workload from two rocprofv3 PMC passes (FETCH_SIZE x 2.0
     # + WRITE_SIZE x 1.0, scales calibrated on a 512 MiB copy); known for the default
     # 200x20x20 beam only
@@ -226,7 +300,7 @@ def main():
                    "speedup_vs_plain_fp64": dt64 / dt},
         "roofline": {"bound": "hbm", "achieved": tan_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": tan_gbs / HBM_PEAK_GBS, "traffic": traffic_tan,
-                     "kernel": "k_tangent_planned<10,5> + k_gather_tangent (matrix-free K(x) p)",
+                     "kernel": "k_tangent<10,5,double,true> + k_gather_tangent (matrix-free K(x) p)",
                      "avg_us": tan_s * 1e6, "bytes_per_launch": tan_bytes},
         "spmv_fp64": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                       "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_cg_spmv/k_spmv_block",
@@ -235,6 +309,7 @@ def main():
                          "frac": force_gbs / HBM_PEAK_GBS, "kernel": "k_force_planned<10,5,false> + k_gather_partials",
                          "avg_us": force_s * 1e6, "bytes_per_launch": force_bytes},
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out))
 
