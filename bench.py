@@ -55,7 +55,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def batched_beams(n, tol, log):
+def batched_beams(n, tol, log, sequential=True):
     """Solves/s on the sampler's batch of uneven beams: batched paths and the plain
     per-instance drop-in path on the same instances."""
     import numpy as np
@@ -93,6 +93,9 @@ def batched_beams(n, tol, log):
         rd.append(max(float(torch.linalg.norm(u[b.inst_slice(i)] - u64[b.inst_slice(i)])
                             / torch.linalg.norm(u64[b.inst_slice(i)])) for i in range(n)))
     res["max_rel_diff_vs_fp64"] = {"fp32_chebyshev": rd[0], "fp64_learned_start": rd[1]}
+    if not sequential:
+        b.close()
+        return res
     # the plain drop-in path, one instance after another
     t_seq, ok = 0.0, 0
     for m, mat in items:
@@ -126,6 +129,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-beams measurement")
     ap.add_argument("--batch-n", type=int, default=256)
     ap.add_argument("--batch-only", action="store_true", help="only the batched-beams measurement")
+    ap.add_argument("--no-seq", action="store_true", help="skip the one-after-another plain solves of the batch")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one launch each of the planned force kernel, the block SpMV, the matrix-free "
                          "tangent and a 512 MiB copy "
@@ -162,7 +166,7 @@ def main():
         log("pmc probe done")
         return
     if args.batch_only:
-        print(json.dumps({"batched": batched_beams(args.batch_n, args.tol, log)}))
+        print(json.dumps({"batched": batched_beams(args.batch_n, args.tol, log, not args.no_seq)}))
         return
     t0 = time.time()
     m = M.clamp_and_tip_load(M.kuhn_beam(nx, ny, nz), -1.0e5)
@@ -269,7 +273,7 @@ def main():
                          f"{t_it * 1e3:.1f} ms/iteration"}
         log(f"cpu baseline: {cpu_solve_s:.1f} s per solve")
 
-    batched = None if args.no_batch else batched_beams(args.batch_n, args.tol, log)
+    batched = None if args.no_batch else batched_beams(args.batch_n, args.tol, log, not args.no_seq)
 
     # HBM bytes per launch on this workload from two rocprofv3 PMC passes (FETCH_SIZE x 2.0
     # + WRITE_SIZE x 1.0, scales calibrated on a 512 MiB copy); known for the default
