@@ -32,6 +32,10 @@ iteration count.
 (seed 2026) solved to 1e-8 -- all at once by the batched FP64 CG (finished instances
 compacted out), with the FP32 Jacobi-Chebyshev stage, from the learned guess, and one
 after another by the plain FP64 drop-in path -- in solves/s.
+
+`dynamic` (not part of `value`): 64 beams of the sampler under gravity and sinusoidal tip
+loads, 200 implicit steps of 1e-4 s solved to 1e-8 for all beams at once, and 20 steps
+of each beam alone (the per-instance path) for the comparison; instance-steps/s.
 """
 from __future__ import annotations
 
@@ -117,6 +121,53 @@ def batched_beams(n, tol, log, sequential=True):
     return res
 
 
+def dynamic_beams_bench(n, steps, seq_steps, tol, log):
+    """Batched implicit time steps of n beams against the same beams stepped one by one."""
+    import numpy as np
+    import torch
+    from tlfea_amd.dynamics import BatchDynamics, dynamic_beams
+    items, amp, freq = dynamic_beams(n, 2026)
+    dyn = BatchDynamics(items, amp, freq, tol=tol)
+    dyn.step()  # warm-up (graph capture)
+    dyn = BatchDynamics(items, amp, freq, tol=tol, batch=dyn.b)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        dyn.step()
+    torch.cuda.synchronize()
+    dt_b = time.perf_counter() - t
+    its = np.array(dyn.iters)
+    res = {"workload": f"{n} beams of the sampler (seed 2026), {dyn.b.n_elem} TET10 elements, {3 * dyn.b.n_nodes} DOF; "
+                       f"gravity + sinusoidal tip load; implicit Euler, dt 1e-4 s, exactly integrated consistent "
+                       f"mass, each step solved to {tol:g}",
+           "batched": {"steps": steps, "s": dt_b, "instance_steps_per_s": n * steps / dt_b,
+                       "cg_iters_median": float(np.median(its)), "cg_iters_max": int(its.max())}}
+    log(f"dynamic batched: {steps} steps of {n} beams in {dt_b:.2f} s, median {np.median(its):.0f} CG iterations")
+    # the same beams one at a time, for seq_steps steps, and the agreement after them
+    dyn2 = BatchDynamics(items, amp, freq, tol=tol, batch=dyn.b)
+    for _ in range(seq_steps):
+        dyn2.step()
+    t_seq, diff = 0.0, 0.0
+    for i, it in enumerate(items):
+        single = BatchDynamics([it], [amp[i]], [freq[i]], tol=tol)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(seq_steps):
+            single.step()
+        torch.cuda.synchronize()
+        t_seq += time.perf_counter() - t
+        sl = dyn2.b.inst_slice(i)
+        nu = float(torch.linalg.norm(single.u))
+        diff = max(diff, float(torch.linalg.norm(dyn2.u[sl] - single.u)) / max(nu, 1e-300))
+        single.b.close()
+    res["one_by_one"] = {"steps": seq_steps, "s": t_seq, "instance_steps_per_s": n * seq_steps / t_seq}
+    res["speedup_vs_one_by_one"] = res["batched"]["instance_steps_per_s"] / res["one_by_one"]["instance_steps_per_s"]
+    res["max_rel_diff_batched_vs_one_by_one"] = diff
+    log(f"dynamic one by one: {seq_steps} steps each in {t_seq:.2f} s; max rel diff {diff:.2e}")
+    dyn.b.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,6 +181,8 @@ def main():
     ap.add_argument("--batch-n", type=int, default=256)
     ap.add_argument("--batch-only", action="store_true", help="only the batched-beams measurement")
     ap.add_argument("--no-seq", action="store_true", help="skip the one-after-another plain solves of the batch")
+    ap.add_argument("--no-dynamic", action="store_true", help="skip the dynamic batched measurement")
+    ap.add_argument("--dynamic-only", action="store_true", help="only the dynamic batched measurement")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one launch each of the planned force kernel, the block SpMV, the matrix-free "
                          "tangent and a 512 MiB copy "
@@ -164,6 +217,9 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         log("pmc probe done")
+        return
+    if args.dynamic_only:
+        print(json.dumps({"dynamic": dynamic_beams_bench(64, 200, 20, args.tol, log)}))
         return
     if args.batch_only:
         print(json.dumps({"batched": batched_beams(args.batch_n, args.tol, log, not args.no_seq)}))
@@ -274,6 +330,7 @@ def main():
         log(f"cpu baseline: {cpu_solve_s:.1f} s per solve")
 
     batched = None if args.no_batch else batched_beams(args.batch_n, args.tol, log, not args.no_seq)
+    dynamic = None if args.no_dynamic else dynamic_beams_bench(64, 200, 20, args.tol, log)
 
     # HBM bytes per launch on this workload from two rocprofv3 PMC passes (FETCH_SIZE x 2.0
     # + WRITE_SIZE x 1.0, scales calibrated on a 512 MiB copy); known for the default
@@ -314,6 +371,7 @@ def main():
                          "avg_us": force_s * 1e6, "bytes_per_launch": force_bytes},
         "cpu_baseline": cpu,
         "batched": batched,
+        "dynamic": dynamic,
     }
     print(json.dumps(out))
 

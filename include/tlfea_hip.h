@@ -254,6 +254,20 @@ int tlfea_batch_set_compaction(tlfea_batch* b, double ratio);
 /* out = K p for every instance at its rest configuration, Dirichlet rows/columns masked
  * (the operator of tlfea_tangent_apply_planned per instance). */
 int tlfea_batch_tangent_apply(tlfea_batch* b, const double* p, double* out, void* stream);
+/* Adds the consistent mass M, applied without assembly, to the operator of
+ * tlfea_batch_cg_solve: it becomes K + c M per instance, with rho_coef[i] = rho_i * c
+ * (host [n_inst]); NULL removes it.  Used by implicit time steps (c = 1/dt^2).
+ * rule 0: the quadrature rule of the mass assembly (mass_matrix_qp_kernel,
+ * FEAT10Data.cu:200-271; FEAT4Data.cu:197-243), whose TET10 mass is indefinite (negative
+ * centroid weight) and TET4 mass rank one, so K + c M loses definiteness once c M
+ * dominates K; rule 1: the exactly integrated mass of straight-sided elements (symmetric
+ * positive definite, same row sums). */
+int tlfea_batch_set_mass(tlfea_batch* b, const double* rho_coef, int rule);
+/* out = (k_scale K + with_mass * c M) p for every instance; masked != 0 applies the
+ * Dirichlet elimination (fixed rows return p, fixed columns are dropped), masked = 0 the
+ * full operator (e.g. the consistent gravity load M g, which sums over all nodes). */
+int tlfea_batch_apply(tlfea_batch* b, double k_scale, int with_mass, int masked, const double* p, double* out,
+                      void* stream);
 /* FP32 Jacobi-Chebyshev preconditioner for tlfea_batch_cg_solve(precond = 1): the
  * diagonal D of K (FP64, matrix-free), lambda_max(D^-1 K) per instance by `power_iters`
  * FP32 power iterations (scaled by `safety` >= 1), and `degree` Chebyshev steps on

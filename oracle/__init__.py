@@ -18,6 +18,9 @@ Parity status of each function (how it is pinned, see tests/test_oracle.py):
   independent numpy sort/unique route.
 * ``mass_values`` -- symmetry, total mass = rho V, row sums = integral of rho N_i,
   TET4 rank-one rule.
+* ``mass_values_exact`` -- the coefficient table against an independent exact rational
+  integration of N_a N_b over the tetrahedron (Dirichlet moment formula), positive
+  definiteness, total mass = rho V, row sums equal to the 5-point rule's.
 * ``stiffness_values`` -- symmetry, rigid modes in the kernel, finite difference of f_int,
   linear-elastic closed form at F = I.
 * ``cg`` / ``band_cholesky_solve`` -- each against dense numpy solves, Euler-Bernoulli
@@ -207,6 +210,55 @@ def mass_values(mesh, detJ, rho, off, col):
     vals = np.zeros(len(col))
     lib().orc_mass_values(mesh.nn, nq, mesh.n_elem, _i(_i32(mesh.conn_colmajor())), _d(_f64(detJ)), _d(qx), _d(qy),
                           _d(qz), _d(qw), C.c_double(rho), _i(_i32(off)), _i(_i32(col)), _d(vals), mesh.n_nodes)
+    return vals
+
+
+# Corner/edge numbering of the TET10 element: edge m joins corners T10_EDGES[m].
+T10_EDGES = ((0, 1), (1, 2), (0, 2), (0, 3), (1, 3), (2, 3))
+
+
+def exact_mass_coefficients(nn):
+    """C with the exactly integrated element mass M_ab = rho V C_ab of a straight-sided element.
+
+    This has no reference counterpart: mass_matrix_qp_kernel (FEAT10Data.cu:200-271)
+    integrates the degree-4 integrand N_a N_b with the 5-point rule, whose negative
+    centroid weight makes the TET10 mass indefinite, and the TET4 one
+    (FEAT4Data.cu:197-243) with one point, which gives a rank-1 mass.  Exactly
+    integrated, TET10: corner-corner 6/420 (diagonal) and 1/420, corner-edge -4/420 when
+    the edge has that corner and -6/420 otherwise, edge-edge 32/420 (diagonal), 16/420
+    for edges sharing a corner and 8/420 for opposite edges; TET4: (1 + delta_ab)/20.
+    """
+    if nn == 4:
+        return (np.ones((4, 4)) + np.eye(4)) / 20.0
+    Cm = np.zeros((10, 10))
+    for a in range(4):
+        for b in range(4):
+            Cm[a, b] = 6.0 if a == b else 1.0
+        for m, e in enumerate(T10_EDGES):
+            Cm[a, 4 + m] = Cm[4 + m, a] = -4.0 if a in e else -6.0
+    for m, e in enumerate(T10_EDGES):
+        for k, f in enumerate(T10_EDGES):
+            Cm[4 + m, 4 + k] = 32.0 if m == k else (16.0 if set(e) & set(f) else 8.0)
+    return Cm / 420.0
+
+
+def mass_values_exact(mesh, detJ, rho, off, col):
+    """Node-level consistent mass on the mass pattern with M_e = rho V_e C
+    (``exact_mass_coefficients``), V_e = sum_q w_q detJ_(e,q)."""
+    nq = 5 if mesh.nn == 10 else 1
+    qw = quad(mesh.nn)[3]
+    vol = np.asarray(detJ, np.float64).reshape(mesh.n_elem, nq) @ qw
+    Cm = exact_mass_coefficients(mesh.nn)
+    conn = np.asarray(mesh.conn_colmajor(), np.int64).reshape(mesh.nn, mesh.n_elem).T
+    off = np.asarray(off, np.int64)
+    col = np.asarray(col, np.int64)
+    vals = np.zeros(len(col))
+    for e in range(mesh.n_elem):
+        for a in range(mesh.nn):
+            i = conn[e, a]
+            row = col[off[i]:off[i + 1]]
+            k = off[i] + np.searchsorted(row, conn[e])
+            np.add.at(vals, k, rho * vol[e] * Cm[a])
     return vals
 
 

@@ -1,0 +1,123 @@
+"""GPU checks of the batched implicit time steps (tlfea_amd/dynamics.py).
+
+Bars: the matrix-free mass term equals the oracle's consistent mass times p to 1e-12, for
+the quadrature rule of the reference's assembly (mass_matrix_qp_kernel, 5-point rule) and
+for the exactly integrated mass; five steps of a small beam match a dense FP64 solve of the
+same implicit Euler equations, built from the oracle's K and M, to 1e-9 (both rules, and a
+soft beam whose K + M/dt^2 is indefinite under the 5-point mass with the exact one); a
+batch of beams stepped together matches each beam stepped alone (the per-instance plain
+path) to 1e-7.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import lame, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _beam(nx, ny, nz, load, E=2e9, nu=0.3, rho=2000.0, tet10=True):
+    from tlfea_amd import mesh as M
+    m = M.clamp_and_tip_load(M.kuhn_beam(nx, ny, nz, 1.0, 0.1, 0.1, tet10=tet10), np.asarray(load, float))
+    return m, dict(E=E, nu=nu, rho=rho)
+
+
+def _dense_KM(m, mat, rule="quadrature"):
+    go, djo = O.grad(m)
+    lam, mu = lame(mat["E"], mat["nu"])
+    doff, dcol = O.dof_pattern(m)
+    kv = O.stiffness_values(m, go, djo, lam, mu, doff, dcol)
+    n = 3 * m.n_nodes
+    K = np.zeros((n, n))
+    for r in range(n):
+        K[r, dcol[doff[r]:doff[r + 1]]] = kv[doff[r]:doff[r + 1]]
+    off, col = O.node_pattern(m)
+    mv = (O.mass_values if rule == "quadrature" else O.mass_values_exact)(m, djo, mat["rho"], off, col)
+    M = np.zeros((n, n))
+    for i in range(m.n_nodes):
+        for k in range(off[i], off[i + 1]):
+            for d in range(3):
+                M[3 * i + d, 3 * col[k] + d] = mv[k]
+    return K, M
+
+
+@pytest.mark.parametrize("rule", ["quadrature", "exact"])
+@pytest.mark.parametrize("tet10", [True, False])
+def test_mass_term_matches_oracle(gpu, tet10, rule):
+    import torch
+    from tlfea_amd.batch import Batch
+    m, mat = _beam(4, 2, 1, [0, 0, -10.0], tet10=tet10)
+    b = Batch([(m, mat)])
+    c = 7.0
+    b.set_mass(np.array([mat["rho"] * c]), rule=rule)
+    p = np.random.default_rng(1).standard_normal(3 * m.n_nodes)
+    free = _np(b.mask) == 0
+    y = _np(b.apply(torch.as_tensor(p).cuda(), k_scale=0.0, mass=True))
+    K, M = _dense_KM(m, mat, rule)
+    ref = c * (M @ (p * free))
+    assert rel_err(y[free], ref[free]) < 1e-12
+    # K + c M in one pass equals the sum of the parts
+    y2 = _np(b.apply(torch.as_tensor(p).cuda(), k_scale=1.0, mass=True))
+    ref2 = (K + c * M) @ (p * free)
+    assert rel_err(y2[free], ref2[free]) < 1e-12
+
+
+@pytest.mark.parametrize("rule,E,rho", [("quadrature", 2e9, 2000.0), ("exact", 2e9, 2000.0),
+                                        ("exact", 1.5e6, 6500.0)])
+def test_steps_match_dense_implicit_euler(gpu, rule, E, rho):
+    from tlfea_amd.dynamics import BatchDynamics
+    m, mat = _beam(4, 1, 1, [0.3, -0.5, -1.0], E=E, rho=rho)
+    dt = 1e-4
+    dyn = BatchDynamics([(m, mat)], amp=[200.0], freq=[15.0], dt=dt, tol=1e-13, mass_rule=rule)
+    K, M = _dense_KM(m, mat, rule)
+    free = _np(dyn.b.mask) == 0
+    A = (K + M / dt ** 2)[np.ix_(free, free)]
+    ev = np.linalg.eigvalsh(A)
+    if E < 1e7:  # the soft beam: definite only with the exact mass
+        Mq = _dense_KM(m, mat, "quadrature")[1]
+        assert ev.min() > 0 and np.linalg.eigvalsh((K + Mq / dt ** 2)[np.ix_(free, free)]).min() < 0
+    tip = _np(dyn.tip)
+    g = np.tile([0.0, 0.0, -9.81], m.n_nodes)
+    fg = (M @ g)
+    u = np.zeros(3 * m.n_nodes)
+    v = np.zeros(3 * m.n_nodes)
+    for k in range(5):
+        t1 = (k + 1) * dt
+        f = fg + tip * 200.0 * np.sin(2 * np.pi * 15.0 * t1)
+        rhs = M @ v / dt ** 2 - K @ u / dt + f / dt
+        v = np.zeros_like(v)
+        v[free] = np.linalg.solve(A, rhs[free])
+        u = u + dt * v
+        ug, vg = dyn.step()
+        assert rel_err(_np(vg)[free], v[free]) < 1e-9, k
+        assert rel_err(_np(ug)[free], u[free]) < 1e-9, k
+
+
+def test_batch_steps_match_single_beams(gpu):
+    from tlfea_amd.dynamics import BatchDynamics, dynamic_beams
+    items, amp, freq = dynamic_beams(4, 31)
+    # smaller beams than the configuration's, same sampler otherwise
+    from tlfea_amd import mesh as M
+    small = []
+    rng = np.random.default_rng(5)
+    for m, mat in items:
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        mm = M.clamp_and_tip_load(M.kuhn_beam(int(rng.integers(4, 12)), 2, 2, 1.0, 0.1, 0.1), d)
+        small.append((mm, mat))
+    dyn = BatchDynamics(small, amp, freq, tol=1e-12)
+    singles = [BatchDynamics([it], [a], [f], tol=1e-12) for it, a, f in zip(small, amp, freq)]
+    for k in range(20):
+        u, v = dyn.step()
+        for i, s in enumerate(singles):
+            us, vs = s.step()
+            sl = dyn.b.inst_slice(i)
+            assert rel_err(_np(u[sl]), _np(us)) < 1e-7, (k, i)
+            assert rel_err(_np(v[sl]), _np(vs)) < 1e-7, (k, i)
+    its = np.array(dyn.iters)
+    assert its.max() < 2000 and its.min() >= 1

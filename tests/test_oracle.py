@@ -247,6 +247,79 @@ def test_mass_matrix(name):
         assert abs(Mm[e0[0], e0[1]] - RHO_STEEL * V[0] / 16) < 1e-12 or Mm[e0[0], e0[1]] > RHO_STEEL * V[0] / 16
 
 
+def _exact_shape_mass(nn):
+    """int N_a N_b dV / V by exact rational arithmetic: N as polynomials in the barycentric
+    coordinates, int L0^a L1^b L2^c L3^d dV = 3! a! b! c! d! V / (a+b+c+d+3)!."""
+    from fractions import Fraction
+
+    def mul(p, q):
+        r = {}
+        for ea, ca in p.items():
+            for eb, cb in q.items():
+                k = tuple(x + y for x, y in zip(ea, eb))
+                r[k] = r.get(k, 0) + ca * cb
+        return r
+
+    def lin(i):
+        e = [0, 0, 0, 0]
+        e[i] = 1
+        return {tuple(e): Fraction(1)}
+
+    if nn == 4:
+        N = [lin(i) for i in range(4)]
+    else:
+        N = []
+        for i in range(4):  # L (2L - 1)
+            sq = mul(lin(i), lin(i))
+            N.append({k: 2 * v for k, v in sq.items()} | {tuple(lin(i))[0]: Fraction(-1)})
+        for i, j in O.T10_EDGES:
+            N.append({k: 4 * v for k, v in mul(lin(i), lin(j)).items()})
+
+    def integ(p):
+        return sum(c * Fraction(6 * math.prod(math.factorial(x) for x in e), math.factorial(sum(e) + 3))
+                   for e, c in p.items())
+
+    return np.array([[float(integ(mul(N[a], N[b]))) for b in range(nn)] for a in range(nn)])
+
+
+@pytest.mark.parametrize("nn", [10, 4])
+def test_exact_mass_coefficients(nn):
+    Cm = O.exact_mass_coefficients(nn)
+    assert np.abs(Cm - _exact_shape_mass(nn)).max() < 1e-15
+    assert abs(Cm.sum() - 1.0) < 1e-14 and np.linalg.eigvalsh(Cm).min() > 0
+
+
+def test_exact_shape_mass_route_matches_quadrature_on_degree_three():
+    # the independent route itself: integrating N_a (degree 2) times a linear function is
+    # degree 3, which the 5-point rule integrates exactly
+    qx, qy, qz, qw = O.quad_t10()
+    Cm = _exact_shape_mass(10)
+    L = np.stack([1 - qx - qy - qz, qx, qy, qz])
+    Nq = np.concatenate([L * (2 * L - 1), np.stack([4 * L[i] * L[j] for i, j in O.T10_EDGES])])
+    # sum_b N_b = 1, so row sums of C are int N_a / V
+    assert np.abs(Cm.sum(1) - 6 * Nq @ qw).max() < 1e-15
+
+
+@pytest.mark.parametrize("name", ["cube", "beam_vtu", "kuhn_t4"])
+def test_exact_mass_matrix(name):
+    m = load_mesh(name)
+    g, dj = O.grad(m)
+    off, col = O.node_pattern(m)
+    import scipy.sparse as sp
+    Mx = sp.csr_matrix((O.mass_values_exact(m, dj, RHO_STEEL, off, col), col, off)).toarray()
+    Mq = sp.csr_matrix((O.mass_values(m, dj, RHO_STEEL, off, col), col, off)).toarray()
+    X = m.xyz()[m.elems]
+    V = np.array([_tet_volume(X[e, :4]) for e in range(m.n_elem)])
+    assert np.abs(Mx - Mx.T).max() < 1e-12 * np.abs(Mx).max()
+    assert abs(Mx.sum() - RHO_STEEL * V.sum()) < 1e-12 * RHO_STEEL * V.sum()
+    # the same row sums as the quadrature rule (the integral of N_i is degree 2)
+    assert np.abs(Mx.sum(1) - Mq.sum(1)).max() < 1e-12 * np.abs(Mx.sum(1)).max()
+    ev = np.linalg.eigvalsh(Mx)
+    assert ev.min() > 0
+    if m.nn == 10:  # the 5-point rule's mass is indefinite, the exact one is not
+        assert np.linalg.eigvalsh(Mq).min() < 0
+
+
 # ---------------------------------------------------------------------------------------
 def _K(m, lam, mu, x=None, y=None, z=None, h=1.0, eta=0.0, ld=0.0):
     g, dj = O.grad(m)

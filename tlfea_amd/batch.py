@@ -91,6 +91,30 @@ class Batch:
         call("tlfea_batch_tangent_apply", self.h, ptr(p), ptr(out), self.stream)
         return out
 
+    def set_mass(self, rho_coef, rule="quadrature"):
+        """Operator K + c M per instance (rho_coef[i] = rho_i * c); None removes it.  rule
+        "quadrature": the mass of the element's quadrature rule (as the reference's mass
+        assembly, indefinite for TET10); "exact": the exactly integrated mass (positive
+        definite)."""
+        if rho_coef is None:
+            call("tlfea_batch_set_mass", self.h, None, 0)
+            self.mass_on = False
+            return
+        rules = {"quadrature": 0, "exact": 1}
+        if rule not in rules:
+            raise ValueError(f"unknown mass rule {rule!r}")
+        rc = np.ascontiguousarray(rho_coef, dtype=np.float64)
+        call("tlfea_batch_set_mass", self.h, rc.ctypes.data, rules[rule])
+        self.mass_on = True
+
+    def apply(self, p, k_scale=1.0, mass=False, masked=True, out=None):
+        """(k_scale K + c M) p (the mass term only when ``mass``); masked=False applies the
+        operator without the Dirichlet elimination."""
+        out = torch.empty(3 * self.n_nodes, dtype=torch.float64, device=DEV) if out is None else out
+        call("tlfea_batch_apply", self.h, float(k_scale), int(bool(mass)), int(bool(masked)), ptr(p), ptr(out),
+             self.stream)
+        return out
+
     def precond_setup(self, degree=6, ratio=30.0, power_iters=20, safety=1.1):
         """FP32 Jacobi-Chebyshev preconditioner; returns the lambda_max(D^-1 K) estimates."""
         lm = np.zeros(self.n_inst)

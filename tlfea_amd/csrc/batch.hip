@@ -48,6 +48,9 @@ struct tlfea_batch {
     double* d_rold = nullptr;  // previous residual (flexible CG)
     float* d_dinv32 = nullptr;  // inverse diagonal of K (1 on fixed DOFs)
     int cheb_degree = 0;        // 0 = no preconditioner set up
+    double* d_mc = nullptr;     // per-instance mass coefficient (rho * c) of the operator K + c M
+    bool mass_on = false;
+    int mass_exact = 0;         // 0: quadrature-rule mass, 1: exactly integrated mass
     // active-work lists (used once finished instances are compacted out of the launches)
     std::vector<int> h_inst_chunk_off;
     int* d_blk_list = nullptr;
@@ -125,7 +128,7 @@ __global__ __launch_bounds__(BT) void k_bgather(Chunks ch, const int* __restrict
         const T sv[3] = {s0, s1, s2};
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            const T v = mask[3 * i + d] ? p[3 * i + d] : sv[d];
+            const T v = (mask && mask[3 * i + d]) ? p[3 * i + d] : sv[d];
             q[3 * i + d] = v;
             dot += (double)p[3 * i + d] * (double)v;
         }
@@ -554,7 +557,7 @@ void free_batch(tlfea_batch* b) {
                     b->d_jinv32, b->d_detJ32, b->d_lam, b->d_mu, b->d_lam32, b->d_mu32, b->d_chunk_inst,
                     b->d_chunk_beg, b->d_chunk_end, b->d_inst_chunk_off, b->d_r, b->d_p, b->d_q, b->d_z,
                     b->d_r32, b->d_z32, b->d_d32, b->d_q32, b->d_part, b->d_s, b->d_i, b->d_all, b->d_cheb,
-                    b->d_rold, b->d_dinv32, b->d_blk_list, b->d_chunk_list, b->d_inst_list};
+                    b->d_rold, b->d_dinv32, b->d_blk_list, b->d_chunk_list, b->d_inst_list, b->d_mc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (b->s_own) (void)hipStreamDestroy(b->s_own);
@@ -651,15 +654,21 @@ int build_chunks(tlfea_batch* b, int chunk_nodes) {
     return TLFEA_OK;
 }
 
-// FP64 batched operator q = K p (masked), chunk partials of p.q into d_part.
-int apply64(tlfea_batch* b, const double* p, double* q, const int* iflags, hipStream_t s) {
-    const TanMat<double> tm{0.0, 0.0, b->d_elem_inst, b->d_lam, b->d_mu, iflags, NI};
+// FP64 batched operator q = (k_scale K + c M) p (masked; the mass term when set up),
+// chunk partials of p.q into d_part.
+int apply64(tlfea_batch* b, const double* p, double* q, const int* iflags, hipStream_t s, double k_scale = 1.0,
+            bool mass = true, bool masked = true) {
+    TanMat<double> tm{0.0, 0.0, b->d_elem_inst, b->d_lam, b->d_mu, iflags, NI};
+    tm.inst_mc = (mass && b->mass_on) ? b->d_mc : nullptr;
+    tm.mass_exact = b->mass_exact;
+    tm.k_scale = k_scale;
+    const unsigned char* mask = masked ? b->d_mask : nullptr;
     if (int rc = launch_tangent_partials<double>(b->plan, b->d_X, b->d_Y, b->d_Z, b->d_X, b->d_Y, b->d_Z, b->d_jinv,
-                                                 b->d_detJ, b->quad, tm, b->d_mask, p, nullptr, s,
+                                                 b->d_detJ, b->quad, tm, mask, p, nullptr, s,
                                                  b->compact ? b->d_blk_list : nullptr, b->n_blk_list))
         return rc;
     k_bgather<double><<<n_chunk_launch(b), BT, 0, s>>>(chunks(b), b->plan->d_g_off, b->plan->d_g_idx,
-                                                       b->plan->d_part, b->d_mask, p, q, b->d_part, iflags);
+                                                       b->plan->d_part, mask, p, q, b->d_part, iflags);
     TLFEA_LAUNCH_CHECK();
     return TLFEA_OK;
 }
@@ -830,6 +839,7 @@ extern "C" int tlfea_batch_precond_setup(tlfea_batch* b, int degree, double rati
                                          double safety, double* lambda_max, void* stream) {
     TLFEA_REQUIRE(b != nullptr, "null batch");
     TLFEA_REQUIRE(degree >= 0 && degree <= 16, "Chebyshev degree out of range [0, 16]");
+    TLFEA_REQUIRE(degree == 0 || !b->mass_on, "the Chebyshev stage covers K only (a mass term is set)");
     TLFEA_REQUIRE(degree == 0 || (ratio > 1.0 && power_iters >= 1 && safety >= 1.0), "bad Chebyshev parameters");
     b->cheb_degree = 0;
     drop_graph(b);
@@ -898,7 +908,31 @@ extern "C" int tlfea_batch_smooth(tlfea_batch* b, const double* f, double* u, in
 extern "C" int tlfea_batch_tangent_apply(tlfea_batch* b, const double* p, double* out, void* stream) {
     TLFEA_REQUIRE(b && p && out, "null argument");
     full_lists(b);
-    return apply64(b, p, out, nullptr, as_stream(stream));
+    return apply64(b, p, out, nullptr, as_stream(stream), 1.0, false);
+}
+
+extern "C" int tlfea_batch_set_mass(tlfea_batch* b, const double* rho_coef, int rule) {
+    TLFEA_REQUIRE(b != nullptr, "null batch");
+    drop_graph(b);
+    if (!rho_coef) {
+        b->mass_on = false;
+        return TLFEA_OK;
+    }
+    TLFEA_REQUIRE(rule == 0 || rule == 1, "mass rule must be 0 (quadrature) or 1 (exact)");
+    TLFEA_REQUIRE(b->cheb_degree == 0, "the Chebyshev stage covers K only: remove it before adding a mass term");
+    if (!b->d_mc && dalloc(&b->d_mc, b->n_inst)) return TLFEA_EHIP;
+    TLFEA_HIP_TRY(hipMemcpy(b->d_mc, rho_coef, sizeof(double) * b->n_inst, hipMemcpyHostToDevice));
+    b->mass_on = true;
+    b->mass_exact = rule;
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_batch_apply(tlfea_batch* b, double k_scale, int with_mass, int masked, const double* p,
+                                 double* out, void* stream) {
+    TLFEA_REQUIRE(b && p && out, "null argument");
+    TLFEA_REQUIRE(!with_mass || b->mass_on, "no mass coefficients set");
+    full_lists(b);
+    return apply64(b, p, out, nullptr, as_stream(stream), k_scale, with_mass != 0, masked != 0);
 }
 
 extern "C" int tlfea_batch_set_compaction(tlfea_batch* b, double ratio) {

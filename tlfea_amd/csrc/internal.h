@@ -49,12 +49,16 @@ int launch_force_planned(const tlfea_force_plan* p, const double* x, const doubl
 template <typename T>
 struct DnTable {
     T d[5][10][3];
+    T N[5][10];  // shape function values (for the mass term)
 };
 DnTable<double> make_dn_table(int nn, const tlfea_quad& quad);
 
 // Material of the tangent: one (lambda, mu) for all elements, or per instance through
 // elem_inst (then inst_done[inst * done_stride], when set, skips the elements of finished
-// instances).
+// instances).  The operator applied is k_scale * K + c M with c = mc (one body) or
+// inst_mc[inst] (rho times the caller's coefficient; null = no mass term) and the
+// consistent mass M of the same quadrature rule (mass_exact = 0, as the reference's
+// mass_matrix_qp_kernel) or exactly integrated on straight-sided elements (mass_exact = 1).
 template <typename T>
 struct TanMat {
     T lam, mu;
@@ -63,6 +67,10 @@ struct TanMat {
     const T* inst_mu;
     const int* inst_done;
     int done_stride;
+    T mc = T(0);
+    const T* inst_mc = nullptr;
+    T k_scale = T(1);
+    int mass_exact = 0;
 };
 
 // Block partials of the matrix-free tangent (tangent.hip) into plan->d_part (read as T),

@@ -46,14 +46,50 @@ __host__ __device__ constexpr double std_dn(int q, int a, int k) {
     return 4.0 * (L[i] * dLk[j] + L[j] * dLk[i]);
 }
 
+// Shape function values of the standard rules, as std_dn.
+template <int NN>
+__host__ __device__ constexpr double std_N(int q, int a) {
+    if (NN == 4) return 0.25;
+    const double s = 1.0 / 6.0;
+    const double px = q == 0 ? 0.25 : (q == 2 ? 0.5 : s);
+    const double py = q == 0 ? 0.25 : (q == 3 ? 0.5 : s);
+    const double pz = q == 0 ? 0.25 : (q == 4 ? 0.5 : s);
+    const double L[4] = {1.0 - px - py - pz, px, py, pz};
+    if (a < 4) return L[a] * (2.0 * L[a] - 1.0);
+    const int E[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
+    return 4.0 * L[E[a - 4][0]] * L[E[a - 4][1]];
+}
+
+// Exactly integrated shape-function mass, int N_a N_b dV / V, of a straight-sided element
+// (the integrand is degree 4 for TET10, beyond the 5-point rule).  TET10: corner-corner
+// 6 or 1, corner-edge -4 when the edge holds the corner and -6 otherwise, edge-edge 32,
+// 16 (edges sharing a corner) or 8 (opposite edges), all over 420; TET4: (1 + d_ab)/20.
+template <int NN>
+__host__ __device__ constexpr double exact_mass(int a, int b) {
+    if (NN == 4) return (a == b ? 2.0 : 1.0) / 20.0;
+    const int E[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
+    if (a > b) {
+        const int t = a;
+        a = b;
+        b = t;
+    }
+    if (b < 4) return (a == b ? 6.0 : 1.0) / 420.0;
+    if (a < 4) return (E[b - 4][0] == a || E[b - 4][1] == a ? -4.0 : -6.0) / 420.0;
+    if (a == b) return 32.0 / 420.0;
+    const int i = E[a - 4][0], j = E[a - 4][1], k = E[b - 4][0], l = E[b - 4][1];
+    return (i == k || i == l || j == k || j == l ? 16.0 : 8.0) / 420.0;
+}
+
 // One quadrature point of the tangent kernel: accumulates Q_q dN_a/dxi into fa.  With
 // STD the dN/dxi values are compile-time constants (q must then be a constant after
-// unrolling), otherwise they come from the kernel-argument table.
-template <int NN, int NQ, typename T, bool STD>
+// unrolling), otherwise they come from the kernel-argument table.  MASS: 0 no mass term,
+// 1 the mass of this quadrature rule (accumulated here), 2 the exact mass (the element
+// volume is accumulated in vol, the term is added after the last point).
+template <int NN, int NQ, typename T, bool STD, int MASS>
 __device__ __forceinline__ void tangent_qp(int q, int64_t tile, int lane, const int (&sl)[NN], const T* sU,
                                            const T* sP, int max_u, const T* __restrict__ jinv_t,
                                            const T* __restrict__ detJ_t, const DnTable<T>& dn, const tlfea_quad& quad,
-                                           T lam, T mu, T (&Mn)[9], T& dJn, T (&fa)[NN][3]) {
+                                           T lam, T mu, T mc, T ks, T (&Mn)[9], T& dJn, T& vol, T (&fa)[NN][3]) {
     // keep the node values in LDS rather than registers across quadrature points (frees
     // ~120 VGPRs in FP64, i.e. one more wave per SIMD)
     __asm__ volatile("" ::: "memory");
@@ -62,6 +98,7 @@ __device__ __forceinline__ void tangent_qp(int q, int64_t tile, int lane, const 
 #pragma unroll
     for (int c = 0; c < 9; c++) M[c / 3][c % 3] = Mn[c];
     const T dV = dJn * (T)quad.w[q];
+    if constexpr (MASS == 2) vol += dV;
     // M and detJ of the next quadrature point are loaded one iteration ahead
     if (q + 1 < NQ) {
         const T* Mp = jinv_t + (tq + 1) * (9 * TILE) + lane;
@@ -70,11 +107,17 @@ __device__ __forceinline__ void tangent_qp(int q, int64_t tile, int lane, const 
         dJn = detJ_t[(tq + 1) * TILE + lane];
     }
     T Ju[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, Jp[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    T pN[3] = {0, 0, 0};  // sum_b N_b p_b (mass term)
 #pragma unroll
     for (int a = 0; a < NN; a++) {
         const int s = sl[a];
         const T ua[3] = {sU[s], sU[max_u + s], sU[2 * max_u + s]};
         const T pa[3] = {sP[s], sP[max_u + s], sP[2 * max_u + s]};
+        if constexpr (MASS == 1) {
+            const T Na = STD ? (T)std_N<NN>(q, a) : dn.N[q][a];
+#pragma unroll
+            for (int i = 0; i < 3; i++) pN[i] += pa[i] * Na;
+        }
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             const T c = STD ? (T)std_dn<NN>(q, a, k) : dn.d[q][a][k];
@@ -124,7 +167,7 @@ __device__ __forceinline__ void tangent_qp(int q, int64_t tile, int lane, const 
         for (int j = 0; j < 3; j++)
             dP[i][j] = (dF[i][0] * S[0][j] + dF[i][1] * S[1][j] + dF[i][2] * S[2][j] + dS[i][j] + H[i][0] * dS[0][j] +
                         H[i][1] * dS[1][j] + H[i][2] * dS[2][j]) *
-                       dV;
+                       (dV * ks);
     T Q[3][3];
 #pragma unroll
     for (int i = 0; i < 3; i++)
@@ -139,9 +182,18 @@ __device__ __forceinline__ void tangent_qp(int q, int64_t tile, int lane, const 
 #pragma unroll
             for (int i = 0; i < 3; i++) fa[a][i] += Q[i][k] * c;
         }
+    if constexpr (MASS == 1) {
+        const T w = mc * dV;
+#pragma unroll
+        for (int a = 0; a < NN; a++) {
+            const T Na = (STD ? (T)std_N<NN>(q, a) : dn.N[q][a]) * w;
+#pragma unroll
+            for (int i = 0; i < 3; i++) fa[a][i] += Na * pN[i];
+        }
+    }
 }
 
-template <int NN, int NQ, typename T, bool STD>
+template <int NN, int NQ, typename T, bool STD, int MASS>
 __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent(
     int n_elem, const int64_t* __restrict__ blk_off, const int* __restrict__ unodes,
     const unsigned short* __restrict__ slot_t, const double* __restrict__ x, const double* __restrict__ y,
@@ -189,19 +241,39 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent(
     __syncthreads();
     if (active) {
         const T lam = tan_lam(mat, inst), mu = tan_mu(mat, inst);
+        const T mc = mat.elem_inst ? (mat.inst_mc ? mat.inst_mc[inst] : T(0)) : mat.mc;
+        const T ks = mat.k_scale;
         T fa[NN][3];
+        T vol = T(0);
 #pragma unroll
         for (int a = 0; a < NN; a++) fa[a][0] = fa[a][1] = fa[a][2] = T(0);
         if constexpr (STD) {
 #pragma unroll
             for (int q = 0; q < NQ; q++)
-                tangent_qp<NN, NQ, T, true>(q, tile, lane, sl, sU, sP, max_u, jinv_t, detJ_t, dn, quad, lam, mu, Mn,
-                                            dJn, fa);
+                tangent_qp<NN, NQ, T, true, MASS>(q, tile, lane, sl, sU, sP, max_u, jinv_t, detJ_t, dn, quad, lam, mu, mc,
+                                                  ks, Mn, dJn, vol, fa);
         } else {
 #pragma unroll 1
             for (int q = 0; q < NQ; q++)
-                tangent_qp<NN, NQ, T, false>(q, tile, lane, sl, sU, sP, max_u, jinv_t, detJ_t, dn, quad, lam, mu, Mn,
-                                             dJn, fa);
+                tangent_qp<NN, NQ, T, false, MASS>(q, tile, lane, sl, sU, sP, max_u, jinv_t, detJ_t, dn, quad, lam, mu,
+                                                   mc, ks, Mn, dJn, vol, fa);
+        }
+        if constexpr (MASS == 2) {
+            // exact element mass rho V C applied to the element's p, one component at a time
+            const T w = mc * vol;
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                T pb[NN];
+#pragma unroll
+                for (int c = 0; c < NN; c++) pb[c] = sP[i * max_u + sl[c]] * w;
+#pragma unroll
+                for (int a = 0; a < NN; a++) {
+                    T acc = T(0);
+#pragma unroll
+                    for (int c = 0; c < NN; c++) acc += (T)exact_mass<NN>(a, c) * pb[c];
+                    fa[a][i] += acc;
+                }
+            }
         }
 #pragma unroll
         for (int a = 0; a < NN; a++)
@@ -262,16 +334,20 @@ DnTable<double> make_dn_table(int nn, const tlfea_quad& quad) {
     for (int q = 0; q < quad.n && q < 5; q++) {
         const double L[4] = {1.0 - quad.x[q] - quad.y[q] - quad.z[q], quad.x[q], quad.y[q], quad.z[q]};
         if (nn == 4) {
-            for (int a = 0; a < 4; a++)
+            for (int a = 0; a < 4; a++) {
                 for (int k = 0; k < 3; k++) t.d[q][a][k] = dL[a][k];
+                t.N[q][a] = L[a];
+            }
             continue;
         }
+        for (int a = 0; a < 4; a++) t.N[q][a] = L[a] * (2.0 * L[a] - 1.0);
         for (int a = 0; a < 4; a++)
             for (int k = 0; k < 3; k++) t.d[q][a][k] = (4.0 * L[a] - 1.0) * dL[a][k];
         const int E[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
         for (int m = 0; m < 6; m++) {
             const int i = E[m][0], j = E[m][1];
             for (int k = 0; k < 3; k++) t.d[q][4 + m][k] = 4.0 * (L[i] * dL[j][k] + L[j] * dL[i][k]);
+            t.N[q][4 + m] = 4.0 * L[i] * L[j];
         }
     }
     return t;
@@ -287,30 +363,48 @@ int launch_tangent_partials(const tlfea_force_plan* pl, const double* x, const d
     const DnTable<double> dn64 = make_dn_table(pl->nn, quad);
     DnTable<T> dn;
     for (int q = 0; q < 5; q++)
-        for (int a = 0; a < 10; a++)
+        for (int a = 0; a < 10; a++) {
             for (int k = 0; k < 3; k++) dn.d[q][a][k] = (T)dn64.d[q][a][k];
+            dn.N[q][a] = (T)dn64.N[q][a];
+        }
     const size_t lds = sizeof(T) * (size_t)pl->max_u * 9;
     T* part = reinterpret_cast<T*>(pl->d_part);  // d_part holds 3*n_partials doubles
+    const bool has_mass = mat.elem_inst ? mat.inst_mc != nullptr : mat.mc != T(0);
+    const int mass = has_mass ? (mat.mass_exact ? 2 : 1) : 0;
     if (pl->nn == 10) {
         // the constant-folded kernel when the rule is exactly the standard 5-point one
         bool std_rule = quad.n == 5;
         for (int q = 0; q < 5 && std_rule; q++)
             for (int a = 0; a < 10; a++)
+            {
                 for (int k = 0; k < 3; k++) std_rule = std_rule && dn64.d[q][a][k] == std_dn<10>(q, a, k);
-        if (std_rule)
-            k_tangent<10, 5, T, true><<<nb, PLAN_BLOCK, lds, s>>>(pl->n_elem, pl->d_blk_off, pl->d_unodes, pl->d_slot,
-                                                                 x, y, z, X0, Y0, Z0, p, mask, jinv_t, detJ_t, dn,
-                                                                 quad, mat, part, pl->max_u, flags, blk_list);
+                std_rule = std_rule && dn64.N[q][a] == std_N<10>(q, a);
+            }
+#define TLFEA_TANGENT_LAUNCH(NN_, NQ_, STD_, MASS_)                                                          \
+    k_tangent<NN_, NQ_, T, STD_, MASS_><<<nb, PLAN_BLOCK, lds, s>>>(                                          \
+        pl->n_elem, pl->d_blk_off, pl->d_unodes, pl->d_slot, x, y, z, X0, Y0, Z0, p, mask, jinv_t, detJ_t, dn, quad, \
+        mat, part, pl->max_u, flags, blk_list)
+        if (std_rule && mass == 2)
+            TLFEA_TANGENT_LAUNCH(10, 5, true, 2);
+        else if (std_rule && mass == 1)
+            TLFEA_TANGENT_LAUNCH(10, 5, true, 1);
+        else if (std_rule)
+            TLFEA_TANGENT_LAUNCH(10, 5, true, 0);
+        else if (mass == 2)
+            TLFEA_TANGENT_LAUNCH(10, 5, false, 2);
+        else if (mass == 1)
+            TLFEA_TANGENT_LAUNCH(10, 5, false, 1);
         else
-            k_tangent<10, 5, T, false><<<nb, PLAN_BLOCK, lds, s>>>(pl->n_elem, pl->d_blk_off, pl->d_unodes,
-                                                                  pl->d_slot, x, y, z, X0, Y0, Z0, p, mask, jinv_t,
-                                                                  detJ_t, dn, quad, mat, part, pl->max_u, flags,
-                                                                  blk_list);
+            TLFEA_TANGENT_LAUNCH(10, 5, false, 0);
     } else {
-        k_tangent<4, 1, T, true><<<nb, PLAN_BLOCK, lds, s>>>(pl->n_elem, pl->d_blk_off, pl->d_unodes, pl->d_slot, x,
-                                                            y, z, X0, Y0, Z0, p, mask, jinv_t, detJ_t, dn, quad, mat,
-                                                            part, pl->max_u, flags, blk_list);
+        if (mass == 2)
+            TLFEA_TANGENT_LAUNCH(4, 1, true, 2);
+        else if (mass == 1)
+            TLFEA_TANGENT_LAUNCH(4, 1, true, 1);
+        else
+            TLFEA_TANGENT_LAUNCH(4, 1, true, 0);
     }
+#undef TLFEA_TANGENT_LAUNCH
     TLFEA_LAUNCH_CHECK();
     return TLFEA_OK;
 }
