@@ -22,8 +22,11 @@ which torch.cuda.Event records on):
                      per node);
   * spmv_fp64     -- the FP64 block SpMV of the plain path, bytes/launch =
                      76*node_nnz + 52*n_nodes;
-  * force_kernel  -- the fused FP64 P + f_int (block-local plan: force + node gather),
-                     bytes/launch = 1280*n_elem + 48*n_nodes.
+  * force_kernel  -- the fused FP64 P + f_int (block-local plan: force + node gather) with
+                     the gradients rebuilt from the rest positions (k_force_rest), on the
+                     reference byte model of a kernel that reads them, bytes/launch =
+                     1280*n_elem + 48*n_nodes (its own HBM traffic in `traffic`);
+  * force_kernel_gradients_read -- the same with the gradients read (k_force_planned).
 cpu_baseline: the FP64 oracle on a bounded sample (a slice of the element assembly and
 10 CG iterations on the full matrix), extrapolated to one solve with the plain GPU CG's
 iteration count.
@@ -53,6 +56,10 @@ HBM_PEAK_GBS = 8000.0
 # 200x20x20 beam: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (KiB), scales calibrated
 # on a 512 MiB copy
 PMC_TANGENT_BYTES = (152015 * 2.0 + 29109 + 30817 * 2.0 + 21561) * 1024
+# the same for the force kernels with k_gather_partials: gradient-reading (k_force_planned)
+# and rest-coordinate (k_force_rest)
+PMC_FORCE_BYTES = (314624 * 2.0 + 29110 + 21891 * 2.0 + 15799) * 1024
+PMC_FORCE_REST_BYTES = (40677 * 2.0 + 29109 + 21894 * 2.0 + 15799) * 1024
 
 
 def log(*a):
@@ -185,7 +192,7 @@ def main():
     ap.add_argument("--dynamic-only", action="store_true", help="only the dynamic batched measurement")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one launch each of the planned force kernel, the block SpMV, the matrix-free "
-                         "tangent and a 512 MiB copy "
+                         "tangent, the rest-coordinate force kernel and a 512 MiB copy "
                          "(the copy calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE), then exit")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -212,6 +219,7 @@ def main():
         b.internal_force_planned(f=f)
         b.spmv(k, p, y)
         b.tangent_apply(p, out=y)
+        b.internal_force_rest(f=f)
         src = torch.randn(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
@@ -290,11 +298,18 @@ def main():
     spmv_bytes = 76 * node_nnz + 52 * m.n_nodes
     spmv_gbs = spmv_bytes / spmv_s / 1e9
     f = torch.zeros(3 * m.n_nodes, dtype=torch.float64, device="cuda")
-    force_s = event_time(lambda: body.internal_force_planned(f=f))
+    # fused P + f_int on the reference byte model: connectivity 40 + gradients 1200 + detJ 40 B
+    # per element, x read and f written (48 B) per node -- for the kernel that reads the
+    # gradients and, as that model is stated for a kernel that rebuilds them, for the
+    # rest-coordinate kernel too
     force_bytes = 1280 * m.n_elem + 48 * m.n_nodes
+    force_s = event_time(lambda: body.internal_force_planned(f=f))
     force_gbs = force_bytes / force_s / 1e9
+    rest_s = event_time(lambda: body.internal_force_rest(f=f))
+    rest_gbs = force_bytes / rest_s / 1e9
     log(f"tangent {tan_s * 1e6:.1f} us {tan_gbs:.0f} GB/s; spmv64 {spmv_s * 1e6:.1f} us {spmv_gbs:.0f} GB/s; "
-        f"force {force_s * 1e6:.1f} us {force_gbs:.0f} GB/s")
+        f"force (gradients read) {force_s * 1e6:.1f} us {force_gbs:.0f} GB/s; force (rest coordinates) "
+        f"{rest_s * 1e6:.1f} us {rest_gbs:.0f} GB/s")
 
     # ---- CPU baseline: oracle on a bounded sample -------------------------------------
     cpu = None
@@ -366,9 +381,15 @@ def main():
         "spmv_fp64": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                       "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_cg_spmv/k_spmv_block",
                       "avg_us": spmv_s * 1e6, "bytes_per_launch": spmv_bytes},
-        "force_kernel": {"bound": "hbm", "achieved": force_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": force_gbs / HBM_PEAK_GBS, "kernel": "k_force_planned<10,5,false> + k_gather_partials",
-                         "avg_us": force_s * 1e6, "bytes_per_launch": force_bytes},
+        "force_kernel": {"bound": "hbm", "achieved": rest_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": rest_gbs / HBM_PEAK_GBS, "traffic": PMC_FORCE_REST_BYTES if default else None,
+                         "kernel": "k_force_rest<10,5,false> + k_gather_partials (gradients rebuilt from the rest "
+                                   "positions; achieved on the reference byte model of the gradient-reading kernel)",
+                         "avg_us": rest_s * 1e6, "bytes_per_launch": force_bytes},
+        "force_kernel_gradients_read": {"bound": "hbm", "achieved": force_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                        "frac": force_gbs / HBM_PEAK_GBS, "traffic": PMC_FORCE_BYTES if default else None,
+                                        "kernel": "k_force_planned<10,5,false> + k_gather_partials",
+                                        "avg_us": force_s * 1e6, "bytes_per_launch": force_bytes},
         "cpu_baseline": cpu,
         "batched": batched,
         "dynamic": dynamic,

@@ -110,6 +110,16 @@ int tlfea_force_plan_sizes(const tlfea_force_plan* p, int* block_elems, int* n_b
 int tlfea_internal_force_planned(const tlfea_force_plan* plan, const double* x, const double* y, const double* z,
                                  const double* grad_t, const double* detJ_t, const tlfea_quad* quad,
                                  const tlfea_material* mat, const double* v, double* f, int accumulate, void* stream);
+/* CalcInternalForce (src/elements/FEAT10Data.cu:307-323, FEAT4Data.cu) with the shape-function
+ * gradients of CalcDnDuPre (FEAT10Data.cu:92-198) rebuilt per quadrature point from the
+ * rest positions X0, Y0, Z0 instead of read from tlfea_precompute_grad's arrays: the same
+ * f_int as tlfea_internal_force_planned (the strain from the displacement gradient x - X,
+ * so at small strains it is the more accurate of the two).  Standard rules only (TET10:
+ * 5 points, TET4: 1 point; TLFEA_EINVAL otherwise); TLFEA_ECAPACITY when a plan block
+ * touches more nodes than 9 (12 with damping) LDS arrays of doubles hold. */
+int tlfea_internal_force_rest(const tlfea_force_plan* plan, const double* x, const double* y, const double* z,
+                              const double* X0, const double* Y0, const double* Z0, const tlfea_quad* quad,
+                              const tlfea_material* mat, const double* v, double* f, int accumulate, void* stream);
 
 /* Matrix-free tangent action y = K(x) p, the operator that compute_hessian_assemble_csr +
  * SolveLinearSystemCG's SpMV apply (src/elements/FEAT10DataFunc.cuh:512-768,
@@ -376,6 +386,10 @@ typedef struct tlfea_body {
     double* f_int;                /* [3*n_nodes] */
     double* constraint;           /* [3*n_fixed] */
     const tlfea_force_plan* force_plan; /* optional; NULL = atomic-scatter force kernel */
+    /* optional, with force_plan: the positions the gradients were computed from (CalcDnDuPre);
+     * when set, f_int is evaluated by tlfea_internal_force_rest's kernel (gradients rebuilt
+     * per quadrature point, grad_t not read) */
+    const double *X0, *Y0, *Z0;
 } tlfea_body;
 
 /* Byte-compatible with the reference's SyncedNesterovParams (SyncedNesterov.cuh:31-35). */

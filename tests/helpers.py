@@ -56,3 +56,57 @@ def rel_err(a, b):
     b = np.asarray(b, dtype=np.float64)
     nb = np.linalg.norm(b)
     return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def _inv3(J):
+    """3x3 inverses and determinants of a stack, by cofactors (works in longdouble)."""
+    c = np.empty_like(J)
+    c[:, 0, 0] = J[:, 1, 1] * J[:, 2, 2] - J[:, 1, 2] * J[:, 2, 1]
+    c[:, 0, 1] = J[:, 0, 2] * J[:, 2, 1] - J[:, 0, 1] * J[:, 2, 2]
+    c[:, 0, 2] = J[:, 0, 1] * J[:, 1, 2] - J[:, 0, 2] * J[:, 1, 1]
+    c[:, 1, 0] = J[:, 1, 2] * J[:, 2, 0] - J[:, 1, 0] * J[:, 2, 2]
+    c[:, 1, 1] = J[:, 0, 0] * J[:, 2, 2] - J[:, 0, 2] * J[:, 2, 0]
+    c[:, 1, 2] = J[:, 0, 2] * J[:, 1, 0] - J[:, 0, 0] * J[:, 1, 2]
+    c[:, 2, 0] = J[:, 1, 0] * J[:, 2, 1] - J[:, 1, 1] * J[:, 2, 0]
+    c[:, 2, 1] = J[:, 0, 1] * J[:, 2, 0] - J[:, 0, 0] * J[:, 2, 1]
+    c[:, 2, 2] = J[:, 0, 0] * J[:, 1, 1] - J[:, 0, 1] * J[:, 1, 0]
+    det = J[:, 0, 0] * c[:, 0, 0] + J[:, 0, 1] * c[:, 1, 0] + J[:, 0, 2] * c[:, 2, 0]
+    return c / det[:, None, None], det
+
+
+def fint_extended(m, x, y, z, lam, mu):
+    """SVK internal force (compute_internal_force, FEAT10DataFunc.cuh:404-460, with the
+    gradients of FEAT10Data.cu:92-198) in numpy longdouble (64-bit mantissa), with the
+    strain from the displacement gradient: a reference for small-strain states, where
+    forming F^T F - I in double loses digits."""
+    import oracle as O
+    ld = np.longdouble
+    qx, qy, qz, qw = O.quad(m.nn)
+    X = m.xyz()[m.elems].astype(ld)
+    xc = np.stack([x, y, z], axis=1)[m.elems].astype(ld)
+    u = xc - X  # exact: differences of doubles fit the longer mantissa here
+    f = np.zeros((m.n_nodes, 3), dtype=ld)
+    lam, mu = ld(lam), ld(mu)
+    edges = ((0, 1), (1, 2), (0, 2), (0, 3), (1, 3), (2, 3))
+    for q in range(len(qw)):
+        Lq = [ld(1) - ld(qx[q]) - ld(qy[q]) - ld(qz[q]), ld(qx[q]), ld(qy[q]), ld(qz[q])]
+        dL = np.array([[-1, -1, -1], [1, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=ld)
+        if m.nn == 4:
+            dN = dL
+        else:
+            dN = np.zeros((10, 3), dtype=ld)
+            for a in range(4):
+                dN[a] = (4 * Lq[a] - 1) * dL[a]
+            for k, (i, j) in enumerate(edges):
+                dN[4 + k] = 4 * (Lq[i] * dL[j] + Lq[j] * dL[i])
+        J = np.einsum("eai,ak->eik", X, dN)
+        Ji, det = _inv3(J)
+        g = np.einsum("ak,ekj->eaj", dN, Ji)
+        H = np.einsum("eai,eaj->eij", u, g)
+        Em = (H + H.transpose(0, 2, 1) + np.einsum("emi,emj->eij", H, H)) / 2
+        trE = Em[:, 0, 0] + Em[:, 1, 1] + Em[:, 2, 2]
+        S = 2 * mu * Em + lam * trE[:, None, None] * np.eye(3, dtype=ld)
+        P = S + np.einsum("eik,ekj->eij", H, S)
+        fe = np.einsum("eij,eaj->eai", P, g) * (det * ld(qw[q]))[:, None, None]
+        np.add.at(f, m.elems, fe)
+    return f.reshape(-1)

@@ -64,10 +64,10 @@ def test_calc_p_and_fint_from_p(gpu, name):
 
 @pytest.mark.parametrize("name", MESHES)
 @pytest.mark.parametrize("damp", [False, True])
-@pytest.mark.parametrize("planned", [False, True])
-def test_fused_internal_force(gpu, name, damp, planned):
+@pytest.mark.parametrize("kind", ["atomic", "planned", "rest"])
+def test_fused_internal_force(gpu, name, damp, kind):
     import torch
-    print(f"[fused force] {name} damp={damp} planned={planned}", flush=True)
+    print(f"[fused force] {name} damp={damp} kind={kind}", flush=True)
     m = load_mesh(name)
     eta, ld = (1e6, 1e5) if damp else (0.0, 0.0)
     b = _body(m, eta=eta, lambda_d=ld)
@@ -75,7 +75,14 @@ def test_fused_internal_force(gpu, name, damp, planned):
     xt = [torch.as_tensor(a).cuda() for a in (x, y, z)]
     v = np.random.default_rng(3).standard_normal(3 * m.n_nodes) * 1e-2
     vt = torch.as_tensor(v).cuda()
-    force = b.internal_force_planned if planned else b.internal_force
+    force = {"atomic": b.internal_force, "planned": b.internal_force_planned, "rest": b.internal_force_rest}[kind]
+    planned = kind != "atomic"
+    if kind == "rest" and b.plan_sizes()["max_block_nodes"] * (12 if damp else 9) * 8 > 160 * 1024:
+        # an unordered mesh whose blocks touch too many nodes for the staged arrays
+        from tlfea_amd.capi import TlfeaError
+        with pytest.raises(TlfeaError, match="LDS budget"):
+            force(x=xt, v=vt if damp else None)
+        return
     f = force(x=xt, v=vt if damp else None)
     go, djo = O.grad(m)
     lam, mu = lame(E_STEEL, NU_STEEL)
@@ -87,6 +94,33 @@ def test_fused_internal_force(gpu, name, damp, planned):
     # accumulate flag adds onto the existing vector
     force(x=xt, v=vt if damp else None, f=f, accumulate=True)
     assert rel_err(_np(f), 2 * fo) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["cube", "C1", "kuhn_t4"])
+def test_rest_form_force_at_small_strain(gpu, name):
+    """At strains near 1e-6, F^T F - I in double (the reference's form, followed by the
+    oracle and by the gradient-reading kernels) keeps only ~6 significant digits of E's
+    cancellation; the rest-coordinate kernel forms E from the displacement gradient.
+    Both are compared with a longdouble evaluation (tests/helpers.py: fint_extended, itself
+    checked against the oracle at large strain)."""
+    import torch
+    from helpers import fint_extended
+    m = load_mesh(name)
+    b = _body(m)
+    rng = np.random.default_rng(5)
+    L = np.ptp(m.xyz(), axis=0).max()
+    u = 1e-7 * L * rng.standard_normal((m.n_nodes, 3))
+    x, y, z = m.x + u[:, 0], m.y + u[:, 1], m.z + u[:, 2]
+    xt = [torch.as_tensor(a).cuda() for a in (x, y, z)]
+    lam, mu = lame(E_STEEL, NU_STEEL)
+    fe = fint_extended(m, x, y, z, lam, mu).astype(np.float64)
+    xd = deformed(m, seed=2)
+    # the longdouble restatement agrees with the oracle where both are accurate
+    assert rel_err(fint_extended(m, *xd, lam, mu).astype(np.float64), O.internal_force(m, *O.grad(m), lam, mu, *xd)) < 1e-12
+    er = rel_err(_np(b.internal_force_rest(x=xt)), fe)
+    ep = rel_err(_np(b.internal_force_planned(x=xt)), fe)
+    print(f"[small strain] {name}: rest form {er:.1e}, gradient-reading form {ep:.1e}", flush=True)
+    assert er < 1e-12
 
 
 def test_fused_force_rest_state_is_zero(gpu):
@@ -107,11 +141,13 @@ def test_fused_force_c2_full_size(gpu):
     xt = [torch.as_tensor(a).cuda() for a in (x, y, z)]
     f = _np(b.internal_force(x=xt))
     fp = _np(b.internal_force_planned(x=xt))
+    fr = _np(b.internal_force_rest(x=xt))
     go, djo = O.grad(m)
     lam, mu = lame(E_STEEL, NU_STEEL)
     fo = O.internal_force(m, go, djo, lam, mu, x, y, z)
     assert rel_err(f, fo) < 1e-12
     assert rel_err(fp, fo) < 1e-12
+    assert rel_err(fr, fo) < 1e-12
     ps = b.plan_sizes()
     print(f"[plan] {ps}", flush=True)
     assert ps["n_blocks"] == (m.n_elem + 255) // 256

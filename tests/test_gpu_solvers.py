@@ -21,13 +21,13 @@ pytestmark = pytest.mark.gpu
 
 
 def _setup(name, kind, E=E_STEEL, nu=NU_STEEL, rho=RHO_STEEL, eta=0.0, lambda_d=0.0, planned=True,
-           random_load=False):
+           random_load=False, rest=False):
     from tlfea_amd.fea import Body, Solver
     m = load_mesh(name)
     if random_load:  # loads on every DOF keep every gradient entry far above rounding noise
         m.f_ext = np.random.default_rng(21).standard_normal(3 * m.n_nodes) * 50.0
     b = Body(m, E, nu, rho, eta, lambda_d)
-    s = Solver(b, kind, planned=planned)
+    s = Solver(b, kind, planned=planned, rest=rest)
     prob = O.Problem(m, E, nu, rho, eta=eta, lambda_d=lambda_d)
     return m, b, s, prob
 
@@ -53,13 +53,16 @@ def _compare(m, b, s, prob, st, tol=1e-9):
         assert rel_err(s.lambda_guess().cpu().numpy(), st["lam"]) < tol
 
 
-@pytest.mark.parametrize("name,damp,planned", [("cube", False, True), ("C1", False, True), ("C1", True, True),
-                                               ("C1", False, False), ("kuhn_t4", False, True)])
-def test_nesterov_trajectory(gpu, name, damp, planned):
+@pytest.mark.parametrize("name,damp,force", [("cube", False, "planned"), ("C1", False, "planned"),
+                                             ("C1", True, "planned"), ("C1", False, "atomic"),
+                                             ("kuhn_t4", False, "planned"), ("C1", False, "rest"), ("C1", True, "rest"),
+                                             ("kuhn_t4", False, "rest")])
+def test_nesterov_trajectory(gpu, name, damp, force):
     from tlfea_amd import capi
-    print(f"[nesterov] {name} damp={damp} planned={planned}", flush=True)
+    print(f"[nesterov] {name} damp={damp} force={force}", flush=True)
     eta, ld = (2e5, 1e5) if damp else (0.0, 0.0)
-    m, b, s, prob = _setup(name, capi.SOLVER_NESTEROV, eta=eta, lambda_d=ld, planned=planned, random_load=True)
+    m, b, s, prob = _setup(name, capi.SOLVER_NESTEROV, eta=eta, lambda_d=ld, planned=force != "atomic",
+                           random_load=True, rest=force == "rest")
     alpha = 2e-10 if name == "cube" else 5e-9  # stable step for steel on each mesh
     p = capi.NesterovParams(alpha=alpha, rho=1e10, inner_tol=0.0, outer_tol=0.0, max_outer=2, max_inner=25,
                             time_step=1e-3)
@@ -92,11 +95,12 @@ def test_nesterov_stopping_rules(gpu):
 
 @pytest.mark.parametrize("nocoop", [False, True])
 @pytest.mark.parametrize("name", ["cube", "C1"])
-def test_adamw_trajectory(gpu, nocoop, name):
+@pytest.mark.parametrize("rest", [False, True])
+def test_adamw_trajectory(gpu, nocoop, name, rest):
     from tlfea_amd import capi
-    print(f"[adamw] {name} nocoop={nocoop}", flush=True)
+    print(f"[adamw] {name} nocoop={nocoop} rest={rest}", flush=True)
     kind = capi.SOLVER_ADAMW_NOCOOP if nocoop else capi.SOLVER_ADAMW
-    m, b, s, prob = _setup(name, kind, eta=1e4, lambda_d=1e3, random_load=True)
+    m, b, s, prob = _setup(name, kind, eta=1e4, lambda_d=1e3, random_load=True, rest=rest)
     p = capi.AdamWParams(lr=2e-4, beta1=0.9, beta2=0.999, eps=1e-8, weight_decay=1e-4, lr_decay=0.995,
                          inner_tol=0.0, outer_tol=0.0, rho=1e10, max_outer=2, max_inner=30, time_step=1e-3,
                          convergence_check_interval=10, inner_rtol=0.0)

@@ -549,3 +549,20 @@ def test_nesterov_trajectory_matches_python_restatement():
             g[3 * n:3 * n + 3] += dt * rho * (x[n] - X[n])
         vkm1, vk, t = vk, y - alpha * g, tn
     assert rel_err(st["v_guess"], vk) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["cube", "kuhn_t4"])
+def test_extended_precision_force_restatement(name):
+    # the longdouble restatement used to judge small-strain accuracy agrees with the oracle
+    # where both are accurate (large deformation), and the oracle's own F^T F - I form loses
+    # digits at strains near 1e-6 (what the longdouble reference is for)
+    from helpers import fint_extended
+    m = load_mesh(name)
+    lam, mu = lame(E_STEEL, NU_STEEL)
+    g, dj = O.grad(m)
+    xd = deformed(m, seed=2)
+    assert rel_err(fint_extended(m, *xd, lam, mu).astype(np.float64), O.internal_force(m, g, dj, lam, mu, *xd)) < 1e-12
+    u = 1e-7 * np.ptp(m.xyz(), axis=0).max() * np.random.default_rng(5).standard_normal((m.n_nodes, 3))
+    xs = (m.x + u[:, 0], m.y + u[:, 1], m.z + u[:, 2])
+    e = rel_err(O.internal_force(m, g, dj, lam, mu, *xs), fint_extended(m, *xs, lam, mu).astype(np.float64))
+    assert 1e-13 < e < 1e-8

@@ -42,7 +42,8 @@ class Body(C.Structure):
                 ("detJ_t", C.c_void_p), ("quad", Quad), ("mat", Material), ("pattern", C.c_void_p),
                 ("m_sell", C.c_void_p), ("f_ext", C.c_void_p), ("n_fixed", C.c_int), ("fixed_nodes", C.c_void_p),
                 ("xref", C.c_void_p), ("yref", C.c_void_p), ("zref", C.c_void_p), ("f_int", C.c_void_p),
-                ("constraint", C.c_void_p), ("force_plan", C.c_void_p)]
+                ("constraint", C.c_void_p), ("force_plan", C.c_void_p), ("X0", C.c_void_p), ("Y0", C.c_void_p),
+                ("Z0", C.c_void_p)]
 
 
 class NesterovParams(C.Structure):
@@ -83,6 +84,8 @@ _SIGS = {
     "tlfea_force_plan_sizes": (i32, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i64), C.POINTER(i32)]),
     "tlfea_internal_force_planned": (i32, [vp, vp, vp, vp, vp, vp, C.POINTER(Quad), C.POINTER(Material), vp, vp, i32,
                                            vp]),
+    "tlfea_internal_force_rest": (i32, [vp, vp, vp, vp, vp, vp, vp, C.POINTER(Quad), C.POINTER(Material), vp, vp, i32,
+                                        vp]),
     "tlfea_tangent_apply_planned": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(Quad), C.POINTER(Material),
                                           vp, vp, vp, vp]),
     "tlfea_precompute_jinv": (i32, [i32, i32, vp, vp, vp, vp, C.POINTER(Quad), vp, vp]),

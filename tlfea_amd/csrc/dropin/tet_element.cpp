@@ -62,6 +62,8 @@ struct TetState {
     int* conn = nullptr;
     double *x = nullptr, *y = nullptr, *z = nullptr;
     double *xj = nullptr, *yj = nullptr, *zj = nullptr;  // constraint targets (*_jac)
+    double *x0 = nullptr, *y0 = nullptr, *z0 = nullptr;  // positions at the last CalcDnDuPre
+    bool have_rest = false;
     double *grad_t = nullptr, *detJ_t = nullptr;
     double *F_t = nullptr, *P_t = nullptr, *Fd_t = nullptr, *Pv_t = nullptr;
     double *f_int = nullptr, *f_ext = nullptr;
@@ -103,6 +105,9 @@ void TetElementData::Initialize() {
     s.xj = dalloc<double>(n_coef);
     s.yj = dalloc<double>(n_coef);
     s.zj = dalloc<double>(n_coef);
+    s.x0 = dalloc<double>(n_coef);
+    s.y0 = dalloc<double>(n_coef);
+    s.z0 = dalloc<double>(n_coef);
     s.grad_t = dalloc<double>(ne_pad * nq_ * nn_ * 3);
     s.detJ_t = dalloc<double>(ne_pad * nq_);
     s.F_t = dalloc<double>(ne_pad * nq_ * 9);
@@ -283,6 +288,12 @@ void TetElementData::CalcDnDuPre() {
     TetState& s = *s_;
     lib_check(tlfea_precompute_grad(nn_, n_elem, s.conn, s.x, s.y, s.z, &s.quad, s.grad_t, s.detJ_t, nullptr),
               "CalcDnDuPre");
+    // the positions the gradients come from, for the force kernel that rebuilds them
+    for (int d = 0; d < 3; d++)
+        hip_check(hipMemcpy(d == 0 ? s.x0 : d == 1 ? s.y0 : s.z0, d == 0 ? s.x : d == 1 ? s.y : s.z,
+                            sizeof(double) * n_coef, hipMemcpyDeviceToDevice),
+                  "CalcDnDuPre");
+    s.have_rest = true;
     hip_check(hipDeviceSynchronize(), "CalcDnDuPre");
 }
 
@@ -538,13 +549,26 @@ tlfea_body TetElementData::BodyDescriptor() {
     b.f_int = s.f_int;
     b.constraint = s.constraint;
     b.force_plan = s.plan;
+    // gradients rebuilt from the CalcDnDuPre positions when the plan's blocks fit the LDS
+    // budget of that kernel (9 arrays of doubles per block node, 12 with damping)
+    if (s.plan && s.have_rest) {
+        int block_elems = 0, n_blocks = 0, max_u = 0;
+        int64_t n_partials = 0;
+        const bool damp = s.mat.eta_damp != 0.0 || s.mat.lambda_damp != 0.0;
+        if (tlfea_force_plan_sizes(s.plan, &block_elems, &n_blocks, &n_partials, &max_u) == TLFEA_OK &&
+            (size_t)max_u * (damp ? 12 : 9) * sizeof(double) <= 160 * 1024) {
+            b.X0 = s.x0;
+            b.Y0 = s.y0;
+            b.Z0 = s.z0;
+        }
+    }
     return b;
 }
 
 // FEAT10Data.cuh:659-723
 void TetElementData::Destroy() {
     TetState& s = *s_;
-    for (double** p : {&s.x, &s.y, &s.z, &s.xj, &s.yj, &s.zj, &s.grad_t, &s.detJ_t, &s.F_t, &s.P_t, &s.Fd_t, &s.Pv_t,
+    for (double** p : {&s.x, &s.y, &s.z, &s.xj, &s.yj, &s.zj, &s.x0, &s.y0, &s.z0, &s.grad_t, &s.detJ_t, &s.F_t, &s.P_t, &s.Fd_t, &s.Pv_t,
                        &s.f_int, &s.f_ext, &s.constraint, &s.m_sell, &s.j_val, &s.jt_val})
         dfree(*p);
     for (int** p : {&s.conn, &s.fixed, &s.j_off, &s.j_col, &s.jt_off, &s.jt_col}) dfree(*p);
@@ -553,6 +577,7 @@ void TetElementData::Destroy() {
     s.pattern = nullptr;
     s.plan = nullptr;
     s.plan_tried = false;
+    s.have_rest = false;
     s.initialized = false;
     is_setup = is_constraints_setup = is_csr_setup = is_cj_csr_setup = is_j_csr_setup = false;
 }

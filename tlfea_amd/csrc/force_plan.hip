@@ -15,6 +15,7 @@
 #include <cstring>
 
 #include "internal.h"
+#include "shape.h"
 
 namespace tlfea_hip {
 
@@ -113,6 +114,175 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_force_planned(
     }
 }
 
+// Planned force with the shape-function gradients rebuilt from the rest coordinates
+// instead of read (standard rules only).  Per quadrature point q, with compile-time
+// dN_a/dxi and the block's rest positions X and displacements u = x - X staged in LDS:
+//     J = sum_a X_a (x) dN_a/dxi,  g_a = J^-T dN_a/dxi,  dV = det J w_q,
+//     H = (sum_a u_a (x) dN_a/dxi) J^-1 = F - I,  E = (H + H^T + H^T H) / 2,
+//     S = lambda tr(E) I + 2 mu E,  P = (I + H) S  (+ Kelvin-Voigt from Fdot = sum v_a (x) g_a),
+//     f_a += P g_a dV = (dV P J^-T) dN_a/dxi.
+// The same f_int as compute_internal_force (FEAT10DataFunc.cuh:404-460) with the gradients
+// of FEAT10Data.cu:92-198: per element only 10 u16 slots are read from HBM (the gradient
+// stream of k_force_planned is 1200 B per TET10 element), and E comes from the
+// displacement gradient, without the cancellation of F^T F - I at small strains.
+template <int NN, int NQ, bool DAMP>
+__global__ __launch_bounds__(PLAN_BLOCK) void k_force_rest(
+    int n_elem, const int64_t* __restrict__ blk_off, const int* __restrict__ unodes,
+    const unsigned short* __restrict__ slot_t, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const double* __restrict__ X0, const double* __restrict__ Y0,
+    const double* __restrict__ Z0, const double* __restrict__ v, tlfea_quad quad, tlfea_material mat,
+    double* __restrict__ part, int max_u, const int* __restrict__ flags) {
+    if (flags && (flags[0] | flags[1])) return;
+    extern __shared__ double lds[];
+    double* sR = lds;              // [3][max_u] rest positions
+    double* sU = lds + 3 * max_u;  // [3][max_u] displacements
+    double* sF = lds + 6 * max_u;  // [3][max_u] accumulated forces
+    double* sV = lds + 9 * max_u;  // [3][max_u] velocities (DAMP only)
+    const int b = blockIdx.x;
+    const int64_t u0 = blk_off[b];
+    const int U = (int)(blk_off[b + 1] - u0);
+    const int e = b * PLAN_BLOCK + threadIdx.x;
+    const int tile = e >> 6, lane = e & (TILE - 1);
+    int sl[NN];
+    if (e < n_elem) {
+#pragma unroll
+        for (int a = 0; a < NN; a++) sl[a] = slot_t[((int64_t)tile * NN + a) * TILE + lane];
+    }
+    for (int u = threadIdx.x; u < U; u += PLAN_BLOCK) {
+        const int n = unodes[u0 + u];
+        const double X = X0[n], Y = Y0[n], Z = Z0[n];
+        sR[u] = X;
+        sR[max_u + u] = Y;
+        sR[2 * max_u + u] = Z;
+        sU[u] = x[n] - X;
+        sU[max_u + u] = y[n] - Y;
+        sU[2 * max_u + u] = z[n] - Z;
+        sF[u] = sF[max_u + u] = sF[2 * max_u + u] = 0.0;
+        if constexpr (DAMP) {
+            sV[u] = v[3 * n];
+            sV[max_u + u] = v[3 * n + 1];
+            sV[2 * max_u + u] = v[3 * n + 2];
+        }
+    }
+    __syncthreads();
+    if (e < n_elem) {
+        double fa[NN][3];
+#pragma unroll
+        for (int a = 0; a < NN; a++) fa[a][0] = fa[a][1] = fa[a][2] = 0.0;
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            // node values are re-read from LDS at every quadrature point rather than held
+            // in registers across them
+            __asm__ volatile("" ::: "memory");
+            double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, Ju[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+            double Jv[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+            for (int a = 0; a < NN; a++) {
+                const int s = sl[a];
+                const double Xa[3] = {sR[s], sR[max_u + s], sR[2 * max_u + s]};
+                const double ua[3] = {sU[s], sU[max_u + s], sU[2 * max_u + s]};
+                double va[3] = {0, 0, 0};
+                if constexpr (DAMP) {
+                    va[0] = sV[s];
+                    va[1] = sV[max_u + s];
+                    va[2] = sV[2 * max_u + s];
+                }
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const double c = std_dn<NN>(q, a, k);
+                    if (c == 0.0) continue;
+#pragma unroll
+                    for (int i = 0; i < 3; i++) {
+                        J[i][k] += Xa[i] * c;
+                        Ju[i][k] += ua[i] * c;
+                        if constexpr (DAMP) Jv[i][k] += va[i] * c;
+                    }
+                }
+            }
+            // J^-1 by cofactors
+            const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+            const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+            const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+            const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+            const double id = 1.0 / det;
+            double Ji[3][3];  // J^-1
+            Ji[0][0] = c00 * id;
+            Ji[1][0] = c01 * id;
+            Ji[2][0] = c02 * id;
+            Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+            Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+            Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+            Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+            Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+            Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+            double H[3][3];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) H[i][j] = Ju[i][0] * Ji[0][j] + Ju[i][1] * Ji[1][j] + Ju[i][2] * Ji[2][j];
+            double S[3][3];  // E first, then S in place
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+                    S[i][j] = 0.5 * (H[i][j] + H[j][i] + H[0][i] * H[0][j] + H[1][i] * H[1][j] + H[2][i] * H[2][j]);
+            const double ltrE = mat.lambda * (S[0][0] + S[1][1] + S[2][2]);
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) S[i][j] = 2.0 * mat.mu * S[i][j] + (i == j ? ltrE : 0.0);
+            double P[3][3];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) P[i][j] = S[i][j] + H[i][0] * S[0][j] + H[i][1] * S[1][j] + H[i][2] * S[2][j];
+            if constexpr (DAMP) {
+                double F[3][3], Fd[3][3], Pv[3][3];
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++) {
+                        F[i][j] = H[i][j] + (i == j ? 1.0 : 0.0);
+                        Fd[i][j] = Jv[i][0] * Ji[0][j] + Jv[i][1] * Ji[1][j] + Jv[i][2] * Ji[2][j];
+                    }
+                kv_stress(F, Fd, mat.eta_damp, mat.lambda_damp, Pv);
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++) P[i][j] += Pv[i][j];
+            }
+            // Q = dV P J^-T, f_a += Q dN_a/dxi
+            const double dV = det * quad.w[q];
+            double Q[3][3];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int k = 0; k < 3; k++)
+                    Q[i][k] = (P[i][0] * Ji[k][0] + P[i][1] * Ji[k][1] + P[i][2] * Ji[k][2]) * dV;
+#pragma unroll
+            for (int a = 0; a < NN; a++)
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const double c = std_dn<NN>(q, a, k);
+                    if (c == 0.0) continue;
+#pragma unroll
+                    for (int i = 0; i < 3; i++) fa[a][i] += Q[i][k] * c;
+                }
+        }
+#pragma unroll
+        for (int a = 0; a < NN; a++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) atomicAdd(&sF[i * max_u + sl[a]], fa[a][i]);
+    }
+    __syncthreads();
+    for (int u = threadIdx.x; u < U; u += PLAN_BLOCK) {
+        double* o = part + 3 * (u0 + u);
+        o[0] = sF[u];
+        o[1] = sF[max_u + u];
+        o[2] = sF[2 * max_u + u];
+    }
+}
+
 // f[3i+d] (+)= sum over node i's partials, in ascending partial order (deterministic).
 __global__ void k_gather_partials(int n_nodes, const int* __restrict__ g_off, const int64_t* __restrict__ g_idx,
                                   const double* __restrict__ part, double* __restrict__ f, int accumulate,
@@ -165,6 +335,49 @@ int launch_force_planned(const tlfea_force_plan* p, const double* x, const doubl
                                                                          p->d_slot, x, y, z, nullptr, grad_t, detJ_t,
                                                                          quad, mat, p->d_part, p->max_u, flags);
         }
+        TLFEA_LAUNCH_CHECK();
+    }
+    k_gather_partials<<<grid_for(p->n_nodes, 256), 256, 0, s>>>(p->n_nodes, p->d_g_off, p->d_g_idx, p->d_part, f,
+                                                                 accumulate, flags);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
+bool is_standard_rule(int nn, const tlfea_quad& quad) {
+    if (quad.n != (nn == 10 ? 5 : 1)) return false;
+    if (nn == 4) return true;  // TET4 gradients are constant: any single point
+    const DnTable<double> dn = make_dn_table(nn, quad);
+    for (int q = 0; q < 5; q++)
+        for (int a = 0; a < 10; a++)
+            for (int k = 0; k < 3; k++)
+                if (dn.d[q][a][k] != std_dn<10>(q, a, k)) return false;
+    return true;
+}
+
+int launch_force_rest(const tlfea_force_plan* p, const double* x, const double* y, const double* z,
+                      const double* X0, const double* Y0, const double* Z0, const tlfea_quad& quad,
+                      const tlfea_material& mat, const double* v, double* f, int accumulate, const int* flags,
+                      hipStream_t s) {
+    const bool damp = (v != nullptr) && (mat.eta_damp != 0.0 || mat.lambda_damp != 0.0);
+    const size_t lds = sizeof(double) * (size_t)p->max_u * (damp ? 12 : 9);
+    const unsigned nb = (unsigned)p->n_blocks;
+    if (nb > 0) {
+#define TLFEA_FORCE_REST(NN_, NQ_, D_)                                                                            \
+    k_force_rest<NN_, NQ_, D_><<<nb, PLAN_BLOCK, lds, s>>>(p->n_elem, p->d_blk_off, p->d_unodes, p->d_slot, x, y, z, \
+                                                           X0, Y0, Z0, D_ ? v : nullptr, quad, mat, p->d_part,       \
+                                                           p->max_u, flags)
+        if (p->nn == 10) {
+            if (damp)
+                TLFEA_FORCE_REST(10, 5, true);
+            else
+                TLFEA_FORCE_REST(10, 5, false);
+        } else {
+            if (damp)
+                TLFEA_FORCE_REST(4, 1, true);
+            else
+                TLFEA_FORCE_REST(4, 1, false);
+        }
+#undef TLFEA_FORCE_REST
         TLFEA_LAUNCH_CHECK();
     }
     k_gather_partials<<<grid_for(p->n_nodes, 256), 256, 0, s>>>(p->n_nodes, p->d_g_off, p->d_g_idx, p->d_part, f,
@@ -308,6 +521,21 @@ extern "C" int tlfea_internal_force_planned(const tlfea_force_plan* plan, const 
     TLFEA_REQUIRE(quad->n == (plan->nn == 10 ? 5 : 1), "quadrature does not match the element");
     return launch_force_planned(plan, x, y, z, grad_t, detJ_t, *quad, *mat, v, f, accumulate, nullptr,
                                 as_stream(stream));
+}
+
+extern "C" int tlfea_internal_force_rest(const tlfea_force_plan* plan, const double* x, const double* y,
+                                         const double* z, const double* X0, const double* Y0, const double* Z0,
+                                         const tlfea_quad* quad, const tlfea_material* mat, const double* v,
+                                         double* f, int accumulate, void* stream) {
+    TLFEA_REQUIRE(plan && x && y && z && X0 && Y0 && Z0 && quad && mat && f, "null argument");
+    TLFEA_REQUIRE(is_standard_rule(plan->nn, *quad),
+                  "the gradients are rebuilt for the standard rule only (TET10: 5 points, TET4: 1 point)");
+    const bool damp = (v != nullptr) && (mat->eta_damp != 0.0 || mat->lambda_damp != 0.0);
+    if ((size_t)plan->max_u * (damp ? 12 : 9) * sizeof(double) > 160 * 1024) {
+        set_error("a block touches %d nodes: beyond the LDS budget of the rest-coordinate force", plan->max_u);
+        return TLFEA_ECAPACITY;
+    }
+    return launch_force_rest(plan, x, y, z, X0, Y0, Z0, *quad, *mat, v, f, accumulate, nullptr, as_stream(stream));
 }
 
 extern "C" int tlfea_fixed_dof_mask(int n_nodes, int n_fixed, const int* fixed_nodes, unsigned char* mask,

@@ -44,6 +44,13 @@ int launch_force_planned(const tlfea_force_plan* p, const double* x, const doubl
                          const double* grad_t, const double* detJ_t, const tlfea_quad& quad,
                          const tlfea_material& mat, const double* v, double* f, int accumulate, const int* flags,
                          hipStream_t s);
+// The same force with the gradients rebuilt from the rest positions (standard rules only,
+// see is_standard_rule; LDS of 9 or, with damping, 12 arrays of max_u doubles).
+bool is_standard_rule(int nn, const tlfea_quad& quad);
+int launch_force_rest(const tlfea_force_plan* p, const double* x, const double* y, const double* z,
+                      const double* X0, const double* Y0, const double* Z0, const tlfea_quad& quad,
+                      const tlfea_material& mat, const double* v, double* f, int accumulate, const int* flags,
+                      hipStream_t s);
 
 // dN/dxi of every node at every quadrature point (a kernel argument).
 template <typename T>

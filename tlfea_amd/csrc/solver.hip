@@ -413,6 +413,21 @@ void free_solver(tlfea_solver* s) {
         hipFree(p);
     delete s;
 }
+
+// Rest positions in the body descriptor select the rest-coordinate force kernel: it needs
+// the force plan, the standard rule and LDS room for the plan's largest block.
+int check_rest_form(const tlfea_body* b) {
+    if (!b->X0 && !b->Y0 && !b->Z0) return TLFEA_OK;
+    TLFEA_REQUIRE(b->X0 && b->Y0 && b->Z0 && b->force_plan, "rest positions need all three arrays and a force plan");
+    TLFEA_REQUIRE(is_standard_rule(b->nn, b->quad), "rest positions need the standard quadrature rule");
+    const bool damp = b->mat.eta_damp != 0.0 || b->mat.lambda_damp != 0.0;
+    if ((size_t)b->force_plan->max_u * (damp ? 12 : 9) * sizeof(double) > 160 * 1024) {
+        set_error("a block touches %d nodes: beyond the LDS budget of the rest-coordinate force",
+                  b->force_plan->max_u);
+        return TLFEA_ECAPACITY;
+    }
+    return TLFEA_OK;
+}
 }  // namespace
 
 extern "C" int tlfea_solver_create(int kind, const tlfea_body* body, tlfea_solver** out) {
@@ -427,6 +442,7 @@ extern "C" int tlfea_solver_create(int kind, const tlfea_body* body, tlfea_solve
                   "force plan does not match the body");
     TLFEA_REQUIRE(body->n_fixed == 0 || (body->fixed_nodes && body->constraint && body->xref && body->yref && body->zref),
                   "constraints declared without arrays");
+    if (int rc = check_rest_form(body)) return rc;
     auto* s = new tlfea_solver();
     s->kind = kind;
     s->b = *body;
@@ -493,6 +509,7 @@ extern "C" int tlfea_solver_update_body(tlfea_solver* s, const tlfea_body* body)
                       body->n_fixed == s->b.n_fixed,
                   "body sizes differ from the solver's");
     TLFEA_REQUIRE(body->pattern && body->m_sell && body->f_ext && body->f_int, "body is missing device arrays");
+    if (int rc = check_rest_form(body)) return rc;
     s->b = *body;
     return TLFEA_OK;
 }
@@ -606,7 +623,11 @@ extern "C" int tlfea_solver_solve(tlfea_solver* S, void* stream) {
             TLFEA_LAUNCH_CHECK();
             // element forces at the scratch positions, damping from v_guess; skipped once the
             // inner loop has converged (f_int then keeps the last evaluation)
-            if (b.force_plan) {
+            if (b.force_plan && b.X0) {
+                if (int rc = launch_force_rest(b.force_plan, b.x, b.y, b.z, b.X0, b.Y0, b.Z0, b.quad, b.mat,
+                                               damp ? S->v_guess : nullptr, b.f_int, 0, S->flags, st))
+                    return rc;
+            } else if (b.force_plan) {
                 if (int rc = launch_force_planned(b.force_plan, b.x, b.y, b.z, b.grad_t, b.detJ_t, b.quad, b.mat,
                                                   damp ? S->v_guess : nullptr, b.f_int, 0, S->flags, st))
                     return rc;

@@ -127,6 +127,16 @@ class Body:
              ptr(self.detJ_t), C.byref(self.quad), C.byref(self.mat), ptr(v), ptr(f), int(bool(accumulate)), self.s)
         return f
 
+    def internal_force_rest(self, x=None, v=None, f=None, accumulate=False):
+        """f_int with the shape-function gradients rebuilt from the rest positions (no
+        gradient arrays read)."""
+        x = self.x if x is None else x
+        f = torch.empty(3 * self.n_nodes, dtype=torch.float64, device=DEV) if f is None else f
+        X0 = self.X0
+        call("tlfea_internal_force_rest", self.plan, ptr(x[0]), ptr(x[1]), ptr(x[2]), ptr(X0[0]), ptr(X0[1]),
+             ptr(X0[2]), C.byref(self.quad), C.byref(self.mat), ptr(v), ptr(f), int(bool(accumulate)), self.s)
+        return f
+
     def elastic_material(self):
         """The material without Kelvin-Voigt damping (the tangent of the elastic force)."""
         return capi.Material(self.mat.lam, self.mat.mu, 0.0, 0.0)
@@ -348,7 +358,9 @@ class Solver:
     """SyncedNesterov / SyncedAdamW / SyncedAdamWNocoop on one Body
     (`src/solvers/SyncedNesterov.cuh:39-179`, `SyncedAdamW.cuh`, `SyncedAdamWNocoop.cuh`)."""
 
-    def __init__(self, body: Body, kind: int = capi.SOLVER_NESTEROV, planned: bool = True):
+    def __init__(self, body: Body, kind: int = capi.SOLVER_NESTEROV, planned: bool = True, rest: bool = False):
+        """planned: the block-local force kernel (else the atomic one); rest (with planned):
+        the force kernel that rebuilds the gradients from the rest positions."""
         self.b = body
         self.kind = kind
         self.m_sell = body.mass_sell()
@@ -360,6 +372,8 @@ class Solver:
                       ptr(self.m_sell), ptr(body.f_ext), nf, ptr(body.fixed) if nf else None, ptr(self.xref[0]),
                       ptr(self.xref[1]), ptr(self.xref[2]), ptr(body.f_int), ptr(self.constraint),
                       body.plan if planned else None)
+        if planned and rest:
+            d.X0, d.Y0, d.Z0 = ptr(body.X0[0]), ptr(body.X0[1]), ptr(body.X0[2])
         self.desc = d
         h = C.c_void_p()
         call("tlfea_solver_create", kind, C.byref(d), C.byref(h))
