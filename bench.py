@@ -15,11 +15,14 @@ reported beside it.  Inputs are resident in HBM when the timed region starts.
 
 Kernel rooflines are measured live with HIP events on the launch stream (the null stream,
 which torch.cuda.Event records on):
-  * roofline      -- the step's dominant kernel pair, the matrix-free tangent
-                     (k_tangent<10,5,double> + k_gather_tangent), bytes/launch =
-                     440*n_elem + 96*n_nodes (inverse Jacobians 360 B, detJ 40 B and
-                     connectivity 40 B per element; x, X0 and p read and y written, 96 B
-                     per node);
+  * roofline      -- the step's dominant kernel pair, the matrix-free tangent at the rest
+                     positions (k_tangent<10,5,double,..,LIN> + k_gather_tangent: the
+                     kernel's linear-elasticity form, chosen when x equals X0),
+                     bytes/launch = 440*n_elem + 48*n_nodes (inverse Jacobians 360 B,
+                     detJ 40 B and connectivity 40 B per element; p read and y written,
+                     48 B per node);
+  * tangent_general -- the same kernel away from the rest positions (x and X0 read too:
+                     440*n_elem + 96*n_nodes);
   * spmv_fp64     -- the FP64 block SpMV of the plain path, bytes/launch =
                      76*node_nnz + 52*n_nodes;
   * force_kernel  -- the fused FP64 P + f_int (block-local plan: force + node gather) with
@@ -56,6 +59,8 @@ HBM_PEAK_GBS = 8000.0
 # 200x20x20 beam: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (KiB), scales calibrated
 # on a 512 MiB copy
 PMC_TANGENT_BYTES = (152015 * 2.0 + 29109 + 30817 * 2.0 + 21561) * 1024
+# the same for the rest-state (linear-elasticity) form the linear-static step runs
+PMC_TANGENT_REST_BYTES = (118710 * 2.0 + 29111 + 30810 * 2.0 + 21438) * 1024
 # the same for the force kernels with k_gather_partials: gradient-reading (k_force_planned)
 # and rest-coordinate (k_force_rest)
 PMC_FORCE_BYTES = (314624 * 2.0 + 29110 + 21891 * 2.0 + 15799) * 1024
@@ -220,6 +225,7 @@ def main():
         b.spmv(k, p, y)
         b.tangent_apply(p, out=y)
         b.internal_force_rest(f=f)
+        b.tangent_apply(p, x=b.X0, out=y)
         src = torch.randn(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
@@ -287,11 +293,21 @@ def main():
         body.x[d].copy_(X0[d])
     p = torch.randn(3 * m.n_nodes, dtype=torch.float64, device="cuda")
     y = torch.empty_like(p)
-    # matrix-free tangent: inverse Jacobians (360 B) + detJ (40 B) + connectivity (40 B) per
-    # element, x, X0 and p read and y written once (96 B) per node
-    tan_s = event_time(lambda: body.tangent_apply(p, out=y))
-    tan_bytes = 440 * m.n_elem + 96 * m.n_nodes
+    # matrix-free tangent as the step runs it (rest positions: the linear-elasticity form):
+    # inverse Jacobians (360 B) + detJ (40 B) + connectivity (40 B) per element, p read and
+    # y written once (48 B) per node
+    tan_s = event_time(lambda: body.tangent_apply(p, x=body.X0, out=y))
+    tan_bytes = 440 * m.n_elem + 48 * m.n_nodes
     tan_gbs = tan_bytes / tan_s / 1e9
+    # the general form (positions away from the rest positions; the step's kernel when a
+    # solve starts from a deformed state): x and X0 read as well, 96 B per node
+    for d in range(3):
+        body.x[d].add_(1e-6)
+    tgen_s = event_time(lambda: body.tangent_apply(p, out=y))
+    tgen_bytes = 440 * m.n_elem + 96 * m.n_nodes
+    tgen_gbs = tgen_bytes / tgen_s / 1e9
+    for d in range(3):
+        body.x[d].copy_(X0[d])
     # FP64 block SpMV: 3x3 block values (72 B) + column index (4 B) per stored block, x read
     # and y written once (48 B per node), row id per node (4 B)
     spmv_s = event_time(lambda: body.spmv(ls64.k, p, y))
@@ -307,7 +323,8 @@ def main():
     force_gbs = force_bytes / force_s / 1e9
     rest_s = event_time(lambda: body.internal_force_rest(f=f))
     rest_gbs = force_bytes / rest_s / 1e9
-    log(f"tangent {tan_s * 1e6:.1f} us {tan_gbs:.0f} GB/s; spmv64 {spmv_s * 1e6:.1f} us {spmv_gbs:.0f} GB/s; "
+    log(f"tangent (rest form) {tan_s * 1e6:.1f} us {tan_gbs:.0f} GB/s; tangent (general) {tgen_s * 1e6:.1f} us "
+        f"{tgen_gbs:.0f} GB/s; spmv64 {spmv_s * 1e6:.1f} us {spmv_gbs:.0f} GB/s; "
         f"force (gradients read) {force_s * 1e6:.1f} us {force_gbs:.0f} GB/s; force (rest coordinates) "
         f"{rest_s * 1e6:.1f} us {rest_gbs:.0f} GB/s")
 
@@ -352,7 +369,7 @@ def main():
     # 200x20x20 beam only
     default = (nx, ny, nz) == (200, 20, 20)
     traffic = (890247 * 2.0 + 31606) * 1024 if default else None
-    traffic_tan = PMC_TANGENT_BYTES if default else None
+    traffic_tan = PMC_TANGENT_REST_BYTES if default else None
     out = {
         "metric": "TET10 solves/s at 1e-8 rel. residual (learned+FP32 vs plain); force-kernel GB/s",
         "value": args.steps / dt,
@@ -376,8 +393,13 @@ def main():
                    "speedup_vs_plain_fp64": dt64 / dt},
         "roofline": {"bound": "hbm", "achieved": tan_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": tan_gbs / HBM_PEAK_GBS, "traffic": traffic_tan,
-                     "kernel": "k_tangent<10,5,double,true> + k_gather_tangent (matrix-free K(x) p)",
+                     "kernel": "k_tangent<10,5,double,true,0,true> + k_gather_tangent (matrix-free K p at the rest "
+                               "positions, the linear-elasticity form of the kernel)",
                      "avg_us": tan_s * 1e6, "bytes_per_launch": tan_bytes},
+        "tangent_general": {"bound": "hbm", "achieved": tgen_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": tgen_gbs / HBM_PEAK_GBS, "traffic": PMC_TANGENT_BYTES if default else None,
+                            "kernel": "k_tangent<10,5,double,true,0,false> + k_gather_tangent (K(x) p away from rest)",
+                            "avg_us": tgen_s * 1e6, "bytes_per_launch": tgen_bytes},
         "spmv_fp64": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                       "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_cg_spmv/k_spmv_block",
                       "avg_us": spmv_s * 1e6, "bytes_per_launch": spmv_bytes},

@@ -86,6 +86,25 @@ def test_tangent_matches_oracle_matrix(gpu, name, masked):
         assert np.array_equal(y[fd], p[fd])
 
 
+@pytest.mark.parametrize("name", ["C1", "beam_highres_t10", "kuhn_t4"])
+def test_tangent_rest_state_form(gpu, name):
+    """At the rest positions the kernel's linear-elasticity form (selected when the
+    positions are the rest positions' own arrays) equals the general form at u = 0 and the
+    oracle's tangent assembled at rest."""
+    import torch
+    m = load_mesh(name)
+    b = _body(m)
+    pt = torch.as_tensor(np.random.default_rng(12).standard_normal(3 * m.n_nodes)).cuda()
+    y_gen = _np(b.tangent_apply(pt))            # separate x arrays equal to X0: general form
+    y_lin = _np(b.tangent_apply(pt, x=b.X0))    # the rest arrays themselves: rest-state form
+    assert rel_err(y_lin, y_gen) < 1e-14
+    go, djo = O.grad(m)
+    lam, mu = lame(E_STEEL, NU_STEEL)
+    doff, dcol = O.dof_pattern(m)
+    vo, _ = O.apply_dirichlet(doff, dcol, O.stiffness_values(m, go, djo, lam, mu, doff, dcol), m.f_ext, m.fixed)
+    assert rel_err(y_lin, O.spmv(doff, dcol, vo, _np(pt))) < 1e-12
+
+
 @pytest.mark.parametrize("name", ["beam_vtu", "kuhn_t4"])
 def test_tangent_symmetric(gpu, name):
     import torch

@@ -662,6 +662,7 @@ int apply64(tlfea_batch* b, const double* p, double* q, const int* iflags, hipSt
     tm.inst_mc = (mass && b->mass_on) ? b->d_mc : nullptr;
     tm.mass_exact = b->mass_exact;
     tm.k_scale = k_scale;
+    tm.at_rest = 1;  // the batch's operator is the tangent at the rest positions
     const unsigned char* mask = masked ? b->d_mask : nullptr;
     if (int rc = launch_tangent_partials<double>(b->plan, b->d_X, b->d_Y, b->d_Z, b->d_X, b->d_Y, b->d_Z, b->d_jinv,
                                                  b->d_detJ, b->quad, tm, mask, p, nullptr, s,
@@ -675,7 +676,8 @@ int apply64(tlfea_batch* b, const double* p, double* q, const int* iflags, hipSt
 
 // FP32 batched operator q = K p (masked), no dot partials.
 int apply32(tlfea_batch* b, const float* p, float* q, const int* iflags, hipStream_t s) {
-    const TanMat<float> tm{0.0f, 0.0f, b->d_elem_inst, b->d_lam32, b->d_mu32, iflags, NI};
+    TanMat<float> tm{0.0f, 0.0f, b->d_elem_inst, b->d_lam32, b->d_mu32, iflags, NI};
+    tm.at_rest = 1;
     if (int rc = launch_tangent_partials<float>(b->plan, b->d_X, b->d_Y, b->d_Z, b->d_X, b->d_Y, b->d_Z, b->d_jinv32,
                                                 b->d_detJ32, b->quad, tm, b->d_mask, p, nullptr, s,
                                                 b->compact ? b->d_blk_list : nullptr, b->n_blk_list))

@@ -78,6 +78,7 @@ struct TanMat {
     const T* inst_mc = nullptr;
     T k_scale = T(1);
     int mass_exact = 0;
+    int at_rest = 0;  // 1: x equals X0 (u = 0), the linear-elasticity form of the kernel
 };
 
 // Block partials of the matrix-free tangent (tangent.hip) into plan->d_part (read as T),
@@ -96,11 +97,16 @@ int launch_tangent_diag_partials(const tlfea_force_plan* pl, const double* x, co
 
 // Matrix-free tangent action (FP64, one body) at positions x (rest X0, inverse Jacobians
 // jinv_t); dot_part (optional) receives per-block partials of p . out (blocks of 256 nodes).
+// at_rest: the caller guarantees x == X0 (the linear-elasticity form of the kernel).
 int launch_tangent_planned(const tlfea_force_plan* pl, const double* x, const double* y, const double* z,
                            const double* X0, const double* Y0, const double* Z0, const double* jinv_t,
                            const double* detJ_t, const tlfea_quad& quad, const tlfea_material& mat,
                            const unsigned char* mask, const double* p, double* out, double* dot_part,
-                           const int* flags, hipStream_t s);
+                           const int* flags, hipStream_t s, bool at_rest = false);
+// true when x, y, z equal X0, Y0, Z0 on every node (same arrays, or equal values: one
+// device pass over d_scratch, one device int, and a synchronisation of s).
+int positions_at_rest(int n_nodes, const double* x, const double* y, const double* z, const double* X0,
+                      const double* Y0, const double* Z0, int* d_scratch, hipStream_t s, bool* at_rest);
 
 // Element precompute (element.hip): any of grad_t / detJ_t / jinv_t may be null.
 int launch_precompute(int nn, int n_elem, const int* conn, const double* x, const double* y, const double* z,

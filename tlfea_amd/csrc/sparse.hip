@@ -1392,6 +1392,10 @@ extern "C" int tlfea_cg_solve_matfree(tlfea_cg* c, const tlfea_force_plan* plan,
     const unsigned gv = RED_BLOCKS;
     // the gather's dot partials (one per 256 nodes) fit d_part: it holds n_slices*64/256+1 more
     const int gn = (int)grid_for(plan->n_nodes, 256);
+    // at the rest positions K is the linear-elasticity operator (the kernel's u = 0 form);
+    // the check uses the flag word before the initialisation resets it
+    bool at_rest = false;
+    if (int rc = positions_at_rest(plan->n_nodes, x, y, z, X0, Y0, Z0, c->d_flags, s, &at_rest)) return rc;
     k_cg_init_masked<<<gv, RED_THREADS, 0, s>>>(n, f, fixed_dof_mask, u, c->d_r, c->d_p, c->d_part);
     k_cg_init_fin<<<1, RED_THREADS, 0, s>>>(gv, c->d_part, c->d_scal, c->d_flags);
     TLFEA_LAUNCH_CHECK();
@@ -1399,7 +1403,7 @@ extern "C" int tlfea_cg_solve_matfree(tlfea_cg* c, const tlfea_force_plan* plan,
     double scal[8] = {0};
     for (int it = 0; it < max_iter; it++) {
         int rc = launch_tangent_planned(plan, x, y, z, X0, Y0, Z0, jinv_t, detJ_t, *quad, *mat, fixed_dof_mask,
-                                        c->d_p, c->d_Kp, c->d_part, c->d_flags, s);
+                                        c->d_p, c->d_Kp, c->d_part, c->d_flags, s, at_rest);
         if (rc != TLFEA_OK) return rc;
         k_cg_alpha<<<1, RED_THREADS, 0, s>>>(gn, c->d_part, c->d_scal, c->d_flags);
         k_cg_update<<<gv, RED_THREADS, 0, s>>>(n, u, c->d_r, c->d_p, c->d_Kp, c->d_part, c->d_scal, c->d_flags);

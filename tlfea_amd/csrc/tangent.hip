@@ -33,8 +33,11 @@ __device__ __forceinline__ T tan_mu(const TanMat<T>& m, int inst) {
 // STD the dN/dxi values are compile-time constants (q must then be a constant after
 // unrolling), otherwise they come from the kernel-argument table.  MASS: 0 no mass term,
 // 1 the mass of this quadrature rule (accumulated here), 2 the exact mass (the element
-// volume is accumulated in vol, the term is added after the last point).
-template <int NN, int NQ, typename T, bool STD, int MASS>
+// volume is accumulated in vol, the term is added after the last point).  LIN: the
+// positions equal the rest positions (u = 0, so H = E = S = 0 and dP = dS, the linear
+// elasticity operator; the same values as the general path at u = 0, with the u terms and
+// their staging left out).
+template <int NN, int NQ, typename T, bool STD, int MASS, bool LIN>
 __device__ __forceinline__ void tangent_qp(int q, int64_t tile, int lane, const int (&sl)[NN], const T* sU,
                                            const T* sP, int max_u, const T* __restrict__ jinv_t,
                                            const T* __restrict__ detJ_t, const DnTable<T>& dn, const tlfea_quad& quad,
@@ -60,7 +63,12 @@ __device__ __forceinline__ void tangent_qp(int q, int64_t tile, int lane, const 
 #pragma unroll
     for (int a = 0; a < NN; a++) {
         const int s = sl[a];
-        const T ua[3] = {sU[s], sU[max_u + s], sU[2 * max_u + s]};
+        T ua[3] = {0, 0, 0};
+        if constexpr (!LIN) {
+            ua[0] = sU[s];
+            ua[1] = sU[max_u + s];
+            ua[2] = sU[2 * max_u + s];
+        }
         const T pa[3] = {sP[s], sP[max_u + s], sP[2 * max_u + s]};
         if constexpr (MASS == 1) {
             const T Na = STD ? (T)std_N<NN>(q, a) : dn.N[q][a];
@@ -73,50 +81,69 @@ __device__ __forceinline__ void tangent_qp(int q, int64_t tile, int lane, const 
             if (STD && c == T(0)) continue;
 #pragma unroll
             for (int i = 0; i < 3; i++) {
-                Ju[i][k] += ua[i] * c;
+                if constexpr (!LIN) Ju[i][k] += ua[i] * c;
                 Jp[i][k] += pa[i] * c;
             }
         }
     }
-    // H = F - I and dF
-    T H[3][3], dF[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            H[i][j] = Ju[i][0] * M[j][0] + Ju[i][1] * M[j][1] + Ju[i][2] * M[j][2];
-            dF[i][j] = Jp[i][0] * M[j][0] + Jp[i][1] * M[j][1] + Jp[i][2] * M[j][2];
-        }
-    T Em[3][3], dE[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            const T hh = H[0][i] * H[0][j] + H[1][i] * H[1][j] + H[2][i] * H[2][j];
-            Em[i][j] = T(0.5) * (H[i][j] + H[j][i] + hh);
-            // dF^T F + F^T dF with F = I + H
-            const T s2 = dF[i][j] + dF[j][i] + dF[0][i] * H[0][j] + dF[1][i] * H[1][j] + dF[2][i] * H[2][j] +
-                         H[0][i] * dF[0][j] + H[1][i] * dF[1][j] + H[2][i] * dF[2][j];
-            dE[i][j] = T(0.5) * s2;
-        }
-    const T trE = Em[0][0] + Em[1][1] + Em[2][2], trdE = dE[0][0] + dE[1][1] + dE[2][2];
-    T S[3][3], dS[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            S[i][j] = T(2) * mu * Em[i][j] + (i == j ? lam * trE : T(0));
-            dS[i][j] = T(2) * mu * dE[i][j] + (i == j ? lam * trdE : T(0));
-        }
-    // dP = dF S + (I + H) dS, scaled by dV
     T dP[3][3];
+    if constexpr (LIN) {
+        // dF, dE = sym(dF), dP = dS = lambda tr(dE) I + 2 mu dE
+        T dF[3][3];
 #pragma unroll
-    for (int i = 0; i < 3; i++)
+        for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int j = 0; j < 3; j++)
-            dP[i][j] = (dF[i][0] * S[0][j] + dF[i][1] * S[1][j] + dF[i][2] * S[2][j] + dS[i][j] + H[i][0] * dS[0][j] +
-                        H[i][1] * dS[1][j] + H[i][2] * dS[2][j]) *
-                       (dV * ks);
+            for (int j = 0; j < 3; j++) dF[i][j] = Jp[i][0] * M[j][0] + Jp[i][1] * M[j][1] + Jp[i][2] * M[j][2];
+        T dE[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) dE[i][j] = T(0.5) * (dF[i][j] + dF[j][i]);
+        const T trdE = dE[0][0] + dE[1][1] + dE[2][2];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) dP[i][j] = (T(2) * mu * dE[i][j] + (i == j ? lam * trdE : T(0))) * (dV * ks);
+    } else {
+        // H = F - I and dF
+        T H[3][3], dF[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                H[i][j] = Ju[i][0] * M[j][0] + Ju[i][1] * M[j][1] + Ju[i][2] * M[j][2];
+                dF[i][j] = Jp[i][0] * M[j][0] + Jp[i][1] * M[j][1] + Jp[i][2] * M[j][2];
+            }
+        T Em[3][3], dE[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const T hh = H[0][i] * H[0][j] + H[1][i] * H[1][j] + H[2][i] * H[2][j];
+                Em[i][j] = T(0.5) * (H[i][j] + H[j][i] + hh);
+                // dF^T F + F^T dF with F = I + H
+                const T s2 = dF[i][j] + dF[j][i] + dF[0][i] * H[0][j] + dF[1][i] * H[1][j] + dF[2][i] * H[2][j] +
+                             H[0][i] * dF[0][j] + H[1][i] * dF[1][j] + H[2][i] * dF[2][j];
+                dE[i][j] = T(0.5) * s2;
+            }
+        const T trE = Em[0][0] + Em[1][1] + Em[2][2], trdE = dE[0][0] + dE[1][1] + dE[2][2];
+        T S[3][3], dS[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                S[i][j] = T(2) * mu * Em[i][j] + (i == j ? lam * trE : T(0));
+                dS[i][j] = T(2) * mu * dE[i][j] + (i == j ? lam * trdE : T(0));
+            }
+        // dP = dF S + (I + H) dS, scaled by dV
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                dP[i][j] = (dF[i][0] * S[0][j] + dF[i][1] * S[1][j] + dF[i][2] * S[2][j] + dS[i][j] + H[i][0] * dS[0][j] +
+                            H[i][1] * dS[1][j] + H[i][2] * dS[2][j]) *
+                           (dV * ks);
+    }
     T Q[3][3];
 #pragma unroll
     for (int i = 0; i < 3; i++)
@@ -142,7 +169,7 @@ __device__ __forceinline__ void tangent_qp(int q, int64_t tile, int lane, const 
     }
 }
 
-template <int NN, int NQ, typename T, bool STD, int MASS>
+template <int NN, int NQ, typename T, bool STD, int MASS, bool LIN>
 __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent(
     int n_elem, const int64_t* __restrict__ blk_off, const int* __restrict__ unodes,
     const unsigned short* __restrict__ slot_t, const double* __restrict__ x, const double* __restrict__ y,
@@ -180,9 +207,11 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent(
     }
     for (int u = threadIdx.x; u < U; u += PLAN_BLOCK) {
         const int n = unodes[u0 + u];
-        sU[u] = (T)(x[n] - X0[n]);
-        sU[max_u + u] = (T)(y[n] - Y0[n]);
-        sU[2 * max_u + u] = (T)(z[n] - Z0[n]);
+        if constexpr (!LIN) {
+            sU[u] = (T)(x[n] - X0[n]);
+            sU[max_u + u] = (T)(y[n] - Y0[n]);
+            sU[2 * max_u + u] = (T)(z[n] - Z0[n]);
+        }
         sF[u] = sF[max_u + u] = sF[2 * max_u + u] = T(0);
 #pragma unroll
         for (int d = 0; d < 3; d++) sP[d * max_u + u] = (mask && mask[3 * n + d]) ? T(0) : p[3 * n + d];
@@ -199,13 +228,13 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent(
         if constexpr (STD) {
 #pragma unroll
             for (int q = 0; q < NQ; q++)
-                tangent_qp<NN, NQ, T, true, MASS>(q, tile, lane, sl, sU, sP, max_u, jinv_t, detJ_t, dn, quad, lam, mu, mc,
-                                                  ks, Mn, dJn, vol, fa);
+                tangent_qp<NN, NQ, T, true, MASS, LIN>(q, tile, lane, sl, sU, sP, max_u, jinv_t, detJ_t, dn, quad, lam,
+                                                       mu, mc, ks, Mn, dJn, vol, fa);
         } else {
 #pragma unroll 1
             for (int q = 0; q < NQ; q++)
-                tangent_qp<NN, NQ, T, false, MASS>(q, tile, lane, sl, sU, sP, max_u, jinv_t, detJ_t, dn, quad, lam, mu,
-                                                   mc, ks, Mn, dJn, vol, fa);
+                tangent_qp<NN, NQ, T, false, MASS, LIN>(q, tile, lane, sl, sU, sP, max_u, jinv_t, detJ_t, dn, quad,
+                                                        lam, mu, mc, ks, Mn, dJn, vol, fa);
         }
         if constexpr (MASS == 2) {
             // exact element mass rho V C applied to the element's p, one component at a time
@@ -329,30 +358,34 @@ int launch_tangent_partials(const tlfea_force_plan* pl, const double* x, const d
                 for (int k = 0; k < 3; k++) std_rule = std_rule && dn64.d[q][a][k] == std_dn<10>(q, a, k);
                 std_rule = std_rule && dn64.N[q][a] == std_N<10>(q, a);
             }
-#define TLFEA_TANGENT_LAUNCH(NN_, NQ_, STD_, MASS_)                                                          \
-    k_tangent<NN_, NQ_, T, STD_, MASS_><<<nb, PLAN_BLOCK, lds, s>>>(                                          \
+#define TLFEA_TANGENT_LAUNCH(NN_, NQ_, STD_, MASS_, LIN_)                                                    \
+    k_tangent<NN_, NQ_, T, STD_, MASS_, LIN_><<<nb, PLAN_BLOCK, lds, s>>>(                                    \
         pl->n_elem, pl->d_blk_off, pl->d_unodes, pl->d_slot, x, y, z, X0, Y0, Z0, p, mask, jinv_t, detJ_t, dn, quad, \
         mat, part, pl->max_u, flags, blk_list)
-        if (std_rule && mass == 2)
-            TLFEA_TANGENT_LAUNCH(10, 5, true, 2);
-        else if (std_rule && mass == 1)
-            TLFEA_TANGENT_LAUNCH(10, 5, true, 1);
+#define TLFEA_TANGENT_BY_MASS(NN_, NQ_, STD_, LIN_)        \
+    do {                                                   \
+        if (mass == 2)                                     \
+            TLFEA_TANGENT_LAUNCH(NN_, NQ_, STD_, 2, LIN_); \
+        else if (mass == 1)                                \
+            TLFEA_TANGENT_LAUNCH(NN_, NQ_, STD_, 1, LIN_); \
+        else                                               \
+            TLFEA_TANGENT_LAUNCH(NN_, NQ_, STD_, 0, LIN_); \
+    } while (0)
+        // the rest-state variant has the standard rule only (elsewhere the general kernel
+        // computes the same values from u = 0)
+        if (std_rule && mat.at_rest)
+            TLFEA_TANGENT_BY_MASS(10, 5, true, true);
         else if (std_rule)
-            TLFEA_TANGENT_LAUNCH(10, 5, true, 0);
-        else if (mass == 2)
-            TLFEA_TANGENT_LAUNCH(10, 5, false, 2);
-        else if (mass == 1)
-            TLFEA_TANGENT_LAUNCH(10, 5, false, 1);
+            TLFEA_TANGENT_BY_MASS(10, 5, true, false);
         else
-            TLFEA_TANGENT_LAUNCH(10, 5, false, 0);
+            TLFEA_TANGENT_BY_MASS(10, 5, false, false);
     } else {
-        if (mass == 2)
-            TLFEA_TANGENT_LAUNCH(4, 1, true, 2);
-        else if (mass == 1)
-            TLFEA_TANGENT_LAUNCH(4, 1, true, 1);
+        if (mat.at_rest)
+            TLFEA_TANGENT_BY_MASS(4, 1, true, true);
         else
-            TLFEA_TANGENT_LAUNCH(4, 1, true, 0);
+            TLFEA_TANGENT_BY_MASS(4, 1, true, false);
     }
+#undef TLFEA_TANGENT_BY_MASS
 #undef TLFEA_TANGENT_LAUNCH
     TLFEA_LAUNCH_CHECK();
     return TLFEA_OK;
@@ -372,14 +405,42 @@ int launch_tangent_planned(const tlfea_force_plan* pl, const double* x, const do
                            const double* X0, const double* Y0, const double* Z0, const double* jinv_t,
                            const double* detJ_t, const tlfea_quad& quad, const tlfea_material& mat,
                            const unsigned char* mask, const double* p, double* out, double* dot_part,
-                           const int* flags, hipStream_t s) {
-    const TanMat<double> tm{mat.lambda, mat.mu, nullptr, nullptr, nullptr, nullptr, 0};
+                           const int* flags, hipStream_t s, bool at_rest) {
+    TanMat<double> tm{mat.lambda, mat.mu, nullptr, nullptr, nullptr, nullptr, 0};
+    tm.at_rest = at_rest ? 1 : 0;
     if (int rc = launch_tangent_partials<double>(pl, x, y, z, X0, Y0, Z0, jinv_t, detJ_t, quad, tm, mask, p, flags,
                                                  s, nullptr, 0))
         return rc;
     k_gather_tangent<<<grid_for(pl->n_nodes, 256), 256, 0, s>>>(pl->n_nodes, pl->d_g_off, pl->d_g_idx, pl->d_part,
                                                                 mask, p, out, dot_part, flags);
     TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
+// flag[0] = 1 when some node's position differs from its rest position (every writer
+// stores the same value)
+__global__ void k_moved(int n, const double* __restrict__ x, const double* __restrict__ y,
+                        const double* __restrict__ z, const double* __restrict__ X0, const double* __restrict__ Y0,
+                        const double* __restrict__ Z0, int* __restrict__ flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && (x[i] != X0[i] || y[i] != Y0[i] || z[i] != Z0[i])) flag[0] = 1;
+}
+
+__global__ void k_zero_int(int* __restrict__ v) { v[0] = 0; }
+
+int positions_at_rest(int n_nodes, const double* x, const double* y, const double* z, const double* X0,
+                      const double* Y0, const double* Z0, int* d_scratch, hipStream_t s, bool* at_rest) {
+    if (x == X0 && y == Y0 && z == Z0) {
+        *at_rest = true;
+        return TLFEA_OK;
+    }
+    k_zero_int<<<1, 1, 0, s>>>(d_scratch);
+    k_moved<<<grid_for(n_nodes, 256), 256, 0, s>>>(n_nodes, x, y, z, X0, Y0, Z0, d_scratch);
+    TLFEA_LAUNCH_CHECK();
+    int h = 1;
+    TLFEA_HIP_TRY(hipMemcpyAsync(&h, d_scratch, sizeof(int), hipMemcpyDeviceToHost, s));
+    TLFEA_HIP_TRY(hipStreamSynchronize(s));
+    *at_rest = h == 0;
     return TLFEA_OK;
 }
 
@@ -515,6 +576,8 @@ extern "C" int tlfea_tangent_apply_planned(const tlfea_force_plan* plan, const d
                   "null argument");
     TLFEA_REQUIRE(quad->n == (plan->nn == 10 ? 5 : 1), "quadrature size does not match the element type");
     TLFEA_REQUIRE(mat->eta_damp == 0.0 && mat->lambda_damp == 0.0, "the matrix-free tangent excludes damping");
+    // the linear-elasticity form when the positions are the rest positions' own arrays
+    const bool same = x == X0 && y == Y0 && z == Z0;
     return launch_tangent_planned(plan, x, y, z, X0, Y0, Z0, jinv_t, detJ_t, *quad, *mat, fixed_dof_mask, p, out,
-                                  nullptr, nullptr, as_stream(stream));
+                                  nullptr, nullptr, as_stream(stream), same);
 }
