@@ -67,6 +67,12 @@ int tlfea_precompute_grad(int nn, int n_elem, const int* conn, const double* x, 
  * doubles.  Not in the reference: the compact input of the matrix-free tangent. */
 int tlfea_precompute_jinv(int nn, int n_elem, const int* conn, const double* x, const double* y, const double* z,
                           const tlfea_quad* quad, double* jinv_t, void* stream);
+/* *affine = 1 when every element has the same M and det J at all quadrature points to
+ * 1e-12 relative (straight-sided elements: J is constant, the per-point values differ by
+ * the rounding of sum_a X_a (x) dN_a/dxi only).  Setup call (synchronises the stream).  With
+ * affine = 1 the matrix-free tangent reads point 0's M and det J only.  Not in the
+ * reference. */
+int tlfea_jinv_affine(int nn, int n_elem, const double* jinv_t, const double* detJ_t, int* affine, void* stream);
 
 /* ---- stress and internal force ---------------------------------------------------- */
 /* Replaces calc_p_kernel / compute_p (src/elements/FEAT10Data.cu:287-305,
@@ -131,10 +137,14 @@ int tlfea_internal_force_rest(const tlfea_force_plan* plan, const double* x, con
  * instead of ~3 kB of assembled 3x3 blocks.  With a DOF mask (uint8 [3*n_nodes],
  * 1 = fixed) the masked rows return y = p (the Dirichlet identity rows of
  * tlfea_apply_dirichlet) and the masked columns are ignored (p read as 0); NULL = no
- * mask.  Kelvin-Voigt damping is not part of this operator (mat damping must be 0). */
+ * mask.  Kelvin-Voigt damping is not part of this operator (mat damping must be 0).
+ * affine (as reported by tlfea_jinv_affine; 0 is always valid): M and det J of quadrature
+ * point 0 only are read.  When x, y, z are the arrays X0, Y0, Z0 themselves the kernel
+ * runs its linear-elasticity form (u = 0: H = S = 0, dP = dS), the same values without
+ * the u terms. */
 int tlfea_tangent_apply_planned(const tlfea_force_plan* plan, const double* x, const double* y, const double* z,
                                 const double* X0, const double* Y0, const double* Z0, const double* jinv_t,
-                                const double* detJ_t, const tlfea_quad* quad, const tlfea_material* mat,
+                                const double* detJ_t, int affine, const tlfea_quad* quad, const tlfea_material* mat,
                                 const unsigned char* fixed_dof_mask, const double* p, double* out, void* stream);
 
 /* FP32 variant of the fused force kernel, displacement form F = I + sum_a u_a (x) g_a:
@@ -234,10 +244,12 @@ int tlfea_constraint_eval(int n_fixed, const int* fixed_nodes, const double* x, 
 /* ---- matrix-free CG (not in the reference) ----------------------------------------- */
 /* The reference's CG (same sqrt(rr/rr0) < tol rule, TLFEA_ENOTPD on p.Kp <= 0) with the
  * operator K(x) applied by tlfea_tangent_apply_planned instead of an assembled matrix:
- * the linear-static system of LinearStaticSolver::Solve after Dirichlet elimination. */
+ * the linear-static system of LinearStaticSolver::Solve after Dirichlet elimination.
+ * When x equals X0 on every node (checked once per call) the operator is applied in its
+ * linear-elasticity form. */
 int tlfea_cg_solve_matfree(tlfea_cg* c, const tlfea_force_plan* plan, const double* x, const double* y,
                            const double* z, const double* X0, const double* Y0, const double* Z0,
-                           const double* jinv_t, const double* detJ_t, const tlfea_quad* quad,
+                           const double* jinv_t, const double* detJ_t, int affine, const tlfea_quad* quad,
                            const tlfea_material* mat, const unsigned char* fixed_dof_mask, const double* f, double* u,
                            double tol, int max_iter, int check_every, int* iters_out, double* rel_res_out,
                            void* stream);

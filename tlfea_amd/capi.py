@@ -86,9 +86,10 @@ _SIGS = {
                                            vp]),
     "tlfea_internal_force_rest": (i32, [vp, vp, vp, vp, vp, vp, vp, C.POINTER(Quad), C.POINTER(Material), vp, vp, i32,
                                         vp]),
-    "tlfea_tangent_apply_planned": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(Quad), C.POINTER(Material),
+    "tlfea_tangent_apply_planned": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, C.POINTER(Quad), C.POINTER(Material),
                                           vp, vp, vp, vp]),
     "tlfea_precompute_jinv": (i32, [i32, i32, vp, vp, vp, vp, C.POINTER(Quad), vp, vp]),
+    "tlfea_jinv_affine": (i32, [i32, i32, vp, vp, C.POINTER(i32), vp]),
     "tlfea_fixed_dof_mask": (i32, [i32, i32, vp, vp, vp]),
     "tlfea_grad_to_f32": (i32, [i32, i32, vp, vp, C.POINTER(Quad), vp, vp, vp]),
     "tlfea_internal_force_f32": (i32, [i32, i32, i32, vp, vp, vp, vp, vp, vp, C.POINTER(Material), vp, i32, vp]),
@@ -118,7 +119,7 @@ _SIGS = {
     "tlfea_cg_create": (i32, [vp, C.POINTER(vp)]),
     "tlfea_cg_destroy": (i32, [vp]),
     "tlfea_cg_solve": (i32, [vp, vp, vp, vp, f64, i32, i32, C.POINTER(i32), C.POINTER(f64), vp]),
-    "tlfea_cg_solve_matfree": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(Quad), C.POINTER(Material),
+    "tlfea_cg_solve_matfree": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, C.POINTER(Quad), C.POINTER(Material),
                                      vp, vp, vp, f64, i32, i32, C.POINTER(i32), C.POINTER(f64), vp]),
     "tlfea_batch_create": (i32, [i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(vp), vp]),
     "tlfea_batch_destroy": (i32, [vp]),

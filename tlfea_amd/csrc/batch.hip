@@ -32,6 +32,7 @@ struct tlfea_batch {
     int* d_elem_inst = nullptr;
     double *d_jinv = nullptr, *d_detJ = nullptr;
     float *d_jinv32 = nullptr, *d_detJ32 = nullptr;
+    int affine = 0;  // every element's M and det J equal at all quadrature points
     double *d_lam = nullptr, *d_mu = nullptr;
     float *d_lam32 = nullptr, *d_mu32 = nullptr;
     // chunk tables
@@ -663,6 +664,7 @@ int apply64(tlfea_batch* b, const double* p, double* q, const int* iflags, hipSt
     tm.mass_exact = b->mass_exact;
     tm.k_scale = k_scale;
     tm.at_rest = 1;  // the batch's operator is the tangent at the rest positions
+    tm.affine = b->affine;
     const unsigned char* mask = masked ? b->d_mask : nullptr;
     if (int rc = launch_tangent_partials<double>(b->plan, b->d_X, b->d_Y, b->d_Z, b->d_X, b->d_Y, b->d_Z, b->d_jinv,
                                                  b->d_detJ, b->quad, tm, mask, p, nullptr, s,
@@ -678,6 +680,7 @@ int apply64(tlfea_batch* b, const double* p, double* q, const int* iflags, hipSt
 int apply32(tlfea_batch* b, const float* p, float* q, const int* iflags, hipStream_t s) {
     TanMat<float> tm{0.0f, 0.0f, b->d_elem_inst, b->d_lam32, b->d_mu32, iflags, NI};
     tm.at_rest = 1;
+    tm.affine = b->affine;
     if (int rc = launch_tangent_partials<float>(b->plan, b->d_X, b->d_Y, b->d_Z, b->d_X, b->d_Y, b->d_Z, b->d_jinv32,
                                                 b->d_detJ32, b->quad, tm, b->d_mask, p, nullptr, s,
                                                 b->compact ? b->d_blk_list : nullptr, b->n_blk_list))
@@ -796,6 +799,7 @@ extern "C" int tlfea_batch_create(int nn, int n_inst, const int* elem_off, const
     if ((rc = launch_precompute(nn, n_elem, b->d_conn, b->d_X, b->d_Y, b->d_Z, b->quad, nullptr, b->d_detJ, b->d_jinv,
                                 s)))
         return fail(rc);
+    if ((rc = tlfea_jinv_affine(nn, n_elem, b->d_jinv, b->d_detJ, &b->affine, s))) return fail(rc);
     k_d2f<<<grid_for((int64_t)pe * nq * 9, 256), 256, 0, s>>>((int64_t)pe * nq * 9, b->d_jinv, b->d_jinv32);
     k_d2f<<<grid_for((int64_t)pe * nq, 256), 256, 0, s>>>((int64_t)pe * nq, b->d_detJ, b->d_detJ32);
     k_d2f<<<grid_for(n_inst, 256), 256, 0, s>>>(n_inst, b->d_lam, b->d_lam32);

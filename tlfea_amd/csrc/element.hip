@@ -583,6 +583,63 @@ extern "C" int tlfea_precompute_jinv(int nn, int n_elem, const int* conn, const 
     return launch_precompute(nn, n_elem, conn, x, y, z, *quad, nullptr, nullptr, jinv_t, as_stream(stream));
 }
 
+namespace tlfea_hip {
+// not_affine[0] = 1 when some element's M = J^-T or det J differs between quadrature points
+// by more than 1e-12 relative (straight-sided elements agree up to the rounding of
+// J = sum_a X_a (x) dN_a/dxi); every writer stores the same value.
+template <int NQ>
+__global__ __launch_bounds__(256) void k_not_affine(int n_elem, const double* __restrict__ jinv_t,
+                                                    const double* __restrict__ detJ_t, int* __restrict__ not_affine) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_elem) return;
+    const int64_t tile = e / TILE;
+    const int lane = (int)(e % TILE);
+    double m0[9], mx = 0.0;
+    const double* M0 = jinv_t + tile * NQ * 9 * TILE + lane;
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+        m0[c] = M0[c * TILE];
+        mx = fmax(mx, fabs(m0[c]));
+    }
+    const double d0 = detJ_t[tile * NQ * TILE + lane];
+    bool ok = true;
+#pragma unroll
+    for (int q = 1; q < NQ; q++) {
+        const double* Mq = jinv_t + (tile * NQ + q) * 9 * TILE + lane;
+#pragma unroll
+        for (int c = 0; c < 9; c++) ok = ok && fabs(Mq[c * TILE] - m0[c]) <= 1e-12 * mx;
+        ok = ok && fabs(detJ_t[(tile * NQ + q) * TILE + lane] - d0) <= 1e-12 * fabs(d0);
+    }
+    if (!ok) not_affine[0] = 1;
+}
+__global__ void k_zero_flag(int* __restrict__ v) { v[0] = 0; }
+}  // namespace tlfea_hip
+
+extern "C" int tlfea_jinv_affine(int nn, int n_elem, const double* jinv_t, const double* detJ_t, int* affine,
+                                 void* stream) {
+    if (int rc = check_elem(nn, n_elem, nullptr)) return rc;
+    TLFEA_REQUIRE(affine != nullptr, "null output");
+    *affine = 1;
+    if (n_elem == 0 || nn == 4) return TLFEA_OK;  // one quadrature point: nothing to compare
+    TLFEA_REQUIRE(jinv_t && detJ_t, "null pointer argument");
+    hipStream_t s = as_stream(stream);
+    int* d_flag = nullptr;
+    TLFEA_HIP_TRY(hipMalloc((void**)&d_flag, sizeof(int)));
+    k_zero_flag<<<1, 1, 0, s>>>(d_flag);
+    k_not_affine<5><<<grid_for(n_elem, 256), 256, 0, s>>>(n_elem, jinv_t, detJ_t, d_flag);
+    int h = 1;
+    const hipError_t e0 = hipGetLastError();
+    const hipError_t e1 = hipMemcpyAsync(&h, d_flag, sizeof(int), hipMemcpyDeviceToHost, s);
+    const hipError_t e2 = hipStreamSynchronize(s);
+    (void)hipFree(d_flag);
+    if (e0 != hipSuccess || e1 != hipSuccess || e2 != hipSuccess) {
+        set_error("affinity check failed");
+        return TLFEA_EHIP;
+    }
+    *affine = h ? 0 : 1;
+    return TLFEA_OK;
+}
+
 extern "C" int tlfea_calc_p(int nn, int n_elem, const int* conn, const double* x, const double* y, const double* z,
                             const double* grad_t, const tlfea_material* mat, const double* v, double* F_t, double* P_t,
                             double* Fdot_t, double* Pvis_t, void* stream) {

@@ -79,6 +79,9 @@ struct TanMat {
     T k_scale = T(1);
     int mass_exact = 0;
     int at_rest = 0;  // 1: x equals X0 (u = 0), the linear-elasticity form of the kernel
+    // 1 when every element's M and det J are the same at all quadrature points
+    // (tlfea_jinv_affine): the kernel then reads point 0's only
+    int affine = 0;
 };
 
 // Block partials of the matrix-free tangent (tangent.hip) into plan->d_part (read as T),
@@ -102,7 +105,7 @@ int launch_tangent_planned(const tlfea_force_plan* pl, const double* x, const do
                            const double* X0, const double* Y0, const double* Z0, const double* jinv_t,
                            const double* detJ_t, const tlfea_quad& quad, const tlfea_material& mat,
                            const unsigned char* mask, const double* p, double* out, double* dot_part,
-                           const int* flags, hipStream_t s, bool at_rest = false);
+                           const int* flags, hipStream_t s, bool at_rest = false, bool affine = false);
 // true when x, y, z equal X0, Y0, Z0 on every node (same arrays, or equal values: one
 // device pass over d_scratch, one device int, and a synchronisation of s).
 int positions_at_rest(int n_nodes, const double* x, const double* y, const double* z, const double* X0,

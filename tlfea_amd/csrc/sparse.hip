@@ -1376,8 +1376,9 @@ __global__ __launch_bounds__(RED_THREADS) void k_cg_init_masked(int64_t n, const
 
 extern "C" int tlfea_cg_solve_matfree(tlfea_cg* c, const tlfea_force_plan* plan, const double* x, const double* y,
                                       const double* z, const double* X0, const double* Y0, const double* Z0,
-                                      const double* jinv_t, const double* detJ_t, const tlfea_quad* quad,
-                                      const tlfea_material* mat, const unsigned char* fixed_dof_mask, const double* f,
+                                      const double* jinv_t, const double* detJ_t, int affine,
+                                      const tlfea_quad* quad, const tlfea_material* mat,
+                                      const unsigned char* fixed_dof_mask, const double* f,
                                       double* u, double tol, int max_iter, int check_every, int* iters_out,
                                       double* rel_res_out, void* stream) {
     TLFEA_REQUIRE(c && plan && x && y && z && X0 && Y0 && Z0 && jinv_t && detJ_t && quad && mat && f && u,
@@ -1403,7 +1404,7 @@ extern "C" int tlfea_cg_solve_matfree(tlfea_cg* c, const tlfea_force_plan* plan,
     double scal[8] = {0};
     for (int it = 0; it < max_iter; it++) {
         int rc = launch_tangent_planned(plan, x, y, z, X0, Y0, Z0, jinv_t, detJ_t, *quad, *mat, fixed_dof_mask,
-                                        c->d_p, c->d_Kp, c->d_part, c->d_flags, s, at_rest);
+                                        c->d_p, c->d_Kp, c->d_part, c->d_flags, s, at_rest, affine != 0);
         if (rc != TLFEA_OK) return rc;
         k_cg_alpha<<<1, RED_THREADS, 0, s>>>(gn, c->d_part, c->d_scal, c->d_flags);
         k_cg_update<<<gv, RED_THREADS, 0, s>>>(n, u, c->d_r, c->d_p, c->d_Kp, c->d_part, c->d_scal, c->d_flags);

@@ -36,8 +36,9 @@ __device__ __forceinline__ T tan_mu(const TanMat<T>& m, int inst) {
 // volume is accumulated in vol, the term is added after the last point).  LIN: the
 // positions equal the rest positions (u = 0, so H = E = S = 0 and dP = dS, the linear
 // elasticity operator; the same values as the general path at u = 0, with the u terms and
-// their staging left out).
-template <int NN, int NQ, typename T, bool STD, int MASS, bool LIN>
+// their staging left out).  AFF: M and det J are the same at every point (affine
+// elements), point 0's are kept.
+template <int NN, int NQ, typename T, bool STD, int MASS, bool LIN, bool AFF>
 __device__ __forceinline__ void tangent_qp(int q, int64_t tile, int lane, const int (&sl)[NN], const T* sU,
                                            const T* sP, int max_u, const T* __restrict__ jinv_t,
                                            const T* __restrict__ detJ_t, const DnTable<T>& dn, const tlfea_quad& quad,
@@ -51,8 +52,9 @@ __device__ __forceinline__ void tangent_qp(int q, int64_t tile, int lane, const 
     for (int c = 0; c < 9; c++) M[c / 3][c % 3] = Mn[c];
     const T dV = dJn * (T)quad.w[q];
     if constexpr (MASS == 2) vol += dV;
-    // M and detJ of the next quadrature point are loaded one iteration ahead
-    if (q + 1 < NQ) {
+    // M and detJ of the next quadrature point are loaded one iteration ahead (affine
+    // elements keep point 0's)
+    if (q + 1 < NQ && !AFF) {
         const T* Mp = jinv_t + (tq + 1) * (9 * TILE) + lane;
 #pragma unroll
         for (int c = 0; c < 9; c++) Mn[c] = Mp[c * TILE];
@@ -169,7 +171,7 @@ __device__ __forceinline__ void tangent_qp(int q, int64_t tile, int lane, const 
     }
 }
 
-template <int NN, int NQ, typename T, bool STD, int MASS, bool LIN>
+template <int NN, int NQ, typename T, bool STD, int MASS, bool LIN, bool AFF>
 __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent(
     int n_elem, const int64_t* __restrict__ blk_off, const int* __restrict__ unodes,
     const unsigned short* __restrict__ slot_t, const double* __restrict__ x, const double* __restrict__ y,
@@ -228,13 +230,13 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent(
         if constexpr (STD) {
 #pragma unroll
             for (int q = 0; q < NQ; q++)
-                tangent_qp<NN, NQ, T, true, MASS, LIN>(q, tile, lane, sl, sU, sP, max_u, jinv_t, detJ_t, dn, quad, lam,
-                                                       mu, mc, ks, Mn, dJn, vol, fa);
+                tangent_qp<NN, NQ, T, true, MASS, LIN, AFF>(q, tile, lane, sl, sU, sP, max_u, jinv_t, detJ_t, dn, quad,
+                                                            lam, mu, mc, ks, Mn, dJn, vol, fa);
         } else {
 #pragma unroll 1
             for (int q = 0; q < NQ; q++)
-                tangent_qp<NN, NQ, T, false, MASS, LIN>(q, tile, lane, sl, sU, sP, max_u, jinv_t, detJ_t, dn, quad,
-                                                        lam, mu, mc, ks, Mn, dJn, vol, fa);
+                tangent_qp<NN, NQ, T, false, MASS, LIN, false>(q, tile, lane, sl, sU, sP, max_u, jinv_t, detJ_t, dn,
+                                                               quad, lam, mu, mc, ks, Mn, dJn, vol, fa);
         }
         if constexpr (MASS == 2) {
             // exact element mass rho V C applied to the element's p, one component at a time
@@ -358,32 +360,38 @@ int launch_tangent_partials(const tlfea_force_plan* pl, const double* x, const d
                 for (int k = 0; k < 3; k++) std_rule = std_rule && dn64.d[q][a][k] == std_dn<10>(q, a, k);
                 std_rule = std_rule && dn64.N[q][a] == std_N<10>(q, a);
             }
-#define TLFEA_TANGENT_LAUNCH(NN_, NQ_, STD_, MASS_, LIN_)                                                    \
-    k_tangent<NN_, NQ_, T, STD_, MASS_, LIN_><<<nb, PLAN_BLOCK, lds, s>>>(                                    \
+#define TLFEA_TANGENT_LAUNCH(NN_, NQ_, STD_, MASS_, LIN_, AFF_)                                              \
+    k_tangent<NN_, NQ_, T, STD_, MASS_, LIN_, AFF_><<<nb, PLAN_BLOCK, lds, s>>>(                              \
         pl->n_elem, pl->d_blk_off, pl->d_unodes, pl->d_slot, x, y, z, X0, Y0, Z0, p, mask, jinv_t, detJ_t, dn, quad, \
         mat, part, pl->max_u, flags, blk_list)
-#define TLFEA_TANGENT_BY_MASS(NN_, NQ_, STD_, LIN_)        \
-    do {                                                   \
-        if (mass == 2)                                     \
-            TLFEA_TANGENT_LAUNCH(NN_, NQ_, STD_, 2, LIN_); \
-        else if (mass == 1)                                \
-            TLFEA_TANGENT_LAUNCH(NN_, NQ_, STD_, 1, LIN_); \
-        else                                               \
-            TLFEA_TANGENT_LAUNCH(NN_, NQ_, STD_, 0, LIN_); \
+#define TLFEA_TANGENT_BY_MASS(NN_, NQ_, STD_, LIN_, AFF_)        \
+    do {                                                         \
+        if (mass == 2)                                           \
+            TLFEA_TANGENT_LAUNCH(NN_, NQ_, STD_, 2, LIN_, AFF_); \
+        else if (mass == 1)                                      \
+            TLFEA_TANGENT_LAUNCH(NN_, NQ_, STD_, 1, LIN_, AFF_); \
+        else                                                     \
+            TLFEA_TANGENT_LAUNCH(NN_, NQ_, STD_, 0, LIN_, AFF_); \
     } while (0)
-        // the rest-state variant has the standard rule only (elsewhere the general kernel
-        // computes the same values from u = 0)
-        if (std_rule && mat.at_rest)
-            TLFEA_TANGENT_BY_MASS(10, 5, true, true);
+        // the rest-state and affine variants have the standard rule only (elsewhere the
+        // general kernel computes the same values from u = 0 and the per-point M)
+        const bool aff = mat.affine != 0;
+        if (std_rule && mat.at_rest && aff)
+            TLFEA_TANGENT_BY_MASS(10, 5, true, true, true);
+        else if (std_rule && mat.at_rest)
+            TLFEA_TANGENT_BY_MASS(10, 5, true, true, false);
+        else if (std_rule && aff)
+            TLFEA_TANGENT_BY_MASS(10, 5, true, false, true);
         else if (std_rule)
-            TLFEA_TANGENT_BY_MASS(10, 5, true, false);
+            TLFEA_TANGENT_BY_MASS(10, 5, true, false, false);
         else
-            TLFEA_TANGENT_BY_MASS(10, 5, false, false);
+            TLFEA_TANGENT_BY_MASS(10, 5, false, false, false);
     } else {
+        // one quadrature point: nothing for the affine variant to skip
         if (mat.at_rest)
-            TLFEA_TANGENT_BY_MASS(4, 1, true, true);
+            TLFEA_TANGENT_BY_MASS(4, 1, true, true, false);
         else
-            TLFEA_TANGENT_BY_MASS(4, 1, true, false);
+            TLFEA_TANGENT_BY_MASS(4, 1, true, false, false);
     }
 #undef TLFEA_TANGENT_BY_MASS
 #undef TLFEA_TANGENT_LAUNCH
@@ -405,9 +413,10 @@ int launch_tangent_planned(const tlfea_force_plan* pl, const double* x, const do
                            const double* X0, const double* Y0, const double* Z0, const double* jinv_t,
                            const double* detJ_t, const tlfea_quad& quad, const tlfea_material& mat,
                            const unsigned char* mask, const double* p, double* out, double* dot_part,
-                           const int* flags, hipStream_t s, bool at_rest) {
+                           const int* flags, hipStream_t s, bool at_rest, bool affine) {
     TanMat<double> tm{mat.lambda, mat.mu, nullptr, nullptr, nullptr, nullptr, 0};
     tm.at_rest = at_rest ? 1 : 0;
+    tm.affine = affine ? 1 : 0;
     if (int rc = launch_tangent_partials<double>(pl, x, y, z, X0, Y0, Z0, jinv_t, detJ_t, quad, tm, mask, p, flags,
                                                  s, nullptr, 0))
         return rc;
@@ -569,7 +578,8 @@ using namespace tlfea_hip;
 
 extern "C" int tlfea_tangent_apply_planned(const tlfea_force_plan* plan, const double* x, const double* y,
                                            const double* z, const double* X0, const double* Y0, const double* Z0,
-                                           const double* jinv_t, const double* detJ_t, const tlfea_quad* quad,
+                                           const double* jinv_t, const double* detJ_t, int affine,
+                                           const tlfea_quad* quad,
                                            const tlfea_material* mat, const unsigned char* fixed_dof_mask,
                                            const double* p, double* out, void* stream) {
     TLFEA_REQUIRE(plan && x && y && z && X0 && Y0 && Z0 && jinv_t && detJ_t && quad && mat && p && out,
@@ -579,5 +589,5 @@ extern "C" int tlfea_tangent_apply_planned(const tlfea_force_plan* plan, const d
     // the linear-elasticity form when the positions are the rest positions' own arrays
     const bool same = x == X0 && y == Y0 && z == Z0;
     return launch_tangent_planned(plan, x, y, z, X0, Y0, Z0, jinv_t, detJ_t, *quad, *mat, fixed_dof_mask, p, out,
-                                  nullptr, nullptr, as_stream(stream), same);
+                                  nullptr, nullptr, as_stream(stream), same, affine != 0);
 }

@@ -51,6 +51,7 @@ class Body:
         self._plan = None
         self._dof_mask = None
         self._jinv = None
+        self._affine = None
         self.precompute()
 
     def set_material(self, E, nu, rho, eta=0.0, lambda_d=0.0):
@@ -160,9 +161,19 @@ class Body:
                  ptr(self.X0[2]), C.byref(self.quad), ptr(self._jinv), self.s)
         return self._jinv
 
-    def tangent_apply(self, p, x=None, out=None, masked=True):
+    @property
+    def affine(self):
+        """True when every element's M and det J agree at all quadrature points."""
+        if self._affine is None:
+            a = C.c_int()
+            call("tlfea_jinv_affine", self.nn, self.n_elem, ptr(self.jinv_t), ptr(self.detJ_t), C.byref(a), self.s)
+            self._affine = bool(a.value)
+        return self._affine
+
+    def tangent_apply(self, p, x=None, out=None, masked=True, affine=True):
         """y = K(x) p without assembly; with masked=True fixed rows return p and fixed
-        columns are dropped (the Dirichlet-eliminated operator)."""
+        columns are dropped (the Dirichlet-eliminated operator); affine: on a mesh of
+        straight-sided elements one quadrature point's M and det J are read."""
         if self.plan is None:
             raise RuntimeError("matrix-free tangent needs a force plan (mesh too scattered for the LDS budget)")
         x = self.x if x is None else x
@@ -170,7 +181,8 @@ class Body:
         mat = self.elastic_material()
         X0 = self.X0
         call("tlfea_tangent_apply_planned", self.plan, ptr(x[0]), ptr(x[1]), ptr(x[2]), ptr(X0[0]), ptr(X0[1]),
-             ptr(X0[2]), ptr(self.jinv_t), ptr(self.detJ_t), C.byref(self.quad), C.byref(mat),
+             ptr(X0[2]), ptr(self.jinv_t), ptr(self.detJ_t), int(affine and self.affine),
+             C.byref(self.quad), C.byref(mat),
              ptr(self.fixed_dof_mask()) if masked else None, ptr(p), ptr(out), self.s)
         return out
 
@@ -322,7 +334,7 @@ class LinearStatic:
             X0 = b.X0
             rc = capi.lib().tlfea_cg_solve_matfree(self.cg, b.plan, ptr(b.x[0]), ptr(b.x[1]), ptr(b.x[2]),
                                                    ptr(X0[0]), ptr(X0[1]), ptr(X0[2]), ptr(b.jinv_t),
-                                                   ptr(b.detJ_t), C.byref(b.quad), C.byref(mat),
+                                                   ptr(b.detJ_t), int(b.affine), C.byref(b.quad), C.byref(mat),
                                                    ptr(b.fixed_dof_mask()), ptr(self.f), ptr(self.u),
                                                    float(self.tol), int(self.max_iter), int(self.check_every),
                                                    C.byref(it), C.byref(res), b.s)
