@@ -84,6 +84,12 @@ def batched_beams(n, tol, log, sequential=True):
     b = Batch(items)
     res = {"workload": f"{n} cantilevers of the beam sampler (seed 2026): {b.n_elem} TET10 elements, "
                        f"{3 * b.n_nodes} DOF in total; linear-static solves to {tol:g}"}
+    # chunk size, graph-replay length and compaction ratio chosen by the autotuner (setup,
+    # not part of the timed solves)
+    t = time.perf_counter()
+    tune = b.autotune(trial_iters=64, tol=tol, full_solves=True)
+    res["autotune"] = dict(tune, tuning_s=time.perf_counter() - t)
+    log(f"autotune: {res['autotune']}")
 
     def timed(tag, fn):
         fn()  # warm-up

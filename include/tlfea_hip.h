@@ -270,6 +270,23 @@ int tlfea_batch_destroy(tlfea_batch* b);
 int tlfea_batch_sizes(const tlfea_batch* b, int* n_inst, int* n_elem, int* n_nodes, int* n_chunks);
 /* Nodes per reduction chunk (>= 64): the granularity of the per-instance dot products. */
 int tlfea_batch_set_chunk(tlfea_batch* b, int chunk_nodes);
+/* Autotuner (not in the reference): times batched CG iterations for every chunk size in
+ * {256, 512, 1024, 2048, 4096} nodes and graph-replay length in {16, 32, 64} iterations
+ * (runs of n and 2n iterations from u = 0 on load f, n >= trial_iters, no instance
+ * finishing; the difference cancels the per-solve graph capture), keeps the fastest pair
+ * (chunk via tlfea_batch_set_chunk, replay length as the default of solves that pass
+ * poll <= 0), and with full_solves != 0 then times full solves to tol for compaction
+ * ratios {0.35, 0.5, 0.65, 0.8, 0.9} and keeps the fastest.  u is a scratch output
+ * [3*n_nodes]; the batch's solution state is overwritten. */
+typedef struct tlfea_batch_tuning {
+    int chunk_nodes;    /* nodes per vector-kernel chunk */
+    int poll;           /* iterations per graph replay between host checks */
+    double compaction;  /* compaction ratio */
+    double us_per_iter; /* time of one batched iteration, all instances active */
+    double solve_s;     /* full-solve time at the chosen ratio (0 without full_solves) */
+} tlfea_batch_tuning;
+int tlfea_batch_autotune(tlfea_batch* b, const double* f, double* u, int trial_iters, double tol, int full_solves,
+                         tlfea_batch_tuning* out, void* stream);
 /* Finished instances are dropped from the launches once the unfinished ones hold less
  * than `ratio` of the launched elements (default 0.6; 0 disables). */
 int tlfea_batch_set_compaction(tlfea_batch* b, double ratio);
@@ -305,7 +322,8 @@ int tlfea_batch_smooth(tlfea_batch* b, const double* f, double* u, int sweeps, v
  * the reference's CG per instance with its own alpha/beta, stopping when
  * ||f_i - K_i u_i|| / ||f_i|| < tol on the free DOFs.  use_guess != 0 starts from u
  * (else from 0).  Finished instances stop updating; the host polls every `poll`
- * iterations, which are replayed from a captured hipGraph when use_graph != 0.  With
+ * iterations (poll <= 0: the batch's default, 32 or the autotuner's choice), which are
+ * replayed from a captured hipGraph when use_graph != 0.  With
  * precond = 1 the CG is the flexible (Polak-Ribiere) variant preconditioned by the FP32
  * Chebyshev stage of tlfea_batch_precond_setup; u, r, alpha, beta and the stopping test
  * stay FP64.

@@ -203,3 +203,25 @@ def test_batch_fp32_chebyshev_preconditioner(gpu):
     b.set_compaction(0.0)
     u2 = _np(b.solve(tol=1e-10, precond=True, graph=False))
     assert rel_err(u2, u1) < 1e-7
+
+
+def test_batch_autotune(gpu):
+    """The autotuner picks a configuration from its candidate sets, applies it, and a solve
+    afterwards (default replay length = the chosen one) gives the solution of an untuned
+    batch to rounding; the choice is deterministic in kind (a valid chunk and replay length)
+    whatever the timings."""
+    from tlfea_amd.batch import Batch
+    items = _items(seed=7, n=6, big=True)
+    ref = Batch(items)
+    u_ref = _np(ref.solve(tol=1e-10))
+    b = Batch(items)
+    t = b.autotune(trial_iters=16, tol=1e-8, full_solves=True)
+    print(f"[autotune] {t}", flush=True)
+    assert t["chunk_nodes"] in (256, 512, 1024, 2048, 4096) and t["poll"] in (16, 32, 64)
+    assert t["compaction"] in (0.35, 0.5, 0.65, 0.8, 0.9) and t["us_per_iter"] > 0 and t["solve_s"] > 0
+    assert b.n_chunks() == sum(-(-m.n_nodes // t["chunk_nodes"]) for m, _ in items)
+    u = _np(b.solve(tol=1e-10))
+    assert (b.status == 0).all()
+    for i in range(len(items)):
+        sl = b.inst_slice(i)
+        assert rel_err(u[sl], u_ref[sl]) < 1e-8, i

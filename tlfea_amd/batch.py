@@ -81,6 +81,17 @@ class Batch:
     def set_compaction(self, ratio):
         call("tlfea_batch_set_compaction", self.h, float(ratio))
 
+    def autotune(self, f=None, trial_iters=64, tol=1e-8, full_solves=False):
+        """Chooses chunk size, graph-replay length and (with full_solves) the compaction
+        ratio by timing batched iterations (tlfea_batch_autotune); returns the choice."""
+        f = self.f if f is None else f
+        u = torch.empty(3 * self.n_nodes, dtype=torch.float64, device=DEV)
+        t = capi.BatchTuning()
+        call("tlfea_batch_autotune", self.h, ptr(f), ptr(u), int(trial_iters), float(tol), int(bool(full_solves)),
+             C.byref(t), self.stream)
+        return dict(chunk_nodes=t.chunk_nodes, poll=t.poll, compaction=t.compaction, us_per_iter=t.us_per_iter,
+                    solve_s=t.solve_s)
+
     def n_chunks(self):
         a = C.c_int()
         call("tlfea_batch_sizes", self.h, None, None, None, C.byref(a))
@@ -129,17 +140,18 @@ class Batch:
         call("tlfea_batch_smooth", self.h, ptr(f), ptr(u), int(sweeps), self.stream)
         return u
 
-    def solve(self, tol=1e-8, max_iter=100000, u0=None, poll=32, graph=True, f=None, out=None, precond=False):
+    def solve(self, tol=1e-8, max_iter=100000, u0=None, poll=None, graph=True, f=None, out=None, precond=False):
         """Returns u (device, all instances); per-instance iterations, relative residuals
         and status land in self.iters / self.rel / self.status.  precond=True uses the
-        FP32 Chebyshev stage set up by precond_setup() inside a flexible FP64 CG."""
+        FP32 Chebyshev stage set up by precond_setup() inside a flexible FP64 CG.  poll:
+        iterations per graph replay (None: the batch's default or the autotuner's)."""
         f = self.f if f is None else f
         u = out if out is not None else torch.empty(3 * self.n_nodes, dtype=torch.float64, device=DEV)
         if u0 is not None:
             u.copy_(u0)
         tot = C.c_int()
         rc = capi.lib().tlfea_batch_cg_solve(self.h, ptr(f), ptr(u), int(u0 is not None), float(tol), int(max_iter),
-                                             int(poll), int(bool(graph)), int(bool(precond)), self.iters.ctypes.data,
+                                             int(poll or 0), int(bool(graph)), int(bool(precond)), self.iters.ctypes.data,
                                              self.rel.ctypes.data, self.status.ctypes.data, C.byref(tot),
                                              self.stream)
         self.total_iters = tot.value

@@ -46,6 +46,11 @@ class Body(C.Structure):
                 ("Z0", C.c_void_p)]
 
 
+class BatchTuning(C.Structure):
+    _fields_ = [("chunk_nodes", C.c_int), ("poll", C.c_int), ("compaction", C.c_double),
+                ("us_per_iter", C.c_double), ("solve_s", C.c_double)]
+
+
 class NesterovParams(C.Structure):
     """Same layout as the reference's SyncedNesterovParams (SyncedNesterov.cuh:31-35)."""
     _fields_ = [("alpha", C.c_double), ("rho", C.c_double), ("inner_tol", C.c_double), ("outer_tol", C.c_double),
@@ -128,6 +133,7 @@ _SIGS = {
     "tlfea_batch_tangent_apply": (i32, [vp, vp, vp, vp]),
     "tlfea_batch_cg_solve": (i32, [vp, vp, vp, i32, f64, i32, i32, i32, i32, vp, vp, vp, C.POINTER(i32), vp]),
     "tlfea_batch_set_compaction": (i32, [vp, f64]),
+    "tlfea_batch_autotune": (i32, [vp, vp, vp, i32, f64, i32, C.POINTER(BatchTuning), vp]),
     "tlfea_batch_smooth": (i32, [vp, vp, vp, i32, vp]),
     "tlfea_batch_set_mass": (i32, [vp, vp, i32]),
     "tlfea_batch_apply": (i32, [vp, f64, i32, i32, vp, vp, vp]),

@@ -16,6 +16,7 @@
 // DOFs (the reference's sqrt(rr/rr0) when the initial guess is zero).  An optional FP32
 // Chebyshev preconditioner applies the same operator in single precision.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <vector>
 
@@ -61,6 +62,7 @@ struct tlfea_batch {
     int n_blk_list = 0, n_chunk_list = 0, n_inst_list = 0;
     int64_t launched_elems = 0;
     double compact_ratio = 0.6;  // compact when active elements fall below this share
+    int poll_default = 32;       // iterations per graph replay when a solve passes poll <= 0
     hipStream_t s_own = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_iters = 0, graph_precond = -1;
@@ -841,6 +843,75 @@ extern "C" int tlfea_batch_set_chunk(tlfea_batch* b, int chunk_nodes) {
     return build_chunks(b, chunk_nodes);
 }
 
+extern "C" int tlfea_batch_autotune(tlfea_batch* b, const double* f, double* u, int trial_iters, double tol,
+                                    int full_solves, tlfea_batch_tuning* out, void* stream) {
+    TLFEA_REQUIRE(b && f && u && out, "null argument");
+    TLFEA_REQUIRE(trial_iters >= 8 && trial_iters <= 65536, "trial iterations out of range [8, 65536]");
+    TLFEA_REQUIRE(!full_solves || tol > 0.0, "full-solve trials need a positive tolerance");
+    const std::vector<int> chunk_set = {256, 512, 1024, 2048, 4096};
+    const std::vector<int> poll_set = {16, 32, 64};
+    const std::vector<double> ratio_set = {0.35, 0.5, 0.65, 0.8, 0.9};
+    std::vector<int> it(b->n_inst), st(b->n_inst);
+    std::vector<double> rr(b->n_inst);
+    int total = 0;
+    using clock = std::chrono::steady_clock;
+    // wall time of one solve of n iterations (tol 0: no instance finishes, nothing is
+    // compacted); the call returns after its final device-to-host copies
+    auto timed = [&](int n, int poll, double t, double& sec) -> int {
+        const auto t0 = clock::now();
+        const int rc = tlfea_batch_cg_solve(b, f, u, 0, t, n, poll, 1, 0, it.data(), rr.data(), st.data(), &total,
+                                            stream);
+        sec = std::chrono::duration<double>(clock::now() - t0).count();
+        return (rc == TLFEA_OK || rc == TLFEA_ENOCONV) ? TLFEA_OK : rc;
+    };
+    const double keep_ratio = b->compact_ratio;
+    b->compact_ratio = 0.0;
+    double best = 1e300;
+    int best_chunk = b->chunk_nodes, best_poll = b->poll_default;
+    for (int chunk : chunk_set) {
+        if (int rc = build_chunks(b, chunk)) return rc;
+        for (int poll : poll_set) {
+            // per-iteration time from runs of n and 2n iterations: the graph capture every
+            // solve pays (and the host round trips at its ends) cancel in the difference
+            const int n = poll * ((trial_iters + poll - 1) / poll);
+            double t1 = 0.0, t2 = 0.0;
+            if (int rc = timed(n, poll, 0.0, t1)) return rc;
+            if (int rc = timed(2 * n, poll, 0.0, t2)) return rc;
+            const double per = (t2 - t1) / n;
+            if (per < best) {
+                best = per;
+                best_chunk = chunk;
+                best_poll = poll;
+            }
+        }
+    }
+    if (int rc = build_chunks(b, best_chunk)) return rc;
+    b->poll_default = best_poll;
+    b->compact_ratio = keep_ratio;
+    out->chunk_nodes = best_chunk;
+    out->poll = best_poll;
+    out->us_per_iter = best * 1e6;
+    out->compaction = keep_ratio;
+    out->solve_s = 0.0;
+    if (full_solves) {
+        // the compaction ratio decides the tail of uneven batches: full solves to tol
+        double best_s = 1e300, best_r = keep_ratio;
+        for (double r : ratio_set) {
+            b->compact_ratio = r;
+            double sec = 0.0;
+            if (int rc = timed(1 << 30, best_poll, tol, sec)) return rc;
+            if (sec < best_s) {
+                best_s = sec;
+                best_r = r;
+            }
+        }
+        b->compact_ratio = best_r;
+        out->compaction = best_r;
+        out->solve_s = best_s;
+    }
+    return TLFEA_OK;
+}
+
 extern "C" int tlfea_batch_precond_setup(tlfea_batch* b, int degree, double ratio, int power_iters,
                                          double safety, double* lambda_max, void* stream) {
     TLFEA_REQUIRE(b != nullptr, "null batch");
@@ -954,7 +1025,7 @@ extern "C" int tlfea_batch_cg_solve(tlfea_batch* b, const double* f, double* u, 
     TLFEA_REQUIRE(b && f && u, "null argument");
     TLFEA_REQUIRE(precond == 0 || b->cheb_degree > 0, "preconditioner requested but not set up");
     TLFEA_REQUIRE(max_iter >= 0 && tol >= 0.0, "bad tolerance or iteration limit");
-    if (poll <= 0) poll = 16;
+    if (poll <= 0) poll = b->poll_default;
     hipStream_t caller = as_stream(stream);
     hipStream_t s = b->s_own;
     const double compact_ratio = b->compact_ratio;
