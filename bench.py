@@ -16,13 +16,14 @@ reported beside it.  Inputs are resident in HBM when the timed region starts.
 Kernel rooflines are measured live with HIP events on the launch stream (the null stream,
 which torch.cuda.Event records on):
   * roofline      -- the step's dominant kernel pair, the matrix-free tangent at the rest
-                     positions (k_tangent<10,5,double,..,LIN> + k_gather_tangent: the
-                     kernel's linear-elasticity form, chosen when x equals X0),
-                     bytes/launch = 440*n_elem + 48*n_nodes (inverse Jacobians 360 B,
-                     detJ 40 B and connectivity 40 B per element; p read and y written,
-                     48 B per node);
+                     positions (k_tangent<10,5,double,..,LIN,AFF> + k_gather_tangent: the
+                     kernel's linear-elasticity form, chosen when x equals X0, on an affine
+                     mesh), bytes/launch = 120*n_elem + 48*n_nodes (inverse Jacobian 72 B
+                     and detJ 8 B of one quadrature point -- the beam's elements are
+                     straight-sided, so all five agree -- and connectivity 40 B per
+                     element; p read and y written, 48 B per node);
   * tangent_general -- the same kernel away from the rest positions (x and X0 read too:
-                     440*n_elem + 96*n_nodes);
+                     120*n_elem + 96*n_nodes);
   * spmv_fp64     -- the FP64 block SpMV of the plain path, bytes/launch =
                      76*node_nnz + 52*n_nodes;
   * force_kernel  -- the fused FP64 P + f_int (block-local plan: force + node gather) with
@@ -58,9 +59,9 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes of one matrix-free tangent launch pair (k_tangent + k_gather_tangent) on the
 # 200x20x20 beam: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (KiB), scales calibrated
 # on a 512 MiB copy
-PMC_TANGENT_BYTES = (152015 * 2.0 + 29109 + 30817 * 2.0 + 21561) * 1024
+PMC_TANGENT_BYTES = (76981 * 2.0 + 29109 + 30795 * 2.0 + 21629) * 1024
 # the same for the rest-state (linear-elasticity) form the linear-static step runs
-PMC_TANGENT_REST_BYTES = (118710 * 2.0 + 29111 + 30810 * 2.0 + 21438) * 1024
+PMC_TANGENT_REST_BYTES = (43693 * 2.0 + 29109 + 30795 * 2.0 + 21629) * 1024
 # the same for the force kernels with k_gather_partials: gradient-reading (k_force_planned)
 # and rest-coordinate (k_force_rest)
 PMC_FORCE_BYTES = (314624 * 2.0 + 29110 + 21891 * 2.0 + 15799) * 1024
@@ -299,18 +300,20 @@ def main():
         body.x[d].copy_(X0[d])
     p = torch.randn(3 * m.n_nodes, dtype=torch.float64, device="cuda")
     y = torch.empty_like(p)
-    # matrix-free tangent as the step runs it (rest positions: the linear-elasticity form):
-    # inverse Jacobians (360 B) + detJ (40 B) + connectivity (40 B) per element, p read and
-    # y written once (48 B) per node
+    # matrix-free tangent as the step runs it (rest positions: the linear-elasticity form;
+    # affine mesh: one quadrature point's geometry): inverse Jacobian (72 B) + detJ (8 B)
+    # per element and quadrature point read (one point when affine, else 5) + connectivity
+    # (40 B) per element, p read and y written once (48 B) per node
+    elem_bytes = (80 if body.affine else 400) + 40
     tan_s = event_time(lambda: body.tangent_apply(p, x=body.X0, out=y))
-    tan_bytes = 440 * m.n_elem + 48 * m.n_nodes
+    tan_bytes = elem_bytes * m.n_elem + 48 * m.n_nodes
     tan_gbs = tan_bytes / tan_s / 1e9
     # the general form (positions away from the rest positions; the step's kernel when a
     # solve starts from a deformed state): x and X0 read as well, 96 B per node
     for d in range(3):
         body.x[d].add_(1e-6)
     tgen_s = event_time(lambda: body.tangent_apply(p, out=y))
-    tgen_bytes = 440 * m.n_elem + 96 * m.n_nodes
+    tgen_bytes = elem_bytes * m.n_elem + 96 * m.n_nodes
     tgen_gbs = tgen_bytes / tgen_s / 1e9
     for d in range(3):
         body.x[d].copy_(X0[d])
@@ -399,12 +402,13 @@ def main():
                    "speedup_vs_plain_fp64": dt64 / dt},
         "roofline": {"bound": "hbm", "achieved": tan_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": tan_gbs / HBM_PEAK_GBS, "traffic": traffic_tan,
-                     "kernel": "k_tangent<10,5,double,true,0,true> + k_gather_tangent (matrix-free K p at the rest "
-                               "positions, the linear-elasticity form of the kernel)",
+                     "kernel": "k_tangent<10,5,double,true,0,true,true> + k_gather_tangent (matrix-free K p at the "
+                               "rest positions, the linear-elasticity form, one quadrature point's geometry)",
                      "avg_us": tan_s * 1e6, "bytes_per_launch": tan_bytes},
         "tangent_general": {"bound": "hbm", "achieved": tgen_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": tgen_gbs / HBM_PEAK_GBS, "traffic": PMC_TANGENT_BYTES if default else None,
-                            "kernel": "k_tangent<10,5,double,true,0,false> + k_gather_tangent (K(x) p away from rest)",
+                            "kernel": "k_tangent<10,5,double,true,0,false,true> + k_gather_tangent (K(x) p away from "
+                                      "rest)",
                             "avg_us": tgen_s * 1e6, "bytes_per_launch": tgen_bytes},
         "spmv_fp64": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                       "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_cg_spmv/k_spmv_block",

@@ -36,8 +36,9 @@ struct tlfea_cg {
     float* d_r32 = nullptr;   // mixed-precision CG: FP32 recursive residual
     float* d_p32 = nullptr;   // FP32 direction
     float* d_q32 = nullptr;   // FP32 K p
-    double* d_part = nullptr;  // [RED_BLOCKS]
-    double* d_scal = nullptr;  // [8]: rr0, rr, alpha, beta, rel, pKp
+    double* d_part = nullptr;   // [RED_BLOCKS + operator blocks]: operator p.Kp partials
+    double* d_part2 = nullptr;  // [RED_BLOCKS]: r.r partials of the update
+    double* d_scal = nullptr;   // [16]: rr0, rr, alpha, beta, rel, pKp; 10, 11: rr by iteration parity
     int* d_flags = nullptr;    // [4]: 0 done, 1 notpd, 2 iters
 };
 
@@ -575,6 +576,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_cg_init_fin(int nparts, const d
     if (threadIdx.x == 0) {
         scal[0] = rr;
         scal[1] = rr;
+        scal[10] = rr;
         scal[4] = (rr == 0.0) ? 0.0 : 1.0;
         flags[0] = (rr == 0.0) ? 1 : 0;
         flags[1] = 0;
@@ -611,30 +613,52 @@ __global__ __launch_bounds__(256) void k_cg_spmv(int64_t n_slices, const int* __
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// alpha = rr / pKp, abort flag on pKp <= 0
-__global__ __launch_bounds__(RED_THREADS) void k_cg_alpha(int nparts, const double* __restrict__ part,
-                                                          double* __restrict__ scal, int* __restrict__ flags) {
-    if (flags[0]) return;
-    const double pKp = sum_parts(part, nparts);
-    if (threadIdx.x == 0) {
-        scal[5] = pKp;
-        if (pKp <= 0.0) {
-            flags[1] = 1;
-            flags[0] = 1;
-        } else {
-            scal[2] = scal[1] / pKp;
-        }
-    }
+// flags[0] as one value for the whole block: block 0 of the fused kernels below sets it
+// while other blocks of the same launch start, so every thread must take the early return
+// (or not) together -- the block synchronises afterwards.
+__device__ __forceinline__ bool block_stopped(const int* flags) {
+    __shared__ int stop;
+    if (threadIdx.x == 0) stop = flags[0];
+    __syncthreads();
+    return stop != 0;
 }
 
-// u += alpha p, r -= alpha Kp, partials of r.r
+// The block-order sum of the partials, in every thread of the block.
+__device__ __forceinline__ double sum_parts_all(const double* part, int n) {
+    __shared__ double bcast;
+    const double s = sum_parts(part, n);
+    if (threadIdx.x == 0) bcast = s;
+    __syncthreads();
+    return bcast;
+}
+
+// alpha = rr / p.Kp (abort flag on p.Kp <= 0), then u += alpha p, r -= alpha Kp and the
+// partials of r.r.  Every block sums the operator's partials itself, in the same fixed
+// order, so all blocks hold the same alpha and no one-block kernel sits between the
+// operator and the update.  rr of this iteration is scal[10 + par] (par = iteration & 1);
+// only block 0 writes scalars.
 __global__ __launch_bounds__(RED_THREADS) void k_cg_update(int64_t n, double* __restrict__ u,
                                                            double* __restrict__ r, const double* __restrict__ p,
-                                                           const double* __restrict__ Kp, double* __restrict__ part,
-                                                           const double* __restrict__ scal,
-                                                           const int* __restrict__ flags) {
-    if (flags[0]) return;
-    const double alpha = scal[2];
+                                                           const double* __restrict__ Kp,
+                                                           const double* __restrict__ in_part, int n_in,
+                                                           double* __restrict__ part, double* __restrict__ scal,
+                                                           int* __restrict__ flags, int par) {
+    if (block_stopped(flags)) return;
+    const double pKp = sum_parts_all(in_part, n_in);
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (pKp <= 0.0) {
+        if (lead) {
+            scal[5] = pKp;
+            flags[1] = 1;
+            flags[0] = 1;
+        }
+        return;
+    }
+    const double alpha = scal[10 + par] / pKp;
+    if (lead) {
+        scal[5] = pKp;
+        scal[2] = alpha;
+    }
     double s = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         u[i] += alpha * p[i];
@@ -646,31 +670,34 @@ __global__ __launch_bounds__(RED_THREADS) void k_cg_update(int64_t n, double* __
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// convergence test sqrt(rr'/rr0) < tol (LinearStaticSolver.cu:418-423) and beta = rr'/rr
-__global__ __launch_bounds__(RED_THREADS) void k_cg_beta(int nparts, const double* __restrict__ part,
-                                                         double* __restrict__ scal, int* __restrict__ flags,
-                                                         double tol) {
-    if (flags[0]) return;
-    const double rn = sum_parts(part, nparts);
-    if (threadIdx.x == 0) {
-        const double rr0 = scal[0];
-        const double rel = rr0 > 0.0 ? sqrt(rn / rr0) : sqrt(rn);
+// Convergence test sqrt(rr'/rr0) < tol (LinearStaticSolver.cu:418-423), beta = rr'/rr and
+// p = beta p + r, with rr' summed by every block from the update's partials (as in
+// k_cg_update).  The next iteration's rr goes to scal[10 + (par ^ 1)], a slot no block of
+// this launch reads.
+__global__ __launch_bounds__(RED_THREADS) void k_cg_dir(int64_t n, double* __restrict__ p,
+                                                        const double* __restrict__ r,
+                                                        const double* __restrict__ part, int n_part,
+                                                        double* __restrict__ scal, int* __restrict__ flags,
+                                                        double tol, int par) {
+    if (block_stopped(flags)) return;
+    const double rn = sum_parts_all(part, n_part);
+    const double rr0 = scal[0];
+    const double rel = rr0 > 0.0 ? sqrt(rn / rr0) : sqrt(rn);
+    const bool done = rel < tol;
+    const double rr = scal[10 + par];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         scal[4] = rel;
         flags[2] += 1;
-        if (rel < tol) {
+        if (done) {
             flags[0] = 1;
         } else {
-            scal[3] = rn / scal[1];
+            scal[3] = rn / rr;
             scal[1] = rn;
+            scal[10 + (par ^ 1)] = rn;
         }
     }
-}
-
-// p = beta p + r
-__global__ void k_cg_dir(int64_t n, double* __restrict__ p, const double* __restrict__ r,
-                         const double* __restrict__ scal, const int* __restrict__ flags) {
-    if (flags[0]) return;
-    const double beta = scal[3];
+    if (done) return;
+    const double beta = rn / rr;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = beta * p[i] + r[i];
 }
@@ -1288,6 +1315,7 @@ extern "C" int tlfea_cg_create(const tlfea_pattern* p, tlfea_cg** out) {
     int rc = TLFEA_OK;
     if ((rc = dmalloc(&c->d_r, c->n)) || (rc = dmalloc(&c->d_p, c->n)) || (rc = dmalloc(&c->d_Kp, c->n)) ||
         (rc = dmalloc(&c->d_part, RED_BLOCKS + (int)(p->n_slices * TILE / 256) + 1)) ||
+        (rc = dmalloc(&c->d_part2, RED_BLOCKS)) ||
         (rc = dmalloc(&c->d_scal, 16)) || (rc = dmalloc(&c->d_flags, 8)) || (rc = dmalloc(&c->d_r32, c->n)) ||
         (rc = dmalloc(&c->d_p32, c->n)) || (rc = dmalloc(&c->d_q32, c->n))) {
         tlfea_cg_destroy(c);
@@ -1305,6 +1333,7 @@ extern "C" int tlfea_cg_destroy(tlfea_cg* c) {
     hipFree(c->d_p);
     hipFree(c->d_Kp);
     hipFree(c->d_part);
+    hipFree(c->d_part2);
     hipFree(c->d_scal);
     hipFree(c->d_flags);
     hipFree(c->d_r32);
@@ -1333,10 +1362,10 @@ extern "C" int tlfea_cg_solve(tlfea_cg* c, const double* k_sell, const double* f
     for (int it = 0; it < max_iter; it++) {
         k_cg_spmv<<<gs, 256, 0, s>>>(p->n_slices, p->d_sell_row, p->d_slice_base, p->d_slice_len, p->d_sell_col,
                                       k_sell, c->d_p, c->d_Kp, c->d_part, c->d_flags);
-        k_cg_alpha<<<1, RED_THREADS, 0, s>>>(gs, c->d_part, c->d_scal, c->d_flags);
-        k_cg_update<<<gv, RED_THREADS, 0, s>>>(n, u, c->d_r, c->d_p, c->d_Kp, c->d_part, c->d_scal, c->d_flags);
-        k_cg_beta<<<1, RED_THREADS, 0, s>>>(gv, c->d_part, c->d_scal, c->d_flags, tol);
-        k_cg_dir<<<gv, 256, 0, s>>>(n, c->d_p, c->d_r, c->d_scal, c->d_flags);
+        k_cg_update<<<gv, RED_THREADS, 0, s>>>(n, u, c->d_r, c->d_p, c->d_Kp, c->d_part, (int)gs, c->d_part2,
+                                               c->d_scal, c->d_flags, it & 1);
+        k_cg_dir<<<gv, RED_THREADS, 0, s>>>(n, c->d_p, c->d_r, c->d_part2, (int)gv, c->d_scal, c->d_flags, tol,
+                                            it & 1);
         TLFEA_LAUNCH_CHECK();
         if ((it + 1) % check_every == 0 || it + 1 == max_iter) {
             TLFEA_HIP_TRY(hipMemcpyAsync(flags, c->d_flags, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
@@ -1406,10 +1435,10 @@ extern "C" int tlfea_cg_solve_matfree(tlfea_cg* c, const tlfea_force_plan* plan,
         int rc = launch_tangent_planned(plan, x, y, z, X0, Y0, Z0, jinv_t, detJ_t, *quad, *mat, fixed_dof_mask,
                                         c->d_p, c->d_Kp, c->d_part, c->d_flags, s, at_rest, affine != 0);
         if (rc != TLFEA_OK) return rc;
-        k_cg_alpha<<<1, RED_THREADS, 0, s>>>(gn, c->d_part, c->d_scal, c->d_flags);
-        k_cg_update<<<gv, RED_THREADS, 0, s>>>(n, u, c->d_r, c->d_p, c->d_Kp, c->d_part, c->d_scal, c->d_flags);
-        k_cg_beta<<<1, RED_THREADS, 0, s>>>(gv, c->d_part, c->d_scal, c->d_flags, tol);
-        k_cg_dir<<<gv, 256, 0, s>>>(n, c->d_p, c->d_r, c->d_scal, c->d_flags);
+        k_cg_update<<<gv, RED_THREADS, 0, s>>>(n, u, c->d_r, c->d_p, c->d_Kp, c->d_part, gn, c->d_part2, c->d_scal,
+                                               c->d_flags, it & 1);
+        k_cg_dir<<<gv, RED_THREADS, 0, s>>>(n, c->d_p, c->d_r, c->d_part2, (int)gv, c->d_scal, c->d_flags, tol,
+                                            it & 1);
         TLFEA_LAUNCH_CHECK();
         if ((it + 1) % check_every == 0 || it + 1 == max_iter) {
             TLFEA_HIP_TRY(hipMemcpyAsync(flags, c->d_flags, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
