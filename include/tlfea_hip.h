@@ -307,6 +307,17 @@ int tlfea_batch_set_mass(tlfea_batch* b, const double* rho_coef, int rule);
  * full operator (e.g. the consistent gravity load M g, which sums over all nodes). */
 int tlfea_batch_apply(tlfea_batch* b, double k_scale, int with_mass, int masked, const double* p, double* out,
                       void* stream);
+/* Displacement of every instance from its rest positions (device SoA [n_nodes] each,
+ * copied): from then on the batch's operator (tlfea_batch_cg_solve, tlfea_batch_apply, the
+ * preconditioner's diagonal) is the tangent K(X + u) (compute_hessian_assemble_csr,
+ * FEAT10DataFunc.cuh:512-768, per instance) instead of K at rest; NULLs return to rest.
+ * The kernels take u itself, not x = X + u: a displacement many orders below the
+ * coordinates would lose its digits in x - X.  For Newton iterations of implicit steps. */
+int tlfea_batch_set_displacement(tlfea_batch* b, const double* ux, const double* uy, const double* uz, void* stream);
+/* f = f_int at the current positions (CalcInternalForce, FEAT10Data.cu:307-323, with each
+ * instance's material; gradients rebuilt from the rest positions as in
+ * tlfea_internal_force_rest), all DOFs (no Dirichlet masking), overwritten. */
+int tlfea_batch_internal_force(tlfea_batch* b, double* f, void* stream);
 /* FP32 Jacobi-Chebyshev preconditioner for tlfea_batch_cg_solve(precond = 1): the
  * diagonal D of K (FP64, matrix-free), lambda_max(D^-1 K) per instance by `power_iters`
  * FP32 power iterations (scaled by `safety` >= 1), and `degree` Chebyshev steps on

@@ -121,3 +121,112 @@ def test_batch_steps_match_single_beams(gpu):
             assert rel_err(_np(v[sl]), _np(vs)) < 1e-7, (k, i)
     its = np.array(dyn.iters)
     assert its.max() < 2000 and its.min() >= 1
+
+
+def _newton_steps_cpu(m, mat, amp, freq, dt, nsteps, tol=1e-13, g=(0.0, 0.0, -9.81)):
+    """Implicit Euler g(v) = M (v - v_n)/dt + f_int(X + u_n + dt v) - f(t1) = 0 solved densely by
+    Newton (SyncedNesterov.cu:32-95's equation; exact consistent mass; f_int from the
+    longdouble restatement, the tangent from the oracle's assembly at x)."""
+    from helpers import fint_extended
+    lam, mu = lame(mat["E"], mat["nu"])
+    K0, M = _dense_KM(m, mat, "exact")
+    n3 = 3 * m.n_nodes
+    free = np.ones(n3, bool)
+    for nd in np.asarray(m.fixed):
+        free[3 * nd:3 * nd + 3] = False
+    F = np.asarray(m.f_ext).reshape(-1, 3).sum(0)
+    tip = np.asarray(m.f_ext) / np.linalg.norm(F)
+    fg = M @ np.tile(np.asarray(g), m.n_nodes)
+    fg[~free] = 0.0
+    X = m.xyz()
+    go, djo = O.grad(m)
+    doff, dcol = O.dof_pattern(m)
+    u = np.zeros(n3)
+    v = np.zeros(n3)
+    traj = []
+    for k in range(nsteps):
+        t1 = (k + 1) * dt
+        f1 = fg + tip * amp * np.sin(2 * np.pi * freq * t1)
+        vn = v.copy()
+        w = vn.copy()
+
+        def resid(w):
+            x = X + (u + dt * w).reshape(-1, 3)
+            r = M @ (w - vn) / dt + fint_extended(m, x[:, 0], x[:, 1], x[:, 2], lam, mu).astype(np.float64) - f1
+            r[~free] = 0.0
+            return r, x
+
+        r, x = resid(w)
+        r0 = np.linalg.norm(r)
+        for _ in range(20):
+            if np.linalg.norm(r) <= tol * r0:
+                break
+            kv = O.stiffness_values(m, go, djo, lam, mu, doff, dcol, x[:, 0].copy(), x[:, 1].copy(), x[:, 2].copy())
+            Kt = np.zeros((n3, n3))
+            for row in range(n3):
+                Kt[row, dcol[doff[row]:doff[row + 1]]] = kv[doff[row]:doff[row + 1]]
+            A = (M / dt + dt * Kt)[np.ix_(free, free)]
+            dw = np.zeros(n3)
+            dw[free] = np.linalg.solve(A, -r[free])
+            w = w + dw
+            r, x = resid(w)
+        v = w
+        u = u + dt * v
+        traj.append(u.copy())
+    return traj
+
+
+def test_newton_steps_match_dense_nonlinear_euler(gpu):
+    """Batched Newton steps (batched f_int at x, K(x) through tlfea_batch_set_displacement) against
+    a dense Newton solve of the same nonlinear implicit-Euler equations: 1e-8 over 8 steps;
+    the linearised steps' distance from that trajectory is the linearisation error."""
+    from tlfea_amd.dynamics import BatchDynamics
+    m, mat = _beam(5, 1, 1, [0.3, -0.5, -1.0], E=2e7, rho=1500.0)
+    dt, amp, freq, nsteps = 1e-4, 400.0, 12.0, 8
+    ref = _newton_steps_cpu(m, mat, amp, freq, dt, nsteps)
+    dyn = BatchDynamics([(m, mat)], amp=[amp], freq=[freq], dt=dt, nonlinear=True, tol_nl=1e-10,
+                        newton_cg_tol=1e-13)
+    lin = BatchDynamics([(m, mat)], amp=[amp], freq=[freq], dt=dt, tol=1e-13)
+    free = _np(dyn.b.mask) == 0
+    worst_lin = 0.0
+    for k in range(nsteps):
+        u, _ = dyn.step()
+        ul, _ = lin.step()
+        assert rel_err(_np(u)[free], ref[k][free]) < 1e-8, k
+        worst_lin = max(worst_lin, rel_err(_np(ul)[free], ref[k][free]))
+    print(f"[newton] iterations per step {dyn.newton}, at the FP64 floor {dyn.floor}; linearised steps off by "
+          f"{worst_lin:.2e}", flush=True)
+    assert max(dyn.newton) <= 4 and worst_lin < 1e-2
+
+
+def test_batch_internal_force_and_moved_operator(gpu):
+    """tlfea_batch_internal_force equals each instance's own f_int (oracle) at deformed
+    positions; with the positions set, the batched operator equals the oracle's tangent
+    assembled there."""
+    import torch
+    from helpers import deformed
+    from tlfea_amd.batch import Batch
+    items = [_beam(4, 2, 1, [0, 0, -10.0], E=E, nu=nu) for E, nu in ((2e9, 0.3), (5e7, 0.42))]
+    b = Batch(items)
+    xs, fo, yo = [], [], []
+    p = np.random.default_rng(9).standard_normal(3 * b.n_nodes)
+    for i, (m, mat) in enumerate(items):
+        xd = deformed(m, seed=20 + i, amp=5e-3)
+        xs.append(np.stack(xd, 1))
+        lam, mu = lame(mat["E"], mat["nu"])
+        go, djo = O.grad(m)
+        fo.append(O.internal_force(m, go, djo, lam, mu, *xd))
+        doff, dcol = O.dof_pattern(m)
+        vo = O.stiffness_values(m, go, djo, lam, mu, doff, dcol, *xd)
+        vo, _ = O.apply_dirichlet(doff, dcol, vo, m.f_ext, m.fixed)
+        yo.append(O.spmv(doff, dcol, vo, p[b.inst_slice(i)]))
+    uall = (np.concatenate(xs) - np.concatenate([m.xyz() for m, _ in items])).reshape(-1)
+    b.set_displacement(torch.as_tensor(uall).cuda())
+    f = _np(b.internal_force())
+    y = _np(b.tangent_apply(torch.as_tensor(p).cuda()))
+    for i in range(len(items)):
+        sl = b.inst_slice(i)
+        assert rel_err(f[sl], fo[i]) < 1e-12, i
+        assert rel_err(y[sl], yo[i]) < 1e-12, i
+    b.set_displacement(None)
+    assert np.abs(_np(b.internal_force())).max() < 1e-6 * np.abs(f).max()

@@ -126,6 +126,21 @@ class Batch:
              self.stream)
         return out
 
+    def set_displacement(self, u=None):
+        """u: displacement from the rest positions, AoS [3*n_nodes] on the device -- the
+        operator becomes K(X + u) and internal_force() f_int(X + u); None: back to rest."""
+        if u is None:
+            call("tlfea_batch_set_displacement", self.h, None, None, None, self.stream)
+            return
+        us = [u[d::3].contiguous() for d in range(3)]
+        call("tlfea_batch_set_displacement", self.h, ptr(us[0]), ptr(us[1]), ptr(us[2]), self.stream)
+
+    def internal_force(self, out=None):
+        """f_int of every instance at the current positions (all DOFs)."""
+        out = torch.empty(3 * self.n_nodes, dtype=torch.float64, device=DEV) if out is None else out
+        call("tlfea_batch_internal_force", self.h, ptr(out), self.stream)
+        return out
+
     def precond_setup(self, degree=6, ratio=30.0, power_iters=20, safety=1.1):
         """FP32 Jacobi-Chebyshev preconditioner; returns the lambda_max(D^-1 K) estimates."""
         lm = np.zeros(self.n_inst)

@@ -53,6 +53,10 @@ struct tlfea_batch {
     double* d_mc = nullptr;     // per-instance mass coefficient (rho * c) of the operator K + c M
     bool mass_on = false;
     int mass_exact = 0;         // 0: quadrature-rule mass, 1: exactly integrated mass
+    // current displacement from the rest positions (tlfea_batch_set_displacement); the
+    // operator is K(X + u) there, K at rest while moved is false
+    double *d_ux = nullptr, *d_uy = nullptr, *d_uz = nullptr;
+    bool moved = false;
     // active-work lists (used once finished instances are compacted out of the launches)
     std::vector<int> h_inst_chunk_off;
     int* d_blk_list = nullptr;
@@ -560,7 +564,8 @@ void free_batch(tlfea_batch* b) {
                     b->d_jinv32, b->d_detJ32, b->d_lam, b->d_mu, b->d_lam32, b->d_mu32, b->d_chunk_inst,
                     b->d_chunk_beg, b->d_chunk_end, b->d_inst_chunk_off, b->d_r, b->d_p, b->d_q, b->d_z,
                     b->d_r32, b->d_z32, b->d_d32, b->d_q32, b->d_part, b->d_s, b->d_i, b->d_all, b->d_cheb,
-                    b->d_rold, b->d_dinv32, b->d_blk_list, b->d_chunk_list, b->d_inst_list, b->d_mc};
+                    b->d_rold, b->d_dinv32, b->d_blk_list, b->d_chunk_list, b->d_inst_list, b->d_mc,
+                    b->d_ux, b->d_uy, b->d_uz};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (b->s_own) (void)hipStreamDestroy(b->s_own);
@@ -659,16 +664,26 @@ int build_chunks(tlfea_batch* b, int chunk_nodes) {
 
 // FP64 batched operator q = (k_scale K + c M) p (masked; the mass term when set up),
 // chunk partials of p.q into d_part.
+// the kernels' (x, X0) pair: (u, null) -- the displacement itself -- once moved, else the
+// rest positions twice
+const double* pos_x(const tlfea_batch* b) { return b->moved ? b->d_ux : b->d_X; }
+const double* pos_y(const tlfea_batch* b) { return b->moved ? b->d_uy : b->d_Y; }
+const double* pos_z(const tlfea_batch* b) { return b->moved ? b->d_uz : b->d_Z; }
+const double* rest_x(const tlfea_batch* b) { return b->moved ? nullptr : b->d_X; }
+const double* rest_y(const tlfea_batch* b) { return b->moved ? nullptr : b->d_Y; }
+const double* rest_z(const tlfea_batch* b) { return b->moved ? nullptr : b->d_Z; }
+
 int apply64(tlfea_batch* b, const double* p, double* q, const int* iflags, hipStream_t s, double k_scale = 1.0,
             bool mass = true, bool masked = true) {
     TanMat<double> tm{0.0, 0.0, b->d_elem_inst, b->d_lam, b->d_mu, iflags, NI};
     tm.inst_mc = (mass && b->mass_on) ? b->d_mc : nullptr;
     tm.mass_exact = b->mass_exact;
     tm.k_scale = k_scale;
-    tm.at_rest = 1;  // the batch's operator is the tangent at the rest positions
+    tm.at_rest = b->moved ? 0 : 1;  // the tangent at the rest positions unless moved
     tm.affine = b->affine;
     const unsigned char* mask = masked ? b->d_mask : nullptr;
-    if (int rc = launch_tangent_partials<double>(b->plan, b->d_X, b->d_Y, b->d_Z, b->d_X, b->d_Y, b->d_Z, b->d_jinv,
+    if (int rc = launch_tangent_partials<double>(b->plan, pos_x(b), pos_y(b), pos_z(b), rest_x(b), rest_y(b),
+                                                 rest_z(b), b->d_jinv,
                                                  b->d_detJ, b->quad, tm, mask, p, nullptr, s,
                                                  b->compact ? b->d_blk_list : nullptr, b->n_blk_list))
         return rc;
@@ -681,9 +696,10 @@ int apply64(tlfea_batch* b, const double* p, double* q, const int* iflags, hipSt
 // FP32 batched operator q = K p (masked), no dot partials.
 int apply32(tlfea_batch* b, const float* p, float* q, const int* iflags, hipStream_t s) {
     TanMat<float> tm{0.0f, 0.0f, b->d_elem_inst, b->d_lam32, b->d_mu32, iflags, NI};
-    tm.at_rest = 1;
+    tm.at_rest = b->moved ? 0 : 1;
     tm.affine = b->affine;
-    if (int rc = launch_tangent_partials<float>(b->plan, b->d_X, b->d_Y, b->d_Z, b->d_X, b->d_Y, b->d_Z, b->d_jinv32,
+    if (int rc = launch_tangent_partials<float>(b->plan, pos_x(b), pos_y(b), pos_z(b), rest_x(b), rest_y(b),
+                                                rest_z(b), b->d_jinv32,
                                                 b->d_detJ32, b->quad, tm, b->d_mask, p, nullptr, s,
                                                 b->compact ? b->d_blk_list : nullptr, b->n_blk_list))
         return rc;
@@ -927,7 +943,8 @@ extern "C" int tlfea_batch_precond_setup(tlfea_batch* b, int degree, double rati
     const int64_t nd = 3 * (int64_t)b->n_nodes;
     // diagonal of K at the rest configuration: partials, then the gather with p = 1
     const TanMat<double> tm{0.0, 0.0, b->d_elem_inst, b->d_lam, b->d_mu, nullptr, NI};
-    if (int rc = launch_tangent_diag_partials(b->plan, b->d_X, b->d_Y, b->d_Z, b->d_X, b->d_Y, b->d_Z, b->d_jinv,
+    if (int rc = launch_tangent_diag_partials(b->plan, pos_x(b), pos_y(b), pos_z(b), rest_x(b), rest_y(b), rest_z(b),
+                                              b->d_jinv,
                                               b->d_detJ, b->quad, tm, s))
         return rc;
     k_fill<<<grid_for(nd, 256), 256, 0, s>>>(nd, 1.0, b->d_z);
@@ -1002,6 +1019,36 @@ extern "C" int tlfea_batch_set_mass(tlfea_batch* b, const double* rho_coef, int 
     b->mass_on = true;
     b->mass_exact = rule;
     return TLFEA_OK;
+}
+
+extern "C" int tlfea_batch_set_displacement(tlfea_batch* b, const double* ux, const double* uy, const double* uz,
+                                             void* stream) {
+    TLFEA_REQUIRE(b != nullptr, "null batch");
+    drop_graph(b);
+    if (!ux && !uy && !uz) {
+        b->moved = false;
+        return TLFEA_OK;
+    }
+    TLFEA_REQUIRE(ux && uy && uz, "the displacement needs all three arrays");
+    if (!b->d_ux && (dalloc(&b->d_ux, b->n_nodes) || dalloc(&b->d_uy, b->n_nodes) || dalloc(&b->d_uz, b->n_nodes)))
+        return TLFEA_EHIP;
+    hipStream_t s = as_stream(stream);
+    TLFEA_HIP_TRY(hipMemcpyAsync(b->d_ux, ux, sizeof(double) * b->n_nodes, hipMemcpyDeviceToDevice, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(b->d_uy, uy, sizeof(double) * b->n_nodes, hipMemcpyDeviceToDevice, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(b->d_uz, uz, sizeof(double) * b->n_nodes, hipMemcpyDeviceToDevice, s));
+    b->moved = true;
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_batch_internal_force(tlfea_batch* b, double* f, void* stream) {
+    TLFEA_REQUIRE(b && f, "null argument");
+    if ((size_t)b->plan->max_u * 9 * sizeof(double) > 160 * 1024) {
+        set_error("a block touches %d nodes: beyond the LDS budget of the rest-coordinate force", b->plan->max_u);
+        return TLFEA_ECAPACITY;
+    }
+    const tlfea_material none{0.0, 0.0, 0.0, 0.0};
+    return launch_force_rest(b->plan, pos_x(b), pos_y(b), pos_z(b), b->d_X, b->d_Y, b->d_Z, b->quad, none, nullptr, f,
+                             0, nullptr, as_stream(stream), b->d_elem_inst, b->d_lam, b->d_mu, b->moved);
 }
 
 extern "C" int tlfea_batch_apply(tlfea_batch* b, double k_scale, int with_mass, int masked, const double* p,

@@ -131,7 +131,8 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_force_rest(
     const unsigned short* __restrict__ slot_t, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, const double* __restrict__ X0, const double* __restrict__ Y0,
     const double* __restrict__ Z0, const double* __restrict__ v, tlfea_quad quad, tlfea_material mat,
-    double* __restrict__ part, int max_u, const int* __restrict__ flags) {
+    const int* __restrict__ elem_inst, const double* __restrict__ inst_lam, const double* __restrict__ inst_mu,
+    double* __restrict__ part, int max_u, const int* __restrict__ flags, int x_is_disp) {
     if (flags && (flags[0] | flags[1])) return;
     extern __shared__ double lds[];
     double* sR = lds;              // [3][max_u] rest positions
@@ -154,9 +155,10 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_force_rest(
         sR[u] = X;
         sR[max_u + u] = Y;
         sR[2 * max_u + u] = Z;
-        sU[u] = x[n] - X;
-        sU[max_u + u] = y[n] - Y;
-        sU[2 * max_u + u] = z[n] - Z;
+        // x, y, z are positions, or with x_is_disp the displacement itself
+        sU[u] = x_is_disp ? x[n] : x[n] - X;
+        sU[max_u + u] = x_is_disp ? y[n] : y[n] - Y;
+        sU[2 * max_u + u] = x_is_disp ? z[n] : z[n] - Z;
         sF[u] = sF[max_u + u] = sF[2 * max_u + u] = 0.0;
         if constexpr (DAMP) {
             sV[u] = v[3 * n];
@@ -166,6 +168,9 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_force_rest(
     }
     __syncthreads();
     if (e < n_elem) {
+        // one material, or per instance of a batch
+        const double lam = elem_inst ? inst_lam[elem_inst[e]] : mat.lambda;
+        const double mu = elem_inst ? inst_mu[elem_inst[e]] : mat.mu;
         double fa[NN][3];
 #pragma unroll
         for (int a = 0; a < NN; a++) fa[a][0] = fa[a][1] = fa[a][2] = 0.0;
@@ -226,11 +231,11 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_force_rest(
 #pragma unroll
                 for (int j = 0; j < 3; j++)
                     S[i][j] = 0.5 * (H[i][j] + H[j][i] + H[0][i] * H[0][j] + H[1][i] * H[1][j] + H[2][i] * H[2][j]);
-            const double ltrE = mat.lambda * (S[0][0] + S[1][1] + S[2][2]);
+            const double ltrE = lam * (S[0][0] + S[1][1] + S[2][2]);
 #pragma unroll
             for (int i = 0; i < 3; i++)
 #pragma unroll
-                for (int j = 0; j < 3; j++) S[i][j] = 2.0 * mat.mu * S[i][j] + (i == j ? ltrE : 0.0);
+                for (int j = 0; j < 3; j++) S[i][j] = 2.0 * mu * S[i][j] + (i == j ? ltrE : 0.0);
             double P[3][3];
 #pragma unroll
             for (int i = 0; i < 3; i++)
@@ -357,15 +362,17 @@ bool is_standard_rule(int nn, const tlfea_quad& quad) {
 int launch_force_rest(const tlfea_force_plan* p, const double* x, const double* y, const double* z,
                       const double* X0, const double* Y0, const double* Z0, const tlfea_quad& quad,
                       const tlfea_material& mat, const double* v, double* f, int accumulate, const int* flags,
-                      hipStream_t s) {
+                      hipStream_t s, const int* elem_inst, const double* inst_lam, const double* inst_mu,
+                      bool x_is_disp) {
     const bool damp = (v != nullptr) && (mat.eta_damp != 0.0 || mat.lambda_damp != 0.0);
     const size_t lds = sizeof(double) * (size_t)p->max_u * (damp ? 12 : 9);
     const unsigned nb = (unsigned)p->n_blocks;
     if (nb > 0) {
 #define TLFEA_FORCE_REST(NN_, NQ_, D_)                                                                            \
     k_force_rest<NN_, NQ_, D_><<<nb, PLAN_BLOCK, lds, s>>>(p->n_elem, p->d_blk_off, p->d_unodes, p->d_slot, x, y, z, \
-                                                           X0, Y0, Z0, D_ ? v : nullptr, quad, mat, p->d_part,       \
-                                                           p->max_u, flags)
+                                                           X0, Y0, Z0, D_ ? v : nullptr, quad, mat, elem_inst,       \
+                                                           inst_lam, inst_mu, p->d_part, p->max_u, flags,            \
+                                                           x_is_disp ? 1 : 0)
         if (p->nn == 10) {
             if (damp)
                 TLFEA_FORCE_REST(10, 5, true);

@@ -47,10 +47,12 @@ int launch_force_planned(const tlfea_force_plan* p, const double* x, const doubl
 // The same force with the gradients rebuilt from the rest positions (standard rules only,
 // see is_standard_rule; LDS of 9 or, with damping, 12 arrays of max_u doubles).
 bool is_standard_rule(int nn, const tlfea_quad& quad);
+// elem_inst / inst_lam / inst_mu (optional): per-instance (lambda, mu) of a batch.
 int launch_force_rest(const tlfea_force_plan* p, const double* x, const double* y, const double* z,
                       const double* X0, const double* Y0, const double* Z0, const tlfea_quad& quad,
                       const tlfea_material& mat, const double* v, double* f, int accumulate, const int* flags,
-                      hipStream_t s);
+                      hipStream_t s, const int* elem_inst = nullptr, const double* inst_lam = nullptr,
+                      const double* inst_mu = nullptr, bool x_is_disp = false);
 
 // dN/dxi of every node at every quadrature point (a kernel argument).
 template <typename T>
@@ -85,7 +87,8 @@ struct TanMat {
 };
 
 // Block partials of the matrix-free tangent (tangent.hip) into plan->d_part (read as T),
-// for all plan blocks or, with blk_list, for the n_list blocks it names.
+// for all plan blocks or, with blk_list, for the n_list blocks it names.  X0 = Y0 = Z0 =
+// null: x, y, z hold the displacement itself.
 template <typename T>
 int launch_tangent_partials(const tlfea_force_plan* pl, const double* x, const double* y, const double* z,
                             const double* X0, const double* Y0, const double* Z0, const T* jinv_t, const T* detJ_t,

@@ -210,9 +210,11 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent(
     for (int u = threadIdx.x; u < U; u += PLAN_BLOCK) {
         const int n = unodes[u0 + u];
         if constexpr (!LIN) {
-            sU[u] = (T)(x[n] - X0[n]);
-            sU[max_u + u] = (T)(y[n] - Y0[n]);
-            sU[2 * max_u + u] = (T)(z[n] - Z0[n]);
+            // without rest arrays x, y, z are the displacement itself (no cancellation of
+            // x - X0 when u is many orders below the coordinates)
+            sU[u] = X0 ? (T)(x[n] - X0[n]) : (T)x[n];
+            sU[max_u + u] = X0 ? (T)(y[n] - Y0[n]) : (T)y[n];
+            sU[2 * max_u + u] = X0 ? (T)(z[n] - Z0[n]) : (T)z[n];
         }
         sF[u] = sF[max_u + u] = sF[2 * max_u + u] = T(0);
 #pragma unroll
@@ -475,9 +477,9 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent_diag(
     const int U = (int)(blk_off[b + 1] - u0);
     for (int u = threadIdx.x; u < U; u += PLAN_BLOCK) {
         const int n = unodes[u0 + u];
-        sU[u] = x[n] - X0[n];
-        sU[max_u + u] = y[n] - Y0[n];
-        sU[2 * max_u + u] = z[n] - Z0[n];
+        sU[u] = X0 ? x[n] - X0[n] : x[n];
+        sU[max_u + u] = X0 ? y[n] - Y0[n] : y[n];
+        sU[2 * max_u + u] = X0 ? z[n] - Z0[n] : z[n];
         sF[u] = sF[max_u + u] = sF[2 * max_u + u] = 0.0;
     }
     __syncthreads();

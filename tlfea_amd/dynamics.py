@@ -14,6 +14,20 @@ eliminated (v = 0 there) instead of the reference's augmented-Lagrangian penalty
 gravity as the consistent nodal load M g, plus A sin(2 pi f t) along the instance's
 tip-load pattern.
 
+With nonlinear=True each step instead solves the implicit-Euler equation itself,
+
+    g(v) = M (v - v_n) / dt + f_int(X + u_n + dt v) - f(t_{n+1}) = 0,
+
+by inexact Newton iterations from v_n, (M/dt + dt K(x)) dv = -g with K(x) the tangent at the
+current positions (tlfea_batch_set_displacement) and f_int from the batched force kernel,
+until ||g||/||g(v_n)|| <= tol_nl for every instance (the dynamic-step criterion of the
+SyncedNesterov loop, which minimises the same g; solved here with the batched CG, each
+solve only to the reduction the worst unconverged instance still needs:
+0.5 tol_nl ||g(v_n)|| / ||g||, at least newton_cg_tol, at most 0.1).  An
+instance whose ratio has stopped halving below 1e-6 sits at the FP64 floor of evaluating
+g (rounding of f_int against the load increment); the step then ends and the instance is
+counted in ``floor`` instead of failing.
+
 M is the exactly integrated consistent mass by default (mass_rule="exact").  The
 reference's 5-point mass (mass_matrix_qp_kernel, FEAT10Data.cu:200-271) is indefinite,
 so K + M/dt^2 is indefinite once M/dt^2 outweighs K -- on soft beams (E of a few MPa)
@@ -42,10 +56,13 @@ def dynamic_beams(n, seed):
 
 
 class BatchDynamics:
-    def __init__(self, items, amp, freq, dt=1e-4, g=(0.0, 0.0, -9.81), tol=1e-8, batch=None, mass_rule="exact"):
+    def __init__(self, items, amp, freq, dt=1e-4, g=(0.0, 0.0, -9.81), tol=1e-8, batch=None, mass_rule="exact",
+                 nonlinear=False, tol_nl=1e-8, newton_cg_tol=1e-10, max_newton=8):
         self.b = batch if batch is not None else Batch(items)
         b = self.b
         self.dt, self.tol = dt, tol
+        self.nonlinear, self.tol_nl, self.newton_cg_tol, self.max_newton = nonlinear, tol_nl, newton_cg_tol, max_newton
+        b.set_displacement(None)
         self.c = 1.0 / dt ** 2
         rho = np.array([mat.get("rho", 1000.0) for _, mat in items])
         b.set_mass(rho * self.c, rule=mass_rule)
@@ -53,6 +70,7 @@ class BatchDynamics:
         self.fixed = b.mask.bool()
         counts = 3 * np.diff(b.node_off)
         inst = torch.as_tensor(np.repeat(np.arange(b.n_inst), counts)).to(DEV)
+        self.inst = inst
         # tip-load pattern of unit total load per instance
         F = torch.zeros(b.n_inst, 3, dtype=torch.float64, device=DEV)
         F.index_add_(0, inst[0::3], b.f.view(-1, 3))
@@ -68,11 +86,28 @@ class BatchDynamics:
         self.v = torch.zeros(n3, dtype=torch.float64, device=DEV)
         self.t = 0.0
         self.iters = []
+        self.newton = []
+        self.floor = []
 
     def load(self, t):
         return self.fg + self.tip * (self.amp * torch.sin(self.omega * t))
 
+    def _norms(self, g):
+        sq = torch.zeros(self.b.n_inst, dtype=torch.float64, device=DEV)
+        sq.index_add_(0, self.inst, g * g)
+        return torch.sqrt(sq)
+
+    def _residual(self, v, f1):
+        """g(v) on the free DOFs, with the batch moved to x = X + u_n + dt v."""
+        b, dt = self.b, self.dt
+        b.set_displacement(self.u + dt * v)
+        g = dt * b.apply(v - self.v, k_scale=0.0, mass=True) + b.internal_force() - f1
+        g[self.fixed] = 0.0
+        return g
+
     def step(self):
+        if self.nonlinear:
+            return self._step_newton()
         b, dt = self.b, self.dt
         t1 = self.t + dt
         rhs = b.apply(self.v, k_scale=0.0, mass=True) - b.apply(self.u, k_scale=1.0) / dt + self.load(t1) / dt
@@ -85,4 +120,53 @@ class BatchDynamics:
         self.v = v1
         self.t = t1
         self.iters.append(b.iters.copy())
+        return self.u, self.v
+
+    def _step_newton(self):
+        b, dt = self.b, self.dt
+        t1 = self.t + dt
+        f1 = self.load(t1)
+        v = self.v.clone()
+        its = np.zeros(b.n_inst, np.int64)
+        g = self._residual(v, f1)
+        g0 = self._norms(g)
+        k = 0
+        prev = None
+        at_floor = 0
+        while True:
+            gn = self._norms(g)
+            ok = gn <= self.tol_nl * g0
+            if bool(ok.all()):
+                break
+            ratio = gn / g0.clamp_min(1e-300)
+            if prev is not None:
+                stalled = (~ok) & (ratio > 0.5 * prev) & (ratio < 1e-6)
+                if bool((ok | stalled).all()):
+                    at_floor = int(stalled.sum())
+                    break
+            prev = ratio
+            if k == self.max_newton:
+                raise RuntimeError(f"time step {len(self.iters) + 1}: Newton did not reach {self.tol_nl:g} in "
+                                   f"{k} iterations (worst ratio {float((gn / g0.clamp_min(1e-300)).max()):.2e})")
+            # (M/dt + dt K(x)) dv = -g  <=>  (K(x) + M/dt^2) dv = -g/dt; instances already
+            # converged get a zero load, which the batched CG finishes at once (and compacts out)
+            rhs = -g / dt
+            rhs[ok[self.inst]] = 0.0
+            worst = float(ratio[~ok].max())
+            cg_tol = min(0.1, max(self.newton_cg_tol, 0.5 * self.tol_nl / max(worst, 1e-300)))
+            dv = b.solve(tol=cg_tol, f=rhs)
+            if b.rc != 0:
+                raise RuntimeError(f"time step {len(self.iters) + 1}, Newton iteration {k + 1}: "
+                                   f"{int((b.status != 0).sum())} instances did not converge")
+            its += b.iters
+            v = v + dv
+            k += 1
+            g = self._residual(v, f1)
+        b.set_displacement(None)
+        self.u = self.u + dt * v
+        self.v = v
+        self.t = t1
+        self.iters.append(its)
+        self.newton.append(k)
+        self.floor.append(at_floor)
         return self.u, self.v
