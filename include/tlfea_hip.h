@@ -42,9 +42,14 @@ typedef struct tlfea_quad {
     double x[5], y[5], z[5], w[5];
 } tlfea_quad;
 
-/* Material: SVK (lambda, mu) plus Kelvin-Voigt damping (eta, lambda_d). */
+/* Material: SVK (lambda, mu) or, with model = 1, compressible Mooney-Rivlin (mu10, mu01,
+ * kappa; src/materials/MooneyRivlin.cuh, selected as material_model == 1 in compute_p,
+ * FEAT10DataFunc.cuh:274-275), plus Kelvin-Voigt damping (eta, lambda_d).  Zero-filled
+ * trailing fields give SVK. */
 typedef struct tlfea_material {
     double lambda, mu, eta_damp, lambda_damp;
+    int model; /* 0 SVK, 1 Mooney-Rivlin */
+    double mu10, mu01, kappa;
 } tlfea_material;
 
 /* ---- general ---------------------------------------------------------------------- */

@@ -23,6 +23,11 @@ Parity status of each function (how it is pinned, see tests/test_oracle.py):
   definiteness, total mass = rho V, row sums equal to the 5-point rule's.
 * ``stiffness_values`` -- symmetry, rigid modes in the kernel, finite difference of f_int,
   linear-elastic closed form at F = I.
+* ``mr_point`` and the Mooney-Rivlin branch of ``calc_p`` / ``internal_force`` /
+  ``stiffness_values`` -- P(I) = 0, P = finite difference of W, A = finite difference of P
+  (and its major symmetry), frame indifference P(QF) = Q P(F), the small-strain limit
+  (linear elasticity with mu = 2 (mu10 + mu01), bulk modulus kappa), f_int = finite
+  difference of the total MR energy, K = finite difference of f_int.
 * ``cg`` / ``band_cholesky_solve`` -- each against dense numpy solves, Euler-Bernoulli
   band on the beam.
 * ``nesterov_step`` / ``adamw_step`` -- restatement only for the iterate sequence
@@ -108,6 +113,38 @@ def num_threads() -> int:
 
 
 # ---------------------------------------------------------------------------------------
+class material:
+    """Context manager selecting the material of calc_p / internal_force / energy /
+    stiffness_values (and the Nesterov/AdamW restatements) for its body:
+    ``with material("mr", mu10, mu01, kappa): ...``; SVK (the lambda, mu arguments)
+    outside.  Mirrors the reference's per-element ``d_material_model`` switch."""
+
+    def __init__(self, model="svk", mu10=0.0, mu01=0.0, kappa=0.0):
+        self.args = (1 if model == "mr" else 0, float(mu10), float(mu01), float(kappa))
+
+    def __enter__(self):
+        lib().orc_set_material(C.c_int(self.args[0]), *[C.c_double(a) for a in self.args[1:]])
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_material(C.c_int(0), C.c_double(0.0), C.c_double(0.0), C.c_double(0.0))
+        return False
+
+
+def mr_point(F, mu10, mu01, kappa):
+    """(W, P, A) of the Mooney-Rivlin material at one 3x3 F (row-major numpy in and out;
+    A[i, j, k, l] = dP_ij / dF_kl)."""
+    F = np.asarray(F, np.float64)
+    F9 = _f64(F.T.reshape(-1))  # column-major
+    W = C.c_double(0.0)
+    P9 = np.zeros(9)
+    A81 = np.zeros(81)
+    lib().orc_mr_point(_d(F9), C.c_double(mu10), C.c_double(mu01), C.c_double(kappa), C.byref(W), _d(P9), _d(A81))
+    P = P9.reshape(3, 3).T
+    A = A81.reshape(3, 3, 3, 3).transpose(3, 2, 1, 0)
+    return W.value, P.copy(), A.copy()
+
+
 def quad_t10():
     q = [np.zeros(5) for _ in range(4)]
     lib().orc_quad_t10(*[_d(a) for a in q])

@@ -129,6 +129,110 @@ void kv_Pvis(const double F[3][3], const double Fd[3][3], double eta, double ld,
         }
 }
 
+// Compressible Mooney-Rivlin (`src/materials/MooneyRivlin.cuh:45-227`):
+//   W = mu10 (J^-2/3 I1 - 3) + mu01 (J^-4/3 I2 - 3) + kappa/2 (J - 1)^2,
+//   I1 = tr C, I2 = (I1^2 - tr C^2)/2, C = F^T F, J = det F,
+//   P = dW/dF = t1 (F - I1/3 F^-T) + t2 (I1 F - F C - 2 I2/3 F^-T) + t3 F^-T,
+//   t1 = 2 mu10 J^-2/3, t2 = 2 mu01 J^-4/3, t3 = kappa (J - 1) J.
+// F^-T divides by det F clamped to +-1e-12 in magnitude, as the reference does
+// (MooneyRivlin.cuh:24-43).
+struct MrParts {
+    double C[3][3], FC[3][3], FFT[3][3], FiT[3][3], I1, I2, J, t1, t2, t3;
+};
+
+void mr_parts(const double F[3][3], double mu10, double mu01, double kappa, MrParts& m) {
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            m.C[i][j] = m.FFT[i][j] = 0.0;
+            for (int k = 0; k < 3; k++) {
+                m.C[i][j] += F[k][i] * F[k][j];
+                m.FFT[i][j] += F[i][k] * F[j][k];
+            }
+        }
+    m.I1 = m.C[0][0] + m.C[1][1] + m.C[2][2];
+    double trC2 = 0.0;
+    for (int i = 0; i < 3; i++)
+        for (int k = 0; k < 3; k++) trC2 += m.C[i][k] * m.C[k][i];
+    m.I2 = 0.5 * (m.I1 * m.I1 - trC2);
+    m.J = det3(F);
+    double dsafe = m.J;
+    if (std::fabs(dsafe) < 1e-12) dsafe = dsafe >= 0.0 ? 1e-12 : -1e-12;
+    // F^-T_ij = cof(F)_ij / det F, cof(F)_ij = (-1)^(i+j) minor_ij
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            m.FiT[i][j] = (F[i1][j1] * F[i2][j2] - F[i1][j2] * F[i2][j1]) / dsafe;
+        }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            m.FC[i][j] = 0.0;
+            for (int k = 0; k < 3; k++) m.FC[i][j] += F[i][k] * m.C[k][j];
+        }
+    const double c = std::cbrt(m.J);
+    m.t1 = 2.0 * mu10 / (c * c);
+    m.t2 = 2.0 * mu01 / (c * c * c * c);
+    m.t3 = kappa * (m.J - 1.0) * m.J;
+}
+
+void mr_P(const double F[3][3], double mu10, double mu01, double kappa, double P[3][3]) {
+    MrParts m;
+    mr_parts(F, mu10, mu01, kappa, m);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            P[i][j] = m.t1 * (F[i][j] - m.I1 / 3.0 * m.FiT[i][j]) +
+                      m.t2 * (m.I1 * F[i][j] - m.FC[i][j] - 2.0 * m.I2 / 3.0 * m.FiT[i][j]) + m.t3 * m.FiT[i][j];
+}
+
+// A_ijkl = dP_ij/dF_kl (`src/materials/MooneyRivlin.cuh:106-227`), from
+//   dJ/dF_kl = J F^-T_kl, d(F^-T)_ij/dF_kl = -F^-T_il F^-T_kj, dI1/dF_kl = 2 F_kl,
+//   dI2/dF_kl = 2 (I1 F - F C)_kl, d(F C)_ij/dF_kl = d_ik C_lj + F_il F_kj + d_jl (F F^T)_ik.
+void mr_A(const double F[3][3], double mu10, double mu01, double kappa, double A[3][3][3][3]) {
+    MrParts m;
+    mr_parts(F, mu10, mu01, kappa, m);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            const double T1 = F[i][j] - m.I1 / 3.0 * m.FiT[i][j];
+            const double T2 = m.I1 * F[i][j] - m.FC[i][j] - 2.0 * m.I2 / 3.0 * m.FiT[i][j];
+            for (int k = 0; k < 3; k++)
+                for (int l = 0; l < 3; l++) {
+                    const double dik = i == k, djl = j == l;
+                    const double dJ = m.J * m.FiT[k][l];  // dJ/dF_kl
+                    const double dFiT = -m.FiT[i][l] * m.FiT[k][j];
+                    const double dI1 = 2.0 * F[k][l];
+                    const double dI2 = 2.0 * (m.I1 * F[k][l] - m.FC[k][l]);
+                    const double dFC = dik * m.C[l][j] + F[i][l] * F[k][j] + djl * m.FFT[i][k];
+                    // t1 ~ J^-2/3, t2 ~ J^-4/3: dt = t * (-p/3) dJ / J
+                    const double dt1 = -2.0 / 3.0 * m.t1 * m.FiT[k][l];
+                    const double dt2 = -4.0 / 3.0 * m.t2 * m.FiT[k][l];
+                    const double dt3 = kappa * (2.0 * m.J - 1.0) * dJ;
+                    const double dT1 = dik * djl - dI1 / 3.0 * m.FiT[i][j] - m.I1 / 3.0 * dFiT;
+                    const double dT2 = dI1 * F[i][j] + m.I1 * dik * djl - dFC - 2.0 / 3.0 * dI2 * m.FiT[i][j] -
+                                       2.0 * m.I2 / 3.0 * dFiT;
+                    A[i][j][k][l] = dt1 * T1 + m.t1 * dT1 + dt2 * T2 + m.t2 * dT2 + dt3 * m.FiT[i][j] + m.t3 * dFiT;
+                }
+        }
+}
+
+double mr_W(const double F[3][3], double mu10, double mu01, double kappa) {
+    MrParts m;
+    mr_parts(F, mu10, mu01, kappa, m);
+    const double c = std::cbrt(m.J);
+    return mu10 * (m.I1 / (c * c) - 3.0) + mu01 * (m.I2 / (c * c * c * c) - 3.0) +
+           0.5 * kappa * (m.J - 1.0) * (m.J - 1.0);
+}
+
+// Material of every call below: 0 SVK (lambda, mu arguments), 1 Mooney-Rivlin with these
+// coefficients (`d_material_model`, FEAT10Data.cuh:533-587).  Set by orc_set_material.
+int g_model = 0;
+double g_mu10 = 0.0, g_mu01 = 0.0, g_kappa = 0.0;
+
+void material_P(const double F[3][3], double lambda, double mu, double P[3][3]) {
+    if (g_model == 1)
+        mr_P(F, g_mu10, g_mu01, g_kappa, P);
+    else
+        svk_P(F, lambda, mu, P);
+}
+
 struct Elem {
     int n_elem, nn, nq;
     const int* conn;
@@ -266,7 +370,7 @@ ORC_API void orc_calc_p(int nn, int nq, int n_elem, const int* conn, const doubl
                 }
                 kv_Pvis(F, Fd, eta, lambda_d, Pv);
             }
-            svk_P(F, lambda, mu, P);
+            material_P(F, lambda, mu, P);
             int64_t o = ((int64_t)e * nq + q) * 9;
             for (int i = 0; i < 3; i++)
                 for (int j = 0; j < 3; j++) {
@@ -316,7 +420,8 @@ ORC_API void orc_internal_force(int nn, int nq, int n_elem, int n_nodes, const i
     orc_fint_from_p(nn, nq, n_elem, conn, P.data(), grad, detJ, qw, f);
 }
 
-// Total SVK strain energy  sum_e,q W(F) detJ w  with W = lambda/2 (tr E)^2 + mu tr(E^2).
+// Total strain energy  sum_e,q W(F) detJ w  with W = lambda/2 (tr E)^2 + mu tr(E^2) (SVK)
+// or the Mooney-Rivlin W above.
 // (Not in the reference: its gradient w.r.t. x is f_int, used to pin f_int by finite
 // differences.)
 ORC_API double orc_energy(int nn, int nq, int n_elem, const int* conn, const double* x, const double* y,
@@ -345,7 +450,8 @@ ORC_API double orc_energy(int nn, int nq, int n_elem, const int* conn, const dou
             double trE2 = 0;
             for (int i = 0; i < 3; i++)
                 for (int j = 0; j < 3; j++) trE2 += Em[i][j] * Em[j][i];
-            tot += (0.5 * lambda * tr * tr + mu * trE2) * detJ[e * nq + q] * qw[q];
+            const double W = g_model == 1 ? mr_W(F, g_mu10, g_mu01, g_kappa) : 0.5 * lambda * tr * tr + mu * trE2;
+            tot += W * detJ[e * nq + q] * qw[q];
         }
     return tot;
 }
@@ -430,7 +536,8 @@ ORC_API void orc_mass_values(int nn, int nq, int n_elem, const int* conn, const 
 // ---------------------------------------------------------------------------------------
 // Tangent stiffness on the DOF-level pattern at the current positions, scaled by h, plus
 // the unscaled Kelvin-Voigt block when damping is on.
-// `src/elements/FEAT10DataFunc.cuh:512-768`, `src/materials/SVK.cuh:41-67`.
+// `src/elements/FEAT10DataFunc.cuh:512-768`, `src/materials/SVK.cuh:41-67`; the
+// Mooney-Rivlin block from mr_A.
 // ---------------------------------------------------------------------------------------
 ORC_API void orc_stiffness_values(int nn, int nq, int n_elem, const int* conn, const double* x, const double* y,
                                   const double* z, const double* grad, const double* detJ, const double* qw,
@@ -466,6 +573,10 @@ ORC_API void orc_stiffness_values(int nn, int nq, int n_elem, const int* conn, c
             }
             double dV = detJ[e * nq + q] * qw[q];
             const bool damp = (eta_d != 0.0 || lambda_d != 0.0);
+            // Mooney-Rivlin block: K_ij[d][c] = sum_JL A[d][J][c][L] g_i[J] g_j[L] dV
+            // (FEAT10DataFunc.cuh:607-640)
+            static thread_local double A[3][3][3][3];
+            if (g_model == 1) mr_A(F, g_mu10, g_mu01, g_kappa, A);
             for (int i = 0; i < nn; i++)
                 for (int j = 0; j < nn; j++) {
                     double hij = g[j][0] * g[i][0] + g[j][1] * g[i][1] + g[j][2] * g[i][2];
@@ -477,6 +588,12 @@ ORC_API void orc_stiffness_values(int nn, int nq, int n_elem, const int* conn, c
                             double K = (lambda * Fh[i][d] * Fh[j][c] + lambda * trE * hij * dl + mu * FjFi * dl +
                                         mu * Fh[j][d] * Fh[i][c] + mu * hij * FFT[d][c] - mu * hij * dl) *
                                        dV;
+                            if (g_model == 1) {
+                                K = 0.0;
+                                for (int J = 0; J < 3; J++)
+                                    for (int L = 0; L < 3; L++) K += A[d][J][c][L] * g[i][J] * g[j][L];
+                                K *= dV;
+                            }
                             double Cv = 0.0;
                             if (damp) {
                                 // C_ij block: eta Fh_j (x) Fh_i + eta FFT (g_i.g_j) + lambda_d Fh_i (x) Fh_j
@@ -891,4 +1008,35 @@ ORC_API void orc_set_num_threads(int n) {
 #else
     (void)n;
 #endif
+}
+
+// ---------------------------------------------------------------------------------------
+// Material selection and the Mooney-Rivlin point functions (F, P column-major 3x3,
+// A[i + 3 j + 9 k + 27 l] = dP_ij/dF_kl).
+// ---------------------------------------------------------------------------------------
+ORC_API void orc_set_material(int model, double mu10, double mu01, double kappa) {
+    g_model = model;
+    g_mu10 = mu10;
+    g_mu01 = mu01;
+    g_kappa = kappa;
+}
+
+ORC_API void orc_mr_point(const double* F9, double mu10, double mu01, double kappa, double* W, double* P9,
+                          double* A81) {
+    double F[3][3], P[3][3], A[3][3][3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) F[i][j] = F9[i + 3 * j];
+    if (W) *W = mr_W(F, mu10, mu01, kappa);
+    if (P9) {
+        mr_P(F, mu10, mu01, kappa, P);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) P9[i + 3 * j] = P[i][j];
+    }
+    if (A81) {
+        mr_A(F, mu10, mu01, kappa, A);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                for (int k = 0; k < 3; k++)
+                    for (int l = 0; l < 3; l++) A81[i + 3 * j + 9 * k + 27 * l] = A[i][j][k][l];
+    }
 }

@@ -566,3 +566,113 @@ def test_extended_precision_force_restatement(name):
     xs = (m.x + u[:, 0], m.y + u[:, 1], m.z + u[:, 2])
     e = rel_err(O.internal_force(m, g, dj, lam, mu, *xs), fint_extended(m, *xs, lam, mu).astype(np.float64))
     assert 1e-13 < e < 1e-8
+
+
+# ---------------------------------------------------------------------------------------
+# Mooney-Rivlin (no test or example of the reference uses it: pinned by the mathematics)
+MR = (0.3e6, 0.1e6, 5e6)  # mu10, mu01, kappa [Pa]
+
+
+def _rand_F(seed, amp=0.2):
+    F = np.eye(3) + amp * np.random.default_rng(seed).standard_normal((3, 3))
+    assert np.linalg.det(F) > 0.2
+    return F
+
+
+def test_mr_point_derivatives_and_invariants():
+    mu10, mu01, kappa = MR
+    W0, P0, A0 = O.mr_point(np.eye(3), *MR)
+    assert abs(W0) < 1e-9 * mu10 and np.abs(P0).max() < 1e-9 * mu10
+    # small-strain limit: isotropic linear elasticity with shear modulus 2 (mu10 + mu01)
+    # and bulk modulus kappa
+    mu = 2 * (mu10 + mu01)
+    eye = np.eye(3)
+    C = (mu * (np.einsum("ik,jl->ijkl", eye, eye) + np.einsum("il,jk->ijkl", eye, eye)) +
+         (kappa - 2 * mu / 3) * np.einsum("ij,kl->ijkl", eye, eye))
+    assert np.abs(A0 - C).max() < 1e-9 * np.abs(C).max()
+    for seed in (1, 2, 3):
+        F = _rand_F(seed)
+        W, P, A = O.mr_point(F, *MR)
+        h = 1e-6
+        for k in range(3):
+            for l in range(3):
+                Fp, Fm = F.copy(), F.copy()
+                Fp[k, l] += h
+                Fm[k, l] -= h
+                Wp, Pp, _ = O.mr_point(Fp, *MR)
+                Wm, Pm, _ = O.mr_point(Fm, *MR)
+                assert abs((Wp - Wm) / (2 * h) - P[k, l]) < 1e-7 * np.abs(P).max()
+                assert np.abs((Pp - Pm) / (2 * h) - A[:, :, k, l]).max() < 1e-7 * np.abs(A).max()
+        # hyperelastic: A is a Hessian (major symmetry)
+        assert np.abs(A - A.transpose(2, 3, 0, 1)).max() < 1e-12 * np.abs(A).max()
+        # frame indifference: W(QF) = W(F), P(QF) = Q P(F)
+        Q, _ = np.linalg.qr(np.random.default_rng(seed + 10).standard_normal((3, 3)))
+        Q *= np.sign(np.linalg.det(Q))
+        Wq, Pq, _ = O.mr_point(Q @ F, *MR)
+        assert abs(Wq - W) < 1e-9 * abs(W)
+        assert np.abs(Pq - Q @ P).max() < 1e-9 * np.abs(P).max()
+
+
+def test_mr_uniaxial_closed_form():
+    """F = diag(s, 1, 1): J = s, I1 = s^2 + 2, I2 = 2 s^2 + 1 and
+    P_11 = t1 (s - I1/(3 s)) + t2 (I1 s - s^3 - 2 I2/(3 s)) + kappa (s - 1)."""
+    mu10, mu01, kappa = MR
+    for s in (0.9, 1.05, 1.3):
+        _, P, _ = O.mr_point(np.diag([s, 1.0, 1.0]), *MR)
+        I1, I2 = s * s + 2, 2 * s * s + 1
+        t1, t2 = 2 * mu10 * s ** (-2 / 3), 2 * mu01 * s ** (-4 / 3)
+        P11 = t1 * (s - I1 / (3 * s)) + t2 * (I1 * s - s ** 3 - 2 * I2 / (3 * s)) + kappa * (s - 1)
+        P22 = t1 * (1 - I1 / 3) + t2 * (I1 - 1 - 2 * I2 / 3) + kappa * (s - 1) * s
+        assert abs(P[0, 0] - P11) < 1e-12 * abs(P11)
+        assert abs(P[1, 1] - P22) < 1e-10 * max(abs(P22), mu10)
+        assert np.abs(P - np.diag(np.diag(P))).max() < 1e-9 * mu10
+
+
+@pytest.mark.parametrize("name", ["cube", "kuhn_t4"])
+def test_mr_element_force_and_stiffness(name):
+    """With the material switched to Mooney-Rivlin: f_int(rest) = 0, f_int = gradient of the
+    total energy, K symmetric with the rigid modes in its kernel at rest and equal to the
+    derivative of f_int at a deformed state."""
+    m = load_mesh(name)
+    g, dj = O.grad(m)
+    X = m.xyz()
+    scale = np.abs(X).max()
+    with O.material("mr", *MR):
+        f0 = O.internal_force(m, g, dj, 0.0, 0.0)
+        # det F within [0.86, 1.14] on both meshes (the TET4 mesh's elements are small)
+        x, y, z = deformed(m, seed=21, amp=1e-2 if m.nn == 10 else 1e-3)
+        f = O.internal_force(m, g, dj, 0.0, 0.0, x, y, z)
+        assert np.abs(f0).max() < 1e-10 * np.abs(f).max()
+        h = 1e-7 * scale
+        for node in (0, 3, m.n_nodes // 2):
+            for d, arr in enumerate((x, y, z)):
+                args = [x, y, z]
+                a = arr.copy()
+                a[node] += h
+                args[d] = a
+                ep = O.energy(m, g, dj, 0.0, 0.0, *args)
+                a = arr.copy()
+                a[node] -= h
+                args[d] = a
+                em = O.energy(m, g, dj, 0.0, 0.0, *args)
+                assert abs((ep - em) / (2 * h) - f[3 * node + d]) < 1e-6 * np.abs(f).max()
+        K0, *_ = _K(m, 0.0, 0.0)
+        assert np.abs(K0 - K0.T).max() < 1e-12 * np.abs(K0).max()
+        modes = [np.tile(np.eye(3)[d], m.n_nodes) for d in range(3)]
+        for d in range(3):
+            w = np.zeros(3)
+            w[d] = 1.0
+            modes.append(np.cross(w, X).reshape(-1))
+        for r in modes:
+            assert np.abs(K0 @ r).max() < 1e-10 * np.abs(K0).max() * np.abs(r).max()
+        Kx, *_ = _K(m, 0.0, 0.0, x, y, z)
+        assert np.abs(Kx - Kx.T).max() < 1e-10 * np.abs(Kx).max()
+        u = np.random.default_rng(22).standard_normal(3 * m.n_nodes)
+        eps = 1e-7 * scale
+        U = u.reshape(-1, 3)
+        fp = O.internal_force(m, g, dj, 0.0, 0.0, x + eps * U[:, 0], y + eps * U[:, 1], z + eps * U[:, 2])
+        fm = O.internal_force(m, g, dj, 0.0, 0.0, x - eps * U[:, 0], y - eps * U[:, 1], z - eps * U[:, 2])
+        assert rel_err(Kx @ u, (fp - fm) / (2 * eps)) < 1e-6
+    # the switch is scoped: SVK again outside
+    lam, mu = lame(1e9, 0.3)
+    assert np.abs(O.internal_force(m, g, dj, lam, mu, x, y, z) - f).max() > 1e-3 * np.abs(f).max()

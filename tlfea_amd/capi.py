@@ -33,7 +33,8 @@ class Quad(C.Structure):
 
 
 class Material(C.Structure):
-    _fields_ = [("lam", C.c_double), ("mu", C.c_double), ("eta_damp", C.c_double), ("lambda_damp", C.c_double)]
+    _fields_ = [("lam", C.c_double), ("mu", C.c_double), ("eta_damp", C.c_double), ("lambda_damp", C.c_double),
+                ("model", C.c_int), ("mu10", C.c_double), ("mu01", C.c_double), ("kappa", C.c_double)]
 
 
 class Body(C.Structure):

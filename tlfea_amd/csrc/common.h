@@ -91,6 +91,107 @@ __device__ __forceinline__ void kv_stress(const T F[3][3], const T Fd[3][3], T e
         for (int j = 0; j < 3; j++) Pv[i][j] = F[i][0] * S[0][j] + F[i][1] * S[1][j] + F[i][2] * S[2][j];
 }
 
+// Compressible Mooney-Rivlin (src/materials/MooneyRivlin.cuh:45-227): the derivative of
+//     W = mu10 (J^-2/3 I1 - 3) + mu01 (J^-4/3 I2 - 3) + kappa/2 (J - 1)^2,
+//     P = t1 (F - I1/3 F^-T) + t2 (I1 F - F C - 2 I2/3 F^-T) + t3 F^-T,
+// t1 = 2 mu10 J^-2/3, t2 = 2 mu01 J^-4/3, t3 = kappa (J - 1) J, C = F^T F,
+// I2 = (I1^2 - tr C^2)/2.  F^-T uses det F clamped to +-1e-12 in magnitude, as the
+// reference does.
+struct MrKin {
+    double C[3][3], FC[3][3], FinvT[3][3], I1, I2, J, t1, t2, t3;
+};
+
+__device__ __forceinline__ void mr_kinematics(const double F[3][3], double mu10, double mu01, double kappa,
+                                              MrKin& k) {
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) k.C[i][j] = F[0][i] * F[0][j] + F[1][i] * F[1][j] + F[2][i] * F[2][j];
+    k.I1 = k.C[0][0] + k.C[1][1] + k.C[2][2];
+    double trC2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) trC2 += k.C[i][j] * k.C[j][i];
+    k.I2 = 0.5 * (k.I1 * k.I1 - trC2);
+    k.J = F[0][0] * (F[1][1] * F[2][2] - F[1][2] * F[2][1]) - F[0][1] * (F[1][0] * F[2][2] - F[1][2] * F[2][0]) +
+          F[0][2] * (F[1][0] * F[2][1] - F[1][1] * F[2][0]);
+    double d = k.J;
+    if (fabs(d) < 1e-12) d = d >= 0.0 ? 1e-12 : -1e-12;
+    const double id = 1.0 / d;
+    // F^-T = cofactor(F) / det F
+    k.FinvT[0][0] = (F[1][1] * F[2][2] - F[1][2] * F[2][1]) * id;
+    k.FinvT[0][1] = (F[1][2] * F[2][0] - F[1][0] * F[2][2]) * id;
+    k.FinvT[0][2] = (F[1][0] * F[2][1] - F[1][1] * F[2][0]) * id;
+    k.FinvT[1][0] = (F[0][2] * F[2][1] - F[0][1] * F[2][2]) * id;
+    k.FinvT[1][1] = (F[0][0] * F[2][2] - F[0][2] * F[2][0]) * id;
+    k.FinvT[1][2] = (F[0][1] * F[2][0] - F[0][0] * F[2][1]) * id;
+    k.FinvT[2][0] = (F[0][1] * F[1][2] - F[0][2] * F[1][1]) * id;
+    k.FinvT[2][1] = (F[0][2] * F[1][0] - F[0][0] * F[1][2]) * id;
+    k.FinvT[2][2] = (F[0][0] * F[1][1] - F[0][1] * F[1][0]) * id;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) k.FC[i][j] = F[i][0] * k.C[0][j] + F[i][1] * k.C[1][j] + F[i][2] * k.C[2][j];
+    const double j13 = cbrt(k.J);
+    const double jm23 = 1.0 / (j13 * j13);
+    k.t1 = 2.0 * mu10 * jm23;
+    k.t2 = 2.0 * mu01 * jm23 * jm23;
+    k.t3 = kappa * (k.J - 1.0) * k.J;
+}
+
+__device__ __forceinline__ void mr_stress(const double F[3][3], double mu10, double mu01, double kappa,
+                                          double P[3][3]) {
+    MrKin k;
+    mr_kinematics(F, mu10, mu01, kappa, k);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            P[i][j] = k.t1 * (F[i][j] - k.I1 / 3.0 * k.FinvT[i][j]) +
+                      k.t2 * (k.I1 * F[i][j] - k.FC[i][j] - 2.0 * k.I2 / 3.0 * k.FinvT[i][j]) + k.t3 * k.FinvT[i][j];
+}
+
+// A[i][j][k][l] = dP_ij / dF_kl of mr_stress (MooneyRivlin.cuh:106-227), written as the
+// chain rule of the three terms: d(t1) = -2/3 t1 F^-T_kl, d(t2) = -4/3 t2 F^-T_kl,
+// d(t3) = kappa (2J - 1) J F^-T_kl, d(F^-T_ij) = -F^-T_il F^-T_kj, d(I1) = 2 F_kl,
+// d(I2) = 2 (I1 F - F C)_kl, d(FC)_ij = delta_ik C_lj + F_il F_kj + delta_jl (F F^T)_ik.
+__device__ __forceinline__ double mr_tangent_entry(const double F[3][3], const double FFt[3][3], const MrKin& k,
+                                                   double kappa, int i, int j, int kk, int l) {
+    const double dik = i == kk ? 1.0 : 0.0, djl = j == l ? 1.0 : 0.0;
+    const double fk = k.FinvT[kk][l];
+    const double term1 = F[i][j] - k.I1 / 3.0 * k.FinvT[i][j];
+    const double term2 = k.I1 * F[i][j] - k.FC[i][j] - 2.0 * k.I2 / 3.0 * k.FinvT[i][j];
+    const double dinv = -k.FinvT[i][l] * k.FinvT[kk][j];
+    const double dT1 = dik * djl - 2.0 / 3.0 * F[kk][l] * k.FinvT[i][j] - k.I1 / 3.0 * dinv;
+    const double dT2 = 2.0 * F[kk][l] * F[i][j] + k.I1 * dik * djl -
+                       (dik * k.C[l][j] + F[i][l] * F[kk][j] + djl * FFt[i][kk]) -
+                       4.0 / 3.0 * (k.I1 * F[kk][l] - k.FC[kk][l]) * k.FinvT[i][j] - 2.0 * k.I2 / 3.0 * dinv;
+    return -2.0 / 3.0 * k.t1 * fk * term1 + k.t1 * dT1 - 4.0 / 3.0 * k.t2 * fk * term2 + k.t2 * dT2 +
+           kappa * (2.0 * k.J - 1.0) * k.J * fk * k.FinvT[i][j] + k.t3 * dinv;
+}
+
+// P of the element's material: SVK (model 0) or Mooney-Rivlin (model 1).  The hot force
+// kernels take the model as a template parameter (a runtime branch costs the SVK variant
+// its register allocation); k_calc_p decides at run time.
+template <bool MR>
+__device__ __forceinline__ void material_stress_t(const double F[3][3], const tlfea_material& m, double P[3][3]) {
+    if constexpr (MR)
+        mr_stress(F, m.mu10, m.mu01, m.kappa, P);
+    else
+        svk_stress(F, m.lambda, m.mu, P);
+}
+__device__ __forceinline__ void material_stress(const double F[3][3], const tlfea_material& m, double P[3][3]) {
+    if (m.model == 1)
+        mr_stress(F, m.mu10, m.mu01, m.kappa, P);
+    else
+        svk_stress(F, m.lambda, m.mu, P);
+}
+
+// Material models this library evaluates (tlfea_material.model).
+#define TLFEA_REQUIRE_MODEL(mat) \
+    TLFEA_REQUIRE((mat).model == 0 || (mat).model == 1, "unknown material model %d (0 SVK, 1 Mooney-Rivlin)", (mat).model)
+
 __device__ __forceinline__ double det3(const double J[3][3]) {
     return J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
            J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);

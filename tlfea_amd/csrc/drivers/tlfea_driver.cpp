@@ -8,6 +8,7 @@
 //                 [--E 2.1e11 --nu 0.3 --load -1000 --tol 1e-10 --max-iter 50000] --out FILE
 //   tlfea_driver nesterov|adamw|adamw_nocoop --mesh tetgen:PREFIX|kuhn:.. [--steps 3] [--E ..]
 //                 [--rho0 ..] [--fix z0|x0] [--node-load NODE,FX] --out FILE
+//   any mode: [--mr MU10,MU01,KAPPA] switches the element to Mooney-Rivlin (SetMooneyRivlin)
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -107,6 +108,14 @@ int run(const std::string& mode, std::map<std::string, std::string>& a, MatrixXR
     b->SetDensity(num(a, "--rho0", 7850.0));
     b->SetDamping(num(a, "--eta", 0.0), num(a, "--lambda-d", 0.0));
     b->SetSVK(num(a, "--E", 2.1e11), num(a, "--nu", 0.3));
+    if (a.count("--mr")) {  // --mr MU10,MU01,KAPPA: compressible Mooney-Rivlin instead of SVK
+        double mu10, mu01, kappa;
+        if (std::sscanf(a["--mr"].c_str(), "%lf,%lf,%lf", &mu10, &mu01, &kappa) != 3) {
+            std::fprintf(stderr, "--mr expects MU10,MU01,KAPPA\n");
+            return 2;
+        }
+        b->SetMooneyRivlin(mu10, mu01, kappa);
+    }
     b->CalcDnDuPre();
     auto t0 = std::chrono::steady_clock::now();
     std::string extra;

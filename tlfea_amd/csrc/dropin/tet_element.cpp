@@ -185,12 +185,16 @@ void TetElementData::SetDamping(Real eta_damp, Real lambda_damp) {
     s_->mat.lambda_damp = lambda_damp;
 }
 
+// SVK with the stored lambda, mu; the Mooney-Rivlin coefficients are cleared
+// (FEAT10Data.cuh:533-552)
 void TetElementData::SetSVK() {
     if (!is_setup) {
         report("element must be set up before setting material.");
         return;
     }
     s_->material_model = 0;
+    s_->mat.model = 0;
+    s_->mat.mu10 = s_->mat.mu01 = s_->mat.kappa = 0.0;
 }
 
 // mu = E / (2(1+nu)), lambda = E nu / ((1+nu)(1-2nu))  (FEAT10Data.cuh:532-553)
@@ -206,8 +210,18 @@ void TetElementData::SetSVK(Real E, Real nu) {
     SetSVK();
 }
 
-void TetElementData::SetMooneyRivlin(Real, Real, Real) {
-    report("the Mooney-Rivlin material is not part of this build (St. Venant-Kirchhoff only); material unchanged");
+// Compressible Mooney-Rivlin: every later P, internal force and tangent uses
+// W = mu10 (J^-2/3 I1 - 3) + mu01 (J^-4/3 I2 - 3) + kappa/2 (J - 1)^2 (FEAT10Data.cuh:571-587)
+void TetElementData::SetMooneyRivlin(Real mu10, Real mu01, Real kappa) {
+    if (!is_setup) {
+        report("element must be set up before setting material.");
+        return;
+    }
+    s_->material_model = 1;
+    s_->mat.model = 1;
+    s_->mat.mu10 = mu10;
+    s_->mat.mu01 = mu01;
+    s_->mat.kappa = kappa;
 }
 
 void TetElementData::SetExternalForce(const VectorXR& h_f_ext) {

@@ -244,7 +244,7 @@ __global__ __launch_bounds__(256) void k_calc_p(int n_elem, int n_tiles, const i
         }
     }
     double P[3][3], Pv[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    svk_stress(F, mat.lambda, mat.mu, P);
+    material_stress(F, mat, P);
     if (damp) kv_stress(F, Fd, mat.eta_damp, mat.lambda_damp, Pv);
 #pragma unroll
     for (int i = 0; i < 3; i++)
@@ -302,7 +302,7 @@ __global__ __launch_bounds__(256) void k_fint_from_p(int n_elem, const int* __re
 // ------------------------------------------------------------------------------------
 // Fused force: F -> P -> nodal forces per element in registers; one lane per element.
 // ------------------------------------------------------------------------------------
-template <int NN, int NQ, bool DAMP>
+template <int NN, int NQ, bool DAMP, bool MR>
 __global__ __launch_bounds__(256) void k_force(int n_elem, const int* __restrict__ conn, const double* __restrict__ x,
                                                const double* __restrict__ y, const double* __restrict__ z,
                                                const double* __restrict__ grad_t, const double* __restrict__ detJ_t,
@@ -348,7 +348,7 @@ __global__ __launch_bounds__(256) void k_force(int n_elem, const int* __restrict
 #pragma unroll
                 for (int j = 0; j < 3; j++) F[i][j] += X[a][i] * g[a][j];
         double P[3][3];
-        svk_stress(F, mat.lambda, mat.mu, P);
+        material_stress_t<MR>(F, mat, P);
         if constexpr (DAMP) {
             double Fd[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, Pv[3][3];
 #pragma unroll
@@ -497,19 +497,26 @@ int tlfea_hip::launch_force(int nn, int n_elem, const int* conn, const double* x
     if (n_elem == 0) return TLFEA_OK;
     const bool damp = (v != nullptr) && (mat.eta_damp != 0.0 || mat.lambda_damp != 0.0);
     const unsigned g = grid_for(n_elem, 256);
-    if (nn == 10) {
-        if (damp)
-            k_force<10, 5, true><<<g, 256, 0, s>>>(n_elem, conn, x, y, z, grad_t, detJ_t, quad, mat, v, f, flags);
-        else
-            k_force<10, 5, false><<<g, 256, 0, s>>>(n_elem, conn, x, y, z, grad_t, detJ_t, quad, mat, nullptr, f,
-                                                    flags);
-    } else {
-        if (damp)
-            k_force<4, 1, true><<<g, 256, 0, s>>>(n_elem, conn, x, y, z, grad_t, detJ_t, quad, mat, v, f, flags);
-        else
-            k_force<4, 1, false><<<g, 256, 0, s>>>(n_elem, conn, x, y, z, grad_t, detJ_t, quad, mat, nullptr, f,
-                                                   flags);
-    }
+    const bool mr = mat.model == 1;
+#define TLFEA_FORCE_ATOMIC(NN_, NQ_, D_, MR_) \
+    k_force<NN_, NQ_, D_, MR_><<<g, 256, 0, s>>>(n_elem, conn, x, y, z, grad_t, detJ_t, quad, mat, D_ ? v : nullptr, f, flags)
+#define TLFEA_FORCE_ATOMIC_BY(NN_, NQ_)                 \
+    do {                                                \
+        if (damp && mr)                                 \
+            TLFEA_FORCE_ATOMIC(NN_, NQ_, true, true);   \
+        else if (damp)                                  \
+            TLFEA_FORCE_ATOMIC(NN_, NQ_, true, false);  \
+        else if (mr)                                    \
+            TLFEA_FORCE_ATOMIC(NN_, NQ_, false, true);  \
+        else                                            \
+            TLFEA_FORCE_ATOMIC(NN_, NQ_, false, false); \
+    } while (0)
+    if (nn == 10)
+        TLFEA_FORCE_ATOMIC_BY(10, 5);
+    else
+        TLFEA_FORCE_ATOMIC_BY(4, 1);
+#undef TLFEA_FORCE_ATOMIC_BY
+#undef TLFEA_FORCE_ATOMIC
     TLFEA_LAUNCH_CHECK();
     return TLFEA_OK;
 }
@@ -646,6 +653,7 @@ extern "C" int tlfea_calc_p(int nn, int n_elem, const int* conn, const double* x
     if (int rc = check_elem(nn, n_elem, nullptr)) return rc;
     if (n_elem == 0) return TLFEA_OK;
     TLFEA_REQUIRE(conn && x && y && z && grad_t && mat, "null pointer argument");
+    TLFEA_REQUIRE_MODEL(*mat);
     const int n_tiles = (int)(padded(n_elem) / TILE);
     const int nq = nn == 10 ? 5 : 1;
     const int64_t threads = (int64_t)n_tiles * nq * TILE;
@@ -680,6 +688,7 @@ extern "C" int tlfea_internal_force(int nn, int n_elem, int n_nodes, const int* 
                                     int accumulate, void* stream) {
     if (int rc = check_elem(nn, n_elem, quad)) return rc;
     TLFEA_REQUIRE(n_nodes >= 0 && f && mat, "bad arguments");
+    TLFEA_REQUIRE_MODEL(*mat);
     hipStream_t s = as_stream(stream);
     if (!accumulate) TLFEA_HIP_TRY(hipMemsetAsync(f, 0, sizeof(double) * 3 * (size_t)n_nodes, s));
     if (n_elem == 0) return TLFEA_OK;
@@ -707,6 +716,7 @@ extern "C" int tlfea_internal_force_f32(int nn, int n_elem, int n_nodes, const i
                                         const tlfea_material* mat, float* f, int accumulate, void* stream) {
     if (int rc = check_elem(nn, n_elem, nullptr)) return rc;
     TLFEA_REQUIRE(n_nodes >= 0 && f && mat, "bad arguments");
+    TLFEA_REQUIRE(mat->model == 0, "the FP32 force covers St. Venant-Kirchhoff only");
     hipStream_t s = as_stream(stream);
     if (!accumulate) TLFEA_HIP_TRY(hipMemsetAsync(f, 0, sizeof(float) * 3 * (size_t)n_nodes, s));
     if (n_elem == 0) return TLFEA_OK;

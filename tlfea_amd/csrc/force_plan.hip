@@ -20,7 +20,7 @@
 namespace tlfea_hip {
 
 
-template <int NN, int NQ, bool DAMP>
+template <int NN, int NQ, bool DAMP, bool MR>
 __global__ __launch_bounds__(PLAN_BLOCK) void k_force_planned(
     int n_elem, const int64_t* __restrict__ blk_off, const int* __restrict__ unodes,
     const unsigned short* __restrict__ slot_t, const double* __restrict__ x, const double* __restrict__ y,
@@ -77,7 +77,7 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_force_planned(
                     for (int j = 0; j < 3; j++) F[i][j] += xa[i] * g[a][j];
             }
             double P[3][3];
-            svk_stress(F, mat.lambda, mat.mu, P);
+            material_stress_t<MR>(F, mat, P);
             if constexpr (DAMP) {
                 double Fd[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, Pv[3][3];
 #pragma unroll
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_force_planned(
 // of FEAT10Data.cu:92-198: per element only 10 u16 slots are read from HBM (the gradient
 // stream of k_force_planned is 1200 B per TET10 element), and E comes from the
 // displacement gradient, without the cancellation of F^T F - I at small strains.
-template <int NN, int NQ, bool DAMP>
+template <int NN, int NQ, bool DAMP, bool MR>
 __global__ __launch_bounds__(PLAN_BLOCK) void k_force_rest(
     int n_elem, const int64_t* __restrict__ blk_off, const int* __restrict__ unodes,
     const unsigned short* __restrict__ slot_t, const double* __restrict__ x, const double* __restrict__ y,
@@ -241,6 +241,14 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_force_rest(
             for (int i = 0; i < 3; i++)
 #pragma unroll
                 for (int j = 0; j < 3; j++) P[i][j] = S[i][j] + H[i][0] * S[0][j] + H[i][1] * S[1][j] + H[i][2] * S[2][j];
+            if constexpr (MR) {  // Mooney-Rivlin from F = I + H (a batch is SVK only)
+                double F[3][3];
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++) F[i][j] = H[i][j] + (i == j ? 1.0 : 0.0);
+                mr_stress(F, mat.mu10, mat.mu01, mat.kappa, P);
+            }
             if constexpr (DAMP) {
                 double F[3][3], Fd[3][3], Pv[3][3];
 #pragma unroll
@@ -321,25 +329,28 @@ int launch_force_planned(const tlfea_force_plan* p, const double* x, const doubl
     const size_t lds = sizeof(double) * (size_t)p->max_u * (damp ? 9 : 6);
     const unsigned nb = (unsigned)p->n_blocks;
     if (nb > 0) {
-        if (p->nn == 10) {
-            if (damp)
-                k_force_planned<10, 5, true><<<nb, PLAN_BLOCK, lds, s>>>(p->n_elem, p->d_blk_off, p->d_unodes,
-                                                                         p->d_slot, x, y, z, v, grad_t, detJ_t, quad,
-                                                                         mat, p->d_part, p->max_u, flags);
-            else
-                k_force_planned<10, 5, false><<<nb, PLAN_BLOCK, lds, s>>>(p->n_elem, p->d_blk_off, p->d_unodes,
-                                                                          p->d_slot, x, y, z, nullptr, grad_t, detJ_t,
-                                                                          quad, mat, p->d_part, p->max_u, flags);
-        } else {
-            if (damp)
-                k_force_planned<4, 1, true><<<nb, PLAN_BLOCK, lds, s>>>(p->n_elem, p->d_blk_off, p->d_unodes,
-                                                                        p->d_slot, x, y, z, v, grad_t, detJ_t, quad,
-                                                                        mat, p->d_part, p->max_u, flags);
-            else
-                k_force_planned<4, 1, false><<<nb, PLAN_BLOCK, lds, s>>>(p->n_elem, p->d_blk_off, p->d_unodes,
-                                                                         p->d_slot, x, y, z, nullptr, grad_t, detJ_t,
-                                                                         quad, mat, p->d_part, p->max_u, flags);
-        }
+        const bool mr = mat.model == 1;
+#define TLFEA_FORCE_PLANNED(NN_, NQ_, D_, MR_)                                                                   \
+    k_force_planned<NN_, NQ_, D_, MR_><<<nb, PLAN_BLOCK, lds, s>>>(p->n_elem, p->d_blk_off, p->d_unodes, p->d_slot, \
+                                                                   x, y, z, D_ ? v : nullptr, grad_t, detJ_t, quad, \
+                                                                   mat, p->d_part, p->max_u, flags)
+#define TLFEA_FORCE_PLANNED_BY(NN_, NQ_)                  \
+    do {                                                  \
+        if (damp && mr)                                   \
+            TLFEA_FORCE_PLANNED(NN_, NQ_, true, true);    \
+        else if (damp)                                    \
+            TLFEA_FORCE_PLANNED(NN_, NQ_, true, false);   \
+        else if (mr)                                      \
+            TLFEA_FORCE_PLANNED(NN_, NQ_, false, true);   \
+        else                                              \
+            TLFEA_FORCE_PLANNED(NN_, NQ_, false, false);  \
+    } while (0)
+        if (p->nn == 10)
+            TLFEA_FORCE_PLANNED_BY(10, 5);
+        else
+            TLFEA_FORCE_PLANNED_BY(4, 1);
+#undef TLFEA_FORCE_PLANNED_BY
+#undef TLFEA_FORCE_PLANNED
         TLFEA_LAUNCH_CHECK();
     }
     k_gather_partials<<<grid_for(p->n_nodes, 256), 256, 0, s>>>(p->n_nodes, p->d_g_off, p->d_g_idx, p->d_part, f,
@@ -368,22 +379,27 @@ int launch_force_rest(const tlfea_force_plan* p, const double* x, const double* 
     const size_t lds = sizeof(double) * (size_t)p->max_u * (damp ? 12 : 9);
     const unsigned nb = (unsigned)p->n_blocks;
     if (nb > 0) {
-#define TLFEA_FORCE_REST(NN_, NQ_, D_)                                                                            \
-    k_force_rest<NN_, NQ_, D_><<<nb, PLAN_BLOCK, lds, s>>>(p->n_elem, p->d_blk_off, p->d_unodes, p->d_slot, x, y, z, \
-                                                           X0, Y0, Z0, D_ ? v : nullptr, quad, mat, elem_inst,       \
-                                                           inst_lam, inst_mu, p->d_part, p->max_u, flags,            \
-                                                           x_is_disp ? 1 : 0)
-        if (p->nn == 10) {
-            if (damp)
-                TLFEA_FORCE_REST(10, 5, true);
-            else
-                TLFEA_FORCE_REST(10, 5, false);
-        } else {
-            if (damp)
-                TLFEA_FORCE_REST(4, 1, true);
-            else
-                TLFEA_FORCE_REST(4, 1, false);
-        }
+        const bool mr = mat.model == 1;
+#define TLFEA_FORCE_REST(NN_, NQ_, D_, MR_)                                                                      \
+    k_force_rest<NN_, NQ_, D_, MR_><<<nb, PLAN_BLOCK, lds, s>>>(                                                 \
+        p->n_elem, p->d_blk_off, p->d_unodes, p->d_slot, x, y, z, X0, Y0, Z0, D_ ? v : nullptr, quad, mat,       \
+        elem_inst, inst_lam, inst_mu, p->d_part, p->max_u, flags, x_is_disp ? 1 : 0)
+#define TLFEA_FORCE_REST_BY(NN_, NQ_)                 \
+    do {                                              \
+        if (damp && mr)                               \
+            TLFEA_FORCE_REST(NN_, NQ_, true, true);   \
+        else if (damp)                                \
+            TLFEA_FORCE_REST(NN_, NQ_, true, false);  \
+        else if (mr)                                  \
+            TLFEA_FORCE_REST(NN_, NQ_, false, true);  \
+        else                                          \
+            TLFEA_FORCE_REST(NN_, NQ_, false, false); \
+    } while (0)
+        if (p->nn == 10)
+            TLFEA_FORCE_REST_BY(10, 5);
+        else
+            TLFEA_FORCE_REST_BY(4, 1);
+#undef TLFEA_FORCE_REST_BY
 #undef TLFEA_FORCE_REST
         TLFEA_LAUNCH_CHECK();
     }
@@ -525,6 +541,7 @@ extern "C" int tlfea_internal_force_planned(const tlfea_force_plan* plan, const 
                                             const tlfea_quad* quad, const tlfea_material* mat, const double* v,
                                             double* f, int accumulate, void* stream) {
     TLFEA_REQUIRE(plan && x && y && z && grad_t && detJ_t && quad && mat && f, "null argument");
+    TLFEA_REQUIRE_MODEL(*mat);
     TLFEA_REQUIRE(quad->n == (plan->nn == 10 ? 5 : 1), "quadrature does not match the element");
     return launch_force_planned(plan, x, y, z, grad_t, detJ_t, *quad, *mat, v, f, accumulate, nullptr,
                                 as_stream(stream));
@@ -535,6 +552,7 @@ extern "C" int tlfea_internal_force_rest(const tlfea_force_plan* plan, const dou
                                          const tlfea_quad* quad, const tlfea_material* mat, const double* v,
                                          double* f, int accumulate, void* stream) {
     TLFEA_REQUIRE(plan && x && y && z && X0 && Y0 && Z0 && quad && mat && f, "null argument");
+    TLFEA_REQUIRE_MODEL(*mat);
     TLFEA_REQUIRE(is_standard_rule(plan->nn, *quad),
                   "the gradients are rebuilt for the standard rule only (TET10: 5 points, TET4: 1 point)");
     const bool damp = (v != nullptr) && (mat->eta_damp != 0.0 || mat->lambda_damp != 0.0);

@@ -435,6 +435,7 @@ extern "C" int tlfea_solver_create(int kind, const tlfea_body* body, tlfea_solve
     *out = nullptr;
     TLFEA_REQUIRE(kind >= 0 && kind <= 2, "unknown solver kind %d", kind);
     TLFEA_REQUIRE(body->nn == 10 || body->nn == 4, "unsupported element");
+    TLFEA_REQUIRE_MODEL(body->mat);
     TLFEA_REQUIRE(body->pattern && body->m_sell && body->f_ext && body->f_int && body->x && body->y && body->z,
                   "body is missing device arrays");
     TLFEA_REQUIRE(body->pattern->n_nodes == body->n_nodes, "pattern does not match the body");
@@ -509,6 +510,7 @@ extern "C" int tlfea_solver_update_body(tlfea_solver* s, const tlfea_body* body)
                       body->n_fixed == s->b.n_fixed,
                   "body sizes differ from the solver's");
     TLFEA_REQUIRE(body->pattern && body->m_sell && body->f_ext && body->f_int, "body is missing device arrays");
+    TLFEA_REQUIRE_MODEL(body->mat);
     if (int rc = check_rest_form(body)) return rc;
     s->b = *body;
     return TLFEA_OK;

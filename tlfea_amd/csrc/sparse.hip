@@ -215,7 +215,7 @@ __global__ void k_mass(int n_elem, const int* __restrict__ slot, const double* _
 //   C_ab = (eta Fh_b (x) Fh_a + eta (g_a.g_b) F F^T + lambda_d Fh_a (x) Fh_b) dV
 // (FEAT10DataFunc.cuh:686-767), scattered with atomics through the slot map.
 // ------------------------------------------------------------------------------------
-template <int NN, int NQ>
+template <int NN, int NQ, bool MR>
 __global__ __launch_bounds__(64) void k_stiffness(int n_elem, const int* __restrict__ conn,
                                                   const double* __restrict__ x, const double* __restrict__ y,
                                                   const double* __restrict__ z, const double* __restrict__ grad_t,
@@ -226,6 +226,9 @@ __global__ __launch_bounds__(64) void k_stiffness(int n_elem, const int* __restr
     __shared__ double sFh[NN][3][TILE];
     __shared__ double sFFt[6][TILE];
     __shared__ double strE[TILE], sdV[TILE];
+    // Mooney-Rivlin: dP/dF per element of the tile (MooneyRivlin.cuh:106-227), the block of
+    // nodes a, b being K_ab[d][c] = sum_JL A[d][J][c][L] g_a[J] g_b[L] dV
+    __shared__ double sA[MR ? 81 : 1][TILE];
     const int tile = blockIdx.x;
     const int lane = threadIdx.x;
     const int e = tile * TILE + lane;
@@ -268,6 +271,26 @@ __global__ __launch_bounds__(64) void k_stiffness(int n_elem, const int* __restr
                     sFh[a][r][lane] = F[r][0] * g[a][0] + F[r][1] * g[a][1] + F[r][2] * g[a][2];
                 }
             sdV[lane] = detJ_t[((int64_t)tile * NQ + q) * TILE + lane] * quad.w[q];
+            if constexpr (MR) {
+                MrKin k;
+                mr_kinematics(F, mat.mu10, mat.mu01, mat.kappa, k);
+                double FFt[3][3];
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++) FFt[i][j] = F[i][0] * F[j][0] + F[i][1] * F[j][1] + F[i][2] * F[j][2];
+                // fully unrolled: run-time indices into k's arrays would put them in scratch
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++)
+#pragma unroll
+                        for (int kk = 0; kk < 3; kk++)
+#pragma unroll
+                            for (int l = 0; l < 3; l++)
+                                sA[((i * 3 + j) * 3 + kk) * 3 + l][lane] =
+                                    mr_tangent_entry(F, FFt, k, mat.kappa, i, j, kk, l);
+            }
         }
         __syncthreads();
         // sweep (element, a, b) triples of the tile
@@ -293,9 +316,19 @@ __global__ __launch_bounds__(64) void k_stiffness(int n_elem, const int* __restr
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
                     const double dl = (d == c) ? 1.0 : 0.0;
-                    double K = (mat.lambda * Fa[d] * Fb[c] + mat.lambda * trE * hab * dl + mat.mu * FbFa * dl +
-                                mat.mu * Fb[d] * Fa[c] + mat.mu * hab * ff[d][c] - mat.mu * hab * dl) *
-                               dV;
+                    double K;
+                    if constexpr (MR) {
+                        double sum = 0.0;
+#pragma unroll
+                        for (int J = 0; J < 3; J++)
+#pragma unroll
+                            for (int L = 0; L < 3; L++) sum += sA[((d * 3 + J) * 3 + c) * 3 + L][el] * ga[J] * gb[L];
+                        K = sum * dV;
+                    } else {
+                        K = (mat.lambda * Fa[d] * Fb[c] + mat.lambda * trE * hab * dl + mat.mu * FbFa * dl +
+                             mat.mu * Fb[d] * Fa[c] + mat.mu * hab * ff[d][c] - mat.mu * hab * dl) *
+                            dV;
+                    }
                     K *= h;
                     if (damp)
                         K += (mat.eta_damp * (Fb[d] * Fa[c]) + mat.eta_damp * ff[d][c] * hab +
@@ -1193,16 +1226,24 @@ extern "C" int tlfea_stiffness_assemble(const tlfea_pattern* p, int nn, int n_el
                                         const tlfea_quad* quad, const tlfea_material* mat, double h, double* k_sell,
                                         void* stream) {
     TLFEA_REQUIRE(p && conn && x && y && z && grad_t && detJ_t && quad && mat && k_sell, "null argument");
+    TLFEA_REQUIRE_MODEL(*mat);
     TLFEA_REQUIRE(nn == p->nn && n_elem == p->n_elem, "element data does not match the pattern");
     hipStream_t s = as_stream(stream);
     TLFEA_HIP_TRY(hipMemsetAsync(k_sell, 0, sizeof(double) * 9 * p->sell_blocks, s));
     const unsigned tiles = (unsigned)(padded(n_elem) / TILE);
-    if (nn == 10)
-        k_stiffness<10, 5><<<tiles, TILE, 0, s>>>(n_elem, conn, x, y, z, grad_t, detJ_t, *quad, *mat, h, p->d_slot,
-                                                  k_sell);
+    const bool mr = mat->model == 1;
+#define TLFEA_STIFFNESS(NN_, NQ_, MR_)                                                                        \
+    k_stiffness<NN_, NQ_, MR_><<<tiles, TILE, 0, s>>>(n_elem, conn, x, y, z, grad_t, detJ_t, *quad, *mat, h, \
+                                                      p->d_slot, k_sell)
+    if (nn == 10 && mr)
+        TLFEA_STIFFNESS(10, 5, true);
+    else if (nn == 10)
+        TLFEA_STIFFNESS(10, 5, false);
+    else if (mr)
+        TLFEA_STIFFNESS(4, 1, true);
     else
-        k_stiffness<4, 1><<<tiles, TILE, 0, s>>>(n_elem, conn, x, y, z, grad_t, detJ_t, *quad, *mat, h, p->d_slot,
-                                                 k_sell);
+        TLFEA_STIFFNESS(4, 1, false);
+#undef TLFEA_STIFFNESS
     TLFEA_LAUNCH_CHECK();
     return TLFEA_OK;
 }
@@ -1416,6 +1457,7 @@ extern "C" int tlfea_cg_solve_matfree(tlfea_cg* c, const tlfea_force_plan* plan,
     TLFEA_REQUIRE(max_iter >= 0, "negative iteration limit");
     TLFEA_REQUIRE(plan->n_nodes == c->p->n_nodes, "plan and CG workspace sizes differ");
     TLFEA_REQUIRE(mat->eta_damp == 0.0 && mat->lambda_damp == 0.0, "the matrix-free tangent excludes damping");
+    TLFEA_REQUIRE(mat->model == 0, "the matrix-free tangent covers St. Venant-Kirchhoff only (use the assembled path)");
     if (check_every <= 0) check_every = 16;
     hipStream_t s = as_stream(stream);
     const int64_t n = c->n;

@@ -588,6 +588,7 @@ extern "C" int tlfea_tangent_apply_planned(const tlfea_force_plan* plan, const d
                   "null argument");
     TLFEA_REQUIRE(quad->n == (plan->nn == 10 ? 5 : 1), "quadrature size does not match the element type");
     TLFEA_REQUIRE(mat->eta_damp == 0.0 && mat->lambda_damp == 0.0, "the matrix-free tangent excludes damping");
+    TLFEA_REQUIRE(mat->model == 0, "the matrix-free tangent covers St. Venant-Kirchhoff only (use the assembled path)");
     // the linear-elasticity form when the positions are the rest positions' own arrays
     const bool same = x == X0 && y == Y0 && z == Z0;
     return launch_tangent_planned(plan, x, y, z, X0, Y0, Z0, jinv_t, detJ_t, *quad, *mat, fixed_dof_mask, p, out,

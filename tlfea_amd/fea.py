@@ -58,6 +58,16 @@ class Body:
         self.E, self.nu, self.rho = E, nu, rho
         self.mat = capi.material(E, nu, eta, lambda_d) if (E or nu) else capi.Material(0, 0, eta, lambda_d)
 
+    def set_mooney_rivlin(self, mu10, mu01, kappa):
+        """Compressible Mooney-Rivlin in place of SVK, the reference's
+        SetMooneyRivlin(mu10, mu01, kappa) (`src/elements/FEAT10Data.cuh:571-587`); damping
+        and rho are kept.  set_svk() switches back."""
+        self.mat.model, self.mat.mu10, self.mat.mu01, self.mat.kappa = 1, float(mu10), float(mu01), float(kappa)
+
+    def set_svk(self):
+        """SVK with the current lambda, mu (`src/elements/FEAT10Data.cuh:533-552`)."""
+        self.mat.model, self.mat.mu10, self.mat.mu01, self.mat.kappa = 0, 0.0, 0.0, 0.0
+
     @property
     def s(self):
         return self.stream
@@ -140,7 +150,8 @@ class Body:
 
     def elastic_material(self):
         """The material without Kelvin-Voigt damping (the tangent of the elastic force)."""
-        return capi.Material(self.mat.lam, self.mat.mu, 0.0, 0.0)
+        m = self.mat
+        return capi.Material(m.lam, m.mu, 0.0, 0.0, m.model, m.mu10, m.mu01, m.kappa)
 
     def fixed_dof_mask(self):
         """uint8 [3*n_nodes], 1 on the DOFs of fixed nodes."""
