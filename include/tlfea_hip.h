@@ -278,7 +278,9 @@ int tlfea_batch_set_chunk(tlfea_batch* b, int chunk_nodes);
 /* Autotuner (not in the reference): times batched CG iterations for every chunk size in
  * {256, 512, 1024, 2048, 4096} nodes and graph-replay length in {16, 32, 64} iterations
  * (runs of n and 2n iterations from u = 0 on load f, n >= trial_iters, no instance
- * finishing; the difference cancels the per-solve graph capture), keeps the fastest pair
+ * finishing, each timed twice with the faster kept; the difference cancels the per-solve
+ * graph capture, and where noise makes it non-positive the 2n run's mean per iteration
+ * stands in), keeps the fastest pair
  * (chunk via tlfea_batch_set_chunk, replay length as the default of solves that pass
  * poll <= 0), and with full_solves != 0 then times full solves to tol for compaction
  * ratios {0.35, 0.5, 0.65, 0.8, 0.9} and keeps the fastest.  u is a scratch output

@@ -888,12 +888,20 @@ extern "C" int tlfea_batch_autotune(tlfea_batch* b, const double* f, double* u, 
         if (int rc = build_chunks(b, chunk)) return rc;
         for (int poll : poll_set) {
             // per-iteration time from runs of n and 2n iterations: the graph capture every
-            // solve pays (and the host round trips at its ends) cancel in the difference
+            // solve pays (and the host round trips at its ends) cancel in the difference.
+            // Each length is timed twice and the faster run kept (the first run after a
+            // chunk change pays one-time costs); a difference lost in the noise falls back
+            // to the 2n run's mean, an upper bound.
             const int n = poll * ((trial_iters + poll - 1) / poll);
-            double t1 = 0.0, t2 = 0.0;
-            if (int rc = timed(n, poll, 0.0, t1)) return rc;
-            if (int rc = timed(2 * n, poll, 0.0, t2)) return rc;
-            const double per = (t2 - t1) / n;
+            double t1 = 1e300, t2 = 1e300;
+            for (int rep = 0; rep < 2; rep++) {
+                double a = 0.0, c = 0.0;
+                if (int rc = timed(n, poll, 0.0, a)) return rc;
+                if (int rc = timed(2 * n, poll, 0.0, c)) return rc;
+                t1 = std::min(t1, a);
+                t2 = std::min(t2, c);
+            }
+            const double per = t2 > t1 ? (t2 - t1) / n : t2 / (2.0 * n);
             if (per < best) {
                 best = per;
                 best_chunk = chunk;
