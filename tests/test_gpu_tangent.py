@@ -98,11 +98,45 @@ def test_tangent_rest_state_form(gpu, name):
     y_gen = _np(b.tangent_apply(pt))            # separate x arrays equal to X0: general form
     y_lin = _np(b.tangent_apply(pt, x=b.X0))    # the rest arrays themselves: rest-state form
     assert rel_err(y_lin, y_gen) < 1e-14
+    # on an affine TET10 mesh the rest-state form integrates from the corner values instead
+    # of the five points (exact for both) with point 0's M: the same operator to rounding
+    # (the bar of the affine form, test_affine_mesh)
+    y_pts = _np(b.tangent_apply(pt, x=b.X0, affine=False))
+    assert rel_err(y_lin, y_pts) < 1e-13
     go, djo = O.grad(m)
     lam, mu = lame(E_STEEL, NU_STEEL)
     doff, dcol = O.dof_pattern(m)
     vo, _ = O.apply_dirichlet(doff, dcol, O.stiffness_values(m, go, djo, lam, mu, doff, dcol), m.f_ext, m.fixed)
     assert rel_err(y_lin, O.spmv(doff, dcol, vo, _np(pt))) < 1e-12
+
+
+def test_shuffled_mesh_large_blocks(gpu):
+    """Elements in random order: each 256-element block touches ~1800 nodes, so every thread
+    stages several groups of nodes.  Rest-state, affine and general forms against the
+    oracle, and the matrix-free CG (with its fused update) against the oracle's CG."""
+    import torch
+    from tlfea_amd.fea import LinearStatic
+    m = load_mesh("beam_highres_t10")
+    m.elems = m.elems[np.random.default_rng(0).permutation(m.n_elem)]
+    b = _body(m)
+    assert b.plan_sizes()["max_block_nodes"] > 1024
+    go, djo = O.grad(m)
+    lam, mu = lame(E_STEEL, NU_STEEL)
+    doff, dcol = O.dof_pattern(m)
+    pt = torch.as_tensor(np.random.default_rng(15).standard_normal(3 * m.n_nodes)).cuda()
+    vr, fo = O.apply_dirichlet(doff, dcol, O.stiffness_values(m, go, djo, lam, mu, doff, dcol), m.f_ext, m.fixed)
+    assert rel_err(_np(b.tangent_apply(pt, x=b.X0)), O.spmv(doff, dcol, vr, _np(pt))) < 1e-12
+    assert rel_err(_np(b.tangent_apply(pt, x=b.X0, affine=False)), O.spmv(doff, dcol, vr, _np(pt))) < 1e-12
+    xd = deformed(m, seed=16, amp=1e-4)
+    _set_positions(b, xd)
+    vd, _ = O.apply_dirichlet(doff, dcol, O.stiffness_values(m, go, djo, lam, mu, doff, dcol, *xd), m.f_ext, m.fixed)
+    assert rel_err(_np(b.tangent_apply(pt)), O.spmv(doff, dcol, vd, _np(pt))) < 1e-12
+    _set_positions(b, (m.x, m.y, m.z))
+    ls = LinearStatic(b, tol=0.0, max_iter=10, matfree=True)
+    ls.assemble()
+    ls.cg_solve()
+    uk, _, _ = O.cg(doff, dcol, vr, fo, tol=0.0, max_iter=10)
+    assert ls.iters == 10 and rel_err(_np(ls.u), uk) < 1e-11
 
 
 def test_affine_mesh(gpu):

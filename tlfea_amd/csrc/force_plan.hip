@@ -430,6 +430,8 @@ void free_plan(tlfea_force_plan* p) {
     (void)hipFree(p->d_part);
     (void)hipFree(p->d_g_off);
     (void)hipFree(p->d_g_idx);
+    (void)hipFree(p->d_own);
+    (void)hipFree(p->d_dot);
     delete p;
 }
 
@@ -505,9 +507,18 @@ extern "C" int tlfea_force_plan_create(int nn, int n_elem, int n_nodes, const in
     std::vector<int64_t> g_idx(unodes.size());
     std::vector<int> cur(g_off.begin(), g_off.end() - 1);
     for (int64_t k = 0; k < (int64_t)unodes.size(); k++) g_idx[cur[unodes[k]]++] = k;
+    // a node's first partial in block order: where the fused CG counts its masked DOFs
+    std::vector<unsigned char> own(unodes.size(), 0), seen(n_nodes, 0);
+    for (size_t k = 0; k < unodes.size(); k++)
+        if (!seen[unodes[k]]) {
+            seen[unodes[k]] = 1;
+            own[k] = 1;
+        }
+    std::vector<double> dot0(nb, 0.0);
     int rc = TLFEA_OK;
     if ((rc = upload(&P->d_blk_off, blk_off)) || (rc = upload(&P->d_unodes, unodes)) ||
-        (rc = upload(&P->d_slot, slot)) || (rc = upload(&P->d_g_off, g_off)) || (rc = upload(&P->d_g_idx, g_idx))) {
+        (rc = upload(&P->d_slot, slot)) || (rc = upload(&P->d_g_off, g_off)) || (rc = upload(&P->d_g_idx, g_idx)) ||
+        (rc = upload(&P->d_own, own)) || (rc = upload(&P->d_dot, dot0))) {
         free_plan(P);
         return rc;
     }

@@ -33,6 +33,8 @@ struct tlfea_force_plan {
     double* d_part = nullptr;       // [n_partials*3] block partial forces
     int* d_g_off = nullptr;         // [n_nodes+1] node -> partial list
     int64_t* d_g_idx = nullptr;     // partial positions, ascending per node
+    unsigned char* d_own = nullptr; // [n_partials] 1 on a node's first partial (lowest block)
+    double* d_dot = nullptr;        // [n_blocks] per-block p.Kp partials of the fused CG
 };
 
 namespace tlfea_hip {
@@ -88,12 +90,16 @@ struct TanMat {
 
 // Block partials of the matrix-free tangent (tangent.hip) into plan->d_part (read as T),
 // for all plan blocks or, with blk_list, for the n_list blocks it names.  X0 = Y0 = Z0 =
-// null: x, y, z hold the displacement itself.
+// null: x, y, z hold the displacement itself.  With dot (all blocks, no blk_list) each
+// block also writes its share of p . (K p) with masked DOFs as identity rows to
+// plan->d_dot[block]: the block's unmasked DOFs' p times its partials, plus p^2 on the
+// masked DOFs of the nodes whose first partial it holds (summed over blocks in order this
+// is the gather's p . out).
 template <typename T>
 int launch_tangent_partials(const tlfea_force_plan* pl, const double* x, const double* y, const double* z,
                             const double* X0, const double* Y0, const double* Z0, const T* jinv_t, const T* detJ_t,
                             const tlfea_quad& quad, const TanMat<T>& mat, const unsigned char* mask, const T* p,
-                            const int* flags, hipStream_t s, const int* blk_list, int n_list);
+                            const int* flags, hipStream_t s, const int* blk_list, int n_list, bool dot = false);
 
 // Block partials of the tangent's diagonal (FP64), gathered like the tangent's.
 int launch_tangent_diag_partials(const tlfea_force_plan* pl, const double* x, const double* y, const double* z,

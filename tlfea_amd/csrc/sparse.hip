@@ -37,7 +37,7 @@ struct tlfea_cg {
     float* d_p32 = nullptr;   // FP32 direction
     float* d_q32 = nullptr;   // FP32 K p
     double* d_part = nullptr;   // [RED_BLOCKS + operator blocks]: operator p.Kp partials
-    double* d_part2 = nullptr;  // [RED_BLOCKS]: r.r partials of the update
+    double* d_part2 = nullptr;  // [max(RED_BLOCKS, node blocks)]: r.r partials of the update
     double* d_scal = nullptr;   // [16]: rr0, rr, alpha, beta, rel, pKp; 10, 11: rr by iteration parity
     int* d_flags = nullptr;    // [4]: 0 done, 1 notpd, 2 iters
 };
@@ -703,6 +703,58 @@ __global__ __launch_bounds__(RED_THREADS) void k_cg_update(int64_t n, double* __
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// The update of the matrix-free CG with the operator's gather folded in: alpha = rr / p.Kp
+// from the tangent blocks' partials (every block sums them in block order), then per node
+// Kp = the sum of its block partials in ascending block order (the order of
+// k_gather_tangent), or p on masked DOFs, u += alpha p, r -= alpha Kp, and the partials
+// of r.r.  Kp is never stored.  One thread per node: the chain of dependent loads (offsets,
+// partial positions, partials) of each node then overlaps with every other node's.
+__global__ __launch_bounds__(RED_THREADS) void k_cg_update_gather(
+    int n_nodes, const int* __restrict__ g_off, const int64_t* __restrict__ g_idx, const double* __restrict__ bpart,
+    const unsigned char* __restrict__ mask, double* __restrict__ u, double* __restrict__ r,
+    const double* __restrict__ p, const double* __restrict__ in_part, int n_in, double* __restrict__ part,
+    double* __restrict__ scal, int* __restrict__ flags, int par) {
+    if (block_stopped(flags)) return;
+    const double pKp = sum_parts_all(in_part, n_in);
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (pKp <= 0.0) {
+        if (lead) {
+            scal[5] = pKp;
+            flags[1] = 1;
+            flags[0] = 1;
+        }
+        return;
+    }
+    const double alpha = scal[10 + par] / pKp;
+    if (lead) {
+        scal[5] = pKp;
+        scal[2] = alpha;
+    }
+    double s = 0.0;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_nodes) {
+        double kp[3] = {0.0, 0.0, 0.0};
+        for (int k = g_off[i]; k < g_off[i + 1]; k++) {
+            const double* pp = bpart + 3 * g_idx[k];
+            kp[0] += pp[0];
+            kp[1] += pp[1];
+            kp[2] += pp[2];
+        }
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const int64_t j = 3 * (int64_t)i + d;
+            const double pj = p[j];
+            const double v = (mask && mask[j]) ? pj : kp[d];
+            u[j] += alpha * pj;
+            const double rj = r[j] - alpha * v;
+            r[j] = rj;
+            s += rj * rj;
+        }
+    }
+    s = block_sum(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
 // Convergence test sqrt(rr'/rr0) < tol (LinearStaticSolver.cu:418-423), beta = rr'/rr and
 // p = beta p + r, with rr' summed by every block from the update's partials (as in
 // k_cg_update).  The next iteration's rr goes to scal[10 + (par ^ 1)], a slot no block of
@@ -1356,7 +1408,7 @@ extern "C" int tlfea_cg_create(const tlfea_pattern* p, tlfea_cg** out) {
     int rc = TLFEA_OK;
     if ((rc = dmalloc(&c->d_r, c->n)) || (rc = dmalloc(&c->d_p, c->n)) || (rc = dmalloc(&c->d_Kp, c->n)) ||
         (rc = dmalloc(&c->d_part, RED_BLOCKS + (int)(p->n_slices * TILE / 256) + 1)) ||
-        (rc = dmalloc(&c->d_part2, RED_BLOCKS)) ||
+        (rc = dmalloc(&c->d_part2, std::max<int64_t>(RED_BLOCKS, grid_for(p->n_nodes, RED_THREADS)))) ||
         (rc = dmalloc(&c->d_scal, 16)) || (rc = dmalloc(&c->d_flags, 8)) || (rc = dmalloc(&c->d_r32, c->n)) ||
         (rc = dmalloc(&c->d_p32, c->n)) || (rc = dmalloc(&c->d_q32, c->n))) {
         tlfea_cg_destroy(c);
@@ -1462,8 +1514,6 @@ extern "C" int tlfea_cg_solve_matfree(tlfea_cg* c, const tlfea_force_plan* plan,
     hipStream_t s = as_stream(stream);
     const int64_t n = c->n;
     const unsigned gv = RED_BLOCKS;
-    // the gather's dot partials (one per 256 nodes) fit d_part: it holds n_slices*64/256+1 more
-    const int gn = (int)grid_for(plan->n_nodes, 256);
     // at the rest positions K is the linear-elasticity operator (the kernel's u = 0 form);
     // the check uses the flag word before the initialisation resets it
     bool at_rest = false;
@@ -1473,13 +1523,19 @@ extern "C" int tlfea_cg_solve_matfree(tlfea_cg* c, const tlfea_force_plan* plan,
     TLFEA_LAUNCH_CHECK();
     int flags[4] = {0, 0, 0, 0};
     double scal[8] = {0};
+    const unsigned gu = grid_for(plan->n_nodes, RED_THREADS);  // one update thread per node
+    TanMat<double> tm{mat->lambda, mat->mu, nullptr, nullptr, nullptr, nullptr, 0};
+    tm.at_rest = at_rest ? 1 : 0;
+    tm.affine = affine != 0 ? 1 : 0;
     for (int it = 0; it < max_iter; it++) {
-        int rc = launch_tangent_planned(plan, x, y, z, X0, Y0, Z0, jinv_t, detJ_t, *quad, *mat, fixed_dof_mask,
-                                        c->d_p, c->d_Kp, c->d_part, c->d_flags, s, at_rest, affine != 0);
+        // block partials and their p.Kp shares; the update sums both (no gather launch)
+        int rc = launch_tangent_partials<double>(plan, x, y, z, X0, Y0, Z0, jinv_t, detJ_t, *quad, tm,
+                                                 fixed_dof_mask, c->d_p, c->d_flags, s, nullptr, 0, true);
         if (rc != TLFEA_OK) return rc;
-        k_cg_update<<<gv, RED_THREADS, 0, s>>>(n, u, c->d_r, c->d_p, c->d_Kp, c->d_part, gn, c->d_part2, c->d_scal,
-                                               c->d_flags, it & 1);
-        k_cg_dir<<<gv, RED_THREADS, 0, s>>>(n, c->d_p, c->d_r, c->d_part2, (int)gv, c->d_scal, c->d_flags, tol,
+        k_cg_update_gather<<<gu, RED_THREADS, 0, s>>>(plan->n_nodes, plan->d_g_off, plan->d_g_idx, plan->d_part,
+                                                      fixed_dof_mask, u, c->d_r, c->d_p, plan->d_dot, plan->n_blocks,
+                                                      c->d_part2, c->d_scal, c->d_flags, it & 1);
+        k_cg_dir<<<gv, RED_THREADS, 0, s>>>(n, c->d_p, c->d_r, c->d_part2, (int)gu, c->d_scal, c->d_flags, tol,
                                             it & 1);
         TLFEA_LAUNCH_CHECK();
         if ((it + 1) % check_every == 0 || it + 1 == max_iter) {

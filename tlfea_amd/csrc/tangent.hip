@@ -171,6 +171,101 @@ __device__ __forceinline__ void tangent_qp(int q, int64_t tile, int lane, const 
     }
 }
 
+// The rest-state operator of an affine TET10 element from its values at the four corners.
+// With M and det J constant, Jp(xi) = sum_a p_a (x) dN_a/dxi and Q(xi) = C : (Jp M^T) M are
+// linear in xi and y_a = det J int Q dN_a/dxi dxi has a degree-2 integrand, which the
+// 5-point rule integrates exactly; so does int L_v L_w dxi = (1 + delta_vw)/120 applied to
+// the linear interpolants through the corners.  At corner w (L_w = 1), with
+// A = sum_c p_c (x) grad L_c over the corners and P_wj = p_w (j = w) or p of edge (w, j):
+//     Jp(w) = -A + 4 sum_j P_wj (x) grad L_j,
+// and with Q_w = C : (Jp(w) M^T) M det J / 120, Qs = sum_w Q_w:
+//     corner a:    y_a = (4 Q_a - Qs) grad L_a,
+//     edge (i, j): y = 4 [(Q_i + Qs) grad L_j + (Q_j + Qs) grad L_i].
+// The same values as the 5-point evaluation to rounding, in about a third of the
+// arithmetic and a third of the LDS reads; outputs go straight to the LDS accumulator.
+template <typename T>
+__device__ __forceinline__ T grad_l_col(const T (&R)[3][3], int d, int c) {
+    // (R grad L_c)_d with grad L_0 = -(1, 1, 1), grad L_k = e_k
+    return c == 0 ? -(R[d][0] + R[d][1] + R[d][2]) : R[d][c - 1];
+}
+
+template <typename T>
+__device__ __forceinline__ void tangent_vertex_t10(const int (&sl)[10], const T* sP, T* sF, int max_u,
+                                                   const T (&Mn)[9], T dJ, T lam, T mu, T ks) {
+    // edge node between corners w and j
+    constexpr int EDGE[4][4] = {{0, 4, 6, 7}, {4, 0, 5, 8}, {6, 5, 0, 9}, {7, 8, 9, 0}};
+    constexpr int EDGES[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
+    T M[3][3];
+#pragma unroll
+    for (int c = 0; c < 9; c++) M[c / 3][c % 3] = Mn[c];
+    const T scale = dJ * ks * (T(1) / T(120));
+    T A[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const T p0 = sP[i * max_u + sl[0]];
+#pragma unroll
+        for (int k = 0; k < 3; k++) A[i][k] = sP[i * max_u + sl[k + 1]] - p0;
+    }
+    T Q[4][3][3];
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        T Jp[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            T P[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) P[j] = sP[i * max_u + sl[j == w ? w : EDGE[w][j]]];
+#pragma unroll
+            for (int k = 0; k < 3; k++) Jp[i][k] = T(4) * (P[k + 1] - P[0]) - A[i][k];
+        }
+        T dF[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) dF[i][j] = Jp[i][0] * M[j][0] + Jp[i][1] * M[j][1] + Jp[i][2] * M[j][2];
+        const T ltr = lam * (dF[0][0] + dF[1][1] + dF[2][2]);
+        T dP[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) dP[i][j] = (mu * (dF[i][j] + dF[j][i]) + (i == j ? ltr : T(0))) * scale;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) Q[w][i][k] = dP[i][0] * M[0][k] + dP[i][1] * M[1][k] + dP[i][2] * M[2][k];
+    }
+    T Qs[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) Qs[i][k] = Q[0][i][k] + Q[1][i][k] + Q[2][i][k] + Q[3][i][k];
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        T R[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) R[i][k] = T(4) * Q[a][i][k] - Qs[i][k];
+#pragma unroll
+        for (int d = 0; d < 3; d++) atomicAdd(&sF[d * max_u + sl[a]], grad_l_col(R, d, a));
+    }
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+        const int ci = EDGES[m][0], cj = EDGES[m][1];
+        T Ri[3][3], Rj[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                Ri[i][k] = Q[ci][i][k] + Qs[i][k];
+                Rj[i][k] = Q[cj][i][k] + Qs[i][k];
+            }
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            atomicAdd(&sF[d * max_u + sl[4 + m]], T(4) * (grad_l_col(Ri, d, cj) + grad_l_col(Rj, d, ci)));
+    }
+}
+
 template <int NN, int NQ, typename T, bool STD, int MASS, bool LIN, bool AFF>
 __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent(
     int n_elem, const int64_t* __restrict__ blk_off, const int* __restrict__ unodes,
@@ -178,7 +273,8 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent(
     const double* __restrict__ z, const double* __restrict__ X0, const double* __restrict__ Y0,
     const double* __restrict__ Z0, const T* __restrict__ p, const unsigned char* __restrict__ mask,
     const T* __restrict__ jinv_t, const T* __restrict__ detJ_t, DnTable<T> dn, tlfea_quad quad, TanMat<T> mat,
-    T* __restrict__ part, int max_u, const int* __restrict__ flags, const int* __restrict__ blk_list) {
+    T* __restrict__ part, int max_u, const int* __restrict__ flags, const int* __restrict__ blk_list,
+    double* __restrict__ dot_part, const unsigned char* __restrict__ own) {
     if (flags && flags[0]) return;
     extern __shared__ unsigned char lds_raw[];
     T* sU = reinterpret_cast<T*>(lds_raw);  // [3][max_u] displacements
@@ -207,21 +303,61 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent(
         for (int c = 0; c < 9; c++) Mn[c] = Mp[c * TILE];
         dJn = detJ_t[(int64_t)tile * NQ * TILE + lane];
     }
-    for (int u = threadIdx.x; u < U; u += PLAN_BLOCK) {
-        const int n = unodes[u0 + u];
-        if constexpr (!LIN) {
-            // without rest arrays x, y, z are the displacement itself (no cancellation of
-            // x - X0 when u is many orders below the coordinates)
-            sU[u] = X0 ? (T)(x[n] - X0[n]) : (T)x[n];
-            sU[max_u + u] = X0 ? (T)(y[n] - Y0[n]) : (T)y[n];
-            sU[2 * max_u + u] = X0 ? (T)(z[n] - Z0[n]) : (T)z[n];
-        }
-        sF[u] = sF[max_u + u] = sF[2 * max_u + u] = T(0);
+    // staging, SPT nodes per thread at a time: every node index is requested before any
+    // value, so each thread waits for two dependent round trips instead of two per node.
+    // With the block p.Kp shares, the masked DOFs' p^2 of the nodes this block owns is
+    // summed here (sP is zero there, so the end of the kernel needs only sP . sF).
+    constexpr int SPT = 4;
+    double mdot = 0.0;
+    for (int base = threadIdx.x; base < U; base += SPT * PLAN_BLOCK) {
+        int nid[SPT];
 #pragma unroll
-        for (int d = 0; d < 3; d++) sP[d * max_u + u] = (mask && mask[3 * n + d]) ? T(0) : p[3 * n + d];
+        for (int j = 0; j < SPT; j++) {
+            const int u = base + j * PLAN_BLOCK;
+            nid[j] = u < U ? unodes[u0 + u] : -1;
+        }
+        T pv[SPT][3], uv[SPT][3];
+        bool mk[SPT][3];
+#pragma unroll
+        for (int j = 0; j < SPT; j++) {
+            const int n = nid[j];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                pv[j][d] = n >= 0 ? p[3 * n + d] : T(0);
+                mk[j][d] = n >= 0 && mask && mask[3 * n + d];
+            }
+            if constexpr (!LIN) {
+                // without rest arrays x, y, z are the displacement itself (no cancellation of
+                // x - X0 when u is many orders below the coordinates)
+                uv[j][0] = n < 0 ? T(0) : X0 ? (T)(x[n] - X0[n]) : (T)x[n];
+                uv[j][1] = n < 0 ? T(0) : X0 ? (T)(y[n] - Y0[n]) : (T)y[n];
+                uv[j][2] = n < 0 ? T(0) : X0 ? (T)(z[n] - Z0[n]) : (T)z[n];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < SPT; j++) {
+            const int u = base + j * PLAN_BLOCK;
+            if (u >= U) break;
+            if constexpr (!LIN) {
+                sU[u] = uv[j][0];
+                sU[max_u + u] = uv[j][1];
+                sU[2 * max_u + u] = uv[j][2];
+            }
+            sF[u] = sF[max_u + u] = sF[2 * max_u + u] = T(0);
+#pragma unroll
+            for (int d = 0; d < 3; d++) sP[d * max_u + u] = mk[j][d] ? T(0) : pv[j][d];
+            if (dot_part && own[u0 + u]) {
+#pragma unroll
+                for (int d = 0; d < 3; d++)
+                    if (mk[j][d]) mdot += (double)pv[j][d] * (double)pv[j][d];
+            }
+        }
     }
     __syncthreads();
-    if (active) {
+    if constexpr (NN == 10 && STD && MASS == 0 && LIN && AFF) {
+        if (active)
+            tangent_vertex_t10<T>(sl, sP, sF, max_u, Mn, dJn, tan_lam(mat, inst), tan_mu(mat, inst), mat.k_scale);
+    } else if (active) {
         const T lam = tan_lam(mat, inst), mu = tan_mu(mat, inst);
         const T mc = mat.elem_inst ? (mat.inst_mc ? mat.inst_mc[inst] : T(0)) : mat.mc;
         const T ks = mat.k_scale;
@@ -263,11 +399,31 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent(
             for (int i = 0; i < 3; i++) atomicAdd(&sF[i * max_u + sl[a]], fa[a][i]);
     }
     __syncthreads();
+    // this block's share of p . out, out = K p on free DOFs (sP = p) and p on masked ones
+    // (sP = 0 there; their p^2 is in mdot)
+    double dot = mdot;
     for (int u = threadIdx.x; u < U; u += PLAN_BLOCK) {
         T* o = part + 3 * (u0 + u);
-        o[0] = sF[u];
-        o[1] = sF[max_u + u];
-        o[2] = sF[2 * max_u + u];
+        const T f0 = sF[u], f1 = sF[max_u + u], f2 = sF[2 * max_u + u];
+        o[0] = f0;
+        o[1] = f1;
+        o[2] = f2;
+        if (dot_part)
+            dot += (double)sP[u] * (double)f0 + (double)sP[max_u + u] * (double)f1 +
+                   (double)sP[2 * max_u + u] * (double)f2;
+    }
+    if (dot_part) {  // uniform over the block
+        __shared__ double red[PLAN_BLOCK / 64];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) dot += __shfl_xor(dot, o, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dot;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s = 0.0;
+#pragma unroll
+            for (int w = 0; w < PLAN_BLOCK / 64; w++) s += red[w];
+            dot_part[b] = s;
+        }
     }
 }
 
@@ -339,8 +495,13 @@ template <typename T>
 int launch_tangent_partials(const tlfea_force_plan* pl, const double* x, const double* y, const double* z,
                             const double* X0, const double* Y0, const double* Z0, const T* jinv_t, const T* detJ_t,
                             const tlfea_quad& quad, const TanMat<T>& mat, const unsigned char* mask, const T* p,
-                            const int* flags, hipStream_t s, const int* blk_list, int n_list) {
+                            const int* flags, hipStream_t s, const int* blk_list, int n_list, bool dot) {
     const int nb = blk_list ? n_list : pl->n_blocks;
+    if (dot && blk_list) {
+        set_error("block p.Kp partials need every plan block");
+        return TLFEA_EINVAL;
+    }
+    double* dot_part = dot ? pl->d_dot : nullptr;
     if (nb == 0) return TLFEA_OK;
     const DnTable<double> dn64 = make_dn_table(pl->nn, quad);
     DnTable<T> dn;
@@ -365,7 +526,7 @@ int launch_tangent_partials(const tlfea_force_plan* pl, const double* x, const d
 #define TLFEA_TANGENT_LAUNCH(NN_, NQ_, STD_, MASS_, LIN_, AFF_)                                              \
     k_tangent<NN_, NQ_, T, STD_, MASS_, LIN_, AFF_><<<nb, PLAN_BLOCK, lds, s>>>(                              \
         pl->n_elem, pl->d_blk_off, pl->d_unodes, pl->d_slot, x, y, z, X0, Y0, Z0, p, mask, jinv_t, detJ_t, dn, quad, \
-        mat, part, pl->max_u, flags, blk_list)
+        mat, part, pl->max_u, flags, blk_list, dot_part, pl->d_own)
 #define TLFEA_TANGENT_BY_MASS(NN_, NQ_, STD_, LIN_, AFF_)        \
     do {                                                         \
         if (mass == 2)                                           \
@@ -405,11 +566,11 @@ template int launch_tangent_partials<double>(const tlfea_force_plan*, const doub
                                              const double*, const double*, const double*, const double*,
                                              const double*, const tlfea_quad&, const TanMat<double>&,
                                              const unsigned char*, const double*, const int*, hipStream_t,
-                                             const int*, int);
+                                             const int*, int, bool);
 template int launch_tangent_partials<float>(const tlfea_force_plan*, const double*, const double*, const double*,
                                             const double*, const double*, const double*, const float*, const float*,
                                             const tlfea_quad&, const TanMat<float>&, const unsigned char*,
-                                            const float*, const int*, hipStream_t, const int*, int);
+                                            const float*, const int*, hipStream_t, const int*, int, bool);
 
 int launch_tangent_planned(const tlfea_force_plan* pl, const double* x, const double* y, const double* z,
                            const double* X0, const double* Y0, const double* Z0, const double* jinv_t,
