@@ -60,13 +60,13 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes of one matrix-free tangent launch pair (k_tangent + k_gather_tangent) on the
 # 200x20x20 beam: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (KiB), scales calibrated
 # on a 512 MiB copy
-PMC_TANGENT_BYTES = (76981 * 2.0 + 29109 + 30795 * 2.0 + 21629) * 1024
+PMC_TANGENT_BYTES = (77026 * 2.0 + 29110 + 30785 * 2.0 + 21492) * 1024
 # the same for the rest-state (linear-elasticity) form the linear-static step runs
-PMC_TANGENT_REST_BYTES = (43693 * 2.0 + 29109 + 30795 * 2.0 + 21629) * 1024
+PMC_TANGENT_REST_BYTES = (43713 * 2.0 + 29109 + 30785 * 2.0 + 21492) * 1024
 # the same for the force kernels with k_gather_partials: gradient-reading (k_force_planned)
 # and rest-coordinate (k_force_rest)
-PMC_FORCE_BYTES = (314624 * 2.0 + 29110 + 21891 * 2.0 + 15799) * 1024
-PMC_FORCE_REST_BYTES = (40677 * 2.0 + 29109 + 21894 * 2.0 + 15799) * 1024
+PMC_FORCE_BYTES = (314624 * 2.0 + 29112 + 21890 * 2.0 + 15799) * 1024
+PMC_FORCE_REST_BYTES = (40677 * 2.0 + 29110 + 21890 * 2.0 + 15799) * 1024
 
 
 def log(*a):
