@@ -514,7 +514,7 @@ extern "C" int tlfea_force_plan_create(int nn, int n_elem, int n_nodes, const in
             seen[unodes[k]] = 1;
             own[k] = 1;
         }
-    std::vector<double> dot0(nb, 0.0);
+    std::vector<double> dot0(2 * (size_t)nb, 0.0);
     int rc = TLFEA_OK;
     if ((rc = upload(&P->d_blk_off, blk_off)) || (rc = upload(&P->d_unodes, unodes)) ||
         (rc = upload(&P->d_slot, slot)) || (rc = upload(&P->d_g_off, g_off)) || (rc = upload(&P->d_g_idx, g_idx)) ||

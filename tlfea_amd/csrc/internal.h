@@ -34,7 +34,7 @@ struct tlfea_force_plan {
     int* d_g_off = nullptr;         // [n_nodes+1] node -> partial list
     int64_t* d_g_idx = nullptr;     // partial positions, ascending per node
     unsigned char* d_own = nullptr; // [n_partials] 1 on a node's first partial (lowest block)
-    double* d_dot = nullptr;        // [n_blocks] per-block p.Kp partials of the fused CG
+    double* d_dot = nullptr;        // [2][n_blocks] per-block p.Kp and p.p shares of the fused CG
 };
 
 namespace tlfea_hip {
@@ -94,7 +94,8 @@ struct TanMat {
 // block also writes its share of p . (K p) with masked DOFs as identity rows to
 // plan->d_dot[block]: the block's unmasked DOFs' p times its partials, plus p^2 on the
 // masked DOFs of the nodes whose first partial it holds (summed over blocks in order this
-// is the gather's p . out).
+// is the gather's p . out); and its share of p . p (p^2 on every DOF of the nodes whose
+// first partial it holds) to plan->d_dot[n_blocks + block].
 template <typename T>
 int launch_tangent_partials(const tlfea_force_plan* pl, const double* x, const double* y, const double* z,
                             const double* X0, const double* Y0, const double* Z0, const T* jinv_t, const T* detJ_t,

@@ -703,56 +703,127 @@ __global__ __launch_bounds__(RED_THREADS) void k_cg_update(int64_t n, double* __
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// The update of the matrix-free CG with the operator's gather folded in: alpha = rr / p.Kp
-// from the tangent blocks' partials (every block sums them in block order), then per node
-// Kp = the sum of its block partials in ascending block order (the order of
-// k_gather_tangent), or p on masked DOFs, u += alpha p, r -= alpha Kp, and the partials
-// of r.r.  Kp is never stored.  One thread per node: the chain of dependent loads (offsets,
-// partial positions, partials) of each node then overlaps with every other node's.
-__global__ __launch_bounds__(RED_THREADS) void k_cg_update_gather(
+// One iteration of the matrix-free CG after its operator launch, in the single-reduction
+// arrangement of Chronopoulos and Gear (J. Comput. Appl. Math. 25 (1989) 153-168): the
+// operator is applied to r, and its launch also writes each block's shares of
+// delta = r.Kr and gamma = r.r (`dot`: [delta shares | gamma shares]).  Every block sums
+// them in block order, tests sqrt(gamma / gamma_0) < tol (the stopping rule of
+// LinearStaticSolver.cu:418-423, applied to r of this iteration), and forms
+//     beta = gamma / gamma_old,  alpha = gamma / (delta - beta gamma / alpha_old)
+// (beta = 0, alpha = gamma / delta at k = 0; the denominator is p.Kp, <= 0 aborts as
+// LinearStaticSolver.cu:404-407 does); then per node: w = the sum of its block partials
+// in ascending block order (K r, or r on masked DOFs), p = r + beta p, s = w + beta s
+// (= K p), u += alpha p, r -= alpha s.  The iterates are those of the reference's CG up
+// to rounding (the same iteration counts and solutions to ~1e-10 on the test beams).
+// gamma_old / alpha_old live in scal[12 + parity] / scal[14 + parity], so no block reads
+// a slot another block of the same launch writes.  One thread per node.
+__global__ __launch_bounds__(RED_THREADS) void k_cgcg_update(
     int n_nodes, const int* __restrict__ g_off, const int64_t* __restrict__ g_idx, const double* __restrict__ bpart,
-    const unsigned char* __restrict__ mask, double* __restrict__ u, double* __restrict__ r,
-    const double* __restrict__ p, const double* __restrict__ in_part, int n_in, double* __restrict__ part,
-    double* __restrict__ scal, int* __restrict__ flags, int par) {
-    if (block_stopped(flags)) return;
-    const double pKp = sum_parts_all(in_part, n_in);
+    const unsigned char* __restrict__ mask, double* __restrict__ u, double* __restrict__ r, double* __restrict__ p,
+    double* __restrict__ sv, const double* __restrict__ dot, int nb, double* __restrict__ scal,
+    int* __restrict__ flags, int k, double tol) {
+    __shared__ int stop;
+    __shared__ double red[2][RED_THREADS / 64];
+    __shared__ double sw[3 * RED_THREADS];
+    if (threadIdx.x == 0) stop = flags[0];
+    // the gather of this thread's node and its share of the delta / gamma sums are all
+    // requested before the block's one barrier
+    const int i0 = blockIdx.x * RED_THREADS;
+    const int i = i0 + threadIdx.x;
+    double w0 = 0.0, w1 = 0.0, w2 = 0.0;
+    if (i < n_nodes) {
+        for (int e = g_off[i]; e < g_off[i + 1]; e++) {
+            const double* pp = bpart + 3 * g_idx[e];
+            w0 += pp[0];
+            w1 += pp[1];
+            w2 += pp[2];
+        }
+    }
+    double dl = 0.0, gm = 0.0;
+    for (int t = threadIdx.x; t < nb; t += RED_THREADS) {
+        dl += dot[t];
+        gm += dot[nb + t];
+    }
+    dl = wave_sum(dl);
+    gm = wave_sum(gm);
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = dl;
+        red[1][threadIdx.x >> 6] = gm;
+    }
+    sw[3 * threadIdx.x] = w0;
+    sw[3 * threadIdx.x + 1] = w1;
+    sw[3 * threadIdx.x + 2] = w2;
+    __syncthreads();
+    if (stop) return;
+    double delta = 0.0, gamma = 0.0;
+#pragma unroll
+    for (int w = 0; w < RED_THREADS / 64; w++) {
+        delta += red[0][w];
+        gamma += red[1][w];
+    }
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    if (pKp <= 0.0) {
+    const double g0 = scal[0];
+    const double rel = g0 > 0.0 ? sqrt(gamma / g0) : sqrt(gamma);
+    if (k > 0 && rel < tol) {
         if (lead) {
-            scal[5] = pKp;
+            scal[4] = rel;
+            scal[1] = gamma;
+            flags[0] = 1;
+        }
+        return;
+    }
+    const int q = (k - 1) & 1;
+    const double beta = k == 0 ? 0.0 : gamma / scal[12 + q];
+    const double den = k == 0 ? delta : delta - beta * gamma / scal[14 + q];
+    if (den <= 0.0) {
+        if (lead) {
+            scal[5] = den;
             flags[1] = 1;
             flags[0] = 1;
         }
         return;
     }
-    const double alpha = scal[10 + par] / pKp;
+    const double alpha = gamma / den;
     if (lead) {
-        scal[5] = pKp;
+        scal[12 + (k & 1)] = gamma;
+        scal[14 + (k & 1)] = alpha;
+        scal[1] = gamma;
         scal[2] = alpha;
+        scal[3] = beta;
+        scal[4] = rel;
+        scal[5] = den;
+        flags[2] = k + 1;
     }
-    double s = 0.0;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n_nodes) {
-        double kp[3] = {0.0, 0.0, 0.0};
-        for (int k = g_off[i]; k < g_off[i + 1]; k++) {
-            const double* pp = bpart + 3 * g_idx[k];
-            kp[0] += pp[0];
-            kp[1] += pp[1];
-            kp[2] += pp[2];
-        }
+    // the block's 3 * RED_THREADS DOFs, contiguous per instruction
+    const int64_t j0 = 3 * (int64_t)i0, jn = 3 * (int64_t)n_nodes;
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-            const int64_t j = 3 * (int64_t)i + d;
-            const double pj = p[j];
-            const double v = (mask && mask[j]) ? pj : kp[d];
-            u[j] += alpha * pj;
-            const double rj = r[j] - alpha * v;
-            r[j] = rj;
-            s += rj * rj;
-        }
+    for (int t3 = 0; t3 < 3; t3++) {
+        const int t = threadIdx.x + t3 * RED_THREADS;
+        const int64_t j = j0 + t;
+        if (j >= jn) break;
+        const double rj = r[j];
+        const double wj = (mask && mask[j]) ? rj : sw[t];
+        const double pj = rj + beta * p[j];
+        const double sj = wj + beta * sv[j];
+        p[j] = pj;
+        sv[j] = sj;
+        u[j] += alpha * pj;
+        r[j] = rj - alpha * sj;
     }
-    s = block_sum(s);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// Relative residual of r against scal[0] and the stopping test (scal[4], flags[0]) after a
+// matrix-free CG that ran out of iterations: one block over the r.r partials.
+__global__ __launch_bounds__(RED_THREADS) void k_cg_tail_rel(const double* __restrict__ part, int n_part,
+                                                             double* __restrict__ scal, int* __restrict__ flags,
+                                                             double tol) {
+    const double rr = sum_parts(part, n_part);
+    if (threadIdx.x == 0 && !flags[1]) {
+        const double rel = scal[0] > 0.0 ? sqrt(rr / scal[0]) : sqrt(rr);
+        scal[4] = rel;
+        scal[1] = rr;
+        if (rel < tol) flags[0] = 1;
+    }
 }
 
 // Convergence test sqrt(rr'/rr0) < tol (LinearStaticSolver.cu:418-423), beta = rr'/rr and
@@ -1483,13 +1554,15 @@ extern "C" int tlfea_cg_solve(tlfea_cg* c, const double* k_sell, const double* f
 __global__ __launch_bounds__(RED_THREADS) void k_cg_init_masked(int64_t n, const double* __restrict__ f,
                                                                 const unsigned char* __restrict__ mask,
                                                                 double* __restrict__ u, double* __restrict__ r,
-                                                                double* __restrict__ p, double* __restrict__ part) {
+                                                                double* __restrict__ p, double* __restrict__ part,
+                                                                double* __restrict__ zero) {
     double s = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const double fi = (mask && mask[i]) ? 0.0 : f[i];
         u[i] = 0.0;
         r[i] = fi;
         p[i] = fi;
+        if (zero) zero[i] = 0.0;
         s += fi * fi;
     }
     s = block_sum(s);
@@ -1518,7 +1591,8 @@ extern "C" int tlfea_cg_solve_matfree(tlfea_cg* c, const tlfea_force_plan* plan,
     // the check uses the flag word before the initialisation resets it
     bool at_rest = false;
     if (int rc = positions_at_rest(plan->n_nodes, x, y, z, X0, Y0, Z0, c->d_flags, s, &at_rest)) return rc;
-    k_cg_init_masked<<<gv, RED_THREADS, 0, s>>>(n, f, fixed_dof_mask, u, c->d_r, c->d_p, c->d_part);
+    // s = K p of the single-reduction iteration starts at zero (p_0 = r_0 with beta = 0)
+    k_cg_init_masked<<<gv, RED_THREADS, 0, s>>>(n, f, fixed_dof_mask, u, c->d_r, c->d_p, c->d_part, c->d_Kp);
     k_cg_init_fin<<<1, RED_THREADS, 0, s>>>(gv, c->d_part, c->d_scal, c->d_flags);
     TLFEA_LAUNCH_CHECK();
     int flags[4] = {0, 0, 0, 0};
@@ -1527,22 +1601,28 @@ extern "C" int tlfea_cg_solve_matfree(tlfea_cg* c, const tlfea_force_plan* plan,
     TanMat<double> tm{mat->lambda, mat->mu, nullptr, nullptr, nullptr, nullptr, 0};
     tm.at_rest = at_rest ? 1 : 0;
     tm.affine = affine != 0 ? 1 : 0;
+    // two launches per iteration: the tangent applied to r (block partials, shares of r.Kr
+    // and r.r) and the single-reduction update (test, alpha, beta, gather, p, s, u, r);
+    // d_Kp holds s = K p
     for (int it = 0; it < max_iter; it++) {
-        // block partials and their p.Kp shares; the update sums both (no gather launch)
         int rc = launch_tangent_partials<double>(plan, x, y, z, X0, Y0, Z0, jinv_t, detJ_t, *quad, tm,
-                                                 fixed_dof_mask, c->d_p, c->d_flags, s, nullptr, 0, true);
+                                                 fixed_dof_mask, c->d_r, c->d_flags, s, nullptr, 0, true);
         if (rc != TLFEA_OK) return rc;
-        k_cg_update_gather<<<gu, RED_THREADS, 0, s>>>(plan->n_nodes, plan->d_g_off, plan->d_g_idx, plan->d_part,
-                                                      fixed_dof_mask, u, c->d_r, c->d_p, plan->d_dot, plan->n_blocks,
-                                                      c->d_part2, c->d_scal, c->d_flags, it & 1);
-        k_cg_dir<<<gv, RED_THREADS, 0, s>>>(n, c->d_p, c->d_r, c->d_part2, (int)gu, c->d_scal, c->d_flags, tol,
-                                            it & 1);
+        k_cgcg_update<<<gu, RED_THREADS, 0, s>>>(plan->n_nodes, plan->d_g_off, plan->d_g_idx, plan->d_part,
+                                                 fixed_dof_mask, u, c->d_r, c->d_p, c->d_Kp, plan->d_dot,
+                                                 plan->n_blocks, c->d_scal, c->d_flags, it, tol);
         TLFEA_LAUNCH_CHECK();
         if ((it + 1) % check_every == 0 || it + 1 == max_iter) {
             TLFEA_HIP_TRY(hipMemcpyAsync(flags, c->d_flags, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
             TLFEA_HIP_TRY(hipStreamSynchronize(s));
             if (flags[0]) break;
         }
+    }
+    if (!flags[0] && max_iter > 0) {
+        // out of iterations: the residual of the last update has not been tested yet
+        k_dot_partial<<<gv, RED_THREADS, 0, s>>>(n, c->d_r, c->d_r, c->d_part);
+        k_cg_tail_rel<<<1, RED_THREADS, 0, s>>>(c->d_part, (int)gv, c->d_scal, c->d_flags, tol);
+        TLFEA_LAUNCH_CHECK();
     }
     TLFEA_HIP_TRY(hipMemcpyAsync(flags, c->d_flags, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
     TLFEA_HIP_TRY(hipMemcpyAsync(scal, c->d_scal, sizeof(double) * 8, hipMemcpyDeviceToHost, s));

@@ -308,7 +308,7 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent(
     // With the block p.Kp shares, the masked DOFs' p^2 of the nodes this block owns is
     // summed here (sP is zero there, so the end of the kernel needs only sP . sF).
     constexpr int SPT = 4;
-    double mdot = 0.0;
+    double mdot = 0.0, pdot = 0.0;
     for (int base = threadIdx.x; base < U; base += SPT * PLAN_BLOCK) {
         int nid[SPT];
 #pragma unroll
@@ -348,8 +348,11 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent(
             for (int d = 0; d < 3; d++) sP[d * max_u + u] = mk[j][d] ? T(0) : pv[j][d];
             if (dot_part && own[u0 + u]) {
 #pragma unroll
-                for (int d = 0; d < 3; d++)
-                    if (mk[j][d]) mdot += (double)pv[j][d] * (double)pv[j][d];
+                for (int d = 0; d < 3; d++) {
+                    const double pp = (double)pv[j][d] * (double)pv[j][d];
+                    pdot += pp;
+                    if (mk[j][d]) mdot += pp;
+                }
             }
         }
     }
@@ -412,17 +415,27 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent(
             dot += (double)sP[u] * (double)f0 + (double)sP[max_u + u] * (double)f1 +
                    (double)sP[2 * max_u + u] * (double)f2;
     }
-    if (dot_part) {  // uniform over the block
-        __shared__ double red[PLAN_BLOCK / 64];
+    if (dot_part) {  // uniform over the block: shares of p . out and of p . p
+        __shared__ double red[2][PLAN_BLOCK / 64];
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) dot += __shfl_xor(dot, o, 64);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dot;
+        for (int o = 32; o > 0; o >>= 1) {
+            dot += __shfl_xor(dot, o, 64);
+            pdot += __shfl_xor(pdot, o, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            red[0][threadIdx.x >> 6] = dot;
+            red[1][threadIdx.x >> 6] = pdot;
+        }
         __syncthreads();
         if (threadIdx.x == 0) {
-            double s = 0.0;
+            double s = 0.0, t = 0.0;
 #pragma unroll
-            for (int w = 0; w < PLAN_BLOCK / 64; w++) s += red[w];
+            for (int w = 0; w < PLAN_BLOCK / 64; w++) {
+                s += red[0][w];
+                t += red[1][w];
+            }
             dot_part[b] = s;
+            dot_part[gridDim.x + b] = t;
         }
     }
 }
