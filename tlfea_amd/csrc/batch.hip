@@ -76,7 +76,8 @@ namespace tlfea_hip {
 
 constexpr int BT = 256;  // threads per chunk workgroup
 // per-instance scalars
-enum { S_FF = 0, S_RZ, S_RR, S_ALPHA, S_BETA, S_PQ, S_REL, S_RZ_OLD, NS };
+// (S_G0/S_G1, S_A0/S_A1: gamma and alpha of the single-reduction CG by iteration parity)
+enum { S_FF = 0, S_RZ, S_RR, S_ALPHA, S_BETA, S_PQ, S_REL, S_RZ_OLD, S_G0, S_G1, S_A0, S_A1, NS };
 // Chebyshev parameters per instance
 enum { C_LMAX = 0, C_THETA, C_DELTA, C_SIGMA, NC };
 // per-instance flags
@@ -231,60 +232,6 @@ __global__ __launch_bounds__(BT) void k_breduce_alpha(Insts in, const int* __res
     }
 }
 
-// u += alpha p, r -= alpha q; chunk partial of r.r
-__global__ __launch_bounds__(BT) void k_bupdate(Chunks ch, double* __restrict__ u, double* __restrict__ r,
-                                                const double* __restrict__ p, const double* __restrict__ q,
-                                                const double* __restrict__ S, const int* __restrict__ I,
-                                                double* __restrict__ cpart) {
-    const int c = chunk_id(ch);
-    const int inst = ch.inst[c];
-    if (I[inst * NI + I_DONE]) {
-        if (threadIdx.x == 0) cpart[c] = 0.0;
-        return;
-    }
-    const double alpha = S[(int64_t)inst * NS + S_ALPHA];
-    double rr = 0.0;
-    for (int64_t k = 3 * (int64_t)ch.beg[c] + threadIdx.x; k < 3 * (int64_t)ch.end[c]; k += BT) {
-        u[k] += alpha * p[k];
-        const double rk = r[k] - alpha * q[k];
-        r[k] = rk;
-        rr += rk * rk;
-    }
-    rr = chunk_sum(rr);
-    if (threadIdx.x == 0) cpart[c] = rr;
-}
-
-// per instance: stopping test on ||r|| / ||f|| and beta = rr' / rr (no preconditioner)
-__global__ __launch_bounds__(BT) void k_breduce_beta(Insts in, const int* __restrict__ ico,
-                                                     const double* __restrict__ cpart, double* __restrict__ S,
-                                                     int* __restrict__ I, int* __restrict__ all, double tol) {
-    int active = 0;
-    for (int j = threadIdx.x; j < in.n; j += BT) {
-        const int i = inst_at(in, j);
-        int* fl = I + (int64_t)i * NI;
-        if (fl[I_DONE]) continue;
-        double rr = 0.0;
-        for (int c = ico[i]; c < ico[i + 1]; c++) rr += cpart[c];
-        double* s = S + (int64_t)i * NS;
-        fl[I_ITERS] += 1;
-        s[S_RR] = rr;
-        s[S_REL] = sqrt(rr / s[S_FF]);
-        if (rr < tol * tol * s[S_FF]) {
-            fl[I_DONE] = 1;
-        } else {
-            s[S_BETA] = rr / s[S_RZ];
-            s[S_RZ] = rr;
-            active++;
-        }
-    }
-    __shared__ int cnt;
-    if (threadIdx.x == 0) cnt = 0;
-    __syncthreads();
-    atomicAdd(&cnt, active);
-    __syncthreads();
-    if (threadIdx.x == 0) all[0] = cnt;
-}
-
 // p = z + beta p
 __global__ __launch_bounds__(BT) void k_bdir(Chunks ch, double* __restrict__ p, const double* __restrict__ z,
                                              const double* __restrict__ S, const int* __restrict__ I) {
@@ -294,6 +241,149 @@ __global__ __launch_bounds__(BT) void k_bdir(Chunks ch, double* __restrict__ p, 
     const double beta = S[(int64_t)inst * NS + S_BETA];
     for (int64_t k = 3 * (int64_t)ch.beg[c] + threadIdx.x; k < 3 * (int64_t)ch.end[c]; k += BT)
         p[k] = beta != 0.0 ? z[k] + beta * p[k] : z[k];
+}
+
+// ---- single-reduction batched CG (Chronopoulos-Gear), the unpreconditioned solve --------
+// Per iteration: the tangent applied to r, k_bgather_rw (w = K r per node and the chunk
+// shares of r.w and r.r) and k_bcg_update, where every chunk sums its own instance's
+// shares in chunk order, tests ||r|| / ||f|| < tol (the rule of the reduction kernels
+// above) and forms beta = gamma / gamma_old, alpha = gamma / (delta - beta gamma / alpha_old)
+// before updating p, s = K p, u and r.  gamma_old / alpha_old sit in parity slots (the
+// instance's leading chunk writes slot par, the others read par ^ 1); both start at
+// (inf, 1), so the first iteration has beta = 0 and alpha = gamma / delta.
+
+// w = K r (r on masked DOFs); chunk shares of r.w and r.r; block 0 clears the active count
+__global__ __launch_bounds__(BT) void k_bgather_rw(Chunks ch, const int* __restrict__ g_off,
+                                                   const int64_t* __restrict__ g_idx, const double* __restrict__ part,
+                                                   const unsigned char* __restrict__ mask,
+                                                   const double* __restrict__ r, double* __restrict__ w,
+                                                   double* __restrict__ cpart, const int* __restrict__ iflags,
+                                                   int* __restrict__ all) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) all[0] = 0;
+    const int c = chunk_id(ch);
+    const int inst = ch.inst[c];
+    if (iflags[inst * NI + I_DONE]) {
+        if (threadIdx.x == 0) cpart[2 * c] = cpart[2 * c + 1] = 0.0;
+        return;
+    }
+    double rw = 0.0, rr = 0.0;
+    for (int i = ch.beg[c] + threadIdx.x; i < ch.end[c]; i += BT) {
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int k = g_off[i]; k < g_off[i + 1]; k++) {
+            const double* pp = part + 3 * g_idx[k];
+            s0 += pp[0];
+            s1 += pp[1];
+            s2 += pp[2];
+        }
+        const double sv[3] = {s0, s1, s2};
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double rk = r[3 * i + d];
+            const double v = (mask && mask[3 * i + d]) ? rk : sv[d];
+            w[3 * i + d] = v;
+            rw += rk * v;
+            rr += rk * rk;
+        }
+    }
+    rw = chunk_sum(rw);
+    rr = chunk_sum(rr);
+    if (threadIdx.x == 0) {
+        cpart[2 * c] = rw;
+        cpart[2 * c + 1] = rr;
+    }
+}
+
+__global__ __launch_bounds__(BT) void k_bcg_update(Chunks ch, const int* __restrict__ ico,
+                                                   const double* __restrict__ cpart, double* __restrict__ u,
+                                                   double* __restrict__ r, double* __restrict__ p,
+                                                   double* __restrict__ sv, const double* __restrict__ w,
+                                                   double* __restrict__ S, int* __restrict__ I,
+                                                   int* __restrict__ all, double tol, int par) {
+    const int c = chunk_id(ch);
+    const int inst = ch.inst[c];
+    int* fl = I + (int64_t)inst * NI;
+    if (fl[I_DONE]) return;
+    // the instance's sums, the same order in every thread of its chunks
+    double delta = 0.0, gamma = 0.0;
+    for (int k = ico[inst]; k < ico[inst + 1]; k++) {
+        delta += cpart[2 * k];
+        gamma += cpart[2 * k + 1];
+    }
+    double* s = S + (int64_t)inst * NS;
+    const bool lead = c == ico[inst] && threadIdx.x == 0;
+    const double ff = s[S_FF];
+    if (gamma < tol * tol * ff) {
+        if (lead) {
+            s[S_RR] = gamma;
+            s[S_REL] = sqrt(gamma / ff);
+            fl[I_DONE] = 1;
+        }
+        return;
+    }
+    const double beta = gamma / s[S_G0 + (par ^ 1)];
+    const double den = delta - beta * gamma / s[S_A0 + (par ^ 1)];
+    if (den <= 0.0) {  // p.Kp <= 0 (LinearStaticSolver.cu:404-407)
+        if (lead) {
+            s[S_PQ] = den;
+            fl[I_NOTPD] = 1;
+            fl[I_DONE] = 1;
+        }
+        return;
+    }
+    const double alpha = gamma / den;
+    if (lead) {
+        s[S_G0 + par] = gamma;
+        s[S_A0 + par] = alpha;
+        s[S_RR] = gamma;
+        s[S_REL] = sqrt(gamma / ff);
+        s[S_ALPHA] = alpha;
+        s[S_BETA] = beta;
+        s[S_PQ] = den;
+        fl[I_ITERS] += 1;
+        atomicAdd(all, 1);
+    }
+    for (int64_t k = 3 * (int64_t)ch.beg[c] + threadIdx.x; k < 3 * (int64_t)ch.end[c]; k += BT) {
+        const double rk = r[k];
+        const double pk = rk + beta * p[k];
+        const double sk = w[k] + beta * sv[k];
+        p[k] = pk;
+        sv[k] = sk;
+        u[k] += alpha * pk;
+        r[k] = rk - alpha * sk;
+    }
+}
+
+// parity slots of a fresh solve
+__global__ void k_bcg_init_slots(int n_inst, double* __restrict__ S) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_inst) return;
+    double* s = S + (int64_t)i * NS;
+    s[S_G0] = s[S_G1] = INFINITY;
+    s[S_A0] = s[S_A1] = 1.0;
+}
+
+// chunk partials of r.r (the test after a solve that ran out of iterations)
+__global__ __launch_bounds__(BT) void k_bdot_rr(Chunks ch, const double* __restrict__ r, double* __restrict__ cpart) {
+    const int c = chunk_id(ch);
+    double rr = 0.0;
+    for (int64_t k = 3 * (int64_t)ch.beg[c] + threadIdx.x; k < 3 * (int64_t)ch.end[c]; k += BT) rr += r[k] * r[k];
+    rr = chunk_sum(rr);
+    if (threadIdx.x == 0) cpart[c] = rr;
+}
+
+__global__ __launch_bounds__(BT) void k_breduce_tail(int n_inst, const int* __restrict__ ico,
+                                                     const double* __restrict__ cpart, double* __restrict__ S,
+                                                     int* __restrict__ I, double tol) {
+    for (int i = threadIdx.x; i < n_inst; i += BT) {
+        int* fl = I + (int64_t)i * NI;
+        if (fl[I_DONE]) continue;
+        double rr = 0.0;
+        for (int c = ico[i]; c < ico[i + 1]; c++) rr += cpart[c];
+        double* s = S + (int64_t)i * NS;
+        s[S_RR] = rr;
+        s[S_REL] = sqrt(rr / s[S_FF]);
+        if (rr < tol * tol * s[S_FF]) fl[I_DONE] = 1;
+    }
 }
 
 // ---- FP32 Jacobi-Chebyshev preconditioner ----------------------------------------------
@@ -738,14 +828,23 @@ int pcg_iteration(tlfea_batch* b, double* u, double tol, hipStream_t s) {
     return TLFEA_OK;
 }
 
-// One CG iteration for every active instance (6 launches, no host synchronisation).
-int cg_iteration(tlfea_batch* b, double* u, double tol, hipStream_t s) {
+// One single-reduction CG iteration for every active instance (3 launches, no host
+// synchronisation); par = parity of the iteration index.  d_q holds w = K r, d_z s = K p.
+int cg_iteration(tlfea_batch* b, double* u, double tol, hipStream_t s, int par) {
     const Chunks ch = chunks(b);
-    if (int rc = apply64(b, b->d_p, b->d_q, b->d_i, s)) return rc;
-    k_breduce_alpha<<<1, BT, 0, s>>>(insts(b), b->d_inst_chunk_off, b->d_part, b->d_s, b->d_i);
-    k_bupdate<<<n_chunk_launch(b), BT, 0, s>>>(ch, u, b->d_r, b->d_p, b->d_q, b->d_s, b->d_i, b->d_part);
-    k_breduce_beta<<<1, BT, 0, s>>>(insts(b), b->d_inst_chunk_off, b->d_part, b->d_s, b->d_i, b->d_all, tol);
-    k_bdir<<<n_chunk_launch(b), BT, 0, s>>>(ch, b->d_p, b->d_r, b->d_s, b->d_i);
+    TanMat<double> tm{0.0, 0.0, b->d_elem_inst, b->d_lam, b->d_mu, b->d_i, NI};
+    tm.inst_mc = b->mass_on ? b->d_mc : nullptr;
+    tm.mass_exact = b->mass_exact;
+    tm.at_rest = b->moved ? 0 : 1;
+    tm.affine = b->affine;
+    if (int rc = launch_tangent_partials<double>(b->plan, pos_x(b), pos_y(b), pos_z(b), rest_x(b), rest_y(b),
+                                                 rest_z(b), b->d_jinv, b->d_detJ, b->quad, tm, b->d_mask, b->d_r,
+                                                 nullptr, s, b->compact ? b->d_blk_list : nullptr, b->n_blk_list))
+        return rc;
+    k_bgather_rw<<<n_chunk_launch(b), BT, 0, s>>>(ch, b->plan->d_g_off, b->plan->d_g_idx, b->plan->d_part, b->d_mask,
+                                                 b->d_r, b->d_q, b->d_part, b->d_i, b->d_all);
+    k_bcg_update<<<n_chunk_launch(b), BT, 0, s>>>(ch, b->d_inst_chunk_off, b->d_part, u, b->d_r, b->d_p, b->d_z,
+                                                 b->d_q, b->d_s, b->d_i, b->d_all, tol, par);
     TLFEA_LAUNCH_CHECK();
     return TLFEA_OK;
 }
@@ -1081,6 +1180,9 @@ extern "C" int tlfea_batch_cg_solve(tlfea_batch* b, const double* f, double* u, 
     TLFEA_REQUIRE(precond == 0 || b->cheb_degree > 0, "preconditioner requested but not set up");
     TLFEA_REQUIRE(max_iter >= 0 && tol >= 0.0, "bad tolerance or iteration limit");
     if (poll <= 0) poll = b->poll_default;
+    // the single-reduction iteration keys its parity slots to the iteration index, which
+    // a replayed graph of an even number of iterations keeps aligned
+    if (!precond && (poll & 1)) poll += 1;
     hipStream_t caller = as_stream(stream);
     hipStream_t s = b->s_own;
     const double compact_ratio = b->compact_ratio;
@@ -1102,7 +1204,10 @@ extern "C" int tlfea_batch_cg_solve(tlfea_batch* b, const double* f, double* u, 
         k_breduce_rz0<<<1, BT, 0, s>>>(insts(b), b->d_inst_chunk_off, b->d_part, b->d_s, b->d_i);
         k_bdir<<<n_chunk_launch(b), BT, 0, s>>>(ch, b->d_p, b->d_z, b->d_s, b->d_i);
     } else {
-        TLFEA_HIP_TRY(hipMemcpyAsync(b->d_p, b->d_r, sizeof(double) * nd, hipMemcpyDeviceToDevice, s));
+        // p = s = 0 and fresh parity slots: the first iteration then takes p = r
+        TLFEA_HIP_TRY(hipMemsetAsync(b->d_p, 0, sizeof(double) * nd, s));
+        TLFEA_HIP_TRY(hipMemsetAsync(b->d_z, 0, sizeof(double) * nd, s));
+        k_bcg_init_slots<<<grid_for(b->n_inst, 256), 256, 0, s>>>(b->n_inst, b->d_s);
     }
     TLFEA_LAUNCH_CHECK();
     int active = 0;
@@ -1110,7 +1215,9 @@ extern "C" int tlfea_batch_cg_solve(tlfea_batch* b, const double* f, double* u, 
     TLFEA_HIP_TRY(hipStreamSynchronize(s));
     int it = 0;
     std::vector<int> hflags((size_t)b->n_inst * NI), done(b->n_inst, 0);
-    auto iteration = [&](hipStream_t st) { return precond ? pcg_iteration(b, u, tol, st) : cg_iteration(b, u, tol, st); };
+    auto iteration = [&](hipStream_t st, int k) {
+        return precond ? pcg_iteration(b, u, tol, st) : cg_iteration(b, u, tol, st, k & 1);
+    };
     if (use_graph && (b->graph == nullptr || b->graph_iters != poll || b->graph_precond != precond)) {
         if (b->graph) (void)hipGraphExecDestroy(b->graph);
         b->graph = nullptr;
@@ -1122,7 +1229,7 @@ extern "C" int tlfea_batch_cg_solve(tlfea_batch* b, const double* f, double* u, 
                 hipGraph_t g;
                 TLFEA_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
                 int rc = TLFEA_OK;
-                for (int k = 0; k < poll && rc == TLFEA_OK; k++) rc = iteration(s);
+                for (int k = 0; k < poll && rc == TLFEA_OK; k++) rc = iteration(s, k);  // it % poll == 0
                 const hipError_t ec = hipStreamEndCapture(s, &g);
                 if (rc) return rc;
                 if (ec != hipSuccess) {
@@ -1142,7 +1249,7 @@ extern "C" int tlfea_batch_cg_solve(tlfea_batch* b, const double* f, double* u, 
             TLFEA_HIP_TRY(hipGraphLaunch(b->graph, s));
         } else {
             for (int k = 0; k < n; k++)
-                if (int rc = iteration(s)) return rc;
+                if (int rc = iteration(s, it + k)) return rc;
         }
         it += n;
         TLFEA_HIP_TRY(hipMemcpyAsync(&active, b->d_all, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1160,6 +1267,12 @@ extern "C" int tlfea_batch_cg_solve(tlfea_batch* b, const double* f, double* u, 
         }
     }
     full_lists(b);
+    if (!precond && active > 0) {
+        // out of iterations: the residual of the last update has not been tested yet
+        k_bdot_rr<<<b->n_chunks, BT, 0, s>>>(chunks(b), b->d_r, b->d_part);
+        k_breduce_tail<<<1, BT, 0, s>>>(b->n_inst, b->d_inst_chunk_off, b->d_part, b->d_s, b->d_i, tol);
+        TLFEA_LAUNCH_CHECK();
+    }
     // the graph bakes in u: a later solve with another output buffer must re-capture
     if (use_graph && b->graph) {
         (void)hipGraphExecDestroy(b->graph);
