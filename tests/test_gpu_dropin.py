@@ -26,15 +26,18 @@ def _run(args, tmp_path):
     return info, pos
 
 
+@pytest.mark.parametrize("assembled", [False, True])
 @pytest.mark.parametrize("spec,name,load", [("kuhn:20,2,2", "C1", -1000.0), ("kuhn4:12,3,2", "kuhn_t4", -500.0),
                                              ("vtu:" + os.path.join(MESHES, "beam.vtu"), "beam_vtu", -1000.0)])
-def test_linear_static_dropin(gpu, tmp_path, spec, name, load):
+def test_linear_static_dropin(gpu, tmp_path, spec, name, load, assembled):
     """examples/beam_linear_static.cc: clamp x = x_min, the load spread over the x = x_max
     nodes in -z, CG at tol 1e-12 -> displacements against the oracle's banded Cholesky solve
-    within 1e-10 (kappa * eps allows it on these meshes)."""
+    within 1e-10 (kappa * eps allows it on these meshes), with the matrix-free operator (the
+    default for SVK without damping) and with the assembled one."""
     tol = 1e-10
-    info, pos = _run(["linear_static", "--mesh", spec, "--tol", "1e-12", "--max-iter", "200000", "--load", str(load)],
-                     tmp_path)
+    info, pos = _run(["linear_static", "--mesh", spec, "--tol", "1e-12", "--max-iter", "200000", "--load", str(load),
+                      "--assembled", "1" if assembled else "0"], tmp_path)
+    assert info["matrix_free"] is (not assembled)
     m = load_mesh(name)
     lam, mu = lame(2.1e11, 0.3)
     g, dj = O.grad(m)

@@ -212,3 +212,4 @@ def test_mr_dropin_linear_static(gpu, tmp_path):
     u = O.band_cholesky_solve(off, col, vals, fb).reshape(-1, 3)
     assert rel_err(pos.T - m.xyz(), u) < 1e-10
     assert info["residual"] < 1e-12
+    assert info["matrix_free"] is False  # the assembled path carries the Mooney-Rivlin tangent

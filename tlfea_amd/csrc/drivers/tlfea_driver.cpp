@@ -5,7 +5,8 @@
 // raw float64 (x block, y block, z block) to --out, plus a JSON summary on stdout.
 //
 //   tlfea_driver linear_static  --mesh kuhn:20,2,2 | kuhn4:12,3,2 | vtu:PATH | tetgen:PREFIX
-//                 [--E 2.1e11 --nu 0.3 --load -1000 --tol 1e-10 --max-iter 50000] --out FILE
+//                 [--E 2.1e11 --nu 0.3 --load -1000 --tol 1e-10 --max-iter 50000]
+//                 [--assembled 1: assemble K as the reference does] --out FILE
 //   tlfea_driver nesterov|adamw|adamw_nocoop --mesh tetgen:PREFIX|kuhn:.. [--steps 3] [--E ..]
 //                 [--rho0 ..] [--fix z0|x0] [--node-load NODE,FX] --out FILE
 //   any mode: [--mr MU10,MU01,KAPPA] switches the element to Mooney-Rivlin (SetMooneyRivlin)
@@ -32,6 +33,12 @@ std::map<std::string, std::string> parse(int argc, char** argv) {
     std::map<std::string, std::string> a;
     for (int i = 2; i + 1 < argc; i += 2) a[argv[i]] = argv[i + 1];
     return a;
+}
+// a double in JSON without losing small magnitudes (std::to_string prints %f)
+std::string sci(double v) {
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), "%.6e", v);
+    return buf;
 }
 double num(std::map<std::string, std::string>& a, const char* k, double d) {
     return a.count(k) ? std::atof(a[k].c_str()) : d;
@@ -121,10 +128,12 @@ int run(const std::string& mode, std::map<std::string, std::string>& a, MatrixXR
     std::string extra;
     if (mode == "linear_static") {
         LinearStaticSolver<TData> solver(b, num(a, "--tol", 1e-10), (int)num(a, "--max-iter", 50000));
+        solver.UseAssembledOperator(num(a, "--assembled", 0) != 0);
         fea.AddSolver("linear_static", &solver);
         fea.GetSolver("linear_static")->Solve();
         extra = ", \"iters\": " + std::to_string(solver.GetLastIterCount()) +
-                ", \"residual\": " + std::to_string(solver.GetLastResidual());
+                ", \"residual\": " + sci(solver.GetLastResidual()) +
+                ", \"matrix_free\": " + (solver.LastSolveWasMatrixFree() ? "true" : "false");
     } else {
         b->CalcMassMatrix();
         b->CalcConstraintData();

@@ -119,6 +119,8 @@ LinearStaticSolver<TData>::~LinearStaticSolver() {
     if (k_) hipFree(k_);
     if (f_) hipFree(f_);
     if (u_) hipFree(u_);
+    if (jinv_) hipFree(jinv_);
+    if (mask_) hipFree(mask_);
 }
 
 template <typename TData>
@@ -133,17 +135,40 @@ void LinearStaticSolver<TData>::Solve() {
             hipMalloc((void**)&u_, sizeof(double) * (size_t)n_dof_) != hipSuccess)
             throw std::runtime_error("LinearStaticSolver: device allocation failed");
     }
-    // assemble the tangent at the current positions (h = 1), copy f_ext, eliminate fixed DOFs
-    lib_check(tlfea_stiffness_assemble(b.pattern, b.nn, b.n_elem, b.conn, b.x, b.y, b.z, b.grad_t, b.detJ_t, &b.quad,
-                                       &b.mat, 1.0, k_, nullptr),
-              "AssembleLinearStiffness");
     if (hipMemcpy(f_, b.f_ext, sizeof(double) * n_dof_, hipMemcpyDeviceToDevice) != hipSuccess)
         throw std::runtime_error("LinearStaticSolver: copy of f_ext failed");
-    if (b.n_fixed > 0)
-        lib_check(tlfea_apply_dirichlet(b.pattern, b.n_fixed, b.fixed_nodes, k_, f_, nullptr), "ApplyDirichletBCs");
     int iters = 0;
     double res = 0.0;
-    const int rc = tlfea_cg_solve(cg_, k_, f_, u_, cg_tol_, cg_max_iter_, 32, &iters, &res, nullptr);
+    int rc = TLFEA_OK;
+    // K(x) applied without assembly: SVK, no Kelvin-Voigt block, rest positions known
+    const bool matfree = !assembled_only_ && b.mat.model == 0 && b.mat.eta_damp == 0.0 &&
+                         b.mat.lambda_damp == 0.0 && b.force_plan && b.X0;
+    last_matfree_ = matfree;
+    if (matfree) {
+        if (!jinv_) {
+            const size_t nq = (size_t)b.quad.n;
+            if (hipMalloc((void**)&jinv_, sizeof(double) * 9 * nq * (size_t)tlfea_padded_elems(b.n_elem)) != hipSuccess ||
+                hipMalloc((void**)&mask_, (size_t)n_dof_) != hipSuccess)
+                throw std::runtime_error("LinearStaticSolver: device allocation failed");
+            lib_check(tlfea_precompute_jinv(b.nn, b.n_elem, b.conn, b.X0, b.Y0, b.Z0, &b.quad, jinv_, nullptr),
+                      "LinearStaticSolver");
+            lib_check(tlfea_jinv_affine(b.nn, b.n_elem, jinv_, b.detJ_t, &affine_, nullptr), "LinearStaticSolver");
+        }
+        // the fixed set may have changed since the last Solve (UpdateNodalFixed)
+        lib_check(tlfea_fixed_dof_mask(b.n_nodes, b.n_fixed, b.n_fixed > 0 ? b.fixed_nodes : nullptr, mask_, nullptr),
+                  "LinearStaticSolver");
+        rc = tlfea_cg_solve_matfree(cg_, b.force_plan, b.x, b.y, b.z, b.X0, b.Y0, b.Z0, jinv_, b.detJ_t, affine_,
+                                    &b.quad, &b.mat, mask_, f_, u_, cg_tol_, cg_max_iter_, 32, &iters, &res, nullptr);
+    } else {
+        // assemble the tangent at the current positions (h = 1), eliminate fixed DOFs
+        lib_check(tlfea_stiffness_assemble(b.pattern, b.nn, b.n_elem, b.conn, b.x, b.y, b.z, b.grad_t, b.detJ_t,
+                                           &b.quad, &b.mat, 1.0, k_, nullptr),
+                  "AssembleLinearStiffness");
+        if (b.n_fixed > 0)
+            lib_check(tlfea_apply_dirichlet(b.pattern, b.n_fixed, b.fixed_nodes, k_, f_, nullptr),
+                      "ApplyDirichletBCs");
+        rc = tlfea_cg_solve(cg_, k_, f_, u_, cg_tol_, cg_max_iter_, 32, &iters, &res, nullptr);
+    }
     if (rc == TLFEA_ENOTPD) {
         std::fprintf(stderr, "[TLFEA ERROR] LinearStaticSolver CG: non-positive p*Kp detected; aborting.\n");
     } else if (rc != TLFEA_OK && rc != TLFEA_ENOCONV) {
