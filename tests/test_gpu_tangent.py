@@ -268,6 +268,29 @@ def test_matfree_cg_against_direct(gpu, name):
     assert rel_err(_np(ls8.u), uo) < 1e-6
 
 
+def test_matfree_cg_iteration_limits(gpu):
+    """Out of iterations the matrix-free CG reports ENOCONV with the relative residual of the
+    returned u (tested after the loop); a tolerance just above that residual with the same
+    limit converges (at the last allowed iteration at the latest)."""
+    import torch
+    from tlfea_amd import capi
+    from tlfea_amd.fea import LinearStatic
+    m = load_mesh("C1")
+    b = _body(m)
+    ls = LinearStatic(b, tol=1e-10, max_iter=7, matfree=True)
+    ls.assemble()
+    assert ls.cg_solve() == capi.TLFEA_ENOCONV and ls.iters == 7
+    go, djo = O.grad(m)
+    lam, mu = lame(E_STEEL, NU_STEEL)
+    doff, dcol = O.dof_pattern(m)
+    vo, fo = O.apply_dirichlet(doff, dcol, O.stiffness_values(m, go, djo, lam, mu, doff, dcol), m.f_ext, m.fixed)
+    true = np.linalg.norm(fo - O.spmv(doff, dcol, vo, _np(ls.u))) / np.linalg.norm(fo)
+    assert abs(ls.res - true) < 1e-8 * true, (ls.res, true)
+    edge = LinearStatic(b, tol=1.01 * ls.res, max_iter=7, matfree=True)
+    edge.assemble()
+    assert edge.cg_solve() == 0 and edge.iters <= 7, (edge.iters, edge.res)
+
+
 def test_matfree_solve_moves_positions(gpu):
     """LinearStaticSolver's beam scenario (tol 1e-10) through the matrix-free operator: the
     tip deflection sits in the Euler-Bernoulli band (beam_linear_static.cc:223-234)."""
