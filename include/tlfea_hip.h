@@ -325,6 +325,10 @@ int tlfea_batch_set_displacement(tlfea_batch* b, const double* ux, const double*
  * instance's material; gradients rebuilt from the rest positions as in
  * tlfea_internal_force_rest), all DOFs (no Dirichlet masking), overwritten. */
 int tlfea_batch_internal_force(tlfea_batch* b, double* f, void* stream);
+/* out[i] = a . c over instance i's DOFs (device double[n_inst]; all DOFs, no masking), by
+ * chunk partials summed per instance in chunk order (deterministic).  Per-instance norms
+ * for the Newton iterations of implicit steps.  Not in the reference. */
+int tlfea_batch_dot(tlfea_batch* b, const double* a, const double* c, double* out, void* stream);
 /* FP32 Jacobi-Chebyshev preconditioner for tlfea_batch_cg_solve(precond = 1): the
  * diagonal D of K (FP64, matrix-free), lambda_max(D^-1 K) per instance by `power_iters`
  * FP32 power iterations (scaled by `safety` >= 1), and `degree` Chebyshev steps on

@@ -190,6 +190,25 @@ def test_batch_guess_zero_load_and_limits(gpu):
     assert rel_err(_np(u64), _np(u)) < 1e-7
 
 
+def test_batch_dot(gpu):
+    """Per-instance dot products over each instance's DOFs equal numpy's, and repeat
+    exactly (fixed summation order)."""
+    import torch
+    from tlfea_amd.batch import Batch
+    items = _items(8, 5)
+    b = Batch(items)
+    rng = np.random.default_rng(9)
+    a = rng.standard_normal(3 * b.n_nodes)
+    c = rng.standard_normal(3 * b.n_nodes)
+    at, ct = torch.as_tensor(a).cuda(), torch.as_tensor(c).cuda()
+    d = _np(b.dot(at, ct))
+    ref = np.array([a[3 * b.node_off[i]:3 * b.node_off[i + 1]] @ c[3 * b.node_off[i]:3 * b.node_off[i + 1]]
+                    for i in range(len(items))])
+    assert np.abs(d - ref).max() < 1e-12 * np.abs(ref).max()
+    n2 = _np(b.dot(at))
+    assert np.all(n2 > 0) and np.array_equal(n2, _np(b.dot(at, at)))
+
+
 def test_batch_fp32_chebyshev_preconditioner(gpu):
     """Flexible FP64 CG with the FP32 Jacobi-Chebyshev stage: same solution as the plain
     batched CG (both at tol 1e-10), true FP64 residual within the floor bar, and several

@@ -141,6 +141,12 @@ class Batch:
         call("tlfea_batch_internal_force", self.h, ptr(out), self.stream)
         return out
 
+    def dot(self, a, c=None):
+        """Per-instance a . c (c = a: squared norms) over all DOFs, a [n_inst] tensor."""
+        out = torch.empty(self.n_inst, dtype=torch.float64, device=DEV)
+        call("tlfea_batch_dot", self.h, ptr(a), ptr(a if c is None else c), ptr(out), self.stream)
+        return out
+
     def precond_setup(self, degree=6, ratio=30.0, power_iters=20, safety=1.1):
         """FP32 Jacobi-Chebyshev preconditioner; returns the lambda_max(D^-1 K) estimates."""
         lm = np.zeros(self.n_inst)

@@ -140,6 +140,7 @@ _SIGS = {
     "tlfea_batch_apply": (i32, [vp, f64, i32, i32, vp, vp, vp]),
     "tlfea_batch_set_displacement": (i32, [vp, vp, vp, vp, vp]),
     "tlfea_batch_internal_force": (i32, [vp, vp, vp]),
+    "tlfea_batch_dot": (i32, [vp, vp, vp, vp, vp]),
     "tlfea_batch_precond_setup": (i32, [vp, i32, f64, i32, f64, vp, vp]),
     "tlfea_mpnn_create": (i32, [i32, C.POINTER(vp)]),
     "tlfea_mpnn_destroy": (i32, [vp]),

@@ -353,6 +353,26 @@ __global__ __launch_bounds__(BT) void k_bcg_update(Chunks ch, const int* __restr
     }
 }
 
+// chunk partials of a.c over every DOF of the chunk
+__global__ __launch_bounds__(BT) void k_bdot_ac(Chunks ch, const double* __restrict__ a, const double* __restrict__ c,
+                                                double* __restrict__ cpart) {
+    const int k = chunk_id(ch);
+    double s = 0.0;
+    for (int64_t j = 3 * (int64_t)ch.beg[k] + threadIdx.x; j < 3 * (int64_t)ch.end[k]; j += BT) s += a[j] * c[j];
+    s = chunk_sum(s);
+    if (threadIdx.x == 0) cpart[k] = s;
+}
+
+// per instance: the sum of its chunk partials in chunk order
+__global__ void k_binst_sum(int n_inst, const int* __restrict__ ico, const double* __restrict__ cpart,
+                            double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_inst) return;
+    double s = 0.0;
+    for (int c = ico[i]; c < ico[i + 1]; c++) s += cpart[c];
+    out[i] = s;
+}
+
 // parity slots of a fresh solve
 __global__ void k_bcg_init_slots(int n_inst, double* __restrict__ S) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1156,6 +1176,16 @@ extern "C" int tlfea_batch_internal_force(tlfea_batch* b, double* f, void* strea
     const tlfea_material none{0.0, 0.0, 0.0, 0.0};
     return launch_force_rest(b->plan, pos_x(b), pos_y(b), pos_z(b), b->d_X, b->d_Y, b->d_Z, b->quad, none, nullptr, f,
                              0, nullptr, as_stream(stream), b->d_elem_inst, b->d_lam, b->d_mu, b->moved);
+}
+
+extern "C" int tlfea_batch_dot(tlfea_batch* b, const double* a, const double* c, double* out, void* stream) {
+    TLFEA_REQUIRE(b && a && c && out, "null argument");
+    hipStream_t s = as_stream(stream);
+    const Chunks all{b->d_chunk_inst, b->d_chunk_beg, b->d_chunk_end, nullptr};
+    k_bdot_ac<<<b->n_chunks, BT, 0, s>>>(all, a, c, b->d_part);
+    k_binst_sum<<<grid_for(b->n_inst, 256), 256, 0, s>>>(b->n_inst, b->d_inst_chunk_off, b->d_part, out);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
 }
 
 extern "C" int tlfea_batch_apply(tlfea_batch* b, double k_scale, int with_mass, int masked, const double* p,

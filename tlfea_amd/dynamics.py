@@ -93,9 +93,7 @@ class BatchDynamics:
         return self.fg + self.tip * (self.amp * torch.sin(self.omega * t))
 
     def _norms(self, g):
-        sq = torch.zeros(self.b.n_inst, dtype=torch.float64, device=DEV)
-        sq.index_add_(0, self.inst, g * g)
-        return torch.sqrt(sq)
+        return torch.sqrt(self.b.dot(g))
 
     def _residual(self, v, f1):
         """g(v) on the free DOFs, with the batch moved to x = X + u_n + dt v."""
