@@ -8,17 +8,19 @@ rho = 7850; nodes at x = 0 fixed; 1e5 N in -z spread over the x = L nodes.
 One step = one linear-static solve to a relative residual of 1e-8 (the reference's
 LinearStaticSolver::Solve: tangent K(x) at the current positions, Dirichlet elimination,
 CG with the stopping rule sqrt(rr/rr0) < tol, position update).  `value` = solves/s of the
-matrix-free FP64 path: CG whose operator K(x) p is applied element by element from the
-shape-function gradients (nothing assembled); the plain FP64 path (the drop-in: block
-assembly + Dirichlet + CG on the assembled matrix) is timed on the same problem and
-reported beside it.  Inputs are resident in HBM when the timed region starts.
+matrix-free FP64 path: CG (in its single-reduction arrangement: two launches per
+iteration) whose operator K(x) p is applied element by element from the inverse Jacobians
+(nothing assembled); the plain FP64 path (block assembly + Dirichlet + CG on the
+assembled matrix, the reference's arithmetic) is timed on the same problem and reported
+beside it.  Inputs are resident in HBM when the timed region starts.
 
 Kernel rooflines are measured live with HIP events on the launch stream (the null stream,
 which torch.cuda.Event records on):
   * roofline      -- the step's dominant kernel pair, the matrix-free tangent at the rest
                      positions (k_tangent<10,5,double,..,LIN,AFF> + k_gather_tangent: the
                      kernel's linear-elasticity form, chosen when x equals X0, on an affine
-                     mesh), bytes/launch = 120*n_elem + 48*n_nodes (inverse Jacobian 72 B
+                     mesh, integrated from the corner values; inside the CG the gather is
+                     part of the update kernel), bytes/launch = 120*n_elem + 48*n_nodes (inverse Jacobian 72 B
                      and detJ 8 B of one quadrature point -- the beam's elements are
                      straight-sided, so all five agree -- and connectivity 40 B per
                      element; p read and y written, 48 B per node);
@@ -421,7 +423,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": tan_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": tan_gbs / HBM_PEAK_GBS, "traffic": traffic_tan,
                      "kernel": "k_tangent<10,5,double,true,0,true,true> + k_gather_tangent (matrix-free K p at the "
-                               "rest positions, the linear-elasticity form, one quadrature point's geometry)",
+                               "rest positions, the linear-elasticity form of an affine mesh integrated from the "
+                               "corner values)",
                      "avg_us": tan_s * 1e6, "bytes_per_launch": tan_bytes},
         "tangent_general": {"bound": "hbm", "achieved": tgen_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": tgen_gbs / HBM_PEAK_GBS, "traffic": PMC_TANGENT_BYTES if default else None,
