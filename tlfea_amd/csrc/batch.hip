@@ -303,12 +303,15 @@ __global__ __launch_bounds__(BT) void k_bcg_update(Chunks ch, const int* __restr
     const int inst = ch.inst[c];
     int* fl = I + (int64_t)inst * NI;
     if (fl[I_DONE]) return;
-    // the instance's sums, the same order in every thread of its chunks
+    // the instance's sums: strided over the block then reduced, the same order in every
+    // chunk block of the instance
     double delta = 0.0, gamma = 0.0;
-    for (int k = ico[inst]; k < ico[inst + 1]; k++) {
+    for (int k = ico[inst] + threadIdx.x; k < ico[inst + 1]; k += BT) {
         delta += cpart[2 * k];
         gamma += cpart[2 * k + 1];
     }
+    delta = chunk_sum(delta);
+    gamma = chunk_sum(gamma);
     double* s = S + (int64_t)inst * NS;
     const bool lead = c == ico[inst] && threadIdx.x == 0;
     const double ff = s[S_FF];
@@ -948,7 +951,7 @@ extern "C" int tlfea_batch_create(int nn, int n_inst, const int* elem_off, const
     if ((rc = tlfea_force_plan_create(nn, n_elem, n_nodes, b->d_conn, &b->plan, stream))) return fail(rc);
     if ((rc = dalloc(&b->d_blk_list, b->plan->n_blocks)) || (rc = dalloc(&b->d_inst_list, n_inst))) return fail(rc);
     b->launched_elems = n_elem;
-    if ((rc = build_chunks(b, 1024))) return fail(rc);
+    if ((rc = build_chunks(b, 512))) return fail(rc);
     if (hipStreamCreateWithFlags(&b->s_own, hipStreamNonBlocking) != hipSuccess) {
         set_error("stream creation failed");
         return fail(TLFEA_EHIP);
