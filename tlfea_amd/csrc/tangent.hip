@@ -277,9 +277,11 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent(
     double* __restrict__ dot_part, const unsigned char* __restrict__ own) {
     if (flags && flags[0]) return;
     extern __shared__ unsigned char lds_raw[];
-    T* sU = reinterpret_cast<T*>(lds_raw);  // [3][max_u] displacements
-    T* sF = sU + 3 * max_u;                 // [3][max_u] accumulated output
-    T* sP = sU + 6 * max_u;                 // [3][max_u] direction (masked)
+    // [3][max_u] each: displacements (not in the rest-state form, whose launch reserves
+    // 6 arrays instead of 9), accumulated output, direction (masked)
+    T* sU = reinterpret_cast<T*>(lds_raw);
+    T* sF = LIN ? sU : sU + 3 * max_u;
+    T* sP = sF + 3 * max_u;
     const int b = blk_list ? blk_list[blockIdx.x] : blockIdx.x;
     const int64_t u0 = blk_off[b];
     const int U = (int)(blk_off[b + 1] - u0);
@@ -523,7 +525,9 @@ int launch_tangent_partials(const tlfea_force_plan* pl, const double* x, const d
             for (int k = 0; k < 3; k++) dn.d[q][a][k] = (T)dn64.d[q][a][k];
             dn.N[q][a] = (T)dn64.N[q][a];
         }
-    const size_t lds = sizeof(T) * (size_t)pl->max_u * 9;
+    // 9 arrays of max_u values (displacements, output, direction); 6 in the rest-state
+    // form, which stages no displacement (more blocks per CU where LDS is the limit)
+    const size_t lds9 = sizeof(T) * (size_t)pl->max_u * 9, lds6 = sizeof(T) * (size_t)pl->max_u * 6;
     T* part = reinterpret_cast<T*>(pl->d_part);  // d_part holds 3*n_partials doubles
     const bool has_mass = mat.elem_inst ? mat.inst_mc != nullptr : mat.mc != T(0);
     const int mass = has_mass ? (mat.mass_exact ? 2 : 1) : 0;
@@ -537,7 +541,7 @@ int launch_tangent_partials(const tlfea_force_plan* pl, const double* x, const d
                 std_rule = std_rule && dn64.N[q][a] == std_N<10>(q, a);
             }
 #define TLFEA_TANGENT_LAUNCH(NN_, NQ_, STD_, MASS_, LIN_, AFF_)                                              \
-    k_tangent<NN_, NQ_, T, STD_, MASS_, LIN_, AFF_><<<nb, PLAN_BLOCK, lds, s>>>(                              \
+    k_tangent<NN_, NQ_, T, STD_, MASS_, LIN_, AFF_><<<nb, PLAN_BLOCK, LIN_ ? lds6 : lds9, s>>>(               \
         pl->n_elem, pl->d_blk_off, pl->d_unodes, pl->d_slot, x, y, z, X0, Y0, Z0, p, mask, jinv_t, detJ_t, dn, quad, \
         mat, part, pl->max_u, flags, blk_list, dot_part, pl->d_own)
 #define TLFEA_TANGENT_BY_MASS(NN_, NQ_, STD_, LIN_, AFF_)        \
