@@ -228,7 +228,7 @@ def main():
                          "(the copy calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE), then exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        print(json.dumps({"error": "this benchmark runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
 
     import numpy as np

@@ -16,7 +16,7 @@ class ElementBase {
 
     virtual ~ElementBase() {}
 
-    // The reference keeps a bitwise device copy of the element object here.  This build
+    // The reference keeps a bitwise device copy of the element object here.  This library
     // has no device mirror of the object (kernels take plain device arrays through the
     // C-ABI); the member is kept for source compatibility and points at the object.
     ElementBase* d_data = nullptr;

@@ -63,7 +63,7 @@ class TetElementData : public ElementBase {
     void RetrieveMassCSRToCPU(std::vector<int>& offsets, std::vector<int>& columns, std::vector<Real>& values);
     void RetrieveInternalForceToCPU(VectorXR& internal_force) override;
     void RetrieveExternalForceToCPU(VectorXR& external_force);
-    // The reference leaves these three as no-ops (FEAT10Data.cuh:618-627); so does this build.
+    // The reference leaves these three as no-ops (FEAT10Data.cuh:618-627); so do these.
     void RetrieveConstraintDataToCPU(VectorXR&) override {}
     void RetrieveConstraintJacobianToCPU(MatrixXR&) override {}
     void RetrieveDeformationGradientToCPU(std::vector<std::vector<MatrixXR>>&) override {}
@@ -82,7 +82,7 @@ class TetElementData : public ElementBase {
     Real* Get_Constraint_Ptr();
     bool Get_Is_Constraint_Setup();
 
-    // ---- not in the reference: access for this build's solvers ----
+    // ---- not in the reference: access for the library's solver classes ----
     // Device descriptor of the element (positions, gradients, mass, loads, constraints).
     // Builds the mass pattern / force plan on first use; CalcMassMatrix must have run.
     tlfea_body BodyDescriptor();

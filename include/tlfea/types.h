@@ -1,7 +1,7 @@
 // tlfea/types.h -- scalar and dense-matrix types of the drop-in API.
 //
 // The reference's public signatures use Eigen types through aliases in namespace tlfea
-// (src/types.h:19-42).  Eigen is not part of this build, so this header provides the
+// (src/types.h:19-42).  Eigen is not a dependency here, so this header provides the
 // subset those signatures and their callers use: a column-major dense Matrix with
 // dynamic or fixed sizes, Map (a non-owning view), comma initialisation, element access,
 // resize/setZero/min/max, element-wise +/-, scalar *, col(), block() copies and eval().

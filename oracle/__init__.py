@@ -1,7 +1,7 @@
 """ctypes front end of the FP64 CPU oracle (``oracle/oracle.cpp``).
 
 TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()`` and the
-CPU-baseline leg of ``bench.py`` as the checker; the product never imports it.
+CPU-baseline leg of ``bench.py``, to check results against; the product never imports it.
 
 Parity status of each function (how it is pinned, see tests/test_oracle.py):
 

@@ -16,7 +16,7 @@ even) weights, features, activations, aggregated means and, in the backward, the
 gradients that enter a GEMM; FP32 accumulation and FP32 biases.  The GPU gradients are
 compared with that restatement.
 
-There is no reference counterpart: the network is one of this build's additions.
+There is no reference counterpart: the network is an addition to the reference's solve path.
 Parameter packing (FP32 vector): W_in [64x16], b_in [64], per layer W_l [64x128], b_l [64],
 W_out [16x64], b_out [16].
 """

@@ -2,7 +2,7 @@
 //
 // TEST INFRASTRUCTURE ONLY.  Nothing in the product (the HIP library, the C++ drop-in
 // classes, the Python host package) links, loads or calls this file; only tests/,
-// __graft_entry__.smoke() and bench.py's CPU-baseline leg use it, as the checker.
+// __graft_entry__.smoke() and bench.py's CPU-baseline leg use it, to check results against.
 //
 // It restates, in plain sequential C++ (OpenMP over elements where the reference's
 // arithmetic is per-element independent), the arithmetic of the reference files below;

@@ -553,7 +553,7 @@ def test_nesterov_trajectory_matches_python_restatement():
 
 @pytest.mark.parametrize("name", ["cube", "kuhn_t4"])
 def test_extended_precision_force_restatement(name):
-    # the longdouble restatement used to judge small-strain accuracy agrees with the oracle
+    # the longdouble restatement used to assess small-strain accuracy agrees with the oracle
     # where both are accurate (large deformation), and the oracle's own F^T F - I form loses
     # digits at strains near 1e-6 (what the longdouble reference is for)
     from helpers import fint_extended

@@ -29,7 +29,7 @@ static_assert(sizeof(SyncedAdamWParams) == sizeof(tlfea_adamw_params), "param la
 namespace detail {
 
 // SyncedNesterov.cuh:39-110: the reference dispatches on the element type and rejects
-// unknown types; ANCF elements are outside this build.
+// unknown types; this library has no ANCF elements.
 VelocitySolverBase::VelocitySolverBase(ElementBase* data, int n_constraints, int kind)
     : data_(data), kind_(kind), n_coef_(data ? data->get_n_coef() : 0), n_constraints_(n_constraints) {
     std::memset(params_, 0, sizeof(params_));
