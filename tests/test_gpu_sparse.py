@@ -220,12 +220,13 @@ def test_cg_not_positive_definite(gpu):
     assert ls.cg_solve() == capi.TLFEA_ENOTPD
 
 
-def test_cg_zero_rhs(gpu):
+@pytest.mark.parametrize("matfree", [False, True])
+def test_cg_zero_rhs(gpu, matfree):
     from tlfea_amd.fea import LinearStatic
     m = load_mesh("cube")
     m.f_ext = np.zeros(3 * m.n_nodes)
     b = _body(m)
-    ls = LinearStatic(b, tol=1e-10, max_iter=100)
+    ls = LinearStatic(b, tol=1e-10, max_iter=100, matfree=matfree)
     ls.assemble()
     assert ls.cg_solve() == 0 and ls.iters == 0
     assert np.all(_np(ls.u) == 0)
