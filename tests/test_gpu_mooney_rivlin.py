@@ -153,18 +153,68 @@ def test_mr_nesterov_trajectory(gpu, force):
 
 
 def test_mr_rejected_where_not_covered(gpu):
-    """The matrix-free tangent and the FP32 force are SVK-only and say so; an unknown model
-    number is refused."""
-    from tlfea_amd.capi import TlfeaError
+    """The FP32 force is SVK-only and says so; an unknown model number is refused."""
+    import ctypes as C
     import torch
+    from tlfea_amd import capi
+    from tlfea_amd.capi import TlfeaError
     m = load_mesh("C1")
     b = _body(m)
-    p = torch.ones(3 * m.n_nodes, dtype=torch.float64, device="cuda")
-    with pytest.raises(TlfeaError, match="St. Venant-Kirchhoff"):
-        b.tangent_apply(p)
+    u32 = torch.zeros(m.n_nodes, dtype=torch.float32, device="cuda")
+    f32 = torch.zeros(3 * m.n_nodes, dtype=torch.float32, device="cuda")
+    g32 = torch.zeros(b.grad_t.numel(), dtype=torch.float32, device="cuda")
+    d32 = torch.zeros(b.detJ_t.numel(), dtype=torch.float32, device="cuda")
+    rc = capi.lib().tlfea_internal_force_f32(b.nn, b.n_elem, b.n_nodes, C.c_void_p(b.conn.data_ptr()),
+                                             C.c_void_p(u32.data_ptr()), C.c_void_p(u32.data_ptr()),
+                                             C.c_void_p(u32.data_ptr()), C.c_void_p(g32.data_ptr()),
+                                             C.c_void_p(d32.data_ptr()), C.byref(b.mat),
+                                             C.c_void_p(f32.data_ptr()), 0, None)
+    assert rc == capi.TLFEA_EINVAL and b"St. Venant-Kirchhoff" in capi.lib().tlfea_last_error()
     b.mat.model = 7
     with pytest.raises(TlfeaError, match="unknown material model"):
         b.internal_force_planned()
+
+
+@pytest.mark.parametrize("name", ["C1", "kuhn_t4"])
+def test_mr_tangent_apply(gpu, name):
+    """The matrix-free Mooney-Rivlin tangent (dP = A : dF without A) against the oracle's
+    assembled tangent times p: at a deformed state (per-point and affine geometry) and at
+    rest (the linear-elasticity form with the matched Lame parameters)."""
+    import torch
+    m = load_mesh(name)
+    b = _body(m)
+    go, djo = O.grad(m)
+    doff, dcol = O.dof_pattern(m)
+    pt = torch.as_tensor(np.random.default_rng(35).standard_normal(3 * m.n_nodes)).cuda()
+    with O.material("mr", *MR):
+        vr, _ = O.apply_dirichlet(doff, dcol, O.stiffness_values(m, go, djo, 0.0, 0.0, doff, dcol), m.f_ext, m.fixed)
+        x, y, z = deformed(m, seed=36, amp=AMP[name] / 2)
+        vd, _ = O.apply_dirichlet(doff, dcol, O.stiffness_values(m, go, djo, 0.0, 0.0, doff, dcol, x, y, z), m.f_ext,
+                                  m.fixed)
+    assert rel_err(_np(b.tangent_apply(pt, x=b.X0)), O.spmv(doff, dcol, vr, _np(pt))) < 1e-12
+    for i, a in enumerate((x, y, z)):
+        b.x[i].copy_(torch.as_tensor(a))
+    ref = O.spmv(doff, dcol, vd, _np(pt))
+    assert rel_err(_np(b.tangent_apply(pt)), ref) < 1e-12
+    assert rel_err(_np(b.tangent_apply(pt, affine=False)), ref) < 1e-12
+
+
+def test_mr_matfree_cg_against_direct(gpu):
+    """Linear-static solve of a Mooney-Rivlin C1 beam through the matrix-free CG against the
+    oracle's banded Cholesky solve of the same tangent, to the assembled CG's bar."""
+    from tlfea_amd.fea import LinearStatic
+    m = load_mesh("C1")
+    b = _body(m)
+    ls = LinearStatic(b, tol=1e-13, max_iter=200000, matfree=True)
+    ls.assemble()
+    assert ls.cg_solve() == 0
+    go, djo = O.grad(m)
+    doff, dcol = O.dof_pattern(m)
+    with O.material("mr", *MR):
+        vo = O.stiffness_values(m, go, djo, 0.0, 0.0, doff, dcol)
+    vo, fo = O.apply_dirichlet(doff, dcol, vo, m.f_ext, m.fixed)
+    ud = O.band_cholesky_solve(doff, dcol, vo, fo)
+    assert rel_err(_np(ls.u), ud) < 1e-10
 
 
 def _run_driver(args, tmp_path):
@@ -198,11 +248,14 @@ def test_mr_dropin_nesterov_on_cube(gpu, tmp_path):
     assert rel_err(du, duo) < 1e-6
 
 
-def test_mr_dropin_linear_static(gpu, tmp_path):
-    """LinearStaticSolver on a Mooney-Rivlin element assembles the tangent at rest; the
-    displacements match the oracle's direct solve of the Mooney-Rivlin K within 1e-10."""
+@pytest.mark.parametrize("assembled", [False, True])
+def test_mr_dropin_linear_static(gpu, tmp_path, assembled):
+    """LinearStaticSolver on a Mooney-Rivlin element (matrix-free by default, or assembled
+    as the reference does); the displacements match the oracle's direct solve of the
+    Mooney-Rivlin K within 1e-10."""
     info, pos = _run_driver(["linear_static", "--mesh", "kuhn:20,2,2", "--tol", "1e-12", "--max-iter", "200000",
-                             "--load", "-1000", "--mr", ",".join(map(repr, MR))], tmp_path)
+                             "--load", "-1000", "--mr", ",".join(map(repr, MR)), "--assembled",
+                             "1" if assembled else "0"], tmp_path)
     m = load_mesh("C1")
     g, dj = O.grad(m)
     off, col = O.dof_pattern(m)
@@ -212,4 +265,4 @@ def test_mr_dropin_linear_static(gpu, tmp_path):
     u = O.band_cholesky_solve(off, col, vals, fb).reshape(-1, 3)
     assert rel_err(pos.T - m.xyz(), u) < 1e-10
     assert info["residual"] < 1e-12
-    assert info["matrix_free"] is False  # the assembled path carries the Mooney-Rivlin tangent
+    assert info["matrix_free"] is (not assembled)

@@ -171,6 +171,58 @@ __device__ __forceinline__ double mr_tangent_entry(const double F[3][3], const d
            kappa * (2.0 * k.J - 1.0) * k.J * fk * k.FinvT[i][j] + k.t3 * dinv;
 }
 
+// dP = A : dF of mr_stress at F for a direction dF, without forming A: the derivative of
+// P = t1 T1 + t2 T2 + t3 G (T1 = F - I1/3 G, T2 = I1 F - F C - 2 I2/3 G, G = F^-T) with
+// dJ = J G:dF, dG = -G dF^T G, dI1 = 2 F:dF, dI2 = 2 (I1 F - F C):dF,
+// d(F C) = dF C + F (dF^T F + F^T dF), dt1 = -2/3 t1 G:dF, dt2 = -4/3 t2 G:dF,
+// dt3 = kappa (2J - 1) J G:dF -- the contraction of mr_tangent_entry with dF.
+__device__ __forceinline__ void mr_directional(const double F[3][3], const double dF[3][3], double mu10,
+                                               double mu01, double kappa, double dP[3][3]) {
+    MrKin k;
+    mr_kinematics(F, mu10, mu01, kappa, k);
+    double gd = 0.0, dI1 = 0.0, dI2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            gd += k.FinvT[i][j] * dF[i][j];
+            dI1 += 2.0 * F[i][j] * dF[i][j];
+            dI2 += 2.0 * (k.I1 * F[i][j] - k.FC[i][j]) * dF[i][j];
+        }
+    double W[3][3], dC[3][3];  // W = dF^T G, dC = dF^T F + F^T dF
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            double w = 0.0, c = 0.0;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                w += dF[a][i] * k.FinvT[a][j];
+                c += dF[a][i] * F[a][j] + F[a][i] * dF[a][j];
+            }
+            W[i][j] = w;
+            dC[i][j] = c;
+        }
+    const double dt1 = -2.0 / 3.0 * k.t1 * gd, dt2 = -4.0 / 3.0 * k.t2 * gd, dt3 = kappa * (2.0 * k.J - 1.0) * k.J * gd;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            double dG = 0.0, dFC = 0.0;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                dG -= k.FinvT[i][a] * W[a][j];
+                dFC += dF[i][a] * k.C[a][j] + F[i][a] * dC[a][j];
+            }
+            const double G = k.FinvT[i][j];
+            const double T1 = F[i][j] - k.I1 / 3.0 * G;
+            const double T2 = k.I1 * F[i][j] - k.FC[i][j] - 2.0 * k.I2 / 3.0 * G;
+            dP[i][j] = dt1 * T1 + k.t1 * (dF[i][j] - dI1 / 3.0 * G - k.I1 / 3.0 * dG) + dt2 * T2 +
+                       k.t2 * (dI1 * F[i][j] + k.I1 * dF[i][j] - dFC - 2.0 * dI2 / 3.0 * G - 2.0 * k.I2 / 3.0 * dG) +
+                       dt3 * G + k.t3 * dG;
+        }
+}
+
 // P of the element's material: SVK (model 0) or Mooney-Rivlin (model 1).  The hot force
 // kernels take the model as a template parameter (a runtime branch costs the SVK variant
 // its register allocation); k_calc_p decides at run time.

@@ -140,9 +140,8 @@ void LinearStaticSolver<TData>::Solve() {
     int iters = 0;
     double res = 0.0;
     int rc = TLFEA_OK;
-    // K(x) applied without assembly: SVK, no Kelvin-Voigt block, rest positions known
-    const bool matfree = !assembled_only_ && b.mat.model == 0 && b.mat.eta_damp == 0.0 &&
-                         b.mat.lambda_damp == 0.0 && b.force_plan && b.X0;
+    // K(x) applied without assembly: no Kelvin-Voigt block, rest positions known
+    const bool matfree = !assembled_only_ && b.mat.eta_damp == 0.0 && b.mat.lambda_damp == 0.0 && b.force_plan && b.X0;
     last_matfree_ = matfree;
     if (matfree) {
         if (!jinv_) {

@@ -86,7 +86,13 @@ struct TanMat {
     // 1 when every element's M and det J are the same at all quadrature points
     // (tlfea_jinv_affine): the kernel then reads point 0's only
     int affine = 0;
+    // 1: compressible Mooney-Rivlin (one body, FP64, no mass term) with these coefficients
+    int mr = 0;
+    T mu10 = T(0), mu01 = T(0), kappa = T(0);
 };
+
+// The tangent material of one body: SVK (lambda, mu) or Mooney-Rivlin (tangent.hip).
+TanMat<double> body_tanmat(const tlfea_material& m, bool at_rest, bool affine);
 
 // Block partials of the matrix-free tangent (tangent.hip) into plan->d_part (read as T),
 // for all plan blocks or, with blk_list, for the n_list blocks it names.  X0 = Y0 = Z0 =

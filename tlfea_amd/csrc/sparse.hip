@@ -1582,7 +1582,7 @@ extern "C" int tlfea_cg_solve_matfree(tlfea_cg* c, const tlfea_force_plan* plan,
     TLFEA_REQUIRE(max_iter >= 0, "negative iteration limit");
     TLFEA_REQUIRE(plan->n_nodes == c->p->n_nodes, "plan and CG workspace sizes differ");
     TLFEA_REQUIRE(mat->eta_damp == 0.0 && mat->lambda_damp == 0.0, "the matrix-free tangent excludes damping");
-    TLFEA_REQUIRE(mat->model == 0, "the matrix-free tangent covers St. Venant-Kirchhoff only (use the assembled path)");
+    TLFEA_REQUIRE_MODEL(*mat);
     if (check_every <= 0) check_every = 16;
     hipStream_t s = as_stream(stream);
     const int64_t n = c->n;
@@ -1598,9 +1598,7 @@ extern "C" int tlfea_cg_solve_matfree(tlfea_cg* c, const tlfea_force_plan* plan,
     int flags[4] = {0, 0, 0, 0};
     double scal[8] = {0};
     const unsigned gu = grid_for(plan->n_nodes, RED_THREADS);  // one update thread per node
-    TanMat<double> tm{mat->lambda, mat->mu, nullptr, nullptr, nullptr, nullptr, 0};
-    tm.at_rest = at_rest ? 1 : 0;
-    tm.affine = affine != 0 ? 1 : 0;
+    const TanMat<double> tm = body_tanmat(*mat, at_rest, affine != 0);
     // two launches per iteration: the tangent applied to r (block partials, shares of r.Kr
     // and r.r) and the single-reduction update (test, alpha, beta, gather, p, s, u, r);
     // d_Kp holds s = K p

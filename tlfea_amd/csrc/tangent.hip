@@ -38,11 +38,14 @@ __device__ __forceinline__ T tan_mu(const TanMat<T>& m, int inst) {
 // elasticity operator; the same values as the general path at u = 0, with the u terms and
 // their staging left out).  AFF: M and det J are the same at every point (affine
 // elements), point 0's are kept.
-template <int NN, int NQ, typename T, bool STD, int MASS, bool LIN, bool AFF>
+// MR: (lam, mu, kap) are Mooney-Rivlin's (mu10, mu01, kappa) and dP is mr_directional
+// (FP64, general form only).
+template <int NN, int NQ, typename T, bool STD, int MASS, bool LIN, bool AFF, bool MR = false>
 __device__ __forceinline__ void tangent_qp(int q, int64_t tile, int lane, const int (&sl)[NN], const T* sU,
                                            const T* sP, int max_u, const T* __restrict__ jinv_t,
                                            const T* __restrict__ detJ_t, const DnTable<T>& dn, const tlfea_quad& quad,
-                                           T lam, T mu, T mc, T ks, T (&Mn)[9], T& dJn, T& vol, T (&fa)[NN][3]) {
+                                           T lam, T mu, T kap, T mc, T ks, T (&Mn)[9], T& dJn, T& vol,
+                                           T (&fa)[NN][3]) {
     // keep the node values in LDS rather than registers across quadrature points (frees
     // ~120 VGPRs in FP64, i.e. one more wave per SIMD)
     __asm__ volatile("" ::: "memory");
@@ -106,6 +109,21 @@ __device__ __forceinline__ void tangent_qp(int q, int64_t tile, int lane, const 
         for (int i = 0; i < 3; i++)
 #pragma unroll
             for (int j = 0; j < 3; j++) dP[i][j] = (T(2) * mu * dE[i][j] + (i == j ? lam * trdE : T(0))) * (dV * ks);
+    } else if constexpr (MR) {
+        static_assert(sizeof(T) == sizeof(double), "the Mooney-Rivlin tangent is FP64");
+        double F[3][3], dF[3][3], dPm[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                F[i][j] = (i == j ? 1.0 : 0.0) + Ju[i][0] * M[j][0] + Ju[i][1] * M[j][1] + Ju[i][2] * M[j][2];
+                dF[i][j] = Jp[i][0] * M[j][0] + Jp[i][1] * M[j][1] + Jp[i][2] * M[j][2];
+            }
+        mr_directional(F, dF, lam, mu, kap, dPm);
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) dP[i][j] = dPm[i][j] * (dV * ks);
     } else {
         // H = F - I and dF
         T H[3][3], dF[3][3];
@@ -266,7 +284,7 @@ __device__ __forceinline__ void tangent_vertex_t10(const int (&sl)[10], const T*
     }
 }
 
-template <int NN, int NQ, typename T, bool STD, int MASS, bool LIN, bool AFF>
+template <int NN, int NQ, typename T, bool STD, int MASS, bool LIN, bool AFF, bool MR = false>
 __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent(
     int n_elem, const int64_t* __restrict__ blk_off, const int* __restrict__ unodes,
     const unsigned short* __restrict__ slot_t, const double* __restrict__ x, const double* __restrict__ y,
@@ -363,7 +381,7 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent(
         if (active)
             tangent_vertex_t10<T>(sl, sP, sF, max_u, Mn, dJn, tan_lam(mat, inst), tan_mu(mat, inst), mat.k_scale);
     } else if (active) {
-        const T lam = tan_lam(mat, inst), mu = tan_mu(mat, inst);
+        const T lam = MR ? mat.mu10 : tan_lam(mat, inst), mu = MR ? mat.mu01 : tan_mu(mat, inst);
         const T mc = mat.elem_inst ? (mat.inst_mc ? mat.inst_mc[inst] : T(0)) : mat.mc;
         const T ks = mat.k_scale;
         T fa[NN][3];
@@ -373,13 +391,14 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent(
         if constexpr (STD) {
 #pragma unroll
             for (int q = 0; q < NQ; q++)
-                tangent_qp<NN, NQ, T, true, MASS, LIN, AFF>(q, tile, lane, sl, sU, sP, max_u, jinv_t, detJ_t, dn, quad,
-                                                            lam, mu, mc, ks, Mn, dJn, vol, fa);
+                tangent_qp<NN, NQ, T, true, MASS, LIN, AFF, MR>(q, tile, lane, sl, sU, sP, max_u, jinv_t, detJ_t, dn,
+                                                                quad, lam, mu, mat.kappa, mc, ks, Mn, dJn, vol, fa);
         } else {
 #pragma unroll 1
             for (int q = 0; q < NQ; q++)
-                tangent_qp<NN, NQ, T, false, MASS, LIN, false>(q, tile, lane, sl, sU, sP, max_u, jinv_t, detJ_t, dn,
-                                                               quad, lam, mu, mc, ks, Mn, dJn, vol, fa);
+                tangent_qp<NN, NQ, T, false, MASS, LIN, false, MR>(q, tile, lane, sl, sU, sP, max_u, jinv_t, detJ_t,
+                                                                   dn, quad, lam, mu, mat.kappa, mc, ks, Mn, dJn, vol,
+                                                                   fa);
         }
         if constexpr (MASS == 2) {
             // exact element mass rho V C applied to the element's p, one component at a time
@@ -531,6 +550,42 @@ int launch_tangent_partials(const tlfea_force_plan* pl, const double* x, const d
     T* part = reinterpret_cast<T*>(pl->d_part);  // d_part holds 3*n_partials doubles
     const bool has_mass = mat.elem_inst ? mat.inst_mc != nullptr : mat.mc != T(0);
     const int mass = has_mass ? (mat.mass_exact ? 2 : 1) : 0;
+    if (mat.mr) {
+        // Mooney-Rivlin: at rest its tangent is linear elasticity with mu = 2 (mu10 + mu01)
+        // and lambda = kappa - 2 mu / 3 (the rest-state forms run with those); elsewhere the
+        // general form with dP = A : dF (FP64, one body, standard rule, no mass term)
+        if (mat.at_rest) {
+            TanMat<T> eq = mat;
+            eq.mr = 0;
+            eq.mu = T(2) * (mat.mu10 + mat.mu01);
+            eq.lam = mat.kappa - T(2) * eq.mu / T(3);
+            return launch_tangent_partials<T>(pl, x, y, z, X0, Y0, Z0, jinv_t, detJ_t, quad, eq, mask, p, flags, s,
+                                              blk_list, n_list, dot);
+        }
+        if constexpr (sizeof(T) != sizeof(double)) {
+            set_error("the Mooney-Rivlin tangent is FP64 only");
+            return TLFEA_EINVAL;
+        } else {
+            if (mass != 0 || mat.elem_inst || !is_standard_rule(pl->nn, quad)) {
+                set_error("the Mooney-Rivlin tangent covers one body, no mass term, the standard rule");
+                return TLFEA_EINVAL;
+            }
+            const size_t lds = sizeof(double) * (size_t)pl->max_u * 9;
+#define TLFEA_TANGENT_MR(NN_, NQ_, AFF_)                                                                       \
+    k_tangent<NN_, NQ_, double, true, 0, false, AFF_, true><<<nb, PLAN_BLOCK, lds, s>>>(                         \
+        pl->n_elem, pl->d_blk_off, pl->d_unodes, pl->d_slot, x, y, z, X0, Y0, Z0, p, mask, jinv_t, detJ_t, dn, quad, \
+        mat, part, pl->max_u, flags, blk_list, dot_part, pl->d_own)
+            if (pl->nn == 10 && mat.affine)
+                TLFEA_TANGENT_MR(10, 5, true);
+            else if (pl->nn == 10)
+                TLFEA_TANGENT_MR(10, 5, false);
+            else
+                TLFEA_TANGENT_MR(4, 1, false);
+#undef TLFEA_TANGENT_MR
+            TLFEA_LAUNCH_CHECK();
+            return TLFEA_OK;
+        }
+    }
     if (pl->nn == 10) {
         // the constant-folded kernel when the rule is exactly the standard 5-point one
         bool std_rule = quad.n == 5;
@@ -589,14 +644,25 @@ template int launch_tangent_partials<float>(const tlfea_force_plan*, const doubl
                                             const tlfea_quad&, const TanMat<float>&, const unsigned char*,
                                             const float*, const int*, hipStream_t, const int*, int, bool);
 
+TanMat<double> body_tanmat(const tlfea_material& m, bool at_rest, bool affine) {
+    TanMat<double> t{m.lambda, m.mu, nullptr, nullptr, nullptr, nullptr, 0};
+    t.at_rest = at_rest ? 1 : 0;
+    t.affine = affine ? 1 : 0;
+    if (m.model == 1) {
+        t.mr = 1;
+        t.mu10 = m.mu10;
+        t.mu01 = m.mu01;
+        t.kappa = m.kappa;
+    }
+    return t;
+}
+
 int launch_tangent_planned(const tlfea_force_plan* pl, const double* x, const double* y, const double* z,
                            const double* X0, const double* Y0, const double* Z0, const double* jinv_t,
                            const double* detJ_t, const tlfea_quad& quad, const tlfea_material& mat,
                            const unsigned char* mask, const double* p, double* out, double* dot_part,
                            const int* flags, hipStream_t s, bool at_rest, bool affine) {
-    TanMat<double> tm{mat.lambda, mat.mu, nullptr, nullptr, nullptr, nullptr, 0};
-    tm.at_rest = at_rest ? 1 : 0;
-    tm.affine = affine ? 1 : 0;
+    const TanMat<double> tm = body_tanmat(mat, at_rest, affine);
     if (int rc = launch_tangent_partials<double>(pl, x, y, z, X0, Y0, Z0, jinv_t, detJ_t, quad, tm, mask, p, flags,
                                                  s, nullptr, 0))
         return rc;
@@ -766,7 +832,7 @@ extern "C" int tlfea_tangent_apply_planned(const tlfea_force_plan* plan, const d
                   "null argument");
     TLFEA_REQUIRE(quad->n == (plan->nn == 10 ? 5 : 1), "quadrature size does not match the element type");
     TLFEA_REQUIRE(mat->eta_damp == 0.0 && mat->lambda_damp == 0.0, "the matrix-free tangent excludes damping");
-    TLFEA_REQUIRE(mat->model == 0, "the matrix-free tangent covers St. Venant-Kirchhoff only (use the assembled path)");
+    TLFEA_REQUIRE_MODEL(*mat);
     // the linear-elasticity form when the positions are the rest positions' own arrays
     const bool same = x == X0 && y == Y0 && z == Z0;
     return launch_tangent_planned(plan, x, y, z, X0, Y0, Z0, jinv_t, detJ_t, *quad, *mat, fixed_dof_mask, p, out,
