@@ -143,6 +143,40 @@ def batched_beams(n, tol, log, sequential=True):
     return res
 
 
+def velocity_solvers_bench(m, E, nu, rho, log, inner=200):
+    """SyncedNesterov / SyncedAdamW / SyncedAdamWNocoop on the beam (one implicit step of
+    1e-3 s each, zero tolerances so that every solve runs one outer iteration of `inner`
+    inner iterations): ms per inner iteration, the rest-coordinate force kernel inside."""
+    import torch
+    from tlfea_amd import capi
+    from tlfea_amd.fea import Body, Solver
+    out = {}
+    for name, kind in (("nesterov", capi.SOLVER_NESTEROV), ("adamw", capi.SOLVER_ADAMW),
+                       ("adamw_nocoop", capi.SOLVER_ADAMW_NOCOOP)):
+        b = Body(m, E, nu, rho)
+        s = Solver(b, kind, planned=True, rest=True)
+        if kind == capi.SOLVER_NESTEROV:
+            p = capi.NesterovParams(alpha=1e-8, rho=1e14, inner_tol=0.0, outer_tol=0.0, max_outer=1,
+                                    max_inner=inner, time_step=1e-3)
+        else:
+            p = capi.AdamWParams(lr=2e-4, beta1=0.9, beta2=0.999, eps=1e-8, weight_decay=1e-4, lr_decay=0.995,
+                                 inner_tol=0.0, outer_tol=0.0, rho=1e14, max_outer=1, max_inner=inner,
+                                 time_step=1e-3, convergence_check_interval=10, inner_rtol=0.0)
+        s.set_parameters(p)
+        s.solve()  # warm-up
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        s.solve()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        n_inner = s.stats()["inner"]
+        out[name] = {"inner_iterations": n_inner, "ms_per_inner": dt * 1e3 / max(n_inner, 1)}
+        log(f"{name}: {n_inner} inner iterations in {dt * 1e3:.1f} ms ({dt * 1e3 / max(n_inner, 1):.3f} ms each)")
+        del s, b
+        torch.cuda.empty_cache()
+    return out
+
+
 def dynamic_beams_bench(n, steps, seq_steps, tol, log):
     """Batched implicit time steps of n beams: Newton steps of the nonlinear equation (the
     headline) and steps linearised at rest, and the Newton steps of the same beams one by
@@ -221,6 +255,8 @@ def main():
     ap.add_argument("--batch-only", action="store_true", help="only the batched-beams measurement")
     ap.add_argument("--no-seq", action="store_true", help="skip the one-after-another plain solves of the batch")
     ap.add_argument("--no-dynamic", action="store_true", help="skip the dynamic batched measurement")
+    ap.add_argument("--no-solvers", action="store_true", help="skip the Nesterov / AdamW inner-iteration timing")
+    ap.add_argument("--solvers-only", action="store_true", help="only the Nesterov / AdamW inner-iteration timing")
     ap.add_argument("--dynamic-only", action="store_true", help="only the dynamic batched measurement")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one launch each of the planned force kernel, the block SpMV, the matrix-free "
@@ -261,6 +297,10 @@ def main():
         return
     if args.dynamic_only:
         print(json.dumps({"dynamic": dynamic_beams_bench(64, 200, 20, args.tol, log)}))
+        return
+    if args.solvers_only:
+        m = M.clamp_and_tip_load(M.kuhn_beam(nx, ny, nz), -1.0e5)
+        print(json.dumps({"velocity_solvers": velocity_solvers_bench(m, E, nu, rho, log)}))
         return
     if args.batch_only:
         print(json.dumps({"batched": batched_beams(args.batch_n, args.tol, log, not args.no_seq)}))
@@ -390,6 +430,7 @@ def main():
                          f"{t_it * 1e3:.1f} ms/iteration"}
         log(f"cpu baseline: {cpu_solve_s:.1f} s per solve")
 
+    velocity = None if args.no_solvers else velocity_solvers_bench(m, E, nu, rho, log)
     batched = None if args.no_batch else batched_beams(args.batch_n, args.tol, log, not args.no_seq)
     dynamic = None if args.no_dynamic else dynamic_beams_bench(64, 200, 20, args.tol, log)
 
@@ -444,6 +485,7 @@ def main():
                                         "kernel": "k_force_planned<10,5,false> + k_gather_partials",
                                         "avg_us": force_s * 1e6, "bytes_per_launch": force_bytes},
         "cpu_baseline": cpu,
+        "velocity_solvers": velocity,
         "batched": batched,
         "dynamic": dynamic,
     }

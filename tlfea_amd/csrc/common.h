@@ -44,6 +44,17 @@ inline int64_t padded(int64_t n) { return (n + TILE - 1) / TILE * TILE; }
 
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
+// The dispatcher hands workgroup b to XCD b % 8, and each of the 8 XCDs has its own L2.
+// xcd_block(b, n) is a bijection of [0, n) that gives XCD x one contiguous run of logical
+// blocks (the first n % 8 XCDs one block more), so the node data a run gathers stays in one
+// L2 instead of being fetched by all eight.  Results never depend on it: every logical
+// block is still run exactly once.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned n) {
+    constexpr unsigned X = 8;
+    const unsigned q = n / X, r = n % X, x = b % X, k = b / X;
+    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
 // ------------------------------------------------------------------------------------
 // Device algebra
 // ------------------------------------------------------------------------------------

@@ -32,7 +32,7 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_force_planned(
     double* sX = lds;                 // [3][max_u]
     double* sF = lds + 3 * max_u;     // [3][max_u]
     double* sV = lds + 6 * max_u;     // [3][max_u] (DAMP only)
-    const int b = blockIdx.x;
+    const int b = (int)xcd_block(blockIdx.x, gridDim.x);
     const int64_t u0 = blk_off[b];
     const int U = (int)(blk_off[b + 1] - u0);
     for (int u = threadIdx.x; u < U; u += PLAN_BLOCK) {
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_force_rest(
     double* sU = lds + 3 * max_u;  // [3][max_u] displacements
     double* sF = lds + 6 * max_u;  // [3][max_u] accumulated forces
     double* sV = lds + 9 * max_u;  // [3][max_u] velocities (DAMP only)
-    const int b = blockIdx.x;
+    const int b = (int)xcd_block(blockIdx.x, gridDim.x);
     const int64_t u0 = blk_off[b];
     const int U = (int)(blk_off[b + 1] - u0);
     const int e = b * PLAN_BLOCK + threadIdx.x;
@@ -301,7 +301,7 @@ __global__ void k_gather_partials(int n_nodes, const int* __restrict__ g_off, co
                                   const double* __restrict__ part, double* __restrict__ f, int accumulate,
                                   const int* __restrict__ flags) {
     if (flags && (flags[0] | flags[1])) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n_nodes) return;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     for (int k = g_off[i]; k < g_off[i + 1]; k++) {

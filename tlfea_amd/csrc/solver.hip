@@ -34,6 +34,7 @@ struct tlfea_solver {
     double *xp = nullptr, *yp = nullptr, *zp = nullptr;
     double *vk = nullptr, *vn = nullptr;  // Nesterov iterates
     double *g = nullptr, *m1 = nullptr, *m2 = nullptr;
+    double* dv = nullptr;  // [n_nodes][4]: v_guess - v_prev per node, padded to 32 bytes
     // constraints per node
     int *cons_off = nullptr, *cons_idx = nullptr;
     unsigned char* node_fixed = nullptr;
@@ -50,6 +51,8 @@ struct tlfea_solver {
 namespace tlfea_hip {
 
 using Body = tlfea_body;
+// one node's v_guess - v_prev, padded so that a lane's gather is two 16-byte loads
+typedef double dv4 __attribute__((ext_vector_type(4)));
 
 // ------------------------------------------------------------------------------------
 // reductions
@@ -102,8 +105,11 @@ __global__ void k_nes_outer_init(int64_t n3, const double* __restrict__ vg, doub
 
 // y = v_k + beta (v_k - v_km1); v_guess = y; x = x_prev + dt y; f_int = 0; rotate v_km1 <- v_k
 // (SyncedNesterov.cu:152-175).  vk holds v_km1 on entry and v_k on exit; vn holds v_k.
+// dv = y - v_prev, the same difference the gradient's mass term forms per entry
+// (SyncedNesterov.cu:54-55), formed once per node for the gradient kernel's gathers.
 __global__ void k_nes_predict(int n_nodes, double beta, double dt, double* __restrict__ vk, const double* __restrict__ vn,
-                              double* __restrict__ vg, const double* __restrict__ xp, const double* __restrict__ yp,
+                              double* __restrict__ vg, const double* __restrict__ vp, dv4* __restrict__ dv,
+                              const double* __restrict__ xp, const double* __restrict__ yp,
                               const double* __restrict__ zp, double* __restrict__ x, double* __restrict__ y,
                               double* __restrict__ z, double* __restrict__ f_int, const int* __restrict__ flags) {
     if (flags[0] | flags[1]) return;
@@ -118,6 +124,7 @@ __global__ void k_nes_predict(int n_nodes, double beta, double dt, double* __res
         vk[3 * i + d] = k;
         f_int[3 * i + d] = 0.0;
     }
+    dv[i] = dv4{yv[0] - vp[3 * i], yv[1] - vp[3 * i + 1], yv[2] - vp[3 * i + 2], 0.0};
     x[i] = xp[i] + dt * yv[0];
     y[i] = yp[i] + dt * yv[1];
     z[i] = zp[i] + dt * yv[2];
@@ -128,18 +135,21 @@ __global__ void k_nes_predict(int n_nodes, double beta, double dt, double* __res
 // (SyncedNesterov.cu:32-95), with c = x - x_ref at fixed nodes (FEAT10DataFunc.cuh:470-481).
 // mode 0 (Nesterov): v_next = y - alpha g -> vn; partials |g|^2, |g_free|^2, |v_next|^2, |v_k|^2.
 // mode 1 (AdamW): g -> gbuf; partials |g|^2, |g_free|^2, |v_guess|^2.
+// The mass term gathers dv = v_guess - v_prev per column (written by the predict kernels),
+// so each entry adds mass_ij * dv_j * inv_dt exactly as the per-entry difference would.
 template <int MODE>
 __global__ __launch_bounds__(S_RED) void k_grad(Body b, int64_t n_slices, const int* __restrict__ sell_row,
                                                 const int64_t* __restrict__ slice_base,
                                                 const int* __restrict__ slice_len, const int* __restrict__ sell_col,
-                                                const double* __restrict__ vg, const double* __restrict__ vp,
+                                                const double* __restrict__ vg, const dv4* __restrict__ dv,
                                                 const double* __restrict__ lam, const int* __restrict__ cons_off,
                                                 const int* __restrict__ cons_idx,
                                                 const unsigned char* __restrict__ node_fixed, double dt, double rho,
                                                 double alpha, const double* __restrict__ vk, double* __restrict__ out,
                                                 double* __restrict__ part, int nblk, const int* __restrict__ flags) {
     if (flags[0] | flags[1]) return;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t t = (int64_t)blk * blockDim.x + threadIdx.x;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     if (t < n_slices * TILE) {
         const int row = sell_row[t];
@@ -149,12 +159,34 @@ __global__ __launch_bounds__(S_RED) void k_grad(Body b, int64_t n_slices, const 
         const int L = slice_len[sl];
         const double inv_dt = 1.0 / dt;
         double r0 = 0.0, r1 = 0.0, r2 = 0.0;
-        for (int k = 0; k < L; k++) {
-            const int col = sell_col[base + (int64_t)k * TILE + lane];
-            const double mv = b.m_sell[base + (int64_t)k * TILE + lane];
-            r0 += mv * (vg[3 * col] - vp[3 * col]) * inv_dt;
-            r1 += mv * (vg[3 * col + 1] - vp[3 * col + 1]) * inv_dt;
-            r2 += mv * (vg[3 * col + 2] - vp[3 * col + 2]) * inv_dt;
+        const int* cp = sell_col + base + lane;
+        const double* mp = b.m_sell + base + lane;
+        int k = 0;
+        // four entries' loads in flight at once; the sums still run in entry order
+        for (; k + 4 <= L; k += 4) {
+            int col[4];
+            double mv[4];
+            dv4 q[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                col[u] = cp[(int64_t)(k + u) * TILE];
+                mv[u] = mp[(int64_t)(k + u) * TILE];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) q[u] = dv[col[u]];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                r0 += mv[u] * q[u].x * inv_dt;
+                r1 += mv[u] * q[u].y * inv_dt;
+                r2 += mv[u] * q[u].z * inv_dt;
+            }
+        }
+        for (; k < L; k++) {
+            const dv4 q = dv[cp[(int64_t)k * TILE]];
+            const double mv = mp[(int64_t)k * TILE];
+            r0 += mv * q.x * inv_dt;
+            r1 += mv * q.y * inv_dt;
+            r2 += mv * q.z * inv_dt;
         }
         if (row >= 0) {
             double gr[3] = {r0, r1, r2};
@@ -200,7 +232,7 @@ __global__ __launch_bounds__(S_RED) void k_grad(Body b, int64_t n_slices, const 
     block_sums<4>(acc);
     if (threadIdx.x == 0)
 #pragma unroll
-        for (int k = 0; k < 4; k++) part[k * nblk + blockIdx.x] = acc[k];
+        for (int k = 0; k < 4; k++) part[k * nblk + blk] = acc[k];
 }
 
 // Combines the partials; sets flags.
@@ -275,7 +307,7 @@ __global__ void k_adam_outer_init(int64_t n3, double* __restrict__ g, double* __
 __global__ void k_adam_predict(int n_nodes, int nocoop, double lr, double t, double c1, double c2, double beta1,
                                double beta2, double eps, double wd, double dt, const double* __restrict__ g,
                                double* __restrict__ m1, double* __restrict__ m2, double* __restrict__ vg,
-                               const double* __restrict__ xp, const double* __restrict__ yp,
+                               const double* __restrict__ vp, dv4* __restrict__ dv, const double* __restrict__ xp, const double* __restrict__ yp,
                                const double* __restrict__ zp, double* __restrict__ x, double* __restrict__ y,
                                double* __restrict__ z, double* __restrict__ f_int, const int* __restrict__ flags) {
     if (flags[0] | flags[1]) return;
@@ -304,6 +336,7 @@ __global__ void k_adam_predict(int n_nodes, int nocoop, double lr, double t, dou
         vg[k] = yv[d];
         f_int[k] = 0.0;
     }
+    dv[i] = dv4{yv[0] - vp[3 * i], yv[1] - vp[3 * i + 1], yv[2] - vp[3 * i + 2], 0.0};
     x[i] = xp[i] + dt * yv[0];
     y[i] = yp[i] + dt * yv[1];
     z[i] = zp[i] + dt * yv[2];
@@ -408,7 +441,7 @@ int dalloc(T** p, int64_t n) {
 void free_solver(tlfea_solver* s) {
     if (!s) return;
     for (void* p : {(void*)s->v_guess, (void*)s->v_prev, (void*)s->lam, (void*)s->xp, (void*)s->yp, (void*)s->zp,
-                    (void*)s->vk, (void*)s->vn, (void*)s->g, (void*)s->m1, (void*)s->m2, (void*)s->cons_off,
+                    (void*)s->vk, (void*)s->vn, (void*)s->g, (void*)s->m1, (void*)s->m2, (void*)s->dv, (void*)s->cons_off,
                     (void*)s->cons_idx, (void*)s->node_fixed, (void*)s->part, (void*)s->scal, (void*)s->flags})
         hipFree(p);
     delete s;
@@ -455,7 +488,7 @@ extern "C" int tlfea_solver_create(int kind, const tlfea_body* body, tlfea_solve
     if ((rc = dalloc(&s->v_guess, n3)) || (rc = dalloc(&s->v_prev, n3)) || (rc = dalloc(&s->lam, 3 * body->n_fixed)) ||
         (rc = dalloc(&s->xp, body->n_nodes)) || (rc = dalloc(&s->yp, body->n_nodes)) ||
         (rc = dalloc(&s->zp, body->n_nodes)) || (rc = dalloc(&s->vk, n3)) || (rc = dalloc(&s->vn, n3)) ||
-        (rc = dalloc(&s->g, n3)) || (rc = dalloc(&s->m1, n3)) || (rc = dalloc(&s->m2, n3)) ||
+        (rc = dalloc(&s->g, n3)) || (rc = dalloc(&s->m1, n3)) || (rc = dalloc(&s->m2, n3)) || (rc = dalloc(&s->dv, 4 * (int64_t)body->n_nodes)) ||
         (rc = dalloc(&s->part, 4 * (int64_t)nblk)) || (rc = dalloc(&s->scal, 16)) || (rc = dalloc(&s->flags, 8)) ||
         (rc = dalloc(&s->node_fixed, body->n_nodes)) || (rc = dalloc(&s->cons_off, body->n_nodes + 1))) {
         free_solver(s);
@@ -603,7 +636,8 @@ extern "C" int tlfea_solver_solve(tlfea_solver* S, void* stream) {
                 const double t_next = 0.5 * (1.0 + std::sqrt(1.0 + 4.0 * t * t));
                 const double beta = (t - 1.0) / t_next;
                 t = t_next;
-                k_nes_predict<<<gN, 256, 0, st>>>(N, beta, dt, S->vk, S->vn, S->v_guess, S->xp, S->yp, S->zp, b.x,
+                k_nes_predict<<<gN, 256, 0, st>>>(N, beta, dt, S->vk, S->vn, S->v_guess, S->v_prev,
+                                                  reinterpret_cast<dv4*>(S->dv), S->xp, S->yp, S->zp, b.x,
                                                   b.y, b.z, b.f_int, S->flags);
             } else {
                 double lr_k, c1 = 0.0, c2 = 0.0;
@@ -620,7 +654,7 @@ extern "C" int tlfea_solver_solve(tlfea_solver* S, void* stream) {
                 }
                 k_adam_predict<<<gN, 256, 0, st>>>(N, nocoop ? 1 : 0, lr_k, t, c1, c2, S->ap.beta1, S->ap.beta2,
                                                    S->ap.eps, S->ap.weight_decay, dt, S->g, S->m1, S->m2, S->v_guess,
-                                                   S->xp, S->yp, S->zp, b.x, b.y, b.z, b.f_int, S->flags);
+                                                   S->v_prev, reinterpret_cast<dv4*>(S->dv), S->xp, S->yp, S->zp, b.x, b.y, b.z, b.f_int, S->flags);
             }
             TLFEA_LAUNCH_CHECK();
             // element forces at the scratch positions, damping from v_guess; skipped once the
@@ -640,12 +674,12 @@ extern "C" int tlfea_solver_solve(tlfea_solver* S, void* stream) {
             const int check = nes ? 1 : ((inner % check_int) == 0);
             if (nes) {
                 k_grad<0><<<gS, S_RED, 0, st>>>(b, P->n_slices, P->d_sell_row, P->d_slice_base, P->d_slice_len,
-                                                P->d_sell_col, S->v_guess, S->v_prev, S->lam, S->cons_off, S->cons_idx,
+                                                P->d_sell_col, S->v_guess, reinterpret_cast<const dv4*>(S->dv), S->lam, S->cons_off, S->cons_idx,
                                                 S->node_fixed, dt, rho, S->np.alpha, S->vk, S->vn, S->part, gS,
                                                 S->flags);
             } else {
                 k_grad<1><<<gS, S_RED, 0, st>>>(b, P->n_slices, P->d_sell_row, P->d_slice_base, P->d_slice_len,
-                                                P->d_sell_col, S->v_guess, S->v_prev, S->lam, S->cons_off, S->cons_idx,
+                                                P->d_sell_col, S->v_guess, reinterpret_cast<const dv4*>(S->dv), S->lam, S->cons_off, S->cons_idx,
                                                 S->node_fixed, dt, rho, 0.0, nullptr, S->g, S->part, gS, S->flags);
             }
             k_finalize<<<1, S_RED, 0, st>>>(S->kind, inner, check, gS, S->part, S->scal, S->flags, inner_tol,

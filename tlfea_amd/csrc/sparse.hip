@@ -728,7 +728,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_cgcg_update(
     if (threadIdx.x == 0) stop = flags[0];
     // the gather of this thread's node and its share of the delta / gamma sums are all
     // requested before the block's one barrier
-    const int i0 = blockIdx.x * RED_THREADS;
+    const int i0 = xcd_block(blockIdx.x, gridDim.x) * RED_THREADS;
     const int i = i0 + threadIdx.x;
     double w0 = 0.0, w1 = 0.0, w2 = 0.0;
     if (i < n_nodes) {

@@ -300,7 +300,8 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent(
     T* sU = reinterpret_cast<T*>(lds_raw);
     T* sF = LIN ? sU : sU + 3 * max_u;
     T* sP = sF + 3 * max_u;
-    const int b = blk_list ? blk_list[blockIdx.x] : blockIdx.x;
+    const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+    const int b = blk_list ? blk_list[lb] : (int)lb;
     const int64_t u0 = blk_off[b];
     const int U = (int)(blk_off[b + 1] - u0);
     const int e = b * PLAN_BLOCK + threadIdx.x;
@@ -469,7 +470,8 @@ __global__ __launch_bounds__(256) void k_gather_tangent(int n_nodes, const int* 
                                                         const double* __restrict__ p, double* __restrict__ out,
                                                         double* __restrict__ dot_part, const int* __restrict__ flags) {
     if (flags && flags[0]) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+    const int i = lb * blockDim.x + threadIdx.x;
     double dotv = 0.0;
     if (i < n_nodes) {
         double s[3] = {0.0, 0.0, 0.0};
@@ -492,7 +494,7 @@ __global__ __launch_bounds__(256) void k_gather_tangent(int n_nodes, const int* 
         for (int o = 32; o > 0; o >>= 1) dotv += __shfl_xor(dotv, o, 64);
         if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dotv;
         __syncthreads();
-        if (threadIdx.x == 0) dot_part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+        if (threadIdx.x == 0) dot_part[lb] = red[0] + red[1] + red[2] + red[3];
     }
 }
 
