@@ -127,12 +127,7 @@ __global__ __launch_bounds__(BT) void k_bgather(Chunks ch, const int* __restrict
     double dot = 0.0;
     for (int i = ch.beg[c] + threadIdx.x; i < ch.end[c]; i += BT) {
         T s0 = T(0), s1 = T(0), s2 = T(0);
-        for (int k = g_off[i]; k < g_off[i + 1]; k++) {
-            const T* pp = part + 3 * g_idx[k];
-            s0 += pp[0];
-            s1 += pp[1];
-            s2 += pp[2];
-        }
+        gather_slots3(g_idx, part, g_off[i], g_off[i + 1], s0, s1, s2);
         const T sv[3] = {s0, s1, s2};
 #pragma unroll
         for (int d = 0; d < 3; d++) {
@@ -269,12 +264,7 @@ __global__ __launch_bounds__(BT) void k_bgather_rw(Chunks ch, const int* __restr
     double rw = 0.0, rr = 0.0;
     for (int i = ch.beg[c] + threadIdx.x; i < ch.end[c]; i += BT) {
         double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-        for (int k = g_off[i]; k < g_off[i + 1]; k++) {
-            const double* pp = part + 3 * g_idx[k];
-            s0 += pp[0];
-            s1 += pp[1];
-            s2 += pp[2];
-        }
+        gather_slots3(g_idx, part, g_off[i], g_off[i + 1], s0, s1, s2);
         const double sv[3] = {s0, s1, s2};
 #pragma unroll
         for (int d = 0; d < 3; d++) {

@@ -304,12 +304,7 @@ __global__ void k_gather_partials(int n_nodes, const int* __restrict__ g_off, co
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n_nodes) return;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int k = g_off[i]; k < g_off[i + 1]; k++) {
-        const double* p = part + 3 * g_idx[k];
-        s0 += p[0];
-        s1 += p[1];
-        s2 += p[2];
-    }
+    gather_slots3(g_idx, part, g_off[i], g_off[i + 1], s0, s1, s2);
     if (accumulate) {
         f[3 * i] += s0;
         f[3 * i + 1] += s1;

@@ -137,4 +137,38 @@ int launch_force(int nn, int n_elem, const int* conn, const double* x, const dou
                  const double* grad_t, const double* detJ_t, const tlfea_quad& quad, const tlfea_material& mat,
                  const double* v, double* f, const int* flags, hipStream_t s);
 
+// Sum of one node's block partials part[3 idx[k] + d] over k in [k0, k1), in ascending k.
+// The slot indices of four entries are loaded before any of their partials, so a node
+// costs two memory round trips per four entries rather than two per entry; the additions
+// run in the same order as a plain loop, so the sums are the same to the bit.
+template <typename T>
+__device__ __forceinline__ void gather_slots3(const int64_t* __restrict__ idx, const T* __restrict__ part, int k0,
+                                              int k1, T& s0, T& s1, T& s2) {
+    int k = k0;
+    for (; k + 4 <= k1; k += 4) {
+        int64_t j[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) j[u] = 3 * idx[k + u];
+        T a[4][3];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            a[u][0] = part[j[u]];
+            a[u][1] = part[j[u] + 1];
+            a[u][2] = part[j[u] + 2];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            s0 += a[u][0];
+            s1 += a[u][1];
+            s2 += a[u][2];
+        }
+    }
+    for (; k < k1; k++) {
+        const T* pp = part + 3 * idx[k];
+        s0 += pp[0];
+        s1 += pp[1];
+        s2 += pp[2];
+    }
+}
+
 }  // namespace tlfea_hip

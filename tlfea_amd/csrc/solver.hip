@@ -303,7 +303,8 @@ __global__ void k_adam_outer_init(int64_t n3, double* __restrict__ g, double* __
 
 // m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2; y = v_g - lr (m_hat/(sqrt(v_hat)+eps) + wd v_g)
 // with m_hat = m * c1, v_hat = v * c2 (nocoop: c = 1/(1-beta^t) from the host) or
-// m_hat = m / (1 - pow(beta, t)) (coop, SyncedAdamW.cu:146-156); then x = x_prev + dt y, f = 0.
+// m_hat = m / c1, v_hat = v / c2 (coop, SyncedAdamW.cu:146-167: c = 1 - pow(beta, t), the
+// same for every DOF, so the host evaluates it once per iteration); then x = x_prev + dt y, f = 0.
 __global__ void k_adam_predict(int n_nodes, int nocoop, double lr, double t, double c1, double c2, double beta1,
                                double beta2, double eps, double wd, double dt, const double* __restrict__ g,
                                double* __restrict__ m1, double* __restrict__ m2, double* __restrict__ vg,
@@ -327,8 +328,8 @@ __global__ void k_adam_predict(int n_nodes, int nocoop, double lr, double t, dou
         } else {
             mt = beta1 * m1[k] + (1 - beta1) * gi;
             vt = beta2 * m2[k] + (1 - beta2) * gi * gi;
-            mh = mt / (1 - pow(beta1, t));
-            vh = vt / (1 - pow(beta2, t));
+            mh = mt / c1;
+            vh = vt / c2;
         }
         m1[k] = mt;
         m2[k] = vt;
@@ -651,6 +652,8 @@ extern "C" int tlfea_solver_solve(tlfea_solver* S, void* stream) {
                     lr = lr * S->ap.lr_decay;
                     t += 1;
                     lr_k = lr;
+                    c1 = 1 - std::pow(S->ap.beta1, t);
+                    c2 = 1 - std::pow(S->ap.beta2, t);
                 }
                 k_adam_predict<<<gN, 256, 0, st>>>(N, nocoop ? 1 : 0, lr_k, t, c1, c2, S->ap.beta1, S->ap.beta2,
                                                    S->ap.eps, S->ap.weight_decay, dt, S->g, S->m1, S->m2, S->v_guess,

@@ -732,12 +732,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_cgcg_update(
     const int i = i0 + threadIdx.x;
     double w0 = 0.0, w1 = 0.0, w2 = 0.0;
     if (i < n_nodes) {
-        for (int e = g_off[i]; e < g_off[i + 1]; e++) {
-            const double* pp = bpart + 3 * g_idx[e];
-            w0 += pp[0];
-            w1 += pp[1];
-            w2 += pp[2];
-        }
+        gather_slots3(g_idx, bpart, g_off[i], g_off[i + 1], w0, w1, w2);
     }
     double dl = 0.0, gm = 0.0;
     for (int t = threadIdx.x; t < nb; t += RED_THREADS) {

@@ -475,12 +475,7 @@ __global__ __launch_bounds__(256) void k_gather_tangent(int n_nodes, const int* 
     double dotv = 0.0;
     if (i < n_nodes) {
         double s[3] = {0.0, 0.0, 0.0};
-        for (int k = g_off[i]; k < g_off[i + 1]; k++) {
-            const double* pp = part + 3 * g_idx[k];
-            s[0] += pp[0];
-            s[1] += pp[1];
-            s[2] += pp[2];
-        }
+        gather_slots3(g_idx, part, g_off[i], g_off[i + 1], s[0], s[1], s[2]);
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             const double v = (mask && mask[3 * i + d]) ? p[3 * i + d] : s[d];
