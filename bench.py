@@ -37,6 +37,10 @@ cpu_baseline: the FP64 oracle on a bounded sample (a slice of the element assemb
 10 CG iterations on the full matrix), extrapolated to one solve with the plain GPU CG's
 iteration count.
 
+`velocity_solvers` (not part of `value`): SyncedNesterov, SyncedAdamW and SyncedAdamWNocoop
+on the same beam, one implicit step of 1e-3 s with zero tolerances (one outer iteration of
+200 inner iterations), ms per inner iteration after a warm-up solve.
+
 `batched` (not part of `value`): the batch of 256 uneven cantilevers of the beam sampler
 (seed 2026) solved to 1e-8 -- all at once by the batched FP64 CG (finished instances
 compacted out), with the FP32 Jacobi-Chebyshev stage, from the learned guess, and one
