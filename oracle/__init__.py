@@ -30,9 +30,12 @@ Parity status of each function (how it is pinned, see tests/test_oracle.py):
   difference of the total MR energy, K = finite difference of f_int.
 * ``cg`` / ``band_cholesky_solve`` -- each against dense numpy solves, Euler-Bernoulli
   band on the beam.
-* ``nesterov_step`` / ``adamw_step`` -- restatement only for the iterate sequence
-  ("parity unpinned" for the exact sequence); the converged state is pinned against the
-  direct solve of the same implicit-Euler equation.
+* ``nesterov_step`` -- the iterate sequence against an independent pure-Python
+  restatement of the loop; one long outer iteration against the direct solve of the same
+  implicit-Euler equation.
+* ``adamw_step`` (coop and nocoop) -- the iterate sequence against torch.optim.AdamW driven
+  by ``step_gradient`` (1e-10; a wrong bias-correction step, decay order, weight decay, eps
+  or gradient lag fails it); the converged state against the direct solve.
 """
 from __future__ import annotations
 

@@ -513,9 +513,9 @@ def test_nesterov_outer_loop_approaches_static_equilibrium():
 
 @pytest.mark.parametrize("nocoop", [False, True])
 def test_adamw_single_outer_approaches_implicit_step(nocoop):
-    """AdamW's iterates are pinned by restatement only ("parity unpinned" for the exact
-    sequence); its one-outer result is checked to approach the direct implicit-step
-    solution (fixed learning rate -> a loose band)."""
+    """AdamW's one-outer result approaches the direct implicit-step solution (fixed
+    learning rate -> a loose band); its iterate sequence is pinned by the torch.optim.AdamW
+    test below."""
     m = load_mesh("cube")
     prob = O.Problem(m, 7e8, 0.33, 2700.0)
     dt, rho = 1e-3, 1e10
@@ -523,6 +523,41 @@ def test_adamw_single_outer_approaches_implicit_step(nocoop):
     prob.adamw_step(st, 1e-4, 0.9, 0.999, 1e-8, 0.0, 0.999, 0.0, 0.0, rho, 1, 4000, dt, 10, 0.0, nocoop=nocoop)
     v_ref, free = _newton(m, prob, dt, "step", rho=rho)
     assert rel_err(st["v_guess"], v_ref) < 5e-2
+
+
+@pytest.mark.parametrize("nocoop", [False, True])
+def test_adamw_trajectory_matches_torch_adamw(nocoop):
+    """The oracle's AdamW inner loop against torch.optim.AdamW, an independent
+    implementation of the same decoupled-weight-decay Adam (SyncedAdamW.cu:146-167,
+    SyncedAdamWNocoop.cu:98-253), driven by the oracle's own step gradient.  The reference's
+    conventions map onto it as: the first update uses g = 0 and its bias correction uses
+    t = 2 (torch's step counter preset to 1), and the learning rate is decayed before each
+    use (lr_k = lr0 * decay^(k+1), set on the parameter group).  Pins the moment updates,
+    the bias corrections, eps outside the square root and the weight decay on v_guess."""
+    import torch
+    m = load_mesh("cube")
+    prob = O.Problem(m, 7e8, 0.33, 2700.0)
+    dt, rho = 1e-3, 1e10
+    lr0, b1, b2, eps, wd, decay = 2e-4, 0.9, 0.999, 1e-8, 1e-4, 0.995
+    n_inner = 12
+    st = prob.new_state()
+    xp, yp, zp = st["x"].copy(), st["y"].copy(), st["z"].copy()
+    v_prev, lam = st["v_prev"].copy(), st["lam"].copy()
+    prob.adamw_step(st, lr0, b1, b2, eps, wd, decay, 0.0, 0.0, rho, 1, n_inner, dt, 10, 0.0, nocoop=nocoop)
+
+    v = torch.zeros(3 * m.n_nodes, dtype=torch.float64, requires_grad=True)
+    opt = torch.optim.AdamW([v], lr=lr0, betas=(b1, b2), eps=eps, weight_decay=wd, foreach=False)
+    opt.state[v] = {"step": torch.tensor(1.0), "exp_avg": torch.zeros_like(v),
+                    "exp_avg_sq": torch.zeros_like(v)}
+    g = np.zeros(3 * m.n_nodes)
+    for k in range(n_inner):
+        opt.param_groups[0]["lr"] = lr0 * decay ** (k + 1)
+        v.grad = torch.from_numpy(g.copy())
+        opt.step()
+        g = prob.step_gradient(xp, yp, zp, v.detach().numpy(), v_prev, lam, dt, rho)
+    ref = v.detach().numpy()
+    assert np.abs(ref).max() > 0.0
+    assert rel_err(st["v_guess"], ref) < 1e-10
 
 
 def test_nesterov_trajectory_matches_python_restatement():
