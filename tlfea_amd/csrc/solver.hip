@@ -103,8 +103,8 @@ __global__ void k_nes_outer_init(int64_t n3, const double* __restrict__ vg, doub
     }
 }
 
-// y = v_k + beta (v_k - v_km1); v_guess = y; x = x_prev + dt y; f_int = 0; rotate v_km1 <- v_k
-// (SyncedNesterov.cu:152-175).  vk holds v_km1 on entry and v_k on exit; vn holds v_k.
+// y = v_k + beta (v_k - v_km1); v_guess = y; x = x_prev + dt y; f_int = 0 (f_int null: the
+// force launch overwrites it); rotate v_km1 <- v_k (SyncedNesterov.cu:152-175).  vk holds v_km1 on entry and v_k on exit; vn holds v_k.
 // dv = y - v_prev, the same difference the gradient's mass term forms per entry
 // (SyncedNesterov.cu:54-55), formed once per node for the gradient kernel's gathers.
 __global__ void k_nes_predict(int n_nodes, double beta, double dt, double* __restrict__ vk, const double* __restrict__ vn,
@@ -122,7 +122,7 @@ __global__ void k_nes_predict(int n_nodes, double beta, double dt, double* __res
         yv[d] = k + beta * (k - km1);
         vg[3 * i + d] = yv[d];
         vk[3 * i + d] = k;
-        f_int[3 * i + d] = 0.0;
+        if (f_int) f_int[3 * i + d] = 0.0;
     }
     dv[i] = dv4{yv[0] - vp[3 * i], yv[1] - vp[3 * i + 1], yv[2] - vp[3 * i + 2], 0.0};
     x[i] = xp[i] + dt * yv[0];
@@ -304,12 +304,13 @@ __global__ void k_adam_outer_init(int64_t n3, double* __restrict__ g, double* __
 // m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2; y = v_g - lr (m_hat/(sqrt(v_hat)+eps) + wd v_g)
 // with m_hat = m * c1, v_hat = v * c2 (nocoop: c = 1/(1-beta^t) from the host) or
 // m_hat = m / c1, v_hat = v / c2 (coop, SyncedAdamW.cu:146-167: c = 1 - pow(beta, t), the
-// same for every DOF, so the host evaluates it once per iteration); then x = x_prev + dt y, f = 0.
-__global__ void k_adam_predict(int n_nodes, int nocoop, double lr, double t, double c1, double c2, double beta1,
+// same for every DOF, so the host evaluates it once per iteration); then x = x_prev + dt y and
+// f = 0 unless f_int is null (the force launch then overwrites it).
+__global__ void k_adam_predict(int n_nodes, int nocoop, double lr, double c1, double c2, double beta1,
                                double beta2, double eps, double wd, double dt, const double* __restrict__ g,
                                double* __restrict__ m1, double* __restrict__ m2, double* __restrict__ vg,
-                               const double* __restrict__ vp, dv4* __restrict__ dv, const double* __restrict__ xp, const double* __restrict__ yp,
-                               const double* __restrict__ zp, double* __restrict__ x, double* __restrict__ y,
+                               const double* __restrict__ vp, dv4* __restrict__ dv, const double* __restrict__ xp,
+                               const double* __restrict__ yp, const double* __restrict__ zp, double* __restrict__ x, double* __restrict__ y,
                                double* __restrict__ z, double* __restrict__ f_int, const int* __restrict__ flags) {
     if (flags[0] | flags[1]) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -335,7 +336,7 @@ __global__ void k_adam_predict(int n_nodes, int nocoop, double lr, double t, dou
         m2[k] = vt;
         yv[d] = v0 - lr * (mh / (sqrt(vh) + eps) + wd * v0);
         vg[k] = yv[d];
-        f_int[k] = 0.0;
+        if (f_int) f_int[k] = 0.0;
     }
     dv[i] = dv4{yv[0] - vp[3 * i], yv[1] - vp[3 * i + 1], yv[2] - vp[3 * i + 2], 0.0};
     x[i] = xp[i] + dt * yv[0];
@@ -619,6 +620,8 @@ extern "C" int tlfea_solver_solve(tlfea_solver* S, void* stream) {
     const unsigned gN = grid_for(N, 256), g3 = grid_for(n3, 256);
     const unsigned gS = (unsigned)S->max_blocks;
     const bool damp = b.mat.eta_damp != 0.0 || b.mat.lambda_damp != 0.0;
+    // the planned force launches overwrite f_int (accumulate = 0); the atomic one adds to it
+    double* const fzero = b.force_plan ? nullptr : b.f_int;
 
     TLFEA_HIP_TRY(hipMemsetAsync(S->flags, 0, sizeof(int) * 8, st));
     k_save_prev<<<gN, 256, 0, st>>>(N, b.x, b.y, b.z, S->xp, S->yp, S->zp);
@@ -639,7 +642,7 @@ extern "C" int tlfea_solver_solve(tlfea_solver* S, void* stream) {
                 t = t_next;
                 k_nes_predict<<<gN, 256, 0, st>>>(N, beta, dt, S->vk, S->vn, S->v_guess, S->v_prev,
                                                   reinterpret_cast<dv4*>(S->dv), S->xp, S->yp, S->zp, b.x,
-                                                  b.y, b.z, b.f_int, S->flags);
+                                                  b.y, b.z, fzero, S->flags);
             } else {
                 double lr_k, c1 = 0.0, c2 = 0.0;
                 if (nocoop) {
@@ -655,9 +658,10 @@ extern "C" int tlfea_solver_solve(tlfea_solver* S, void* stream) {
                     c1 = 1 - std::pow(S->ap.beta1, t);
                     c2 = 1 - std::pow(S->ap.beta2, t);
                 }
-                k_adam_predict<<<gN, 256, 0, st>>>(N, nocoop ? 1 : 0, lr_k, t, c1, c2, S->ap.beta1, S->ap.beta2,
+                k_adam_predict<<<gN, 256, 0, st>>>(N, nocoop ? 1 : 0, lr_k, c1, c2, S->ap.beta1, S->ap.beta2,
                                                    S->ap.eps, S->ap.weight_decay, dt, S->g, S->m1, S->m2, S->v_guess,
-                                                   S->v_prev, reinterpret_cast<dv4*>(S->dv), S->xp, S->yp, S->zp, b.x, b.y, b.z, b.f_int, S->flags);
+                                                   S->v_prev, reinterpret_cast<dv4*>(S->dv), S->xp, S->yp, S->zp,
+                                                   b.x, b.y, b.z, fzero, S->flags);
             }
             TLFEA_LAUNCH_CHECK();
             // element forces at the scratch positions, damping from v_guess; skipped once the
