@@ -24,9 +24,10 @@ class LinearStaticSolver : public SolverBase {
     int GetLastIterCount() const { return last_iter_count_; }
     Real GetLastResidual() const { return last_residual_; }
     // Not in the reference.  Solve applies K(x) without assembling it (the same operator
-    // and CG stopping rule, ~5x faster on large meshes) whenever the element has no damping
-    // and its rest positions are known (CalcDnDuPre); otherwise, or with
-    // UseAssembledOperator(true), it assembles K as the reference does.
+    // and CG stopping rule, ~5x faster on large meshes) whenever the element's rest
+    // positions are known (CalcDnDuPre) -- with Kelvin-Voigt damping (operator K + C) only
+    // while the positions are the rest positions; otherwise, or with
+    // UseAssembledOperator(true), it assembles K (+ C) as the reference does.
     void UseAssembledOperator(bool on) { assembled_only_ = on; }
     bool LastSolveWasMatrixFree() const { return last_matfree_; }
 

@@ -142,7 +142,11 @@ int tlfea_internal_force_rest(const tlfea_force_plan* plan, const double* x, con
  * instead of ~3 kB of assembled 3x3 blocks.  With a DOF mask (uint8 [3*n_nodes],
  * 1 = fixed) the masked rows return y = p (the Dirichlet identity rows of
  * tlfea_apply_dirichlet) and the masked columns are ignored (p read as 0); NULL = no
- * mask.  Kelvin-Voigt damping is not part of this operator (mat damping must be 0).
+ * mask.  With Kelvin-Voigt damping in mat the operator is the reference's K + C
+ * (FEAT10DataFunc.cuh:686-767), available only when x, y, z are X0, Y0, Z0: at F = I the
+ * C block is the linear-elasticity block with (lambda_d, eta) for (lambda, mu), so K + C is
+ * the linear-elasticity operator with lambda + lambda_d and mu + eta (TLFEA_EINVAL away
+ * from rest).
  * affine (as reported by tlfea_jinv_affine; 0 is always valid): M and det J of quadrature
  * point 0 only are read.  When x, y, z are the arrays X0, Y0, Z0 themselves the kernel
  * runs its linear-elasticity form (u = 0: H = S = 0, dP = dS), the same values without
@@ -251,7 +255,8 @@ int tlfea_constraint_eval(int n_fixed, const int* fixed_nodes, const double* x, 
  * operator K(x) applied by tlfea_tangent_apply_planned instead of an assembled matrix:
  * the linear-static system of LinearStaticSolver::Solve after Dirichlet elimination.
  * When x equals X0 on every node (checked once per call) the operator is applied in its
- * linear-elasticity form. */
+ * linear-elasticity form; with Kelvin-Voigt damping (operator K + C) the call needs that
+ * and returns TLFEA_EINVAL before any iteration otherwise. */
 int tlfea_cg_solve_matfree(tlfea_cg* c, const tlfea_force_plan* plan, const double* x, const double* y,
                            const double* z, const double* X0, const double* Y0, const double* Z0,
                            const double* jinv_t, const double* detJ_t, int affine, const tlfea_quad* quad,

@@ -16,9 +16,9 @@ pytestmark = pytest.mark.gpu
 MESHES = ["cube", "beam_vtu", "C1", "kuhn_t4"]
 
 
-def _body(m, E=E_STEEL, nu=NU_STEEL, eta=0.0):
+def _body(m, E=E_STEEL, nu=NU_STEEL, eta=0.0, lambda_d=0.0):
     from tlfea_amd.fea import Body
-    return Body(m, E, nu, RHO_STEEL, eta, 0.0)
+    return Body(m, E, nu, RHO_STEEL, eta, lambda_d)
 
 
 def _np(t):
@@ -209,6 +209,8 @@ def test_tangent_is_force_derivative(gpu, name):
 
 
 def test_tangent_rejects_damping(gpu):
+    """With Kelvin-Voigt damping the operator K + C exists only at the rest positions: a
+    call whose positions are other arrays than X0 is rejected."""
     import ctypes as C
 
     import torch
@@ -223,6 +225,67 @@ def test_tangent_rejects_damping(gpu):
                                                 ptr(X0[1]), ptr(X0[2]), ptr(b.jinv_t), ptr(b.detJ_t), 0,
                                                 C.byref(b.quad), C.byref(b.mat), None, ptr(p), ptr(out), None)
     assert rc == capi.TLFEA_EINVAL
+
+
+# Kelvin-Voigt coefficients of the damped tests: a quarter of mu and half of lambda, so
+# that K + C differs from K by tens of percent
+ETA_D, LD_D = 0.25 * lame(E_STEEL, NU_STEEL)[1], 0.5 * lame(E_STEEL, NU_STEEL)[0]
+
+
+def _damped_oracle(m):
+    go, djo = O.grad(m)
+    lam, mu = lame(E_STEEL, NU_STEEL)
+    doff, dcol = O.dof_pattern(m)
+    return doff, dcol, O.stiffness_values(m, go, djo, lam, mu, doff, dcol, eta=ETA_D, lambda_d=LD_D)
+
+
+@pytest.mark.parametrize("name", ["cube", "C1", "kuhn_t4"])
+def test_damped_tangent_at_rest(gpu, name):
+    """K + C applied at the rest positions (the rest-state operator with lambda + lambda_d,
+    mu + eta) against the oracle's assembled damped tangent (FEAT10DataFunc.cuh:512-767)
+    times p, 1e-12; the same body away from rest is rejected."""
+    import torch
+    from tlfea_amd import capi
+    print(f"[damped tangent] {name}", flush=True)
+    m = load_mesh(name)
+    b = _body(m, eta=ETA_D, lambda_d=LD_D)
+    p = np.random.default_rng(21).standard_normal(3 * m.n_nodes)
+    pt = torch.as_tensor(p).cuda()
+    kp = _np(b.tangent_apply(pt, x=b.X0, masked=False, damping=True))
+    doff, dcol, vo = _damped_oracle(m)
+    ref = O.spmv(doff, dcol, vo, p)
+    assert rel_err(kp, ref) < 1e-12
+    k_only = _np(b.tangent_apply(pt, x=b.X0, masked=False))
+    assert rel_err(k_only, ref) > 1e-2  # the damping block is really in it
+    with pytest.raises(capi.TlfeaError):
+        b.tangent_apply(pt, masked=False, damping=True)
+
+
+@pytest.mark.parametrize("name", ["cube", "C1"])
+def test_damped_matfree_cg_against_direct(gpu, name):
+    """A damped body's linear-static system is (K + C) u = f in the reference (the block is
+    added unscaled, h = 1): the matrix-free CG at rest and the assembled CG both against the
+    oracle's banded Cholesky solve of it; once the positions have moved the matrix-free CG
+    rejects the damped operator (no iteration runs)."""
+    import torch
+    from tlfea_amd import capi
+    from tlfea_amd.fea import LinearStatic
+    print(f"[damped matfree cg] {name}", flush=True)
+    m = load_mesh(name)
+    b = _body(m, eta=ETA_D, lambda_d=LD_D)
+    doff, dcol, vo = _damped_oracle(m)
+    vo, fo = O.apply_dirichlet(doff, dcol, vo, m.f_ext, m.fixed)
+    ud = O.band_cholesky_solve(doff, dcol, vo, fo)
+    for matfree in (True, False):
+        ls = LinearStatic(b, tol=1e-13, max_iter=200000, matfree=matfree)
+        ls.assemble()
+        assert ls.cg_solve() == 0, (matfree, ls.iters, ls.res)
+        assert rel_err(_np(ls.u), ud) < CG_DISP_BAR[name], matfree
+    b.x[2].add_(torch.full_like(b.x[2], 1e-4))
+    moved = LinearStatic(b, tol=1e-8, max_iter=100, matfree=True)
+    moved.assemble()
+    with pytest.raises(capi.TlfeaError):
+        moved.cg_solve()
 
 
 CG_DISP_BAR = {"cube": 1e-10, "beam_vtu": 1e-10, "C1": 1e-10, "beam_highres_t4": 20 * 3.2e7 * 2.2e-16}

@@ -391,6 +391,27 @@ def test_stiffness_damping_block():
     assert rel_err(C @ v, f1 - f0) < 1e-10
 
 
+@pytest.mark.parametrize("name", ["cube", "kuhn_t4"])
+def test_damped_stiffness_at_rest_is_shifted_linear_elasticity(name):
+    """At the rest positions K + C (the Kelvin-Voigt block added unscaled,
+    FEAT10DataFunc.cuh:686-767) equals the undamped tangent with lambda + lambda_d and
+    mu + eta: the identity the matrix-free operator relies on for damped bodies.  Away
+    from rest it does not hold (the check that would catch a form that ignored x)."""
+    m = load_mesh(name)
+    lam, mu = lame(2e9, 0.3)
+    eta, ld = 4e7, 9e6
+    g, dj = O.grad(m)
+    off, col = O.dof_pattern(m)
+    kc = O.stiffness_values(m, g, dj, lam, mu, off, col, eta=eta, lambda_d=ld)
+    ks = O.stiffness_values(m, g, dj, lam + ld, mu + eta, off, col)
+    assert rel_err(kc, ks) < 1e-13
+    assert rel_err(kc, O.stiffness_values(m, g, dj, lam, mu, off, col)) > 1e-2
+    x, y, z = deformed(m, seed=3, amp=2e-2)
+    kcx = O.stiffness_values(m, g, dj, lam, mu, off, col, x, y, z, eta=eta, lambda_d=ld)
+    ksx = O.stiffness_values(m, g, dj, lam + ld, mu + eta, off, col, x, y, z)
+    assert rel_err(kcx, ksx) > 1e-4
+
+
 # ---------------------------------------------------------------------------------------
 def test_dirichlet_against_dense_elimination():
     m = load_mesh("cube")

@@ -6,10 +6,11 @@
 //
 //   tlfea_driver linear_static  --mesh kuhn:20,2,2 | kuhn4:12,3,2 | vtu:PATH | tetgen:PREFIX
 //                 [--E 2.1e11 --nu 0.3 --load -1000 --tol 1e-10 --max-iter 50000]
-//                 [--assembled 1: assemble K as the reference does] --out FILE
+//                 [--assembled 1: assemble K as the reference does] [--solves N] --out FILE
 //   tlfea_driver nesterov|adamw|adamw_nocoop --mesh tetgen:PREFIX|kuhn:.. [--steps 3] [--E ..]
 //                 [--rho0 ..] [--fix z0|x0] [--node-load NODE,FX] --out FILE
 //   any mode: [--mr MU10,MU01,KAPPA] switches the element to Mooney-Rivlin (SetMooneyRivlin)
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -130,10 +131,17 @@ int run(const std::string& mode, std::map<std::string, std::string>& a, MatrixXR
         LinearStaticSolver<TData> solver(b, num(a, "--tol", 1e-10), (int)num(a, "--max-iter", 50000));
         solver.UseAssembledOperator(num(a, "--assembled", 0) != 0);
         fea.AddSolver("linear_static", &solver);
-        fea.GetSolver("linear_static")->Solve();
+        // --solves N: N Solve() calls in a row (each re-linearises at the moved positions)
+        const int n_solves = std::max(1, (int)num(a, "--solves", 1));
+        std::string each;
+        for (int k = 0; k < n_solves; k++) {
+            fea.GetSolver("linear_static")->Solve();
+            each += std::string(k ? "," : "") + (solver.LastSolveWasMatrixFree() ? "true" : "false");
+        }
         extra = ", \"iters\": " + std::to_string(solver.GetLastIterCount()) +
                 ", \"residual\": " + sci(solver.GetLastResidual()) +
-                ", \"matrix_free\": " + (solver.LastSolveWasMatrixFree() ? "true" : "false");
+                ", \"matrix_free\": " + (solver.LastSolveWasMatrixFree() ? "true" : "false") +
+                ", \"matrix_free_each\": [" + each + "]";
     } else {
         b->CalcMassMatrix();
         b->CalcConstraintData();

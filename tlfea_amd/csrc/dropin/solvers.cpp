@@ -140,9 +140,11 @@ void LinearStaticSolver<TData>::Solve() {
     int iters = 0;
     double res = 0.0;
     int rc = TLFEA_OK;
-    // K(x) applied without assembly: no Kelvin-Voigt block, rest positions known
-    const bool matfree = !assembled_only_ && b.mat.eta_damp == 0.0 && b.mat.lambda_damp == 0.0 && b.force_plan && b.X0;
-    last_matfree_ = matfree;
+    // K(x) applied without assembly when the rest positions are known; with Kelvin-Voigt
+    // damping the operator is K + C, which the matrix-free CG applies at the rest positions
+    // only (it rejects the call elsewhere, and the tangent is then assembled)
+    const bool damp = b.mat.eta_damp != 0.0 || b.mat.lambda_damp != 0.0;
+    bool matfree = !assembled_only_ && b.force_plan && b.X0;
     if (matfree) {
         if (!jinv_) {
             const size_t nq = (size_t)b.quad.n;
@@ -158,7 +160,13 @@ void LinearStaticSolver<TData>::Solve() {
                   "LinearStaticSolver");
         rc = tlfea_cg_solve_matfree(cg_, b.force_plan, b.x, b.y, b.z, b.X0, b.Y0, b.Z0, jinv_, b.detJ_t, affine_,
                                     &b.quad, &b.mat, mask_, f_, u_, cg_tol_, cg_max_iter_, 32, &iters, &res, nullptr);
-    } else {
+        if (damp && rc == TLFEA_EINVAL) {
+            matfree = false;
+            rc = TLFEA_OK;
+        }
+    }
+    last_matfree_ = matfree;
+    if (!matfree) {
         // assemble the tangent at the current positions (h = 1), eliminate fixed DOFs
         lib_check(tlfea_stiffness_assemble(b.pattern, b.nn, b.n_elem, b.conn, b.x, b.y, b.z, b.grad_t, b.detJ_t,
                                            &b.quad, &b.mat, 1.0, k_, nullptr),

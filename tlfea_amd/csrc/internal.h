@@ -89,6 +89,8 @@ struct TanMat {
     // 1: compressible Mooney-Rivlin (one body, FP64, no mass term) with these coefficients
     int mr = 0;
     T mu10 = T(0), mu01 = T(0), kappa = T(0);
+    // Kelvin-Voigt coefficients (one body, rest positions only): the operator is K + C
+    T eta = T(0), lam_d = T(0);
 };
 
 // The tangent material of one body: SVK (lambda, mu) or Mooney-Rivlin (tangent.hip).

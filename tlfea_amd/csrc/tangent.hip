@@ -547,6 +547,30 @@ int launch_tangent_partials(const tlfea_force_plan* pl, const double* x, const d
     T* part = reinterpret_cast<T*>(pl->d_part);  // d_part holds 3*n_partials doubles
     const bool has_mass = mat.elem_inst ? mat.inst_mc != nullptr : mat.mc != T(0);
     const int mass = has_mass ? (mat.mass_exact ? 2 : 1) : 0;
+    if (mat.eta != T(0) || mat.lam_d != T(0)) {
+        // Kelvin-Voigt: the reference adds C_ab = (eta Fh_b (x) Fh_a + eta (g_a.g_b) F F^T
+        // + lambda_d Fh_a (x) Fh_b) dV to K (FEAT10DataFunc.cuh:686-767).  At F = I that is
+        // eta g_b (x) g_a + eta (g_a.g_b) I + lambda_d g_a (x) g_b, the linear-elasticity
+        // block with (lambda_d, eta) for (lambda, mu); so K + C at rest is the rest-state
+        // operator with lambda + lambda_d and mu + eta (after Mooney-Rivlin's equivalent
+        // Lame parameters).  Away from rest the operator is not built without assembly.
+        if (!mat.at_rest || mat.elem_inst || has_mass) {
+            set_error("Kelvin-Voigt damping in the matrix-free tangent needs one body at its rest positions "
+                      "without a mass term");
+            return TLFEA_EINVAL;
+        }
+        TanMat<T> eq = mat;
+        if (eq.mr) {
+            eq.mr = 0;
+            eq.mu = T(2) * (mat.mu10 + mat.mu01);
+            eq.lam = mat.kappa - T(2) * eq.mu / T(3);
+        }
+        eq.lam += mat.lam_d;
+        eq.mu += mat.eta;
+        eq.eta = eq.lam_d = T(0);
+        return launch_tangent_partials<T>(pl, x, y, z, X0, Y0, Z0, jinv_t, detJ_t, quad, eq, mask, p, flags, s,
+                                          blk_list, n_list, dot);
+    }
     if (mat.mr) {
         // Mooney-Rivlin: at rest its tangent is linear elasticity with mu = 2 (mu10 + mu01)
         // and lambda = kappa - 2 mu / 3 (the rest-state forms run with those); elsewhere the
@@ -651,6 +675,8 @@ TanMat<double> body_tanmat(const tlfea_material& m, bool at_rest, bool affine) {
         t.mu01 = m.mu01;
         t.kappa = m.kappa;
     }
+    t.eta = m.eta_damp;
+    t.lam_d = m.lambda_damp;
     return t;
 }
 
@@ -828,9 +854,9 @@ extern "C" int tlfea_tangent_apply_planned(const tlfea_force_plan* plan, const d
     TLFEA_REQUIRE(plan && x && y && z && X0 && Y0 && Z0 && jinv_t && detJ_t && quad && mat && p && out,
                   "null argument");
     TLFEA_REQUIRE(quad->n == (plan->nn == 10 ? 5 : 1), "quadrature size does not match the element type");
-    TLFEA_REQUIRE(mat->eta_damp == 0.0 && mat->lambda_damp == 0.0, "the matrix-free tangent excludes damping");
     TLFEA_REQUIRE_MODEL(*mat);
-    // the linear-elasticity form when the positions are the rest positions' own arrays
+    // the linear-elasticity form when the positions are the rest positions' own arrays (the
+    // only form that takes Kelvin-Voigt damping: K + C)
     const bool same = x == X0 && y == Y0 && z == Z0;
     return launch_tangent_planned(plan, x, y, z, X0, Y0, Z0, jinv_t, detJ_t, *quad, *mat, fixed_dof_mask, p, out,
                                   nullptr, nullptr, as_stream(stream), same, affine != 0);

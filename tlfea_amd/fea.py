@@ -181,15 +181,16 @@ class Body:
             self._affine = bool(a.value)
         return self._affine
 
-    def tangent_apply(self, p, x=None, out=None, masked=True, affine=True):
+    def tangent_apply(self, p, x=None, out=None, masked=True, affine=True, damping=False):
         """y = K(x) p without assembly; with masked=True fixed rows return p and fixed
         columns are dropped (the Dirichlet-eliminated operator); affine: on a mesh of
-        straight-sided elements one quadrature point's M and det J are read."""
+        straight-sided elements one quadrature point's M and det J are read; damping: the
+        reference's K + C with the body's Kelvin-Voigt coefficients (x must be X0)."""
         if self.plan is None:
             raise RuntimeError("matrix-free tangent needs a force plan (mesh too scattered for the LDS budget)")
         x = self.x if x is None else x
         out = torch.empty(3 * self.n_nodes, dtype=torch.float64, device=DEV) if out is None else out
-        mat = self.elastic_material()
+        mat = self.mat if damping else self.elastic_material()
         X0 = self.X0
         call("tlfea_tangent_apply_planned", self.plan, ptr(x[0]), ptr(x[1]), ptr(x[2]), ptr(X0[0]), ptr(X0[1]),
              ptr(X0[2]), ptr(self.jinv_t), ptr(self.detJ_t), int(affine and self.affine),
@@ -296,7 +297,9 @@ class LinearStatic:
         """mixed=True: FP32 Krylov recursion with FP64 accumulation and reliable updates
         (tlfea_cg_solve_mixed); tol is then the true FP64 relative residual.
         matfree=True: FP64 CG with K(x) p applied element by element
-        (tlfea_cg_solve_matfree); nothing is assembled."""
+        (tlfea_cg_solve_matfree); nothing is assembled.  The operator is the assembled
+        path's: K + C with Kelvin-Voigt damping, which the matrix-free CG applies at the
+        rest positions only (it raises elsewhere)."""
         if mixed and matfree:
             raise ValueError("mixed and matfree are exclusive")
         self.b = body
@@ -341,7 +344,7 @@ class LinearStatic:
             self.refreshes = nref.value
         elif self.matfree:
             b = self.b
-            mat = b.elastic_material()
+            mat = b.mat
             X0 = b.X0
             rc = capi.lib().tlfea_cg_solve_matfree(self.cg, b.plan, ptr(b.x[0]), ptr(b.x[1]), ptr(b.x[2]),
                                                    ptr(X0[0]), ptr(X0[1]), ptr(X0[2]), ptr(b.jinv_t),
