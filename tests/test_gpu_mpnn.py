@@ -133,5 +133,11 @@ def test_warm_start_on_held_out_beams(gpu):
     zero_net = WarmStart(params=np.zeros_like(init_params()))
     u00 = zero_net.guess(b, graph=g)
     assert zero_net.fell_back.all() and torch.count_nonzero(u00).item() == 0
-    b.solve(tol=1e-8, u0=u00)
-    assert np.all(np.abs(b.iters - it_zero) <= np.maximum(3, 0.05 * it_zero))
+    u_fb = b.solve(tol=1e-8, u0=u00)
+    for i in range(b.n_inst):
+        sl = b.inst_slice(i)
+        assert (torch.linalg.norm(u_fb[sl] - u_zero[sl]) / torch.linalg.norm(u_zero[sl])).item() < 1e-6
+    # the same zero-start solve twice: the tangent's in-block LDS atomics change the
+    # summation order between runs, and on these beams (kappa up to 1e9) finite-precision
+    # CG counts move with it (measured up to 8.6%: 12,869 against 14,074)
+    assert np.all(np.abs(b.iters - it_zero) <= np.maximum(3, 0.2 * it_zero))

@@ -208,9 +208,9 @@ def test_tangent_is_force_derivative(gpu, name):
     assert rel_err(kp, fd) < 1e-7
 
 
-def test_tangent_rejects_damping(gpu):
-    """With Kelvin-Voigt damping the operator K + C exists only at the rest positions: a
-    call whose positions are other arrays than X0 is rejected."""
+def test_tangent_rejects_damped_mooney_rivlin_away_from_rest(gpu):
+    """With Kelvin-Voigt damping the Mooney-Rivlin operator K + C exists only at the rest
+    positions: a call whose positions are other arrays than X0 is rejected."""
     import ctypes as C
 
     import torch
@@ -218,6 +218,7 @@ def test_tangent_rejects_damping(gpu):
     from tlfea_amd.capi import ptr
     m = load_mesh("cube")
     b = _body(m, eta=1e3)
+    b.set_mooney_rivlin(2e8, 5e7, 3e9)
     p = torch.zeros(3 * m.n_nodes, dtype=torch.float64, device="cuda")
     out = torch.empty_like(p)
     X0 = b.X0
@@ -232,20 +233,20 @@ def test_tangent_rejects_damping(gpu):
 ETA_D, LD_D = 0.25 * lame(E_STEEL, NU_STEEL)[1], 0.5 * lame(E_STEEL, NU_STEEL)[0]
 
 
-def _damped_oracle(m):
+def _damped_oracle(m, xyz=(None, None, None)):
     go, djo = O.grad(m)
     lam, mu = lame(E_STEEL, NU_STEEL)
     doff, dcol = O.dof_pattern(m)
-    return doff, dcol, O.stiffness_values(m, go, djo, lam, mu, doff, dcol, eta=ETA_D, lambda_d=LD_D)
+    return doff, dcol, O.stiffness_values(m, go, djo, lam, mu, doff, dcol, *xyz, eta=ETA_D, lambda_d=LD_D)
 
 
 @pytest.mark.parametrize("name", ["cube", "C1", "kuhn_t4"])
-def test_damped_tangent_at_rest(gpu, name):
-    """K + C applied at the rest positions (the rest-state operator with lambda + lambda_d,
-    mu + eta) against the oracle's assembled damped tangent (FEAT10DataFunc.cuh:512-767)
-    times p, 1e-12; the same body away from rest is rejected."""
+def test_damped_tangent(gpu, name):
+    """K + C (FEAT10DataFunc.cuh:512-767) applied without assembly against the oracle's
+    assembled damped tangent times p, 1e-12: at the rest positions (the rest-state operator
+    with lambda + lambda_d, mu + eta) and at deformed positions (the general form, C p =
+    F (lambda_d tr(dE) I + 2 eta dE) added to its material term)."""
     import torch
-    from tlfea_amd import capi
     print(f"[damped tangent] {name}", flush=True)
     m = load_mesh(name)
     b = _body(m, eta=ETA_D, lambda_d=LD_D)
@@ -257,35 +258,36 @@ def test_damped_tangent_at_rest(gpu, name):
     assert rel_err(kp, ref) < 1e-12
     k_only = _np(b.tangent_apply(pt, x=b.X0, masked=False))
     assert rel_err(k_only, ref) > 1e-2  # the damping block is really in it
-    with pytest.raises(capi.TlfeaError):
-        b.tangent_apply(pt, masked=False, damping=True)
+    xyz = deformed(m, seed=5, amp=2e-2)
+    xt = [torch.as_tensor(a).cuda() for a in xyz]
+    kpx = _np(b.tangent_apply(pt, x=xt, masked=False, damping=True))
+    _, _, vx = _damped_oracle(m, xyz)
+    assert rel_err(kpx, O.spmv(doff, dcol, vx, p)) < 1e-12
 
 
 @pytest.mark.parametrize("name", ["cube", "C1"])
 def test_damped_matfree_cg_against_direct(gpu, name):
     """A damped body's linear-static system is (K + C) u = f in the reference (the block is
-    added unscaled, h = 1): the matrix-free CG at rest and the assembled CG both against the
-    oracle's banded Cholesky solve of it; once the positions have moved the matrix-free CG
-    rejects the damped operator (no iteration runs)."""
+    added unscaled, h = 1): the matrix-free CG and the assembled CG both against the
+    oracle's banded Cholesky solve of it, at the rest positions and at moved ones."""
     import torch
-    from tlfea_amd import capi
     from tlfea_amd.fea import LinearStatic
     print(f"[damped matfree cg] {name}", flush=True)
     m = load_mesh(name)
     b = _body(m, eta=ETA_D, lambda_d=LD_D)
-    doff, dcol, vo = _damped_oracle(m)
-    vo, fo = O.apply_dirichlet(doff, dcol, vo, m.f_ext, m.fixed)
-    ud = O.band_cholesky_solve(doff, dcol, vo, fo)
-    for matfree in (True, False):
-        ls = LinearStatic(b, tol=1e-13, max_iter=200000, matfree=matfree)
-        ls.assemble()
-        assert ls.cg_solve() == 0, (matfree, ls.iters, ls.res)
-        assert rel_err(_np(ls.u), ud) < CG_DISP_BAR[name], matfree
-    b.x[2].add_(torch.full_like(b.x[2], 1e-4))
-    moved = LinearStatic(b, tol=1e-8, max_iter=100, matfree=True)
-    moved.assemble()
-    with pytest.raises(capi.TlfeaError):
-        moved.cg_solve()
+    for moved in (False, True):
+        if moved:
+            xyz = deformed(m, seed=9, amp=1e-2)
+            for d in range(3):
+                b.x[d].copy_(torch.as_tensor(xyz[d]))
+        doff, dcol, vo = _damped_oracle(m, xyz if moved else (None, None, None))
+        vo, fo = O.apply_dirichlet(doff, dcol, vo, m.f_ext, m.fixed)
+        ud = O.band_cholesky_solve(doff, dcol, vo, fo)
+        for matfree in (True, False):
+            ls = LinearStatic(b, tol=1e-13, max_iter=200000, matfree=matfree)
+            ls.assemble()
+            assert ls.cg_solve() == 0, (moved, matfree, ls.iters, ls.res)
+            assert rel_err(_np(ls.u), ud) < CG_DISP_BAR[name], (moved, matfree)
 
 
 CG_DISP_BAR = {"cube": 1e-10, "beam_vtu": 1e-10, "C1": 1e-10, "beam_highres_t4": 20 * 3.2e7 * 2.2e-16}

@@ -44,8 +44,8 @@ template <int NN, int NQ, typename T, bool STD, int MASS, bool LIN, bool AFF, bo
 __device__ __forceinline__ void tangent_qp(int q, int64_t tile, int lane, const int (&sl)[NN], const T* sU,
                                            const T* sP, int max_u, const T* __restrict__ jinv_t,
                                            const T* __restrict__ detJ_t, const DnTable<T>& dn, const tlfea_quad& quad,
-                                           T lam, T mu, T kap, T mc, T ks, T (&Mn)[9], T& dJn, T& vol,
-                                           T (&fa)[NN][3]) {
+                                           T lam, T mu, T kap, T eta, T lamd, T mc, T ks, T (&Mn)[9], T& dJn,
+                                           T& vol, T (&fa)[NN][3]) {
     // keep the node values in LDS rather than registers across quadrature points (frees
     // ~120 VGPRs in FP64, i.e. one more wave per SIMD)
     __asm__ volatile("" ::: "memory");
@@ -152,8 +152,10 @@ __device__ __forceinline__ void tangent_qp(int q, int64_t tile, int lane, const 
         for (int i = 0; i < 3; i++)
 #pragma unroll
             for (int j = 0; j < 3; j++) {
+                // the Kelvin-Voigt block C p = F (lambda_d tr(dE) I + 2 eta dE) joins the
+                // material term (FEAT10DataFunc.cuh:686-767 with dF = sum_b p_b (x) g_b)
                 S[i][j] = T(2) * mu * Em[i][j] + (i == j ? lam * trE : T(0));
-                dS[i][j] = T(2) * mu * dE[i][j] + (i == j ? lam * trdE : T(0));
+                dS[i][j] = T(2) * (mu + eta) * dE[i][j] + (i == j ? (lam + lamd) * trdE : T(0));
             }
         // dP = dF S + (I + H) dS, scaled by dV
 #pragma unroll
@@ -393,13 +395,14 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent(
 #pragma unroll
             for (int q = 0; q < NQ; q++)
                 tangent_qp<NN, NQ, T, true, MASS, LIN, AFF, MR>(q, tile, lane, sl, sU, sP, max_u, jinv_t, detJ_t, dn,
-                                                                quad, lam, mu, mat.kappa, mc, ks, Mn, dJn, vol, fa);
+                                                                quad, lam, mu, mat.kappa, mat.eta, mat.lam_d, mc, ks,
+                                                                Mn, dJn, vol, fa);
         } else {
 #pragma unroll 1
             for (int q = 0; q < NQ; q++)
                 tangent_qp<NN, NQ, T, false, MASS, LIN, false, MR>(q, tile, lane, sl, sU, sP, max_u, jinv_t, detJ_t,
-                                                                   dn, quad, lam, mu, mat.kappa, mc, ks, Mn, dJn, vol,
-                                                                   fa);
+                                                                   dn, quad, lam, mu, mat.kappa, mat.eta, mat.lam_d, mc,
+                                                                   ks, Mn, dJn, vol, fa);
         }
         if constexpr (MASS == 2) {
             // exact element mass rho V C applied to the element's p, one component at a time
@@ -549,27 +552,31 @@ int launch_tangent_partials(const tlfea_force_plan* pl, const double* x, const d
     const int mass = has_mass ? (mat.mass_exact ? 2 : 1) : 0;
     if (mat.eta != T(0) || mat.lam_d != T(0)) {
         // Kelvin-Voigt: the reference adds C_ab = (eta Fh_b (x) Fh_a + eta (g_a.g_b) F F^T
-        // + lambda_d Fh_a (x) Fh_b) dV to K (FEAT10DataFunc.cuh:686-767).  At F = I that is
-        // eta g_b (x) g_a + eta (g_a.g_b) I + lambda_d g_a (x) g_b, the linear-elasticity
-        // block with (lambda_d, eta) for (lambda, mu); so K + C at rest is the rest-state
-        // operator with lambda + lambda_d and mu + eta (after Mooney-Rivlin's equivalent
-        // Lame parameters).  Away from rest the operator is not built without assembly.
-        if (!mat.at_rest || mat.elem_inst || has_mass) {
-            set_error("Kelvin-Voigt damping in the matrix-free tangent needs one body at its rest positions "
-                      "without a mass term");
+        // + lambda_d Fh_a (x) Fh_b) dV to K (FEAT10DataFunc.cuh:686-767), i.e.
+        // C p = F (lambda_d tr(dE) I + 2 eta dE) per point with dE = sym(F^T dF): the SVK
+        // material term with (lambda_d, eta).  Away from rest the general form adds it to
+        // its dS (SVK only); at F = I it is the linear-elasticity block with (lambda_d, eta),
+        // so K + C at rest is the rest-state operator with lambda + lambda_d and mu + eta
+        // (after Mooney-Rivlin's equivalent Lame parameters).
+        if (mat.elem_inst || has_mass || (!mat.at_rest && mat.mr)) {
+            set_error("Kelvin-Voigt damping in the matrix-free tangent needs one body without a mass term "
+                      "(and, for Mooney-Rivlin, the rest positions)");
             return TLFEA_EINVAL;
         }
-        TanMat<T> eq = mat;
-        if (eq.mr) {
-            eq.mr = 0;
-            eq.mu = T(2) * (mat.mu10 + mat.mu01);
-            eq.lam = mat.kappa - T(2) * eq.mu / T(3);
+        if (mat.at_rest) {
+            TanMat<T> eq = mat;
+            if (eq.mr) {
+                eq.mr = 0;
+                eq.mu = T(2) * (mat.mu10 + mat.mu01);
+                eq.lam = mat.kappa - T(2) * eq.mu / T(3);
+            }
+            eq.lam += mat.lam_d;
+            eq.mu += mat.eta;
+            eq.eta = eq.lam_d = T(0);
+            return launch_tangent_partials<T>(pl, x, y, z, X0, Y0, Z0, jinv_t, detJ_t, quad, eq, mask, p, flags, s,
+                                              blk_list, n_list, dot);
         }
-        eq.lam += mat.lam_d;
-        eq.mu += mat.eta;
-        eq.eta = eq.lam_d = T(0);
-        return launch_tangent_partials<T>(pl, x, y, z, X0, Y0, Z0, jinv_t, detJ_t, quad, eq, mask, p, flags, s,
-                                          blk_list, n_list, dot);
+        // away from rest eta and lambda_d ride into the general form's dS (SVK)
     }
     if (mat.mr) {
         // Mooney-Rivlin: at rest its tangent is linear elasticity with mu = 2 (mu10 + mu01)
