@@ -277,7 +277,9 @@ def test_damped_matfree_cg_against_direct(gpu, name):
     b = _body(m, eta=ETA_D, lambda_d=LD_D)
     for moved in (False, True):
         if moved:
-            xyz = deformed(m, seed=9, amp=1e-2)
+            # strains of ~0.2% (a 1% perturbation of the node positions makes K + C of the
+            # slender C1 beam indefinite)
+            xyz = deformed(m, seed=9, amp=1e-4)
             for d in range(3):
                 b.x[d].copy_(torch.as_tensor(xyz[d]))
         doff, dcol, vo = _damped_oracle(m, xyz if moved else (None, None, None))
