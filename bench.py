@@ -66,13 +66,13 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes of one matrix-free tangent launch pair (k_tangent + k_gather_tangent) on the
 # 200x20x20 beam: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (KiB), scales calibrated
 # on a 512 MiB copy
-PMC_TANGENT_BYTES = (77026 * 2.0 + 29110 + 30785 * 2.0 + 21492) * 1024
+PMC_TANGENT_BYTES = (51220 * 2.0 + 29066 + 29871 * 2.0 + 21776) * 1024
 # the same for the rest-state (linear-elasticity) form the linear-static step runs
-PMC_TANGENT_REST_BYTES = (43713 * 2.0 + 29109 + 30785 * 2.0 + 21492) * 1024
+PMC_TANGENT_REST_BYTES = (35012 * 2.0 + 29066 + 29871 * 2.0 + 21776) * 1024
 # the same for the force kernels with k_gather_partials: gradient-reading (k_force_planned)
 # and rest-coordinate (k_force_rest)
-PMC_FORCE_BYTES = (314624 * 2.0 + 29112 + 21890 * 2.0 + 15799) * 1024
-PMC_FORCE_REST_BYTES = (40677 * 2.0 + 29110 + 21890 * 2.0 + 15799) * 1024
+PMC_FORCE_BYTES = (306420 * 2.0 + 29085 + 20867 * 2.0 + 15799) * 1024
+PMC_FORCE_REST_BYTES = (23356 * 2.0 + 29066 + 20867 * 2.0 + 15799) * 1024
 
 
 def log(*a):
