@@ -25,8 +25,7 @@ class LinearStaticSolver : public SolverBase {
     Real GetLastResidual() const { return last_residual_; }
     // Not in the reference.  Solve applies K(x) without assembling it (the same operator
     // and CG stopping rule, ~5x faster on large meshes) whenever the element's rest
-    // positions are known (CalcDnDuPre) -- with Kelvin-Voigt damping (operator K + C) for
-    // Mooney-Rivlin only while the positions are the rest positions; otherwise, or with
+    // positions are known (CalcDnDuPre), with Kelvin-Voigt damping as K + C; otherwise, or with
     // UseAssembledOperator(true), it assembles K (+ C) as the reference does.
     void UseAssembledOperator(bool on) { assembled_only_ = on; }
     bool LastSolveWasMatrixFree() const { return last_matfree_; }

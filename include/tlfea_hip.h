@@ -145,8 +145,7 @@ int tlfea_internal_force_rest(const tlfea_force_plan* plan, const double* x, con
  * mask.  With Kelvin-Voigt damping in mat the operator is the reference's K + C
  * (FEAT10DataFunc.cuh:686-767): C p = F (lambda_d tr(dE) I + 2 eta dE) per point, dE as
  * above, added to dS; at F = I that makes K + C the linear-elasticity operator with
- * lambda + lambda_d and mu + eta.  Mooney-Rivlin with damping needs x, y, z = X0, Y0, Z0
- * (TLFEA_EINVAL otherwise).
+ * lambda + lambda_d and mu + eta (Mooney-Rivlin: the same C p added to A : dF).
  * affine (as reported by tlfea_jinv_affine; 0 is always valid): M and det J of quadrature
  * point 0 only are read.  When x, y, z are the arrays X0, Y0, Z0 themselves the kernel
  * runs its linear-elasticity form (u = 0: H = S = 0, dP = dS), the same values without
@@ -255,9 +254,7 @@ int tlfea_constraint_eval(int n_fixed, const int* fixed_nodes, const double* x, 
  * operator K(x) applied by tlfea_tangent_apply_planned instead of an assembled matrix:
  * the linear-static system of LinearStaticSolver::Solve after Dirichlet elimination.
  * When x equals X0 on every node (checked once per call) the operator is applied in its
- * linear-elasticity form.  With Kelvin-Voigt damping the operator is K + C; for
- * Mooney-Rivlin with damping the call needs x = X0 and returns TLFEA_EINVAL before any
- * iteration otherwise. */
+ * linear-elasticity form.  With Kelvin-Voigt damping the operator is K + C. */
 int tlfea_cg_solve_matfree(tlfea_cg* c, const tlfea_force_plan* plan, const double* x, const double* y,
                            const double* z, const double* X0, const double* Y0, const double* Z0,
                            const double* jinv_t, const double* detJ_t, int affine, const tlfea_quad* quad,

@@ -199,6 +199,30 @@ def test_mr_tangent_apply(gpu, name):
     assert rel_err(_np(b.tangent_apply(pt, affine=False)), ref) < 1e-12
 
 
+@pytest.mark.parametrize("name", ["cube", "C1"])
+def test_mr_damped_tangent_apply(gpu, name):
+    """A damped Mooney-Rivlin body's K + C without assembly (C p = F (lambda_d tr(dE) I +
+    2 eta dE) added to A : dF) against the oracle's assembled damped Mooney-Rivlin tangent
+    times p, at rest and at a deformed state, 1e-12."""
+    import torch
+    m = load_mesh(name)
+    eta, ld = 1e10, 2e10
+    b = _body(m, eta=eta, lambda_d=ld)
+    go, djo = O.grad(m)
+    doff, dcol = O.dof_pattern(m)
+    p = np.random.default_rng(37).standard_normal(3 * m.n_nodes)
+    pt = torch.as_tensor(p).cuda()
+    x, y, z = deformed(m, seed=38, amp=AMP[name] / 2)
+    with O.material("mr", *MR):
+        vr = O.stiffness_values(m, go, djo, 0.0, 0.0, doff, dcol, eta=eta, lambda_d=ld)
+        vd = O.stiffness_values(m, go, djo, 0.0, 0.0, doff, dcol, x, y, z, eta=eta, lambda_d=ld)
+    assert rel_err(_np(b.tangent_apply(pt, x=b.X0, masked=False, damping=True)), O.spmv(doff, dcol, vr, p)) < 1e-12
+    xt = [torch.as_tensor(a).cuda() for a in (x, y, z)]
+    ref = O.spmv(doff, dcol, vd, p)
+    assert rel_err(_np(b.tangent_apply(pt, x=xt, masked=False, damping=True)), ref) < 1e-12
+    assert rel_err(_np(b.tangent_apply(pt, x=xt, masked=False)), ref) > 1e-3  # the block is in it
+
+
 def test_mr_matfree_cg_against_direct(gpu):
     """Linear-static solve of a Mooney-Rivlin C1 beam through the matrix-free CG against the
     oracle's banded Cholesky solve of the same tangent, to the assembled CG's bar."""

@@ -140,10 +140,8 @@ void LinearStaticSolver<TData>::Solve() {
     int iters = 0;
     double res = 0.0;
     int rc = TLFEA_OK;
-    // K(x) applied without assembly when the rest positions are known; with Kelvin-Voigt
-    // damping the operator is K + C, which the matrix-free CG applies everywhere for SVK and
-    // at the rest positions for Mooney-Rivlin (it rejects the call elsewhere, and the
-    // tangent is then assembled)
+    // K(x) applied without assembly when the rest positions are known (with Kelvin-Voigt
+    // damping the operator is K + C); a rejected call falls back to the assembled tangent
     const bool damp = b.mat.eta_damp != 0.0 || b.mat.lambda_damp != 0.0;
     bool matfree = !assembled_only_ && b.force_plan && b.X0;
     if (matfree) {

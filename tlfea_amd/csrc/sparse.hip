@@ -1585,9 +1585,6 @@ extern "C" int tlfea_cg_solve_matfree(tlfea_cg* c, const tlfea_force_plan* plan,
     // the check uses the flag word before the initialisation resets it
     bool at_rest = false;
     if (int rc = positions_at_rest(plan->n_nodes, x, y, z, X0, Y0, Z0, c->d_flags, s, &at_rest)) return rc;
-    // with Kelvin-Voigt damping the operator is K + C; away from rest only for SVK
-    TLFEA_REQUIRE(at_rest || mat->model == 0 || (mat->eta_damp == 0.0 && mat->lambda_damp == 0.0),
-                  "Kelvin-Voigt damping with Mooney-Rivlin in the matrix-free CG needs x = X0");
     // s = K p of the single-reduction iteration starts at zero (p_0 = r_0 with beta = 0)
     k_cg_init_masked<<<gv, RED_THREADS, 0, s>>>(n, f, fixed_dof_mask, u, c->d_r, c->d_p, c->d_part, c->d_Kp);
     k_cg_init_fin<<<1, RED_THREADS, 0, s>>>(gv, c->d_part, c->d_scal, c->d_flags);

@@ -120,6 +120,25 @@ __device__ __forceinline__ void tangent_qp(int q, int64_t tile, int lane, const 
                 dF[i][j] = Jp[i][0] * M[j][0] + Jp[i][1] * M[j][1] + Jp[i][2] * M[j][2];
             }
         mr_directional(F, dF, lam, mu, kap, dPm);
+        if (eta != T(0) || lamd != T(0)) {
+            // Kelvin-Voigt block: C p = F (lambda_d tr(dE) I + 2 eta dE), dE = sym(F^T dF)
+            double dE[3][3], Sv[3][3];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+                    dE[i][j] = 0.5 * (dF[0][i] * F[0][j] + dF[1][i] * F[1][j] + dF[2][i] * F[2][j] +
+                                      F[0][i] * dF[0][j] + F[1][i] * dF[1][j] + F[2][i] * dF[2][j]);
+            const double trdE = dE[0][0] + dE[1][1] + dE[2][2];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) Sv[i][j] = 2.0 * eta * dE[i][j] + (i == j ? lamd * trdE : 0.0);
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) dPm[i][j] += F[i][0] * Sv[0][j] + F[i][1] * Sv[1][j] + F[i][2] * Sv[2][j];
+        }
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
@@ -554,13 +573,12 @@ int launch_tangent_partials(const tlfea_force_plan* pl, const double* x, const d
         // Kelvin-Voigt: the reference adds C_ab = (eta Fh_b (x) Fh_a + eta (g_a.g_b) F F^T
         // + lambda_d Fh_a (x) Fh_b) dV to K (FEAT10DataFunc.cuh:686-767), i.e.
         // C p = F (lambda_d tr(dE) I + 2 eta dE) per point with dE = sym(F^T dF): the SVK
-        // material term with (lambda_d, eta).  Away from rest the general form adds it to
-        // its dS (SVK only); at F = I it is the linear-elasticity block with (lambda_d, eta),
-        // so K + C at rest is the rest-state operator with lambda + lambda_d and mu + eta
-        // (after Mooney-Rivlin's equivalent Lame parameters).
-        if (mat.elem_inst || has_mass || (!mat.at_rest && mat.mr)) {
-            set_error("Kelvin-Voigt damping in the matrix-free tangent needs one body without a mass term "
-                      "(and, for Mooney-Rivlin, the rest positions)");
+        // material term with (lambda_d, eta).  Away from rest the general forms add it (to
+        // dS for SVK, to A:dF for Mooney-Rivlin); at F = I it is the linear-elasticity block
+        // with (lambda_d, eta), so K + C at rest is the rest-state operator with
+        // lambda + lambda_d and mu + eta (after Mooney-Rivlin's equivalent Lame parameters).
+        if (mat.elem_inst || has_mass) {
+            set_error("Kelvin-Voigt damping in the matrix-free tangent needs one body without a mass term");
             return TLFEA_EINVAL;
         }
         if (mat.at_rest) {
@@ -576,7 +594,7 @@ int launch_tangent_partials(const tlfea_force_plan* pl, const double* x, const d
             return launch_tangent_partials<T>(pl, x, y, z, X0, Y0, Z0, jinv_t, detJ_t, quad, eq, mask, p, flags, s,
                                               blk_list, n_list, dot);
         }
-        // away from rest eta and lambda_d ride into the general form's dS (SVK)
+        // away from rest eta and lambda_d ride into the general forms
     }
     if (mat.mr) {
         // Mooney-Rivlin: at rest its tangent is linear elasticity with mu = 2 (mu10 + mu01)

@@ -185,8 +185,7 @@ class Body:
         """y = K(x) p without assembly; with masked=True fixed rows return p and fixed
         columns are dropped (the Dirichlet-eliminated operator); affine: on a mesh of
         straight-sided elements one quadrature point's M and det J are read; damping: the
-        reference's K + C with the body's Kelvin-Voigt coefficients (Mooney-Rivlin: x must
-        be X0)."""
+        reference's K + C with the body's Kelvin-Voigt coefficients."""
         if self.plan is None:
             raise RuntimeError("matrix-free tangent needs a force plan (mesh too scattered for the LDS budget)")
         x = self.x if x is None else x
@@ -299,8 +298,7 @@ class LinearStatic:
         (tlfea_cg_solve_mixed); tol is then the true FP64 relative residual.
         matfree=True: FP64 CG with K(x) p applied element by element
         (tlfea_cg_solve_matfree); nothing is assembled.  The operator is the assembled
-        path's: K + C with Kelvin-Voigt damping (for Mooney-Rivlin at the rest positions
-        only: the matrix-free CG raises elsewhere)."""
+        path's: K + C with Kelvin-Voigt damping."""
         if mixed and matfree:
             raise ValueError("mixed and matfree are exclusive")
         self.b = body
