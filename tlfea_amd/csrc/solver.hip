@@ -309,39 +309,38 @@ __global__ void k_adam_outer_init(int64_t n3, double* __restrict__ g, double* __
 __global__ void k_adam_predict(int n_nodes, int nocoop, double lr, double c1, double c2, double beta1,
                                double beta2, double eps, double wd, double dt, const double* __restrict__ g,
                                double* __restrict__ m1, double* __restrict__ m2, double* __restrict__ vg,
-                               const double* __restrict__ vp, dv4* __restrict__ dv, const double* __restrict__ xp,
-                               const double* __restrict__ yp, const double* __restrict__ zp, double* __restrict__ x, double* __restrict__ y,
-                               double* __restrict__ z, double* __restrict__ f_int, const int* __restrict__ flags) {
+                               const double* __restrict__ vp, double* __restrict__ dv, const double* __restrict__ xp,
+                               const double* __restrict__ yp, const double* __restrict__ zp, double* __restrict__ x,
+                               double* __restrict__ y, double* __restrict__ z, double* __restrict__ f_int,
+                               const int* __restrict__ flags) {
     if (flags[0] | flags[1]) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_nodes) return;
-    double yv[3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        const int64_t k = 3 * (int64_t)i + d;
-        const double gi = g[k], v0 = vg[k];
-        double mt, vt, mh, vh;
-        if (nocoop) {
-            mt = beta1 * m1[k] + (1.0 - beta1) * gi;
-            vt = beta2 * m2[k] + (1.0 - beta2) * gi * gi;
-            mh = mt * c1;
-            vh = vt * c2;
-        } else {
-            mt = beta1 * m1[k] + (1 - beta1) * gi;
-            vt = beta2 * m2[k] + (1 - beta2) * gi * gi;
-            mh = mt / c1;
-            vh = vt / c2;
-        }
-        m1[k] = mt;
-        m2[k] = vt;
-        yv[d] = v0 - lr * (mh / (sqrt(vh) + eps) + wd * v0);
-        vg[k] = yv[d];
-        if (f_int) f_int[k] = 0.0;
+    // one thread per DOF: the five per-DOF streams are read and written coalesced
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= 3 * (int64_t)n_nodes) return;
+    const int64_t i = k / 3;
+    const int d = (int)(k - 3 * i);
+    const double gi = g[k], v0 = vg[k];
+    double mt, vt, mh, vh;
+    if (nocoop) {
+        mt = beta1 * m1[k] + (1.0 - beta1) * gi;
+        vt = beta2 * m2[k] + (1.0 - beta2) * gi * gi;
+        mh = mt * c1;
+        vh = vt * c2;
+    } else {
+        mt = beta1 * m1[k] + (1 - beta1) * gi;
+        vt = beta2 * m2[k] + (1 - beta2) * gi * gi;
+        mh = mt / c1;
+        vh = vt / c2;
     }
-    dv[i] = dv4{yv[0] - vp[3 * i], yv[1] - vp[3 * i + 1], yv[2] - vp[3 * i + 2], 0.0};
-    x[i] = xp[i] + dt * yv[0];
-    y[i] = yp[i] + dt * yv[1];
-    z[i] = zp[i] + dt * yv[2];
+    m1[k] = mt;
+    m2[k] = vt;
+    const double yv = v0 - lr * (mh / (sqrt(vh) + eps) + wd * v0);
+    vg[k] = yv;
+    if (f_int) f_int[k] = 0.0;
+    dv[4 * i + d] = yv - vp[k];  // component d of the node's padded difference
+    double* const xo = d == 0 ? x : (d == 1 ? y : z);
+    const double* const xq = d == 0 ? xp : (d == 1 ? yp : zp);
+    xo[i] = xq[i] + dt * yv;
 }
 
 // ------------------------------------------------------------------------------------
@@ -658,10 +657,10 @@ extern "C" int tlfea_solver_solve(tlfea_solver* S, void* stream) {
                     c1 = 1 - std::pow(S->ap.beta1, t);
                     c2 = 1 - std::pow(S->ap.beta2, t);
                 }
-                k_adam_predict<<<gN, 256, 0, st>>>(N, nocoop ? 1 : 0, lr_k, c1, c2, S->ap.beta1, S->ap.beta2,
+                k_adam_predict<<<g3, 256, 0, st>>>(N, nocoop ? 1 : 0, lr_k, c1, c2, S->ap.beta1, S->ap.beta2,
                                                    S->ap.eps, S->ap.weight_decay, dt, S->g, S->m1, S->m2, S->v_guess,
-                                                   S->v_prev, reinterpret_cast<dv4*>(S->dv), S->xp, S->yp, S->zp,
-                                                   b.x, b.y, b.z, fzero, S->flags);
+                                                   S->v_prev, S->dv, S->xp, S->yp, S->zp, b.x, b.y, b.z, fzero,
+                                                   S->flags);
             }
             TLFEA_LAUNCH_CHECK();
             // element forces at the scratch positions, damping from v_guess; skipped once the
