@@ -104,30 +104,30 @@ __global__ void k_nes_outer_init(int64_t n3, const double* __restrict__ vg, doub
 }
 
 // y = v_k + beta (v_k - v_km1); v_guess = y; x = x_prev + dt y; f_int = 0 (f_int null: the
-// force launch overwrites it); rotate v_km1 <- v_k (SyncedNesterov.cu:152-175).  vk holds v_km1 on entry and v_k on exit; vn holds v_k.
+// force launch overwrites it); rotate v_km1 <- v_k (SyncedNesterov.cu:152-175).  vk holds
+// v_km1 on entry and v_k on exit; vn holds v_k.
 // dv = y - v_prev, the same difference the gradient's mass term forms per entry
 // (SyncedNesterov.cu:54-55), formed once per node for the gradient kernel's gathers.
 __global__ void k_nes_predict(int n_nodes, double beta, double dt, double* __restrict__ vk, const double* __restrict__ vn,
-                              double* __restrict__ vg, const double* __restrict__ vp, dv4* __restrict__ dv,
+                              double* __restrict__ vg, const double* __restrict__ vp, double* __restrict__ dv,
                               const double* __restrict__ xp, const double* __restrict__ yp,
                               const double* __restrict__ zp, double* __restrict__ x, double* __restrict__ y,
                               double* __restrict__ z, double* __restrict__ f_int, const int* __restrict__ flags) {
     if (flags[0] | flags[1]) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_nodes) return;
-    double yv[3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        const double k = vn[3 * i + d], km1 = vk[3 * i + d];
-        yv[d] = k + beta * (k - km1);
-        vg[3 * i + d] = yv[d];
-        vk[3 * i + d] = k;
-        if (f_int) f_int[3 * i + d] = 0.0;
-    }
-    dv[i] = dv4{yv[0] - vp[3 * i], yv[1] - vp[3 * i + 1], yv[2] - vp[3 * i + 2], 0.0};
-    x[i] = xp[i] + dt * yv[0];
-    y[i] = yp[i] + dt * yv[1];
-    z[i] = zp[i] + dt * yv[2];
+    // one thread per DOF: the per-DOF streams are read and written coalesced
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= 3 * (int64_t)n_nodes) return;
+    const int64_t i = k / 3;
+    const int d = (int)(k - 3 * i);
+    const double vkk = vn[k], km1 = vk[k];
+    const double yv = vkk + beta * (vkk - km1);
+    vg[k] = yv;
+    vk[k] = vkk;
+    if (f_int) f_int[k] = 0.0;
+    dv[4 * i + d] = yv - vp[k];  // component d of the node's padded difference
+    double* const xo = d == 0 ? x : (d == 1 ? y : z);
+    const double* const xq = d == 0 ? xp : (d == 1 ? yp : zp);
+    xo[i] = xq[i] + dt * yv;
 }
 
 // Per node row on the sliced mass layout:
@@ -639,9 +639,8 @@ extern "C" int tlfea_solver_solve(tlfea_solver* S, void* stream) {
                 const double t_next = 0.5 * (1.0 + std::sqrt(1.0 + 4.0 * t * t));
                 const double beta = (t - 1.0) / t_next;
                 t = t_next;
-                k_nes_predict<<<gN, 256, 0, st>>>(N, beta, dt, S->vk, S->vn, S->v_guess, S->v_prev,
-                                                  reinterpret_cast<dv4*>(S->dv), S->xp, S->yp, S->zp, b.x,
-                                                  b.y, b.z, fzero, S->flags);
+                k_nes_predict<<<g3, 256, 0, st>>>(N, beta, dt, S->vk, S->vn, S->v_guess, S->v_prev, S->dv, S->xp,
+                                                  S->yp, S->zp, b.x, b.y, b.z, fzero, S->flags);
             } else {
                 double lr_k, c1 = 0.0, c2 = 0.0;
                 if (nocoop) {
