@@ -444,7 +444,7 @@ void free_solver(tlfea_solver* s) {
     for (void* p : {(void*)s->v_guess, (void*)s->v_prev, (void*)s->lam, (void*)s->xp, (void*)s->yp, (void*)s->zp,
                     (void*)s->vk, (void*)s->vn, (void*)s->g, (void*)s->m1, (void*)s->m2, (void*)s->dv, (void*)s->cons_off,
                     (void*)s->cons_idx, (void*)s->node_fixed, (void*)s->part, (void*)s->scal, (void*)s->flags})
-        hipFree(p);
+        (void)hipFree(p);
     delete s;
 }
 
