@@ -33,7 +33,7 @@ def test_linear_static_dropin(gpu, tmp_path, spec, name, load, assembled):
     """examples/beam_linear_static.cc: clamp x = x_min, the load spread over the x = x_max
     nodes in -z, CG at tol 1e-12 -> displacements against the oracle's banded Cholesky solve
     within 1e-10 (kappa * eps allows it on these meshes), with the matrix-free operator (the
-    default for SVK without damping) and with the assembled one."""
+    default once CalcDnDuPre has recorded the rest positions) and with the assembled one."""
     tol = 1e-10
     info, pos = _run(["linear_static", "--mesh", spec, "--tol", "1e-12", "--max-iter", "200000", "--load", str(load),
                       "--assembled", "1" if assembled else "0"], tmp_path)
