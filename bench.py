@@ -30,9 +30,14 @@ which torch.cuda.Event records on):
                      76*node_nnz + 52*n_nodes;
   * force_kernel  -- the fused FP64 P + f_int (block-local plan: force + node gather) with
                      the gradients rebuilt from the rest positions (k_force_rest), on the
-                     reference byte model of a kernel that reads them, bytes/launch =
-                     1280*n_elem + 48*n_nodes (its own HBM traffic in `traffic`);
-  * force_kernel_gradients_read -- the same with the gradients read (k_force_planned).
+                     bytes it moves: connectivity 40 B per element, rest and current
+                     positions read (48 B) and f_int written (24 B) per node, bytes/launch =
+                     40*n_elem + 72*n_nodes;
+  * force_kernel_gradients_read -- the same with the gradients read (k_force_planned), on
+                     the reference byte model 1280*n_elem + 48*n_nodes (connectivity 40,
+                     gradients 1200, detJ 40 per element; x read and f written per node);
+  * hbm_copy      -- a 512 MiB device copy and a 512 MiB read-only reduction (torch), the
+                     bandwidth a plain stream reaches on this device, beside the 8 TB/s peak.
 cpu_baseline: the FP64 oracle on a bounded sample (a slice of the element assembly and
 10 CG iterations on the full matrix), extrapolated to one solve with the plain GPU CG's
 iteration count.
@@ -387,19 +392,33 @@ def main():
     spmv_bytes = 76 * node_nnz + 52 * m.n_nodes
     spmv_gbs = spmv_bytes / spmv_s / 1e9
     f = torch.zeros(3 * m.n_nodes, dtype=torch.float64, device="cuda")
-    # fused P + f_int on the reference byte model: connectivity 40 + gradients 1200 + detJ 40 B
-    # per element, x read and f written (48 B) per node -- for the kernel that reads the
-    # gradients and, as that model is stated for a kernel that rebuilds them, for the
-    # rest-coordinate kernel too
+    # fused P + f_int reading the gradients, on the reference byte model: connectivity 40 +
+    # gradients 1200 + detJ 40 B per element, x read and f written (48 B) per node
     force_bytes = 1280 * m.n_elem + 48 * m.n_nodes
     force_s = event_time(lambda: body.internal_force_planned(f=f))
     force_gbs = force_bytes / force_s / 1e9
+    # the rest-coordinate form reads no gradients: connectivity 40 B per element, rest and
+    # current positions read (48 B) and f written (24 B) per node
+    rest_bytes = 40 * m.n_elem + 72 * m.n_nodes
     rest_s = event_time(lambda: body.internal_force_rest(f=f))
-    rest_gbs = force_bytes / rest_s / 1e9
+    rest_gbs = rest_bytes / rest_s / 1e9
+    # what a plain stream reaches: a 512 MiB copy (read + write) and a read-only reduction
+    src = torch.randn(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    copy_s = event_time(lambda: dst.copy_(src))
+    acc = torch.empty((), dtype=torch.float64, device="cuda")
+    read_s = event_time(lambda: torch.sum(src, out=acc))
+    copy_gbs = 2 * src.numel() * 8 / copy_s / 1e9
+    read_gbs = src.numel() * 8 / read_s / 1e9
+    del src, dst
+    for tag, gbs in (("tangent", tan_gbs), ("tangent general", tgen_gbs), ("spmv", spmv_gbs),
+                     ("force (gradients read)", force_gbs), ("force (rest coordinates)", rest_gbs)):
+        # an algorithmic byte model above the peak would mean the kernel skips work it is charged for
+        assert gbs <= HBM_PEAK_GBS, f"{tag}: {gbs:.0f} GB/s exceeds the HBM peak: byte model is wrong"
     log(f"tangent (rest form) {tan_s * 1e6:.1f} us {tan_gbs:.0f} GB/s; tangent (general) {tgen_s * 1e6:.1f} us "
         f"{tgen_gbs:.0f} GB/s; spmv64 {spmv_s * 1e6:.1f} us {spmv_gbs:.0f} GB/s; "
         f"force (gradients read) {force_s * 1e6:.1f} us {force_gbs:.0f} GB/s; force (rest coordinates) "
-        f"{rest_s * 1e6:.1f} us {rest_gbs:.0f} GB/s")
+        f"{rest_s * 1e6:.1f} us {rest_gbs:.0f} GB/s; copy {copy_gbs:.0f} GB/s, read {read_gbs:.0f} GB/s")
 
     # ---- CPU baseline: oracle on a bounded sample -------------------------------------
     cpu = None
@@ -482,12 +501,15 @@ def main():
         "force_kernel": {"bound": "hbm", "achieved": rest_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": rest_gbs / HBM_PEAK_GBS, "traffic": PMC_FORCE_REST_BYTES if default else None,
                          "kernel": "k_force_rest<10,5,false> + k_gather_partials (gradients rebuilt from the rest "
-                                   "positions; achieved on the reference byte model of the gradient-reading kernel)",
-                         "avg_us": rest_s * 1e6, "bytes_per_launch": force_bytes},
+                                   "positions; on the bytes this kernel moves: connectivity, rest and current "
+                                   "positions, f_int)",
+                         "avg_us": rest_s * 1e6, "bytes_per_launch": rest_bytes},
         "force_kernel_gradients_read": {"bound": "hbm", "achieved": force_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                         "frac": force_gbs / HBM_PEAK_GBS, "traffic": PMC_FORCE_BYTES if default else None,
                                         "kernel": "k_force_planned<10,5,false> + k_gather_partials",
                                         "avg_us": force_s * 1e6, "bytes_per_launch": force_bytes},
+        "hbm_copy": {"copy_gbs": copy_gbs, "read_gbs": read_gbs, "peak": HBM_PEAK_GBS,
+                     "what": "torch 512 MiB device copy (read + write bytes) and read-only sum, HIP events"},
         "cpu_baseline": cpu,
         "velocity_solvers": velocity,
         "batched": batched,

@@ -87,6 +87,12 @@ class TetElementData : public ElementBase {
     // Builds the mass pattern / force plan on first use; CalcMassMatrix must have run.
     tlfea_body BodyDescriptor();
     const tlfea_pattern* Pattern();
+    // Host copy of the constraint Jacobian CSR that ConvertToCSR_ConstraintJac (J, rows =
+    // constraints) or ConvertToCSR_ConstraintJacT (J^T, rows = DOFs) built; the reference
+    // exposes these arrays to its kernels only (FEAT10Data.cuh:262-283).  Empty when the
+    // CSR has not been built since the last SetNodalFixed / UpdateNodalFixed.
+    void RetrieveConstraintJacobianCSRToCPU(bool transpose, std::vector<int>& offsets, std::vector<int>& columns,
+                                            std::vector<Real>& values);
     int nodes_per_element() const { return nn_; }
     void* Stream() const { return nullptr; }
 

@@ -24,4 +24,20 @@ int LoadVtuTet(const std::string& filename, MatrixXR& nodes, MatrixXi& elements)
 void GenerateKuhnBeam(int nx, int ny, int nz, Real lx, Real ly, Real lz, bool tet10, MatrixXR& nodes,
                       MatrixXi& elements);
 
+// The reference declares its mesh utilities in tlfea::ANCFCPUUtils (cpu_utils.h:8-163,
+// mesh_utils.h:242-275); its callers write ANCFCPUUtils::FEAT10_read_nodes(...).
+namespace ANCFCPUUtils {
+using tlfea::FEAT10_read_elements;
+using tlfea::FEAT10_read_nodes;
+using tlfea::FEAT10_remap_tetgen_indices;
+
+// Legacy ASCII VTK (POINTS at x, y, z; CELLS; CELL_TYPES 24 or 10; POINT_DATA VECTORS
+// displacement = (x, y, z) - nodes), the layout of mesh_utils.cc:962-1077.  Returns false
+// when the file cannot be opened or the sizes disagree.
+bool WriteFEAT10ToVTK(const std::string& filename, const MatrixXR& nodes, const MatrixXi& elements, const VectorXR& x,
+                      const VectorXR& y, const VectorXR& z);
+bool WriteFEAT4ToVTK(const std::string& filename, const MatrixXR& nodes, const MatrixXi& elements, const VectorXR& x,
+                     const VectorXR& y, const VectorXR& z);
+}  // namespace ANCFCPUUtils
+
 }  // namespace tlfea

@@ -33,6 +33,8 @@ Parity status of each function (how it is pinned, see tests/test_oracle.py):
 * ``nesterov_step`` -- the iterate sequence against an independent pure-Python
   restatement of the loop; one long outer iteration against the direct solve of the same
   implicit-Euler equation.
+* ``constraint_jacobian_csr`` -- J J^T = I, J^T equal to the transpose of J, J x = the
+  fixed DOFs of x, row sums of J^T equal to the fixed-DOF indicator.
 * ``adamw_step`` (coop and nocoop) -- the iterate sequence against torch.optim.AdamW driven
   by ``step_gradient`` (1e-10; a wrong bias-correction step, decay order, weight decay, eps
   or gradient lag fails it); the converged state against the direct solve.
@@ -347,6 +349,26 @@ def band_cholesky_solve(off, col, vals, b):
     return x
 
 
+def constraint_jacobian_csr(fixed, n_nodes, transpose=False):
+    """Selection Jacobian of the fixed-node constraints c = x - x_ref at the fixed nodes.
+
+    J (FEAT10Data.cu:420-437,478-506): row 3k + d holds one entry 1.0 at DOF
+    3 fixed[k] + d.  J^T (FEAT10Data.cu:439-473,510-562): rows are the 3N DOFs, row
+    3 fixed[k] + d holds 1.0 at column 3k + d (columns ascending within a row).
+    Returns (offsets, columns, values)."""
+    fixed = np.asarray(fixed, np.int64)
+    nc = 3 * len(fixed)
+    dofs = (3 * fixed[:, None] + np.arange(3)[None, :]).reshape(-1)  # constraint t -> DOF
+    if not transpose:
+        return np.arange(nc + 1, dtype=np.int32), dofs.astype(np.int32), np.ones(nc)
+    rows = 3 * int(n_nodes)
+    off = np.zeros(rows + 1, np.int64)
+    np.add.at(off, dofs + 1, 1)
+    off = np.cumsum(off)
+    order = np.lexsort((np.arange(nc), dofs))  # by DOF, then constraint index
+    return off.astype(np.int32), order.astype(np.int32), np.ones(nc)
+
+
 # ---------------------------------------------------------------------------------------
 class Problem:
     """Everything the implicit-Euler solvers need, held alive for ctypes."""
@@ -370,6 +392,14 @@ class Problem:
                               _d(self.qw), self.lam, self.mu, eta, lambda_d, _i(self.m_off), _i(self.m_col),
                               _d(self.m_val), _d(self.f_ext), len(self.fixed), _i(self.fixed), _d(self.xref),
                               _d(self.yref), _d(self.zref))
+
+    def set_fixed(self, fixed):
+        """Replaces the fixed-node list (FEAT10Data.cu:674-751); the constraint targets stay
+        the positions at construction.  The state's multipliers are the caller's: they keep
+        their constraint-number indexing, and a new count needs a new array."""
+        self.fixed = _i32(fixed)
+        self.model.n_fixed = len(self.fixed)
+        self.model.fixed = _i(self.fixed)
 
     def new_state(self):
         n = self.mesh.n_nodes

@@ -102,3 +102,131 @@ def test_iterative_dropin_on_cube(gpu, tmp_path, mode):
     # AdamW from rest amplifies rounding noise (see test_gpu_solvers); a 5% band there
     assert rel_err(du, duo) < (1e-6 if mode == "nesterov" else 5e-2)
     assert len(info["inner_iters"]) == 3
+
+
+def _write_raw(path, *arrays):
+    with open(path, "wb") as fh:
+        for a in arrays:
+            np.ascontiguousarray(a, dtype=np.float64).tofile(fh)
+
+
+@pytest.mark.parametrize("mode", ["nesterov", "adamw", "adamw_nocoop"])
+def test_iterative_dropin_trajectory(gpu, tmp_path, mode):
+    """The drop-in solver classes against the oracle on a well-conditioned trajectory (the
+    C-ABI tests' setting in test_gpu_solvers): steel cube, loads on every DOF, a deformed
+    start set by UpdatePositions after CalcDnDuPre, Kelvin-Voigt damping for AdamW, zero
+    tolerances so that both sides run exactly 2 Solve() x 2 outer x 25-30 inner iterations.
+    The wrapper's parameter plumbing, state reset and body refresh are checked at the
+    C-ABI bars (Nesterov 1e-8, AdamW 1e-6)."""
+    from helpers import E_STEEL, NU_STEEL, RHO_STEEL, deformed
+    m = load_mesh("cube")
+    load = np.random.default_rng(21).standard_normal(3 * m.n_nodes) * 50.0
+    x0, y0, z0 = deformed(m, seed=22, amp=2e-5, rot=0.0)
+    lf, sf = str(tmp_path / "load.bin"), str(tmp_path / "start.bin")
+    _write_raw(lf, load)
+    _write_raw(sf, x0, y0, z0)
+    eta, ld = (0.0, 0.0) if mode == "nesterov" else (1e4, 1e3)
+    args = [mode, "--mesh", "tetgen:" + os.path.join(MESHES, "cube.1"), "--fix", "z0", "--load-file", lf,
+            "--start-file", sf, "--E", repr(E_STEEL), "--nu", repr(NU_STEEL), "--rho0", repr(RHO_STEEL),
+            "--eta", repr(eta), "--lambda-d", repr(ld), "--steps", "2", "--inner-tol", "0", "--outer-tol", "0",
+            "--max-outer", "2", "--rho", "1e10", "--dt", "1e-3"]
+    args += ["--alpha", "2e-10", "--max-inner", "25"] if mode == "nesterov" else ["--max-inner", "30"]
+    info, pos = _run(args, tmp_path)
+    prob = O.Problem(m, E_STEEL, NU_STEEL, RHO_STEEL, eta=eta, lambda_d=ld, f_ext=load)
+    st = prob.new_state()
+    st["x"][:], st["y"][:], st["z"][:] = x0, y0, z0
+    counts = []
+    for _ in range(2):
+        if mode == "nesterov":
+            counts.append(prob.nesterov_step(st, 2e-10, 1e10, 0.0, 0.0, 2, 25, 1e-3))
+        else:
+            counts.append(prob.adamw_step(st, 2e-4, 0.9, 0.999, 1e-8, 1e-4, 0.995, 0.0, 0.0, 1e10, 2, 30, 1e-3, 10,
+                                          0.0, nocoop=(mode == "adamw_nocoop")))
+    assert info["inner_iters"] == counts
+    X0 = m.xyz()
+    duo = np.stack([st["x"], st["y"], st["z"]], 1) - X0
+    assert rel_err(pos.T - X0, duo) < (1e-8 if mode == "nesterov" else 1e-6)
+
+
+@pytest.mark.parametrize("fix2", ["x0", "z0:rev"])
+def test_update_nodal_fixed_dropin(gpu, tmp_path, fix2):
+    """UpdateNodalFixed between Solve calls (FEAT10Data.cu:674-751) through the drop-in, then
+    CalcConstraintData and ConvertToCSR_ConstraintJacT as a caller does: the cube is held at
+    z = 0 for the first step and then at x = x_min ("x0": another count, the multipliers
+    start at zero) or at the same nodes in reverse order ("z0:rev": same count, the
+    multipliers keep their constraint numbers, as the reference's solver keeps
+    d_lambda_guess).  Three Nesterov steps against the oracle doing the same."""
+    from helpers import E_STEEL, NU_STEEL, RHO_STEEL
+    m = load_mesh("cube")
+    load = np.random.default_rng(5).standard_normal(3 * m.n_nodes) * 50.0
+    lf = str(tmp_path / "load.bin")
+    _write_raw(lf, load)
+    args = ["nesterov", "--mesh", "tetgen:" + os.path.join(MESHES, "cube.1"), "--fix", "z0", "--load-file", lf,
+            "--E", repr(E_STEEL), "--nu", repr(NU_STEEL), "--rho0", repr(RHO_STEEL), "--steps", "3",
+            "--inner-tol", "0", "--outer-tol", "0", "--max-outer", "2", "--max-inner", "25", "--rho", "1e10",
+            "--dt", "1e-3", "--alpha", "2e-10", "--update-fixed-after", "1", "--fix2", fix2]
+    info, pos = _run(args, tmp_path)
+    z0 = np.nonzero(np.abs(m.z) < 1e-8)[0]
+    L = np.ptp(m.x)
+    new = z0[::-1] if fix2 == "z0:rev" else np.nonzero(np.abs(m.x - m.x.min()) < 5e-4 * L)[0]
+    assert info["n_fixed"] == len(z0) and info["n_fixed_final"] == len(new)
+    assert fix2 == "z0:rev" or len(new) != len(z0)
+    prob = O.Problem(m, E_STEEL, NU_STEEL, RHO_STEEL, f_ext=load, fixed=z0)
+    st = prob.new_state()
+    for step in range(3):
+        if step == 1:
+            prob.set_fixed(new)
+            if len(new) != len(z0):
+                st["lam"] = np.zeros(3 * len(new))
+        prob.nesterov_step(st, 2e-10, 1e10, 0.0, 0.0, 2, 25, 1e-3)
+    X0 = m.xyz()
+    duo = np.stack([st["x"], st["y"], st["z"]], 1) - X0
+    assert rel_err(pos.T - X0, duo) < 1e-8
+
+
+def test_constraint_jacobian_csr_dropin(gpu, tmp_path):
+    """ConvertToCSR_ConstraintJac and ConvertToCSR_ConstraintJacT (FEAT10Data.cu:478-562)
+    through the drop-in: offsets, columns and values equal the oracle's selection matrices
+    exactly (J: one 1.0 per constraint row at DOF 3 fixed[k] + d; J^T: rows over the 3N
+    DOFs, columns ascending)."""
+    jf = str(tmp_path / "jac.txt")
+    info, _ = _run(["nesterov", "--mesh", "tetgen:" + os.path.join(MESHES, "cube.1"), "--fix", "z0:rev",
+                    "--steps", "0", "--jac-out", jf], tmp_path)
+    arrays = {}
+    for line in open(jf):
+        tok = line.split()
+        arrays[tok[0]] = np.array(tok[2:], dtype=np.float64)
+        assert int(tok[1]) == len(tok) - 2
+    m = load_mesh("cube")
+    fixed = np.nonzero(np.abs(m.z) < 1e-8)[0][::-1]
+    for t, pre in ((False, "j_"), (True, "jt_")):
+        o, c, v = O.constraint_jacobian_csr(fixed, m.n_nodes, transpose=t)
+        assert np.array_equal(arrays[pre + "offsets"].astype(np.int64), o)
+        assert np.array_equal(arrays[pre + "columns"].astype(np.int64), c)
+        assert np.array_equal(arrays[pre + "values"], v)
+
+
+@pytest.mark.parametrize("spec,name", [("kuhn:20,2,2", "C1"), ("kuhn4:12,3,2", "kuhn_t4")])
+def test_vtk_output_after_solve(gpu, tmp_path, spec, name):
+    """WriteOutputVTK (FEAT10Data.cu:759-794) and ANCFCPUUtils::WriteFEAT10ToVTK /
+    WriteFEAT4ToVTK (mesh_utils.cc:962-1077) after a drop-in linear-static solve: both
+    files parse, hold the solved positions to the 6 significant digits of the format, the
+    connectivity exactly and the cell type, and the second one the displacement field."""
+    vtk, feat = str(tmp_path / "out.vtk"), str(tmp_path / "feat.vtk")
+    load = "-500" if name == "kuhn_t4" else "-1000"
+    info, pos = _run(["linear_static", "--mesh", spec, "--tol", "1e-10", "--max-iter", "200000", "--load", load,
+                      "--vtk", vtk, "--vtk-feat", feat], tmp_path)
+    m = load_mesh(name)
+    P = pos.T
+    nn = m.nn
+    for path, title, ptype in ((vtk, f"T{nn} mesh output", "float"),
+                               (feat, "FEAT10 Tetrahedral Mesh" if nn == 10 else "FEAT4 Tetrahedral Mesh", "double")):
+        v = M.read_vtk_legacy(path)
+        assert v["header"] == "# vtk DataFile Version 3.0" and v["title"] == title
+        assert v["point_type"] == ptype
+        assert np.array_equal(v["cells"], m.elems)
+        assert np.all(v["cell_types"] == (24 if nn == 10 else 10))
+        assert np.all(np.abs(v["points"] - P) <= 5.01e-6 * np.abs(P) + 1e-300)
+    d = M.read_vtk_legacy(feat)["point_vectors"]["displacement"]
+    du = P - m.xyz()
+    assert np.all(np.abs(d - du) <= 5.01e-6 * np.abs(du) + 1e-12 * np.abs(m.xyz()).max())

@@ -732,3 +732,29 @@ def test_mr_element_force_and_stiffness(name):
     # the switch is scoped: SVK again outside
     lam, mu = lame(1e9, 0.3)
     assert np.abs(O.internal_force(m, g, dj, lam, mu, x, y, z) - f).max() > 1e-3 * np.abs(f).max()
+
+
+# ---------------------------------------------------------------------------------------
+def test_constraint_jacobian_csr_is_the_fixed_dof_selection():
+    """J selects the fixed DOFs and J^T is its transpose (FEAT10Data.cu:420-562): J J^T = I,
+    J^T equals J transposed, J x picks x at 3 fixed[k] + d in constraint order, and J^T's
+    row sums mark exactly the fixed DOFs.  An off-by-one in the DOF or a swapped d/k order
+    fails these."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(3)
+    n_nodes = 40
+    fixed = rng.choice(n_nodes, 11, replace=False)  # unsorted, as UpdateNodalFixed may pass
+    nc = 3 * len(fixed)
+    o, c, v = O.constraint_jacobian_csr(fixed, n_nodes)
+    J = sp.csr_matrix((v, c, o), shape=(nc, 3 * n_nodes))
+    ot, ct, vt = O.constraint_jacobian_csr(fixed, n_nodes, transpose=True)
+    JT = sp.csr_matrix((vt, ct, ot), shape=(3 * n_nodes, nc))
+    assert abs(J @ JT - sp.eye(nc)).max() == 0.0
+    assert abs(J.T - JT).max() == 0.0
+    x = rng.standard_normal(3 * n_nodes)
+    assert np.array_equal(J @ x, x.reshape(-1, 3)[fixed].reshape(-1))
+    mark = np.zeros(3 * n_nodes)
+    mark.reshape(-1, 3)[fixed] = 1.0
+    assert np.array_equal(np.asarray(JT.sum(axis=1)).ravel(), mark)
+    for r in range(3 * n_nodes):  # columns ascending within each J^T row
+        assert np.all(np.diff(ct[ot[r]:ot[r + 1]]) > 0)

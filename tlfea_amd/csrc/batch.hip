@@ -1301,10 +1301,13 @@ extern "C" int tlfea_batch_cg_solve(tlfea_batch* b, const double* f, double* u, 
         (void)hipGraphExecDestroy(b->graph);
         b->graph = nullptr;
     }
+    // s is a non-blocking stream, unordered with the null stream: the copies of the
+    // per-instance results go on s, after the tail kernels that write them
     std::vector<double> S((size_t)b->n_inst * NS);
     std::vector<int> I((size_t)b->n_inst * NI);
-    TLFEA_HIP_TRY(hipMemcpy(S.data(), b->d_s, sizeof(double) * S.size(), hipMemcpyDeviceToHost));
-    TLFEA_HIP_TRY(hipMemcpy(I.data(), b->d_i, sizeof(int) * I.size(), hipMemcpyDeviceToHost));
+    TLFEA_HIP_TRY(hipMemcpyAsync(S.data(), b->d_s, sizeof(double) * S.size(), hipMemcpyDeviceToHost, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(I.data(), b->d_i, sizeof(int) * I.size(), hipMemcpyDeviceToHost, s));
+    TLFEA_HIP_TRY(hipStreamSynchronize(s));
     int bad = 0;
     for (int i = 0; i < b->n_inst; i++) {
         if (iters) iters[i] = I[(size_t)i * NI + I_ITERS];

@@ -9,7 +9,18 @@
 //                 [--assembled 1: assemble K as the reference does] [--solves N] --out FILE
 //   tlfea_driver nesterov|adamw|adamw_nocoop --mesh tetgen:PREFIX|kuhn:.. [--steps 3] [--E ..]
 //                 [--rho0 ..] [--fix z0|x0] [--node-load NODE,FX] --out FILE
+//                 [--update-fixed-after K --fix2 SPEC]: before step K, UpdateNodalFixed to the
+//                 SPEC set (x0 | y0 | z0, ":rev" reverses the order), CalcConstraintData,
+//                 ConvertToCSR_ConstraintJacT (FEAT10Data.cu:674-751)
+//                 [--jac-out FILE]: the J and J^T CSR (ConvertToCSR_ConstraintJac/JacT) as text
+//                 [--load-file F]: f_ext from raw float64 [3N]; [--start-file F]: positions
+//                 (raw float64 x, y, z blocks) set by UpdatePositions after CalcDnDuPre, so the
+//                 constraint targets and rest geometry stay the mesh's
 //   any mode: [--mr MU10,MU01,KAPPA] switches the element to Mooney-Rivlin (SetMooneyRivlin)
+//             [--vtk FILE] WriteOutputVTK; [--vtk-feat FILE] ANCFCPUUtils::WriteFEAT10ToVTK /
+//             WriteFEAT4ToVTK of the final positions against the input nodes
+//   tlfea_driver write_vtk --mesh SPEC --vtk-feat FILE [--shear S]: no device work; the mesh
+//             at x + S (y z, -x z, x y) through WriteFEAT10ToVTK / WriteFEAT4ToVTK
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -19,6 +30,7 @@
 #include <fstream>
 #include <map>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../../include/tlfea/FEASolver.h"
@@ -61,6 +73,53 @@ bool load_mesh(const std::string& spec, MatrixXR& nodes, MatrixXi& elems) {
     return false;
 }
 
+// Node selections of the scenarios: nodes on the x = x_min ("x0"), y = 0 or z = 0 face;
+// a ":rev" suffix lists them in descending order.
+std::vector<int> select_nodes(const std::string& spec, const VectorXR& hx, const VectorXR& hy, const VectorXR& hz) {
+    const bool rev = spec.size() > 4 && spec.compare(spec.size() - 4, 4, ":rev") == 0;
+    const std::string face = rev ? spec.substr(0, spec.size() - 4) : spec;
+    const Real xmin = hx.minCoeff(), L = hx.maxCoeff() - xmin;
+    std::vector<int> out;
+    for (std::ptrdiff_t i = 0; i < hx.size(); i++) {
+        if (face == "z0" && std::abs(hz(i)) < 1e-8) out.push_back((int)i);
+        if (face == "y0" && std::abs(hy(i)) < 1e-8) out.push_back((int)i);
+        if (face == "x0" && std::abs(hx(i) - xmin) < 5e-4 * L) out.push_back((int)i);
+    }
+    if (rev) std::reverse(out.begin(), out.end());
+    return out;
+}
+
+VectorXi to_vectorxi(const std::vector<int>& v) {
+    VectorXi r((std::ptrdiff_t)v.size());
+    for (size_t k = 0; k < v.size(); k++) r((std::ptrdiff_t)k) = v[k];
+    return r;
+}
+
+template <typename T>
+void write_array(std::ofstream& f, const char* name, const std::vector<T>& v) {
+    f << name << " " << v.size();
+    char buf[40];
+    for (const T& x : v) {
+        std::snprintf(buf, sizeof(buf), std::is_integral<T>::value ? " %.0f" : " %.17g", (double)x);
+        f << buf;
+    }
+    f << "\n";
+}
+
+template <typename TData>
+void write_jacobians(const std::string& path, TData* b) {
+    std::ofstream f(path);
+    for (int t = 0; t < 2; t++) {
+        std::vector<int> off, col;
+        std::vector<Real> val;
+        b->RetrieveConstraintJacobianCSRToCPU(t == 1, off, col, val);
+        const std::string p = t ? "jt_" : "j_";
+        write_array(f, (p + "offsets").c_str(), off);
+        write_array(f, (p + "columns").c_str(), col);
+        write_array(f, (p + "values").c_str(), val);
+    }
+}
+
 void write_positions(const std::string& path, const VectorXR& x, const VectorXR& y, const VectorXR& z) {
     std::ofstream f(path, std::ios::binary);
     for (const VectorXR* v : {&x, &y, &z}) f.write(reinterpret_cast<const char*>(v->data()), sizeof(double) * v->size());
@@ -82,16 +141,19 @@ int run(const std::string& mode, std::map<std::string, std::string>& a, MatrixXR
     b->Initialize();
     const bool t10 = TData::N_NODES_PER_ELEM == 10;
     // boundary conditions and loads, as the reference drivers set them
-    std::vector<int> fixed;
     VectorXR f(3 * n_nodes);
     f.setZero();
     const std::string fix = a.count("--fix") ? a["--fix"] : (mode == "linear_static" ? "x0" : "z0");
-    const Real xmin = hx.minCoeff(), xmax = hx.maxCoeff(), L = xmax - xmin;
-    for (int i = 0; i < n_nodes; i++) {
-        if (fix == "z0" && std::abs(hz(i)) < 1e-8) fixed.push_back(i);
-        if (fix == "x0" && std::abs(hx(i) - xmin) < 5e-4 * L) fixed.push_back(i);
-    }
-    if (a.count("--node-load")) {
+    const Real xmax = hx.maxCoeff(), L = xmax - hx.minCoeff();
+    const std::vector<int> fixed = select_nodes(fix, hx, hy, hz);
+    if (a.count("--load-file")) {
+        std::ifstream lf(a["--load-file"], std::ios::binary);
+        lf.read(reinterpret_cast<char*>(f.data()), sizeof(double) * 3 * (size_t)n_nodes);
+        if (!lf) {
+            std::fprintf(stderr, "could not read %d loads from %s\n", 3 * n_nodes, a["--load-file"].c_str());
+            return 2;
+        }
+    } else if (a.count("--node-load")) {
         int node;
         double fx;
         std::sscanf(a["--node-load"].c_str(), "%d,%lf", &node, &fx);
@@ -103,9 +165,7 @@ int run(const std::string& mode, std::map<std::string, std::string>& a, MatrixXR
         const Real load = num(a, "--load", -1000.0);
         for (int i : tip) f(3 * i + 2) = load / (Real)tip.size();
     }
-    VectorXi hf((std::ptrdiff_t)fixed.size());
-    for (size_t k = 0; k < fixed.size(); k++) hf((std::ptrdiff_t)k) = fixed[k];
-    b->SetNodalFixed(hf);
+    b->SetNodalFixed(to_vectorxi(fixed));
     b->SetExternalForce(f);
     if (t10)
         b->Setup(Quadrature::tet5pt_x, Quadrature::tet5pt_y, Quadrature::tet5pt_z, Quadrature::tet5pt_weights, hx, hy,
@@ -125,6 +185,16 @@ int run(const std::string& mode, std::map<std::string, std::string>& a, MatrixXR
         b->SetMooneyRivlin(mu10, mu01, kappa);
     }
     b->CalcDnDuPre();
+    if (a.count("--start-file")) {
+        VectorXR sx(n_nodes), sy(n_nodes), sz(n_nodes);
+        std::ifstream sf(a["--start-file"], std::ios::binary);
+        for (VectorXR* v : {&sx, &sy, &sz}) sf.read(reinterpret_cast<char*>(v->data()), sizeof(double) * n_nodes);
+        if (!sf) {
+            std::fprintf(stderr, "could not read %d positions from %s\n", 3 * n_nodes, a["--start-file"].c_str());
+            return 2;
+        }
+        b->UpdatePositions(sx, sy, sz);
+    }
     auto t0 = std::chrono::steady_clock::now();
     std::string extra;
     if (mode == "linear_static") {
@@ -146,7 +216,23 @@ int run(const std::string& mode, std::map<std::string, std::string>& a, MatrixXR
         b->CalcMassMatrix();
         b->CalcConstraintData();
         b->ConvertToCSR_ConstraintJacT();
+        if (a.count("--jac-out")) {
+            b->ConvertToCSR_ConstraintJac();
+            write_jacobians(a["--jac-out"], b);
+        }
         const int steps = (int)num(a, "--steps", 3);
+        // --update-fixed-after K: the fixed set changes before step K (prescribed motion
+        // switching supports), as FEAT10Data.cu:674-751 allows between Solve calls
+        const int upd_after = (int)num(a, "--update-fixed-after", -1);
+        size_t n_fixed_now = fixed.size();
+        auto maybe_update = [&](int k) {
+            if (k != upd_after) return;
+            const std::vector<int> f2 = select_nodes(a.count("--fix2") ? a["--fix2"] : "x0", hx, hy, hz);
+            b->UpdateNodalFixed(to_vectorxi(f2));
+            b->CalcConstraintData();
+            b->ConvertToCSR_ConstraintJacT();
+            n_fixed_now = f2.size();
+        };
         std::string iters;
         if (mode == "nesterov") {
             SyncedNesterovSolver solver(b, b->get_n_constraint());
@@ -157,6 +243,7 @@ int run(const std::string& mode, std::map<std::string, std::string>& a, MatrixXR
             solver.SetParameters(&p);
             fea.AddSolver("nesterov", &solver);
             for (int k = 0; k < steps; k++) {
+                maybe_update(k);
                 fea.GetSolver("nesterov")->Solve();
                 iters += (k ? "," : "") + std::to_string(solver.GetLastInnerIterations());
             }
@@ -171,7 +258,8 @@ int run(const std::string& mode, std::map<std::string, std::string>& a, MatrixXR
                 solver.SetParameters(&p);
                 fea.AddSolver("adamw", &solver);
                 for (int k = 0; k < steps; k++) {
-                    fea.GetSolver("adamw")->Solve();
+                    maybe_update(k);
+                fea.GetSolver("adamw")->Solve();
                     iters += (k ? "," : "") + std::to_string(solver.GetLastInnerIterations());
                 }
             } else {
@@ -180,18 +268,24 @@ int run(const std::string& mode, std::map<std::string, std::string>& a, MatrixXR
                 solver.SetParameters(&p);
                 fea.AddSolver("adamw_nocoop", &solver);
                 for (int k = 0; k < steps; k++) {
-                    fea.GetSolver("adamw_nocoop")->Solve();
+                    maybe_update(k);
+                fea.GetSolver("adamw_nocoop")->Solve();
                     iters += (k ? "," : "") + std::to_string(solver.GetLastInnerIterations());
                 }
             }
         }
-        extra = ", \"inner_iters\": [" + iters + "]";
+        extra = ", \"inner_iters\": [" + iters + "], \"n_fixed_final\": " + std::to_string(n_fixed_now);
     }
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     VectorXR x, y, z;
     b->RetrievePositionToCPU(x, y, z);
     if (a.count("--out")) write_positions(a["--out"], x, y, z);
     if (a.count("--vtk")) b->WriteOutputVTK(a["--vtk"]);
+    if (a.count("--vtk-feat")) {
+        const bool ok = t10 ? ANCFCPUUtils::WriteFEAT10ToVTK(a["--vtk-feat"], nodes, elems, x, y, z)
+                            : ANCFCPUUtils::WriteFEAT4ToVTK(a["--vtk-feat"], nodes, elems, x, y, z);
+        if (!ok) return 1;
+    }
     Real tip = 0.0;
     for (int i = 0; i < n_nodes; i++)
         if (std::abs(hx(i) - xmax) < 5e-4 * L) tip = std::min(tip, z(i) - hz(i));
@@ -215,6 +309,22 @@ int main(int argc, char** argv) {
     if (!a.count("--mesh") || !load_mesh(a["--mesh"], nodes, elems)) {
         std::fprintf(stderr, "could not load mesh\n");
         return 2;
+    }
+    if (mode == "write_vtk") {  // no device work: the reference's VTK writers on a sheared mesh
+        const Real sh = num(a, "--shear", 0.01);
+        const std::ptrdiff_t n = nodes.rows();
+        VectorXR x(n), y(n), z(n);
+        for (std::ptrdiff_t i = 0; i < n; i++) {
+            const Real X = nodes(i, 0), Y = nodes(i, 1), Z = nodes(i, 2);
+            x(i) = X + sh * Y * Z;
+            y(i) = Y - sh * X * Z;
+            z(i) = Z + sh * X * Y;
+        }
+        const bool ok = elems.cols() == 10 ? ANCFCPUUtils::WriteFEAT10ToVTK(a["--vtk-feat"], nodes, elems, x, y, z)
+                                           : ANCFCPUUtils::WriteFEAT4ToVTK(a["--vtk-feat"], nodes, elems, x, y, z);
+        std::printf("{\"n_nodes\": %td, \"n_elem\": %td, \"ok\": %s}\n", nodes.rows(), elems.rows(),
+                    ok ? "true" : "false");
+        return ok ? 0 : 1;
     }
     if (mode == "mesh_info") {  // no device work: dumps the mesh as raw float64 nodes + int32 elements
         if (a.count("--out")) {

@@ -250,4 +250,49 @@ void GenerateKuhnBeam(int nx, int ny, int nz, Real lx, Real ly, Real lz, bool te
                 }
 }
 
+// Legacy ASCII VTK of a tetrahedral mesh at deformed positions, with the displacement as
+// point data: the layout of WriteFEAT10ToVTK / WriteFEAT4ToVTK (mesh_utils.cc:962-1077).
+// Values go through the stream's default formatting (6 significant digits), as there.
+static bool write_tet_vtk(const std::string& filename, int nn, const char* title, const MatrixXR& nodes,
+                          const MatrixXi& elements, const VectorXR& x, const VectorXR& y, const VectorXR& z) {
+    std::ofstream file(filename);
+    if (!file.is_open()) {
+        err("Failed to open file for writing: " + filename);
+        return false;
+    }
+    const std::ptrdiff_t n_nodes = nodes.rows(), n_el = elements.rows();
+    if (elements.cols() != nn || nodes.cols() < 3 || x.size() != n_nodes || y.size() != n_nodes ||
+        z.size() != n_nodes) {
+        err("VTK writer: array sizes do not match the mesh");
+        return false;
+    }
+    file << "# vtk DataFile Version 3.0\n" << title << "\nASCII\nDATASET UNSTRUCTURED_GRID\n";
+    file << "POINTS " << n_nodes << " double\n";
+    for (std::ptrdiff_t i = 0; i < n_nodes; i++) file << x(i) << " " << y(i) << " " << z(i) << "\n";
+    file << "\nCELLS " << n_el << " " << n_el * (nn + 1) << "\n";
+    for (std::ptrdiff_t e = 0; e < n_el; e++) {
+        file << nn;
+        for (int a = 0; a < nn; a++) file << " " << elements(e, a);
+        file << "\n";
+    }
+    file << "\nCELL_TYPES " << n_el << "\n";
+    const char* ctype = nn == 10 ? "24\n" : "10\n";  // VTK_QUADRATIC_TETRA / VTK_TETRA
+    for (std::ptrdiff_t e = 0; e < n_el; e++) file << ctype;
+    file << "\nPOINT_DATA " << n_nodes << "\nVECTORS displacement double\n";
+    for (std::ptrdiff_t i = 0; i < n_nodes; i++)
+        file << x(i) - nodes(i, 0) << " " << y(i) - nodes(i, 1) << " " << z(i) - nodes(i, 2) << "\n";
+    return true;
+}
+
+namespace ANCFCPUUtils {
+bool WriteFEAT10ToVTK(const std::string& filename, const MatrixXR& nodes, const MatrixXi& elements, const VectorXR& x,
+                      const VectorXR& y, const VectorXR& z) {
+    return write_tet_vtk(filename, 10, "FEAT10 Tetrahedral Mesh", nodes, elements, x, y, z);
+}
+bool WriteFEAT4ToVTK(const std::string& filename, const MatrixXR& nodes, const MatrixXi& elements, const VectorXR& x,
+                     const VectorXR& y, const VectorXR& z) {
+    return write_tet_vtk(filename, 4, "FEAT4 Tetrahedral Mesh", nodes, elements, x, y, z);
+}
+}  // namespace ANCFCPUUtils
+
 }  // namespace tlfea

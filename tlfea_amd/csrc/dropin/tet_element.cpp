@@ -408,6 +408,23 @@ void TetElementData::ConvertToCSR_ConstraintJacT() {
     is_cj_csr_setup = true;
 }
 
+void TetElementData::RetrieveConstraintJacobianCSRToCPU(bool transpose, std::vector<int>& offsets,
+                                                       std::vector<int>& columns, std::vector<Real>& values) {
+    offsets.clear();
+    columns.clear();
+    values.clear();
+    if (!(transpose ? is_cj_csr_setup : is_j_csr_setup)) return;
+    TetState& s = *s_;
+    const size_t rows = transpose ? 3 * (size_t)n_coef : (size_t)n_constraint;
+    offsets.resize(rows + 1);
+    d2h(offsets.data(), transpose ? s.jt_off : s.j_off, rows + 1);
+    const size_t nnz = (size_t)offsets[rows];
+    columns.resize(nnz);
+    values.resize(nnz);
+    d2h(columns.data(), transpose ? s.jt_col : s.j_col, nnz);
+    d2h(values.data(), transpose ? s.jt_val : s.j_val, nnz);
+}
+
 void TetElementData::RetrieveMassCSRToCPU(std::vector<int>& offsets, std::vector<int>& columns,
                                           std::vector<Real>& values) {
     offsets.assign((size_t)n_coef + 1, 0);

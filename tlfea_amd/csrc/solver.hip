@@ -35,9 +35,10 @@ struct tlfea_solver {
     double *vk = nullptr, *vn = nullptr;  // Nesterov iterates
     double *g = nullptr, *m1 = nullptr, *m2 = nullptr;
     double* dv = nullptr;  // [n_nodes][4]: v_guess - v_prev per node, padded to 32 bytes
-    // constraints per node
+    // constraints per node, built from the fixed list h_fixed
     int *cons_off = nullptr, *cons_idx = nullptr;
     unsigned char* node_fixed = nullptr;
+    std::vector<int> h_fixed;
     // reductions
     double* part = nullptr;  // [4][max_blocks]
     int max_blocks = 0;
@@ -462,6 +463,44 @@ int check_rest_form(const tlfea_body* b) {
     }
     return TLFEA_OK;
 }
+
+// Node -> constraint list (J^T in CSR over nodes, FEAT10Data.cu:420-473) and the per-node
+// fixed flag, from the body's fixed-node list.  Called at create and again whenever the
+// list changes (the drop-in's UpdateNodalFixed, FEAT10Data.cu:674-751): the gradient's
+// J^T(lambda + rho c) term and the free-DOF norm then follow the new nodes.
+int build_constraint_tables(tlfea_solver* s, const tlfea_body& body) {
+    std::vector<int> fixed(body.n_fixed);
+    if (body.n_fixed > 0 &&
+        hipMemcpy(fixed.data(), body.fixed_nodes, sizeof(int) * body.n_fixed, hipMemcpyDeviceToHost) != hipSuccess) {
+        tlfea_hip::set_error("could not read the fixed-node list");
+        return TLFEA_EHIP;
+    }
+    std::vector<int> cnt(body.n_nodes + 1, 0), idx(body.n_fixed);
+    std::vector<unsigned char> nf(body.n_nodes, 0);
+    for (int k = 0; k < body.n_fixed; k++) {
+        if (fixed[k] < 0 || fixed[k] >= body.n_nodes) {
+            tlfea_hip::set_error("fixed node %d out of range", fixed[k]);
+            return TLFEA_EINVAL;
+        }
+        cnt[fixed[k] + 1]++;
+        nf[fixed[k]] = 1;
+    }
+    for (int i = 0; i < body.n_nodes; i++) cnt[i + 1] += cnt[i];
+    std::vector<int> cur(cnt.begin(), cnt.end() - 1);
+    for (int k = 0; k < body.n_fixed; k++) idx[cur[fixed[k]]++] = k;
+    (void)hipFree(s->cons_idx);
+    s->cons_idx = nullptr;
+    if (int rc = dalloc(&s->cons_idx, body.n_fixed)) return rc;
+    if (hipMemcpy(s->cons_off, cnt.data(), sizeof(int) * (body.n_nodes + 1), hipMemcpyHostToDevice) != hipSuccess ||
+        (body.n_fixed > 0 &&
+         hipMemcpy(s->cons_idx, idx.data(), sizeof(int) * body.n_fixed, hipMemcpyHostToDevice) != hipSuccess) ||
+        hipMemcpy(s->node_fixed, nf.data(), body.n_nodes, hipMemcpyHostToDevice) != hipSuccess) {
+        tlfea_hip::set_error("copy of the constraint tables failed");
+        return TLFEA_EHIP;
+    }
+    s->h_fixed.swap(fixed);
+    return TLFEA_OK;
+}
 }  // namespace
 
 extern "C" int tlfea_solver_create(int kind, const tlfea_body* body, tlfea_solver** out) {
@@ -495,39 +534,9 @@ extern "C" int tlfea_solver_create(int kind, const tlfea_body* body, tlfea_solve
         free_solver(s);
         return rc;
     }
-    // node -> constraint list (J^T in CSR over nodes, FEAT10Data.cu:420-473)
-    std::vector<int> fixed(body->n_fixed);
-    if (body->n_fixed > 0 &&
-        hipMemcpy(fixed.data(), body->fixed_nodes, sizeof(int) * body->n_fixed, hipMemcpyDeviceToHost) != hipSuccess) {
-        free_solver(s);
-        tlfea_hip::set_error("could not read the fixed-node list");
-        return TLFEA_EHIP;
-    }
-    std::vector<int> cnt(body->n_nodes + 1, 0), idx(body->n_fixed);
-    std::vector<unsigned char> nf(body->n_nodes, 0);
-    for (int k = 0; k < body->n_fixed; k++) {
-        if (fixed[k] < 0 || fixed[k] >= body->n_nodes) {
-            free_solver(s);
-            tlfea_hip::set_error("fixed node %d out of range", fixed[k]);
-            return TLFEA_EINVAL;
-        }
-        cnt[fixed[k] + 1]++;
-        nf[fixed[k]] = 1;
-    }
-    for (int i = 0; i < body->n_nodes; i++) cnt[i + 1] += cnt[i];
-    std::vector<int> cur(cnt.begin(), cnt.end() - 1);
-    for (int k = 0; k < body->n_fixed; k++) idx[cur[fixed[k]]++] = k;
-    if ((rc = dalloc(&s->cons_idx, body->n_fixed))) {
+    if ((rc = build_constraint_tables(s, *body))) {
         free_solver(s);
         return rc;
-    }
-    if (hipMemcpy(s->cons_off, cnt.data(), sizeof(int) * (body->n_nodes + 1), hipMemcpyHostToDevice) != hipSuccess ||
-        (body->n_fixed > 0 &&
-         hipMemcpy(s->cons_idx, idx.data(), sizeof(int) * body->n_fixed, hipMemcpyHostToDevice) != hipSuccess) ||
-        hipMemcpy(s->node_fixed, nf.data(), body->n_nodes, hipMemcpyHostToDevice) != hipSuccess) {
-        free_solver(s);
-        tlfea_hip::set_error("copy of the constraint tables failed");
-        return TLFEA_EHIP;
     }
     *out = s;
     return TLFEA_OK;
@@ -540,12 +549,26 @@ extern "C" int tlfea_solver_destroy(tlfea_solver* s) {
 
 extern "C" int tlfea_solver_update_body(tlfea_solver* s, const tlfea_body* body) {
     TLFEA_REQUIRE(s && body, "null argument");
-    TLFEA_REQUIRE(body->nn == s->b.nn && body->n_elem == s->b.n_elem && body->n_nodes == s->b.n_nodes &&
-                      body->n_fixed == s->b.n_fixed,
+    TLFEA_REQUIRE(body->nn == s->b.nn && body->n_elem == s->b.n_elem && body->n_nodes == s->b.n_nodes,
                   "body sizes differ from the solver's");
     TLFEA_REQUIRE(body->pattern && body->m_sell && body->f_ext && body->f_int, "body is missing device arrays");
+    TLFEA_REQUIRE(body->n_fixed == 0 || (body->fixed_nodes && body->constraint && body->xref && body->yref && body->zref),
+                  "constraints declared without arrays");
     TLFEA_REQUIRE_MODEL(body->mat);
     if (int rc = check_rest_form(body)) return rc;
+    // a changed fixed set (UpdateNodalFixed) rebuilds the node -> constraint tables.  With
+    // the same count the multipliers are kept, indexed by constraint number as the
+    // reference's solver keeps d_lambda_guess; a new count starts them at zero.
+    std::vector<int> fixed(body->n_fixed);
+    if (body->n_fixed > 0)
+        TLFEA_HIP_TRY(hipMemcpy(fixed.data(), body->fixed_nodes, sizeof(int) * body->n_fixed, hipMemcpyDeviceToHost));
+    if (body->n_fixed != s->b.n_fixed) {
+        (void)hipFree(s->lam);
+        s->lam = nullptr;
+        if (int rc = dalloc(&s->lam, 3 * (int64_t)body->n_fixed)) return rc;
+    }
+    if (fixed != s->h_fixed)
+        if (int rc = build_constraint_tables(s, *body)) return rc;
     s->b = *body;
     return TLFEA_OK;
 }

@@ -273,3 +273,69 @@ def write_vtk_legacy(path: str, m: TetMesh, x=None, y=None, z=None) -> None:
         ctype = 24 if nn == 10 else 10
         for _ in range(m.n_elem):
             out.write(f"{ctype}\n")
+
+
+def write_feat_vtk(path: str, m: TetMesh, x, y, z) -> None:
+    """Legacy ASCII VTK of the mesh at positions (x, y, z) with the displacement as point
+    data, the layout of the reference's ``WriteFEAT10ToVTK`` / ``WriteFEAT4ToVTK``
+    (``src/utils/mesh_utils.cc:962-1077``): values with 6 significant digits, as a C++
+    stream writes them by default."""
+    nn = m.nn
+    title = "FEAT10 Tetrahedral Mesh" if nn == 10 else "FEAT4 Tetrahedral Mesh"
+    with open(path, "w") as out:
+        out.write(f"# vtk DataFile Version 3.0\n{title}\nASCII\nDATASET UNSTRUCTURED_GRID\n")
+        out.write(f"POINTS {len(x)} double\n")
+        for i in range(len(x)):
+            out.write(f"{x[i]:g} {y[i]:g} {z[i]:g}\n")
+        out.write(f"\nCELLS {m.n_elem} {m.n_elem * (nn + 1)}\n")
+        for e in range(m.n_elem):
+            out.write(f"{nn} " + " ".join(str(int(v)) for v in m.elems[e]) + "\n")
+        out.write(f"\nCELL_TYPES {m.n_elem}\n")
+        ctype = "24" if nn == 10 else "10"
+        out.write("".join(f"{ctype}\n" for _ in range(m.n_elem)))
+        out.write(f"\nPOINT_DATA {len(x)}\nVECTORS displacement double\n")
+        for i in range(len(x)):
+            out.write(f"{x[i] - m.x[i]:g} {y[i] - m.y[i]:g} {z[i] - m.z[i]:g}\n")
+
+
+def read_vtk_legacy(path: str) -> dict:
+    """Parses a legacy ASCII VTK unstructured grid of tetrahedra as the writers above and
+    the reference's write them: returns ``title``, ``points`` (n x 3), ``point_type``,
+    ``cells`` (n_cells x nn), ``cell_types``, and ``point_vectors`` {name: n x 3}."""
+    with open(path) as fh:
+        lines = fh.read().split("\n")
+    out = {"header": lines[0], "title": lines[1], "format": lines[2], "dataset": lines[3], "point_vectors": {}}
+    i = 4
+    while i < len(lines):
+        tok = lines[i].split()
+        if not tok:
+            i += 1
+            continue
+        if tok[0] == "POINTS":
+            n = int(tok[1])
+            out["point_type"] = tok[2]
+            out["points"] = np.array([lines[i + 1 + k].split() for k in range(n)], dtype=np.float64).reshape(n, 3)
+            i += 1 + n
+        elif tok[0] == "CELLS":
+            n, total = int(tok[1]), int(tok[2])
+            rows = [lines[i + 1 + k].split() for k in range(n)]
+            counts = np.array([int(r[0]) for r in rows])
+            assert counts.sum() + n == total, "CELLS size field does not match its rows"
+            assert np.all(counts == counts[0]), "mixed cell sizes"
+            out["cells"] = np.array([r[1:] for r in rows], dtype=np.int64)
+            i += 1 + n
+        elif tok[0] == "CELL_TYPES":
+            n = int(tok[1])
+            out["cell_types"] = np.array([int(lines[i + 1 + k]) for k in range(n)])
+            i += 1 + n
+        elif tok[0] == "POINT_DATA":
+            n = int(tok[1])
+            vec = lines[i + 1].split()
+            assert vec[0] == "VECTORS", "only VECTORS point data is written here"
+            out["point_vectors"][vec[1]] = np.array([lines[i + 2 + k].split() for k in range(n)],
+                                                    dtype=np.float64).reshape(n, 3)
+            i += 2 + n
+        else:
+            raise ValueError(f"unexpected VTK line: {lines[i]!r}")
+    return out
+
