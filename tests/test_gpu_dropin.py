@@ -148,12 +148,12 @@ def test_iterative_dropin_trajectory(gpu, tmp_path, mode):
     assert rel_err(pos.T - X0, duo) < (1e-8 if mode == "nesterov" else 1e-6)
 
 
-@pytest.mark.parametrize("fix2", ["x0", "z0:rev"])
+@pytest.mark.parametrize("fix2", ["x0+z0", "z0:rev"])
 def test_update_nodal_fixed_dropin(gpu, tmp_path, fix2):
     """UpdateNodalFixed between Solve calls (FEAT10Data.cu:674-751) through the drop-in, then
     CalcConstraintData and ConvertToCSR_ConstraintJacT as a caller does: the cube is held at
-    z = 0 for the first step and then at x = x_min ("x0": another count, the multipliers
-    start at zero) or at the same nodes in reverse order ("z0:rev": same count, the
+    z = 0 for the first step and then at z = 0 and x = x_min ("x0+z0": another count, the
+    multipliers start at zero) or at the same nodes in reverse order ("z0:rev": same count, the
     multipliers keep their constraint numbers, as the reference's solver keeps
     d_lambda_guess).  Three Nesterov steps against the oracle doing the same."""
     from helpers import E_STEEL, NU_STEEL, RHO_STEEL
@@ -168,7 +168,8 @@ def test_update_nodal_fixed_dropin(gpu, tmp_path, fix2):
     info, pos = _run(args, tmp_path)
     z0 = np.nonzero(np.abs(m.z) < 1e-8)[0]
     L = np.ptp(m.x)
-    new = z0[::-1] if fix2 == "z0:rev" else np.nonzero(np.abs(m.x - m.x.min()) < 5e-4 * L)[0]
+    new = z0[::-1] if fix2 == "z0:rev" else np.nonzero((np.abs(m.x - m.x.min()) < 5e-4 * L) |
+                                                       (np.abs(m.z) < 1e-8))[0]
     assert info["n_fixed"] == len(z0) and info["n_fixed_final"] == len(new)
     assert fix2 == "z0:rev" or len(new) != len(z0)
     prob = O.Problem(m, E_STEEL, NU_STEEL, RHO_STEEL, f_ext=load, fixed=z0)

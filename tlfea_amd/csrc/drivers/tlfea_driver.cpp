@@ -10,7 +10,7 @@
 //   tlfea_driver nesterov|adamw|adamw_nocoop --mesh tetgen:PREFIX|kuhn:.. [--steps 3] [--E ..]
 //                 [--rho0 ..] [--fix z0|x0] [--node-load NODE,FX] --out FILE
 //                 [--update-fixed-after K --fix2 SPEC]: before step K, UpdateNodalFixed to the
-//                 SPEC set (x0 | y0 | z0, ":rev" reverses the order), CalcConstraintData,
+//                 SPEC set (x0 | y0 | z0 | x0+z0, ":rev" reverses the order), CalcConstraintData,
 //                 ConvertToCSR_ConstraintJacT (FEAT10Data.cu:674-751)
 //                 [--jac-out FILE]: the J and J^T CSR (ConvertToCSR_ConstraintJac/JacT) as text
 //                 [--load-file F]: f_ext from raw float64 [3N]; [--start-file F]: positions
@@ -73,17 +73,18 @@ bool load_mesh(const std::string& spec, MatrixXR& nodes, MatrixXi& elems) {
     return false;
 }
 
-// Node selections of the scenarios: nodes on the x = x_min ("x0"), y = 0 or z = 0 face;
-// a ":rev" suffix lists them in descending order.
+// Node selections of the scenarios: nodes on the x = x_min ("x0"), y = 0 or z = 0 face, or
+// on any of several faces ("x0+z0"), in ascending order; a ":rev" suffix reverses it.
 std::vector<int> select_nodes(const std::string& spec, const VectorXR& hx, const VectorXR& hy, const VectorXR& hz) {
     const bool rev = spec.size() > 4 && spec.compare(spec.size() - 4, 4, ":rev") == 0;
-    const std::string face = rev ? spec.substr(0, spec.size() - 4) : spec;
+    const std::string faces = "+" + (rev ? spec.substr(0, spec.size() - 4) : spec) + "+";
+    auto has = [&](const char* f) { return faces.find(std::string("+") + f + "+") != std::string::npos; };
     const Real xmin = hx.minCoeff(), L = hx.maxCoeff() - xmin;
     std::vector<int> out;
     for (std::ptrdiff_t i = 0; i < hx.size(); i++) {
-        if (face == "z0" && std::abs(hz(i)) < 1e-8) out.push_back((int)i);
-        if (face == "y0" && std::abs(hy(i)) < 1e-8) out.push_back((int)i);
-        if (face == "x0" && std::abs(hx(i) - xmin) < 5e-4 * L) out.push_back((int)i);
+        if ((has("z0") && std::abs(hz(i)) < 1e-8) || (has("y0") && std::abs(hy(i)) < 1e-8) ||
+            (has("x0") && std::abs(hx(i) - xmin) < 5e-4 * L))
+            out.push_back((int)i);
     }
     if (rev) std::reverse(out.begin(), out.end());
     return out;
