@@ -261,6 +261,22 @@ int tlfea_cg_solve_matfree(tlfea_cg* c, const tlfea_force_plan* plan, const doub
                            const tlfea_material* mat, const unsigned char* fixed_dof_mask, const double* f, double* u,
                            double tol, int max_iter, int check_every, int* iters_out, double* rel_res_out,
                            void* stream);
+/* No counterpart in the reference.  tlfea_cg_solve_matfree without the tlfea_cg handle,
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): the same kernels, arguments, stopping
+ * rule and status codes, with the CG's own workspace (r, p, K p, reduction partials,
+ * scalars, flags) in a static device array of the library, 72 MiB per GPU, held until the
+ * process ends.  It calls no hipMalloc or hipFree.  Bodies of more than
+ * TLFEA_NOALLOC_MAX_NODES nodes return TLFEA_ECAPACITY.  The force plan (a setup product of
+ * tlfea_force_plan_create) still carries the operator's block partials.  Every call on a
+ * GPU shares that GPU's array and the plan's partials: calls that overlap on one GPU (two
+ * streams, two host threads) race. */
+#define TLFEA_NOALLOC_MAX_NODES 1048576
+int tlfea_cg_solve_matfree_noalloc(const tlfea_force_plan* plan, const double* x, const double* y, const double* z,
+                                   const double* X0, const double* Y0, const double* Z0, const double* jinv_t,
+                                   const double* detJ_t, int affine, const tlfea_quad* quad, const tlfea_material* mat,
+                                   const unsigned char* fixed_dof_mask, const double* f, double* u, double tol,
+                                   int max_iter, int check_every, int* iters_out, double* rel_res_out, void* stream);
 /* DOF mask (uint8 [3*n_nodes], 1 = DOF of a fixed node) from a fixed-node list. */
 int tlfea_fixed_dof_mask(int n_nodes, int n_fixed, const int* fixed_nodes, unsigned char* mask, void* stream);
 

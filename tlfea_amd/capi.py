@@ -127,6 +127,9 @@ _SIGS = {
     "tlfea_cg_solve": (i32, [vp, vp, vp, vp, f64, i32, i32, C.POINTER(i32), C.POINTER(f64), vp]),
     "tlfea_cg_solve_matfree": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, C.POINTER(Quad), C.POINTER(Material),
                                      vp, vp, vp, f64, i32, i32, C.POINTER(i32), C.POINTER(f64), vp]),
+    "tlfea_cg_solve_matfree_noalloc": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, C.POINTER(Quad),
+                                             C.POINTER(Material), vp, vp, vp, f64, i32, i32, C.POINTER(i32),
+                                             C.POINTER(f64), vp]),
     "tlfea_batch_create": (i32, [i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(vp), vp]),
     "tlfea_batch_destroy": (i32, [vp]),
     "tlfea_batch_sizes": (i32, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),

@@ -15,6 +15,8 @@
 #include <numeric>
 #include <vector>
 
+#include <mutex>
+
 #include "internal.h"
 
 namespace tlfea_hip {
@@ -1564,30 +1566,31 @@ __global__ __launch_bounds__(RED_THREADS) void k_cg_init_masked(int64_t n, const
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-extern "C" int tlfea_cg_solve_matfree(tlfea_cg* c, const tlfea_force_plan* plan, const double* x, const double* y,
-                                      const double* z, const double* X0, const double* Y0, const double* Z0,
-                                      const double* jinv_t, const double* detJ_t, int affine,
-                                      const tlfea_quad* quad, const tlfea_material* mat,
-                                      const unsigned char* fixed_dof_mask, const double* f,
-                                      double* u, double tol, int max_iter, int check_every, int* iters_out,
-                                      double* rel_res_out, void* stream) {
-    TLFEA_REQUIRE(c && plan && x && y && z && X0 && Y0 && Z0 && jinv_t && detJ_t && quad && mat && f && u,
-                  "null argument");
-    TLFEA_REQUIRE(quad->n == (plan->nn == 10 ? 5 : 1), "quadrature size does not match the element type");
-    TLFEA_REQUIRE(max_iter >= 0, "negative iteration limit");
-    TLFEA_REQUIRE(plan->n_nodes == c->p->n_nodes, "plan and CG workspace sizes differ");
-    TLFEA_REQUIRE_MODEL(*mat);
+namespace {
+// Device workspace of one matrix-free CG solve: r, p, s = K p (3N each), RED_BLOCKS
+// reduction partials, 16 scalars and 8 flag words.
+struct MatfreeWork {
+    double *r, *p, *Kp, *part, *scal;
+    int* flags;
+};
+
+// The body of tlfea_cg_solve_matfree and tlfea_cg_solve_matfree_noalloc.  Every buffer of
+// `w` is initialised here (nothing carries over from an earlier solve).
+int matfree_cg(const MatfreeWork& w, const tlfea_force_plan* plan, const double* x, const double* y, const double* z,
+               const double* X0, const double* Y0, const double* Z0, const double* jinv_t, const double* detJ_t,
+               int affine, const tlfea_quad* quad, const tlfea_material* mat, const unsigned char* fixed_dof_mask,
+               const double* f, double* u, double tol, int max_iter, int check_every, int* iters_out,
+               double* rel_res_out, hipStream_t s) {
     if (check_every <= 0) check_every = 16;
-    hipStream_t s = as_stream(stream);
-    const int64_t n = c->n;
+    const int64_t n = 3 * (int64_t)plan->n_nodes;
     const unsigned gv = RED_BLOCKS;
     // at the rest positions K is the linear-elasticity operator (the kernel's u = 0 form);
     // the check uses the flag word before the initialisation resets it
     bool at_rest = false;
-    if (int rc = positions_at_rest(plan->n_nodes, x, y, z, X0, Y0, Z0, c->d_flags, s, &at_rest)) return rc;
+    if (int rc = positions_at_rest(plan->n_nodes, x, y, z, X0, Y0, Z0, w.flags, s, &at_rest)) return rc;
     // s = K p of the single-reduction iteration starts at zero (p_0 = r_0 with beta = 0)
-    k_cg_init_masked<<<gv, RED_THREADS, 0, s>>>(n, f, fixed_dof_mask, u, c->d_r, c->d_p, c->d_part, c->d_Kp);
-    k_cg_init_fin<<<1, RED_THREADS, 0, s>>>(gv, c->d_part, c->d_scal, c->d_flags);
+    k_cg_init_masked<<<gv, RED_THREADS, 0, s>>>(n, f, fixed_dof_mask, u, w.r, w.p, w.part, w.Kp);
+    k_cg_init_fin<<<1, RED_THREADS, 0, s>>>(gv, w.part, w.scal, w.flags);
     TLFEA_LAUNCH_CHECK();
     int flags[4] = {0, 0, 0, 0};
     double scal[8] = {0};
@@ -1595,29 +1598,29 @@ extern "C" int tlfea_cg_solve_matfree(tlfea_cg* c, const tlfea_force_plan* plan,
     const TanMat<double> tm = body_tanmat(*mat, at_rest, affine != 0);
     // two launches per iteration: the tangent applied to r (block partials, shares of r.Kr
     // and r.r) and the single-reduction update (test, alpha, beta, gather, p, s, u, r);
-    // d_Kp holds s = K p
+    // w.Kp holds s = K p
     for (int it = 0; it < max_iter; it++) {
         int rc = launch_tangent_partials<double>(plan, x, y, z, X0, Y0, Z0, jinv_t, detJ_t, *quad, tm,
-                                                 fixed_dof_mask, c->d_r, c->d_flags, s, nullptr, 0, true);
+                                                 fixed_dof_mask, w.r, w.flags, s, nullptr, 0, true);
         if (rc != TLFEA_OK) return rc;
         k_cgcg_update<<<gu, RED_THREADS, 0, s>>>(plan->n_nodes, plan->d_g_off, plan->d_g_idx, plan->d_part,
-                                                 fixed_dof_mask, u, c->d_r, c->d_p, c->d_Kp, plan->d_dot,
-                                                 plan->n_blocks, c->d_scal, c->d_flags, it, tol);
+                                                 fixed_dof_mask, u, w.r, w.p, w.Kp, plan->d_dot, plan->n_blocks,
+                                                 w.scal, w.flags, it, tol);
         TLFEA_LAUNCH_CHECK();
         if ((it + 1) % check_every == 0 || it + 1 == max_iter) {
-            TLFEA_HIP_TRY(hipMemcpyAsync(flags, c->d_flags, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
+            TLFEA_HIP_TRY(hipMemcpyAsync(flags, w.flags, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
             TLFEA_HIP_TRY(hipStreamSynchronize(s));
             if (flags[0]) break;
         }
     }
     if (!flags[0] && max_iter > 0) {
         // out of iterations: the residual of the last update has not been tested yet
-        k_dot_partial<<<gv, RED_THREADS, 0, s>>>(n, c->d_r, c->d_r, c->d_part);
-        k_cg_tail_rel<<<1, RED_THREADS, 0, s>>>(c->d_part, (int)gv, c->d_scal, c->d_flags, tol);
+        k_dot_partial<<<gv, RED_THREADS, 0, s>>>(n, w.r, w.r, w.part);
+        k_cg_tail_rel<<<1, RED_THREADS, 0, s>>>(w.part, (int)gv, w.scal, w.flags, tol);
         TLFEA_LAUNCH_CHECK();
     }
-    TLFEA_HIP_TRY(hipMemcpyAsync(flags, c->d_flags, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
-    TLFEA_HIP_TRY(hipMemcpyAsync(scal, c->d_scal, sizeof(double) * 8, hipMemcpyDeviceToHost, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(flags, w.flags, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(scal, w.scal, sizeof(double) * 8, hipMemcpyDeviceToHost, s));
     TLFEA_HIP_TRY(hipStreamSynchronize(s));
     if (iters_out) *iters_out = flags[2];
     if (rel_res_out) *rel_res_out = scal[4];
@@ -1627,6 +1630,89 @@ extern "C" int tlfea_cg_solve_matfree(tlfea_cg* c, const tlfea_force_plan* plan,
     }
     if (!flags[0] && max_iter > 0) return TLFEA_ENOCONV;
     return TLFEA_OK;
+}
+
+int check_matfree_args(const tlfea_force_plan* plan, const double* x, const double* y, const double* z,
+                       const double* X0, const double* Y0, const double* Z0, const double* jinv_t,
+                       const double* detJ_t, const tlfea_quad* quad, const tlfea_material* mat, const double* f,
+                       const double* u, int max_iter) {
+    TLFEA_REQUIRE(plan && x && y && z && X0 && Y0 && Z0 && jinv_t && detJ_t && quad && mat && f && u,
+                  "null argument");
+    TLFEA_REQUIRE(quad->n == (plan->nn == 10 ? 5 : 1), "quadrature size does not match the element type");
+    TLFEA_REQUIRE(max_iter >= 0, "negative iteration limit");
+    TLFEA_REQUIRE_MODEL(*mat);
+    return TLFEA_OK;
+}
+
+// ---- the workspace of tlfea_cg_solve_matfree_noalloc --------------------------------
+// A file-scope device array: every GPU the library runs on holds its own copy (72 MiB
+// plus 4.4 KiB) until the process ends.  Host code never passes `scratch` itself to a
+// kernel -- in host code the name is a stand-in, not a device address -- but the address
+// hipGetSymbolAddress returns for the current device, looked up once per device.
+constexpr int64_t NOALLOC_MAX_DOF = 3 * (int64_t)TLFEA_NOALLOC_MAX_NODES;
+constexpr int64_t NOALLOC_WORDS = 3 * NOALLOC_MAX_DOF + RED_BLOCKS + 16 + 32;
+static __device__ __align__(256) double scratch[NOALLOC_WORDS];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+int scratch_of_current_device(double** out) {
+    int dev = 0;
+    TLFEA_HIP_TRY(hipGetDevice(&dev));
+    TLFEA_REQUIRE(dev >= 0 && dev < NOALLOC_MAX_DEVICES, "device %d beyond the %d the workspace table holds", dev,
+                  NOALLOC_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || ptr == nullptr) {
+            set_error("hipGetSymbolAddress of the static CG workspace failed on device %d: %s", dev,
+                      hipGetErrorString(e));
+            return TLFEA_EHIP;
+        }
+        g_scratch_ptr[dev] = static_cast<double*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return TLFEA_OK;
+}
+}  // namespace
+
+extern "C" int tlfea_cg_solve_matfree(tlfea_cg* c, const tlfea_force_plan* plan, const double* x, const double* y,
+                                      const double* z, const double* X0, const double* Y0, const double* Z0,
+                                      const double* jinv_t, const double* detJ_t, int affine,
+                                      const tlfea_quad* quad, const tlfea_material* mat,
+                                      const unsigned char* fixed_dof_mask, const double* f,
+                                      double* u, double tol, int max_iter, int check_every, int* iters_out,
+                                      double* rel_res_out, void* stream) {
+    TLFEA_REQUIRE(c, "null argument");
+    if (int rc = check_matfree_args(plan, x, y, z, X0, Y0, Z0, jinv_t, detJ_t, quad, mat, f, u, max_iter)) return rc;
+    TLFEA_REQUIRE(plan->n_nodes == c->p->n_nodes, "plan and CG workspace sizes differ");
+    const MatfreeWork w{c->d_r, c->d_p, c->d_Kp, c->d_part, c->d_scal, c->d_flags};
+    return matfree_cg(w, plan, x, y, z, X0, Y0, Z0, jinv_t, detJ_t, affine, quad, mat, fixed_dof_mask, f, u, tol,
+                      max_iter, check_every, iters_out, rel_res_out, as_stream(stream));
+}
+
+extern "C" int tlfea_cg_solve_matfree_noalloc(const tlfea_force_plan* plan, const double* x, const double* y,
+                                              const double* z, const double* X0, const double* Y0, const double* Z0,
+                                              const double* jinv_t, const double* detJ_t, int affine,
+                                              const tlfea_quad* quad, const tlfea_material* mat,
+                                              const unsigned char* fixed_dof_mask, const double* f, double* u,
+                                              double tol, int max_iter, int check_every, int* iters_out,
+                                              double* rel_res_out, void* stream) {
+    if (int rc = check_matfree_args(plan, x, y, z, X0, Y0, Z0, jinv_t, detJ_t, quad, mat, f, u, max_iter)) return rc;
+    if (plan->n_nodes > TLFEA_NOALLOC_MAX_NODES) {
+        set_error("tlfea_cg_solve_matfree_noalloc: %d nodes exceed the static workspace's %d", plan->n_nodes,
+                  TLFEA_NOALLOC_MAX_NODES);
+        return TLFEA_ECAPACITY;
+    }
+    double* ws = nullptr;
+    if (int rc = scratch_of_current_device(&ws)) return rc;
+    const MatfreeWork w{ws, ws + NOALLOC_MAX_DOF, ws + 2 * NOALLOC_MAX_DOF, ws + 3 * NOALLOC_MAX_DOF,
+                        ws + 3 * NOALLOC_MAX_DOF + RED_BLOCKS,
+                        reinterpret_cast<int*>(ws + 3 * NOALLOC_MAX_DOF + RED_BLOCKS + 16)};
+    return matfree_cg(w, plan, x, y, z, X0, Y0, Z0, jinv_t, detJ_t, affine, quad, mat, fixed_dof_mask, f, u, tol,
+                      max_iter, check_every, iters_out, rel_res_out, as_stream(stream));
 }
 
 extern "C" int tlfea_matrix_to_f32(const tlfea_pattern* p, const double* k_sell, float* k32, void* stream) {
