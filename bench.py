@@ -406,8 +406,7 @@ def main():
     src = torch.randn(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
     dst = torch.empty_like(src)
     copy_s = event_time(lambda: dst.copy_(src))
-    acc = torch.empty((), dtype=torch.float64, device="cuda")
-    read_s = event_time(lambda: torch.sum(src, out=acc))
+    read_s = event_time(lambda: torch.sum(src))
     copy_gbs = 2 * src.numel() * 8 / copy_s / 1e9
     read_gbs = src.numel() * 8 / read_s / 1e9
     del src, dst

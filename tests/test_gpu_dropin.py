@@ -155,14 +155,17 @@ def test_update_nodal_fixed_dropin(gpu, tmp_path, fix2):
     z = 0 for the first step and then at z = 0 and x = x_min ("x0+z0": another count, the
     multipliers start at zero) or at the same nodes in reverse order ("z0:rev": same count, the
     multipliers keep their constraint numbers, as the reference's solver keeps
-    d_lambda_guess).  Three Nesterov steps against the oracle doing the same."""
-    from helpers import E_STEEL, NU_STEEL, RHO_STEEL
+    d_lambda_guess).  Three Nesterov steps against the oracle doing the same, with the
+    reference example's aluminium-like material (test_feat10_nesterov.cc: E 7e8, nu 0.33,
+    rho 2700) so that the displacements stay far above the rounding of the unit-size
+    positions they are read from."""
+    E, nu, rho0 = 7e8, 0.33, 2700.0
     m = load_mesh("cube")
     load = np.random.default_rng(5).standard_normal(3 * m.n_nodes) * 50.0
     lf = str(tmp_path / "load.bin")
     _write_raw(lf, load)
     args = ["nesterov", "--mesh", "tetgen:" + os.path.join(MESHES, "cube.1"), "--fix", "z0", "--load-file", lf,
-            "--E", repr(E_STEEL), "--nu", repr(NU_STEEL), "--rho0", repr(RHO_STEEL), "--steps", "3",
+            "--E", repr(E), "--nu", repr(nu), "--rho0", repr(rho0), "--steps", "3",
             "--inner-tol", "0", "--outer-tol", "0", "--max-outer", "2", "--max-inner", "25", "--rho", "1e10",
             "--dt", "1e-3", "--alpha", "2e-10", "--update-fixed-after", "1", "--fix2", fix2]
     info, pos = _run(args, tmp_path)
@@ -172,7 +175,7 @@ def test_update_nodal_fixed_dropin(gpu, tmp_path, fix2):
                                                        (np.abs(m.z) < 1e-8))[0]
     assert info["n_fixed"] == len(z0) and info["n_fixed_final"] == len(new)
     assert fix2 == "z0:rev" or len(new) != len(z0)
-    prob = O.Problem(m, E_STEEL, NU_STEEL, RHO_STEEL, f_ext=load, fixed=z0)
+    prob = O.Problem(m, E, nu, rho0, f_ext=load, fixed=z0)
     st = prob.new_state()
     for step in range(3):
         if step == 1:
