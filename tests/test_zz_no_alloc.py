@@ -17,6 +17,18 @@ from tlfea_amd.fea import Body, LinearStatic, ptr
 pytestmark = pytest.mark.gpu
 
 
+def _direct(m):
+    """The oracle's banded Cholesky solve of the Dirichlet-eliminated linear-static system."""
+    import oracle as O
+    from helpers import lame
+    lam, mu = lame(E_STEEL, NU_STEEL)
+    g, dj = O.grad(m)
+    off, col = O.dof_pattern(m)
+    vals = O.stiffness_values(m, g, dj, lam, mu, off, col)
+    vals, fb = O.apply_dirichlet(off, col, vals, m.f_ext, m.fixed)
+    return O.band_cholesky_solve(off, col, vals, fb)
+
+
 def _noalloc(ls, tol):
     b = ls.b
     ls.assemble()
@@ -45,12 +57,15 @@ def test_noalloc_matches_matfree_on_two_calls(gpu, first, second):
         assert rc == capi.TLFEA_OK
         assert res < tol
         # the same kernels and order of operations; the tangent's in-block LDS sums follow
-        # wave scheduling, so the two solves agree to rounding, not bit for bit.  At 1e-12
-        # the recursive residual is two decades above the FP64 floor, where two runs of the
-        # same kernels end several percent apart in iteration count (C1: 1173 and 1229
-        # measured); the displacements agree within the drop-in's 1e-10 bar
-        assert abs(it - it_ref) <= 0.1 * it_ref, (it, it_ref)
-        assert rel_err(u.cpu().numpy(), u_ref.cpu().numpy()) < 1e-10
+        # wave scheduling, so the two solves agree to rounding, not bit for bit.  Near the
+        # FP64 floor (tol 1e-12) that rounding moves the iteration count a lot -- two runs of
+        # the same kernels took 1173 / 1229 iterations on C1 and 562 / 681 on the 468-DOF
+        # TET4 beam -- so the outputs are compared: with each other and with the oracle's
+        # direct solve, at the drop-in's 1e-10 bar
+        un, ur = u.cpu().numpy(), u_ref.cpu().numpy()
+        assert rel_err(un, ur) < 1e-10
+        ud = _direct(m)
+        assert rel_err(un, ud) < 1e-10 and rel_err(ur, ud) < 1e-10
         ls.close()
         b.close()
 
