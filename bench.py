@@ -38,13 +38,19 @@ which torch.cuda.Event records on):
                      gradients 1200, detJ 40 per element; x read and f written per node);
   * hbm_copy      -- a 512 MiB device copy and a 512 MiB read-only reduction (torch), the
                      bandwidth a plain stream reaches on this device, beside the 8 TB/s peak.
-cpu_baseline: the FP64 oracle on a bounded sample (a slice of the element assembly and
-10 CG iterations on the full matrix), extrapolated to one solve with the plain GPU CG's
-iteration count.
+cpu_baseline: the FP64 oracle on the box's host cores.  Measured end to end on the
+small beam (20x2x2 cells, 3,075 DOF: assembly + Dirichlet + CG to the same tolerance) on
+all threads of the process's CPU share and on 1 thread; on the 2M-DOF beam a bounded
+sample -- the OpenMP assembly of 1/40 of the elements and 10 CG iterations on the full
+matrix -- extrapolated to one solve with the plain GPU CG's iteration count (`value`).
+config.true_rel_residual: ||(f - K u)_free|| / ||f_free|| of the timed solution, K applied
+by the GPU tangent (the CG itself stops on its recursive residual, as the reference does).
 
 `velocity_solvers` (not part of `value`): SyncedNesterov, SyncedAdamW and SyncedAdamWNocoop
 on the same beam, one implicit step of 1e-3 s with zero tolerances (one outer iteration of
-200 inner iterations), ms per inner iteration after a warm-up solve.
+200 inner iterations), ms per inner iteration after a warm-up solve; and `c1_nesterov`, the
+small beam's implicit step of 1e-3 s solved by SyncedNesterov (with the momentum restart)
+to a relative gradient of 1e-8, solves/s.
 
 `batched` (not part of `value`): the batch of 256 uneven cantilevers of the beam sampler
 (seed 2026) solved to 1e-8 -- all at once by the batched FP64 CG (finished instances
@@ -186,6 +192,46 @@ def velocity_solvers_bench(m, E, nu, rho, log, inner=200):
     return out
 
 
+def c1_nesterov_bench(E, nu, rho, tol, log, reps=3):
+    """The 20x2x2-cell beam's implicit step (dt 1e-3 s, penalty 1e10 at lambda = 0, step
+    1e-8) solved by SyncedNesterov to ||g_free|| <= tol ||g_free(0)||, with the gradient
+    restart of the momentum (the reference's sequence alone converges sublinearly)."""
+    import torch
+    from tlfea_amd import capi, mesh as M
+    from tlfea_amd.fea import Body, Solver
+    m = M.beam_config("C1")
+    b = Body(m, E, nu, rho)
+    s = Solver(b, capi.SOLVER_NESTEROV, planned=True, rest=True)
+    p = capi.NesterovParams(alpha=1e-8, rho=1e10, inner_tol=0.0, outer_tol=0.0, max_outer=1, max_inner=200000,
+                            time_step=1e-3)
+    s.set_rel_tol(tol)
+    s.set_restart(True)
+    s.set_poll_interval(256)
+
+    def run():
+        for d in range(3):
+            b.x[d].copy_(b.X0[d])
+        s.set_parameters(p)
+        s.solve()
+
+    run()  # warm-up
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        run()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / reps
+    st = s.stats()
+    out = {"workload": f"20x2x2-cell beam ({3 * m.n_nodes} DOF), one implicit step of 1e-3 s to a relative "
+                       f"gradient of {tol:g} (restarted momentum)",
+           "solves_per_s": 1.0 / dt, "s": dt, "inner_iterations": st["inner"], "restarts": s.restarts(),
+           "final_rel_gradient": st["rel"], "ms_per_inner": dt * 1e3 / max(st["inner"], 1)}
+    log(f"C1 Nesterov to {tol:g}: {dt * 1e3:.1f} ms, {st['inner']} inner iterations, {s.restarts()} restarts")
+    s.close()
+    b.close()
+    return out
+
+
 def dynamic_beams_bench(n, steps, seq_steps, tol, log):
     """Batched implicit time steps of n beams: Newton steps of the nonlinear equation (the
     headline) and steps linearised at rest, and the Newton steps of the same beams one by
@@ -309,7 +355,9 @@ def main():
         return
     if args.solvers_only:
         m = M.clamp_and_tip_load(M.kuhn_beam(nx, ny, nz), -1.0e5)
-        print(json.dumps({"velocity_solvers": velocity_solvers_bench(m, E, nu, rho, log)}))
+        vs = velocity_solvers_bench(m, E, nu, rho, log)
+        vs["c1_nesterov"] = c1_nesterov_bench(E, nu, rho, args.tol, log)
+        print(json.dumps({"velocity_solvers": vs}))
         return
     if args.batch_only:
         print(json.dumps({"batched": batched_beams(args.batch_n, args.tol, log, not args.no_seq)}))
@@ -350,6 +398,11 @@ def main():
     # the two solutions agree (same Krylov sequence, different summation order)
     u_diff = (torch.linalg.norm(ls.u - ls64.u) / torch.linalg.norm(ls64.u)).item()
     log(f"matrix-free vs assembled displacement rel diff {u_diff:.3e}")
+    # true residual of the timed solution: K u by the GPU tangent at the rest positions
+    free = body.fixed_dof_mask() == 0
+    true_res = (torch.linalg.norm((body.f_ext - body.tangent_apply(ls.u, x=body.X0))[free])
+                / torch.linalg.norm(body.f_ext[free])).item()
+    log(f"matrix-free solution: true rel. residual {true_res:.3e} (recursive {ls.res:.3e})")
 
     # ---- kernel rooflines (HIP events on the null stream) ---------------------------
     R = args.kernel_reps
@@ -419,40 +472,91 @@ def main():
         f"force (gradients read) {force_s * 1e6:.1f} us {force_gbs:.0f} GB/s; force (rest coordinates) "
         f"{rest_s * 1e6:.1f} us {rest_gbs:.0f} GB/s; copy {copy_gbs:.0f} GB/s, read {read_gbs:.0f} GB/s")
 
-    # ---- CPU baseline: oracle on a bounded sample -------------------------------------
+    # ---- CPU baseline: the oracle on the host cores ----------------------------------
     cpu = None
     if not args.no_cpu:
         import oracle as O
-        ncores = min(os.cpu_count() or 1, 16)
+        # the process's CPU share (OMP_NUM_THREADS is set to it on the GPU boxes)
+        ncores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        model = "unknown"
+        try:
+            with open("/proc/cpuinfo") as fh:
+                model = next(l.split(":", 1)[1].strip() for l in fh if l.startswith("model name"))
+        except (OSError, StopIteration):
+            pass
+        lam, mu = E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
+        # (a) the small beam end to end: assembly + Dirichlet + CG to tol
+        m1 = M.beam_config("C1")
+        g1, dj1 = O.grad(m1)
+        off1, col1 = O.dof_pattern(m1)
+
+        def c1_solve(threads, reps=5):
+            O.set_num_threads(threads)
+            t = time.perf_counter()
+            for _ in range(reps):
+                vals = O.stiffness_values(m1, g1, dj1, lam, mu, off1, col1, parallel=threads > 1)
+                ve, fb = O.apply_dirichlet(off1, col1, vals, m1.f_ext, m1.fixed)
+                _, it1, _ = O.cg(off1, col1, ve, fb, tol=args.tol, max_iter=200000)
+            return (time.perf_counter() - t) / reps, it1
+
+        c1_all, c1_iters = c1_solve(ncores)
+        c1_one, _ = c1_solve(1)
+        # the GPU's matrix-free path on the same small beam, for comparison
+        b1 = Body(m1, E, nu, rho)
+        ls1 = LinearStatic(b1, tol=args.tol, max_iter=200000, check_every=64, matfree=True)
+        X1 = [t.clone() for t in b1.x]
+
+        def gpu_c1():
+            for d in range(3):
+                b1.x[d].copy_(X1[d])
+            ls1.solve()
+        gpu_c1()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(10):
+            gpu_c1()
+        torch.cuda.synchronize()
+        c1_gpu = (time.perf_counter() - t) / 10
+        ls1.close()
+        b1.close()
+        # (b) the 2M-DOF beam: OpenMP assembly of a slice of the elements on a per-slice
+        # pattern + 10 CG iterations on the full matrix, extrapolated
         O.set_num_threads(ncores)
         go, djo = O.grad(m)
-        lam, mu = E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
-        # assembly sample: a slice of the elements on a per-slice pattern
         frac_el = max(1, m.n_elem // 40)
         sub = M.TetMesh(m.x, m.y, m.z, m.elems[:frac_el])
         soff, scol = O.dof_pattern(sub)
         sg, sdj = go[: frac_el * 150], djo[: frac_el * 5]
         ta = time.perf_counter()
-        O.stiffness_values(sub, sg, sdj, lam, mu, soff, scol)
+        O.stiffness_values(sub, sg, sdj, lam, mu, soff, scol, parallel=True)
         t_asm = (time.perf_counter() - ta) * (m.n_elem / frac_el)
-        # CG iteration sample on the full matrix (values from the GPU's CSR copy)
         iters64 = ls64.iters
         doff_t, dcol_t = body.dof_csr()
         vals = body.stiffness_csr_values(ls64.k).cpu().numpy()
         doff_n, dcol_n = doff_t.cpu().numpy(), dcol_t.cpu().numpy()
-        b = np.random.default_rng(0).standard_normal(3 * m.n_nodes)
+        bvec = np.random.default_rng(0).standard_normal(3 * m.n_nodes)
         tc = time.perf_counter()
-        O.cg(doff_n, dcol_n, vals, b, tol=0.0, max_iter=10)
+        O.cg(doff_n, dcol_n, vals, bvec, tol=0.0, max_iter=10)
         t_it = (time.perf_counter() - tc) / 10
         cpu_solve_s = t_asm + iters64 * t_it
+        del vals, doff_n, dcol_n
         cpu = {"value": 1.0 / cpu_solve_s, "unit": "solves/s", "cores": ncores, "kind": "port",
-               "sample": f"oracle FP64 stiffness assembly of {frac_el} of {m.n_elem} elements (1 thread) + 10 CG "
-                         f"iterations on the full {dof_nnz}-nnz CSR ({ncores} OpenMP threads), extrapolated to the "
+               "cpu_model": model, "nproc": os.cpu_count(),
+               "sample": f"oracle FP64 (OpenMP, {ncores} threads): stiffness assembly of {frac_el} of {m.n_elem} "
+                         f"elements + 10 CG iterations on the full {dof_nnz}-nnz CSR, extrapolated to the "
                          f"{iters64} iterations of the plain FP64 GPU CG: {t_asm:.1f} s assembly + "
-                         f"{t_it * 1e3:.1f} ms/iteration"}
-        log(f"cpu baseline: {cpu_solve_s:.1f} s per solve")
+                         f"{t_it * 1e3:.1f} ms/iteration",
+               "c1_measured": {"workload": f"20x2x2-cell beam, {3 * m1.n_nodes} DOF: assembly + Dirichlet + CG "
+                                           f"to {args.tol:g} ({c1_iters} iterations), end to end",
+                               "s_all_threads": c1_all, "threads": ncores, "s_1_thread": c1_one,
+                               "gpu_matfree_s": c1_gpu}}
+        log(f"cpu baseline ({model}, {ncores} of {os.cpu_count()} threads): C2 {cpu_solve_s:.1f} s per solve "
+            f"(extrapolated); C1 measured {c1_all * 1e3:.1f} ms ({ncores} threads), {c1_one * 1e3:.1f} ms "
+            f"(1 thread), GPU {c1_gpu * 1e3:.1f} ms")
 
     velocity = None if args.no_solvers else velocity_solvers_bench(m, E, nu, rho, log)
+    if velocity is not None:
+        velocity["c1_nesterov"] = c1_nesterov_bench(E, nu, rho, args.tol, log)
     batched = None if args.no_batch else batched_beams(args.batch_n, args.tol, log, not args.no_seq)
     dynamic = None if args.no_dynamic else dynamic_beams_bench(64, 200, 20, args.tol, log)
 
@@ -479,7 +583,8 @@ def main():
                                f"on K(x) + position update), cantilever {nx}x{ny}x{nz} cells x 6 TET10 "
                                f"({m.n_elem} el, {m.n_nodes} nodes, {3 * m.n_nodes} DOF), 1e5 N tip load",
                    "n_elem": m.n_elem, "n_dof": 3 * m.n_nodes, "cg_iterations": iters,
-                   "final_rel_residual": ls.res, "K_nnz": dof_nnz, "u_rel_diff_vs_assembled": u_diff,
+                   "final_rel_residual": ls.res, "true_rel_residual": true_res, "K_nnz": dof_nnz,
+                   "u_rel_diff_vs_assembled": u_diff,
                    "plain_fp64": {"solves_per_s": args.steps / dt64, "ms_per_solve": dt64 / args.steps * 1e3,
                                   "cg_iterations": ls64.iters, "final_rel_residual": ls64.res},
                    "speedup_vs_plain_fp64": dt64 / dt},

@@ -501,6 +501,14 @@ int tlfea_solver_set_parameters(tlfea_solver* s, const void* params);
  * ||g_free|| <= rel_tol * ||g_free at the first inner iteration|| (fixed DOFs excluded).
  * 0 (the default) disables it. */
 int tlfea_solver_set_rel_tol(tlfea_solver* s, double rel_tol);
+/* Optional momentum restart of the Nesterov solver, not in the reference (whose t-sequence
+ * SyncedNesterov.cu:152-175 never restarts, so its iterates converge sublinearly): with
+ * on = 1, t restarts at 1 whenever g(y_k).(v_(k+1) - v_k) > 0 (the gradient test of
+ * O'Donoghue and Candes, 2015), which makes the inner loop converge linearly on these
+ * strongly convex steps.  0 (the default) keeps the reference's sequence.  EINVAL for the
+ * AdamW kinds.  tlfea_solver_restarts gives the count of the last Solve. */
+int tlfea_solver_set_restart(tlfea_solver* s, int on);
+int tlfea_solver_restarts(const tlfea_solver* s, int* restarts);
 /* How often (in inner iterations) the host reads the device convergence flag to stop
  * launching; results do not depend on it.  Default 16. */
 int tlfea_solver_set_poll_interval(tlfea_solver* s, int every);

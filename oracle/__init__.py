@@ -28,6 +28,8 @@ Parity status of each function (how it is pinned, see tests/test_oracle.py):
   (and its major symmetry), frame indifference P(QF) = Q P(F), the small-strain limit
   (linear elasticity with mu = 2 (mu10 + mu01), bulk modulus kappa), f_int = finite
   difference of the total MR energy, K = finite difference of f_int.
+* ``tangent_apply`` -- equal to ``spmv`` of the assembled ``stiffness_values`` (SVK and
+  Mooney-Rivlin, with damping, deformed states), symmetric; ``true_rel_residual`` uses it.
 * ``cg`` / ``band_cholesky_solve`` -- each against dense numpy solves, Euler-Bernoulli
   band on the beam.
 * ``nesterov_step`` -- the iterate sequence against an independent pure-Python
@@ -304,17 +306,45 @@ def mass_values_exact(mesh, detJ, rho, off, col):
     return vals
 
 
-def stiffness_values(mesh, g, detJ, lam, mu, off, col, x=None, y=None, z=None, h=1.0, eta=0.0, lambda_d=0.0):
+def stiffness_values(mesh, g, detJ, lam, mu, off, col, x=None, y=None, z=None, h=1.0, eta=0.0, lambda_d=0.0,
+                     parallel=False):
     nq = 5 if mesh.nn == 10 else 1
     qw = quad(mesh.nn)[3]
     x = _f64(mesh.x if x is None else x)
     y = _f64(mesh.y if y is None else y)
     z = _f64(mesh.z if z is None else z)
     vals = np.zeros(len(col))
-    lib().orc_stiffness_values(mesh.nn, nq, mesh.n_elem, _i(_i32(mesh.conn_colmajor())), _d(x), _d(y), _d(z),
+    fn = lib().orc_stiffness_values_omp if parallel else lib().orc_stiffness_values
+    fn(mesh.nn, nq, mesh.n_elem, _i(_i32(mesh.conn_colmajor())), _d(x), _d(y), _d(z),
                                _d(_f64(g)), _d(_f64(detJ)), _d(qw), C.c_double(lam), C.c_double(mu), C.c_double(eta),
                                C.c_double(lambda_d), C.c_double(h), _i(_i32(off)), _i(_i32(col)), _d(vals))
     return vals
+
+
+def tangent_apply(mesh, g, detJ, lam, mu, p, x=None, y=None, z=None, h=1.0, eta=0.0, lambda_d=0.0):
+    """(h K(x) + C) p element by element (OpenMP), the operator ``stiffness_values``
+    assembles, without the CSR."""
+    nq = 5 if mesh.nn == 10 else 1
+    qw = quad(mesh.nn)[3]
+    x = _f64(mesh.x if x is None else x)
+    y = _f64(mesh.y if y is None else y)
+    z = _f64(mesh.z if z is None else z)
+    out = np.zeros(3 * mesh.n_nodes)
+    lib().orc_tangent_apply(mesh.nn, nq, mesh.n_elem, mesh.n_nodes, _i(_i32(mesh.conn_colmajor())), _d(x), _d(y),
+                            _d(z), _d(_f64(g)), _d(_f64(detJ)), _d(qw), C.c_double(lam), C.c_double(mu),
+                            C.c_double(eta), C.c_double(lambda_d), C.c_double(h), _d(_f64(p)), _d(out))
+    return out
+
+
+def true_rel_residual(mesh, g, detJ, lam, mu, u, f, fixed):
+    """||(f - K u)_free|| / ||f_free|| of the Dirichlet-eliminated linear-static system
+    (LinearStaticSolver.cu:316-342: fixed DOFs removed, u = 0 there) at the rest positions:
+    the true residual behind the CG's stopping rule sqrt(rr/rr0) (:418)."""
+    free = np.ones(3 * mesh.n_nodes, bool)
+    free.reshape(-1, 3)[np.asarray(fixed, np.int64)] = False
+    uu = np.where(free, np.asarray(u, np.float64).reshape(-1), 0.0)
+    r = (np.asarray(f, np.float64).reshape(-1) - tangent_apply(mesh, g, detJ, lam, mu, uu))[free]
+    return float(np.linalg.norm(r) / np.linalg.norm(np.asarray(f, np.float64).reshape(-1)[free]))
 
 
 def apply_dirichlet(off, col, vals, f, fixed):

@@ -539,74 +539,169 @@ ORC_API void orc_mass_values(int nn, int nq, int n_elem, const int* conn, const 
 // `src/elements/FEAT10DataFunc.cuh:512-768`, `src/materials/SVK.cuh:41-67`; the
 // Mooney-Rivlin block from mr_A.
 // ---------------------------------------------------------------------------------------
+// Per (e, q) kinematics of the tangent: node ids, g_a, F g_a, tr E, F F^T, dV and (Mooney-
+// Rivlin) A = dP/dF.
+struct TanPoint {
+    int gn[10];
+    double g[10][3], Fh[10][3], FFT[3][3], trE, dV;
+    double A[3][3][3][3];
+};
+
+static void tangent_point(const Elem& E, int e, int q, const double* x, const double* y, const double* z,
+                          const double* grad, const double* detJ, const double* qw, TanPoint& t) {
+    const int nn = E.nn, nq = E.nq;
+    const double* G = grad + ((int64_t)e * nq + q) * nn * 3;
+    double F[3][3] = {{0}};
+    for (int a = 0; a < nn; a++) {
+        t.gn[a] = E.node(e, a);
+        double xa[3] = {x[t.gn[a]], y[t.gn[a]], z[t.gn[a]]};
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) F[i][j] += xa[i] * G[a + nn * j];
+    }
+    double C[3][3] = {{0}};
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            t.FFT[i][j] = 0.0;
+            for (int k = 0; k < 3; k++) {
+                C[i][j] += F[k][i] * F[k][j];
+                t.FFT[i][j] += F[i][k] * F[j][k];
+            }
+        }
+    t.trE = 0.5 * (C[0][0] + C[1][1] + C[2][2] - 3.0);
+    for (int a = 0; a < nn; a++) {
+        for (int d = 0; d < 3; d++) t.g[a][d] = G[a + nn * d];
+        for (int r = 0; r < 3; r++) {
+            t.Fh[a][r] = 0.0;
+            for (int c = 0; c < 3; c++) t.Fh[a][r] += F[r][c] * t.g[a][c];
+        }
+    }
+    t.dV = detJ[e * nq + q] * qw[q];
+    // Mooney-Rivlin block: K_ij[d][c] = sum_JL A[d][J][c][L] g_i[J] g_j[L] dV
+    // (FEAT10DataFunc.cuh:607-640)
+    if (g_model == 1) mr_A(F, g_mu10, g_mu01, g_kappa, t.A);
+}
+
+// Entry [d][c] of the (i, j) node block: h K + C (Kelvin-Voigt, unscaled) at one point.
+// K (SVK.cuh:41-67): lambda Fh_i,d Fh_j,c + lambda trE (g_i.g_j) delta + mu (Fh_j.Fh_i) delta
+// + mu Fh_j,d Fh_i,c + mu (g_i.g_j) (F F^T)_dc - mu (g_i.g_j) delta, times dV.
+// C (FEAT10DataFunc.cuh:686-767): eta Fh_j (x) Fh_i + eta F F^T (g_i.g_j) + lambda_d Fh_i (x) Fh_j.
+static inline double tangent_entry(const TanPoint& t, int i, int j, int d, int c, double lambda, double mu,
+                                   double eta_d, double lambda_d, double h) {
+    const double hij = t.g[j][0] * t.g[i][0] + t.g[j][1] * t.g[i][1] + t.g[j][2] * t.g[i][2];
+    const double FjFi = t.Fh[j][0] * t.Fh[i][0] + t.Fh[j][1] * t.Fh[i][1] + t.Fh[j][2] * t.Fh[i][2];
+    const double dl = (d == c) ? 1.0 : 0.0;
+    double K = (lambda * t.Fh[i][d] * t.Fh[j][c] + lambda * t.trE * hij * dl + mu * FjFi * dl +
+                mu * t.Fh[j][d] * t.Fh[i][c] + mu * hij * t.FFT[d][c] - mu * hij * dl) *
+               t.dV;
+    if (g_model == 1) {
+        K = 0.0;
+        for (int J = 0; J < 3; J++)
+            for (int L = 0; L < 3; L++) K += t.A[d][J][c][L] * t.g[i][J] * t.g[j][L];
+        K *= t.dV;
+    }
+    double Cv = 0.0;
+    if (eta_d != 0.0 || lambda_d != 0.0)
+        Cv = (eta_d * (t.Fh[j][d] * t.Fh[i][c]) + eta_d * t.FFT[d][c] * hij + lambda_d * (t.Fh[i][d] * t.Fh[j][c])) *
+             t.dV;
+    return h * K + Cv;
+}
+
 ORC_API void orc_stiffness_values(int nn, int nq, int n_elem, const int* conn, const double* x, const double* y,
                                   const double* z, const double* grad, const double* detJ, const double* qw,
                                   double lambda, double mu, double eta_d, double lambda_d, double h,
                                   const int* offsets, const int* cols, double* vals) {
     Elem E{n_elem, nn, nq, conn};
+    static thread_local TanPoint t;
     for (int e = 0; e < n_elem; e++)
         for (int q = 0; q < nq; q++) {
-            const double* G = grad + ((int64_t)e * nq + q) * nn * 3;
-            int gn[10];
-            double F[3][3] = {{0}};
-            for (int a = 0; a < nn; a++) {
-                gn[a] = E.node(e, a);
-                double xa[3] = {x[gn[a]], y[gn[a]], z[gn[a]]};
-                for (int i = 0; i < 3; i++)
-                    for (int j = 0; j < 3; j++) F[i][j] += xa[i] * G[a + nn * j];
-            }
-            double C[3][3] = {{0}}, FFT[3][3] = {{0}};
-            for (int i = 0; i < 3; i++)
-                for (int j = 0; j < 3; j++)
-                    for (int k = 0; k < 3; k++) {
-                        C[i][j] += F[k][i] * F[k][j];
-                        FFT[i][j] += F[i][k] * F[j][k];
-                    }
-            double trE = 0.5 * (C[0][0] + C[1][1] + C[2][2] - 3.0);
-            double Fh[10][3], g[10][3];
-            for (int a = 0; a < nn; a++) {
-                for (int d = 0; d < 3; d++) g[a][d] = G[a + nn * d];
-                for (int r = 0; r < 3; r++) {
-                    Fh[a][r] = 0.0;
-                    for (int c = 0; c < 3; c++) Fh[a][r] += F[r][c] * g[a][c];
-                }
-            }
-            double dV = detJ[e * nq + q] * qw[q];
-            const bool damp = (eta_d != 0.0 || lambda_d != 0.0);
-            // Mooney-Rivlin block: K_ij[d][c] = sum_JL A[d][J][c][L] g_i[J] g_j[L] dV
-            // (FEAT10DataFunc.cuh:607-640)
-            static thread_local double A[3][3][3][3];
-            if (g_model == 1) mr_A(F, g_mu10, g_mu01, g_kappa, A);
+            tangent_point(E, e, q, x, y, z, grad, detJ, qw, t);
             for (int i = 0; i < nn; i++)
-                for (int j = 0; j < nn; j++) {
-                    double hij = g[j][0] * g[i][0] + g[j][1] * g[i][1] + g[j][2] * g[i][2];
-                    double FjFi = Fh[j][0] * Fh[i][0] + Fh[j][1] * Fh[i][1] + Fh[j][2] * Fh[i][2];
+                for (int j = 0; j < nn; j++)
                     for (int d = 0; d < 3; d++) {
-                        int row = 3 * gn[i] + d;
+                        const int row = 3 * t.gn[i] + d;
                         for (int c = 0; c < 3; c++) {
-                            double dl = (d == c) ? 1.0 : 0.0;
-                            double K = (lambda * Fh[i][d] * Fh[j][c] + lambda * trE * hij * dl + mu * FjFi * dl +
-                                        mu * Fh[j][d] * Fh[i][c] + mu * hij * FFT[d][c] - mu * hij * dl) *
-                                       dV;
-                            if (g_model == 1) {
-                                K = 0.0;
-                                for (int J = 0; J < 3; J++)
-                                    for (int L = 0; L < 3; L++) K += A[d][J][c][L] * g[i][J] * g[j][L];
-                                K *= dV;
-                            }
-                            double Cv = 0.0;
-                            if (damp) {
-                                // C_ij block: eta Fh_j (x) Fh_i + eta FFT (g_i.g_j) + lambda_d Fh_i (x) Fh_j
-                                Cv = (eta_d * (Fh[j][d] * Fh[i][c]) + eta_d * FFT[d][c] * hij +
-                                      lambda_d * (Fh[i][d] * Fh[j][c])) *
-                                     dV;
-                            }
-                            int k = find_col(cols, offsets[row], offsets[row + 1], 3 * gn[j] + c);
-                            if (k >= 0) vals[k] += h * K + Cv;
+                            const int k = find_col(cols, offsets[row], offsets[row + 1], 3 * t.gn[j] + c);
+                            if (k >= 0) vals[k] += tangent_entry(t, i, j, d, c, lambda, mu, eta_d, lambda_d, h);
                         }
                     }
-                }
         }
+}
+
+// The same assembly with the elements split over the OpenMP threads and atomic adds into
+// the values: for timing the CPU baseline on all cores (the order of the adds, so the last
+// bits, depend on the thread schedule; the tests use the sequential assembly above).
+ORC_API void orc_stiffness_values_omp(int nn, int nq, int n_elem, const int* conn, const double* x, const double* y,
+                                      const double* z, const double* grad, const double* detJ, const double* qw,
+                                      double lambda, double mu, double eta_d, double lambda_d, double h,
+                                      const int* offsets, const int* cols, double* vals) {
+    Elem E{n_elem, nn, nq, conn};
+#pragma omp parallel
+    {
+        TanPoint t;
+#pragma omp for schedule(static)
+        for (int e = 0; e < n_elem; e++)
+            for (int q = 0; q < nq; q++) {
+                tangent_point(E, e, q, x, y, z, grad, detJ, qw, t);
+                for (int i = 0; i < nn; i++)
+                    for (int j = 0; j < nn; j++)
+                        for (int d = 0; d < 3; d++) {
+                            const int row = 3 * t.gn[i] + d;
+                            for (int c = 0; c < 3; c++) {
+                                const int k = find_col(cols, offsets[row], offsets[row + 1], 3 * t.gn[j] + c);
+                                if (k < 0) continue;
+                                const double v = tangent_entry(t, i, j, d, c, lambda, mu, eta_d, lambda_d, h);
+#pragma omp atomic
+                                vals[k] += v;
+                            }
+                        }
+            }
+    }
+}
+
+// out = (h K(x) + C) p element by element, nothing assembled: the operator the assembled
+// values above hold, applied as sum_e sum_q K_e,q p_e.  Elements are split over the OpenMP
+// threads, each summing into its own array; the arrays are added in thread order.  (Not in
+// the reference: the true residual f - K u of large solutions without the 2 GB CSR.)
+ORC_API void orc_tangent_apply(int nn, int nq, int n_elem, int n_nodes, const int* conn, const double* x,
+                               const double* y, const double* z, const double* grad, const double* detJ,
+                               const double* qw, double lambda, double mu, double eta_d, double lambda_d, double h,
+                               const double* p, double* out) {
+    Elem E{n_elem, nn, nq, conn};
+    const int64_t n3 = 3 * (int64_t)n_nodes;
+    int nt = 1;
+#ifdef _OPENMP
+    nt = omp_get_max_threads();
+#endif
+    std::vector<std::vector<double>> part(nt);
+#pragma omp parallel num_threads(nt)
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        std::vector<double>& acc = part[tid];
+        acc.assign(n3, 0.0);
+        TanPoint t;
+#pragma omp for schedule(static)
+        for (int e = 0; e < n_elem; e++)
+            for (int q = 0; q < nq; q++) {
+                tangent_point(E, e, q, x, y, z, grad, detJ, qw, t);
+                for (int i = 0; i < nn; i++)
+                    for (int d = 0; d < 3; d++) {
+                        double s = 0.0;
+                        for (int j = 0; j < nn; j++)
+                            for (int c = 0; c < 3; c++)
+                                s += tangent_entry(t, i, j, d, c, lambda, mu, eta_d, lambda_d, h) * p[3 * t.gn[j] + c];
+                        acc[3 * t.gn[i] + d] += s;
+                    }
+            }
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < n3; k++) {
+        double s = 0.0;
+        for (int t = 0; t < nt; t++) s += part[t][k];
+        out[k] = s;
+    }
 }
 
 // Dirichlet elimination: fixed rows -> identity row, f = 0; free rows -> fixed columns 0.

@@ -74,17 +74,21 @@ def _inv3(J):
     return c / det[:, None, None], det
 
 
-def fint_extended(m, x, y, z, lam, mu):
+def fint_extended(m, x, y, z, lam, mu, u=None):
     """SVK internal force (compute_internal_force, FEAT10DataFunc.cuh:404-460, with the
     gradients of FEAT10Data.cu:92-198) in numpy longdouble (64-bit mantissa), with the
     strain from the displacement gradient: a reference for small-strain states, where
-    forming F^T F - I in double loses digits."""
+    forming F^T F - I in double loses digits.  ``u`` (n_nodes x 3) gives the displacement
+    itself, so that no rounding of x = X + u to the coordinates' precision enters."""
     import oracle as O
     ld = np.longdouble
     qx, qy, qz, qw = O.quad(m.nn)
     X = m.xyz()[m.elems].astype(ld)
-    xc = np.stack([x, y, z], axis=1)[m.elems].astype(ld)
-    u = xc - X  # exact: differences of doubles fit the longer mantissa here
+    if u is None:
+        xc = np.stack([x, y, z], axis=1)[m.elems].astype(ld)
+        u = xc - X  # exact: differences of doubles fit the longer mantissa here
+    else:
+        u = np.asarray(u, dtype=ld)[m.elems]
     f = np.zeros((m.n_nodes, 3), dtype=ld)
     lam, mu = ld(lam), ld(mu)
     edges = ((0, 1), (1, 2), (0, 2), (0, 3), (1, 3), (2, 3))

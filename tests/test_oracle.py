@@ -758,3 +758,47 @@ def test_constraint_jacobian_csr_is_the_fixed_dof_selection():
     assert np.array_equal(np.asarray(JT.sum(axis=1)).ravel(), mark)
     for r in range(3 * n_nodes):  # columns ascending within each J^T row
         assert np.all(np.diff(ct[ot[r]:ot[r + 1]]) > 0)
+
+
+@pytest.mark.parametrize("name", ["C1", "kuhn_t4", "cube"])
+@pytest.mark.parametrize("model", ["svk", "svk_damped", "mr"])
+def test_tangent_apply_equals_assembled_spmv(name, model):
+    """The element-by-element (h K + C) p equals the assembled values times p at a deformed
+    state (a wrong block index or a transposed block fails it), and p.Kq = q.Kp."""
+    m = load_mesh(name)
+    lam, mu = lame(E_STEEL, NU_STEEL)
+    g, dj = O.grad(m)
+    x, y, z = deformed(m, seed=4, amp=2e-3)
+    eta, ld = (0.3 * mu, 0.2 * lam) if model == "svk_damped" else (0.0, 0.0)
+    off, col = O.dof_pattern(m)
+    rng = np.random.default_rng(8)
+    p, q = rng.standard_normal((2, 3 * m.n_nodes))
+    ctx = O.material("mr", 0.3 * mu, 0.1 * mu, lam + 2 * mu / 3) if model == "mr" else O.material()
+    with ctx:
+        vals = O.stiffness_values(m, g, dj, lam, mu, off, col, x=x, y=y, z=z, h=0.7, eta=eta, lambda_d=ld)
+        kp = O.tangent_apply(m, g, dj, lam, mu, p, x=x, y=y, z=z, h=0.7, eta=eta, lambda_d=ld)
+        kq = O.tangent_apply(m, g, dj, lam, mu, q, x=x, y=y, z=z, h=0.7, eta=eta, lambda_d=ld)
+    assert rel_err(kp, O.spmv(off, col, vals, p)) < 1e-13
+    assert abs(q @ kp - p @ kq) < 1e-12 * np.linalg.norm(kp) * np.linalg.norm(q)
+
+
+def test_true_rel_residual_of_direct_solve_and_of_a_perturbation():
+    """The true residual of the banded Cholesky solution on C1 is at the FP64 floor
+    eps ||K||_1 ||u|| / ||f|| (1.9e-10 here: the beam is stiff), and a perturbation e of it
+    gives exactly ||K e|| / ||f|| on the free DOFs."""
+    m = load_mesh("C1")
+    lam, mu = lame(E_STEEL, NU_STEEL)
+    g, dj = O.grad(m)
+    off, col = O.dof_pattern(m)
+    vals = O.stiffness_values(m, g, dj, lam, mu, off, col)
+    ve, fb = O.apply_dirichlet(off, col, vals, m.f_ext, m.fixed)
+    u = O.band_cholesky_solve(off, col, ve, fb)
+    k1 = np.bincount(col, weights=np.abs(ve), minlength=u.size).max()  # ||K||_1 (symmetric)
+    floor = np.finfo(float).eps * k1 * np.linalg.norm(u) / np.linalg.norm(fb)
+    assert 1e-12 < floor < 1e-9
+    assert O.true_rel_residual(m, g, dj, lam, mu, u, m.f_ext, m.fixed) < 20 * floor
+    e = np.random.default_rng(2).standard_normal(u.size) * 1e-3 * np.abs(u).max()
+    e.reshape(-1, 3)[m.fixed] = 0.0
+    want = np.linalg.norm(O.spmv(off, col, ve, e)) / np.linalg.norm(fb)
+    got = O.true_rel_residual(m, g, dj, lam, mu, u + e, m.f_ext, m.fixed)
+    assert abs(got - want) < 1e-6 * want

@@ -28,6 +28,7 @@ struct tlfea_solver {
     bool params_set = false;
     double rel_tol = 0.0;
     int poll = 16;
+    int restart = 0;  // Nesterov gradient restart (tlfea_solver_set_restart)
     int64_t n3 = 0;
     // state
     double *v_guess = nullptr, *v_prev = nullptr, *lam = nullptr;
@@ -35,17 +36,18 @@ struct tlfea_solver {
     double *vk = nullptr, *vn = nullptr;  // Nesterov iterates
     double *g = nullptr, *m1 = nullptr, *m2 = nullptr;
     double* dv = nullptr;  // [n_nodes][4]: v_guess - v_prev per node, padded to 32 bytes
+    double *ux = nullptr, *uy = nullptr, *uz = nullptr;  // displacements x - X0 (rest form)
     // constraints per node, built from the fixed list h_fixed
     int *cons_off = nullptr, *cons_idx = nullptr;
     unsigned char* node_fixed = nullptr;
     std::vector<int> h_fixed;
     // reductions
-    double* part = nullptr;  // [4][max_blocks]
+    double* part = nullptr;  // [5][max_blocks]
     int max_blocks = 0;
     double* scal = nullptr;  // device scalars, see k_finalize
     int* flags = nullptr;    // 0 inner_flag, 1 outer_flag, 2 inner count, 3 outer count
     // stats
-    int last_inner = 0, last_outer = 0;
+    int last_inner = 0, last_outer = 0, last_restarts = 0;
     double last_norm_g = 0.0, last_rel = 0.0;
 };
 
@@ -101,6 +103,8 @@ __global__ void k_nes_outer_init(int64_t n3, const double* __restrict__ vg, doub
         scal[0] = 0.0;  // prev_norm_g
         scal[1] = 0.0;  // norm_g
         scal[4] = -1.0; // norm_g_free at first inner iteration
+        scal[11] = 1.0;                       // t_k of the device t-sequence (restart mode)
+        scal[12] = 0.5 * (1.0 + sqrt(5.0));   // t_(k+1)
     }
 }
 
@@ -109,15 +113,40 @@ __global__ void k_nes_outer_init(int64_t n3, const double* __restrict__ vg, doub
 // v_km1 on entry and v_k on exit; vn holds v_k.
 // dv = y - v_prev, the same difference the gradient's mass term forms per entry
 // (SyncedNesterov.cu:54-55), formed once per node for the gradient kernel's gathers.
-__global__ void k_nes_predict(int n_nodes, double beta, double dt, double* __restrict__ vk, const double* __restrict__ vn,
-                              double* __restrict__ vg, const double* __restrict__ vp, double* __restrict__ dv,
-                              const double* __restrict__ xp, const double* __restrict__ yp,
-                              const double* __restrict__ zp, double* __restrict__ x, double* __restrict__ y,
-                              double* __restrict__ z, double* __restrict__ f_int, const int* __restrict__ flags) {
+// With tseq (restart mode) beta = (t_k - 1) / t_(k+1) comes from the device t-sequence
+// (scal[11], scal[12]) instead of the argument.
+// Displacements (u non-null, rest positions X0 known): u = (x_prev - X0) + dt y as well, for
+// the force kernel: x_prev - X0 is exact while x_prev stays within a factor 2 of X0
+// (Sterbenz), so u carries the rounding of its own size, not that of the coordinates.
+struct DispOut {
+    double *ux, *uy, *uz;
+    const double *X0, *Y0, *Z0;
+};
+__device__ __forceinline__ void write_positions(int64_t i, int d, double dt, double yv, const double* __restrict__ xp,
+                                                const double* __restrict__ yp, const double* __restrict__ zp,
+                                                double* __restrict__ x, double* __restrict__ y,
+                                                double* __restrict__ z, const DispOut& u) {
+    double* const xo = d == 0 ? x : (d == 1 ? y : z);
+    const double* const xq = d == 0 ? xp : (d == 1 ? yp : zp);
+    xo[i] = xq[i] + dt * yv;
+    if (u.ux) {
+        double* const uo = d == 0 ? u.ux : (d == 1 ? u.uy : u.uz);
+        const double* const Xq = d == 0 ? u.X0 : (d == 1 ? u.Y0 : u.Z0);
+        uo[i] = (xq[i] - Xq[i]) + dt * yv;
+    }
+}
+
+__global__ void k_nes_predict(int n_nodes, double beta, const double* __restrict__ tseq, double dt,
+                              double* __restrict__ vk, const double* __restrict__ vn, double* __restrict__ vg,
+                              const double* __restrict__ vp, double* __restrict__ dv, const double* __restrict__ xp,
+                              const double* __restrict__ yp, const double* __restrict__ zp, double* __restrict__ x,
+                              double* __restrict__ y, double* __restrict__ z, DispOut u, double* __restrict__ f_int,
+                              const int* __restrict__ flags) {
     if (flags[0] | flags[1]) return;
     // one thread per DOF: the per-DOF streams are read and written coalesced
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= 3 * (int64_t)n_nodes) return;
+    if (tseq) beta = (tseq[11] - 1.0) / tseq[12];
     const int64_t i = k / 3;
     const int d = (int)(k - 3 * i);
     const double vkk = vn[k], km1 = vk[k];
@@ -126,15 +155,14 @@ __global__ void k_nes_predict(int n_nodes, double beta, double dt, double* __res
     vk[k] = vkk;
     if (f_int) f_int[k] = 0.0;
     dv[4 * i + d] = yv - vp[k];  // component d of the node's padded difference
-    double* const xo = d == 0 ? x : (d == 1 ? y : z);
-    const double* const xq = d == 0 ? xp : (d == 1 ? yp : zp);
-    xo[i] = xq[i] + dt * yv;
+    write_positions(i, d, dt, yv, xp, yp, zp, x, y, z, u);
 }
 
 // Per node row on the sliced mass layout:
 //   g = M (v_guess - v_prev)/dt + f_int - f_ext + dt sum_{c at this DOF} (lambda_c + rho c_c)
 // (SyncedNesterov.cu:32-95), with c = x - x_ref at fixed nodes (FEAT10DataFunc.cuh:470-481).
-// mode 0 (Nesterov): v_next = y - alpha g -> vn; partials |g|^2, |g_free|^2, |v_next|^2, |v_k|^2.
+// mode 0 (Nesterov): v_next = y - alpha g -> vn; partials |g|^2, |g_free|^2, |v_next|^2, |v_k|^2
+//   and g.(v_next - v_k) (the restart test).
 // mode 1 (AdamW): g -> gbuf; partials |g|^2, |g_free|^2, |v_guess|^2.
 // The mass term gathers dv = v_guess - v_prev per column (written by the predict kernels),
 // so each entry adds mass_ij * dv_j * inv_dt exactly as the per-entry difference would.
@@ -151,7 +179,7 @@ __global__ __launch_bounds__(S_RED) void k_grad(Body b, int64_t n_slices, const 
     if (flags[0] | flags[1]) return;
     const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
     const int64_t t = (int64_t)blk * blockDim.x + threadIdx.x;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (t < n_slices * TILE) {
         const int row = sell_row[t];
         const int64_t sl = t >> 6;
@@ -220,6 +248,7 @@ __global__ __launch_bounds__(S_RED) void k_grad(Body b, int64_t n_slices, const 
                     acc[2] += vnx * vnx;
                     const double k = vk[3 * row + d];
                     acc[3] += k * k;
+                    acc[4] += gr[d] * (vnx - k);
                 }
             } else {
 #pragma unroll
@@ -230,29 +259,32 @@ __global__ __launch_bounds__(S_RED) void k_grad(Body b, int64_t n_slices, const 
             }
         }
     }
-    block_sums<4>(acc);
+    block_sums<5>(acc);
     if (threadIdx.x == 0)
 #pragma unroll
-        for (int k = 0; k < 4; k++) part[k * nblk + blk] = acc[k];
+        for (int k = 0; k < 5; k++) part[k * nblk + blk] = acc[k];
 }
 
 // Combines the partials; sets flags.
 //   scal: 0 prev_norm_g, 1 norm_g, 2 norm_v_next, 3 norm_v_k (or |v|), 4 gfree0, 5 gfree,
 //         6 norm_g0 (AdamW), 7 rel
 // kind 0 Nesterov: stop if inner>0 and (| |g|-|g_prev| | < tol or | |v_next|-|v_k| | < tol)
-//   (SyncedNesterov.cu:226-271); prev_norm_g = norm_g.
+//   (SyncedNesterov.cu:226-271); prev_norm_g = norm_g.  With restart, advance the device
+//   t-sequence (scal[11] = t_(k+1), scal[12] = t_(k+2)), or restart it at t_(k+1) = 1 when
+//   g.(v_next - v_k) > 0 (the gradient test of O'Donoghue & Candes, Found. Comput. Math. 15
+//   (2015) 715-732): the next step then carries no momentum.
 // kind 1/2 AdamW (on check iterations only): stop if |g| <= tol (1+|v|) or
 //   (rtol > 0 and |g| <= rtol |g0|) (SyncedAdamW.cu:238-272).
 __global__ __launch_bounds__(S_RED) void k_finalize(int kind, int inner, int check, int nblk,
                                                     const double* __restrict__ part, double* __restrict__ scal,
                                                     int* __restrict__ flags, double inner_tol, double inner_rtol,
-                                                    double rel_tol) {
+                                                    double rel_tol, int restart) {
     if (flags[0] | flags[1]) return;
-    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (int i = threadIdx.x; i < nblk; i += blockDim.x)
 #pragma unroll
-        for (int k = 0; k < 4; k++) s[k] += part[k * nblk + i];
-    block_sums<4>(s);
+        for (int k = 0; k < 5; k++) s[k] += part[k * nblk + i];
+    block_sums<5>(s);
     if (threadIdx.x != 0) return;
     const double ng = sqrt(s[0]), ngf = sqrt(s[1]);
     flags[2] += 1;
@@ -268,6 +300,12 @@ __global__ __launch_bounds__(S_RED) void k_finalize(int kind, int inner, int che
         scal[0] = ng;
         scal[2] = nvn;
         scal[3] = nvk;
+        if (restart) {
+            const double t1 = s[4] > 0.0 ? 1.0 : scal[12];
+            scal[11] = t1;
+            scal[12] = 0.5 * (1.0 + sqrt(1.0 + 4.0 * t1 * t1));
+            if (s[4] > 0.0) flags[6] += 1;  // restarts of this solve
+        }
     } else if (check) {
         scal[1] = ng;
         if (scal[6] < 0.0) scal[6] = ng;
@@ -312,7 +350,7 @@ __global__ void k_adam_predict(int n_nodes, int nocoop, double lr, double c1, do
                                double* __restrict__ m1, double* __restrict__ m2, double* __restrict__ vg,
                                const double* __restrict__ vp, double* __restrict__ dv, const double* __restrict__ xp,
                                const double* __restrict__ yp, const double* __restrict__ zp, double* __restrict__ x,
-                               double* __restrict__ y, double* __restrict__ z, double* __restrict__ f_int,
+                               double* __restrict__ y, double* __restrict__ z, DispOut u, double* __restrict__ f_int,
                                const int* __restrict__ flags) {
     if (flags[0] | flags[1]) return;
     // one thread per DOF: the five per-DOF streams are read and written coalesced
@@ -339,9 +377,7 @@ __global__ void k_adam_predict(int n_nodes, int nocoop, double lr, double c1, do
     vg[k] = yv;
     if (f_int) f_int[k] = 0.0;
     dv[4 * i + d] = yv - vp[k];  // component d of the node's padded difference
-    double* const xo = d == 0 ? x : (d == 1 ? y : z);
-    const double* const xq = d == 0 ? xp : (d == 1 ? yp : zp);
-    xo[i] = xq[i] + dt * yv;
+    write_positions(i, d, dt, yv, xp, yp, zp, x, y, z, u);
 }
 
 // ------------------------------------------------------------------------------------
@@ -443,7 +479,8 @@ int dalloc(T** p, int64_t n) {
 void free_solver(tlfea_solver* s) {
     if (!s) return;
     for (void* p : {(void*)s->v_guess, (void*)s->v_prev, (void*)s->lam, (void*)s->xp, (void*)s->yp, (void*)s->zp,
-                    (void*)s->vk, (void*)s->vn, (void*)s->g, (void*)s->m1, (void*)s->m2, (void*)s->dv, (void*)s->cons_off,
+                    (void*)s->vk, (void*)s->vn, (void*)s->g, (void*)s->m1, (void*)s->m2, (void*)s->dv, (void*)s->ux,
+                    (void*)s->uy, (void*)s->uz, (void*)s->cons_off,
                     (void*)s->cons_idx, (void*)s->node_fixed, (void*)s->part, (void*)s->scal, (void*)s->flags})
         (void)hipFree(p);
     delete s;
@@ -529,7 +566,9 @@ extern "C" int tlfea_solver_create(int kind, const tlfea_body* body, tlfea_solve
         (rc = dalloc(&s->xp, body->n_nodes)) || (rc = dalloc(&s->yp, body->n_nodes)) ||
         (rc = dalloc(&s->zp, body->n_nodes)) || (rc = dalloc(&s->vk, n3)) || (rc = dalloc(&s->vn, n3)) ||
         (rc = dalloc(&s->g, n3)) || (rc = dalloc(&s->m1, n3)) || (rc = dalloc(&s->m2, n3)) || (rc = dalloc(&s->dv, 4 * (int64_t)body->n_nodes)) ||
-        (rc = dalloc(&s->part, 4 * (int64_t)nblk)) || (rc = dalloc(&s->scal, 16)) || (rc = dalloc(&s->flags, 8)) ||
+        (rc = dalloc(&s->part, 5 * (int64_t)nblk)) || (rc = dalloc(&s->scal, 16)) || (rc = dalloc(&s->flags, 8)) ||
+        (rc = dalloc(&s->ux, body->n_nodes)) || (rc = dalloc(&s->uy, body->n_nodes)) ||
+        (rc = dalloc(&s->uz, body->n_nodes)) ||
         (rc = dalloc(&s->node_fixed, body->n_nodes)) || (rc = dalloc(&s->cons_off, body->n_nodes + 1))) {
         free_solver(s);
         return rc;
@@ -592,6 +631,19 @@ extern "C" int tlfea_solver_set_rel_tol(tlfea_solver* s, double rel_tol) {
     return TLFEA_OK;
 }
 
+extern "C" int tlfea_solver_set_restart(tlfea_solver* s, int on) {
+    TLFEA_REQUIRE(s, "null solver");
+    TLFEA_REQUIRE(on == 0 || s->kind == TLFEA_SOLVER_NESTEROV, "restart applies to the Nesterov solver only");
+    s->restart = on != 0;
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_solver_restarts(const tlfea_solver* s, int* restarts) {
+    TLFEA_REQUIRE(s && restarts, "null argument");
+    *restarts = s->last_restarts;
+    return TLFEA_OK;
+}
+
 extern "C" int tlfea_solver_set_poll_interval(tlfea_solver* s, int every) {
     TLFEA_REQUIRE(s && every >= 1, "bad argument");
     s->poll = every;
@@ -644,6 +696,10 @@ extern "C" int tlfea_solver_solve(tlfea_solver* S, void* stream) {
     const bool damp = b.mat.eta_damp != 0.0 || b.mat.lambda_damp != 0.0;
     // the planned force launches overwrite f_int (accumulate = 0); the atomic one adds to it
     double* const fzero = b.force_plan ? nullptr : b.f_int;
+    // the rest-coordinate force takes the displacements the predict kernels form
+    const bool rest_form = b.force_plan && b.X0;
+    const DispOut uo = rest_form ? DispOut{S->ux, S->uy, S->uz, b.X0, b.Y0, b.Z0} : DispOut{};
+    const double* const tseq = (nes && S->restart) ? S->scal : nullptr;
 
     TLFEA_HIP_TRY(hipMemsetAsync(S->flags, 0, sizeof(int) * 8, st));
     k_save_prev<<<gN, 256, 0, st>>>(N, b.x, b.y, b.z, S->xp, S->yp, S->zp);
@@ -662,8 +718,8 @@ extern "C" int tlfea_solver_solve(tlfea_solver* S, void* stream) {
                 const double t_next = 0.5 * (1.0 + std::sqrt(1.0 + 4.0 * t * t));
                 const double beta = (t - 1.0) / t_next;
                 t = t_next;
-                k_nes_predict<<<g3, 256, 0, st>>>(N, beta, dt, S->vk, S->vn, S->v_guess, S->v_prev, S->dv, S->xp,
-                                                  S->yp, S->zp, b.x, b.y, b.z, fzero, S->flags);
+                k_nes_predict<<<g3, 256, 0, st>>>(N, beta, tseq, dt, S->vk, S->vn, S->v_guess, S->v_prev, S->dv,
+                                                  S->xp, S->yp, S->zp, b.x, b.y, b.z, uo, fzero, S->flags);
             } else {
                 double lr_k, c1 = 0.0, c2 = 0.0;
                 if (nocoop) {
@@ -681,15 +737,16 @@ extern "C" int tlfea_solver_solve(tlfea_solver* S, void* stream) {
                 }
                 k_adam_predict<<<g3, 256, 0, st>>>(N, nocoop ? 1 : 0, lr_k, c1, c2, S->ap.beta1, S->ap.beta2,
                                                    S->ap.eps, S->ap.weight_decay, dt, S->g, S->m1, S->m2, S->v_guess,
-                                                   S->v_prev, S->dv, S->xp, S->yp, S->zp, b.x, b.y, b.z, fzero,
-                                                   S->flags);
+                                                   S->v_prev, S->dv, S->xp, S->yp, S->zp, b.x, b.y, b.z, uo,
+                                                   fzero, S->flags);
             }
             TLFEA_LAUNCH_CHECK();
             // element forces at the scratch positions, damping from v_guess; skipped once the
             // inner loop has converged (f_int then keeps the last evaluation)
-            if (b.force_plan && b.X0) {
-                if (int rc = launch_force_rest(b.force_plan, b.x, b.y, b.z, b.X0, b.Y0, b.Z0, b.quad, b.mat,
-                                               damp ? S->v_guess : nullptr, b.f_int, 0, S->flags, st))
+            if (rest_form) {
+                if (int rc = launch_force_rest(b.force_plan, S->ux, S->uy, S->uz, b.X0, b.Y0, b.Z0, b.quad, b.mat,
+                                               damp ? S->v_guess : nullptr, b.f_int, 0, S->flags, st, nullptr,
+                                               nullptr, nullptr, true))
                     return rc;
             } else if (b.force_plan) {
                 if (int rc = launch_force_planned(b.force_plan, b.x, b.y, b.z, b.grad_t, b.detJ_t, b.quad, b.mat,
@@ -711,7 +768,7 @@ extern "C" int tlfea_solver_solve(tlfea_solver* S, void* stream) {
                                                 S->node_fixed, dt, rho, 0.0, nullptr, S->g, S->part, gS, S->flags);
             }
             k_finalize<<<1, S_RED, 0, st>>>(S->kind, inner, check, gS, S->part, S->scal, S->flags, inner_tol,
-                                            nes ? 0.0 : S->ap.inner_rtol, S->rel_tol);
+                                            nes ? 0.0 : S->ap.inner_rtol, S->rel_tol, tseq ? 1 : 0);
             TLFEA_LAUNCH_CHECK();
             if ((inner + 1) % S->poll == 0) {
                 TLFEA_HIP_TRY(hipMemcpyAsync(hflags, S->flags, sizeof(int) * 8, hipMemcpyDeviceToHost, st));
@@ -745,6 +802,7 @@ extern "C" int tlfea_solver_solve(tlfea_solver* S, void* stream) {
     TLFEA_HIP_TRY(hipStreamSynchronize(st));
     S->last_inner = hflags[2];
     S->last_outer = hflags[3];
+    S->last_restarts = hflags[6];
     S->last_norm_g = hs[1];
     S->last_rel = hs[7];
     return TLFEA_OK;

@@ -418,6 +418,15 @@ class Solver:
     def set_poll_interval(self, every):
         call("tlfea_solver_set_poll_interval", self.h, int(every))
 
+    def set_restart(self, on=True):
+        """Nesterov momentum restart on g.(v_(k+1) - v_k) > 0 (not in the reference)."""
+        call("tlfea_solver_set_restart", self.h, int(bool(on)))
+
+    def restarts(self):
+        n = C.c_int()
+        call("tlfea_solver_restarts", self.h, C.byref(n))
+        return n.value
+
     def solve(self):
         call("tlfea_solver_solve", self.h, self.b.s)
 
