@@ -355,6 +355,20 @@ int tlfea_batch_dot(tlfea_batch* b, const double* a, const double* c, double* ou
  * lambda_max (host [n_inst], may be NULL) receives the unscaled estimates. */
 int tlfea_batch_precond_setup(tlfea_batch* b, int degree, double ratio, int power_iters, double safety,
                               double* lambda_max, void* stream);
+/* Two-level coarse correction for the FP32 stage, not in the reference: the nodes of each
+ * instance are grouped into planes across its longest axis (the mesh's lattice planes, or
+ * equal-width bins of an unstructured mesh); the coarse space holds the 6 rigid motions of
+ * each plane's free nodes, and A_c = P^T K P is assembled by probing with the FP64 operator
+ * and factored per instance as a block-tridiagonal matrix (blocks of W planes, W = the
+ * widest plane span of an element, at most 3: TLFEA_ECAPACITY beyond).  tlfea_batch_cg_solve
+ * (precond = 1) then applies the additive two-level preconditioner z = P A_c^-1 P^T r +
+ * Chebyshev(r) inside the flexible FP64 CG: the beams' bending modes, which the Chebyshev
+ * stage cannot reach, go to the coarse solve.  Needs tlfea_batch_precond_setup first and no mass term;
+ * enable = 0 removes it.  n_coarse (may be NULL) receives the coarse dimension. */
+int tlfea_batch_coarse_setup(tlfea_batch* b, int enable, int* n_coarse, void* stream);
+/* z = P A_c^-1 P^T r for every instance (FP64; r and z [3 n_nodes] on the device): the
+ * coarse part of the preconditioner on its own. */
+int tlfea_batch_coarse_apply(tlfea_batch* b, const double* r, double* z, void* stream);
 /* u += Chebyshev(f - K u), `sweeps` times: the FP32 stage of tlfea_batch_precond_setup used
  * as a smoother of an initial guess (it damps the error components of the upper part of
  * the spectrum of D^-1 K). */

@@ -155,6 +155,19 @@ class Batch:
         self.precond_degree = degree
         return lm
 
+    def coarse_setup(self, enable=True):
+        """Two-level coarse correction (rigid motions of node planes) inside the FP32 stage;
+        returns the coarse dimension (needs precond_setup first)."""
+        nc = C.c_int()
+        call("tlfea_batch_coarse_setup", self.h, int(bool(enable)), C.byref(nc), self.stream)
+        return nc.value
+
+    def coarse_apply(self, r, out=None):
+        """z = P A_c^-1 P^T r, the coarse part of the two-level preconditioner (FP64)."""
+        out = torch.empty(3 * self.n_nodes, dtype=torch.float64, device=DEV) if out is None else out
+        call("tlfea_batch_coarse_apply", self.h, ptr(r), ptr(out), self.stream)
+        return out
+
     def smooth(self, u, sweeps=2, f=None):
         """u += Chebyshev(f - K u), in place (needs precond_setup)."""
         f = self.f if f is None else f

@@ -18,6 +18,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <memory>
+#include <thread>
 #include <vector>
 
 #include "internal.h"
@@ -70,6 +72,19 @@ struct tlfea_batch {
     hipStream_t s_own = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_iters = 0, graph_precond = -1;
+    // two-level coarse correction (tlfea_batch_coarse_setup): 6 rigid modes per node plane
+    // across each instance's long axis; coarse DOF 6 p + j of (global, padded) plane p;
+    // blocks of W planes (NB = 6 W coarse DOFs) form a block-tridiagonal matrix
+    bool coarse_on = false;
+    int cw = 0, cnb = 0, n_planes = 0, n_cblocks = 0;
+    int* d_node_plane = nullptr;   // [n_nodes]: global plane, -1 for fixed nodes
+    double* d_node_r = nullptr;    // [3 n_nodes]: (X - centroid) / size of the node's plane
+    int *d_plane_off = nullptr, *d_plane_nodes = nullptr;  // plane -> free nodes (CSR)
+    int* d_plane_inst = nullptr;   // [n_planes]
+    int* d_plane_local = nullptr;  // [n_planes]: plane index within its instance
+    int* d_inst_cblk = nullptr;    // [n_inst + 1]: first coarse block of each instance
+    double *d_cdiag = nullptr, *d_clow = nullptr, *d_cg = nullptr;  // [n_cblocks][NB][NB]
+    double *d_crhs = nullptr, *d_cx = nullptr;                       // [6 n_planes]
 };
 
 namespace tlfea_hip {
@@ -451,17 +466,43 @@ __global__ __launch_bounds__(BT) void k_cheb_step(Chunks ch, int k, const float*
     }
 }
 
-// z = z32 in FP64; chunk partials of z.r and z.r_old
+// The coarse correction's prolongation (node_plane non-null): (P x)_k = t_p + theta_p x r_n
+// at DOF k = 3 n + d of a free node n of plane p (0 on fixed nodes).
+struct CoarseProlong {
+    const int* node_plane;
+    const double* node_r;
+    const double* x;
+};
+__device__ __forceinline__ double coarse_value(const CoarseProlong& cp, int64_t k) {
+    const int64_t n = k / 3;
+    const int d = (int)(k - 3 * n);
+    const int pl = cp.node_plane[n];
+    if (pl < 0) return 0.0;
+    const double* t = cp.x + 6 * (int64_t)pl;
+    const double* r = cp.node_r + 3 * n;
+    if (d == 0) return t[0] + t[4] * r[2] - t[5] * r[1];
+    if (d == 1) return t[1] + t[5] * r[0] - t[3] * r[2];
+    return t[2] + t[3] * r[1] - t[4] * r[0];
+}
+
+// z = P x on every DOF (FP64)
+__global__ void k_coarse_prolong64(int64_t n3, CoarseProlong cp, double* __restrict__ z) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n3) z[k] = coarse_value(cp, k);
+}
+
+// z = z32 in FP64 (plus P x of the coarse solve when set up: the additive two-level
+// preconditioner); chunk partials of z.r and z.r_old
 __global__ __launch_bounds__(BT) void k_cheb_fin(Chunks ch, const float* __restrict__ zc,
                                                  const double* __restrict__ r, const double* __restrict__ rold,
                                                  const int* __restrict__ I, double* __restrict__ z,
-                                                 double* __restrict__ cpart) {
+                                                 double* __restrict__ cpart, CoarseProlong cp) {
     const int c = chunk_id(ch);
     const int inst = ch.inst[c];
     double zr = 0.0, zro = 0.0;
     if (!I[inst * NI + I_DONE]) {
         for (int64_t k = 3 * (int64_t)ch.beg[c] + threadIdx.x; k < 3 * (int64_t)ch.end[c]; k += BT) {
-            const double zk = (double)zc[k];
+            const double zk = (double)zc[k] + (cp.node_plane ? coarse_value(cp, k) : 0.0);
             z[k] = zk;
             zr += zk * r[k];
             zro += zk * rold[k];
@@ -643,6 +684,325 @@ __global__ void k_d2f(int64_t n, const double* __restrict__ a, float* __restrict
     if (i < n) b[i] = (float)a[i];
 }
 
+
+// ---- two-level coarse correction --------------------------------------------------------
+// Coarse space: for each plane p of nodes across an instance's long axis, the 6 rigid
+// motions u_n = t_p + theta_p x r_n of its free nodes (r_n = (X_n - c_p) / size_p), so
+// P^T v = [sum_n v_n ; sum_n r_n x v_n] per plane.  Elements span at most W consecutive
+// planes, so P^T K P couples planes at most W apart: grouping W planes per block makes it
+// block-tridiagonal (blocks of NB = 6 W).  It is assembled by probing with the batched
+// operator (planes coloured mod 2W+1), factored as D_k = A_kk - G_k A_(k-1,k)^T with
+// G_k = A_(k,k-1) D_(k-1)^-1, and solved by a block forward and backward sweep.
+constexpr int CNB_MAX = 18;  // W <= 3
+
+__device__ __forceinline__ void rigid_mode(int j, const double* r, double v[3]) {
+    v[0] = v[1] = v[2] = 0.0;
+    if (j < 3) {
+        v[j] = 1.0;
+    } else {  // e_(j-3) x r
+        const int a = j - 3;
+        const double e[3] = {a == 0 ? 1.0 : 0.0, a == 1 ? 1.0 : 0.0, a == 2 ? 1.0 : 0.0};
+        v[0] = e[1] * r[2] - e[2] * r[1];
+        v[1] = e[2] * r[0] - e[0] * r[2];
+        v[2] = e[0] * r[1] - e[1] * r[0];
+    }
+}
+
+// q = P e_(p, j) summed over the planes p with local index = color mod (2W + 1)
+__global__ void k_coarse_probe_vec(int n_nodes, const int* __restrict__ node_plane, const double* __restrict__ node_r,
+                                   const int* __restrict__ plane_local, int period, int color, int j,
+                                   double* __restrict__ q) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= n_nodes) return;
+    const int pl = node_plane[n];
+    double v[3] = {0.0, 0.0, 0.0};
+    if (pl >= 0 && plane_local[pl] % period == color) rigid_mode(j, node_r + 3 * (int64_t)n, v);
+    q[3 * (int64_t)n] = v[0];
+    q[3 * (int64_t)n + 1] = v[1];
+    q[3 * (int64_t)n + 2] = v[2];
+}
+
+// the 6 sums of P^T v over plane p, in lane order then by wave reduction (64 threads)
+__device__ __forceinline__ void plane_restrict(int p, const int* __restrict__ plane_off,
+                                               const int* __restrict__ plane_nodes, const double* __restrict__ node_r,
+                                               const double* __restrict__ v, double out[6]) {
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int k = plane_off[p] + (int)threadIdx.x; k < plane_off[p + 1]; k += 64) {
+        const int n = plane_nodes[k];
+        const double* r = node_r + 3 * (int64_t)n;
+        const double a = v[3 * (int64_t)n], b = v[3 * (int64_t)n + 1], c = v[3 * (int64_t)n + 2];
+        acc[0] += a;
+        acc[1] += b;
+        acc[2] += c;
+        acc[3] += r[1] * c - r[2] * b;
+        acc[4] += r[2] * a - r[0] * c;
+        acc[5] += r[0] * b - r[1] * a;
+    }
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        double x = acc[i];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        out[i] = x;
+    }
+}
+
+// A(p', (p*, j)) = (P^T w)_p' for the coloured plane p* within W of p' (same instance);
+// diagonal blocks and the lower blocks A_(k, k-1) are stored (row-major NB x NB).
+__global__ __launch_bounds__(64) void k_coarse_probe_store(int W, int period, int color, int j,
+                                                           const int* __restrict__ plane_off,
+                                                           const int* __restrict__ plane_nodes,
+                                                           const int* __restrict__ plane_local,
+                                                           const int* __restrict__ plane_inst,
+                                                           const int* __restrict__ inst_plane_n,
+                                                           const double* __restrict__ node_r,
+                                                           const double* __restrict__ w, double* __restrict__ cdiag,
+                                                           double* __restrict__ clow) {
+    const int pp = blockIdx.x;  // p'
+    const int lp = plane_local[pp];
+    // the plane with local index = color (mod period) within W of p'
+    const int dm = ((lp - color) % period + period) % period;
+    int ls = -1;
+    if (dm <= W)
+        ls = lp - dm;
+    else if (period - dm <= W)
+        ls = lp + (period - dm);
+    if (ls < 0 || ls >= inst_plane_n[plane_inst[pp]]) return;
+    const int ps = pp + (ls - lp);
+    double out[6];
+    plane_restrict(pp, plane_off, plane_nodes, node_r, w, out);
+    if (threadIdx.x != 0) return;
+    const int NB = 6 * W;
+    const int bp = pp / W, bs = ps / W;
+    const int row0 = 6 * (pp % W), col = 6 * (ps % W) + j;
+    double* dst = bs == bp ? cdiag + (int64_t)bp * NB * NB : (bs == bp - 1 ? clow + (int64_t)bp * NB * NB : nullptr);
+    if (!dst) return;  // the upper block is the transpose of the next lower one
+    for (int i = 0; i < 6; i++) dst[(row0 + i) * NB + col] = out[i];
+}
+
+// In-place inverse of the SPD NB x NB matrix m (LDS) by Cholesky: thread 0 factors, then
+// lanes 0..NB-1 solve one column each.  A pivot at or below 1e-10 of its diagonal entry
+// drops that coarse DOF (its row and column of the inverse are zero: no rigid mode of a
+// plane without the free nodes to carry it).  Called by all threads of the workgroup.
+__device__ void spd_inverse(double* m, double* L, int* dead, int NB) {
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < NB * NB; i++) L[i] = 0.0;
+        for (int c = 0; c < NB; c++) {
+            double d = m[c * NB + c];
+            const double d0 = d;
+            for (int k = 0; k < c; k++) d -= L[c * NB + k] * L[c * NB + k];
+            dead[c] = (!(d > 1e-10 * fabs(d0)) || d0 <= 0.0) ? 1 : 0;
+            const double lc = dead[c] ? 0.0 : sqrt(d);
+            L[c * NB + c] = lc;
+            for (int i = c + 1; i < NB; i++) {
+                double v = m[i * NB + c];
+                for (int k = 0; k < c; k++) v -= L[i * NB + k] * L[c * NB + k];
+                L[i * NB + c] = dead[c] ? 0.0 : v / lc;
+            }
+        }
+    }
+    __syncthreads();
+    // column c of the inverse: L y = e_c, L^T x = y (dead rows skipped)
+    const int c = threadIdx.x;
+    if (c < NB) {
+        double y[CNB_MAX], x[CNB_MAX];
+        for (int i = 0; i < NB; i++) {
+            double v = (i == c) ? 1.0 : 0.0;
+            for (int k = 0; k < i; k++) v -= L[i * NB + k] * y[k];
+            y[i] = dead[i] ? 0.0 : v / L[i * NB + i];
+        }
+        for (int i = NB - 1; i >= 0; i--) {
+            double v = y[i];
+            for (int k = i + 1; k < NB; k++) v -= L[k * NB + i] * x[k];
+            x[i] = dead[i] ? 0.0 : v / L[i * NB + i];
+        }
+        for (int i = 0; i < NB; i++) m[i * NB + c] = x[i];
+    }
+    __syncthreads();
+}
+
+// Block factorisation per instance (one workgroup of 64): cdiag[k] becomes D_k^-1, cg[k] = G_k.
+__global__ __launch_bounds__(64) void k_coarse_factor(int NB, const int* __restrict__ inst_cblk,
+                                                      double* __restrict__ cdiag, const double* __restrict__ clow,
+                                                      double* __restrict__ cg) {
+    __shared__ double Dprev[CNB_MAX * CNB_MAX], M[CNB_MAX * CNB_MAX], T[CNB_MAX * CNB_MAX],
+        L[CNB_MAX * CNB_MAX];
+    __shared__ int dead[CNB_MAX];
+    const int i = blockIdx.x;
+    const int k0 = inst_cblk[i], k1 = inst_cblk[i + 1];
+    const int nn = NB * NB;
+    for (int k = k0; k < k1; k++) {
+        double* D = cdiag + (int64_t)k * nn;
+        for (int e = threadIdx.x; e < nn; e += 64) M[e] = D[e];
+        __syncthreads();
+        if (k > k0) {
+            const double* A = clow + (int64_t)k * nn;  // A_(k, k-1)
+            // G = A Dprev^-1 (Dprev holds the inverse), then M -= G A^T
+            for (int e = threadIdx.x; e < nn; e += 64) {
+                const int r = e / NB, c = e % NB;
+                double v = 0.0;
+                for (int t = 0; t < NB; t++) v += A[r * NB + t] * Dprev[t * NB + c];
+                T[e] = v;
+                cg[(int64_t)k * nn + e] = v;
+            }
+            __syncthreads();
+            for (int e = threadIdx.x; e < nn; e += 64) {
+                const int r = e / NB, c = e % NB;
+                double v = 0.0;
+                for (int t = 0; t < NB; t++) v += T[r * NB + t] * A[c * NB + t];
+                M[e] -= v;
+            }
+            __syncthreads();
+        } else {
+            for (int e = threadIdx.x; e < nn; e += 64) cg[(int64_t)k * nn + e] = 0.0;
+        }
+        // symmetrise (the probed entries agree to rounding) and invert
+        for (int e = threadIdx.x; e < nn; e += 64) {
+            const int r = e / NB, c = e % NB;
+            if (c > r) {
+                const double v = 0.5 * (M[r * NB + c] + M[c * NB + r]);
+                T[e] = v;
+            }
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < nn; e += 64) {
+            const int r = e / NB, c = e % NB;
+            if (c > r) M[e] = M[c * NB + r] = T[e];
+        }
+        __syncthreads();
+        spd_inverse(M, L, dead, NB);
+        for (int e = threadIdx.x; e < nn; e += 64) {
+            D[e] = M[e];
+            Dprev[e] = M[e];
+        }
+        __syncthreads();
+    }
+}
+
+// coarse right-hand side: P^T r per plane (instances already converged skipped)
+__global__ __launch_bounds__(64) void k_coarse_restrict(const int* __restrict__ plane_off,
+                                                        const int* __restrict__ plane_nodes,
+                                                        const int* __restrict__ plane_inst,
+                                                        const double* __restrict__ node_r,
+                                                        const int* __restrict__ I, const double* __restrict__ r,
+                                                        double* __restrict__ rhs) {
+    const int p = blockIdx.x;
+    if (I[(int64_t)plane_inst[p] * NI + I_DONE]) return;
+    double out[6];
+    plane_restrict(p, plane_off, plane_nodes, node_r, r, out);
+    if (threadIdx.x < 6) rhs[6 * (int64_t)p + threadIdx.x] = out[threadIdx.x];
+}
+
+// block forward / backward sweep per instance: x = A_c^-1 rhs.  The sweep is a chain of
+// small dependent matrix-vector products; wave 0 runs it out of LDS (the block size NB a
+// compile-time constant, so each product's loads and broadcasts issue together) while
+// waves 1-3 stage the next chunk of factor blocks and right-hand sides (double-buffered),
+// so the chain never waits on a global load.  CS_WORDS doubles per buffer.
+constexpr int CS_WORDS = 4608;
+template <int NB, bool TRANS>
+__device__ __forceinline__ double lds_matvec(const double* __restrict__ Mx, double v) {
+    const int l = threadIdx.x < NB ? threadIdx.x : 0;
+    double m[NB], vt[NB];
+#pragma unroll
+    for (int t = 0; t < NB; t++) {
+        m[t] = TRANS ? Mx[t * NB + l] : Mx[l * NB + t];
+        vt[t] = __shfl(v, t, 64);
+    }
+    double y = 0.0;
+#pragma unroll
+    for (int t = 0; t < NB; t++) y += m[t] * vt[t];
+    return threadIdx.x < NB ? y : 0.0;
+}
+
+template <int NB>
+__global__ __launch_bounds__(256) void k_coarse_solve(const int* __restrict__ inst_cblk, const int* __restrict__ I,
+                                                      const double* __restrict__ dinv, const double* __restrict__ clow,
+                                                      const double* __restrict__ cg, const double* __restrict__ rhs,
+                                                      double* __restrict__ x) {
+    extern __shared__ double lds[];  // [2][CS_WORDS]
+    constexpr int nn = NB * NB;
+    const int i = blockIdx.x;
+    if (I[(int64_t)i * NI + I_DONE]) return;
+    const int k0 = inst_cblk[i], k1 = inst_cblk[i + 1], nb = k1 - k0;
+    const int tid = threadIdx.x, l = tid & 63;
+    const bool sweeper = tid < 64;
+    // ---- forward: z_k = b_k - G_k z_(k-1); a buffer holds CF blocks of G and then of b ----
+    constexpr int CF = CS_WORDS / (nn + NB);
+    const int nchf = (nb + CF - 1) / CF;
+    auto stage_f = [&](int c, int buf, int t0, int nt) {
+        const int kb = k0 + c * CF, ke = min(kb + CF, k1);
+        double* dst = lds + buf * CS_WORDS;
+        const double* src = cg + (int64_t)kb * nn;
+        for (int e = t0; e < (ke - kb) * nn; e += nt) dst[e] = src[e];
+        const double* rs = rhs + (int64_t)kb * NB;
+        for (int e = t0; e < (ke - kb) * NB; e += nt) dst[CF * nn + e] = rs[e];
+    };
+    stage_f(0, 0, tid, 256);
+    __syncthreads();
+    double z = 0.0;
+    for (int c = 0; c < nchf; c++) {
+        if (sweeper) {
+            const double* G = lds + (c & 1) * CS_WORDS;
+            const int kb = k0 + c * CF, ke = min(kb + CF, k1);
+            for (int k = kb; k < ke; k++) {
+                const double bk = l < NB ? G[CF * nn + (k - kb) * NB + l] : 0.0;
+                const double gz = k > k0 ? lds_matvec<NB, false>(G + (k - kb) * nn, z) : 0.0;
+                z = bk - gz;
+                if (l < NB) x[(int64_t)k * NB + l] = z;
+            }
+        } else if (c + 1 < nchf) {
+            stage_f(c + 1, (c + 1) & 1, tid - 64, 192);
+        }
+        __syncthreads();
+    }
+    // ---- backward: x_k = D_k^-1 (z_k - A_(k+1,k)^T x_(k+1)); a buffer holds CB blocks of
+    // (D_k^-1, A_(k+1,k)) and then their z_k, from the last block down ----
+    constexpr int CB = CS_WORDS / (2 * nn + NB);
+    const int nchb = (nb + CB - 1) / CB;
+    auto stage_b = [&](int c, int buf, int t0, int nt) {
+        const int ke = k1 - c * CB, kb = max(ke - CB, k0);
+        double* dst = lds + buf * CS_WORDS;
+        for (int e = t0; e < (ke - kb) * nn; e += nt) {
+            const int k = kb + e / nn, o = e - (k - kb) * nn;
+            dst[2 * (k - kb) * nn + o] = dinv[(int64_t)k * nn + o];
+            dst[(2 * (k - kb) + 1) * nn + o] = k + 1 < k1 ? clow[(int64_t)(k + 1) * nn + o] : 0.0;
+        }
+        const double* zs = x + (int64_t)kb * NB;
+        for (int e = t0; e < (ke - kb) * NB; e += nt) dst[2 * CB * nn + e] = zs[e];
+    };
+    stage_b(0, 0, tid, 256);
+    __syncthreads();
+    double xn = 0.0;
+    for (int c = 0; c < nchb; c++) {
+        if (sweeper) {
+            const double* B = lds + (c & 1) * CS_WORDS;
+            const int ke = k1 - c * CB, kb = max(ke - CB, k0);
+            for (int k = ke - 1; k >= kb; k--) {
+                double t = l < NB ? B[2 * CB * nn + (k - kb) * NB + l] : 0.0;
+                if (k < k1 - 1) t -= lds_matvec<NB, true>(B + (2 * (k - kb) + 1) * nn, xn);
+                xn = lds_matvec<NB, false>(B + 2 * (k - kb) * nn, t);
+                if (l < NB) x[(int64_t)k * NB + l] = xn;
+            }
+        } else if (c + 1 < nchb) {
+            stage_b(c + 1, (c + 1) & 1, tid - 64, 192);
+        }
+        __syncthreads();
+    }
+}
+
+int launch_coarse_solve(int NB, int n_inst, const int* inst_cblk, const int* I, const double* dinv,
+                        const double* clow, const double* cg, const double* rhs, double* x, hipStream_t s) {
+    const size_t lds = 2 * CS_WORDS * sizeof(double);
+    if (NB == 6)
+        k_coarse_solve<6><<<n_inst, 256, lds, s>>>(inst_cblk, I, dinv, clow, cg, rhs, x);
+    else if (NB == 12)
+        k_coarse_solve<12><<<n_inst, 256, lds, s>>>(inst_cblk, I, dinv, clow, cg, rhs, x);
+    else
+        k_coarse_solve<18><<<n_inst, 256, lds, s>>>(inst_cblk, I, dinv, clow, cg, rhs, x);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
 }  // namespace tlfea_hip
 
 using namespace tlfea_hip;
@@ -668,7 +1028,9 @@ void free_batch(tlfea_batch* b) {
                     b->d_chunk_beg, b->d_chunk_end, b->d_inst_chunk_off, b->d_r, b->d_p, b->d_q, b->d_z,
                     b->d_r32, b->d_z32, b->d_d32, b->d_q32, b->d_part, b->d_s, b->d_i, b->d_all, b->d_cheb,
                     b->d_rold, b->d_dinv32, b->d_blk_list, b->d_chunk_list, b->d_inst_list, b->d_mc,
-                    b->d_ux, b->d_uy, b->d_uz};
+                    b->d_ux, b->d_uy, b->d_uz, b->d_node_plane, b->d_node_r, b->d_plane_off,
+                    b->d_plane_nodes, b->d_plane_inst, b->d_plane_local, b->d_inst_cblk, b->d_cdiag,
+                    b->d_clow, b->d_cg, b->d_crhs, b->d_cx};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (b->s_own) (void)hipStreamDestroy(b->s_own);
@@ -813,16 +1175,28 @@ int apply32(tlfea_batch* b, const float* p, float* q, const int* iflags, hipStre
     return TLFEA_OK;
 }
 
-// z = Chebyshev(r) for the active instances, then z.r / z.r_old chunk partials.
+// z = Chebyshev(r) for the active instances, plus the coarse correction P A_c^-1 P^T r
+// (FP64) when it is set up -- the additive two-level preconditioner -- then z.r / z.r_old
+// chunk partials.
 int precondition(tlfea_batch* b, hipStream_t s) {
     const Chunks ch = chunks(b);
-    k_cheb_init<<<n_chunk_launch(b), BT, 0, s>>>(ch, b->d_r, b->d_dinv32, b->d_cheb, b->d_i, b->d_r32, b->d_d32, b->d_z32);
+    CoarseProlong cp{nullptr, nullptr, nullptr};
+    if (b->coarse_on) {
+        k_coarse_restrict<<<b->n_planes, 64, 0, s>>>(b->d_plane_off, b->d_plane_nodes, b->d_plane_inst, b->d_node_r,
+                                                     b->d_i, b->d_r, b->d_crhs);
+        if (int rc = launch_coarse_solve(b->cnb, b->n_inst, b->d_inst_cblk, b->d_i, b->d_cdiag, b->d_clow, b->d_cg,
+                                         b->d_crhs, b->d_cx, s))
+            return rc;
+        cp = CoarseProlong{b->d_node_plane, b->d_node_r, b->d_cx};
+    }
+    k_cheb_init<<<n_chunk_launch(b), BT, 0, s>>>(ch, b->d_r, b->d_dinv32, b->d_cheb, b->d_i, b->d_r32, b->d_d32,
+                                                b->d_z32);
     for (int k = 1; k < b->cheb_degree; k++) {
         if (int rc = apply32(b, b->d_d32, b->d_q32, b->d_i, s)) return rc;
         k_cheb_step<<<n_chunk_launch(b), BT, 0, s>>>(ch, k, b->d_q32, b->d_dinv32, b->d_cheb, b->d_i, b->d_r32, b->d_d32,
                                                 b->d_z32);
     }
-    k_cheb_fin<<<n_chunk_launch(b), BT, 0, s>>>(ch, b->d_z32, b->d_r, b->d_rold, b->d_i, b->d_z, b->d_part);
+    k_cheb_fin<<<n_chunk_launch(b), BT, 0, s>>>(ch, b->d_z32, b->d_r, b->d_rold, b->d_i, b->d_z, b->d_part, cp);
     TLFEA_LAUNCH_CHECK();
     return TLFEA_OK;
 }
@@ -1089,6 +1463,250 @@ extern "C" int tlfea_batch_precond_setup(tlfea_batch* b, int degree, double rati
     }
     TLFEA_HIP_TRY(hipStreamSynchronize(s));
     b->cheb_degree = degree;
+    return TLFEA_OK;
+}
+
+namespace {
+// Node planes of one instance across its long axis: the distinct coordinates along that
+// axis when the nodes lie on few of them (a structured mesh), else equal-width bins.
+// Returns the plane count; plane[n - n0] receives the local plane of node n.
+int instance_planes(const double* X, const double* Y, const double* Z, int n0, int n1, std::vector<int>& plane) {
+    const double* c[3] = {X, Y, Z};
+    double lo[3], hi[3];
+    for (int d = 0; d < 3; d++) {
+        lo[d] = 1e300;
+        hi[d] = -1e300;
+        for (int n = n0; n < n1; n++) {
+            lo[d] = std::min(lo[d], c[d][n]);
+            hi[d] = std::max(hi[d], c[d][n]);
+        }
+    }
+    int ax = 0;
+    for (int d = 1; d < 3; d++)
+        if (hi[d] - lo[d] > hi[ax] - lo[ax]) ax = d;
+    const double ext = hi[ax] - lo[ax];
+    const int nn = n1 - n0;
+    plane.assign(nn, 0);
+    if (!(ext > 0.0)) return 1;
+    std::vector<std::pair<double, int>> v(nn);
+    for (int n = n0; n < n1; n++) v[n - n0] = {c[ax][n], n - n0};
+    std::sort(v.begin(), v.end());
+    const double tol = 1e-9 * ext;
+    int np = 0;
+    for (int k = 0; k < nn; k++) {
+        if (k > 0 && v[k].first - v[k - 1].first > tol) np++;
+        plane[v[k].second] = np;
+    }
+    np++;
+    if (np * 4 <= nn) return np;  // lattice planes
+    // unstructured: bins of equal width, about as thick as the mean node spacing
+    const double t1 = hi[(ax + 1) % 3] - lo[(ax + 1) % 3], t2 = hi[(ax + 2) % 3] - lo[(ax + 2) % 3];
+    const double h = std::cbrt(std::max(ext * std::max(t1, 1e-300) * std::max(t2, 1e-300), 1e-300) / nn);
+    np = std::max(2, std::min(nn / 4, (int)std::lround(ext / std::max(h, 1e-300))));
+    for (int n = n0; n < n1; n++)
+        plane[n - n0] = std::min(np - 1, (int)(((c[ax][n]) - lo[ax]) / ext * np));
+    return np;
+}
+}  // namespace
+
+extern "C" int tlfea_batch_coarse_setup(tlfea_batch* b, int enable, int* n_coarse, void* stream) {
+    TLFEA_REQUIRE(b != nullptr, "null batch");
+    drop_graph(b);
+    full_lists(b);
+    b->coarse_on = false;
+    if (!enable) return TLFEA_OK;
+    TLFEA_REQUIRE(b->cheb_degree > 0, "the coarse correction runs inside the Chebyshev stage: set that up first");
+    TLFEA_REQUIRE(!b->mass_on, "the coarse correction covers K only (a mass term is set)");
+    hipStream_t s = as_stream(stream);
+    const int N = b->n_nodes, NE = b->n_elem, nn = b->nn;
+    // host staging without zero fill (every entry is written before it is read)
+    std::unique_ptr<double[]> X(new double[N]), Y(new double[N]), Z(new double[N]);
+    std::unique_ptr<unsigned char[]> mask(new unsigned char[3 * (size_t)N]);
+    std::unique_ptr<int[]> conn(new int[(size_t)NE * nn]);
+    TLFEA_HIP_TRY(hipStreamSynchronize(s));
+    TLFEA_HIP_TRY(hipMemcpy(X.get(), b->d_X, sizeof(double) * N, hipMemcpyDeviceToHost));
+    TLFEA_HIP_TRY(hipMemcpy(Y.get(), b->d_Y, sizeof(double) * N, hipMemcpyDeviceToHost));
+    TLFEA_HIP_TRY(hipMemcpy(Z.get(), b->d_Z, sizeof(double) * N, hipMemcpyDeviceToHost));
+    TLFEA_HIP_TRY(hipMemcpy(mask.get(), b->d_mask, 3 * (size_t)N, hipMemcpyDeviceToHost));
+    TLFEA_HIP_TRY(hipMemcpy(conn.get(), b->d_conn, sizeof(int) * (size_t)NE * nn, hipMemcpyDeviceToHost));
+    // instances on host threads: each owns its node, element and plane ranges
+    const int nthreads =
+        std::max(1, std::min<int>(b->n_inst, std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
+    auto parallel_instances = [&](auto&& fn) {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nthreads; t++)
+            pool.emplace_back([&, t] {
+                for (int i = (int)((int64_t)b->n_inst * t / nthreads); i < (int)((int64_t)b->n_inst * (t + 1) / nthreads);
+                     i++)
+                    fn(i);
+            });
+        for (auto& th : pool) th.join();
+    };
+    // planes per instance and the widest element span W
+    std::unique_ptr<int[]> local(new int[N]);
+    std::vector<int> inst_np(b->n_inst), inst_w(b->n_inst, 1);
+    parallel_instances([&](int i) {
+        const int n0 = b->h_node_off[i], n1 = b->h_node_off[i + 1];
+        std::vector<int> pl;
+        inst_np[i] = instance_planes(X.get(), Y.get(), Z.get(), n0, n1, pl);
+        for (int n = n0; n < n1; n++) local[n] = pl[n - n0];
+        for (int e = b->h_elem_off[i]; e < b->h_elem_off[i + 1]; e++) {
+            int mn = 1 << 30, mx = -1;
+            for (int a = 0; a < nn; a++) {
+                const int q = local[conn[e + (size_t)a * NE]];
+                mn = std::min(mn, q);
+                mx = std::max(mx, q);
+            }
+            inst_w[i] = std::max(inst_w[i], mx - mn);
+        }
+    });
+    const int W = *std::max_element(inst_w.begin(), inst_w.end());
+    if (W > 3) {
+        set_error("coarse correction: elements span %d node planes (at most 3 supported)", W);
+        return TLFEA_ECAPACITY;
+    }
+    const int NB = 6 * W;
+    // global planes, padded per instance to a multiple of W
+    std::vector<int> inst_poff(b->n_inst + 1, 0), inst_pn(b->n_inst);
+    for (int i = 0; i < b->n_inst; i++) {
+        inst_pn[i] = (inst_np[i] + W - 1) / W * W;
+        inst_poff[i + 1] = inst_poff[i] + inst_pn[i];
+    }
+    const int P = inst_poff[b->n_inst];
+    std::vector<int> plane_inst(P), plane_local(P), inst_cblk(b->n_inst + 1), cnt(P + 1, 0);
+    inst_cblk[b->n_inst] = P / W;
+    // free nodes per plane, centroid and size, node offsets r = (X - c) / size
+    std::unique_ptr<int[]> node_plane(new int[N]);
+    std::unique_ptr<double[]> node_r(new double[3 * (size_t)N]);
+    std::vector<double> cen(3 * (size_t)P, 0.0), size(P, 0.0);
+    parallel_instances([&](int i) {
+        inst_cblk[i] = inst_poff[i] / W;
+        for (int q = 0; q < inst_pn[i]; q++) {
+            plane_inst[inst_poff[i] + q] = i;
+            plane_local[inst_poff[i] + q] = q;
+        }
+        const int n0 = b->h_node_off[i], n1 = b->h_node_off[i + 1];
+        for (int n = n0; n < n1; n++) {
+            const bool fixed = mask[3 * (size_t)n] || mask[3 * (size_t)n + 1] || mask[3 * (size_t)n + 2];
+            node_plane[n] = fixed ? -1 : inst_poff[i] + local[n];
+            if (fixed) continue;
+            const int g = node_plane[n];
+            cnt[g + 1]++;
+            cen[3 * (size_t)g] += X[n];
+            cen[3 * (size_t)g + 1] += Y[n];
+            cen[3 * (size_t)g + 2] += Z[n];
+        }
+        for (int g = inst_poff[i]; g < inst_poff[i + 1]; g++)
+            if (cnt[g + 1] > 0)
+                for (int d = 0; d < 3; d++) cen[3 * (size_t)g + d] /= cnt[g + 1];
+        for (int n = n0; n < n1; n++) {
+            const int g = node_plane[n];
+            if (g < 0) continue;
+            const double r[3] = {X[n] - cen[3 * (size_t)g], Y[n] - cen[3 * (size_t)g + 1],
+                                 Z[n] - cen[3 * (size_t)g + 2]};
+            size[g] = std::max(size[g], std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]));
+        }
+        for (int n = n0; n < n1; n++) {
+            const int g = node_plane[n];
+            if (g < 0) {
+                node_r[3 * (size_t)n] = node_r[3 * (size_t)n + 1] = node_r[3 * (size_t)n + 2] = 0.0;
+                continue;
+            }
+            const double sc = size[g] > 0.0 ? 1.0 / size[g] : 1.0;
+            node_r[3 * (size_t)n] = (X[n] - cen[3 * (size_t)g]) * sc;
+            node_r[3 * (size_t)n + 1] = (Y[n] - cen[3 * (size_t)g + 1]) * sc;
+            node_r[3 * (size_t)n + 2] = (Z[n] - cen[3 * (size_t)g + 2]) * sc;
+        }
+    });
+    for (int g = 0; g < P; g++) cnt[g + 1] += cnt[g];
+    std::vector<int> plane_nodes(std::max(cnt[P], 1));
+    parallel_instances([&](int i) {
+        std::vector<int> cur(cnt.begin() + inst_poff[i], cnt.begin() + inst_poff[i + 1]);
+        for (int n = b->h_node_off[i]; n < b->h_node_off[i + 1]; n++)
+            if (node_plane[n] >= 0) plane_nodes[cur[node_plane[n] - inst_poff[i]]++] = n;
+    });
+    // device tables
+    for (void* q : {(void*)b->d_node_plane, (void*)b->d_node_r, (void*)b->d_plane_off, (void*)b->d_plane_nodes,
+                    (void*)b->d_plane_inst, (void*)b->d_plane_local, (void*)b->d_inst_cblk, (void*)b->d_cdiag,
+                    (void*)b->d_clow, (void*)b->d_cg, (void*)b->d_crhs, (void*)b->d_cx})
+        if (q) (void)hipFree(q);
+    b->d_node_plane = b->d_plane_off = b->d_plane_nodes = b->d_plane_inst = b->d_plane_local = b->d_inst_cblk =
+        nullptr;
+    b->d_node_r = b->d_cdiag = b->d_clow = b->d_cg = b->d_crhs = b->d_cx = nullptr;
+    const size_t nblk = (size_t)P / W, bsz = (size_t)NB * NB;
+    int rc;
+    int* d_inst_pn = nullptr;
+    if ((rc = dalloc(&b->d_node_plane, N)) || (rc = dalloc(&b->d_node_r, 3 * (size_t)N)) ||
+        (rc = dalloc(&b->d_plane_off, P + 1)) || (rc = dalloc(&b->d_plane_nodes, plane_nodes.size())) ||
+        (rc = dalloc(&b->d_plane_inst, P)) || (rc = dalloc(&b->d_plane_local, P)) ||
+        (rc = dalloc(&b->d_inst_cblk, b->n_inst + 1)) || (rc = dalloc(&b->d_cdiag, nblk * bsz)) ||
+        (rc = dalloc(&b->d_clow, nblk * bsz)) || (rc = dalloc(&b->d_cg, nblk * bsz)) ||
+        (rc = dalloc(&b->d_crhs, 6 * (size_t)P)) || (rc = dalloc(&b->d_cx, 6 * (size_t)P)) ||
+        (rc = dalloc(&d_inst_pn, b->n_inst)))
+        return rc;
+    TLFEA_HIP_TRY(hipMemcpy(b->d_node_plane, node_plane.get(), sizeof(int) * N, hipMemcpyHostToDevice));
+    TLFEA_HIP_TRY(hipMemcpy(b->d_node_r, node_r.get(), sizeof(double) * 3 * (size_t)N, hipMemcpyHostToDevice));
+    TLFEA_HIP_TRY(hipMemcpy(b->d_plane_off, cnt.data(), sizeof(int) * (P + 1), hipMemcpyHostToDevice));
+    TLFEA_HIP_TRY(hipMemcpy(b->d_plane_nodes, plane_nodes.data(), sizeof(int) * plane_nodes.size(),
+                            hipMemcpyHostToDevice));
+    TLFEA_HIP_TRY(hipMemcpy(b->d_plane_inst, plane_inst.data(), sizeof(int) * P, hipMemcpyHostToDevice));
+    TLFEA_HIP_TRY(hipMemcpy(b->d_plane_local, plane_local.data(), sizeof(int) * P, hipMemcpyHostToDevice));
+    TLFEA_HIP_TRY(hipMemcpy(b->d_inst_cblk, inst_cblk.data(), sizeof(int) * (b->n_inst + 1), hipMemcpyHostToDevice));
+    TLFEA_HIP_TRY(hipMemcpy(d_inst_pn, inst_pn.data(), sizeof(int) * b->n_inst, hipMemcpyHostToDevice));
+    TLFEA_HIP_TRY(hipMemset(b->d_cdiag, 0, sizeof(double) * nblk * bsz));
+    TLFEA_HIP_TRY(hipMemset(b->d_clow, 0, sizeof(double) * nblk * bsz));
+    TLFEA_HIP_TRY(hipMemset(b->d_cx, 0, sizeof(double) * 6 * (size_t)P));
+    b->cw = W;
+    b->cnb = NB;
+    b->n_planes = P;
+    b->n_cblocks = (int)nblk;
+    // P^T K P by probing: planes coloured by local index mod 2W+1 are further apart than any
+    // element reaches, so each coloured rigid mode's image restricted to a plane belongs to
+    // the one coloured plane within W of it
+    const int period = 2 * W + 1;
+    for (int color = 0; color < period && rc == TLFEA_OK; color++)
+        for (int j = 0; j < 6 && rc == TLFEA_OK; j++) {
+            k_coarse_probe_vec<<<grid_for(N, 256), 256, 0, s>>>(N, b->d_node_plane, b->d_node_r, b->d_plane_local,
+                                                                period, color, j, b->d_p);
+            if ((rc = apply64(b, b->d_p, b->d_q, nullptr, s, 1.0, false, true))) break;
+            k_coarse_probe_store<<<P, 64, 0, s>>>(W, period, color, j, b->d_plane_off, b->d_plane_nodes,
+                                                 b->d_plane_local, b->d_plane_inst, d_inst_pn, b->d_node_r, b->d_q,
+                                                 b->d_cdiag, b->d_clow);
+            if (hipGetLastError() != hipSuccess) {
+                set_error("coarse probe launch failed");
+                rc = TLFEA_EHIP;
+            }
+        }
+    if (rc == TLFEA_OK) {
+        k_coarse_factor<<<b->n_inst, 64, 0, s>>>(NB, b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
+        if (hipGetLastError() != hipSuccess) {
+            set_error("coarse factor launch failed");
+            rc = TLFEA_EHIP;
+        }
+    }
+    const hipError_t es = hipStreamSynchronize(s);
+    (void)hipFree(d_inst_pn);
+    if (rc) return rc;
+    TLFEA_HIP_TRY(es);
+    if (n_coarse) *n_coarse = 6 * P;
+    b->coarse_on = true;
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_batch_coarse_apply(tlfea_batch* b, const double* r, double* z, void* stream) {
+    TLFEA_REQUIRE(b && r && z, "null argument");
+    TLFEA_REQUIRE(b->coarse_on, "the coarse correction is not set up");
+    hipStream_t s = as_stream(stream);
+    k_bclear_flags<<<grid_for(b->n_inst, 256), 256, 0, s>>>(b->n_inst, b->d_i);
+    k_coarse_restrict<<<b->n_planes, 64, 0, s>>>(b->d_plane_off, b->d_plane_nodes, b->d_plane_inst, b->d_node_r,
+                                                 b->d_i, r, b->d_crhs);
+    if (int rc = launch_coarse_solve(b->cnb, b->n_inst, b->d_inst_cblk, b->d_i, b->d_cdiag, b->d_clow, b->d_cg,
+                                     b->d_crhs, b->d_cx, s))
+        return rc;
+    const int64_t n3 = 3 * (int64_t)b->n_nodes;
+    k_coarse_prolong64<<<grid_for(n3, 256), 256, 0, s>>>(n3, CoarseProlong{b->d_node_plane, b->d_node_r, b->d_cx},
+                                                         z);
+    TLFEA_LAUNCH_CHECK();
     return TLFEA_OK;
 }
 

@@ -229,7 +229,7 @@ __device__ __forceinline__ T grad_l_col(const T (&R)[3][3], int d, int c) {
 }
 
 template <typename T>
-__device__ __forceinline__ void tangent_vertex_t10(const int (&sl)[10], const T* sP, T* sF, int max_u,
+__device__ __forceinline__ void tangent_vertex_t10(const int (&sl)[10], const T* sP, double* sF, int max_u,
                                                    const T (&Mn)[9], T dJ, T lam, T mu, T ks) {
     // edge node between corners w and j
     constexpr int EDGE[4][4] = {{0, 4, 6, 7}, {4, 0, 5, 8}, {6, 5, 0, 9}, {7, 8, 9, 0}};
@@ -286,7 +286,7 @@ __device__ __forceinline__ void tangent_vertex_t10(const int (&sl)[10], const T*
 #pragma unroll
             for (int k = 0; k < 3; k++) R[i][k] = T(4) * Q[a][i][k] - Qs[i][k];
 #pragma unroll
-        for (int d = 0; d < 3; d++) atomicAdd(&sF[d * max_u + sl[a]], grad_l_col(R, d, a));
+        for (int d = 0; d < 3; d++) atomicAdd(&sF[d * max_u + sl[a]], (double)grad_l_col(R, d, a));
     }
 #pragma unroll
     for (int m = 0; m < 6; m++) {
@@ -301,7 +301,7 @@ __device__ __forceinline__ void tangent_vertex_t10(const int (&sl)[10], const T*
             }
 #pragma unroll
         for (int d = 0; d < 3; d++)
-            atomicAdd(&sF[d * max_u + sl[4 + m]], T(4) * (grad_l_col(Ri, d, cj) + grad_l_col(Rj, d, ci)));
+            atomicAdd(&sF[d * max_u + sl[4 + m]], (double)(T(4) * (grad_l_col(Ri, d, cj) + grad_l_col(Rj, d, ci))));
     }
 }
 
@@ -316,11 +316,12 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent(
     double* __restrict__ dot_part, const unsigned char* __restrict__ own) {
     if (flags && flags[0]) return;
     extern __shared__ unsigned char lds_raw[];
-    // [3][max_u] each: displacements (not in the rest-state form, whose launch reserves
-    // 6 arrays instead of 9), accumulated output, direction (masked)
-    T* sU = reinterpret_cast<T*>(lds_raw);
-    T* sF = LIN ? sU : sU + 3 * max_u;
-    T* sP = sF + 3 * max_u;
+    // [3][max_u] each: accumulated output (FP64 whatever T is: LDS float atomics on nodes
+    // shared by many elements of a block serialise far longer than FP64 ones on gfx950),
+    // displacements (not in the rest-state form), direction (masked)
+    double* sF = reinterpret_cast<double*>(lds_raw);
+    T* sU = reinterpret_cast<T*>(sF + 3 * max_u);
+    T* sP = LIN ? sU : sU + 3 * max_u;
     const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
     const int b = blk_list ? blk_list[lb] : (int)lb;
     const int64_t u0 = blk_off[b];
@@ -385,7 +386,7 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent(
                 sU[max_u + u] = uv[j][1];
                 sU[2 * max_u + u] = uv[j][2];
             }
-            sF[u] = sF[max_u + u] = sF[2 * max_u + u] = T(0);
+            sF[u] = sF[max_u + u] = sF[2 * max_u + u] = 0.0;
 #pragma unroll
             for (int d = 0; d < 3; d++) sP[d * max_u + u] = mk[j][d] ? T(0) : pv[j][d];
             if (dot_part && own[u0 + u]) {
@@ -443,7 +444,7 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent(
 #pragma unroll
         for (int a = 0; a < NN; a++)
 #pragma unroll
-            for (int i = 0; i < 3; i++) atomicAdd(&sF[i * max_u + sl[a]], fa[a][i]);
+            for (int i = 0; i < 3; i++) atomicAdd(&sF[i * max_u + sl[a]], (double)fa[a][i]);
     }
     __syncthreads();
     // this block's share of p . out, out = K p on free DOFs (sP = p) and p on masked ones
@@ -451,7 +452,7 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent(
     double dot = mdot;
     for (int u = threadIdx.x; u < U; u += PLAN_BLOCK) {
         T* o = part + 3 * (u0 + u);
-        const T f0 = sF[u], f1 = sF[max_u + u], f2 = sF[2 * max_u + u];
+        const T f0 = (T)sF[u], f1 = (T)sF[max_u + u], f2 = (T)sF[2 * max_u + u];
         o[0] = f0;
         o[1] = f1;
         o[2] = f2;
@@ -563,9 +564,10 @@ int launch_tangent_partials(const tlfea_force_plan* pl, const double* x, const d
             for (int k = 0; k < 3; k++) dn.d[q][a][k] = (T)dn64.d[q][a][k];
             dn.N[q][a] = (T)dn64.N[q][a];
         }
-    // 9 arrays of max_u values (displacements, output, direction); 6 in the rest-state
-    // form, which stages no displacement (more blocks per CU where LDS is the limit)
-    const size_t lds9 = sizeof(T) * (size_t)pl->max_u * 9, lds6 = sizeof(T) * (size_t)pl->max_u * 6;
+    // 3 FP64 arrays of max_u (output) and 6 of T (displacements, direction); 3 of T in the
+    // rest-state form, which stages no displacement (more blocks per CU where LDS is the limit)
+    const size_t lds9 = (3 * sizeof(double) + 6 * sizeof(T)) * (size_t)pl->max_u,
+                 lds6 = (3 * sizeof(double) + 3 * sizeof(T)) * (size_t)pl->max_u;
     T* part = reinterpret_cast<T*>(pl->d_part);  // d_part holds 3*n_partials doubles
     const bool has_mass = mat.elem_inst ? mat.inst_mc != nullptr : mat.mc != T(0);
     const int mass = has_mass ? (mat.mass_exact ? 2 : 1) : 0;
