@@ -6,24 +6,30 @@ Workload (single large mesh): cantilever 1.0 x 0.1 x 0.1 m, 200x20x20 hex cells 
 rho = 7850; nodes at x = 0 fixed; 1e5 N in -z spread over the x = L nodes.
 
 One step = one linear-static solve to a relative residual of 1e-8 (the reference's
-LinearStaticSolver::Solve: tangent K(x) at the current positions, Dirichlet elimination,
-CG with the stopping rule sqrt(rr/rr0) < tol, position update).  `value` = solves/s of the
-matrix-free FP64 path: CG (in its single-reduction arrangement: two launches per
-iteration) whose operator K(x) p is applied element by element from the inverse Jacobians
-(nothing assembled); the plain FP64 path (block assembly + Dirichlet + CG on the
-assembled matrix, the reference's arithmetic) is timed on the same problem and reported
-beside it.  Inputs are resident in HBM when the timed region starts.
+LinearStaticSolver::Solve: tangent K at the current positions, Dirichlet elimination,
+CG to sqrt(rr/rr0) < tol).  `value` = solves/s of the FP32 + FP64-refinement path: the
+flexible FP64 CG (u, r, alpha, beta and the stopping test in FP64) preconditioned by the
+FP32 two-level stage -- Jacobi-Chebyshev on the FP32 operator plus a coarse solve over the
+rigid motions of the beam's node planes (P^T K P probed with the FP64 operator, block-
+tridiagonal factorisation) -- with that setup inside every step; the Chebyshev degree is
+picked by timing one solve each of degree 4, 6 and 8 before the timed steps.  Beside it,
+on the same problem: the plain FP64 path (block assembly + Dirichlet + CG on the assembled
+matrix, the reference's arithmetic) and the matrix-free FP64 CG (K(x) p applied element by
+element from the inverse Jacobians, in the single-reduction arrangement).  Inputs are
+resident in HBM when the timed region starts.
 
 Kernel rooflines are measured live with HIP events on the launch stream (the null stream,
 which torch.cuda.Event records on):
-  * roofline      -- the step's dominant kernel pair, the matrix-free tangent at the rest
-                     positions (k_tangent<10,5,double,..,LIN,AFF> + k_gather_tangent: the
-                     kernel's linear-elasticity form, chosen when x equals X0, on an affine
-                     mesh, integrated from the corner values; inside the CG the gather is
-                     part of the update kernel), bytes/launch = 120*n_elem + 48*n_nodes (inverse Jacobian 72 B
-                     and detJ 8 B of one quadrature point -- the beam's elements are
-                     straight-sided, so all five agree -- and connectivity 40 B per
-                     element; p read and y written, 48 B per node);
+  * roofline      -- the step's dominant kernel pair, the FP32 matrix-free tangent at the
+                     rest positions (k_tangent<10,5,float,..,LIN,AFF> + k_bgather<float>: the
+                     linear-elasticity form of an affine mesh integrated from the corner
+                     values), bytes/launch = 80*n_elem + 24*n_nodes (inverse Jacobian 36 B,
+                     detJ 4 B of one quadrature point -- the beam's elements are straight-
+                     sided, so all five agree -- and connectivity 40 B per element; p read
+                     and y written, 24 B per node);
+  * tangent_fp64  -- the same in FP64 (k_tangent<10,5,double,..> + k_gather_tangent, the
+                     operator of the matrix-free FP64 CG), bytes/launch = 120*n_elem +
+                     48*n_nodes;
   * tangent_general -- the same kernel away from the rest positions (x and X0 read too:
                      120*n_elem + 96*n_nodes);
   * spmv_fp64     -- the FP64 block SpMV of the plain path, bytes/launch =
@@ -82,6 +88,9 @@ HBM_PEAK_GBS = 8000.0
 PMC_TANGENT_BYTES = (51220 * 2.0 + 29066 + 29871 * 2.0 + 21776) * 1024
 # the same for the rest-state (linear-elasticity) form the linear-static step runs
 PMC_TANGENT_REST_BYTES = (35012 * 2.0 + 29066 + 29871 * 2.0 + 21776) * 1024
+# the FP32 tangent pair of the two-level stage (k_tangent<float> + k_bgather<float>); None
+# until measured
+PMC_TANGENT32_REST_BYTES = None
 # the same for the force kernels with k_gather_partials: gradient-reading (k_force_planned)
 # and rest-coordinate (k_force_rest)
 PMC_FORCE_BYTES = (306420 * 2.0 + 29085 + 20867 * 2.0 + 15799) * 1024
@@ -323,6 +332,8 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cells", type=str, default="200,20,20")
+    ap.add_argument("--cheb-degree", type=int, default=0,
+                    help="Chebyshev degree of the FP32 stage (0: the fastest of 4, 6, 8 on one trial solve each)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-beams measurement")
     ap.add_argument("--batch-n", type=int, default=256)
     ap.add_argument("--batch-only", action="store_true", help="only the batched-beams measurement")
@@ -422,6 +433,45 @@ def main():
                 / torch.linalg.norm(body.f_ext[free])).item()
     log(f"matrix-free solution: true rel. residual {true_res:.3e} (recursive {ls.res:.3e})")
 
+    # FP32 two-level path (the headline): the batch machinery on this one mesh; every step
+    # sets up the preconditioner for K and solves in the flexible FP64 CG
+    from tlfea_amd.batch import Batch
+    bc = Batch([(m, dict(E=E, nu=nu))])
+
+    def step2(deg):
+        bc.precond_setup(degree=deg, ratio=30.0, power_iters=20)
+        bc.coarse_setup()
+        u = bc.solve(tol=args.tol, precond=True, max_iter=20000)
+        if bc.status[0] != 0:
+            raise RuntimeError(f"two-level solve did not converge: status {bc.status[0]}")
+        return u
+
+    trial = {}
+    degs = [args.cheb_degree] if args.cheb_degree > 0 else [4, 6, 8]
+    for deg in degs:
+        step2(deg)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        step2(deg)
+        torch.cuda.synchronize()
+        trial[deg] = time.perf_counter() - t
+    deg2 = min(trial, key=trial.get)
+    log(f"FP32 two-level trials (s per solve by Chebyshev degree): {trial}; using {deg2}")
+    for _ in range(args.warmup):
+        step2(deg2)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        u2 = step2(deg2)
+    torch.cuda.synchronize()
+    dt2 = time.perf_counter() - t
+    iters2, rel2 = int(bc.iters[0]), float(bc.rel[0])
+    u2_diff = (torch.linalg.norm(u2 - ls.u) / torch.linalg.norm(ls.u)).item()
+    true_res2 = (torch.linalg.norm((body.f_ext - body.tangent_apply(u2, x=body.X0))[free])
+                 / torch.linalg.norm(body.f_ext[free])).item()
+    log(f"FP32 two-level solve: {dt2 / args.steps * 1e3:.1f} ms, {iters2} iterations, rel res {rel2:.3e}, true "
+        f"{true_res2:.3e}, rel diff vs FP64 {u2_diff:.3e}")
+
     # ---- kernel rooflines (HIP events on the null stream) ---------------------------
     R = args.kernel_reps
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -440,6 +490,13 @@ def main():
         body.x[d].copy_(X0[d])
     p = torch.randn(3 * m.n_nodes, dtype=torch.float64, device="cuda")
     y = torch.empty_like(p)
+    # FP32 tangent of the two-level stage: inverse Jacobian (36 B) + detJ (4 B) of one point and
+    # connectivity (40 B) per element, p read and y written once (24 B) per node
+    p32 = p.float()
+    y32 = torch.empty_like(p32)
+    t32_s = event_time(lambda: bc.tangent_apply_f32(p32, out=y32))
+    t32_bytes = (40 if body.affine else 200) * m.n_elem + 40 * m.n_elem + 24 * m.n_nodes
+    t32_gbs = t32_bytes / t32_s / 1e9
     # matrix-free tangent as the step runs it (rest positions: the linear-elasticity form;
     # affine mesh: one quadrature point's geometry): inverse Jacobian (72 B) + detJ (8 B)
     # per element and quadrature point read (one point when affine, else 5) + connectivity
@@ -481,11 +538,12 @@ def main():
     copy_gbs = 2 * src.numel() * 8 / copy_s / 1e9
     read_gbs = src.numel() * 8 / read_s / 1e9
     del src, dst
-    for tag, gbs in (("tangent", tan_gbs), ("tangent general", tgen_gbs), ("spmv", spmv_gbs),
+    for tag, gbs in (("tangent", tan_gbs), ("tangent fp32", t32_gbs), ("tangent general", tgen_gbs), ("spmv", spmv_gbs),
                      ("force (gradients read)", force_gbs), ("force (rest coordinates)", rest_gbs)):
         # an algorithmic byte model above the peak would mean the kernel skips work it is charged for
         assert gbs <= HBM_PEAK_GBS, f"{tag}: {gbs:.0f} GB/s exceeds the HBM peak: byte model is wrong"
-    log(f"tangent (rest form) {tan_s * 1e6:.1f} us {tan_gbs:.0f} GB/s; tangent (general) {tgen_s * 1e6:.1f} us "
+    log(f"tangent FP32 (rest form) {t32_s * 1e6:.1f} us {t32_gbs:.0f} GB/s; "
+        f"tangent (rest form) {tan_s * 1e6:.1f} us {tan_gbs:.0f} GB/s; tangent (general) {tgen_s * 1e6:.1f} us "
         f"{tgen_gbs:.0f} GB/s; spmv64 {spmv_s * 1e6:.1f} us {spmv_gbs:.0f} GB/s; "
         f"force (gradients read) {force_s * 1e6:.1f} us {force_gbs:.0f} GB/s; force (rest coordinates) "
         f"{rest_s * 1e6:.1f} us {rest_gbs:.0f} GB/s; copy {copy_gbs:.0f} GB/s, read {read_gbs:.0f} GB/s")
@@ -586,27 +644,39 @@ def main():
     traffic_tan = PMC_TANGENT_REST_BYTES if default else None
     out = {
         "metric": "TET10 solves/s at 1e-8 rel. residual (learned+FP32 vs plain); force-kernel GB/s",
-        "value": args.steps / dt,
+        "value": args.steps / dt2,
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": dt2 / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"linear-static solve to a relative residual of {args.tol:g} (matrix-free FP64 CG "
-                               f"on K(x) + position update), cantilever {nx}x{ny}x{nz} cells x 6 TET10 "
-                               f"({m.n_elem} el, {m.n_nodes} nodes, {3 * m.n_nodes} DOF), 1e5 N tip load",
-                   "n_elem": m.n_elem, "n_dof": 3 * m.n_nodes, "cg_iterations": iters,
-                   "final_rel_residual": ls.res, "true_rel_residual": true_res, "K_nnz": dof_nnz,
-                   "u_rel_diff_vs_assembled": u_diff,
+        "config": {"workload": f"linear-static solve to a relative residual of {args.tol:g} (flexible FP64 CG with "
+                               f"the FP32 two-level preconditioner, set up per solve), cantilever {nx}x{ny}x{nz} "
+                               f"cells x 6 TET10 ({m.n_elem} el, {m.n_nodes} nodes, {3 * m.n_nodes} DOF), 1e5 N "
+                               f"tip load",
+                   "n_elem": m.n_elem, "n_dof": 3 * m.n_nodes, "iterations": iters2, "chebyshev_degree": deg2,
+                   "degree_trials_s": {str(k): v for k, v in trial.items()},
+                   "final_rel_residual": rel2, "true_rel_residual": true_res2,
+                   "u_rel_diff_vs_matfree_fp64": u2_diff, "K_nnz": dof_nnz,
                    "plain_fp64": {"solves_per_s": args.steps / dt64, "ms_per_solve": dt64 / args.steps * 1e3,
-                                  "cg_iterations": ls64.iters, "final_rel_residual": ls64.res},
-                   "speedup_vs_plain_fp64": dt64 / dt},
-        "roofline": {"bound": "hbm", "achieved": tan_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "cg_iterations": ls64.iters, "final_rel_residual": ls64.res,
+                                  "what": "block assembly + Dirichlet + CG on the assembled matrix"},
+                   "matfree_fp64": {"solves_per_s": args.steps / dt, "ms_per_solve": ms_per_step,
+                                    "cg_iterations": iters, "final_rel_residual": ls.res,
+                                    "true_rel_residual": true_res, "u_rel_diff_vs_assembled": u_diff},
+                   "speedup_vs_plain_fp64": dt64 / dt2, "speedup_vs_matfree_fp64": dt / dt2},
+        "roofline": {"bound": "hbm", "achieved": t32_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": t32_gbs / HBM_PEAK_GBS, "traffic": PMC_TANGENT32_REST_BYTES if default else None,
+                     "kernel": "k_tangent<10,5,float,true,0,true,true> + k_bgather<float> (the FP32 operator of the "
+                               "two-level stage at the rest positions, the linear-elasticity form of an affine mesh "
+                               "integrated from the corner values)",
+                     "avg_us": t32_s * 1e6, "bytes_per_launch": t32_bytes},
+        "tangent_fp64": {"bound": "hbm", "achieved": tan_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": tan_gbs / HBM_PEAK_GBS, "traffic": traffic_tan,
                      "kernel": "k_tangent<10,5,double,true,0,true,true> + k_gather_tangent (matrix-free K p at the "
                                "rest positions, the linear-elasticity form of an affine mesh integrated from the "

@@ -319,6 +319,9 @@ int tlfea_batch_set_compaction(tlfea_batch* b, double ratio);
 /* out = K p for every instance at its rest configuration, Dirichlet rows/columns masked
  * (the operator of tlfea_tangent_apply_planned per instance). */
 int tlfea_batch_tangent_apply(tlfea_batch* b, const double* p, double* out, void* stream);
+/* The same operator in FP32 (inverse Jacobians, det J and Lame parameters rounded to FP32,
+ * accumulation in FP64 LDS per block): the operator of the FP32 preconditioner stages. */
+int tlfea_batch_tangent_apply_f32(tlfea_batch* b, const float* p, float* out, void* stream);
 /* Adds the consistent mass M, applied without assembly, to the operator of
  * tlfea_batch_cg_solve: it becomes K + c M per instance, with rho_coef[i] = rho_i * c
  * (host [n_inst]); NULL removes it.  Used by implicit time steps (c = 1/dt^2).

@@ -102,6 +102,12 @@ class Batch:
         call("tlfea_batch_tangent_apply", self.h, ptr(p), ptr(out), self.stream)
         return out
 
+    def tangent_apply_f32(self, p, out=None):
+        """K p with the FP32 operator of the preconditioner (p and out float32)."""
+        out = torch.empty(3 * self.n_nodes, dtype=torch.float32, device=DEV) if out is None else out
+        call("tlfea_batch_tangent_apply_f32", self.h, ptr(p), ptr(out), self.stream)
+        return out
+
     def set_mass(self, rho_coef, rule="quadrature"):
         """Operator K + c M per instance (rho_coef[i] = rho_i * c); None removes it.  rule
         "quadrature": the mass of the element's quadrature rule (as the reference's mass

@@ -135,6 +135,7 @@ _SIGS = {
     "tlfea_batch_sizes": (i32, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
     "tlfea_batch_set_chunk": (i32, [vp, i32]),
     "tlfea_batch_tangent_apply": (i32, [vp, vp, vp, vp]),
+    "tlfea_batch_tangent_apply_f32": (i32, [vp, vp, vp, vp]),
     "tlfea_batch_cg_solve": (i32, [vp, vp, vp, i32, f64, i32, i32, i32, i32, vp, vp, vp, C.POINTER(i32), vp]),
     "tlfea_batch_set_compaction": (i32, [vp, f64]),
     "tlfea_batch_autotune": (i32, [vp, vp, vp, i32, f64, i32, C.POINTER(BatchTuning), vp]),

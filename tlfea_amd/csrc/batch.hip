@@ -1743,6 +1743,12 @@ extern "C" int tlfea_batch_tangent_apply(tlfea_batch* b, const double* p, double
     return apply64(b, p, out, nullptr, as_stream(stream), 1.0, false);
 }
 
+extern "C" int tlfea_batch_tangent_apply_f32(tlfea_batch* b, const float* p, float* out, void* stream) {
+    TLFEA_REQUIRE(b && p && out, "null argument");
+    full_lists(b);
+    return apply32(b, p, out, nullptr, as_stream(stream));
+}
+
 extern "C" int tlfea_batch_set_mass(tlfea_batch* b, const double* rho_coef, int rule) {
     TLFEA_REQUIRE(b != nullptr, "null batch");
     drop_graph(b);
