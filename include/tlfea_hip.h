@@ -319,8 +319,9 @@ int tlfea_batch_set_compaction(tlfea_batch* b, double ratio);
 /* out = K p for every instance at its rest configuration, Dirichlet rows/columns masked
  * (the operator of tlfea_tangent_apply_planned per instance). */
 int tlfea_batch_tangent_apply(tlfea_batch* b, const double* p, double* out, void* stream);
-/* The same operator in FP32 (inverse Jacobians, det J and Lame parameters rounded to FP32,
- * accumulation in FP64 LDS per block): the operator of the FP32 preconditioner stages. */
+/* The batch's operator (K, plus c M once tlfea_batch_set_mass has added it) in FP32
+ * (inverse Jacobians, det J, Lame parameters and c rounded to FP32, accumulation in FP64
+ * LDS per block): the operator of the FP32 preconditioner stages. */
 int tlfea_batch_tangent_apply_f32(tlfea_batch* b, const float* p, float* out, void* stream);
 /* Adds the consistent mass M, applied without assembly, to the operator of
  * tlfea_batch_cg_solve: it becomes K + c M per instance, with rho_coef[i] = rho_i * c
@@ -329,7 +330,8 @@ int tlfea_batch_tangent_apply_f32(tlfea_batch* b, const float* p, float* out, vo
  * FEAT10Data.cu:200-271; FEAT4Data.cu:197-243), whose TET10 mass is indefinite (negative
  * centroid weight) and TET4 mass rank one, so K + c M loses definiteness once c M
  * dominates K; rule 1: the exactly integrated mass of straight-sided elements (symmetric
- * positive definite, same row sums). */
+ * positive definite, same row sums).  Removes the preconditioner (tlfea_batch_precond_setup
+ * and tlfea_batch_coarse_setup were for the previous operator: set them up again). */
 int tlfea_batch_set_mass(tlfea_batch* b, const double* rho_coef, int rule);
 /* out = (k_scale K + with_mass * c M) p for every instance; masked != 0 applies the
  * Dirichlet elimination (fixed rows return p, fixed columns are dropped), masked = 0 the
@@ -351,8 +353,9 @@ int tlfea_batch_internal_force(tlfea_batch* b, double* f, void* stream);
  * chunk partials summed per instance in chunk order (deterministic).  Per-instance norms
  * for the Newton iterations of implicit steps.  Not in the reference. */
 int tlfea_batch_dot(tlfea_batch* b, const double* a, const double* c, double* out, void* stream);
-/* FP32 Jacobi-Chebyshev preconditioner for tlfea_batch_cg_solve(precond = 1): the
- * diagonal D of K (FP64, matrix-free), lambda_max(D^-1 K) per instance by `power_iters`
+/* FP32 Jacobi-Chebyshev preconditioner for tlfea_batch_cg_solve(precond = 1) of the batch's
+ * operator A (K, or K + c M with a mass term, at the current positions): the diagonal D of
+ * A (FP64, matrix-free), lambda_max(D^-1 A) per instance by `power_iters`
  * FP32 power iterations (scaled by `safety` >= 1), and `degree` Chebyshev steps on
  * [lambda_max / ratio, lambda_max] applied with the FP32 operator.  degree = 0 removes it.
  * lambda_max (host [n_inst], may be NULL) receives the unscaled estimates. */
@@ -361,12 +364,13 @@ int tlfea_batch_precond_setup(tlfea_batch* b, int degree, double ratio, int powe
 /* Two-level coarse correction for the FP32 stage, not in the reference: the nodes of each
  * instance are grouped into planes across its longest axis (the mesh's lattice planes, or
  * equal-width bins of an unstructured mesh); the coarse space holds the 6 rigid motions of
- * each plane's free nodes, and A_c = P^T K P is assembled by probing with the FP64 operator
+ * each plane's free nodes, and A_c = P^T A P (A = K, or K + c M with a mass term) is
+ * assembled by probing with the FP64 operator
  * and factored per instance as a block-tridiagonal matrix (blocks of W planes, W = the
  * widest plane span of an element, at most 3: TLFEA_ECAPACITY beyond).  tlfea_batch_cg_solve
  * (precond = 1) then applies the additive two-level preconditioner z = P A_c^-1 P^T r +
  * Chebyshev(r) inside the flexible FP64 CG: the beams' bending modes, which the Chebyshev
- * stage cannot reach, go to the coarse solve.  Needs tlfea_batch_precond_setup first and no mass term;
+ * stage cannot reach, go to the coarse solve.  Needs tlfea_batch_precond_setup first;
  * enable = 0 removes it.  n_coarse (may be NULL) receives the coarse dimension. */
 int tlfea_batch_coarse_setup(tlfea_batch* b, int enable, int* n_coarse, void* stream);
 /* z = P A_c^-1 P^T r for every instance (FP64; r and z [3 n_nodes] on the device): the

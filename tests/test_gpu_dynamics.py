@@ -230,3 +230,62 @@ def test_batch_internal_force_and_moved_operator(gpu):
         assert rel_err(y[sl], yo[i]) < 1e-12, i
     b.set_displacement(None)
     assert np.abs(_np(b.internal_force())).max() < 1e-6 * np.abs(f).max()
+
+
+@pytest.mark.parametrize("rule", ["quadrature", "exact"])
+def test_fp32_operator_and_diagonal_with_mass(gpu, rule):
+    """With a mass term the FP32 operator of the preconditioner is K + c M (against the
+    oracle's dense matrices, FP32 bar 1e-5), and the preconditioner's diagonal includes c M:
+    its lambda_max(D^-1 A) estimate equals the dense one to 1%, where a diagonal without the
+    mass term would be off by more than 10%."""
+    import torch
+    from tlfea_amd.batch import Batch
+    m, mat = _beam(4, 2, 1, [0, 0, -10.0], E=2e7, rho=1500.0)
+    b = Batch([(m, mat)])
+    c = 1e8
+    b.set_mass(np.array([mat["rho"] * c]), rule=rule)
+    K, M = _dense_KM(m, mat, rule)
+    free = _np(b.mask) == 0
+    A = (K + c * M)[np.ix_(free, free)]
+    p = np.random.default_rng(3).standard_normal(3 * m.n_nodes)
+    y = _np(b.tangent_apply_f32(torch.as_tensor(p, dtype=torch.float32).cuda()))
+    assert rel_err(y[free], A @ p[free]) < 1e-5
+    lm = b.precond_setup(degree=2, ratio=30.0, power_iters=400, safety=1.0)[0]
+
+    def lmax(d):
+        s = 1.0 / np.sqrt(d)
+        return np.linalg.eigvalsh(A * s[:, None] * s[None, :]).max()
+    ref = lmax(np.diag(A))
+    wrong = lmax(np.diag(K[np.ix_(free, free)]))
+    assert abs(wrong / ref - 1) > 0.1
+    assert abs(lm / ref - 1) < 1e-2, (lm, ref, wrong)
+
+
+@pytest.mark.parametrize("nonlinear", [False, True])
+def test_preconditioned_steps_match_plain(gpu, nonlinear):
+    """Time steps whose solves run the flexible FP64 CG with the FP32 two-level preconditioner
+    of K + M/dt^2 (set up once at rest) land on the plain FP64 CG's trajectory (1e-7 over 12
+    steps: both solve to 1e-10) in a third of its iterations or fewer."""
+    from tlfea_amd import mesh as M
+    from tlfea_amd.dynamics import BatchDynamics, dynamic_beams
+    items, amp, freq = dynamic_beams(4, 41)
+    rng = np.random.default_rng(8)
+    small = []
+    for m, mat in items:
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        small.append((M.clamp_and_tip_load(M.kuhn_beam(int(rng.integers(8, 20)), 2, 2, 1.0, 0.1, 0.1), d), mat))
+    kw = dict(tol=1e-10, tol_nl=1e-10, newton_cg_tol=1e-12, nonlinear=nonlinear)
+    plain = BatchDynamics(small, amp, freq, **kw)
+    pre = BatchDynamics(small, amp, freq, cheb_degree=4, coarse=True, **kw)
+    for k in range(12):
+        u0, v0 = plain.step()
+        u1, v1 = pre.step()
+        for i in range(len(small)):
+            sl = plain.b.inst_slice(i)
+            assert rel_err(_np(u1[sl]), _np(u0[sl])) < 1e-7, (k, i)
+            assert rel_err(_np(v1[sl]), _np(v0[sl])) < 1e-7, (k, i)
+    ip, iq = np.array(plain.iters), np.array(pre.iters)
+    print(f"[precond dynamics nonlinear={nonlinear}] CG iterations per instance-step: plain median "
+          f"{np.median(ip):.0f}, preconditioned median {np.median(iq):.0f}", flush=True)
+    assert np.median(iq) * 3 <= np.median(ip)

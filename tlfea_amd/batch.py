@@ -103,7 +103,8 @@ class Batch:
         return out
 
     def tangent_apply_f32(self, p, out=None):
-        """K p with the FP32 operator of the preconditioner (p and out float32)."""
+        """(K + c M) p with the FP32 operator of the preconditioner (the mass term when set;
+        p and out float32)."""
         out = torch.empty(3 * self.n_nodes, dtype=torch.float32, device=DEV) if out is None else out
         call("tlfea_batch_tangent_apply_f32", self.h, ptr(p), ptr(out), self.stream)
         return out
@@ -112,7 +113,8 @@ class Batch:
         """Operator K + c M per instance (rho_coef[i] = rho_i * c); None removes it.  rule
         "quadrature": the mass of the element's quadrature rule (as the reference's mass
         assembly, indefinite for TET10); "exact": the exactly integrated mass (positive
-        definite)."""
+        definite).  Drops the preconditioner (set it up again for the new operator)."""
+        self.precond_degree = 0
         if rho_coef is None:
             call("tlfea_batch_set_mass", self.h, None, 0)
             self.mass_on = False
@@ -154,7 +156,8 @@ class Batch:
         return out
 
     def precond_setup(self, degree=6, ratio=30.0, power_iters=20, safety=1.1):
-        """FP32 Jacobi-Chebyshev preconditioner; returns the lambda_max(D^-1 K) estimates."""
+        """FP32 Jacobi-Chebyshev preconditioner of the operator (K, or K + c M when a mass
+        term is set, at the current positions); returns the lambda_max(D^-1 A) estimates."""
         lm = np.zeros(self.n_inst)
         call("tlfea_batch_precond_setup", self.h, int(degree), float(ratio), int(power_iters), float(safety),
              lm.ctypes.data, self.stream)

@@ -53,6 +53,7 @@ struct tlfea_batch {
     float* d_dinv32 = nullptr;  // inverse diagonal of K (1 on fixed DOFs)
     int cheb_degree = 0;        // 0 = no preconditioner set up
     double* d_mc = nullptr;     // per-instance mass coefficient (rho * c) of the operator K + c M
+    float* d_mc32 = nullptr;    // the same in FP32, for the preconditioner's operator
     bool mass_on = false;
     int mass_exact = 0;         // 0: quadrature-rule mass, 1: exactly integrated mass
     // current displacement from the rest positions (tlfea_batch_set_displacement); the
@@ -1027,7 +1028,7 @@ void free_batch(tlfea_batch* b) {
                     b->d_jinv32, b->d_detJ32, b->d_lam, b->d_mu, b->d_lam32, b->d_mu32, b->d_chunk_inst,
                     b->d_chunk_beg, b->d_chunk_end, b->d_inst_chunk_off, b->d_r, b->d_p, b->d_q, b->d_z,
                     b->d_r32, b->d_z32, b->d_d32, b->d_q32, b->d_part, b->d_s, b->d_i, b->d_all, b->d_cheb,
-                    b->d_rold, b->d_dinv32, b->d_blk_list, b->d_chunk_list, b->d_inst_list, b->d_mc,
+                    b->d_rold, b->d_dinv32, b->d_blk_list, b->d_chunk_list, b->d_inst_list, b->d_mc, b->d_mc32,
                     b->d_ux, b->d_uy, b->d_uz, b->d_node_plane, b->d_node_r, b->d_plane_off,
                     b->d_plane_nodes, b->d_plane_inst, b->d_plane_local, b->d_inst_cblk, b->d_cdiag,
                     b->d_clow, b->d_cg, b->d_crhs, b->d_cx};
@@ -1158,9 +1159,11 @@ int apply64(tlfea_batch* b, const double* p, double* q, const int* iflags, hipSt
     return TLFEA_OK;
 }
 
-// FP32 batched operator q = K p (masked), no dot partials.
+// FP32 batched operator q = (K + c M) p (masked; the mass term when set up), no dot partials.
 int apply32(tlfea_batch* b, const float* p, float* q, const int* iflags, hipStream_t s) {
     TanMat<float> tm{0.0f, 0.0f, b->d_elem_inst, b->d_lam32, b->d_mu32, iflags, NI};
+    tm.inst_mc = b->mass_on ? b->d_mc32 : nullptr;
+    tm.mass_exact = b->mass_exact;
     tm.at_rest = b->moved ? 0 : 1;
     tm.affine = b->affine;
     if (int rc = launch_tangent_partials<float>(b->plan, pos_x(b), pos_y(b), pos_z(b), rest_x(b), rest_y(b),
@@ -1426,7 +1429,6 @@ extern "C" int tlfea_batch_precond_setup(tlfea_batch* b, int degree, double rati
                                          double safety, double* lambda_max, void* stream) {
     TLFEA_REQUIRE(b != nullptr, "null batch");
     TLFEA_REQUIRE(degree >= 0 && degree <= 16, "Chebyshev degree out of range [0, 16]");
-    TLFEA_REQUIRE(degree == 0 || !b->mass_on, "the Chebyshev stage covers K only (a mass term is set)");
     TLFEA_REQUIRE(degree == 0 || (ratio > 1.0 && power_iters >= 1 && safety >= 1.0), "bad Chebyshev parameters");
     b->cheb_degree = 0;
     drop_graph(b);
@@ -1435,8 +1437,10 @@ extern "C" int tlfea_batch_precond_setup(tlfea_batch* b, int degree, double rati
     hipStream_t s = as_stream(stream);
     const Chunks ch = chunks(b);
     const int64_t nd = 3 * (int64_t)b->n_nodes;
-    // diagonal of K at the rest configuration: partials, then the gather with p = 1
-    const TanMat<double> tm{0.0, 0.0, b->d_elem_inst, b->d_lam, b->d_mu, nullptr, NI};
+    // diagonal of K + c M at the current configuration: partials, then the gather with p = 1
+    TanMat<double> tm{0.0, 0.0, b->d_elem_inst, b->d_lam, b->d_mu, nullptr, NI};
+    tm.inst_mc = b->mass_on ? b->d_mc : nullptr;
+    tm.mass_exact = b->mass_exact;
     if (int rc = launch_tangent_diag_partials(b->plan, pos_x(b), pos_y(b), pos_z(b), rest_x(b), rest_y(b), rest_z(b),
                                               b->d_jinv,
                                               b->d_detJ, b->quad, tm, s))
@@ -1516,7 +1520,6 @@ extern "C" int tlfea_batch_coarse_setup(tlfea_batch* b, int enable, int* n_coars
     b->coarse_on = false;
     if (!enable) return TLFEA_OK;
     TLFEA_REQUIRE(b->cheb_degree > 0, "the coarse correction runs inside the Chebyshev stage: set that up first");
-    TLFEA_REQUIRE(!b->mass_on, "the coarse correction covers K only (a mass term is set)");
     hipStream_t s = as_stream(stream);
     const int N = b->n_nodes, NE = b->n_elem, nn = b->nn;
     // host staging without zero fill (every entry is written before it is read)
@@ -1660,7 +1663,7 @@ extern "C" int tlfea_batch_coarse_setup(tlfea_batch* b, int enable, int* n_coars
     b->cnb = NB;
     b->n_planes = P;
     b->n_cblocks = (int)nblk;
-    // P^T K P by probing: planes coloured by local index mod 2W+1 are further apart than any
+    // P^T A P (A = K + c M when a mass term is set) by probing: planes coloured by local index mod 2W+1 are further apart than any
     // element reaches, so each coloured rigid mode's image restricted to a plane belongs to
     // the one coloured plane within W of it
     const int period = 2 * W + 1;
@@ -1668,7 +1671,7 @@ extern "C" int tlfea_batch_coarse_setup(tlfea_batch* b, int enable, int* n_coars
         for (int j = 0; j < 6 && rc == TLFEA_OK; j++) {
             k_coarse_probe_vec<<<grid_for(N, 256), 256, 0, s>>>(N, b->d_node_plane, b->d_node_r, b->d_plane_local,
                                                                 period, color, j, b->d_p);
-            if ((rc = apply64(b, b->d_p, b->d_q, nullptr, s, 1.0, false, true))) break;
+            if ((rc = apply64(b, b->d_p, b->d_q, nullptr, s))) break;
             k_coarse_probe_store<<<P, 64, 0, s>>>(W, period, color, j, b->d_plane_off, b->d_plane_nodes,
                                                  b->d_plane_local, b->d_plane_inst, d_inst_pn, b->d_node_r, b->d_q,
                                                  b->d_cdiag, b->d_clow);
@@ -1752,14 +1755,18 @@ extern "C" int tlfea_batch_tangent_apply_f32(tlfea_batch* b, const float* p, flo
 extern "C" int tlfea_batch_set_mass(tlfea_batch* b, const double* rho_coef, int rule) {
     TLFEA_REQUIRE(b != nullptr, "null batch");
     drop_graph(b);
+    TLFEA_REQUIRE(!rho_coef || rule == 0 || rule == 1, "mass rule must be 0 (quadrature) or 1 (exact)");
+    // the preconditioner was set up for the previous operator
+    b->cheb_degree = 0;
+    b->coarse_on = false;
     if (!rho_coef) {
         b->mass_on = false;
         return TLFEA_OK;
     }
-    TLFEA_REQUIRE(rule == 0 || rule == 1, "mass rule must be 0 (quadrature) or 1 (exact)");
-    TLFEA_REQUIRE(b->cheb_degree == 0, "the Chebyshev stage covers K only: remove it before adding a mass term");
-    if (!b->d_mc && dalloc(&b->d_mc, b->n_inst)) return TLFEA_EHIP;
+    if (!b->d_mc && (dalloc(&b->d_mc, b->n_inst) || dalloc(&b->d_mc32, b->n_inst))) return TLFEA_EHIP;
     TLFEA_HIP_TRY(hipMemcpy(b->d_mc, rho_coef, sizeof(double) * b->n_inst, hipMemcpyHostToDevice));
+    std::vector<float> mc32(rho_coef, rho_coef + b->n_inst);
+    TLFEA_HIP_TRY(hipMemcpy(b->d_mc32, mc32.data(), sizeof(float) * b->n_inst, hipMemcpyHostToDevice));
     b->mass_on = true;
     b->mass_exact = rule;
     return TLFEA_OK;

@@ -754,7 +754,9 @@ int positions_at_rest(int n_nodes, const double* x, const double* y, const doubl
 // Diagonal of the tangent, d[(a,i)] = K_(a,i),(a,i): the diagonal terms of the element
 // blocks of compute_hessian_assemble_csr (FEAT10DataFunc.cuh:512-768) at the positions x,
 //   dV [ (lambda + mu) (F g_a)_i^2 + |g_a|^2 (lambda tr E + mu ((F F^T)_ii - 1)) + mu |F g_a|^2 ],
-// written as block partials like the tangent (gathered by the same kernels).
+// scaled by k_scale, plus c M_aa when the material carries a mass term (c dV N_a^2 per point
+// for the quadrature-rule mass, c V exact_mass(a, a) for the exact one), written as block
+// partials like the tangent (gathered by the same kernels).
 // ------------------------------------------------------------------------------------
 template <int NN, int NQ>
 __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent_diag(
@@ -781,6 +783,9 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent_diag(
     if (e < n_elem) {
         const int inst = mat.elem_inst ? mat.elem_inst[e] : 0;
         const double lam = tan_lam(mat, inst), mu = tan_mu(mat, inst);
+        const double mc = mat.elem_inst ? (mat.inst_mc ? mat.inst_mc[inst] : 0.0) : mat.mc;
+        const double ks = mat.k_scale;
+        const int mass = mc != 0.0 ? (mat.mass_exact ? 2 : 1) : 0;
         const int tile = e >> 6, lane = e & (TILE - 1);
         int sl[NN];
 #pragma unroll
@@ -788,6 +793,7 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent_diag(
         double da[NN][3];
 #pragma unroll
         for (int a = 0; a < NN; a++) da[a][0] = da[a][1] = da[a][2] = 0.0;
+        double vol = 0.0;
 #pragma unroll 1
         for (int q = 0; q < NQ; q++) {
             const int64_t tq = (int64_t)tile * NQ + q;
@@ -795,6 +801,7 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent_diag(
 #pragma unroll
             for (int c = 0; c < 9; c++) M[c / 3][c % 3] = jinv_t[(tq * 9 + c) * TILE + lane];
             const double dV = detJ_t[tq * TILE + lane] * quad.w[q];
+            vol += dV;
             double Ju[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 #pragma unroll
             for (int a = 0; a < NN; a++) {
@@ -830,11 +837,21 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent_diag(
 #pragma unroll
                 for (int i = 0; i < 3; i++) Fh[i] = F[i][0] * g[0] + F[i][1] * g[1] + F[i][2] * g[2];
                 const double FhFh = Fh[0] * Fh[0] + Fh[1] * Fh[1] + Fh[2] * Fh[2];
+                const double m1 = mass == 1 ? mc * dV * dn.N[q][a] * dn.N[q][a] : 0.0;
 #pragma unroll
                 for (int i = 0; i < 3; i++)
-                    da[a][i] += dV * ((lam + mu) * Fh[i] * Fh[i] + gg * (lam * trE + mu * (FFd[i] - 1.0)) + mu * FhFh);
+                    da[a][i] += ks * dV * ((lam + mu) * Fh[i] * Fh[i] + gg * (lam * trE + mu * (FFd[i] - 1.0)) +
+                                           mu * FhFh) + m1;
             }
         }
+        if (mass == 2)
+#pragma unroll
+            for (int a = 0; a < NN; a++) {
+                const double m2 = mc * vol * exact_mass<NN>(a, a);
+                da[a][0] += m2;
+                da[a][1] += m2;
+                da[a][2] += m2;
+            }
 #pragma unroll
         for (int a = 0; a < NN; a++)
 #pragma unroll

@@ -28,6 +28,12 @@ instance whose ratio has stopped halving below 1e-6 sits at the FP64 floor of ev
 g (rounding of f_int against the load increment); the step then ends and the instance is
 counted in ``floor`` instead of failing.
 
+With cheb_degree > 0 every solve is the flexible FP64 CG preconditioned by the FP32
+Jacobi-Chebyshev stage of that degree on K + M/dt^2 (and, with coarse=True, the additive
+coarse correction P A_c^-1 P^T over rigid motions of node planes), both set up once at the
+rest state and kept for every step: the mass term dominates the low modes at these time
+steps, so the frozen preconditioner stays close to the current operator.
+
 M is the exactly integrated consistent mass by default (mass_rule="exact").  The
 reference's 5-point mass (mass_matrix_qp_kernel, FEAT10Data.cu:200-271) is indefinite,
 so K + M/dt^2 is indefinite once M/dt^2 outweighs K -- on soft beams (E of a few MPa)
@@ -57,15 +63,20 @@ def dynamic_beams(n, seed):
 
 class BatchDynamics:
     def __init__(self, items, amp, freq, dt=1e-4, g=(0.0, 0.0, -9.81), tol=1e-8, batch=None, mass_rule="exact",
-                 nonlinear=False, tol_nl=1e-8, newton_cg_tol=1e-10, max_newton=8):
+                 nonlinear=False, tol_nl=1e-8, newton_cg_tol=1e-10, max_newton=8, cheb_degree=0, coarse=False):
         self.b = batch if batch is not None else Batch(items)
         b = self.b
+        self.precond = cheb_degree > 0
         self.dt, self.tol = dt, tol
         self.nonlinear, self.tol_nl, self.newton_cg_tol, self.max_newton = nonlinear, tol_nl, newton_cg_tol, max_newton
         b.set_displacement(None)
         self.c = 1.0 / dt ** 2
         rho = np.array([mat.get("rho", 1000.0) for _, mat in items])
         b.set_mass(rho * self.c, rule=mass_rule)
+        if self.precond:
+            b.precond_setup(cheb_degree)
+            if coarse:
+                b.coarse_setup(True)
         n3 = 3 * b.n_nodes
         self.fixed = b.mask.bool()
         counts = 3 * np.diff(b.node_off)
@@ -110,7 +121,7 @@ class BatchDynamics:
         t1 = self.t + dt
         rhs = b.apply(self.v, k_scale=0.0, mass=True) - b.apply(self.u, k_scale=1.0) / dt + self.load(t1) / dt
         rhs[self.fixed] = 0.0
-        v1 = b.solve(tol=self.tol, f=rhs, u0=self.v)
+        v1 = b.solve(tol=self.tol, f=rhs, u0=self.v, precond=self.precond)
         if b.rc != 0:
             raise RuntimeError(f"time step {len(self.iters) + 1}: {int((b.status != 0).sum())} instances did not "
                                f"converge")
@@ -152,7 +163,7 @@ class BatchDynamics:
             rhs[ok[self.inst]] = 0.0
             worst = float(ratio[~ok].max())
             cg_tol = min(0.1, max(self.newton_cg_tol, 0.5 * self.tol_nl / max(worst, 1e-300)))
-            dv = b.solve(tol=cg_tol, f=rhs)
+            dv = b.solve(tol=cg_tol, f=rhs, precond=self.precond)
             if b.rc != 0:
                 raise RuntimeError(f"time step {len(self.iters) + 1}, Newton iteration {k + 1}: "
                                    f"{int((b.status != 0).sum())} instances did not converge")
