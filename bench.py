@@ -185,6 +185,46 @@ def batched_beams(n, tol, log, sequential=True):
     return res
 
 
+def batched_nesterov_bench(n, tol, log, dt=1e-3, max_inner=50000):
+    """One implicit step of dt from rest for every beam of the sampler's batch by the batched
+    SyncedNesterov loop (tlfea_batch_nesterov_solve: per-instance step size from a power
+    iteration, restarted momentum) to ||g|| <= tol ||g(0)||."""
+    import numpy as np
+    import torch
+    from tlfea_amd import mesh as M
+    from tlfea_amd.batch import Batch
+    items = M.random_beam_batch(n, 2026)
+    b = Batch(items)
+    rho = np.array([mat["rho"] for _, mat in items])
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    b.set_mass(rho / dt ** 2, rule="exact")
+    alpha = 1.0 / (1.1 * dt * b.lambda_max())
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t
+    zero = torch.zeros(3 * b.n_nodes, dtype=torch.float64, device="cuda")
+
+    def run():
+        return b.nesterov_solve(dt, rho, alpha, zero, zero, b.f, rel_tol=tol, max_inner=max_inner)
+    run()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    run()
+    torch.cuda.synchronize()
+    s = time.perf_counter() - t
+    ok = int((b.status == 0).sum())
+    res = {"workload": f"{n} cantilevers of the beam sampler (seed 2026), {b.n_elem} TET10 elements: one implicit "
+                       f"step of {dt:g} s from rest per beam, SyncedNesterov to ||g|| <= {tol:g} ||g(0)|| (restarted "
+                       f"momentum, step 1/(1.1 dt lambda_max) per beam)",
+           "solves_per_s": ok / s, "s": s, "converged": ok, "setup_s": setup_s,
+           "inner_iters_median": float(np.median(b.iters)), "inner_iters_max": int(b.iters.max()),
+           "batched_iterations": b.total_iters, "ms_per_batched_iteration": s * 1e3 / max(b.total_iters, 1)}
+    log(f"batched Nesterov step: {s:.2f} s, {ok}/{n} converged, median {np.median(b.iters):.0f} / max "
+        f"{b.iters.max()} inner iterations")
+    b.close()
+    return res
+
+
 def velocity_solvers_bench(m, E, nu, rho, log, inner=200):
     """SyncedNesterov / SyncedAdamW / SyncedAdamWNocoop on the beam (one implicit step of
     1e-3 s each, zero tolerances so that every solve runs one outer iteration of `inner`
@@ -272,11 +312,12 @@ def dynamic_beams_bench(n, steps, seq_steps, tol, log):
                        f"per instance (inexact Newton, batched CG inside); linearised steps: CG to {tol:g}"}
     final = {}
     b = None
-    for mode in ("newton", "linearised"):
-        dyn = BatchDynamics(items, amp, freq, tol=tol, tol_nl=tol, nonlinear=mode == "newton", batch=b)
+    for mode in ("newton", "linearised", "nesterov"):
+        kw = dict(tol=tol, tol_nl=tol, nonlinear=mode == "newton", nesterov=mode == "nesterov", batch=b)
+        dyn = BatchDynamics(items, amp, freq, **kw)
         b = dyn.b
         dyn.step()  # warm-up (graph capture)
-        dyn = BatchDynamics(items, amp, freq, tol=tol, tol_nl=tol, nonlinear=mode == "newton", batch=b)
+        dyn = BatchDynamics(items, amp, freq, **kw)
         torch.cuda.synchronize()
         t = time.perf_counter()
         for _ in range(steps):
@@ -284,8 +325,9 @@ def dynamic_beams_bench(n, steps, seq_steps, tol, log):
         torch.cuda.synchronize()
         dt_b = time.perf_counter() - t
         its = np.array(dyn.iters)
+        key = "inner_iters" if mode == "nesterov" else "cg_iters"
         r = {"steps": steps, "s": dt_b, "instance_steps_per_s": n * steps / dt_b,
-             "cg_iters_median": float(np.median(its)), "cg_iters_max": int(its.max())}
+             f"{key}_median": float(np.median(its)), f"{key}_max": int(its.max())}
         if mode == "newton":
             r["newton_iters_mean"] = float(np.mean(dyn.newton))
             r["newton_iters_max"] = int(np.max(dyn.newton))
@@ -294,10 +336,11 @@ def dynamic_beams_bench(n, steps, seq_steps, tol, log):
         final[mode] = dyn.u.clone()
         res.setdefault("n_elem", b.n_elem)
         res.setdefault("n_dof", 3 * b.n_nodes)
-        log(f"dynamic {mode}: {steps} steps of {n} beams in {dt_b:.2f} s, median {np.median(its):.0f} CG iterations")
-    res["linearised_vs_newton_max_rel_diff"] = max(
-        float(torch.linalg.norm(final["linearised"][b.inst_slice(i)] - final["newton"][b.inst_slice(i)])
-              / torch.linalg.norm(final["newton"][b.inst_slice(i)])) for i in range(n))
+        log(f"dynamic {mode}: {steps} steps of {n} beams in {dt_b:.2f} s, median {np.median(its):.0f} iterations")
+    for other in ("linearised", "nesterov"):
+        res[f"{other}_vs_newton_max_rel_diff"] = max(
+            float(torch.linalg.norm(final[other][b.inst_slice(i)] - final["newton"][b.inst_slice(i)])
+                  / torch.linalg.norm(final["newton"][b.inst_slice(i)])) for i in range(n))
     # the same beams one at a time (Newton), for seq_steps steps, and the agreement after them
     dyn2 = BatchDynamics(items, amp, freq, tol=tol, tol_nl=tol, nonlinear=True, batch=b)
     for _ in range(seq_steps):
@@ -389,7 +432,9 @@ def main():
         print(json.dumps({"velocity_solvers": vs}))
         return
     if args.batch_only:
-        print(json.dumps({"batched": batched_beams(args.batch_n, args.tol, log, not args.no_seq)}))
+        bb = batched_beams(args.batch_n, args.tol, log, not args.no_seq)
+        bb["nesterov_step"] = batched_nesterov_bench(args.batch_n, args.tol, log)
+        print(json.dumps({"batched": bb}))
         return
     t0 = time.time()
     m = M.clamp_and_tip_load(M.kuhn_beam(nx, ny, nz), -1.0e5)
@@ -634,6 +679,8 @@ def main():
     if velocity is not None:
         velocity["c1_nesterov"] = c1_nesterov_bench(E, nu, rho, args.tol, log)
     batched = None if args.no_batch else batched_beams(args.batch_n, args.tol, log, not args.no_seq)
+    if batched is not None:
+        batched["nesterov_step"] = batched_nesterov_bench(args.batch_n, args.tol, log)
     dynamic = None if args.no_dynamic else dynamic_beams_bench(64, 200, 20, args.tol, log)
 
     # HBM bytes per launch on this workload from two rocprofv3 PMC passes (FETCH_SIZE x 2.0

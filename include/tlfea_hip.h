@@ -395,6 +395,33 @@ int tlfea_batch_smooth(tlfea_batch* b, const double* f, double* u, int sweeps, v
 int tlfea_batch_cg_solve(tlfea_batch* b, const double* f, double* u, int use_guess, double tol, int max_iter,
                          int poll, int use_graph, int precond, int* iters, double* rel_res, int* status,
                          int* total_iters, void* stream);
+/* Batched SyncedNesterov: one implicit step for every instance at once, replacing the
+ * inner loop of SyncedNesterovSolver::Solve (src/solvers/SyncedNesterov.cu:97-360, the
+ * gradient of :32-95) run per body.  Each instance minimises over v the step equation
+ *   g(v) = M (v - v_n) / dt + f_int(X + u_n + dt v) - f_ext = 0
+ * with its own step size alpha[i] (host [n_inst]), momentum sequence and stopping test:
+ * y = v_k + beta (v_k - v_(k-1)), v_next = y - alpha g(y), beta = (t_k - 1) / t_(k+1).
+ * M is the consistent mass with density rho[i] (host [n_inst]; mass_rule 0 the 5-point
+ * rule of mass_matrix_qp_kernel, 1 exactly integrated), f_int the SVK force of each
+ * instance's material from the rest positions (as tlfea_batch_internal_force), u_n, v_n,
+ * f_ext device AoS [3 n_nodes] (the displacement and velocity at the step start, the
+ * load).  Fixed DOFs are eliminated (v = 0 there) instead of being constrained by the
+ * augmented-Lagrangian outer loop.  v (device, [3 n_nodes]) starts from itself
+ * (use_guess != 0) or from v_n and returns each instance's last iterate.  Stopping tests:
+ * the reference's, | |g| - |g_prev| | < inner_tol or | |v_next| - |v_k| | < inner_tol after
+ * the first iteration (inner_tol > 0), and the relative |g| <= rel_tol |g_0| (rel_tol > 0;
+ * g_0 the gradient at the first iterate); restart != 0 restarts the momentum when
+ * g.(v_next - v_k) > 0.  Iterations are replayed from a captured hipGraph of `poll` of them
+ * (use_graph != 0), finished instances drop out as in tlfea_batch_cg_solve.  Safeguard
+ * (not in the reference): an instance whose |g| turns non-finite or exceeds 1e4 |g_0| halves
+ * its step size and starts again from v_n (at most 40 times).  Per instance (host arrays,
+ * may be NULL): iterations, |g| / |g_0|, restarts, step halvings, status (TLFEA_ENOCONV
+ * when max_inner or the halvings ran out).  Standard quadrature rules only. */
+int tlfea_batch_nesterov_solve(tlfea_batch* b, double dt, const double* rho, int mass_rule, const double* alpha,
+                               const double* u_n, const double* v_n, const double* f_ext, double* v, int use_guess,
+                               double inner_tol, double rel_tol, int max_inner, int restart, int poll, int use_graph,
+                               int* iters, double* rel, int* restarts, int* halvings, int* status, int* total_iters,
+                               void* stream);
 
 /* ---- learned warm start (not in the reference) -------------------------------------- */
 /* A message-passing network on the mesh node graph (CSR off/col of tlfea_pattern_node_csr,

@@ -289,3 +289,165 @@ def test_preconditioned_steps_match_plain(gpu, nonlinear):
     print(f"[precond dynamics nonlinear={nonlinear}] CG iterations per instance-step: plain median "
           f"{np.median(ip):.0f}, preconditioned median {np.median(iq):.0f}", flush=True)
     assert np.median(iq) * 3 <= np.median(ip)
+
+
+def _step_problem(m, mat, dt, amp=300.0, g=(0.0, 0.0, -9.81)):
+    """One implicit step from rest: exact mass M, the load M g + amp * tip pattern, and the
+    free-DOF mask."""
+    K, M = _dense_KM(m, mat, "exact")
+    n3 = 3 * m.n_nodes
+    free = np.ones(n3, bool)
+    for nd in np.asarray(m.fixed):
+        free[3 * nd:3 * nd + 3] = False
+    F = np.asarray(m.f_ext).reshape(-1, 3).sum(0)
+    f = M @ np.tile(np.asarray(g), m.n_nodes) + np.asarray(m.f_ext) / np.linalg.norm(F) * amp
+    f[~free] = 0.0
+    return K, M, free, f
+
+
+@pytest.mark.parametrize("restart", [False, True])
+def test_batch_nesterov_iterates_match_restatement(gpu, restart):
+    """The batched SyncedNesterov inner loop (tlfea_batch_nesterov_solve) against a numpy
+    restatement of SyncedNesterov.cu:97-360 with the fixed DOFs eliminated, driven by the
+    oracle's internal force (longdouble) and dense exact mass: the iterate after 3 and after
+    40 iterations agrees to 1e-10, and so does the restart count."""
+    import torch
+    from helpers import fint_extended
+    from tlfea_amd.batch import Batch
+    m, mat = _beam(4, 1, 1, [0.3, -0.5, -1.0], E=2e7, rho=1500.0)
+    dt = 1e-4
+    lam, mu = lame(mat["E"], mat["nu"])
+    K, M, free, f = _step_problem(m, mat, dt)
+    lmax = np.linalg.eigvalsh((K + M / dt ** 2)[np.ix_(free, free)]).max()
+    alpha = 1.0 / (1.1 * dt * lmax)
+    X = m.xyz()
+    n3 = 3 * m.n_nodes
+    b = Batch([(m, mat)])
+    zero = torch.zeros(n3, dtype=torch.float64, device="cuda")
+    fd = torch.as_tensor(f).cuda()
+    for K_it in (3, 40):
+        # restatement
+        vn = np.zeros(n3)
+        vk, vkm1 = vn.copy(), vn.copy()
+        t, t1 = 1.0, 0.5 * (1 + np.sqrt(5.0))
+        rs = 0
+        for _ in range(K_it):
+            beta = (t - 1.0) / t1
+            y = vk + beta * (vk - vkm1)
+            y[~free] = 0.0
+            vkm1 = vk
+            u = (dt * y).reshape(-1, 3)
+            fi = fint_extended(m, X[:, 0] + u[:, 0], X[:, 1] + u[:, 1], X[:, 2] + u[:, 2], lam, mu,
+                               u=u).astype(np.float64)
+            gr = M @ (y - vn) / dt + fi - f
+            gr[~free] = 0.0
+            vnext = y - alpha * gr
+            vnext[~free] = 0.0
+            if restart and gr @ (vnext - vkm1) > 0:
+                t = 1.0
+                rs += 1
+            else:
+                t = t1
+            t1 = 0.5 * (1 + np.sqrt(1 + 4 * t * t))
+            vk = vnext
+        v = b.nesterov_solve(dt, mat["rho"], alpha, zero, zero, fd, rel_tol=1e-300, max_inner=K_it,
+                             restart=restart, graph=False)
+        assert int(b.iters[0]) == K_it
+        assert rel_err(_np(v), vk) < 1e-10, K_it
+        assert int(b.restarts[0]) == rs, (K_it, int(b.restarts[0]), rs)
+    b.close()
+
+
+def test_batch_nesterov_step_against_dense_newton(gpu):
+    """Three beams of different sizes and materials in one batch, one implicit step each by
+    the batched Nesterov solve to ||g|| <= 1e-11 ||g(v_n)||: every instance's velocity equals
+    the dense Newton solution of the same nonlinear step (longdouble f_int) to 1e-7, and the
+    same instance solved alone (a batch of one) to 1e-8; instances stop on their own."""
+    import torch
+    from tlfea_amd.batch import Batch
+    specs = [(_beam(4, 1, 1, [0.3, -0.5, -1.0], E=2e7, rho=1500.0), 300.0),
+             (_beam(6, 2, 1, [0.0, 1.0, -1.0], E=5e8, rho=4000.0), 900.0),
+             (_beam(3, 1, 2, [1.0, 0.0, -0.2], E=3e6, nu=0.4, rho=1100.0), 40.0)]
+    dt = 1e-4
+    items = [s for s, _ in specs]
+    b = Batch(items)
+    rho = np.array([mat["rho"] for _, mat in items])
+    b.set_mass(rho / dt ** 2, rule="exact")
+    alpha = 1.0 / (1.1 * dt * b.lambda_max())
+    fs, refs = [], []
+    for (m, mat), amp in specs:
+        K, M, free, f = _step_problem(m, mat, dt, amp=amp)
+        fs.append(f)
+        # the dense Newton reference of one step with the same load
+        # (a frequency of 1/(4 dt) puts the sine at 1 at t = dt)
+        refs.append(_newton_steps_cpu(m, mat, amp, 0.25 / dt, dt, 1, tol=1e-14)[0] / dt)
+    f = torch.as_tensor(np.concatenate(fs)).cuda()
+    zero = torch.zeros(3 * b.n_nodes, dtype=torch.float64, device="cuda")
+    v = _np(b.nesterov_solve(dt, rho, alpha, zero, zero, f, rel_tol=1e-11, restart=True))
+    assert b.rc == 0 and (b.rel <= 1e-11).all()
+    its = b.iters.copy()
+    print(f"[batched nesterov] iterations per instance {its.tolist()}, restarts {b.restarts.tolist()}", flush=True)
+    assert len(set(its.tolist())) > 1
+    for i, ((m, mat), amp) in enumerate(specs):
+        sl = b.inst_slice(i)
+        free = _np(b.mask[sl]) == 0
+        assert rel_err(v[sl][free], refs[i][free]) < 1e-7, i
+        one = Batch([(m, mat)])
+        zi = torch.zeros(3 * m.n_nodes, dtype=torch.float64, device="cuda")
+        vi = _np(one.nesterov_solve(dt, mat["rho"], alpha[i], zi, zi, torch.as_tensor(fs[i]).cuda(), rel_tol=1e-11))
+        assert rel_err(v[sl], vi) < 1e-8, i
+        one.close()
+    b.close()
+
+
+def test_nesterov_steps_match_newton_steps(gpu):
+    """Time steps of the dynamic batch by batched Nesterov solves land on the Newton
+    trajectory of the same nonlinear implicit-Euler equations (1e-7 over 8 steps, both to
+    1e-11)."""
+    from tlfea_amd import mesh as M
+    from tlfea_amd.dynamics import BatchDynamics, dynamic_beams
+    items, amp, freq = dynamic_beams(3, 77)
+    rng = np.random.default_rng(2)
+    small = []
+    for m, mat in items:
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        small.append((M.clamp_and_tip_load(M.kuhn_beam(int(rng.integers(4, 10)), 2, 1, 1.0, 0.1, 0.1), d), mat))
+    kw = dict(tol_nl=1e-11, newton_cg_tol=1e-13)
+    newton = BatchDynamics(small, amp, freq, nonlinear=True, **kw)
+    nes = BatchDynamics(small, amp, freq, nesterov=True, **kw)
+    for k in range(8):
+        u0, v0 = newton.step()
+        u1, v1 = nes.step()
+        for i in range(len(small)):
+            sl = newton.b.inst_slice(i)
+            assert rel_err(_np(u1[sl]), _np(u0[sl])) < 1e-7, (k, i)
+            assert rel_err(_np(v1[sl]), _np(v0[sl])) < 1e-7, (k, i)
+    its = np.array(nes.iters)
+    print(f"[nesterov dynamics] inner iterations per instance-step: median {np.median(its):.0f}, max {its.max()}",
+          flush=True)
+
+
+def test_batch_nesterov_step_halving_safeguard(gpu):
+    """A step size 8x beyond 1/lambda_max diverges; the batched loop halves it per instance
+    and starts again from v_n until it converges, to the same velocity as a run with a
+    stable step size (1e-8), while the instance with a stable step is unaffected."""
+    import torch
+    from tlfea_amd.batch import Batch
+    specs = [_beam(4, 1, 1, [0.3, -0.5, -1.0], E=2e7, rho=1500.0), _beam(5, 1, 1, [0.0, 1.0, -1.0], E=8e7, rho=2500.0)]
+    dt = 1e-4
+    b = Batch(specs)
+    rho = np.array([mat["rho"] for _, mat in specs])
+    b.set_mass(rho / dt ** 2, rule="exact")
+    good = 1.0 / (1.1 * dt * b.lambda_max())
+    zero = torch.zeros(3 * b.n_nodes, dtype=torch.float64, device="cuda")
+    f = b.f * 300.0
+    ref = _np(b.nesterov_solve(dt, rho, good, zero, zero, f, rel_tol=1e-11))
+    assert b.rc == 0 and (b.halvings == 0).all()
+    bad = good * np.array([8.0, 1.0])
+    v = _np(b.nesterov_solve(dt, rho, bad, zero, zero, f, rel_tol=1e-11))
+    assert b.rc == 0, b.status
+    assert b.halvings[0] >= 3 and b.halvings[1] == 0, b.halvings
+    for i in range(2):
+        sl = b.inst_slice(i)
+        assert rel_err(v[sl], ref[sl]) < 1e-8, i

@@ -203,6 +203,60 @@ class Batch:
         self.rc = rc
         return u
 
+    def dof_inst(self):
+        """Instance index of every DOF (device int64 [3 n_nodes])."""
+        if getattr(self, "_dof_inst", None) is None:
+            counts = 3 * np.diff(self.node_off)
+            self._dof_inst = torch.as_tensor(np.repeat(np.arange(self.n_inst), counts)).to(DEV)
+        return self._dof_inst
+
+    def lambda_max(self, iters=60, mass=True, seed=0):
+        """Per-instance largest eigenvalue of the batch's operator on the free DOFs (K, plus
+        c M when mass and a mass term is set), by FP64 power iteration from a positive start;
+        the Rayleigh quotient returned approaches it from below."""
+        inst = self.dof_inst()
+        fixed = self.mask.bool()
+        gen = torch.Generator().manual_seed(seed)
+        x = (torch.rand(3 * self.n_nodes, generator=gen, dtype=torch.float64) + 0.5).to(DEV)
+        x[fixed] = 0.0
+        mass = bool(mass and getattr(self, "mass_on", False))
+        for _ in range(iters):
+            y = self.apply(x, mass=mass)
+            y[fixed] = 0.0
+            x = y / torch.sqrt(self.dot(y)).clamp_min(1e-300)[inst]
+        y = self.apply(x, mass=mass)
+        y[fixed] = 0.0
+        return (self.dot(x, y) / self.dot(x).clamp_min(1e-300)).cpu().numpy()
+
+    def nesterov_solve(self, dt, rho, alpha, u_n, v_n, f_ext, v=None, mass_rule="exact", inner_tol=0.0,
+                       rel_tol=1e-8, max_inner=200000, restart=True, poll=None, graph=True):
+        """One implicit step per instance by the batched SyncedNesterov inner loop
+        (tlfea_batch_nesterov_solve): v minimising M (v - v_n)/dt + f_int(X + u_n + dt v) -
+        f_ext per instance, from v (when given, updated in place) or v_n.  rho, alpha: per
+        instance (density, step size).  Per-instance iterations, |g|/|g_0|, restarts and
+        status land in self.iters / self.rel / self.restarts / self.status, step-size halvings of
+        the divergence safeguard in self.halvings."""
+        rules = {"quadrature": 0, "exact": 1}
+        if mass_rule not in rules:
+            raise ValueError(f"unknown mass rule {mass_rule!r}")
+        use_guess = v is not None
+        out = v if use_guess else torch.empty(3 * self.n_nodes, dtype=torch.float64, device=DEV)
+        rho = np.ascontiguousarray(np.broadcast_to(np.asarray(rho, np.float64), (self.n_inst,)))
+        alpha = np.ascontiguousarray(np.broadcast_to(np.asarray(alpha, np.float64), (self.n_inst,)))
+        self.restarts = np.zeros(self.n_inst, np.int32)
+        self.halvings = np.zeros(self.n_inst, np.int32)
+        tot = C.c_int()
+        rc = capi.lib().tlfea_batch_nesterov_solve(
+            self.h, float(dt), rho.ctypes.data, rules[mass_rule], alpha.ctypes.data, ptr(u_n), ptr(v_n), ptr(f_ext),
+            ptr(out), int(use_guess), float(inner_tol), float(rel_tol), int(max_inner), int(bool(restart)),
+            int(poll or 0), int(bool(graph)), self.iters.ctypes.data, self.rel.ctypes.data, self.restarts.ctypes.data,
+            self.halvings.ctypes.data, self.status.ctypes.data, C.byref(tot), self.stream)
+        self.total_iters = tot.value
+        if rc not in (capi.TLFEA_OK, capi.TLFEA_ENOCONV):
+            capi.check(rc)
+        self.rc = rc
+        return out
+
     def close(self):
         if getattr(self, "h", None) is not None:
             capi.lib().tlfea_batch_destroy(self.h)

@@ -86,6 +86,14 @@ struct tlfea_batch {
     int* d_inst_cblk = nullptr;    // [n_inst + 1]: first coarse block of each instance
     double *d_cdiag = nullptr, *d_clow = nullptr, *d_cg = nullptr;  // [n_cblocks][NB][NB]
     double *d_crhs = nullptr, *d_cx = nullptr;                       // [6 n_planes]
+    // batched SyncedNesterov (tlfea_batch_nesterov_solve): v_(k-1), y, dv = y - v_n (AoS
+    // [3 n_nodes]), the displacement u_n + dt y (SoA), chunk partials [n_chunks][4],
+    // per-instance scalars [n_inst][NSN], mass coefficients rho_i / dt, restart counts
+    double *d_nvk = nullptr, *d_ny = nullptr, *d_ndv = nullptr;
+    double *d_nux = nullptr, *d_nuy = nullptr, *d_nuz = nullptr;
+    double *d_npart = nullptr, *d_ns = nullptr, *d_nmc = nullptr;
+    int* d_nrs = nullptr;
+    int n_npart = 0;
 };
 
 namespace tlfea_hip {
@@ -97,7 +105,8 @@ enum { S_FF = 0, S_RZ, S_RR, S_ALPHA, S_BETA, S_PQ, S_REL, S_RZ_OLD, S_G0, S_G1,
 // Chebyshev parameters per instance
 enum { C_LMAX = 0, C_THETA, C_DELTA, C_SIGMA, NC };
 // per-instance flags
-enum { I_DONE = 0, I_ITERS, I_NOTPD, I_ZERO_F, NI };
+// (I_RESET, I_HALVE: the batched Nesterov loop's restart-from-v_n request and step halvings)
+enum { I_DONE = 0, I_ITERS, I_NOTPD, I_ZERO_F, I_RESET, I_HALVE, NI };
 
 __device__ __forceinline__ double chunk_sum(double v) {
     __shared__ double red[BT / 64];
@@ -1031,7 +1040,8 @@ void free_batch(tlfea_batch* b) {
                     b->d_rold, b->d_dinv32, b->d_blk_list, b->d_chunk_list, b->d_inst_list, b->d_mc, b->d_mc32,
                     b->d_ux, b->d_uy, b->d_uz, b->d_node_plane, b->d_node_r, b->d_plane_off,
                     b->d_plane_nodes, b->d_plane_inst, b->d_plane_local, b->d_inst_cblk, b->d_cdiag,
-                    b->d_clow, b->d_cg, b->d_crhs, b->d_cx};
+                    b->d_clow, b->d_cg, b->d_crhs, b->d_cx, b->d_nvk, b->d_ny, b->d_ndv, b->d_nux,
+                    b->d_nuy, b->d_nuz, b->d_npart, b->d_ns, b->d_nmc, b->d_nrs};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (b->s_own) (void)hipStreamDestroy(b->s_own);
@@ -1948,6 +1958,340 @@ extern "C" int tlfea_batch_cg_solve(tlfea_batch* b, const double* f, double* u, 
             st = TLFEA_ENOTPD;
         else if (!I[(size_t)i * NI + I_DONE])
             st = TLFEA_ENOCONV;
+        if (status) status[i] = st;
+        bad += st != TLFEA_OK;
+    }
+    if (total_iters) *total_iters = it;
+    if (bad) {
+        set_error("%d of %d instances did not converge", bad, b->n_inst);
+        return TLFEA_ENOCONV;
+    }
+    return TLFEA_OK;
+}
+
+// ---- batched SyncedNesterov ---------------------------------------------------------------
+// One implicit step per instance, g(v) = M (v - v_n) / dt + f_int(X + u_n + dt v) - f_ext = 0
+// (SyncedNesterov.cu:32-95), minimised by the reference's inner Nesterov loop
+// (SyncedNesterov.cu:97-360) for every instance at once, each with its own step size,
+// momentum sequence and stopping test.  Fixed DOFs are eliminated (v = 0 there) instead of
+// carried by the augmented-Lagrangian outer loop: with no constraint left that loop ends
+// after its first pass.
+namespace tlfea_hip {
+
+// per-instance Nesterov scalars: t_k and t_(k+1) of the momentum sequence, |g| of the
+// previous iteration, |g| at the first iteration, |g| now, |g| / |g_0|, the step size
+enum { N_T = 0, N_T1, N_GPREV, N_G0, N_G, N_REL, N_ALPHA, NSN };
+
+__global__ void k_bnes_init(int n_inst, const double* __restrict__ alpha, double* __restrict__ ns,
+                            int* __restrict__ I, int* __restrict__ rs, int* __restrict__ all) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) all[0] = n_inst;
+    if (i >= n_inst) return;
+    double* s = ns + (int64_t)i * NSN;
+    s[N_T] = 1.0;
+    s[N_T1] = 0.5 * (1.0 + sqrt(5.0));
+    s[N_GPREV] = s[N_G0] = s[N_G] = s[N_REL] = 0.0;
+    s[N_ALPHA] = alpha[i];
+    int* fl = I + (int64_t)i * NI;
+    fl[I_DONE] = fl[I_ITERS] = fl[I_NOTPD] = fl[I_ZERO_F] = fl[I_RESET] = fl[I_HALVE] = 0;
+    rs[i] = 0;
+}
+
+// y = v_k + beta (v_k - v_(k-1)), v_(k-1) <- v_k, dv = y - v_n, u = u_n + dt y
+// (SyncedNesterov.cu:152-175); y = 0 on fixed DOFs.  beta = (t_k - 1) / t_(k+1).  An
+// instance whose step size was just halved starts again from v_k = v_(k-1) = v_n.
+__global__ __launch_bounds__(BT) void k_bnes_predict(Chunks ch, const int* __restrict__ I,
+                                                     const double* __restrict__ ns, double dt,
+                                                     const unsigned char* __restrict__ mask,
+                                                     const double* __restrict__ v, double* __restrict__ vkm1,
+                                                     double* __restrict__ y, const double* __restrict__ vn,
+                                                     const double* __restrict__ un, double* __restrict__ dv,
+                                                     double* __restrict__ ux, double* __restrict__ uy,
+                                                     double* __restrict__ uz, int* __restrict__ all) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) all[0] = 0;  // counted again by k_bnes_finalize
+    const int c = chunk_id(ch);
+    const int inst = ch.inst[c];
+    if (I[inst * NI + I_DONE]) return;
+    const double beta = (ns[inst * NSN + N_T] - 1.0) / ns[inst * NSN + N_T1];
+    const bool reset = I[inst * NI + I_RESET] != 0;
+    for (int n = ch.beg[c] + threadIdx.x; n < ch.end[c]; n += BT) {
+        double uo[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const int64_t k = 3 * (int64_t)n + d;
+            const double vk = reset ? vn[k] : v[k];
+            const double yv = mask[k] ? 0.0 : vk + beta * (vk - (reset ? vk : vkm1[k]));
+            y[k] = yv;
+            vkm1[k] = vk;
+            dv[k] = mask[k] ? 0.0 : yv - vn[k];
+            uo[d] = un[k] + dt * yv;
+        }
+        ux[n] = uo[0];
+        uy[n] = uo[1];
+        uz[n] = uo[2];
+    }
+}
+
+// g = (f_int + M dv / dt) - f_ext from the block partials (0 on fixed DOFs), v_next = y -
+// alpha g into v; chunk partials |g|^2, |v_next|^2, |v_k|^2, g.(v_next - v_k)
+// (SyncedNesterov.cu:32-95, 177-224).
+__global__ __launch_bounds__(BT) void k_bnes_grad(Chunks ch, const int* __restrict__ g_off,
+                                                  const int64_t* __restrict__ g_idx, const double* __restrict__ part,
+                                                  const unsigned char* __restrict__ mask,
+                                                  const double* __restrict__ fext, const double* __restrict__ y,
+                                                  const double* __restrict__ vk, double* __restrict__ v,
+                                                  const double* __restrict__ ns, const int* __restrict__ I,
+                                                  double* __restrict__ cpart) {
+    const int c = chunk_id(ch);
+    const int inst = ch.inst[c];
+    if (I[inst * NI + I_DONE]) return;
+    const double alpha = ns[inst * NSN + N_ALPHA];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int n = ch.beg[c] + threadIdx.x; n < ch.end[c]; n += BT) {
+        double s3[3] = {0.0, 0.0, 0.0};
+        gather_slots3(g_idx, part, g_off[n], g_off[n + 1], s3[0], s3[1], s3[2]);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const int64_t k = 3 * (int64_t)n + d;
+            const double g = mask[k] ? 0.0 : s3[d] - fext[k];
+            const double vnext = mask[k] ? 0.0 : y[k] - alpha * g;
+            const double vo = vk[k];
+            v[k] = vnext;
+            acc[0] += g * g;
+            acc[1] += vnext * vnext;
+            acc[2] += vo * vo;
+            acc[3] += g * (vnext - vo);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const double t = chunk_sum(acc[j]);
+        if (threadIdx.x == 0) cpart[4 * (int64_t)c + j] = t;
+    }
+}
+
+// Per instance: the stopping tests of SyncedNesterov.cu:226-271 (inner_tol > 0: | |g| -
+// |g_prev| | or | |v_next| - |v_k| | below it after the first iteration), the relative test
+// |g| <= rel_tol |g_0| (rel_tol > 0) and |g| = 0; then the momentum sequence advances, or,
+// with restart, starts again (t_(k+1) = 1: no momentum in the next step) when
+// g.(v_next - v_k) > 0 (O'Donoghue & Candes, Found. Comput. Math. 15 (2015) 715-732).
+// Safeguard (not in the reference, whose step size is fixed): a gradient that is not
+// finite or has grown past 1e4 |g_0| means the step size exceeds 2 / L along the path (the
+// tangent stiffens under large deflection); the instance then halves its step size and
+// starts again from v_n, at most 40 times before it is given up as not converged.
+// One wave per listed instance; all[0] (zeroed by the predict kernel) counts the
+// instances still running.
+__global__ __launch_bounds__(64) void k_bnes_finalize(Insts in, const int* __restrict__ ico,
+                                                      const double* __restrict__ cpart, double* __restrict__ ns,
+                                                      int* __restrict__ I, int* __restrict__ rs, int* __restrict__ all,
+                                                      double inner_tol, double rel_tol, int restart) {
+    const int i = inst_at(in, blockIdx.x);
+    int* fl = I + (int64_t)i * NI;
+    if (fl[I_DONE]) return;
+    // the chunk partials in a fixed order: lane l sums chunks l, l + 64, ..., then a
+    // butterfly over the lanes
+    double sm[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int c = ico[i] + (int)threadIdx.x; c < ico[i + 1]; c += 64)
+#pragma unroll
+        for (int k = 0; k < 4; k++) sm[k] += cpart[4 * (int64_t)c + k];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) sm[k] += __shfl_xor(sm[k], o, 64);
+    if (threadIdx.x == 0) {
+        double* s = ns + (int64_t)i * NSN;
+        const double ng = sqrt(sm[0]), nvn = sqrt(sm[1]), nvk = sqrt(sm[2]);
+        const int it = fl[I_ITERS];
+        fl[I_RESET] = 0;
+        if (it == 0) s[N_G0] = ng;
+        fl[I_ITERS] = it + 1;
+        if (!(ng <= 1e4 * s[N_G0])) {  // also catches a NaN or an infinite |g|
+            if (it == 0 || fl[I_HALVE] >= 40) {
+                fl[I_NOTPD] = 1;
+                fl[I_DONE] = 1;
+                s[N_REL] = ng / s[N_G0];
+                return;
+            }
+            s[N_ALPHA] *= 0.5;
+            fl[I_HALVE] += 1;
+            fl[I_RESET] = 1;
+            s[N_T] = 1.0;
+            s[N_T1] = 0.5 * (1.0 + sqrt(5.0));
+            s[N_GPREV] = s[N_G0];
+            atomicAdd(all, 1);
+            return;
+        }
+        bool stop = ng == 0.0;
+        if (it > 0 && inner_tol > 0.0 && (fabs(ng - s[N_GPREV]) < inner_tol || fabs(nvn - nvk) < inner_tol))
+            stop = true;
+        s[N_REL] = s[N_G0] > 0.0 ? ng / s[N_G0] : 0.0;
+        if (rel_tol > 0.0 && s[N_REL] <= rel_tol) stop = true;
+        s[N_GPREV] = ng;
+        s[N_G] = ng;
+        if (restart && sm[3] > 0.0) {
+            s[N_T] = 1.0;
+            rs[i] += 1;
+        } else {
+            s[N_T] = s[N_T1];
+        }
+        s[N_T1] = 0.5 * (1.0 + sqrt(1.0 + 4.0 * s[N_T] * s[N_T]));
+        if (stop)
+            fl[I_DONE] = 1;
+        else
+            atomicAdd(all, 1);
+    }
+}
+
+}  // namespace tlfea_hip
+
+namespace {
+int nesterov_iteration(tlfea_batch* b, double dt, int mass_exact, const double* u_n, const double* v_n,
+                       const double* f_ext, double* v, double inner_tol, double rel_tol, int restart, hipStream_t s) {
+    const Chunks ch = chunks(b);
+    const int nc = n_chunk_launch(b);
+    k_bnes_predict<<<nc, BT, 0, s>>>(ch, b->d_i, b->d_ns, dt, b->d_mask, v, b->d_nvk, b->d_ny, v_n, u_n, b->d_ndv,
+                                     b->d_nux, b->d_nuy, b->d_nuz, b->d_all);
+    TLFEA_LAUNCH_CHECK();
+    ForceMass fm;
+    fm.dv = b->d_ndv;
+    fm.inst_mc = b->d_nmc;
+    fm.mass_exact = mass_exact;
+    fm.inst_done = b->d_i + I_DONE;
+    fm.done_stride = NI;
+    fm.blk_list = b->compact ? b->d_blk_list : nullptr;
+    fm.n_list = b->n_blk_list;
+    if (int rc = launch_force_rest_mass_partials(b->plan, b->d_nux, b->d_nuy, b->d_nuz, b->d_X, b->d_Y, b->d_Z,
+                                                 b->quad, b->d_elem_inst, b->d_lam, b->d_mu, fm, s))
+        return rc;
+    k_bnes_grad<<<nc, BT, 0, s>>>(ch, b->plan->d_g_off, b->plan->d_g_idx, b->plan->d_part, b->d_mask, f_ext,
+                                  b->d_ny, b->d_nvk, v, b->d_ns, b->d_i, b->d_npart);
+    const Insts in = insts(b);
+    k_bnes_finalize<<<in.n, 64, 0, s>>>(in, b->d_inst_chunk_off, b->d_npart, b->d_ns, b->d_i, b->d_nrs, b->d_all,
+                                        inner_tol, rel_tol, restart);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+}  // namespace
+
+extern "C" int tlfea_batch_nesterov_solve(tlfea_batch* b, double dt, const double* rho, int mass_rule,
+                                          const double* alpha, const double* u_n, const double* v_n,
+                                          const double* f_ext, double* v, int use_guess, double inner_tol,
+                                          double rel_tol, int max_inner, int restart, int poll, int use_graph,
+                                          int* iters, double* rel, int* restarts, int* halvings, int* status,
+                                          int* total_iters, void* stream) {
+    TLFEA_REQUIRE(b && rho && alpha && u_n && v_n && f_ext && v, "null argument");
+    TLFEA_REQUIRE(dt > 0.0, "time step must be positive");
+    TLFEA_REQUIRE(mass_rule == 0 || mass_rule == 1, "mass rule must be 0 (quadrature) or 1 (exact)");
+    TLFEA_REQUIRE(max_inner >= 0 && inner_tol >= 0.0 && rel_tol >= 0.0, "bad tolerance or iteration limit");
+    TLFEA_REQUIRE(inner_tol > 0.0 || rel_tol > 0.0, "no stopping test: set inner_tol or rel_tol");
+    TLFEA_REQUIRE(is_standard_rule(b->nn, b->quad), "the batched Nesterov force needs the standard quadrature rule");
+    if ((size_t)b->plan->max_u * 12 * sizeof(double) > 160 * 1024) {
+        set_error("a block touches %d nodes: beyond the LDS budget of the force with a mass term", b->plan->max_u);
+        return TLFEA_ECAPACITY;
+    }
+    for (int i = 0; i < b->n_inst; i++)
+        TLFEA_REQUIRE(alpha[i] > 0.0 && rho[i] > 0.0, "instance %d: step size and density must be positive", i);
+    if (poll <= 0) poll = b->poll_default;
+    hipStream_t caller = as_stream(stream);
+    hipStream_t s = b->s_own;
+    TLFEA_HIP_TRY(hipStreamSynchronize(caller));
+    const int N = b->n_nodes;
+    const size_t nd = 3 * (size_t)N;
+    int rc;
+    if (!b->d_nvk && ((rc = dalloc(&b->d_nvk, nd)) || (rc = dalloc(&b->d_ny, nd)) || (rc = dalloc(&b->d_ndv, nd)) ||
+                      (rc = dalloc(&b->d_nux, N)) || (rc = dalloc(&b->d_nuy, N)) || (rc = dalloc(&b->d_nuz, N)) ||
+                      (rc = dalloc(&b->d_ns, (size_t)b->n_inst * NSN)) || (rc = dalloc(&b->d_nmc, b->n_inst)) ||
+                      (rc = dalloc(&b->d_nrs, b->n_inst))))
+        return rc;
+    if (b->n_npart < b->n_chunks) {
+        if (b->d_npart) (void)hipFree(b->d_npart);
+        b->d_npart = nullptr;
+        b->n_npart = 0;
+        if ((rc = dalloc(&b->d_npart, 4 * (size_t)b->n_chunks))) return rc;
+        b->n_npart = b->n_chunks;
+    }
+    std::vector<double> mc(b->n_inst);
+    for (int i = 0; i < b->n_inst; i++) mc[i] = rho[i] / dt;
+    std::vector<double> al(alpha, alpha + b->n_inst);
+    double* d_al = nullptr;
+    if ((rc = dalloc(&d_al, b->n_inst))) return rc;
+    std::unique_ptr<double, void (*)(double*)> al_guard(d_al, [](double* p) { (void)hipFree(p); });
+    TLFEA_HIP_TRY(hipMemcpyAsync(d_al, al.data(), sizeof(double) * b->n_inst, hipMemcpyHostToDevice, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(b->d_nmc, mc.data(), sizeof(double) * b->n_inst, hipMemcpyHostToDevice, s));
+    drop_graph(b);
+    full_lists(b);
+    // v_k = v_(k-1) = v_guess (SyncedNesterov.cu:129-134): the caller's v, or v_n
+    if (!use_guess) TLFEA_HIP_TRY(hipMemcpyAsync(v, v_n, sizeof(double) * nd, hipMemcpyDeviceToDevice, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(b->d_nvk, v, sizeof(double) * nd, hipMemcpyDeviceToDevice, s));
+    k_bnes_init<<<grid_for(b->n_inst, 256), 256, 0, s>>>(b->n_inst, d_al, b->d_ns, b->d_i, b->d_nrs, b->d_all);
+    TLFEA_LAUNCH_CHECK();
+    int active = b->n_inst, it = 0;
+    std::vector<int> hflags((size_t)b->n_inst * NI), done(b->n_inst, 0);
+    rc = TLFEA_OK;
+    while (rc == TLFEA_OK && active > 0 && it < max_inner) {
+        const int n = std::min(poll, max_inner - it);
+        if (use_graph && n == poll) {
+            if (!b->graph) {
+                hipGraph_t g;
+                TLFEA_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                for (int k = 0; k < poll && rc == TLFEA_OK; k++)
+                    rc = nesterov_iteration(b, dt, mass_rule, u_n, v_n, f_ext, v, inner_tol, rel_tol, restart, s);
+                const hipError_t ec = hipStreamEndCapture(s, &g);
+                if (rc) break;
+                if (ec != hipSuccess) {
+                    set_error("graph capture failed: %s", hipGetErrorString(ec));
+                    rc = TLFEA_EHIP;
+                    break;
+                }
+                const hipError_t ei = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (ei != hipSuccess) {
+                    set_error("graph instantiation failed: %s", hipGetErrorString(ei));
+                    b->graph = nullptr;
+                    rc = TLFEA_EHIP;
+                    break;
+                }
+                b->graph_iters = -poll;  // not a CG graph
+                b->graph_precond = -1;
+            }
+            TLFEA_HIP_TRY(hipGraphLaunch(b->graph, s));
+        } else {
+            for (int k = 0; k < n && rc == TLFEA_OK; k++)
+                rc = nesterov_iteration(b, dt, mass_rule, u_n, v_n, f_ext, v, inner_tol, rel_tol, restart, s);
+            if (rc) break;
+        }
+        it += n;
+        TLFEA_HIP_TRY(hipMemcpyAsync(&active, b->d_all, sizeof(int), hipMemcpyDeviceToHost, s));
+        TLFEA_HIP_TRY(hipStreamSynchronize(s));
+        if (active > 0 && b->compact_ratio > 0.0) {
+            TLFEA_HIP_TRY(hipMemcpy(hflags.data(), b->d_i, sizeof(int) * hflags.size(), hipMemcpyDeviceToHost));
+            int64_t act_elems = 0;
+            for (int i = 0; i < b->n_inst; i++) {
+                done[i] = hflags[(size_t)i * NI + I_DONE];
+                if (!done[i]) act_elems += b->h_elem_off[i + 1] - b->h_elem_off[i];
+            }
+            if ((double)act_elems < b->compact_ratio * (double)b->launched_elems)
+                if ((rc = compact_lists(b, done))) break;
+        }
+    }
+    // the graph bakes in the caller's buffers
+    drop_graph(b);
+    full_lists(b);
+    if (rc) return rc;
+    std::vector<double> S((size_t)b->n_inst * NSN);
+    std::vector<int> I((size_t)b->n_inst * NI), R(b->n_inst);
+    TLFEA_HIP_TRY(hipMemcpyAsync(S.data(), b->d_ns, sizeof(double) * S.size(), hipMemcpyDeviceToHost, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(I.data(), b->d_i, sizeof(int) * I.size(), hipMemcpyDeviceToHost, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(R.data(), b->d_nrs, sizeof(int) * R.size(), hipMemcpyDeviceToHost, s));
+    TLFEA_HIP_TRY(hipStreamSynchronize(s));
+    int bad = 0;
+    for (int i = 0; i < b->n_inst; i++) {
+        if (iters) iters[i] = I[(size_t)i * NI + I_ITERS];
+        if (rel) rel[i] = S[(size_t)i * NSN + N_REL];
+        if (restarts) restarts[i] = R[i];
+        if (halvings) halvings[i] = I[(size_t)i * NI + I_HALVE];
+        // done and not given up by the safeguard
+        const int st = I[(size_t)i * NI + I_DONE] && !I[(size_t)i * NI + I_NOTPD] ? TLFEA_OK : TLFEA_ENOCONV;
         if (status) status[i] = st;
         bad += st != TLFEA_OK;
     }

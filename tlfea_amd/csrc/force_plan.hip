@@ -125,27 +125,34 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_force_planned(
 // of FEAT10Data.cu:92-198: per element only 10 u16 slots are read from HBM (the gradient
 // stream of k_force_planned is 1200 B per TET10 element), and E comes from the
 // displacement gradient, without the cancellation of F^T F - I at small strains.
-template <int NN, int NQ, bool DAMP, bool MR>
+// MASS (batched implicit steps, ForceMass): 1 adds c_i M dv with the quadrature rule's
+// mass (mass_matrix_qp_kernel, FEAT10Data.cu:200-271), 2 with the exactly integrated one,
+// dv staged in the velocity arrays.
+template <int NN, int NQ, bool DAMP, bool MR, int MASS = 0>
 __global__ __launch_bounds__(PLAN_BLOCK) void k_force_rest(
     int n_elem, const int64_t* __restrict__ blk_off, const int* __restrict__ unodes,
     const unsigned short* __restrict__ slot_t, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, const double* __restrict__ X0, const double* __restrict__ Y0,
     const double* __restrict__ Z0, const double* __restrict__ v, tlfea_quad quad, tlfea_material mat,
     const int* __restrict__ elem_inst, const double* __restrict__ inst_lam, const double* __restrict__ inst_mu,
-    double* __restrict__ part, int max_u, const int* __restrict__ flags, int x_is_disp) {
+    double* __restrict__ part, int max_u, const int* __restrict__ flags, int x_is_disp, ForceMass fm) {
     if (flags && (flags[0] | flags[1])) return;
+    static_assert(MASS == 0 || !DAMP, "the mass term shares the velocity arrays");
     extern __shared__ double lds[];
     double* sR = lds;              // [3][max_u] rest positions
     double* sU = lds + 3 * max_u;  // [3][max_u] displacements
     double* sF = lds + 6 * max_u;  // [3][max_u] accumulated forces
-    double* sV = lds + 9 * max_u;  // [3][max_u] velocities (DAMP only)
-    const int b = (int)xcd_block(blockIdx.x, gridDim.x);
+    double* sV = lds + 9 * max_u;  // [3][max_u] velocities (DAMP) or dv (MASS)
+    const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+    const int b = fm.blk_list ? fm.blk_list[lb] : (int)lb;
     const int64_t u0 = blk_off[b];
     const int U = (int)(blk_off[b + 1] - u0);
     const int e = b * PLAN_BLOCK + threadIdx.x;
     const int tile = e >> 6, lane = e & (TILE - 1);
+    bool active = e < n_elem;
+    if (active && fm.inst_done && fm.inst_done[elem_inst[e] * fm.done_stride]) active = false;
     int sl[NN];
-    if (e < n_elem) {
+    if (active) {
 #pragma unroll
         for (int a = 0; a < NN; a++) sl[a] = slot_t[((int64_t)tile * NN + a) * TILE + lane];
     }
@@ -165,12 +172,19 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_force_rest(
             sV[max_u + u] = v[3 * n + 1];
             sV[2 * max_u + u] = v[3 * n + 2];
         }
+        if constexpr (MASS != 0) {
+            sV[u] = fm.dv[3 * n];
+            sV[max_u + u] = fm.dv[3 * n + 1];
+            sV[2 * max_u + u] = fm.dv[3 * n + 2];
+        }
     }
     __syncthreads();
-    if (e < n_elem) {
+    if (active) {
         // one material, or per instance of a batch
         const double lam = elem_inst ? inst_lam[elem_inst[e]] : mat.lambda;
         const double mu = elem_inst ? inst_mu[elem_inst[e]] : mat.mu;
+        double mc = 0.0, vol = 0.0;
+        if constexpr (MASS != 0) mc = fm.inst_mc[elem_inst[e]];
         double fa[NN][3];
 #pragma unroll
         for (int a = 0; a < NN; a++) fa[a][0] = fa[a][1] = fa[a][2] = 0.0;
@@ -266,6 +280,23 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_force_rest(
             }
             // Q = dV P J^-T, f_a += Q dN_a/dxi
             const double dV = det * quad.w[q];
+            if constexpr (MASS == 2) vol += dV;
+            if constexpr (MASS == 1) {
+                // c dV N_a (sum_b N_b dv_b) at this point
+                double Nv[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+                for (int a = 0; a < NN; a++) {
+                    const double Na = std_N<NN>(q, a);
+#pragma unroll
+                    for (int i = 0; i < 3; i++) Nv[i] += Na * sV[i * max_u + sl[a]];
+                }
+#pragma unroll
+                for (int a = 0; a < NN; a++) {
+                    const double w = mc * dV * std_N<NN>(q, a);
+#pragma unroll
+                    for (int i = 0; i < 3; i++) fa[a][i] += w * Nv[i];
+                }
+            }
             double Q[3][3];
 #pragma unroll
             for (int i = 0; i < 3; i++)
@@ -281,6 +312,23 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_force_rest(
 #pragma unroll
                     for (int i = 0; i < 3; i++) fa[a][i] += Q[i][k] * c;
                 }
+        }
+        if constexpr (MASS == 2) {
+            // c V sum_b C_ab dv_b, C the exact shape-function products over V
+            const double w = mc * vol;
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                double db[NN];
+#pragma unroll
+                for (int c = 0; c < NN; c++) db[c] = sV[i * max_u + sl[c]] * w;
+#pragma unroll
+                for (int a = 0; a < NN; a++) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int c = 0; c < NN; c++) acc += exact_mass<NN>(a, c) * db[c];
+                    fa[a][i] += acc;
+                }
+            }
         }
 #pragma unroll
         for (int a = 0; a < NN; a++)
@@ -378,7 +426,7 @@ int launch_force_rest(const tlfea_force_plan* p, const double* x, const double* 
 #define TLFEA_FORCE_REST(NN_, NQ_, D_, MR_)                                                                      \
     k_force_rest<NN_, NQ_, D_, MR_><<<nb, PLAN_BLOCK, lds, s>>>(                                                 \
         p->n_elem, p->d_blk_off, p->d_unodes, p->d_slot, x, y, z, X0, Y0, Z0, D_ ? v : nullptr, quad, mat,       \
-        elem_inst, inst_lam, inst_mu, p->d_part, p->max_u, flags, x_is_disp ? 1 : 0)
+        elem_inst, inst_lam, inst_mu, p->d_part, p->max_u, flags, x_is_disp ? 1 : 0, ForceMass{})
 #define TLFEA_FORCE_REST_BY(NN_, NQ_)                 \
     do {                                              \
         if (damp && mr)                               \
@@ -404,6 +452,37 @@ int launch_force_rest(const tlfea_force_plan* p, const double* x, const double* 
     return TLFEA_OK;
 }
 
+int launch_force_rest_mass_partials(const tlfea_force_plan* p, const double* ux, const double* uy, const double* uz,
+                                    const double* X0, const double* Y0, const double* Z0, const tlfea_quad& quad,
+                                    const int* elem_inst, const double* inst_lam, const double* inst_mu,
+                                    const ForceMass& fm, hipStream_t s) {
+    if (!elem_inst || !inst_lam || !inst_mu || !fm.dv || !fm.inst_mc) {
+        set_error("the force with a mass term needs per-instance materials, dv and mass coefficients");
+        return TLFEA_EINVAL;
+    }
+    const unsigned nb = fm.blk_list ? (unsigned)fm.n_list : (unsigned)p->n_blocks;
+    if (nb == 0) return TLFEA_OK;
+    const size_t lds = sizeof(double) * (size_t)p->max_u * 12;
+    const tlfea_material none{0.0, 0.0, 0.0, 0.0};
+#define TLFEA_FORCE_MASS(NN_, NQ_, M_)                                                                         \
+    k_force_rest<NN_, NQ_, false, false, M_><<<nb, PLAN_BLOCK, lds, s>>>(                                      \
+        p->n_elem, p->d_blk_off, p->d_unodes, p->d_slot, ux, uy, uz, X0, Y0, Z0, nullptr, quad, none, elem_inst, \
+        inst_lam, inst_mu, p->d_part, p->max_u, nullptr, 1, fm)
+    if (p->nn == 10) {
+        if (fm.mass_exact)
+            TLFEA_FORCE_MASS(10, 5, 2);
+        else
+            TLFEA_FORCE_MASS(10, 5, 1);
+    } else {
+        if (fm.mass_exact)
+            TLFEA_FORCE_MASS(4, 1, 2);
+        else
+            TLFEA_FORCE_MASS(4, 1, 1);
+    }
+#undef TLFEA_FORCE_MASS
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
 
 __global__ void k_fixed_mask(int n_fixed, const int* __restrict__ fixed, unsigned char* __restrict__ mask) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;

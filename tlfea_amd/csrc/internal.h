@@ -55,6 +55,25 @@ int launch_force_rest(const tlfea_force_plan* p, const double* x, const double* 
                       const tlfea_material& mat, const double* v, double* f, int accumulate, const int* flags,
                       hipStream_t s, const int* elem_inst = nullptr, const double* inst_lam = nullptr,
                       const double* inst_mu = nullptr, bool x_is_disp = false);
+// Options of the rest-coordinate force for batched implicit steps: a consistent-mass term
+// c_i M dv added to the same block partials (c_i = inst_mc[instance], M the exactly
+// integrated mass when mass_exact, else the quadrature rule's), the elements of instances
+// whose inst_done[inst * done_stride] is set skipped, and only the blocks of blk_list (n_list
+// of them) launched.  SVK without damping only.
+struct ForceMass {
+    const double* dv = nullptr;
+    const double* inst_mc = nullptr;
+    int mass_exact = 0;
+    const int* inst_done = nullptr;
+    int done_stride = 0;
+    const int* blk_list = nullptr;
+    int n_list = 0;
+};
+// f_int(X + u) + c M dv as block partials in plan->d_part (no gather); u = (ux, uy, uz).
+int launch_force_rest_mass_partials(const tlfea_force_plan* p, const double* ux, const double* uy, const double* uz,
+                                    const double* X0, const double* Y0, const double* Z0, const tlfea_quad& quad,
+                                    const int* elem_inst, const double* inst_lam, const double* inst_mu,
+                                    const ForceMass& fm, hipStream_t s);
 
 // dN/dxi of every node at every quadrature point (a kernel argument).
 template <typename T>

@@ -28,6 +28,11 @@ instance whose ratio has stopped halving below 1e-6 sits at the FP64 floor of ev
 g (rounding of f_int against the load increment); the step then ends and the instance is
 counted in ``floor`` instead of failing.
 
+With nesterov=True each step is instead one SyncedNesterov solve of the same equation for
+every instance at once (tlfea_batch_nesterov_solve: the reference's inner Nesterov loop
+with momentum restart, per-instance step size 1/(safety dt lambda_max(K + M/dt^2)) from a
+power iteration at rest, from v_n) until ||g||/||g(v_n)|| <= tol_nl.
+
 With cheb_degree > 0 every solve is the flexible FP64 CG preconditioned by the FP32
 Jacobi-Chebyshev stage of that degree on K + M/dt^2 (and, with coarse=True, the additive
 coarse correction P A_c^-1 P^T over rigid motions of node planes), both set up once at the
@@ -63,16 +68,22 @@ def dynamic_beams(n, seed):
 
 class BatchDynamics:
     def __init__(self, items, amp, freq, dt=1e-4, g=(0.0, 0.0, -9.81), tol=1e-8, batch=None, mass_rule="exact",
-                 nonlinear=False, tol_nl=1e-8, newton_cg_tol=1e-10, max_newton=8, cheb_degree=0, coarse=False):
+                 nonlinear=False, tol_nl=1e-8, newton_cg_tol=1e-10, max_newton=8, cheb_degree=0, coarse=False,
+                 nesterov=False, alpha_safety=1.1, max_inner=200000):
         self.b = batch if batch is not None else Batch(items)
         b = self.b
         self.precond = cheb_degree > 0
+        self.nesterov, self.max_inner, self.mass_rule = nesterov, max_inner, mass_rule
         self.dt, self.tol = dt, tol
         self.nonlinear, self.tol_nl, self.newton_cg_tol, self.max_newton = nonlinear, tol_nl, newton_cg_tol, max_newton
         b.set_displacement(None)
         self.c = 1.0 / dt ** 2
         rho = np.array([mat.get("rho", 1000.0) for _, mat in items])
+        self.rho = rho
         b.set_mass(rho * self.c, rule=mass_rule)
+        if nesterov:
+            # the step's Hessian is M/dt + dt K = dt (K + M/dt^2)
+            self.alpha = 1.0 / (alpha_safety * dt * b.lambda_max())
         if self.precond:
             b.precond_setup(cheb_degree)
             if coarse:
@@ -115,6 +126,8 @@ class BatchDynamics:
         return g
 
     def step(self):
+        if self.nesterov:
+            return self._step_nesterov()
         if self.nonlinear:
             return self._step_newton()
         b, dt = self.b, self.dt
@@ -127,6 +140,21 @@ class BatchDynamics:
                                f"converge")
         self.u = self.u + dt * v1
         self.v = v1
+        self.t = t1
+        self.iters.append(b.iters.copy())
+        return self.u, self.v
+
+    def _step_nesterov(self):
+        b, dt = self.b, self.dt
+        t1 = self.t + dt
+        f1 = self.load(t1)
+        v = b.nesterov_solve(dt, self.rho, self.alpha, self.u, self.v, f1, mass_rule=self.mass_rule,
+                             rel_tol=self.tol_nl, max_inner=self.max_inner)
+        if b.rc != 0:
+            raise RuntimeError(f"time step {len(self.iters) + 1}: {int((b.status != 0).sum())} instances did not "
+                               f"converge")
+        self.u = self.u + dt * v
+        self.v = v
         self.t = t1
         self.iters.append(b.iters.copy())
         return self.u, self.v
