@@ -207,112 +207,120 @@ __global__ void k_mass(int n_elem, const int* __restrict__ slot, const double* _
 }
 
 // ------------------------------------------------------------------------------------
-// Tangent stiffness.  A block of 64 lanes handles one element tile at one quadrature point
-// per pass: each lane computes F, F F^T, tr E and Fh_a = F g_a of its element into LDS,
-// then the 64 lanes sweep the element tile's (a,b) pairs, each producing the 3x3 block
+// Tangent stiffness.  A block of ST_THREADS lanes handles ST_ELEMS elements: first one lane
+// per (element, quadrature point) computes F, F F^T, tr E, g_a and Fh_a = F g_a into LDS
+// (all points of the block's elements at once), then each lane takes (element, a, b)
+// triples and sums over the quadrature points the 3x3 block
 //   K_ab = (lambda Fh_a (x) Fh_b + lambda trE (g_a.g_b) I + mu (Fh_b.Fh_a) I
 //           + mu Fh_b (x) Fh_a + mu (g_a.g_b) F F^T - mu (g_a.g_b) I) dV
 // (src/materials/SVK.cuh:41-67 at src/elements/FEAT10DataFunc.cuh:512-684) plus the
 // Kelvin-Voigt block
 //   C_ab = (eta Fh_b (x) Fh_a + eta (g_a.g_b) F F^T + lambda_d Fh_a (x) Fh_b) dV
-// (FEAT10DataFunc.cuh:686-767), scattered with atomics through the slot map.
+// (FEAT10DataFunc.cuh:686-767) before scattering it with 9 atomics through the slot map:
+// one scatter per element block instead of one per quadrature point.
 // ------------------------------------------------------------------------------------
+constexpr int ST_THREADS = 256;
+template <bool MR>
+constexpr int st_elems() { return MR ? 8 : 16; }  // LDS: 43.5 KB (SVK), 47.6 KB (Mooney-Rivlin)
+
 template <int NN, int NQ, bool MR>
-__global__ __launch_bounds__(64) void k_stiffness(int n_elem, const int* __restrict__ conn,
-                                                  const double* __restrict__ x, const double* __restrict__ y,
-                                                  const double* __restrict__ z, const double* __restrict__ grad_t,
-                                                  const double* __restrict__ detJ_t, tlfea_quad quad,
-                                                  tlfea_material mat, double h, const int* __restrict__ slot,
-                                                  double* __restrict__ k_sell) {
-    __shared__ double sg[NN][3][TILE];
-    __shared__ double sFh[NN][3][TILE];
-    __shared__ double sFFt[6][TILE];
-    __shared__ double strE[TILE], sdV[TILE];
-    // Mooney-Rivlin: dP/dF per element of the tile (MooneyRivlin.cuh:106-227), the block of
+__global__ __launch_bounds__(ST_THREADS) void k_stiffness(int n_elem, const int* __restrict__ conn,
+                                                          const double* __restrict__ x, const double* __restrict__ y,
+                                                          const double* __restrict__ z,
+                                                          const double* __restrict__ grad_t,
+                                                          const double* __restrict__ detJ_t, tlfea_quad quad,
+                                                          tlfea_material mat, double h, const int* __restrict__ slot,
+                                                          double* __restrict__ k_sell) {
+    constexpr int TE = st_elems<MR>();
+    __shared__ double sg[NQ][NN][3][TE];
+    __shared__ double sFh[NQ][NN][3][TE];
+    __shared__ double sFFt[NQ][6][TE];
+    __shared__ double strE[NQ][TE], sdV[NQ][TE];
+    // Mooney-Rivlin: dP/dF per (point, element) (MooneyRivlin.cuh:106-227), the block of
     // nodes a, b being K_ab[d][c] = sum_JL A[d][J][c][L] g_a[J] g_b[L] dV
-    __shared__ double sA[MR ? 81 : 1][TILE];
-    const int tile = blockIdx.x;
-    const int lane = threadIdx.x;
-    const int e = tile * TILE + lane;
-    const int n_valid = min(TILE, n_elem - tile * TILE);
+    __shared__ double sA[MR ? NQ : 1][MR ? 81 : 1][TE];
+    const int e0 = blockIdx.x * TE;
+    const int n_valid = min(TE, n_elem - e0);
     const bool damp = (mat.eta_damp != 0.0 || mat.lambda_damp != 0.0);
-    for (int q = 0; q < NQ; q++) {
-        if (lane < n_valid) {
-            const double* G = grad_t + ((int64_t)tile * NQ + q) * (NN * 3 * TILE) + lane;
-            double F[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-            double g[NN][3];
+    for (int pq = threadIdx.x; pq < n_valid * NQ; pq += ST_THREADS) {
+        const int el = pq % n_valid, q = pq / n_valid;
+        const int e = e0 + el;
+        const int tile = e >> 6, lane = e & (TILE - 1);
+        const double* G = grad_t + ((int64_t)tile * NQ + q) * (NN * 3 * TILE) + lane;
+        double F[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        double g[NN][3];
 #pragma unroll
-            for (int a = 0; a < NN; a++) {
-                const int n = conn[e + (int64_t)a * n_elem];
-                const double xa[3] = {x[n], y[n], z[n]};
+        for (int a = 0; a < NN; a++) {
+            const int n = conn[e + (int64_t)a * n_elem];
+            const double xa[3] = {x[n], y[n], z[n]};
 #pragma unroll
-                for (int d = 0; d < 3; d++) g[a][d] = G[(a + NN * d) * TILE];
-#pragma unroll
-                for (int i = 0; i < 3; i++)
-#pragma unroll
-                    for (int j = 0; j < 3; j++) F[i][j] += xa[i] * g[a][j];
-            }
-            double C00 = 0, C11 = 0, C22 = 0;
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                C00 += F[k][0] * F[k][0];
-                C11 += F[k][1] * F[k][1];
-                C22 += F[k][2] * F[k][2];
-            }
-            strE[lane] = 0.5 * (C00 + C11 + C22 - 3.0);
-            int idx = 0;
+            for (int d = 0; d < 3; d++) g[a][d] = G[(a + NN * d) * TILE];
 #pragma unroll
             for (int i = 0; i < 3; i++)
 #pragma unroll
-                for (int j = i; j < 3; j++) sFFt[idx++][lane] = F[i][0] * F[j][0] + F[i][1] * F[j][1] + F[i][2] * F[j][2];
-#pragma unroll
-            for (int a = 0; a < NN; a++)
-#pragma unroll
-                for (int r = 0; r < 3; r++) {
-                    sg[a][r][lane] = g[a][r];
-                    sFh[a][r][lane] = F[r][0] * g[a][0] + F[r][1] * g[a][1] + F[r][2] * g[a][2];
-                }
-            sdV[lane] = detJ_t[((int64_t)tile * NQ + q) * TILE + lane] * quad.w[q];
-            if constexpr (MR) {
-                MrKin k;
-                mr_kinematics(F, mat.mu10, mat.mu01, mat.kappa, k);
-                double FFt[3][3];
-#pragma unroll
-                for (int i = 0; i < 3; i++)
-#pragma unroll
-                    for (int j = 0; j < 3; j++) FFt[i][j] = F[i][0] * F[j][0] + F[i][1] * F[j][1] + F[i][2] * F[j][2];
-                // fully unrolled: run-time indices into k's arrays would put them in scratch
-#pragma unroll
-                for (int i = 0; i < 3; i++)
-#pragma unroll
-                    for (int j = 0; j < 3; j++)
-#pragma unroll
-                        for (int kk = 0; kk < 3; kk++)
-#pragma unroll
-                            for (int l = 0; l < 3; l++)
-                                sA[((i * 3 + j) * 3 + kk) * 3 + l][lane] =
-                                    mr_tangent_entry(F, FFt, k, mat.kappa, i, j, kk, l);
-            }
+                for (int j = 0; j < 3; j++) F[i][j] += xa[i] * g[a][j];
         }
-        __syncthreads();
-        // sweep (element, a, b) triples of the tile
-        const int total = n_valid * NN * NN;
-        for (int t = lane; t < total; t += TILE) {
-            const int el = t % n_valid;  // consecutive lanes -> different elements (LDS banks)
-            const int ab = t / n_valid;
-            const int a = ab / NN, b = ab % NN;
-            const double ga[3] = {sg[a][0][el], sg[a][1][el], sg[a][2][el]};
-            const double gb[3] = {sg[b][0][el], sg[b][1][el], sg[b][2][el]};
-            const double Fa[3] = {sFh[a][0][el], sFh[a][1][el], sFh[a][2][el]};
-            const double Fb[3] = {sFh[b][0][el], sFh[b][1][el], sFh[b][2][el]};
-            const double ff[3][3] = {{sFFt[0][el], sFFt[1][el], sFFt[2][el]},
-                                     {sFFt[1][el], sFFt[3][el], sFFt[4][el]},
-                                     {sFFt[2][el], sFFt[4][el], sFFt[5][el]}};
+        double C00 = 0, C11 = 0, C22 = 0;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            C00 += F[k][0] * F[k][0];
+            C11 += F[k][1] * F[k][1];
+            C22 += F[k][2] * F[k][2];
+        }
+        strE[q][el] = 0.5 * (C00 + C11 + C22 - 3.0);
+        int idx = 0;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = i; j < 3; j++) sFFt[q][idx++][el] = F[i][0] * F[j][0] + F[i][1] * F[j][1] + F[i][2] * F[j][2];
+#pragma unroll
+        for (int a = 0; a < NN; a++)
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                sg[q][a][r][el] = g[a][r];
+                sFh[q][a][r][el] = F[r][0] * g[a][0] + F[r][1] * g[a][1] + F[r][2] * g[a][2];
+            }
+        sdV[q][el] = detJ_t[((int64_t)tile * NQ + q) * TILE + lane] * quad.w[q];
+        if constexpr (MR) {
+            MrKin k;
+            mr_kinematics(F, mat.mu10, mat.mu01, mat.kappa, k);
+            double FFt[3][3];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) FFt[i][j] = F[i][0] * F[j][0] + F[i][1] * F[j][1] + F[i][2] * F[j][2];
+            // fully unrolled: run-time indices into k's arrays would put them in scratch
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+#pragma unroll
+                    for (int kk = 0; kk < 3; kk++)
+#pragma unroll
+                        for (int l = 0; l < 3; l++)
+                            sA[q][((i * 3 + j) * 3 + kk) * 3 + l][el] = mr_tangent_entry(F, FFt, k, mat.kappa, i, j, kk, l);
+        }
+    }
+    __syncthreads();
+    // (element, a, b) triples of the block, summed over the points, then scattered
+    const int total = n_valid * NN * NN;
+    for (int t = threadIdx.x; t < total; t += ST_THREADS) {
+        const int el = t % n_valid;  // consecutive lanes -> different elements (LDS banks)
+        const int ab = t / n_valid;
+        const int a = ab / NN, b = ab % NN;
+        double acc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll 1
+        for (int q = 0; q < NQ; q++) {
+            const double ga[3] = {sg[q][a][0][el], sg[q][a][1][el], sg[q][a][2][el]};
+            const double gb[3] = {sg[q][b][0][el], sg[q][b][1][el], sg[q][b][2][el]};
+            const double Fa[3] = {sFh[q][a][0][el], sFh[q][a][1][el], sFh[q][a][2][el]};
+            const double Fb[3] = {sFh[q][b][0][el], sFh[q][b][1][el], sFh[q][b][2][el]};
+            const double ff[3][3] = {{sFFt[q][0][el], sFFt[q][1][el], sFFt[q][2][el]},
+                                     {sFFt[q][1][el], sFFt[q][3][el], sFFt[q][4][el]},
+                                     {sFFt[q][2][el], sFFt[q][4][el], sFFt[q][5][el]}};
             const double hab = gb[0] * ga[0] + gb[1] * ga[1] + gb[2] * ga[2];
             const double FbFa = Fb[0] * Fa[0] + Fb[1] * Fa[1] + Fb[2] * Fa[2];
-            const double trE = strE[el], dV = sdV[el];
-            const int64_t blk = slot[((int64_t)(tile * TILE + el)) * NN * NN + ab];
-            const int64_t kb = 9 * (blk & ~(int64_t)(TILE - 1)) + (blk & (TILE - 1));
+            const double trE = strE[q][el], dV = sdV[q][el];
 #pragma unroll
             for (int d = 0; d < 3; d++)
 #pragma unroll
@@ -324,7 +332,7 @@ __global__ __launch_bounds__(64) void k_stiffness(int n_elem, const int* __restr
 #pragma unroll
                         for (int J = 0; J < 3; J++)
 #pragma unroll
-                            for (int L = 0; L < 3; L++) sum += sA[((d * 3 + J) * 3 + c) * 3 + L][el] * ga[J] * gb[L];
+                            for (int L = 0; L < 3; L++) sum += sA[q][((d * 3 + J) * 3 + c) * 3 + L][el] * ga[J] * gb[L];
                         K = sum * dV;
                     } else {
                         K = (mat.lambda * Fa[d] * Fb[c] + mat.lambda * trE * hab * dl + mat.mu * FbFa * dl +
@@ -336,10 +344,15 @@ __global__ __launch_bounds__(64) void k_stiffness(int n_elem, const int* __restr
                         K += (mat.eta_damp * (Fb[d] * Fa[c]) + mat.eta_damp * ff[d][c] * hab +
                               mat.lambda_damp * (Fa[d] * Fb[c])) *
                              dV;
-                    unsafeAtomicAdd(&k_sell[kb + (3 * d + c) * TILE], K);
+                    acc[d][c] += K;
                 }
         }
-        __syncthreads();
+        const int64_t blk = slot[((int64_t)(e0 + el)) * NN * NN + ab];
+        const int64_t kb = 9 * (blk & ~(int64_t)(TILE - 1)) + (blk & (TILE - 1));
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) unsafeAtomicAdd(&k_sell[kb + (3 * d + c) * TILE], acc[d][c]);
     }
 }
 
@@ -1350,11 +1363,10 @@ extern "C" int tlfea_stiffness_assemble(const tlfea_pattern* p, int nn, int n_el
     TLFEA_REQUIRE(nn == p->nn && n_elem == p->n_elem, "element data does not match the pattern");
     hipStream_t s = as_stream(stream);
     TLFEA_HIP_TRY(hipMemsetAsync(k_sell, 0, sizeof(double) * 9 * p->sell_blocks, s));
-    const unsigned tiles = (unsigned)(padded(n_elem) / TILE);
     const bool mr = mat->model == 1;
-#define TLFEA_STIFFNESS(NN_, NQ_, MR_)                                                                        \
-    k_stiffness<NN_, NQ_, MR_><<<tiles, TILE, 0, s>>>(n_elem, conn, x, y, z, grad_t, detJ_t, *quad, *mat, h, \
-                                                      p->d_slot, k_sell)
+#define TLFEA_STIFFNESS(NN_, NQ_, MR_)                                                                          \
+    k_stiffness<NN_, NQ_, MR_><<<(unsigned)((n_elem + st_elems<MR_>() - 1) / st_elems<MR_>()), ST_THREADS, 0, s>>>( \
+        n_elem, conn, x, y, z, grad_t, detJ_t, *quad, *mat, h, p->d_slot, k_sell)
     if (nn == 10 && mr)
         TLFEA_STIFFNESS(10, 5, true);
     else if (nn == 10)
