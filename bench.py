@@ -54,21 +54,24 @@ by the GPU tangent (the CG itself stops on its recursive residual, as the refere
 
 `velocity_solvers` (not part of `value`): SyncedNesterov, SyncedAdamW and SyncedAdamWNocoop
 on the same beam, one implicit step of 1e-3 s with zero tolerances (one outer iteration of
-200 inner iterations), ms per inner iteration after a warm-up solve; and `c1_nesterov`, the
-small beam's implicit step of 1e-3 s solved by SyncedNesterov (with the momentum restart)
-to a relative gradient of 1e-8, solves/s.
+200 inner iterations), ms per inner iteration after a warm-up solve, with the assembled mass
+and (`*_mf`) with the mass term applied inside the force kernel; and `c1_nesterov`
+(`c1_nesterov_mf`), the small beam's implicit step of 1e-3 s solved by SyncedNesterov (with
+the momentum restart) to a relative gradient of 1e-8, solves/s.
 
 `batched` (not part of `value`): the batch of 256 uneven cantilevers of the beam sampler
 (seed 2026) solved to 1e-8 -- all at once by the batched FP64 CG (finished instances
 compacted out), from the learned guess, with the FP32 two-level preconditioner (Chebyshev
 stage + rigid-plane coarse correction, setup timed and included) inside the FP64 flexible
 CG with and without the learned guess, and one after another by the plain FP64 drop-in
-path -- in solves/s.
+path -- in solves/s; and `nesterov_step`, one implicit step of 1e-3 s from rest for every
+beam by the batched SyncedNesterov loop to a relative gradient of 1e-8.
 
 `dynamic` (not part of `value`): 64 beams of the sampler under gravity and sinusoidal tip
 loads, 200 implicit steps of 1e-4 s for all beams at once -- Newton steps of the nonlinear
-implicit-Euler equation to ||g||/||g(v_n)|| <= 1e-8, and steps linearised at rest -- and
-20 Newton steps of each beam alone (the per-instance path); instance-steps/s.
+implicit-Euler equation to ||g||/||g(v_n)|| <= 1e-8, steps linearised at rest, and the same
+steps by the batched SyncedNesterov loop to the same relative gradient -- and 20 Newton
+steps of each beam alone (the per-instance path); instance-steps/s.
 """
 from __future__ import annotations
 
@@ -228,15 +231,19 @@ def batched_nesterov_bench(n, tol, log, dt=1e-3, max_inner=50000):
 def velocity_solvers_bench(m, E, nu, rho, log, inner=200):
     """SyncedNesterov / SyncedAdamW / SyncedAdamWNocoop on the beam (one implicit step of
     1e-3 s each, zero tolerances so that every solve runs one outer iteration of `inner`
-    inner iterations): ms per inner iteration, the rest-coordinate force kernel inside."""
+    inner iterations): ms per inner iteration, the rest-coordinate force kernel inside; the
+    *_mf lines apply the mass term inside that kernel instead of from the assembled mass."""
     import torch
     from tlfea_amd import capi
     from tlfea_amd.fea import Body, Solver
     out = {}
-    for name, kind in (("nesterov", capi.SOLVER_NESTEROV), ("adamw", capi.SOLVER_ADAMW),
-                       ("adamw_nocoop", capi.SOLVER_ADAMW_NOCOOP)):
+    for name, kind, mf in (("nesterov", capi.SOLVER_NESTEROV, False), ("adamw", capi.SOLVER_ADAMW, False),
+                           ("adamw_nocoop", capi.SOLVER_ADAMW_NOCOOP, False),
+                           ("nesterov_mf", capi.SOLVER_NESTEROV, True), ("adamw_mf", capi.SOLVER_ADAMW, True)):
         b = Body(m, E, nu, rho)
         s = Solver(b, kind, planned=True, rest=True)
+        if mf:
+            s.set_matrix_free_mass(rho)
         if kind == capi.SOLVER_NESTEROV:
             p = capi.NesterovParams(alpha=1e-8, rho=1e14, inner_tol=0.0, outer_tol=0.0, max_outer=1,
                                     max_inner=inner, time_step=1e-3)
@@ -259,7 +266,7 @@ def velocity_solvers_bench(m, E, nu, rho, log, inner=200):
     return out
 
 
-def c1_nesterov_bench(E, nu, rho, tol, log, reps=3):
+def c1_nesterov_bench(E, nu, rho, tol, log, reps=3, mf=False):
     """The 20x2x2-cell beam's implicit step (dt 1e-3 s, penalty 1e10 at lambda = 0, step
     1e-8) solved by SyncedNesterov to ||g_free|| <= tol ||g_free(0)||, with the gradient
     restart of the momentum (the reference's sequence alone converges sublinearly)."""
@@ -269,6 +276,8 @@ def c1_nesterov_bench(E, nu, rho, tol, log, reps=3):
     m = M.beam_config("C1")
     b = Body(m, E, nu, rho)
     s = Solver(b, capi.SOLVER_NESTEROV, planned=True, rest=True)
+    if mf:  # the mass term inside the force kernel
+        s.set_matrix_free_mass(rho)
     p = capi.NesterovParams(alpha=1e-8, rho=1e10, inner_tol=0.0, outer_tol=0.0, max_outer=1, max_inner=200000,
                             time_step=1e-3)
     s.set_rel_tol(tol)
@@ -293,7 +302,9 @@ def c1_nesterov_bench(E, nu, rho, tol, log, reps=3):
                        f"gradient of {tol:g} (restarted momentum)",
            "solves_per_s": 1.0 / dt, "s": dt, "inner_iterations": st["inner"], "restarts": s.restarts(),
            "final_rel_gradient": st["rel"], "ms_per_inner": dt * 1e3 / max(st["inner"], 1)}
-    log(f"C1 Nesterov to {tol:g}: {dt * 1e3:.1f} ms, {st['inner']} inner iterations, {s.restarts()} restarts")
+    out["matrix_free_mass"] = mf
+    log(f"C1 Nesterov to {tol:g}{' (matrix-free mass)' if mf else ''}: {dt * 1e3:.1f} ms, {st['inner']} inner "
+        f"iterations, {s.restarts()} restarts")
     s.close()
     b.close()
     return out
@@ -429,6 +440,7 @@ def main():
         m = M.clamp_and_tip_load(M.kuhn_beam(nx, ny, nz), -1.0e5)
         vs = velocity_solvers_bench(m, E, nu, rho, log)
         vs["c1_nesterov"] = c1_nesterov_bench(E, nu, rho, args.tol, log)
+        vs["c1_nesterov_mf"] = c1_nesterov_bench(E, nu, rho, args.tol, log, mf=True)
         print(json.dumps({"velocity_solvers": vs}))
         return
     if args.batch_only:
@@ -678,6 +690,7 @@ def main():
     velocity = None if args.no_solvers else velocity_solvers_bench(m, E, nu, rho, log)
     if velocity is not None:
         velocity["c1_nesterov"] = c1_nesterov_bench(E, nu, rho, args.tol, log)
+        velocity["c1_nesterov_mf"] = c1_nesterov_bench(E, nu, rho, args.tol, log, mf=True)
     batched = None if args.no_batch else batched_beams(args.batch_n, args.tol, log, not args.no_seq)
     if batched is not None:
         batched["nesterov_step"] = batched_nesterov_bench(args.batch_n, args.tol, log)

@@ -557,6 +557,15 @@ int tlfea_solver_set_rel_tol(tlfea_solver* s, double rel_tol);
  * AdamW kinds.  tlfea_solver_restarts gives the count of the last Solve. */
 int tlfea_solver_set_restart(tlfea_solver* s, int on);
 int tlfea_solver_restarts(const tlfea_solver* s, int* restarts);
+/* Not in the reference: applies the mass term of the velocity solvers' gradient
+ * (M (v_guess - v_prev) / dt, SyncedNesterov.cu:32-95 and the AdamW kinds' equivalent)
+ * without the assembled mass: per element, inside the rest-coordinate force kernel, with the
+ * quadrature rule of mass_matrix_qp_kernel (FEAT10Data.cu:200-271) and density rho, so
+ * the inner iteration reads no mass matrix.  The same operator as a mass assembled with
+ * tlfea_mass_assemble at that density (the caller's statement; the body's m_sell is then
+ * not read).  rho = 0 returns to the assembled mass.  Needs a body with a force plan and
+ * rest positions, SVK without damping, the standard rule (EINVAL otherwise). */
+int tlfea_solver_set_matrix_free_mass(tlfea_solver* s, double rho);
 /* How often (in inner iterations) the host reads the device convergence flag to stop
  * launching; results do not depend on it.  Default 16. */
 int tlfea_solver_set_poll_interval(tlfea_solver* s, int every);

@@ -56,13 +56,18 @@ def _compare(m, b, s, prob, st, tol=1e-9):
 @pytest.mark.parametrize("name,damp,force", [("cube", False, "planned"), ("C1", False, "planned"),
                                              ("C1", True, "planned"), ("C1", False, "atomic"),
                                              ("kuhn_t4", False, "planned"), ("C1", False, "rest"), ("C1", True, "rest"),
-                                             ("kuhn_t4", False, "rest")])
+                                             ("kuhn_t4", False, "rest"), ("C1", False, "rest_mf"),
+                                             ("cube", False, "rest_mf"), ("kuhn_t4", False, "rest_mf")])
 def test_nesterov_trajectory(gpu, name, damp, force):
+    """force "rest_mf": the mass term applied per element inside the force kernel
+    (tlfea_solver_set_matrix_free_mass) against the oracle's assembled mass."""
     from tlfea_amd import capi
     print(f"[nesterov] {name} damp={damp} force={force}", flush=True)
     eta, ld = (2e5, 1e5) if damp else (0.0, 0.0)
     m, b, s, prob = _setup(name, capi.SOLVER_NESTEROV, eta=eta, lambda_d=ld, planned=force != "atomic",
-                           random_load=True, rest=force == "rest")
+                           random_load=True, rest=force in ("rest", "rest_mf"))
+    if force == "rest_mf":
+        s.set_matrix_free_mass(RHO_STEEL)
     alpha = 2e-10 if name == "cube" else 5e-9  # stable step for steel on each mesh
     p = capi.NesterovParams(alpha=alpha, rho=1e10, inner_tol=0.0, outer_tol=0.0, max_outer=2, max_inner=25,
                             time_step=1e-3)
@@ -240,7 +245,60 @@ def _step_newton_extended(m, prob, dt, rho):
     return v, grad
 
 
-def test_nesterov_c1_step_to_1e8_with_restart(gpu):
+@pytest.mark.parametrize("kind", ["nesterov", "adamw", "adamw_nocoop"])
+def test_matrix_free_mass_matches_assembled(gpu, kind):
+    """The velocity solvers with the mass term applied per element inside the force kernel
+    equal the same solvers on the assembled mass, run side by side from the same deformed
+    start: positions, v_guess and f_int after each of two Solve() calls (Nesterov 1e-9,
+    AdamW 1e-6, see the module note); a damped body is refused."""
+    import torch
+    from tlfea_amd import capi
+    from tlfea_amd.fea import Body, Solver
+    kinds = {"nesterov": capi.SOLVER_NESTEROV, "adamw": capi.SOLVER_ADAMW, "adamw_nocoop": capi.SOLVER_ADAMW_NOCOOP}
+    m = load_mesh("C1")
+    m.f_ext = np.random.default_rng(21).standard_normal(3 * m.n_nodes) * 50.0
+    runs = []
+    for mf in (False, True):
+        b = Body(m, E_STEEL, NU_STEEL, RHO_STEEL)
+        s = Solver(b, kinds[kind], planned=True, rest=True)
+        if mf:
+            s.set_matrix_free_mass(RHO_STEEL)
+        if kind == "nesterov":
+            p = capi.NesterovParams(alpha=5e-9, rho=1e10, inner_tol=0.0, outer_tol=0.0, max_outer=2, max_inner=25,
+                                    time_step=1e-3)
+        else:
+            p = capi.AdamWParams(lr=2e-4, beta1=0.9, beta2=0.999, eps=1e-8, weight_decay=1e-4, lr_decay=0.995,
+                                 inner_tol=0.0, outer_tol=0.0, rho=1e10, max_outer=2, max_inner=30, time_step=1e-3,
+                                 convergence_check_interval=10, inner_rtol=0.0)
+        s.set_parameters(p)
+        x, y, z = deformed(m, seed=22, amp=2e-5, rot=0.0)
+        for d, a in enumerate((x, y, z)):
+            b.x[d].copy_(torch.as_tensor(a))
+        runs.append((b, s))
+    tol = 1e-9 if kind == "nesterov" else 1e-6
+    X0 = m.xyz()
+    for step in range(2):
+        out = []
+        for b, s in runs:
+            s.solve()
+            out.append((np.stack([t.cpu().numpy() for t in b.x], 1) - X0, s.v_guess().cpu().numpy(),
+                        b.f_int.cpu().numpy()))
+        assert runs[0][1].stats()["inner"] == runs[1][1].stats()["inner"]
+        for k in range(3):
+            assert rel_err(out[1][k], out[0][k]) < tol, (step, k)
+    for b, s in runs:
+        s.close()
+        b.close()
+    bd = Body(m, E_STEEL, NU_STEEL, RHO_STEEL, 1e4, 1e3)
+    sd = Solver(bd, capi.SOLVER_NESTEROV, planned=True, rest=True)
+    with pytest.raises(RuntimeError):
+        sd.set_matrix_free_mass(RHO_STEEL)
+    sd.close()
+    bd.close()
+
+
+@pytest.mark.parametrize("mf", [False, True])
+def test_nesterov_c1_step_to_1e8_with_restart(gpu, mf):
     """The C1 beam (steel, 1000 N tip load): one implicit step of 1e-3 s solved by
     SyncedNesterov to ||g_free|| <= 1e-8 ||g_free(0)||, with the momentum restart (the
     reference's t-sequence alone converges sublinearly: the oracle's restatement is at 1e-3
@@ -252,6 +310,8 @@ def test_nesterov_c1_step_to_1e8_with_restart(gpu):
     velocity and the two velocities."""
     from tlfea_amd import capi
     m, b, s, prob = _setup("C1", capi.SOLVER_NESTEROV, rest=True)
+    if mf:  # the mass term inside the force kernel
+        s.set_matrix_free_mass(RHO_STEEL)
     dt, rho = 1e-3, 1e10
     p = capi.NesterovParams(alpha=1e-8, rho=rho, inner_tol=0.0, outer_tol=0.0, max_outer=1, max_inner=200000,
                             time_step=dt)

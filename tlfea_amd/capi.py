@@ -175,6 +175,7 @@ _SIGS = {
     "tlfea_solver_set_parameters": (i32, [vp, vp]),
     "tlfea_solver_set_restart": (i32, [vp, i32]),
     "tlfea_solver_restarts": (i32, [vp, C.POINTER(i32)]),
+    "tlfea_solver_set_matrix_free_mass": (i32, [vp, f64]),
     "tlfea_solver_set_rel_tol": (i32, [vp, f64]),
     "tlfea_solver_set_poll_interval": (i32, [vp, i32]),
     "tlfea_solver_solve": (i32, [vp, vp]),

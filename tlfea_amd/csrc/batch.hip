@@ -2160,8 +2160,9 @@ int nesterov_iteration(tlfea_batch* b, double dt, int mass_exact, const double* 
     fm.done_stride = NI;
     fm.blk_list = b->compact ? b->d_blk_list : nullptr;
     fm.n_list = b->n_blk_list;
+    const tlfea_material none{0.0, 0.0, 0.0, 0.0};
     if (int rc = launch_force_rest_mass_partials(b->plan, b->d_nux, b->d_nuy, b->d_nuz, b->d_X, b->d_Y, b->d_Z,
-                                                 b->quad, b->d_elem_inst, b->d_lam, b->d_mu, fm, s))
+                                                 b->quad, none, b->d_elem_inst, b->d_lam, b->d_mu, fm, s))
         return rc;
     k_bnes_grad<<<nc, BT, 0, s>>>(ch, b->plan->d_g_off, b->plan->d_g_idx, b->plan->d_part, b->d_mask, f_ext,
                                   b->d_ny, b->d_nvk, v, b->d_ns, b->d_i, b->d_npart);

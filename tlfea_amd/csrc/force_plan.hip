@@ -150,7 +150,7 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_force_rest(
     const int e = b * PLAN_BLOCK + threadIdx.x;
     const int tile = e >> 6, lane = e & (TILE - 1);
     bool active = e < n_elem;
-    if (active && fm.inst_done && fm.inst_done[elem_inst[e] * fm.done_stride]) active = false;
+    if (active && fm.inst_done && elem_inst && fm.inst_done[elem_inst[e] * fm.done_stride]) active = false;
     int sl[NN];
     if (active) {
 #pragma unroll
@@ -173,9 +173,10 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_force_rest(
             sV[2 * max_u + u] = v[3 * n + 2];
         }
         if constexpr (MASS != 0) {
-            sV[u] = fm.dv[3 * n];
-            sV[max_u + u] = fm.dv[3 * n + 1];
-            sV[2 * max_u + u] = fm.dv[3 * n + 2];
+            const double* dvn = fm.dv + (int64_t)fm.dv_stride * n;
+            sV[u] = dvn[0];
+            sV[max_u + u] = dvn[1];
+            sV[2 * max_u + u] = dvn[2];
         }
     }
     __syncthreads();
@@ -184,7 +185,7 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_force_rest(
         const double lam = elem_inst ? inst_lam[elem_inst[e]] : mat.lambda;
         const double mu = elem_inst ? inst_mu[elem_inst[e]] : mat.mu;
         double mc = 0.0, vol = 0.0;
-        if constexpr (MASS != 0) mc = fm.inst_mc[elem_inst[e]];
+        if constexpr (MASS != 0) mc = elem_inst ? fm.inst_mc[elem_inst[e]] : fm.mc;
         double fa[NN][3];
 #pragma unroll
         for (int a = 0; a < NN; a++) fa[a][0] = fa[a][1] = fa[a][2] = 0.0;
@@ -454,20 +455,24 @@ int launch_force_rest(const tlfea_force_plan* p, const double* x, const double* 
 
 int launch_force_rest_mass_partials(const tlfea_force_plan* p, const double* ux, const double* uy, const double* uz,
                                     const double* X0, const double* Y0, const double* Z0, const tlfea_quad& quad,
-                                    const int* elem_inst, const double* inst_lam, const double* inst_mu,
-                                    const ForceMass& fm, hipStream_t s) {
-    if (!elem_inst || !inst_lam || !inst_mu || !fm.dv || !fm.inst_mc) {
-        set_error("the force with a mass term needs per-instance materials, dv and mass coefficients");
+                                    const tlfea_material& mat, const int* elem_inst, const double* inst_lam,
+                                    const double* inst_mu, const ForceMass& fm, hipStream_t s) {
+    const bool batch = elem_inst != nullptr;
+    if ((batch && (!inst_lam || !inst_mu || !fm.inst_mc)) || !fm.dv || fm.dv_stride < 3) {
+        set_error("the force with a mass term needs materials, dv and mass coefficients");
+        return TLFEA_EINVAL;
+    }
+    if (!batch && (mat.model != 0 || mat.eta_damp != 0.0 || mat.lambda_damp != 0.0)) {
+        set_error("the force with a mass term covers SVK without damping");
         return TLFEA_EINVAL;
     }
     const unsigned nb = fm.blk_list ? (unsigned)fm.n_list : (unsigned)p->n_blocks;
     if (nb == 0) return TLFEA_OK;
     const size_t lds = sizeof(double) * (size_t)p->max_u * 12;
-    const tlfea_material none{0.0, 0.0, 0.0, 0.0};
-#define TLFEA_FORCE_MASS(NN_, NQ_, M_)                                                                         \
-    k_force_rest<NN_, NQ_, false, false, M_><<<nb, PLAN_BLOCK, lds, s>>>(                                      \
-        p->n_elem, p->d_blk_off, p->d_unodes, p->d_slot, ux, uy, uz, X0, Y0, Z0, nullptr, quad, none, elem_inst, \
-        inst_lam, inst_mu, p->d_part, p->max_u, nullptr, 1, fm)
+#define TLFEA_FORCE_MASS(NN_, NQ_, M_)                                                                        \
+    k_force_rest<NN_, NQ_, false, false, M_><<<nb, PLAN_BLOCK, lds, s>>>(                                     \
+        p->n_elem, p->d_blk_off, p->d_unodes, p->d_slot, ux, uy, uz, X0, Y0, Z0, nullptr, quad, mat, elem_inst, \
+        inst_lam, inst_mu, p->d_part, p->max_u, fm.flags, 1, fm)
     if (p->nn == 10) {
         if (fm.mass_exact)
             TLFEA_FORCE_MASS(10, 5, 2);

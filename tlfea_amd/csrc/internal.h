@@ -55,14 +55,19 @@ int launch_force_rest(const tlfea_force_plan* p, const double* x, const double* 
                       const tlfea_material& mat, const double* v, double* f, int accumulate, const int* flags,
                       hipStream_t s, const int* elem_inst = nullptr, const double* inst_lam = nullptr,
                       const double* inst_mu = nullptr, bool x_is_disp = false);
-// Options of the rest-coordinate force for batched implicit steps: a consistent-mass term
-// c_i M dv added to the same block partials (c_i = inst_mc[instance], M the exactly
-// integrated mass when mass_exact, else the quadrature rule's), the elements of instances
-// whose inst_done[inst * done_stride] is set skipped, and only the blocks of blk_list (n_list
-// of them) launched.  SVK without damping only.
+// Options of the rest-coordinate force for implicit steps: a consistent-mass term c M dv
+// added to the same block partials (c = inst_mc[instance] in a batch, mc for one body; M the
+// exactly integrated mass when mass_exact, else the quadrature rule's; dv read with
+// dv_stride doubles per node), the elements of instances whose inst_done[inst *
+// done_stride] is set skipped, only the blocks of blk_list (n_list of them) launched, and
+// the launch skipped while flags[0] | flags[1] (one body's solver flags).  SVK without
+// damping only.
 struct ForceMass {
     const double* dv = nullptr;
+    int dv_stride = 3;
     const double* inst_mc = nullptr;
+    double mc = 0.0;
+    const int* flags = nullptr;
     int mass_exact = 0;
     const int* inst_done = nullptr;
     int done_stride = 0;
@@ -70,10 +75,11 @@ struct ForceMass {
     int n_list = 0;
 };
 // f_int(X + u) + c M dv as block partials in plan->d_part (no gather); u = (ux, uy, uz).
+// elem_inst / inst_lam / inst_mu for a batch, else mat's (lambda, mu).
 int launch_force_rest_mass_partials(const tlfea_force_plan* p, const double* ux, const double* uy, const double* uz,
                                     const double* X0, const double* Y0, const double* Z0, const tlfea_quad& quad,
-                                    const int* elem_inst, const double* inst_lam, const double* inst_mu,
-                                    const ForceMass& fm, hipStream_t s);
+                                    const tlfea_material& mat, const int* elem_inst, const double* inst_lam,
+                                    const double* inst_mu, const ForceMass& fm, hipStream_t s);
 
 // dN/dxi of every node at every quadrature point (a kernel argument).
 template <typename T>

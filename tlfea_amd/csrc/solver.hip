@@ -29,6 +29,7 @@ struct tlfea_solver {
     double rel_tol = 0.0;
     int poll = 16;
     int restart = 0;  // Nesterov gradient restart (tlfea_solver_set_restart)
+    double mf_rho = 0.0;  // > 0: matrix-free mass of this density (tlfea_solver_set_matrix_free_mass)
     int64_t n3 = 0;
     // state
     double *v_guess = nullptr, *v_prev = nullptr, *lam = nullptr;
@@ -256,6 +257,64 @@ __global__ __launch_bounds__(S_RED) void k_grad(Body b, int64_t n_slices, const 
                     out[3 * row + d] = gr[d];
                     acc[2] += vg[3 * row + d] * vg[3 * row + d];
                 }
+            }
+        }
+    }
+    block_sums<5>(acc);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < 5; k++) part[k * nblk + blk] = acc[k];
+}
+
+// The same gradient with the mass term applied without the assembled matrix: the block
+// partials of the rest-coordinate force kernel carry f_int + (rho/dt) M dv (the quadrature
+// rule's mass of mass_matrix_qp_kernel, FEAT10Data.cu:200-271, per element), gathered per
+// node here; one thread per node, the same five partial sums.
+template <int MODE>
+__global__ __launch_bounds__(S_RED) void k_grad_mf(Body b, const int* __restrict__ g_off,
+                                                   const int64_t* __restrict__ g_idx, const double* __restrict__ fpart,
+                                                   const double* __restrict__ vg, const double* __restrict__ lam,
+                                                   const int* __restrict__ cons_off, const int* __restrict__ cons_idx,
+                                                   const unsigned char* __restrict__ node_fixed, double dt, double rho,
+                                                   double alpha, const double* __restrict__ vk, double* __restrict__ out,
+                                                   double* __restrict__ part, int nblk, const int* __restrict__ flags) {
+    if (flags[0] | flags[1]) return;
+    const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+    const int row = (int)(blk * blockDim.x + threadIdx.x);
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (row < b.n_nodes) {
+        double gr[3] = {0.0, 0.0, 0.0};
+        gather_slots3(g_idx, fpart, g_off[row], g_off[row + 1], gr[0], gr[1], gr[2]);
+#pragma unroll
+        for (int d = 0; d < 3; d++) gr[d] -= b.f_ext[3 * row + d];
+        for (int k = cons_off[row]; k < cons_off[row + 1]; k++) {
+            const int ci = cons_idx[k];
+            const double xs[3] = {b.x[row], b.y[row], b.z[row]};
+            const double xr[3] = {b.xref[row], b.yref[row], b.zref[row]};
+#pragma unroll
+            for (int d = 0; d < 3; d++) gr[d] += dt * 1.0 * (lam[3 * ci + d] + rho * (xs[d] - xr[d]));
+        }
+        const bool fixed = node_fixed[row] != 0;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            acc[0] += gr[d] * gr[d];
+            if (!fixed) acc[1] += gr[d] * gr[d];
+        }
+        if (MODE == 0) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const double vnx = vg[3 * row + d] - alpha * gr[d];
+                out[3 * row + d] = vnx;
+                acc[2] += vnx * vnx;
+                const double k = vk[3 * row + d];
+                acc[3] += k * k;
+                acc[4] += gr[d] * (vnx - k);
+            }
+        } else {
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                out[3 * row + d] = gr[d];
+                acc[2] += vg[3 * row + d] * vg[3 * row + d];
             }
         }
     }
@@ -638,6 +697,26 @@ extern "C" int tlfea_solver_set_restart(tlfea_solver* s, int on) {
     return TLFEA_OK;
 }
 
+extern "C" int tlfea_solver_set_matrix_free_mass(tlfea_solver* s, double rho) {
+    TLFEA_REQUIRE(s != nullptr, "null solver");
+    TLFEA_REQUIRE(rho >= 0.0, "density must be non-negative");
+    if (rho > 0.0) {
+        const tlfea_body& b = s->b;
+        TLFEA_REQUIRE(b.force_plan && b.X0, "the matrix-free mass rides on the rest-coordinate force: a body "
+                                            "with a force plan and rest positions is needed");
+        TLFEA_REQUIRE(b.mat.model == 0 && b.mat.eta_damp == 0.0 && b.mat.lambda_damp == 0.0,
+                      "the matrix-free mass covers SVK bodies without damping");
+        TLFEA_REQUIRE(is_standard_rule(b.nn, b.quad), "the matrix-free mass needs the standard quadrature rule");
+        if ((size_t)b.force_plan->max_u * 12 * sizeof(double) > 160 * 1024) {
+            set_error("a block touches %d nodes: beyond the LDS budget of the force with a mass term",
+                      b.force_plan->max_u);
+            return TLFEA_ECAPACITY;
+        }
+    }
+    s->mf_rho = rho;
+    return TLFEA_OK;
+}
+
 extern "C" int tlfea_solver_restarts(const tlfea_solver* s, int* restarts) {
     TLFEA_REQUIRE(s && restarts, "null argument");
     *restarts = s->last_restarts;
@@ -700,6 +779,9 @@ extern "C" int tlfea_solver_solve(tlfea_solver* S, void* stream) {
     const bool rest_form = b.force_plan && b.X0;
     const DispOut uo = rest_form ? DispOut{S->ux, S->uy, S->uz, b.X0, b.Y0, b.Z0} : DispOut{};
     const double* const tseq = (nes && S->restart) ? S->scal : nullptr;
+    // matrix-free mass: set up for a rest-coordinate body (tlfea_solver_set_matrix_free_mass)
+    const bool mf = S->mf_rho > 0.0 && rest_form;
+    const unsigned gM = grid_for(N, S_RED);
 
     TLFEA_HIP_TRY(hipMemsetAsync(S->flags, 0, sizeof(int) * 8, st));
     k_save_prev<<<gN, 256, 0, st>>>(N, b.x, b.y, b.z, S->xp, S->yp, S->zp);
@@ -743,7 +825,16 @@ extern "C" int tlfea_solver_solve(tlfea_solver* S, void* stream) {
             TLFEA_LAUNCH_CHECK();
             // element forces at the scratch positions, damping from v_guess; skipped once the
             // inner loop has converged (f_int then keeps the last evaluation)
-            if (rest_form) {
+            if (mf) {
+                ForceMass fm;
+                fm.dv = S->dv;
+                fm.dv_stride = 4;
+                fm.mc = S->mf_rho / dt;
+                fm.flags = S->flags;
+                if (int rc = launch_force_rest_mass_partials(b.force_plan, S->ux, S->uy, S->uz, b.X0, b.Y0, b.Z0,
+                                                             b.quad, b.mat, nullptr, nullptr, nullptr, fm, st))
+                    return rc;
+            } else if (rest_form) {
                 if (int rc = launch_force_rest(b.force_plan, S->ux, S->uy, S->uz, b.X0, b.Y0, b.Z0, b.quad, b.mat,
                                                damp ? S->v_guess : nullptr, b.f_int, 0, S->flags, st, nullptr,
                                                nullptr, nullptr, true))
@@ -757,7 +848,16 @@ extern "C" int tlfea_solver_solve(tlfea_solver* S, void* stream) {
                 return rc;
             }
             const int check = nes ? 1 : ((inner % check_int) == 0);
-            if (nes) {
+            const tlfea_force_plan* fp = b.force_plan;
+            if (mf && nes) {
+                k_grad_mf<0><<<gM, S_RED, 0, st>>>(b, fp->d_g_off, fp->d_g_idx, fp->d_part, S->v_guess, S->lam,
+                                                   S->cons_off, S->cons_idx, S->node_fixed, dt, rho, S->np.alpha,
+                                                   S->vk, S->vn, S->part, gM, S->flags);
+            } else if (mf) {
+                k_grad_mf<1><<<gM, S_RED, 0, st>>>(b, fp->d_g_off, fp->d_g_idx, fp->d_part, S->v_guess, S->lam,
+                                                   S->cons_off, S->cons_idx, S->node_fixed, dt, rho, 0.0, nullptr,
+                                                   S->g, S->part, gM, S->flags);
+            } else if (nes) {
                 k_grad<0><<<gS, S_RED, 0, st>>>(b, P->n_slices, P->d_sell_row, P->d_slice_base, P->d_slice_len,
                                                 P->d_sell_col, S->v_guess, reinterpret_cast<const dv4*>(S->dv), S->lam, S->cons_off, S->cons_idx,
                                                 S->node_fixed, dt, rho, S->np.alpha, S->vk, S->vn, S->part, gS,
@@ -767,7 +867,7 @@ extern "C" int tlfea_solver_solve(tlfea_solver* S, void* stream) {
                                                 P->d_sell_col, S->v_guess, reinterpret_cast<const dv4*>(S->dv), S->lam, S->cons_off, S->cons_idx,
                                                 S->node_fixed, dt, rho, 0.0, nullptr, S->g, S->part, gS, S->flags);
             }
-            k_finalize<<<1, S_RED, 0, st>>>(S->kind, inner, check, gS, S->part, S->scal, S->flags, inner_tol,
+            k_finalize<<<1, S_RED, 0, st>>>(S->kind, inner, check, mf ? gM : gS, S->part, S->scal, S->flags, inner_tol,
                                             nes ? 0.0 : S->ap.inner_rtol, S->rel_tol, tseq ? 1 : 0);
             TLFEA_LAUNCH_CHECK();
             if ((inner + 1) % S->poll == 0) {
@@ -775,6 +875,13 @@ extern "C" int tlfea_solver_solve(tlfea_solver* S, void* stream) {
                 TLFEA_HIP_TRY(hipStreamSynchronize(st));
                 inner_done = hflags[0] != 0;
             }
+        }
+        // with the matrix-free mass the partials held f_int + mass term: f_int itself at the
+        // last evaluated positions (flags + 4: two always-zero words, so this launch runs)
+        if (mf) {
+            if (int rc = launch_force_rest(b.force_plan, S->ux, S->uy, S->uz, b.X0, b.Y0, b.Z0, b.quad, b.mat, nullptr,
+                                           b.f_int, 0, S->flags + 4, st, nullptr, nullptr, nullptr, true))
+                return rc;
         }
         // Nesterov keeps the last iterate in vn: v_guess = v_next (SyncedNesterov.cu:283)
         if (nes) k_copy<<<g3, 256, 0, st>>>(n3, S->vn, S->v_guess, S->flags, 1);

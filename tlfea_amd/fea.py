@@ -379,6 +379,19 @@ class LinearStatic:
             pass
 
 
+_HIP_MEMCPY = None
+
+
+def _hip_memcpy():
+    """hipMemcpy of the HIP runtime, loaded once."""
+    global _HIP_MEMCPY
+    if _HIP_MEMCPY is None:
+        fn = C.CDLL("libamdhip64.so").hipMemcpy
+        fn.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _HIP_MEMCPY = fn
+    return _HIP_MEMCPY
+
+
 class Solver:
     """SyncedNesterov / SyncedAdamW / SyncedAdamWNocoop on one Body
     (`src/solvers/SyncedNesterov.cuh:39-179`, `SyncedAdamW.cuh`, `SyncedAdamWNocoop.cuh`)."""
@@ -422,6 +435,12 @@ class Solver:
         """Nesterov momentum restart on g.(v_(k+1) - v_k) > 0 (not in the reference)."""
         call("tlfea_solver_set_restart", self.h, int(bool(on)))
 
+    def set_matrix_free_mass(self, rho):
+        """Mass term of the gradient applied per element inside the force kernel (density rho,
+        the quadrature rule of the assembled mass) instead of from the assembled matrix; 0
+        returns to the assembled mass (not in the reference)."""
+        call("tlfea_solver_set_matrix_free_mass", self.h, float(rho))
+
     def restarts(self):
         n = C.c_int()
         call("tlfea_solver_restarts", self.h, C.byref(n))
@@ -438,10 +457,7 @@ class Solver:
     def _vec(self, addr, n):
         out = torch.empty(n, dtype=torch.float64, device=DEV)
         torch.cuda.synchronize()
-        import ctypes
-        hip = ctypes.CDLL("libamdhip64.so")
-        hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
-        rc = hip.hipMemcpy(ptr(out), addr, 8 * n, 3)  # device to device
+        rc = _hip_memcpy()(ptr(out), addr, 8 * n, 3)  # device to device
         if rc != 0:
             raise RuntimeError(f"hipMemcpy failed: {rc}")
         return out
