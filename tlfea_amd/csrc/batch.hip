@@ -86,6 +86,13 @@ struct tlfea_batch {
     int* d_inst_cblk = nullptr;    // [n_inst + 1]: first coarse block of each instance
     double *d_cdiag = nullptr, *d_clow = nullptr, *d_cg = nullptr;  // [n_cblocks][NB][NB]
     double *d_crhs = nullptr, *d_cx = nullptr;                       // [6 n_planes]
+    // instances whose coarse matrix is applied as a dense inverse (long block chains, where
+    // the sweep's latency exceeds reading the inverse): offsets into d_cdense (-1 = sweep),
+    // and their coarse rows (global coarse DOF, instance)
+    int64_t* d_cdense_off = nullptr;
+    double* d_cdense = nullptr;
+    int *d_cd_row = nullptr, *d_cd_inst = nullptr;
+    int n_cd_rows = 0;
     // batched SyncedNesterov (tlfea_batch_nesterov_solve): v_(k-1), y, dv = y - v_n (AoS
     // [3 n_nodes]), the displacement u_n + dt y (SoA), chunk partials [n_chunks][4],
     // per-instance scalars [n_inst][NSN], mass coefficients rho_i / dt, restart counts
@@ -136,6 +143,23 @@ struct Insts {
 };
 __device__ __forceinline__ int inst_at(const Insts& in, int j) { return in.list ? in.list[j] : j; }
 
+// The K chunk partials cpart[K c + k] summed over instance i's chunks by one wave, in a
+// fixed order (lane l takes chunks l, l + 64, ..., then a butterfly); lane 0 holds the
+// sums.  The per-instance reduction kernels run one wave per instance.
+template <int K>
+__device__ __forceinline__ void wave_inst_sums(const int* __restrict__ ico, const double* __restrict__ cpart, int i,
+                                               double (&s)[K]) {
+#pragma unroll
+    for (int k = 0; k < K; k++) s[k] = 0.0;
+    for (int c = ico[i] + (int)threadIdx.x; c < ico[i + 1]; c += 64)
+#pragma unroll
+        for (int k = 0; k < K; k++) s[k] += cpart[K * (int64_t)c + k];
+#pragma unroll
+    for (int k = 0; k < K; k++)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s[k] += __shfl_xor(s[k], o, 64);
+}
+
 // q = sum of partials per node (or p on masked DOFs); chunk partial of p.q
 template <typename T>
 __global__ __launch_bounds__(BT) void k_bgather(Chunks ch, const int* __restrict__ g_off,
@@ -166,9 +190,11 @@ __global__ __launch_bounds__(BT) void k_bgather(Chunks ch, const int* __restrict
 }
 
 // r = f - q on free DOFs (0 on masked); chunk partials of f.f and r.r
+// (all, when set: its active count is zeroed for k_breduce_init)
 __global__ __launch_bounds__(BT) void k_binit(Chunks ch, const double* __restrict__ f, const double* __restrict__ q,
                                               const unsigned char* __restrict__ mask, double* __restrict__ r,
-                                              double* __restrict__ cpart) {
+                                              double* __restrict__ cpart, int* __restrict__ all = nullptr) {
+    if (all && blockIdx.x == 0 && threadIdx.x == 0) all[0] = 0;
     const int c = chunk_id(ch);
     double ff = 0.0, rr = 0.0;
     for (int64_t k = 3 * (int64_t)ch.beg[c] + threadIdx.x; k < 3 * (int64_t)ch.end[c]; k += BT) {
@@ -187,17 +213,16 @@ __global__ __launch_bounds__(BT) void k_binit(Chunks ch, const double* __restric
     }
 }
 
-// per instance: ||f||^2, r.r, stopping test at the initial guess
-__global__ __launch_bounds__(BT) void k_breduce_init(int n_inst, const int* __restrict__ ico,
+// per instance (one wave each): ||f||^2, r.r, stopping test at the initial guess; all[0]
+// (zeroed by k_binit) counts the instances left running
+__global__ __launch_bounds__(64) void k_breduce_init(int n_inst, const int* __restrict__ ico,
                                                      const double* __restrict__ cpart, double* __restrict__ S,
                                                      int* __restrict__ I, int* __restrict__ all, double tol) {
-    int active = 0;
-    for (int i = threadIdx.x; i < n_inst; i += BT) {
-        double ff = 0.0, rr = 0.0;
-        for (int c = ico[i]; c < ico[i + 1]; c++) {
-            ff += cpart[2 * c];
-            rr += cpart[2 * c + 1];
-        }
+    const int i = blockIdx.x;
+    double sm[2];
+    wave_inst_sums<2>(ico, cpart, i, sm);
+    if (threadIdx.x == 0) {
+        const double ff = sm[0], rr = sm[1];
         double* s = S + (int64_t)i * NS;
         int* fl = I + (int64_t)i * NI;
         s[S_FF] = ff;
@@ -209,14 +234,8 @@ __global__ __launch_bounds__(BT) void k_breduce_init(int n_inst, const int* __re
         fl[I_NOTPD] = 0;
         fl[I_ZERO_F] = ff == 0.0;
         fl[I_DONE] = (ff == 0.0 || rr <= tol * tol * ff) ? 1 : 0;
-        active += !fl[I_DONE];
+        if (!fl[I_DONE]) atomicAdd(all, 1);
     }
-    __shared__ int cnt;
-    if (threadIdx.x == 0) cnt = 0;
-    __syncthreads();
-    atomicAdd(&cnt, active);
-    __syncthreads();
-    if (threadIdx.x == 0) all[0] = cnt;
 }
 
 __global__ void k_bclear_flags(int n_inst, int* __restrict__ I) {
@@ -232,15 +251,16 @@ __global__ __launch_bounds__(BT) void k_bzero_u(Chunks ch, const int* __restrict
 }
 
 // per instance: alpha = rz / p.q, abort on p.q <= 0 (LinearStaticSolver.cu:404-407)
-__global__ __launch_bounds__(BT) void k_breduce_alpha(Insts in, const int* __restrict__ ico,
+__global__ __launch_bounds__(64) void k_breduce_alpha(Insts in, const int* __restrict__ ico,
                                                       const double* __restrict__ cpart, double* __restrict__ S,
                                                       int* __restrict__ I) {
-    for (int j = threadIdx.x; j < in.n; j += BT) {
-        const int i = inst_at(in, j);
-        int* fl = I + (int64_t)i * NI;
-        if (fl[I_DONE]) continue;
-        double pq = 0.0;
-        for (int c = ico[i]; c < ico[i + 1]; c++) pq += cpart[c];
+    const int i = inst_at(in, blockIdx.x);
+    int* fl = I + (int64_t)i * NI;
+    if (fl[I_DONE]) return;
+    double sm[1];
+    wave_inst_sums<1>(ico, cpart, i, sm);
+    if (threadIdx.x == 0) {
+        const double pq = sm[0];
         double* s = S + (int64_t)i * NS;
         s[S_PQ] = pq;
         if (pq <= 0.0) {
@@ -381,14 +401,12 @@ __global__ __launch_bounds__(BT) void k_bdot_ac(Chunks ch, const double* __restr
     if (threadIdx.x == 0) cpart[k] = s;
 }
 
-// per instance: the sum of its chunk partials in chunk order
-__global__ void k_binst_sum(int n_inst, const int* __restrict__ ico, const double* __restrict__ cpart,
-                            double* __restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_inst) return;
-    double s = 0.0;
-    for (int c = ico[i]; c < ico[i + 1]; c++) s += cpart[c];
-    out[i] = s;
+// per instance (one wave each): the sum of its chunk partials
+__global__ __launch_bounds__(64) void k_binst_sum(int n_inst, const int* __restrict__ ico,
+                                                  const double* __restrict__ cpart, double* __restrict__ out) {
+    double sm[1];
+    wave_inst_sums<1>(ico, cpart, blockIdx.x, sm);
+    if (threadIdx.x == 0) out[blockIdx.x] = sm[0];
 }
 
 // parity slots of a fresh solve
@@ -409,14 +427,16 @@ __global__ __launch_bounds__(BT) void k_bdot_rr(Chunks ch, const double* __restr
     if (threadIdx.x == 0) cpart[c] = rr;
 }
 
-__global__ __launch_bounds__(BT) void k_breduce_tail(int n_inst, const int* __restrict__ ico,
+__global__ __launch_bounds__(64) void k_breduce_tail(int n_inst, const int* __restrict__ ico,
                                                      const double* __restrict__ cpart, double* __restrict__ S,
                                                      int* __restrict__ I, double tol) {
-    for (int i = threadIdx.x; i < n_inst; i += BT) {
-        int* fl = I + (int64_t)i * NI;
-        if (fl[I_DONE]) continue;
-        double rr = 0.0;
-        for (int c = ico[i]; c < ico[i + 1]; c++) rr += cpart[c];
+    const int i = blockIdx.x;
+    int* fl = I + (int64_t)i * NI;
+    if (fl[I_DONE]) return;
+    double sm[1];
+    wave_inst_sums<1>(ico, cpart, i, sm);
+    if (threadIdx.x == 0) {
+        const double rr = sm[0];
         double* s = S + (int64_t)i * NS;
         s[S_RR] = rr;
         s[S_REL] = sqrt(rr / s[S_FF]);
@@ -527,24 +547,27 @@ __global__ __launch_bounds__(BT) void k_cheb_fin(Chunks ch, const float* __restr
 }
 
 // per instance after the first preconditioning: rz = z.r, p = z (beta = 0)
-__global__ __launch_bounds__(BT) void k_breduce_rz0(Insts in, const int* __restrict__ ico,
+__global__ __launch_bounds__(64) void k_breduce_rz0(Insts in, const int* __restrict__ ico,
                                                     const double* __restrict__ cpart, double* __restrict__ S,
                                                     const int* __restrict__ I) {
-    for (int j = threadIdx.x; j < in.n; j += BT) {
-        const int i = inst_at(in, j);
-        if (I[(int64_t)i * NI + I_DONE]) continue;
-        double zr = 0.0;
-        for (int c = ico[i]; c < ico[i + 1]; c++) zr += cpart[2 * c];
-        S[(int64_t)i * NS + S_RZ] = zr;
+    const int i = inst_at(in, blockIdx.x);
+    if (I[(int64_t)i * NI + I_DONE]) return;
+    double sm[2];
+    wave_inst_sums<2>(ico, cpart, i, sm);
+    if (threadIdx.x == 0) {
+        S[(int64_t)i * NS + S_RZ] = sm[0];
         S[(int64_t)i * NS + S_BETA] = 0.0;
     }
 }
 
 // u += alpha p, r_old = r, r -= alpha q; chunk partial of r.r
+// (all[0] zeroed for k_breduce_check's count)
 __global__ __launch_bounds__(BT) void k_bupdate_keep(Chunks ch, double* __restrict__ u, double* __restrict__ r,
                                                      double* __restrict__ rold, const double* __restrict__ p,
                                                      const double* __restrict__ q, const double* __restrict__ S,
-                                                     const int* __restrict__ I, double* __restrict__ cpart) {
+                                                     const int* __restrict__ I, double* __restrict__ cpart,
+                                                     int* __restrict__ all) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) all[0] = 0;
     const int c = chunk_id(ch);
     const int inst = ch.inst[c];
     if (I[inst * NI + I_DONE]) {
@@ -565,17 +588,18 @@ __global__ __launch_bounds__(BT) void k_bupdate_keep(Chunks ch, double* __restri
     if (threadIdx.x == 0) cpart[c] = rr;
 }
 
-// per instance: stopping test only (preconditioned path)
-__global__ __launch_bounds__(BT) void k_breduce_check(Insts in, const int* __restrict__ ico,
+// per instance: stopping test only (preconditioned path); all[0] (zeroed by
+// k_bupdate_keep) counts the instances left running
+__global__ __launch_bounds__(64) void k_breduce_check(Insts in, const int* __restrict__ ico,
                                                       const double* __restrict__ cpart, double* __restrict__ S,
                                                       int* __restrict__ I, int* __restrict__ all, double tol) {
-    int active = 0;
-    for (int j = threadIdx.x; j < in.n; j += BT) {
-        const int i = inst_at(in, j);
-        int* fl = I + (int64_t)i * NI;
-        if (fl[I_DONE]) continue;
-        double rr = 0.0;
-        for (int c = ico[i]; c < ico[i + 1]; c++) rr += cpart[c];
+    const int i = inst_at(in, blockIdx.x);
+    int* fl = I + (int64_t)i * NI;
+    if (fl[I_DONE]) return;
+    double sm[1];
+    wave_inst_sums<1>(ico, cpart, i, sm);
+    if (threadIdx.x == 0) {
+        const double rr = sm[0];
         double* s = S + (int64_t)i * NS;
         fl[I_ITERS] += 1;
         s[S_RR] = rr;
@@ -583,31 +607,22 @@ __global__ __launch_bounds__(BT) void k_breduce_check(Insts in, const int* __res
         if (rr < tol * tol * s[S_FF])
             fl[I_DONE] = 1;
         else
-            active++;
+            atomicAdd(all, 1);
     }
-    __shared__ int cnt;
-    if (threadIdx.x == 0) cnt = 0;
-    __syncthreads();
-    atomicAdd(&cnt, active);
-    __syncthreads();
-    if (threadIdx.x == 0) all[0] = cnt;
 }
 
 // per instance: flexible beta = z.(r - r_old) / rz_old
-__global__ __launch_bounds__(BT) void k_breduce_fbeta(Insts in, const int* __restrict__ ico,
+__global__ __launch_bounds__(64) void k_breduce_fbeta(Insts in, const int* __restrict__ ico,
                                                       const double* __restrict__ cpart, double* __restrict__ S,
                                                       const int* __restrict__ I) {
-    for (int j = threadIdx.x; j < in.n; j += BT) {
-        const int i = inst_at(in, j);
-        if (I[(int64_t)i * NI + I_DONE]) continue;
-        double zr = 0.0, zro = 0.0;
-        for (int c = ico[i]; c < ico[i + 1]; c++) {
-            zr += cpart[2 * c];
-            zro += cpart[2 * c + 1];
-        }
+    const int i = inst_at(in, blockIdx.x);
+    if (I[(int64_t)i * NI + I_DONE]) return;
+    double sm[2];
+    wave_inst_sums<2>(ico, cpart, i, sm);
+    if (threadIdx.x == 0) {
         double* s = S + (int64_t)i * NS;
-        s[S_BETA] = (zr - zro) / s[S_RZ];
-        s[S_RZ] = zr;
+        s[S_BETA] = (sm[0] - sm[1]) / s[S_RZ];
+        s[S_RZ] = sm[0];
     }
 }
 
@@ -642,13 +657,11 @@ __global__ __launch_bounds__(BT) void k_pow_scale(Chunks ch, const float* __rest
 }
 
 // per instance: lambda = |y| / |v_prev| (|v_prev| = 1 after the first pass), stored in C
-__global__ __launch_bounds__(BT) void k_pow_reduce(int n_inst, const int* __restrict__ ico,
+__global__ __launch_bounds__(64) void k_pow_reduce(int n_inst, const int* __restrict__ ico,
                                                    const double* __restrict__ cpart, double* __restrict__ C) {
-    for (int i = threadIdx.x; i < n_inst; i += BT) {
-        double yy = 0.0;
-        for (int c = ico[i]; c < ico[i + 1]; c++) yy += cpart[c];
-        C[(int64_t)i * NC + C_LMAX] = sqrt(yy);
-    }
+    double sm[1];
+    wave_inst_sums<1>(ico, cpart, blockIdx.x, sm);
+    if (threadIdx.x == 0) C[(int64_t)blockIdx.x * NC + C_LMAX] = sqrt(sm[0]);
 }
 
 // v = y / |y| per instance (the previous pass's norm)
@@ -924,16 +937,16 @@ __device__ __forceinline__ double lds_matvec(const double* __restrict__ Mx, doub
     return threadIdx.x < NB ? y : 0.0;
 }
 
+// The sweep over blocks k0..k1-1 (one workgroup of 256); rhs and x indexed from block k0
+// (x[(k - k0) NB + l]); rhs null: the unit vector of coarse DOF `unit` (a column of the
+// inverse).
 template <int NB>
-__global__ __launch_bounds__(256) void k_coarse_solve(const int* __restrict__ inst_cblk, const int* __restrict__ I,
-                                                      const double* __restrict__ dinv, const double* __restrict__ clow,
-                                                      const double* __restrict__ cg, const double* __restrict__ rhs,
-                                                      double* __restrict__ x) {
-    extern __shared__ double lds[];  // [2][CS_WORDS]
+__device__ __forceinline__ void coarse_sweep(int k0, int k1, const double* __restrict__ dinv,
+                                             const double* __restrict__ clow, const double* __restrict__ cg,
+                                             const double* __restrict__ rhs, int unit, double* __restrict__ x,
+                                             double* lds) {
     constexpr int nn = NB * NB;
-    const int i = blockIdx.x;
-    if (I[(int64_t)i * NI + I_DONE]) return;
-    const int k0 = inst_cblk[i], k1 = inst_cblk[i + 1], nb = k1 - k0;
+    const int nb = k1 - k0;
     const int tid = threadIdx.x, l = tid & 63;
     const bool sweeper = tid < 64;
     // ---- forward: z_k = b_k - G_k z_(k-1); a buffer holds CF blocks of G and then of b ----
@@ -944,8 +957,9 @@ __global__ __launch_bounds__(256) void k_coarse_solve(const int* __restrict__ in
         double* dst = lds + buf * CS_WORDS;
         const double* src = cg + (int64_t)kb * nn;
         for (int e = t0; e < (ke - kb) * nn; e += nt) dst[e] = src[e];
-        const double* rs = rhs + (int64_t)kb * NB;
-        for (int e = t0; e < (ke - kb) * NB; e += nt) dst[CF * nn + e] = rs[e];
+        const int64_t r0 = (int64_t)(kb - k0) * NB;
+        for (int e = t0; e < (ke - kb) * NB; e += nt)
+            dst[CF * nn + e] = rhs ? rhs[r0 + e] : (r0 + e == unit ? 1.0 : 0.0);
     };
     stage_f(0, 0, tid, 256);
     __syncthreads();
@@ -958,7 +972,7 @@ __global__ __launch_bounds__(256) void k_coarse_solve(const int* __restrict__ in
                 const double bk = l < NB ? G[CF * nn + (k - kb) * NB + l] : 0.0;
                 const double gz = k > k0 ? lds_matvec<NB, false>(G + (k - kb) * nn, z) : 0.0;
                 z = bk - gz;
-                if (l < NB) x[(int64_t)k * NB + l] = z;
+                if (l < NB) x[(int64_t)(k - k0) * NB + l] = z;
             }
         } else if (c + 1 < nchf) {
             stage_f(c + 1, (c + 1) & 1, tid - 64, 192);
@@ -977,7 +991,7 @@ __global__ __launch_bounds__(256) void k_coarse_solve(const int* __restrict__ in
             dst[2 * (k - kb) * nn + o] = dinv[(int64_t)k * nn + o];
             dst[(2 * (k - kb) + 1) * nn + o] = k + 1 < k1 ? clow[(int64_t)(k + 1) * nn + o] : 0.0;
         }
-        const double* zs = x + (int64_t)kb * NB;
+        const double* zs = x + (int64_t)(kb - k0) * NB;
         for (int e = t0; e < (ke - kb) * NB; e += nt) dst[2 * CB * nn + e] = zs[e];
     };
     stage_b(0, 0, tid, 256);
@@ -991,7 +1005,7 @@ __global__ __launch_bounds__(256) void k_coarse_solve(const int* __restrict__ in
                 double t = l < NB ? B[2 * CB * nn + (k - kb) * NB + l] : 0.0;
                 if (k < k1 - 1) t -= lds_matvec<NB, true>(B + (2 * (k - kb) + 1) * nn, xn);
                 xn = lds_matvec<NB, false>(B + 2 * (k - kb) * nn, t);
-                if (l < NB) x[(int64_t)k * NB + l] = xn;
+                if (l < NB) x[(int64_t)(k - k0) * NB + l] = xn;
             }
         } else if (c + 1 < nchb) {
             stage_b(c + 1, (c + 1) & 1, tid - 64, 192);
@@ -1000,15 +1014,75 @@ __global__ __launch_bounds__(256) void k_coarse_solve(const int* __restrict__ in
     }
 }
 
-int launch_coarse_solve(int NB, int n_inst, const int* inst_cblk, const int* I, const double* dinv,
-                        const double* clow, const double* cg, const double* rhs, double* x, hipStream_t s) {
+// x = A_c^-1 rhs per instance by the sweep; instances with a dense inverse (dense_off >= 0)
+// are left to k_coarse_dense_apply
+template <int NB>
+__global__ __launch_bounds__(256) void k_coarse_solve(const int* __restrict__ inst_cblk, const int* __restrict__ I,
+                                                      const int64_t* __restrict__ dense_off,
+                                                      const double* __restrict__ dinv, const double* __restrict__ clow,
+                                                      const double* __restrict__ cg, const double* __restrict__ rhs,
+                                                      double* __restrict__ x) {
+    extern __shared__ double lds[];  // [2][CS_WORDS]
+    const int i = blockIdx.x;
+    if (I[(int64_t)i * NI + I_DONE] || (dense_off && dense_off[i] >= 0)) return;
+    const int k0 = inst_cblk[i], k1 = inst_cblk[i + 1];
+    coarse_sweep<NB>(k0, k1, dinv, clow, cg, rhs + (int64_t)k0 * NB, -1, x + (int64_t)k0 * NB, lds);
+}
+
+// column blockIdx.x of the instance's A_c^-1 (n_c = (k1 - k0) NB rows) into Ainv
+template <int NB>
+__global__ __launch_bounds__(256) void k_coarse_inverse(int k0, int k1, const double* __restrict__ dinv,
+                                                        const double* __restrict__ clow, const double* __restrict__ cg,
+                                                        double* __restrict__ Ainv) {
+    extern __shared__ double lds[];  // [2][CS_WORDS]
+    const int64_t nc = (int64_t)(k1 - k0) * NB;
+    coarse_sweep<NB>(k0, k1, dinv, clow, cg, nullptr, (int)blockIdx.x, Ainv + nc * blockIdx.x, lds);
+}
+
+// x = A_c^-1 rhs for the instances with a dense inverse: one wave per coarse row (the
+// inverse is symmetric, so row r is read as its contiguous column r)
+__global__ __launch_bounds__(64) void k_coarse_dense_apply(int NB, const int* __restrict__ rows,
+                                                           const int* __restrict__ row_inst,
+                                                           const int* __restrict__ inst_cblk,
+                                                           const int64_t* __restrict__ dense_off,
+                                                           const int* __restrict__ I, const double* __restrict__ Ainv,
+                                                           const double* __restrict__ rhs, double* __restrict__ x) {
+    const int g = rows[blockIdx.x], i = row_inst[blockIdx.x];
+    if (I[(int64_t)i * NI + I_DONE]) return;
+    const int64_t base = (int64_t)inst_cblk[i] * NB, nc = (int64_t)(inst_cblk[i + 1] - inst_cblk[i]) * NB;
+    const double* A = Ainv + dense_off[i] + (g - base) * nc;
+    const double* b = rhs + base;
+    double acc = 0.0;
+    for (int64_t j = threadIdx.x; j < nc; j += 64) acc += A[j] * b[j];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (threadIdx.x == 0) x[g] = acc;
+}
+
+int launch_coarse_solve(int NB, int n_inst, const int* inst_cblk, const int* I, const int64_t* dense_off,
+                        const double* dinv, const double* clow, const double* cg, const double* rhs, double* x,
+                        hipStream_t s) {
     const size_t lds = 2 * CS_WORDS * sizeof(double);
     if (NB == 6)
-        k_coarse_solve<6><<<n_inst, 256, lds, s>>>(inst_cblk, I, dinv, clow, cg, rhs, x);
+        k_coarse_solve<6><<<n_inst, 256, lds, s>>>(inst_cblk, I, dense_off, dinv, clow, cg, rhs, x);
     else if (NB == 12)
-        k_coarse_solve<12><<<n_inst, 256, lds, s>>>(inst_cblk, I, dinv, clow, cg, rhs, x);
+        k_coarse_solve<12><<<n_inst, 256, lds, s>>>(inst_cblk, I, dense_off, dinv, clow, cg, rhs, x);
     else
-        k_coarse_solve<18><<<n_inst, 256, lds, s>>>(inst_cblk, I, dinv, clow, cg, rhs, x);
+        k_coarse_solve<18><<<n_inst, 256, lds, s>>>(inst_cblk, I, dense_off, dinv, clow, cg, rhs, x);
+    TLFEA_LAUNCH_CHECK();
+    return TLFEA_OK;
+}
+
+int launch_coarse_inverse(int NB, int k0, int k1, const double* dinv, const double* clow, const double* cg,
+                          double* Ainv, hipStream_t s) {
+    const size_t lds = 2 * CS_WORDS * sizeof(double);
+    const unsigned cols = (unsigned)((k1 - k0) * NB);
+    if (NB == 6)
+        k_coarse_inverse<6><<<cols, 256, lds, s>>>(k0, k1, dinv, clow, cg, Ainv);
+    else if (NB == 12)
+        k_coarse_inverse<12><<<cols, 256, lds, s>>>(k0, k1, dinv, clow, cg, Ainv);
+    else
+        k_coarse_inverse<18><<<cols, 256, lds, s>>>(k0, k1, dinv, clow, cg, Ainv);
     TLFEA_LAUNCH_CHECK();
     return TLFEA_OK;
 }
@@ -1041,7 +1115,8 @@ void free_batch(tlfea_batch* b) {
                     b->d_ux, b->d_uy, b->d_uz, b->d_node_plane, b->d_node_r, b->d_plane_off,
                     b->d_plane_nodes, b->d_plane_inst, b->d_plane_local, b->d_inst_cblk, b->d_cdiag,
                     b->d_clow, b->d_cg, b->d_crhs, b->d_cx, b->d_nvk, b->d_ny, b->d_ndv, b->d_nux,
-                    b->d_nuy, b->d_nuz, b->d_npart, b->d_ns, b->d_nmc, b->d_nrs};
+                    b->d_nuy, b->d_nuz, b->d_npart, b->d_ns, b->d_nmc, b->d_nrs, b->d_cdense_off, b->d_cdense,
+                    b->d_cd_row, b->d_cd_inst};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (b->s_own) (void)hipStreamDestroy(b->s_own);
@@ -1191,15 +1266,26 @@ int apply32(tlfea_batch* b, const float* p, float* q, const int* iflags, hipStre
 // z = Chebyshev(r) for the active instances, plus the coarse correction P A_c^-1 P^T r
 // (FP64) when it is set up -- the additive two-level preconditioner -- then z.r / z.r_old
 // chunk partials.
+// x_c = A_c^-1 rhs_c for every active instance: the block sweep, or the dense inverse
+int coarse_solve_all(tlfea_batch* b, hipStream_t s) {
+    if (int rc = launch_coarse_solve(b->cnb, b->n_inst, b->d_inst_cblk, b->d_i, b->d_cdense_off, b->d_cdiag,
+                                     b->d_clow, b->d_cg, b->d_crhs, b->d_cx, s))
+        return rc;
+    if (b->n_cd_rows > 0) {
+        k_coarse_dense_apply<<<b->n_cd_rows, 64, 0, s>>>(b->cnb, b->d_cd_row, b->d_cd_inst, b->d_inst_cblk,
+                                                         b->d_cdense_off, b->d_i, b->d_cdense, b->d_crhs, b->d_cx);
+        TLFEA_LAUNCH_CHECK();
+    }
+    return TLFEA_OK;
+}
+
 int precondition(tlfea_batch* b, hipStream_t s) {
     const Chunks ch = chunks(b);
     CoarseProlong cp{nullptr, nullptr, nullptr};
     if (b->coarse_on) {
         k_coarse_restrict<<<b->n_planes, 64, 0, s>>>(b->d_plane_off, b->d_plane_nodes, b->d_plane_inst, b->d_node_r,
                                                      b->d_i, b->d_r, b->d_crhs);
-        if (int rc = launch_coarse_solve(b->cnb, b->n_inst, b->d_inst_cblk, b->d_i, b->d_cdiag, b->d_clow, b->d_cg,
-                                         b->d_crhs, b->d_cx, s))
-            return rc;
+        if (int rc = coarse_solve_all(b, s)) return rc;
         cp = CoarseProlong{b->d_node_plane, b->d_node_r, b->d_cx};
     }
     k_cheb_init<<<n_chunk_launch(b), BT, 0, s>>>(ch, b->d_r, b->d_dinv32, b->d_cheb, b->d_i, b->d_r32, b->d_d32,
@@ -1218,11 +1304,13 @@ int precondition(tlfea_batch* b, hipStream_t s) {
 int pcg_iteration(tlfea_batch* b, double* u, double tol, hipStream_t s) {
     const Chunks ch = chunks(b);
     if (int rc = apply64(b, b->d_p, b->d_q, b->d_i, s)) return rc;
-    k_breduce_alpha<<<1, BT, 0, s>>>(insts(b), b->d_inst_chunk_off, b->d_part, b->d_s, b->d_i);
-    k_bupdate_keep<<<n_chunk_launch(b), BT, 0, s>>>(ch, u, b->d_r, b->d_rold, b->d_p, b->d_q, b->d_s, b->d_i, b->d_part);
-    k_breduce_check<<<1, BT, 0, s>>>(insts(b), b->d_inst_chunk_off, b->d_part, b->d_s, b->d_i, b->d_all, tol);
+    const Insts in = insts(b);
+    k_breduce_alpha<<<in.n, 64, 0, s>>>(in, b->d_inst_chunk_off, b->d_part, b->d_s, b->d_i);
+    k_bupdate_keep<<<n_chunk_launch(b), BT, 0, s>>>(ch, u, b->d_r, b->d_rold, b->d_p, b->d_q, b->d_s, b->d_i, b->d_part,
+                                                   b->d_all);
+    k_breduce_check<<<in.n, 64, 0, s>>>(in, b->d_inst_chunk_off, b->d_part, b->d_s, b->d_i, b->d_all, tol);
     if (int rc = precondition(b, s)) return rc;
-    k_breduce_fbeta<<<1, BT, 0, s>>>(insts(b), b->d_inst_chunk_off, b->d_part, b->d_s, b->d_i);
+    k_breduce_fbeta<<<in.n, 64, 0, s>>>(in, b->d_inst_chunk_off, b->d_part, b->d_s, b->d_i);
     k_bdir<<<n_chunk_launch(b), BT, 0, s>>>(ch, b->d_p, b->d_z, b->d_s, b->d_i);
     TLFEA_LAUNCH_CHECK();
     return TLFEA_OK;
@@ -1464,7 +1552,7 @@ extern "C" int tlfea_batch_precond_setup(tlfea_batch* b, int degree, double rati
     for (int it = 0; it < power_iters; it++) {
         if (int rc = apply32(b, b->d_d32, b->d_q32, nullptr, s)) return rc;
         k_pow_scale<<<b->n_chunks, BT, 0, s>>>(ch, b->d_q32, b->d_dinv32, b->d_mask, b->d_z32, b->d_part);
-        k_pow_reduce<<<1, BT, 0, s>>>(b->n_inst, b->d_inst_chunk_off, b->d_part, b->d_cheb);
+        k_pow_reduce<<<b->n_inst, 64, 0, s>>>(b->n_inst, b->d_inst_chunk_off, b->d_part, b->d_cheb);
         k_pow_norm<<<b->n_chunks, BT, 0, s>>>(ch, b->d_z32, b->d_cheb, b->d_d32);
     }
     k_cheb_params<<<grid_for(b->n_inst, 256), 256, 0, s>>>(b->n_inst, safety, ratio, b->d_cheb);
@@ -1520,6 +1608,68 @@ int instance_planes(const double* X, const double* Y, const double* Z, int n0, i
     for (int n = n0; n < n1; n++)
         plane[n - n0] = std::min(np - 1, (int)(((c[ax][n]) - lo[ax]) / ext * np));
     return np;
+}
+}  // namespace
+
+namespace {
+// Instances whose coarse matrix is applied as a dense inverse: the sweep's chain costs about
+// 1 us per block (forward and backward) whatever the instance's size, while the dense
+// product streams (n_c)^2 doubles; instances are taken longest chain first while that
+// lowers the modelled time of the coarse solve (the longest remaining chain against the
+// dense bytes at ~4 TB/s), within 4 GiB, and only for a clear (> 20%) gain.  Each chosen
+// inverse is formed column by column by the same sweep, all columns in parallel.
+int coarse_dense_setup(tlfea_batch* b, const std::vector<int>& inst_cblk, hipStream_t s) {
+    const int NB = b->cnb, n = b->n_inst;
+    constexpr double T_BLOCK = 1.0e-6, BW = 4.0e12;
+    constexpr double BUDGET = 4.0 * (1 << 30);
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    auto nbk = [&](int i) { return inst_cblk[i + 1] - inst_cblk[i]; };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return nbk(a) > nbk(c); });
+    const double sweep_only = nbk(order[0]) * T_BLOCK;
+    double best = sweep_only, bytes = 0.0;
+    int best_m = 0;
+    for (int m = 1; m <= n; m++) {
+        const double nc = (double)nbk(order[m - 1]) * NB;
+        bytes += nc * nc * sizeof(double);
+        if (bytes > BUDGET) break;
+        const double cost = bytes / BW + (m < n ? nbk(order[m]) * T_BLOCK : 0.0);
+        if (cost < best) {
+            best = cost;
+            best_m = m;
+        }
+    }
+    if (best_m == 0 || best > 0.8 * sweep_only) return TLFEA_OK;
+    std::vector<int64_t> off(n, -1);
+    std::vector<int> rows, rinst;
+    int64_t total = 0;
+    for (int j = 0; j < best_m; j++) {
+        const int i = order[j];
+        const int64_t nc = (int64_t)nbk(i) * NB;
+        off[i] = total;
+        total += nc * nc;
+        for (int64_t r = 0; r < nc; r++) {
+            rows.push_back((int)((int64_t)inst_cblk[i] * NB + r));
+            rinst.push_back(i);
+        }
+    }
+    int rc;
+    if ((rc = dalloc(&b->d_cdense, (size_t)total)) || (rc = dalloc(&b->d_cdense_off, (size_t)n)) ||
+        (rc = dalloc(&b->d_cd_row, rows.size())) || (rc = dalloc(&b->d_cd_inst, rinst.size())))
+        return rc;
+    TLFEA_HIP_TRY(hipMemcpyAsync(b->d_cdense_off, off.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(b->d_cd_row, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(b->d_cd_inst, rinst.data(), sizeof(int) * rinst.size(), hipMemcpyHostToDevice, s));
+    for (int j = 0; j < best_m; j++) {
+        const int i = order[j];
+        if ((rc = launch_coarse_inverse(NB, inst_cblk[i], inst_cblk[i + 1], b->d_cdiag, b->d_clow, b->d_cg,
+                                        b->d_cdense + off[i], s)))
+            return rc;
+    }
+    // the host tables above stay alive until the copies have run
+    TLFEA_HIP_TRY(hipStreamSynchronize(s));
+    b->n_cd_rows = (int)rows.size();
+    return TLFEA_OK;
 }
 }  // namespace
 
@@ -1641,8 +1791,13 @@ extern "C" int tlfea_batch_coarse_setup(tlfea_batch* b, int enable, int* n_coars
     // device tables
     for (void* q : {(void*)b->d_node_plane, (void*)b->d_node_r, (void*)b->d_plane_off, (void*)b->d_plane_nodes,
                     (void*)b->d_plane_inst, (void*)b->d_plane_local, (void*)b->d_inst_cblk, (void*)b->d_cdiag,
-                    (void*)b->d_clow, (void*)b->d_cg, (void*)b->d_crhs, (void*)b->d_cx})
+                    (void*)b->d_clow, (void*)b->d_cg, (void*)b->d_crhs, (void*)b->d_cx, (void*)b->d_cdense_off,
+                    (void*)b->d_cdense, (void*)b->d_cd_row, (void*)b->d_cd_inst})
         if (q) (void)hipFree(q);
+    b->d_cdense_off = nullptr;
+    b->d_cdense = nullptr;
+    b->d_cd_row = b->d_cd_inst = nullptr;
+    b->n_cd_rows = 0;
     b->d_node_plane = b->d_plane_off = b->d_plane_nodes = b->d_plane_inst = b->d_plane_local = b->d_inst_cblk =
         nullptr;
     b->d_node_r = b->d_cdiag = b->d_clow = b->d_cg = b->d_crhs = b->d_cx = nullptr;
@@ -1697,6 +1852,7 @@ extern "C" int tlfea_batch_coarse_setup(tlfea_batch* b, int enable, int* n_coars
             rc = TLFEA_EHIP;
         }
     }
+    if (rc == TLFEA_OK) rc = coarse_dense_setup(b, inst_cblk, s);
     const hipError_t es = hipStreamSynchronize(s);
     (void)hipFree(d_inst_pn);
     if (rc) return rc;
@@ -1713,9 +1869,7 @@ extern "C" int tlfea_batch_coarse_apply(tlfea_batch* b, const double* r, double*
     k_bclear_flags<<<grid_for(b->n_inst, 256), 256, 0, s>>>(b->n_inst, b->d_i);
     k_coarse_restrict<<<b->n_planes, 64, 0, s>>>(b->d_plane_off, b->d_plane_nodes, b->d_plane_inst, b->d_node_r,
                                                  b->d_i, r, b->d_crhs);
-    if (int rc = launch_coarse_solve(b->cnb, b->n_inst, b->d_inst_cblk, b->d_i, b->d_cdiag, b->d_clow, b->d_cg,
-                                     b->d_crhs, b->d_cx, s))
-        return rc;
+    if (int rc = coarse_solve_all(b, s)) return rc;
     const int64_t n3 = 3 * (int64_t)b->n_nodes;
     k_coarse_prolong64<<<grid_for(n3, 256), 256, 0, s>>>(n3, CoarseProlong{b->d_node_plane, b->d_node_r, b->d_cx},
                                                          z);
@@ -1817,7 +1971,7 @@ extern "C" int tlfea_batch_dot(tlfea_batch* b, const double* a, const double* c,
     hipStream_t s = as_stream(stream);
     const Chunks all{b->d_chunk_inst, b->d_chunk_beg, b->d_chunk_end, nullptr};
     k_bdot_ac<<<b->n_chunks, BT, 0, s>>>(all, a, c, b->d_part);
-    k_binst_sum<<<grid_for(b->n_inst, 256), 256, 0, s>>>(b->n_inst, b->d_inst_chunk_off, b->d_part, out);
+    k_binst_sum<<<b->n_inst, 64, 0, s>>>(b->n_inst, b->d_inst_chunk_off, b->d_part, out);
     TLFEA_LAUNCH_CHECK();
     return TLFEA_OK;
 }
@@ -1858,14 +2012,14 @@ extern "C" int tlfea_batch_cg_solve(tlfea_batch* b, const double* f, double* u, 
     if (!use_guess) TLFEA_HIP_TRY(hipMemsetAsync(u, 0, sizeof(double) * nd, s));
     // r = f - K u0
     if (int rc = apply64(b, u, b->d_q, nullptr, s)) return rc;
-    k_binit<<<b->n_chunks, BT, 0, s>>>(ch, f, b->d_q, b->d_mask, b->d_r, b->d_part);
-    k_breduce_init<<<1, BT, 0, s>>>(b->n_inst, b->d_inst_chunk_off, b->d_part, b->d_s, b->d_i, b->d_all, tol);
+    k_binit<<<b->n_chunks, BT, 0, s>>>(ch, f, b->d_q, b->d_mask, b->d_r, b->d_part, b->d_all);
+    k_breduce_init<<<b->n_inst, 64, 0, s>>>(b->n_inst, b->d_inst_chunk_off, b->d_part, b->d_s, b->d_i, b->d_all, tol);
     k_bzero_u<<<b->n_chunks, BT, 0, s>>>(ch, b->d_i, u);
     if (precond) {
         // z0 = M^-1 r0, rz = r0.z0, p = z0 (r_old = r0 makes the z.r_old partial harmless)
         TLFEA_HIP_TRY(hipMemcpyAsync(b->d_rold, b->d_r, sizeof(double) * nd, hipMemcpyDeviceToDevice, s));
         if (int rc = precondition(b, s)) return rc;
-        k_breduce_rz0<<<1, BT, 0, s>>>(insts(b), b->d_inst_chunk_off, b->d_part, b->d_s, b->d_i);
+        k_breduce_rz0<<<insts(b).n, 64, 0, s>>>(insts(b), b->d_inst_chunk_off, b->d_part, b->d_s, b->d_i);
         k_bdir<<<n_chunk_launch(b), BT, 0, s>>>(ch, b->d_p, b->d_z, b->d_s, b->d_i);
     } else {
         // p = s = 0 and fresh parity slots: the first iteration then takes p = r
@@ -1934,7 +2088,7 @@ extern "C" int tlfea_batch_cg_solve(tlfea_batch* b, const double* f, double* u, 
     if (!precond && active > 0) {
         // out of iterations: the residual of the last update has not been tested yet
         k_bdot_rr<<<b->n_chunks, BT, 0, s>>>(chunks(b), b->d_r, b->d_part);
-        k_breduce_tail<<<1, BT, 0, s>>>(b->n_inst, b->d_inst_chunk_off, b->d_part, b->d_s, b->d_i, tol);
+        k_breduce_tail<<<b->n_inst, 64, 0, s>>>(b->n_inst, b->d_inst_chunk_off, b->d_part, b->d_s, b->d_i, tol);
         TLFEA_LAUNCH_CHECK();
     }
     // the graph bakes in u: a later solve with another output buffer must re-capture
