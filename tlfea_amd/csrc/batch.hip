@@ -92,7 +92,11 @@ struct tlfea_batch {
     int64_t* d_cdense_off = nullptr;
     double* d_cdense = nullptr;
     int *d_cd_row = nullptr, *d_cd_inst = nullptr;
-    int n_cd_rows = 0;
+    int n_cd_rows = 0, n_cd_inst = 0;
+    std::vector<int64_t> h_cdense_off;  // per instance (-1 = sweep)
+    int* d_cinst_pn = nullptr;          // padded plane count per instance
+    std::vector<int> h_inst_cblk;
+    bool coarse_geom = false;           // the geometric tables above are built
     // batched SyncedNesterov (tlfea_batch_nesterov_solve): v_(k-1), y, dv = y - v_n (AoS
     // [3 n_nodes]), the displacement u_n + dt y (SoA), chunk partials [n_chunks][4],
     // per-instance scalars [n_inst][NSN], mass coefficients rho_i / dt, restart counts
@@ -475,11 +479,16 @@ __global__ __launch_bounds__(BT) void k_cheb_init(Chunks ch, const double* __res
     }
 }
 
-// step k >= 1: res -= q (q = K d), d = rho_k rho_{k-1} d + (2 rho_k / delta) D^-1 res, z += d
-__global__ __launch_bounds__(BT) void k_cheb_step(Chunks ch, int k, const float* __restrict__ q,
-                                                  const float* __restrict__ dinv, const double* __restrict__ C,
-                                                  const int* __restrict__ I, float* __restrict__ res,
-                                                  float* __restrict__ d, float* __restrict__ zc) {
+
+// The FP32 operator's gather fused with the Chebyshev step: q = the node's partial sums (d
+// on masked DOFs, the operator's identity rows), then res -= q, d = a d + b D^-1 res,
+// zc += d, as k_bgather<float> followed by k_cheb_step.
+__global__ __launch_bounds__(BT) void k_bgather_cheb(Chunks ch, const int* __restrict__ g_off,
+                                                     const int64_t* __restrict__ g_idx, const float* __restrict__ part,
+                                                     const unsigned char* __restrict__ mask, int k,
+                                                     const float* __restrict__ dinv, const double* __restrict__ C,
+                                                     const int* __restrict__ I, float* __restrict__ res,
+                                                     float* __restrict__ d, float* __restrict__ zc) {
     const int c = chunk_id(ch);
     const int inst = ch.inst[c];
     if (I[inst * NI + I_DONE]) return;
@@ -487,12 +496,21 @@ __global__ __launch_bounds__(BT) void k_cheb_step(Chunks ch, int k, const float*
     const float delta = (float)C[(int64_t)inst * NC + C_DELTA];
     const float rp = cheb_rho(sigma, k - 1), rk = cheb_rho(sigma, k);
     const float a = rk * rp, bcoef = 2.0f * rk / delta;
-    for (int64_t j = 3 * (int64_t)ch.beg[c] + threadIdx.x; j < 3 * (int64_t)ch.end[c]; j += BT) {
-        const float rj = res[j] - q[j];
-        res[j] = rj;
-        const float dj = a * d[j] + bcoef * dinv[j] * rj;
-        d[j] = dj;
-        zc[j] += dj;
+    for (int i = ch.beg[c] + threadIdx.x; i < ch.end[c]; i += BT) {
+        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+        gather_slots3(g_idx, part, g_off[i], g_off[i + 1], s0, s1, s2);
+        const float sv[3] = {s0, s1, s2};
+#pragma unroll
+        for (int e = 0; e < 3; e++) {
+            const int64_t j = 3 * (int64_t)i + e;
+            const float dold = d[j];
+            const float qj = mask[j] ? dold : sv[e];
+            const float rj = res[j] - qj;
+            res[j] = rj;
+            const float dj = a * dold + bcoef * dinv[j] * rj;
+            d[j] = dj;
+            zc[j] += dj;
+        }
     }
 }
 
@@ -746,11 +764,14 @@ __global__ void k_coarse_probe_vec(int n_nodes, const int* __restrict__ node_pla
 }
 
 // the 6 sums of P^T v over plane p, in lane order then by wave reduction (64 threads)
+// (NT threads: a workgroup of NT; with NT > 64 the waves' sums combine through LDS and
+// every thread returns the total)
+template <int NT = 64>
 __device__ __forceinline__ void plane_restrict(int p, const int* __restrict__ plane_off,
                                                const int* __restrict__ plane_nodes, const double* __restrict__ node_r,
                                                const double* __restrict__ v, double out[6]) {
     double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int k = plane_off[p] + (int)threadIdx.x; k < plane_off[p + 1]; k += 64) {
+    for (int k = plane_off[p] + (int)threadIdx.x; k < plane_off[p + 1]; k += NT) {
         const int n = plane_nodes[k];
         const double* r = node_r + 3 * (int64_t)n;
         const double a = v[3 * (int64_t)n], b = v[3 * (int64_t)n + 1], c = v[3 * (int64_t)n + 2];
@@ -767,6 +788,20 @@ __device__ __forceinline__ void plane_restrict(int p, const int* __restrict__ pl
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
         out[i] = x;
+    }
+    if constexpr (NT > 64) {
+        __shared__ double red[NT / 64][6];
+        if ((threadIdx.x & 63) == 0)
+#pragma unroll
+            for (int i = 0; i < 6; i++) red[threadIdx.x >> 6][i] = out[i];
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            double x = 0.0;
+#pragma unroll
+            for (int w = 0; w < NT / 64; w++) x += red[w][i];
+            out[i] = x;
+        }
     }
 }
 
@@ -803,28 +838,36 @@ __global__ __launch_bounds__(64) void k_coarse_probe_store(int W, int period, in
     for (int i = 0; i < 6; i++) dst[(row0 + i) * NB + col] = out[i];
 }
 
-// In-place inverse of the SPD NB x NB matrix m (LDS) by Cholesky: thread 0 factors, then
-// lanes 0..NB-1 solve one column each.  A pivot at or below 1e-10 of its diagonal entry
+// In-place inverse of the SPD NB x NB matrix m (LDS) by Cholesky: the workgroup factors
+// together, then lanes 0..NB-1 solve one column each.  A pivot at or below 1e-10 of its diagonal entry
 // drops that coarse DOF (its row and column of the inverse are zero: no rigid mode of a
 // plane without the free nodes to carry it).  Called by all threads of the workgroup.
 __device__ void spd_inverse(double* m, double* L, int* dead, int NB) {
-    if (threadIdx.x == 0) {
-        for (int i = 0; i < NB * NB; i++) L[i] = 0.0;
-        for (int c = 0; c < NB; c++) {
-            double d = m[c * NB + c];
-            const double d0 = d;
-            for (int k = 0; k < c; k++) d -= L[c * NB + k] * L[c * NB + k];
-            dead[c] = (!(d > 1e-10 * fabs(d0)) || d0 <= 0.0) ? 1 : 0;
-            const double lc = dead[c] ? 0.0 : sqrt(d);
-            L[c * NB + c] = lc;
-            for (int i = c + 1; i < NB; i++) {
-                double v = m[i * NB + c];
-                for (int k = 0; k < c; k++) v -= L[i * NB + k] * L[c * NB + k];
-                L[i * NB + c] = dead[c] ? 0.0 : v / lc;
-            }
-        }
-    }
+    // right-looking Cholesky, the workgroup's threads over each step's column and trailing
+    // block: L starts as the lower triangle of m and is updated in place
+    __shared__ double d0s[CNB_MAX];
+    const int t = threadIdx.x, nt = blockDim.x;
+    for (int e = t; e < NB * NB; e += nt) L[e] = (e % NB) <= (e / NB) ? m[e] : 0.0;
+    if (t < NB) d0s[t] = m[t * NB + t];
     __syncthreads();
+    for (int c = 0; c < NB; c++) {
+        const double d = L[c * NB + c], d0 = d0s[c];
+        const bool dd = !(d > 1e-10 * fabs(d0)) || d0 <= 0.0;
+        const double lc = dd ? 0.0 : sqrt(d);
+        __syncthreads();  // every thread has read the pivot
+        if (t == 0) {
+            dead[c] = dd ? 1 : 0;
+            L[c * NB + c] = lc;
+        }
+        for (int i = c + 1 + t; i < NB; i += nt) L[i * NB + c] = dd ? 0.0 : L[i * NB + c] / lc;
+        __syncthreads();
+        const int m2 = NB - c - 1;
+        for (int e = t; e < m2 * m2; e += nt) {
+            const int i = c + 1 + e / m2, j = c + 1 + e % m2;
+            if (j <= i) L[i * NB + j] -= L[i * NB + c] * L[j * NB + c];
+        }
+        __syncthreads();
+    }
     // column c of the inverse: L y = e_c, L^T x = y (dead rows skipped)
     const int c = threadIdx.x;
     if (c < NB) {
@@ -849,17 +892,20 @@ __global__ __launch_bounds__(64) void k_coarse_factor(int NB, const int* __restr
                                                       double* __restrict__ cdiag, const double* __restrict__ clow,
                                                       double* __restrict__ cg) {
     __shared__ double Dprev[CNB_MAX * CNB_MAX], M[CNB_MAX * CNB_MAX], T[CNB_MAX * CNB_MAX],
-        L[CNB_MAX * CNB_MAX];
+        L[CNB_MAX * CNB_MAX], A[CNB_MAX * CNB_MAX];
     __shared__ int dead[CNB_MAX];
     const int i = blockIdx.x;
     const int k0 = inst_cblk[i], k1 = inst_cblk[i + 1];
     const int nn = NB * NB;
     for (int k = k0; k < k1; k++) {
         double* D = cdiag + (int64_t)k * nn;
-        for (int e = threadIdx.x; e < nn; e += 64) M[e] = D[e];
+        const double* Ag = clow + (int64_t)k * nn;  // A_(k, k-1), staged in LDS
+        for (int e = threadIdx.x; e < nn; e += 64) {
+            M[e] = D[e];
+            A[e] = k > k0 ? Ag[e] : 0.0;
+        }
         __syncthreads();
         if (k > k0) {
-            const double* A = clow + (int64_t)k * nn;  // A_(k, k-1)
             // G = A Dprev^-1 (Dprev holds the inverse), then M -= G A^T
             for (int e = threadIdx.x; e < nn; e += 64) {
                 const int r = e / NB, c = e % NB;
@@ -903,7 +949,7 @@ __global__ __launch_bounds__(64) void k_coarse_factor(int NB, const int* __restr
 }
 
 // coarse right-hand side: P^T r per plane (instances already converged skipped)
-__global__ __launch_bounds__(64) void k_coarse_restrict(const int* __restrict__ plane_off,
+__global__ __launch_bounds__(256) void k_coarse_restrict(const int* __restrict__ plane_off,
                                                         const int* __restrict__ plane_nodes,
                                                         const int* __restrict__ plane_inst,
                                                         const double* __restrict__ node_r,
@@ -912,7 +958,7 @@ __global__ __launch_bounds__(64) void k_coarse_restrict(const int* __restrict__ 
     const int p = blockIdx.x;
     if (I[(int64_t)plane_inst[p] * NI + I_DONE]) return;
     double out[6];
-    plane_restrict(p, plane_off, plane_nodes, node_r, r, out);
+    plane_restrict<256>(p, plane_off, plane_nodes, node_r, r, out);
     if (threadIdx.x < 6) rhs[6 * (int64_t)p + threadIdx.x] = out[threadIdx.x];
 }
 
@@ -1039,24 +1085,35 @@ __global__ __launch_bounds__(256) void k_coarse_inverse(int k0, int k1, const do
     coarse_sweep<NB>(k0, k1, dinv, clow, cg, nullptr, (int)blockIdx.x, Ainv + nc * blockIdx.x, lds);
 }
 
-// x = A_c^-1 rhs for the instances with a dense inverse: one wave per coarse row (the
-// inverse is symmetric, so row r is read as its contiguous column r)
-__global__ __launch_bounds__(64) void k_coarse_dense_apply(int NB, const int* __restrict__ rows,
-                                                           const int* __restrict__ row_inst,
-                                                           const int* __restrict__ inst_cblk,
-                                                           const int64_t* __restrict__ dense_off,
-                                                           const int* __restrict__ I, const double* __restrict__ Ainv,
-                                                           const double* __restrict__ rhs, double* __restrict__ x) {
-    const int g = rows[blockIdx.x], i = row_inst[blockIdx.x];
+// x = A_c^-1 rhs for the instances with a dense inverse: one wave per coarse row, four rows
+// per workgroup (the inverse is symmetric, so row r is read as its contiguous column r)
+__global__ __launch_bounds__(256) void k_coarse_dense_apply(int NB, int n_rows, const int* __restrict__ rows,
+                                                            const int* __restrict__ row_inst,
+                                                            const int* __restrict__ inst_cblk,
+                                                            const int64_t* __restrict__ dense_off,
+                                                            const int* __restrict__ I, const double* __restrict__ Ainv,
+                                                            const double* __restrict__ rhs, double* __restrict__ x) {
+    const int w = (int)(blockIdx.x * 4 + (threadIdx.x >> 6)), l = threadIdx.x & 63;
+    if (w >= n_rows) return;
+    const int g = rows[w], i = row_inst[w];
     if (I[(int64_t)i * NI + I_DONE]) return;
     const int64_t base = (int64_t)inst_cblk[i] * NB, nc = (int64_t)(inst_cblk[i + 1] - inst_cblk[i]) * NB;
     const double* A = Ainv + dense_off[i] + (g - base) * nc;
     const double* b = rhs + base;
-    double acc = 0.0;
-    for (int64_t j = threadIdx.x; j < nc; j += 64) acc += A[j] * b[j];
+    // four independent sums in flight, added in a fixed order
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int64_t j = l;
+    for (; j + 192 < nc; j += 256) {
+        a0 += A[j] * b[j];
+        a1 += A[j + 64] * b[j + 64];
+        a2 += A[j + 128] * b[j + 128];
+        a3 += A[j + 192] * b[j + 192];
+    }
+    for (; j < nc; j += 64) a0 += A[j] * b[j];
+    double acc = (a0 + a1) + (a2 + a3);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-    if (threadIdx.x == 0) x[g] = acc;
+    if (l == 0) x[g] = acc;
 }
 
 int launch_coarse_solve(int NB, int n_inst, const int* inst_cblk, const int* I, const int64_t* dense_off,
@@ -1116,7 +1173,7 @@ void free_batch(tlfea_batch* b) {
                     b->d_plane_nodes, b->d_plane_inst, b->d_plane_local, b->d_inst_cblk, b->d_cdiag,
                     b->d_clow, b->d_cg, b->d_crhs, b->d_cx, b->d_nvk, b->d_ny, b->d_ndv, b->d_nux,
                     b->d_nuy, b->d_nuz, b->d_npart, b->d_ns, b->d_nmc, b->d_nrs, b->d_cdense_off, b->d_cdense,
-                    b->d_cd_row, b->d_cd_inst};
+                    b->d_cd_row, b->d_cd_inst, b->d_cinst_pn};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (b->s_own) (void)hipStreamDestroy(b->s_own);
@@ -1244,18 +1301,21 @@ int apply64(tlfea_batch* b, const double* p, double* q, const int* iflags, hipSt
     return TLFEA_OK;
 }
 
-// FP32 batched operator q = (K + c M) p (masked; the mass term when set up), no dot partials.
-int apply32(tlfea_batch* b, const float* p, float* q, const int* iflags, hipStream_t s) {
+// Block partials of the FP32 batched operator (K + c M) p (masked; the mass term when set up).
+int tangent32_partials(tlfea_batch* b, const float* p, const int* iflags, hipStream_t s) {
     TanMat<float> tm{0.0f, 0.0f, b->d_elem_inst, b->d_lam32, b->d_mu32, iflags, NI};
     tm.inst_mc = b->mass_on ? b->d_mc32 : nullptr;
     tm.mass_exact = b->mass_exact;
     tm.at_rest = b->moved ? 0 : 1;
     tm.affine = b->affine;
-    if (int rc = launch_tangent_partials<float>(b->plan, pos_x(b), pos_y(b), pos_z(b), rest_x(b), rest_y(b),
-                                                rest_z(b), b->d_jinv32,
-                                                b->d_detJ32, b->quad, tm, b->d_mask, p, nullptr, s,
-                                                b->compact ? b->d_blk_list : nullptr, b->n_blk_list))
-        return rc;
+    return launch_tangent_partials<float>(b->plan, pos_x(b), pos_y(b), pos_z(b), rest_x(b), rest_y(b), rest_z(b),
+                                          b->d_jinv32, b->d_detJ32, b->quad, tm, b->d_mask, p, nullptr, s,
+                                          b->compact ? b->d_blk_list : nullptr, b->n_blk_list);
+}
+
+// FP32 batched operator q = (K + c M) p (masked; the mass term when set up), no dot partials.
+int apply32(tlfea_batch* b, const float* p, float* q, const int* iflags, hipStream_t s) {
+    if (int rc = tangent32_partials(b, p, iflags, s)) return rc;
     k_bgather<float><<<n_chunk_launch(b), BT, 0, s>>>(chunks(b), b->plan->d_g_off, b->plan->d_g_idx,
                                                 reinterpret_cast<const float*>(b->plan->d_part), b->d_mask, p, q,
                                                 nullptr, iflags);
@@ -1268,12 +1328,14 @@ int apply32(tlfea_batch* b, const float* p, float* q, const int* iflags, hipStre
 // chunk partials.
 // x_c = A_c^-1 rhs_c for every active instance: the block sweep, or the dense inverse
 int coarse_solve_all(tlfea_batch* b, hipStream_t s) {
-    if (int rc = launch_coarse_solve(b->cnb, b->n_inst, b->d_inst_cblk, b->d_i, b->d_cdense_off, b->d_cdiag,
-                                     b->d_clow, b->d_cg, b->d_crhs, b->d_cx, s))
-        return rc;
+    if (b->n_cd_inst < b->n_inst)  // some instances sweep
+        if (int rc = launch_coarse_solve(b->cnb, b->n_inst, b->d_inst_cblk, b->d_i, b->d_cdense_off, b->d_cdiag,
+                                         b->d_clow, b->d_cg, b->d_crhs, b->d_cx, s))
+            return rc;
     if (b->n_cd_rows > 0) {
-        k_coarse_dense_apply<<<b->n_cd_rows, 64, 0, s>>>(b->cnb, b->d_cd_row, b->d_cd_inst, b->d_inst_cblk,
-                                                         b->d_cdense_off, b->d_i, b->d_cdense, b->d_crhs, b->d_cx);
+        k_coarse_dense_apply<<<(b->n_cd_rows + 3) / 4, 256, 0, s>>>(b->cnb, b->n_cd_rows, b->d_cd_row, b->d_cd_inst,
+                                                                    b->d_inst_cblk, b->d_cdense_off, b->d_i,
+                                                                    b->d_cdense, b->d_crhs, b->d_cx);
         TLFEA_LAUNCH_CHECK();
     }
     return TLFEA_OK;
@@ -1283,7 +1345,7 @@ int precondition(tlfea_batch* b, hipStream_t s) {
     const Chunks ch = chunks(b);
     CoarseProlong cp{nullptr, nullptr, nullptr};
     if (b->coarse_on) {
-        k_coarse_restrict<<<b->n_planes, 64, 0, s>>>(b->d_plane_off, b->d_plane_nodes, b->d_plane_inst, b->d_node_r,
+        k_coarse_restrict<<<b->n_planes, 256, 0, s>>>(b->d_plane_off, b->d_plane_nodes, b->d_plane_inst, b->d_node_r,
                                                      b->d_i, b->d_r, b->d_crhs);
         if (int rc = coarse_solve_all(b, s)) return rc;
         cp = CoarseProlong{b->d_node_plane, b->d_node_r, b->d_cx};
@@ -1291,9 +1353,10 @@ int precondition(tlfea_batch* b, hipStream_t s) {
     k_cheb_init<<<n_chunk_launch(b), BT, 0, s>>>(ch, b->d_r, b->d_dinv32, b->d_cheb, b->d_i, b->d_r32, b->d_d32,
                                                 b->d_z32);
     for (int k = 1; k < b->cheb_degree; k++) {
-        if (int rc = apply32(b, b->d_d32, b->d_q32, b->d_i, s)) return rc;
-        k_cheb_step<<<n_chunk_launch(b), BT, 0, s>>>(ch, k, b->d_q32, b->d_dinv32, b->d_cheb, b->d_i, b->d_r32, b->d_d32,
-                                                b->d_z32);
+        if (int rc = tangent32_partials(b, b->d_d32, b->d_i, s)) return rc;
+        k_bgather_cheb<<<n_chunk_launch(b), BT, 0, s>>>(ch, b->plan->d_g_off, b->plan->d_g_idx,
+                                                       reinterpret_cast<const float*>(b->plan->d_part), b->d_mask, k,
+                                                       b->d_dinv32, b->d_cheb, b->d_i, b->d_r32, b->d_d32, b->d_z32);
     }
     k_cheb_fin<<<n_chunk_launch(b), BT, 0, s>>>(ch, b->d_z32, b->d_r, b->d_rold, b->d_i, b->d_z, b->d_part, cp);
     TLFEA_LAUNCH_CHECK();
@@ -1618,10 +1681,12 @@ namespace {
 // lowers the modelled time of the coarse solve (the longest remaining chain against the
 // dense bytes at ~4 TB/s), within 4 GiB, and only for a clear (> 20%) gain.  Each chosen
 // inverse is formed column by column by the same sweep, all columns in parallel.
-int coarse_dense_setup(tlfea_batch* b, const std::vector<int>& inst_cblk, hipStream_t s) {
+int coarse_dense_select(tlfea_batch* b, hipStream_t s) {
     const int NB = b->cnb, n = b->n_inst;
+    const std::vector<int>& inst_cblk = b->h_inst_cblk;
     constexpr double T_BLOCK = 1.0e-6, BW = 4.0e12;
     constexpr double BUDGET = 4.0 * (1 << 30);
+    b->h_cdense_off.assign(n, -1);
     std::vector<int> order(n);
     for (int i = 0; i < n; i++) order[i] = i;
     auto nbk = [&](int i) { return inst_cblk[i + 1] - inst_cblk[i]; };
@@ -1640,13 +1705,12 @@ int coarse_dense_setup(tlfea_batch* b, const std::vector<int>& inst_cblk, hipStr
         }
     }
     if (best_m == 0 || best > 0.8 * sweep_only) return TLFEA_OK;
-    std::vector<int64_t> off(n, -1);
     std::vector<int> rows, rinst;
     int64_t total = 0;
     for (int j = 0; j < best_m; j++) {
         const int i = order[j];
         const int64_t nc = (int64_t)nbk(i) * NB;
-        off[i] = total;
+        b->h_cdense_off[i] = total;
         total += nc * nc;
         for (int64_t r = 0; r < nc; r++) {
             rows.push_back((int)((int64_t)inst_cblk[i] * NB + r));
@@ -1657,30 +1721,34 @@ int coarse_dense_setup(tlfea_batch* b, const std::vector<int>& inst_cblk, hipStr
     if ((rc = dalloc(&b->d_cdense, (size_t)total)) || (rc = dalloc(&b->d_cdense_off, (size_t)n)) ||
         (rc = dalloc(&b->d_cd_row, rows.size())) || (rc = dalloc(&b->d_cd_inst, rinst.size())))
         return rc;
-    TLFEA_HIP_TRY(hipMemcpyAsync(b->d_cdense_off, off.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(b->d_cdense_off, b->h_cdense_off.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice,
+                                 s));
     TLFEA_HIP_TRY(hipMemcpyAsync(b->d_cd_row, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice, s));
     TLFEA_HIP_TRY(hipMemcpyAsync(b->d_cd_inst, rinst.data(), sizeof(int) * rinst.size(), hipMemcpyHostToDevice, s));
-    for (int j = 0; j < best_m; j++) {
-        const int i = order[j];
-        if ((rc = launch_coarse_inverse(NB, inst_cblk[i], inst_cblk[i + 1], b->d_cdiag, b->d_clow, b->d_cg,
-                                        b->d_cdense + off[i], s)))
-            return rc;
-    }
     // the host tables above stay alive until the copies have run
     TLFEA_HIP_TRY(hipStreamSynchronize(s));
     b->n_cd_rows = (int)rows.size();
+    b->n_cd_inst = best_m;
+    return TLFEA_OK;
+}
+
+// The chosen instances' A_c^-1, column by column (after the factorisation).
+int coarse_dense_inverses(tlfea_batch* b, hipStream_t s) {
+    for (int i = 0; i < b->n_inst; i++) {
+        if (b->h_cdense_off.empty() || b->h_cdense_off[i] < 0) continue;
+        if (int rc = launch_coarse_inverse(b->cnb, b->h_inst_cblk[i], b->h_inst_cblk[i + 1], b->d_cdiag, b->d_clow,
+                                           b->d_cg, b->d_cdense + b->h_cdense_off[i], s))
+            return rc;
+    }
     return TLFEA_OK;
 }
 }  // namespace
 
-extern "C" int tlfea_batch_coarse_setup(tlfea_batch* b, int enable, int* n_coarse, void* stream) {
-    TLFEA_REQUIRE(b != nullptr, "null batch");
-    drop_graph(b);
-    full_lists(b);
-    b->coarse_on = false;
-    if (!enable) return TLFEA_OK;
-    TLFEA_REQUIRE(b->cheb_degree > 0, "the coarse correction runs inside the Chebyshev stage: set that up first");
-    hipStream_t s = as_stream(stream);
+namespace {
+// The geometric part of the coarse space, built once per batch (the rest positions and
+// fixed DOFs do not change after creation): node planes, rigid-mode offsets, plane tables,
+// the block layout and the choice of dense inverses, with the device arrays they need.
+int coarse_geometry(tlfea_batch* b, hipStream_t s) {
     const int N = b->n_nodes, NE = b->n_elem, nn = b->nn;
     // host staging without zero fill (every entry is written before it is read)
     std::unique_ptr<double[]> X(new double[N]), Y(new double[N]), Z(new double[N]);
@@ -1792,12 +1860,13 @@ extern "C" int tlfea_batch_coarse_setup(tlfea_batch* b, int enable, int* n_coars
     for (void* q : {(void*)b->d_node_plane, (void*)b->d_node_r, (void*)b->d_plane_off, (void*)b->d_plane_nodes,
                     (void*)b->d_plane_inst, (void*)b->d_plane_local, (void*)b->d_inst_cblk, (void*)b->d_cdiag,
                     (void*)b->d_clow, (void*)b->d_cg, (void*)b->d_crhs, (void*)b->d_cx, (void*)b->d_cdense_off,
-                    (void*)b->d_cdense, (void*)b->d_cd_row, (void*)b->d_cd_inst})
+                    (void*)b->d_cdense, (void*)b->d_cd_row, (void*)b->d_cd_inst, (void*)b->d_cinst_pn})
         if (q) (void)hipFree(q);
+    b->d_cinst_pn = nullptr;
     b->d_cdense_off = nullptr;
     b->d_cdense = nullptr;
     b->d_cd_row = b->d_cd_inst = nullptr;
-    b->n_cd_rows = 0;
+    b->n_cd_rows = b->n_cd_inst = 0;
     b->d_node_plane = b->d_plane_off = b->d_plane_nodes = b->d_plane_inst = b->d_plane_local = b->d_inst_cblk =
         nullptr;
     b->d_node_r = b->d_cdiag = b->d_clow = b->d_cg = b->d_crhs = b->d_cx = nullptr;
@@ -1821,16 +1890,29 @@ extern "C" int tlfea_batch_coarse_setup(tlfea_batch* b, int enable, int* n_coars
     TLFEA_HIP_TRY(hipMemcpy(b->d_plane_local, plane_local.data(), sizeof(int) * P, hipMemcpyHostToDevice));
     TLFEA_HIP_TRY(hipMemcpy(b->d_inst_cblk, inst_cblk.data(), sizeof(int) * (b->n_inst + 1), hipMemcpyHostToDevice));
     TLFEA_HIP_TRY(hipMemcpy(d_inst_pn, inst_pn.data(), sizeof(int) * b->n_inst, hipMemcpyHostToDevice));
-    TLFEA_HIP_TRY(hipMemset(b->d_cdiag, 0, sizeof(double) * nblk * bsz));
-    TLFEA_HIP_TRY(hipMemset(b->d_clow, 0, sizeof(double) * nblk * bsz));
-    TLFEA_HIP_TRY(hipMemset(b->d_cx, 0, sizeof(double) * 6 * (size_t)P));
     b->cw = W;
     b->cnb = NB;
     b->n_planes = P;
     b->n_cblocks = (int)nblk;
-    // P^T A P (A = K + c M when a mass term is set) by probing: planes coloured by local index mod 2W+1 are further apart than any
-    // element reaches, so each coloured rigid mode's image restricted to a plane belongs to
-    // the one coloured plane within W of it
+    b->d_cinst_pn = d_inst_pn;
+    b->h_inst_cblk = inst_cblk;
+    if ((rc = coarse_dense_select(b, s))) return rc;
+    b->coarse_geom = true;
+    return TLFEA_OK;
+}
+
+// The operator-dependent part, every call: A_c = P^T A P by probing, its block
+// factorisation and the dense inverses.
+int coarse_numeric(tlfea_batch* b, hipStream_t s) {
+    const int N = b->n_nodes, W = b->cw, NB = b->cnb, P = b->n_planes;
+    const size_t bsz = (size_t)NB * NB, nblk = (size_t)b->n_cblocks;
+    int rc = TLFEA_OK;
+    TLFEA_HIP_TRY(hipMemsetAsync(b->d_cdiag, 0, sizeof(double) * nblk * bsz, s));
+    TLFEA_HIP_TRY(hipMemsetAsync(b->d_clow, 0, sizeof(double) * nblk * bsz, s));
+    TLFEA_HIP_TRY(hipMemsetAsync(b->d_cx, 0, sizeof(double) * 6 * (size_t)P, s));
+    // P^T A P (A = K + c M when a mass term is set) by probing: planes coloured by local
+    // index mod 2W+1 are further apart than any element reaches, so each coloured rigid
+    // mode's image restricted to a plane belongs to the one coloured plane within W of it
     const int period = 2 * W + 1;
     for (int color = 0; color < period && rc == TLFEA_OK; color++)
         for (int j = 0; j < 6 && rc == TLFEA_OK; j++) {
@@ -1838,8 +1920,8 @@ extern "C" int tlfea_batch_coarse_setup(tlfea_batch* b, int enable, int* n_coars
                                                                 period, color, j, b->d_p);
             if ((rc = apply64(b, b->d_p, b->d_q, nullptr, s))) break;
             k_coarse_probe_store<<<P, 64, 0, s>>>(W, period, color, j, b->d_plane_off, b->d_plane_nodes,
-                                                 b->d_plane_local, b->d_plane_inst, d_inst_pn, b->d_node_r, b->d_q,
-                                                 b->d_cdiag, b->d_clow);
+                                                 b->d_plane_local, b->d_plane_inst, b->d_cinst_pn, b->d_node_r,
+                                                 b->d_q, b->d_cdiag, b->d_clow);
             if (hipGetLastError() != hipSuccess) {
                 set_error("coarse probe launch failed");
                 rc = TLFEA_EHIP;
@@ -1852,12 +1934,28 @@ extern "C" int tlfea_batch_coarse_setup(tlfea_batch* b, int enable, int* n_coars
             rc = TLFEA_EHIP;
         }
     }
-    if (rc == TLFEA_OK) rc = coarse_dense_setup(b, inst_cblk, s);
+    if (rc == TLFEA_OK) rc = coarse_dense_inverses(b, s);
     const hipError_t es = hipStreamSynchronize(s);
-    (void)hipFree(d_inst_pn);
     if (rc) return rc;
     TLFEA_HIP_TRY(es);
-    if (n_coarse) *n_coarse = 6 * P;
+    return TLFEA_OK;
+}
+}  // namespace
+
+extern "C" int tlfea_batch_coarse_setup(tlfea_batch* b, int enable, int* n_coarse, void* stream) {
+    TLFEA_REQUIRE(b != nullptr, "null batch");
+    drop_graph(b);
+    full_lists(b);
+    b->coarse_on = false;
+    if (!enable) return TLFEA_OK;
+    TLFEA_REQUIRE(b->cheb_degree > 0, "the coarse correction runs inside the Chebyshev stage: set that up first");
+    hipStream_t s = as_stream(stream);
+    if (!b->coarse_geom) {
+        TLFEA_HIP_TRY(hipStreamSynchronize(s));
+        if (int rc = coarse_geometry(b, s)) return rc;
+    }
+    if (int rc = coarse_numeric(b, s)) return rc;
+    if (n_coarse) *n_coarse = 6 * b->n_planes;
     b->coarse_on = true;
     return TLFEA_OK;
 }
@@ -1867,7 +1965,7 @@ extern "C" int tlfea_batch_coarse_apply(tlfea_batch* b, const double* r, double*
     TLFEA_REQUIRE(b->coarse_on, "the coarse correction is not set up");
     hipStream_t s = as_stream(stream);
     k_bclear_flags<<<grid_for(b->n_inst, 256), 256, 0, s>>>(b->n_inst, b->d_i);
-    k_coarse_restrict<<<b->n_planes, 64, 0, s>>>(b->d_plane_off, b->d_plane_nodes, b->d_plane_inst, b->d_node_r,
+    k_coarse_restrict<<<b->n_planes, 256, 0, s>>>(b->d_plane_off, b->d_plane_nodes, b->d_plane_inst, b->d_node_r,
                                                  b->d_i, r, b->d_crhs);
     if (int rc = coarse_solve_all(b, s)) return rc;
     const int64_t n3 = 3 * (int64_t)b->n_nodes;
