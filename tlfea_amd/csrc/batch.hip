@@ -97,6 +97,7 @@ struct tlfea_batch {
     int* d_cinst_pn = nullptr;          // padded plane count per instance
     std::vector<int> h_inst_cblk;
     bool coarse_geom = false;           // the geometric tables above are built
+    bool restrict_wide = false;         // planes of 512 free nodes or more on average
     // batched SyncedNesterov (tlfea_batch_nesterov_solve): v_(k-1), y, dv = y - v_n (AoS
     // [3 n_nodes]), the displacement u_n + dt y (SoA), chunk partials [n_chunks][4],
     // per-instance scalars [n_inst][NSN], mass coefficients rho_i / dt, restart counts
@@ -949,7 +950,8 @@ __global__ __launch_bounds__(64) void k_coarse_factor(int NB, const int* __restr
 }
 
 // coarse right-hand side: P^T r per plane (instances already converged skipped)
-__global__ __launch_bounds__(256) void k_coarse_restrict(const int* __restrict__ plane_off,
+template <int NT>
+__global__ __launch_bounds__(NT) void k_coarse_restrict(const int* __restrict__ plane_off,
                                                         const int* __restrict__ plane_nodes,
                                                         const int* __restrict__ plane_inst,
                                                         const double* __restrict__ node_r,
@@ -958,7 +960,7 @@ __global__ __launch_bounds__(256) void k_coarse_restrict(const int* __restrict__
     const int p = blockIdx.x;
     if (I[(int64_t)plane_inst[p] * NI + I_DONE]) return;
     double out[6];
-    plane_restrict<256>(p, plane_off, plane_nodes, node_r, r, out);
+    plane_restrict<NT>(p, plane_off, plane_nodes, node_r, r, out);
     if (threadIdx.x < 6) rhs[6 * (int64_t)p + threadIdx.x] = out[threadIdx.x];
 }
 
@@ -1326,6 +1328,15 @@ int apply32(tlfea_batch* b, const float* p, float* q, const int* iflags, hipStre
 // z = Chebyshev(r) for the active instances, plus the coarse correction P A_c^-1 P^T r
 // (FP64) when it is set up -- the additive two-level preconditioner -- then z.r / z.r_old
 // chunk partials.
+// rhs_c = P^T r per plane: four waves per plane on large planes, one on small ones
+void launch_restrict(tlfea_batch* b, const int* plane_off, const int* plane_nodes, const int* plane_inst,
+                     const double* node_r, const int* I, const double* r, double* rhs, hipStream_t s) {
+    if (b->restrict_wide)
+        k_coarse_restrict<256><<<b->n_planes, 256, 0, s>>>(plane_off, plane_nodes, plane_inst, node_r, I, r, rhs);
+    else
+        k_coarse_restrict<64><<<b->n_planes, 64, 0, s>>>(plane_off, plane_nodes, plane_inst, node_r, I, r, rhs);
+}
+
 // x_c = A_c^-1 rhs_c for every active instance: the block sweep, or the dense inverse
 int coarse_solve_all(tlfea_batch* b, hipStream_t s) {
     if (b->n_cd_inst < b->n_inst)  // some instances sweep
@@ -1345,8 +1356,8 @@ int precondition(tlfea_batch* b, hipStream_t s) {
     const Chunks ch = chunks(b);
     CoarseProlong cp{nullptr, nullptr, nullptr};
     if (b->coarse_on) {
-        k_coarse_restrict<<<b->n_planes, 256, 0, s>>>(b->d_plane_off, b->d_plane_nodes, b->d_plane_inst, b->d_node_r,
-                                                     b->d_i, b->d_r, b->d_crhs);
+        launch_restrict(b, b->d_plane_off, b->d_plane_nodes, b->d_plane_inst, b->d_node_r, b->d_i, b->d_r,
+                        b->d_crhs, s);
         if (int rc = coarse_solve_all(b, s)) return rc;
         cp = CoarseProlong{b->d_node_plane, b->d_node_r, b->d_cx};
     }
@@ -1896,6 +1907,7 @@ int coarse_geometry(tlfea_batch* b, hipStream_t s) {
     b->n_cblocks = (int)nblk;
     b->d_cinst_pn = d_inst_pn;
     b->h_inst_cblk = inst_cblk;
+    b->restrict_wide = P > 0 && cnt[P] >= 512 * (int64_t)P;
     if ((rc = coarse_dense_select(b, s))) return rc;
     b->coarse_geom = true;
     return TLFEA_OK;
@@ -1965,8 +1977,7 @@ extern "C" int tlfea_batch_coarse_apply(tlfea_batch* b, const double* r, double*
     TLFEA_REQUIRE(b->coarse_on, "the coarse correction is not set up");
     hipStream_t s = as_stream(stream);
     k_bclear_flags<<<grid_for(b->n_inst, 256), 256, 0, s>>>(b->n_inst, b->d_i);
-    k_coarse_restrict<<<b->n_planes, 256, 0, s>>>(b->d_plane_off, b->d_plane_nodes, b->d_plane_inst, b->d_node_r,
-                                                 b->d_i, r, b->d_crhs);
+    launch_restrict(b, b->d_plane_off, b->d_plane_nodes, b->d_plane_inst, b->d_node_r, b->d_i, r, b->d_crhs, s);
     if (int rc = coarse_solve_all(b, s)) return rc;
     const int64_t n3 = 3 * (int64_t)b->n_nodes;
     k_coarse_prolong64<<<grid_for(n3, 256), 256, 0, s>>>(n3, CoarseProlong{b->d_node_plane, b->d_node_r, b->d_cx},
