@@ -91,9 +91,8 @@ HBM_PEAK_GBS = 8000.0
 PMC_TANGENT_BYTES = (51220 * 2.0 + 29066 + 29871 * 2.0 + 21776) * 1024
 # the same for the rest-state (linear-elasticity) form the linear-static step runs
 PMC_TANGENT_REST_BYTES = (35012 * 2.0 + 29066 + 29871 * 2.0 + 21776) * 1024
-# the FP32 tangent pair of the two-level stage (k_tangent<float> + k_bgather<float>); None
-# until measured
-PMC_TANGENT32_REST_BYTES = None
+# the FP32 tangent pair of the two-level stage (k_tangent<float> + k_bgather<float>)
+PMC_TANGENT32_REST_BYTES = (22553 * 2.0 + 14533 + 19216 * 2.0 + 7938) * 1024
 # the same for the force kernels with k_gather_partials: gradient-reading (k_force_planned)
 # and rest-coordinate (k_force_rest)
 PMC_FORCE_BYTES = (306420 * 2.0 + 29085 + 20867 * 2.0 + 15799) * 1024
@@ -398,7 +397,7 @@ def main():
     ap.add_argument("--dynamic-only", action="store_true", help="only the dynamic batched measurement")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one launch each of the planned force kernel, the block SpMV, the matrix-free "
-                         "tangent, the rest-coordinate force kernel and a 512 MiB copy "
+                         "tangent, the rest-coordinate force kernel, the FP32 tangent and a 512 MiB copy "
                          "(the copy calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE), then exit")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -427,6 +426,16 @@ def main():
         b.tangent_apply(p, out=y)
         b.internal_force_rest(f=f)
         b.tangent_apply(p, x=b.X0, out=y)
+        # the FP32 operator of the two-level stage (a one-instance batch): k_tangent<float> +
+        # k_bgather<float>
+        from tlfea_amd.batch import Batch
+        bc = Batch([(m, dict(E=E, nu=nu))])
+        p32 = p.float()
+        y32 = torch.empty_like(p32)
+        torch.cuda.synchronize()
+        bc.tangent_apply_f32(p32, out=y32)
+        torch.cuda.synchronize()
+        bc.close()
         src = torch.randn(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
