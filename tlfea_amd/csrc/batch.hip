@@ -843,7 +843,8 @@ __global__ __launch_bounds__(64) void k_coarse_probe_store(int W, int period, in
 // together, then lanes 0..NB-1 solve one column each.  A pivot at or below 1e-10 of its diagonal entry
 // drops that coarse DOF (its row and column of the inverse are zero: no rigid mode of a
 // plane without the free nodes to carry it).  Called by all threads of the workgroup.
-__device__ void spd_inverse(double* m, double* L, int* dead, int NB) {
+template <int NB>
+__device__ void spd_inverse(double* m, double* L, int* dead) {
     // right-looking Cholesky, the workgroup's threads over each step's column and trailing
     // block: L starts as the lower triangle of m and is updated in place
     __shared__ double d0s[CNB_MAX];
@@ -872,24 +873,31 @@ __device__ void spd_inverse(double* m, double* L, int* dead, int NB) {
     // column c of the inverse: L y = e_c, L^T x = y (dead rows skipped)
     const int c = threadIdx.x;
     if (c < NB) {
-        double y[CNB_MAX], x[CNB_MAX];
+        // NB a compile-time constant: y and x stay in registers
+        double y[NB], x[NB];
+#pragma unroll
         for (int i = 0; i < NB; i++) {
             double v = (i == c) ? 1.0 : 0.0;
+#pragma unroll
             for (int k = 0; k < i; k++) v -= L[i * NB + k] * y[k];
             y[i] = dead[i] ? 0.0 : v / L[i * NB + i];
         }
+#pragma unroll
         for (int i = NB - 1; i >= 0; i--) {
             double v = y[i];
+#pragma unroll
             for (int k = i + 1; k < NB; k++) v -= L[k * NB + i] * x[k];
             x[i] = dead[i] ? 0.0 : v / L[i * NB + i];
         }
+#pragma unroll
         for (int i = 0; i < NB; i++) m[i * NB + c] = x[i];
     }
     __syncthreads();
 }
 
 // Block factorisation per instance (one workgroup of 64): cdiag[k] becomes D_k^-1, cg[k] = G_k.
-__global__ __launch_bounds__(64) void k_coarse_factor(int NB, const int* __restrict__ inst_cblk,
+template <int NB>
+__global__ __launch_bounds__(64) void k_coarse_factor(const int* __restrict__ inst_cblk,
                                                       double* __restrict__ cdiag, const double* __restrict__ clow,
                                                       double* __restrict__ cg) {
     __shared__ double Dprev[CNB_MAX * CNB_MAX], M[CNB_MAX * CNB_MAX], T[CNB_MAX * CNB_MAX],
@@ -940,7 +948,7 @@ __global__ __launch_bounds__(64) void k_coarse_factor(int NB, const int* __restr
             if (c > r) M[e] = M[c * NB + r] = T[e];
         }
         __syncthreads();
-        spd_inverse(M, L, dead, NB);
+        spd_inverse<NB>(M, L, dead);
         for (int e = threadIdx.x; e < nn; e += 64) {
             D[e] = M[e];
             Dprev[e] = M[e];
@@ -1940,7 +1948,12 @@ int coarse_numeric(tlfea_batch* b, hipStream_t s) {
             }
         }
     if (rc == TLFEA_OK) {
-        k_coarse_factor<<<b->n_inst, 64, 0, s>>>(NB, b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
+        if (NB == 6)
+            k_coarse_factor<6><<<b->n_inst, 64, 0, s>>>(b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
+        else if (NB == 12)
+            k_coarse_factor<12><<<b->n_inst, 64, 0, s>>>(b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
+        else
+            k_coarse_factor<18><<<b->n_inst, 64, 0, s>>>(b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
         if (hipGetLastError() != hipSuccess) {
             set_error("coarse factor launch failed");
             rc = TLFEA_EHIP;
