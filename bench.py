@@ -395,6 +395,9 @@ def main():
     ap.add_argument("--no-solvers", action="store_true", help="skip the Nesterov / AdamW inner-iteration timing")
     ap.add_argument("--solvers-only", action="store_true", help="only the Nesterov / AdamW inner-iteration timing")
     ap.add_argument("--dynamic-only", action="store_true", help="only the dynamic batched measurement")
+    ap.add_argument("--no-c1", action="store_true", help="with --solvers-only: skip the C1 Nesterov solves")
+    ap.add_argument("--batch-nesterov-only", action="store_true",
+                    help="only the batched SyncedNesterov step on the sampler's batch")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one launch each of the planned force kernel, the block SpMV, the matrix-free "
                          "tangent, the rest-coordinate force kernel, the FP32 tangent and a 512 MiB copy "
@@ -448,9 +451,13 @@ def main():
     if args.solvers_only:
         m = M.clamp_and_tip_load(M.kuhn_beam(nx, ny, nz), -1.0e5)
         vs = velocity_solvers_bench(m, E, nu, rho, log)
-        vs["c1_nesterov"] = c1_nesterov_bench(E, nu, rho, args.tol, log)
-        vs["c1_nesterov_mf"] = c1_nesterov_bench(E, nu, rho, args.tol, log, mf=True)
+        if not args.no_c1:
+            vs["c1_nesterov"] = c1_nesterov_bench(E, nu, rho, args.tol, log)
+            vs["c1_nesterov_mf"] = c1_nesterov_bench(E, nu, rho, args.tol, log, mf=True)
         print(json.dumps({"velocity_solvers": vs}))
+        return
+    if args.batch_nesterov_only:
+        print(json.dumps({"batched": {"nesterov_step": batched_nesterov_bench(args.batch_n, args.tol, log)}}))
         return
     if args.batch_only:
         bb = batched_beams(args.batch_n, args.tol, log, not args.no_seq)

@@ -98,11 +98,11 @@ struct tlfea_batch {
     std::vector<int> h_inst_cblk;
     bool coarse_geom = false;           // the geometric tables above are built
     bool restrict_wide = false;         // planes of 512 free nodes or more on average
-    // batched SyncedNesterov (tlfea_batch_nesterov_solve): v_(k-1), y, dv = y - v_n (AoS
-    // [3 n_nodes]), the displacement u_n + dt y (SoA), chunk partials [n_chunks][4],
-    // per-instance scalars [n_inst][NSN], mass coefficients rho_i / dt, restart counts
-    double *d_nvk = nullptr, *d_ny = nullptr, *d_ndv = nullptr;
-    double *d_nux = nullptr, *d_nuy = nullptr, *d_nuz = nullptr;
+    // batched SyncedNesterov (tlfea_batch_nesterov_solve): v_(k-1) (AoS [3 n_nodes]), the
+    // instance of every node, chunk partials [n_chunks][4], per-instance scalars
+    // [n_inst][NSN], mass coefficients rho_i / dt, restart counts
+    double* d_nvk = nullptr;
+    int* d_node_inst = nullptr;
     double *d_npart = nullptr, *d_ns = nullptr, *d_nmc = nullptr;
     int* d_nrs = nullptr;
     int n_npart = 0;
@@ -1181,8 +1181,7 @@ void free_batch(tlfea_batch* b) {
                     b->d_rold, b->d_dinv32, b->d_blk_list, b->d_chunk_list, b->d_inst_list, b->d_mc, b->d_mc32,
                     b->d_ux, b->d_uy, b->d_uz, b->d_node_plane, b->d_node_r, b->d_plane_off,
                     b->d_plane_nodes, b->d_plane_inst, b->d_plane_local, b->d_inst_cblk, b->d_cdiag,
-                    b->d_clow, b->d_cg, b->d_crhs, b->d_cx, b->d_nvk, b->d_ny, b->d_ndv, b->d_nux,
-                    b->d_nuy, b->d_nuz, b->d_npart, b->d_ns, b->d_nmc, b->d_nrs, b->d_cdense_off, b->d_cdense,
+                    b->d_clow, b->d_cg, b->d_crhs, b->d_cx, b->d_nvk, b->d_node_inst, b->d_npart, b->d_ns, b->d_nmc, b->d_nrs, b->d_cdense_off, b->d_cdense,
                     b->d_cd_row, b->d_cd_inst, b->d_cinst_pn};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -2273,55 +2272,29 @@ __global__ void k_bnes_init(int n_inst, const double* __restrict__ alpha, double
     rs[i] = 0;
 }
 
-// y = v_k + beta (v_k - v_(k-1)), v_(k-1) <- v_k, dv = y - v_n, u = u_n + dt y
-// (SyncedNesterov.cu:152-175); y = 0 on fixed DOFs.  beta = (t_k - 1) / t_(k+1).  An
-// instance whose step size was just halved starts again from v_k = v_(k-1) = v_n.
-__global__ __launch_bounds__(BT) void k_bnes_predict(Chunks ch, const int* __restrict__ I,
-                                                     const double* __restrict__ ns, double dt,
-                                                     const unsigned char* __restrict__ mask,
-                                                     const double* __restrict__ v, double* __restrict__ vkm1,
-                                                     double* __restrict__ y, const double* __restrict__ vn,
-                                                     const double* __restrict__ un, double* __restrict__ dv,
-                                                     double* __restrict__ ux, double* __restrict__ uy,
-                                                     double* __restrict__ uz, int* __restrict__ all) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) all[0] = 0;  // counted again by k_bnes_finalize
-    const int c = chunk_id(ch);
-    const int inst = ch.inst[c];
-    if (I[inst * NI + I_DONE]) return;
-    const double beta = (ns[inst * NSN + N_T] - 1.0) / ns[inst * NSN + N_T1];
-    const bool reset = I[inst * NI + I_RESET] != 0;
-    for (int n = ch.beg[c] + threadIdx.x; n < ch.end[c]; n += BT) {
-        double uo[3];
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            const int64_t k = 3 * (int64_t)n + d;
-            const double vk = reset ? vn[k] : v[k];
-            const double yv = mask[k] ? 0.0 : vk + beta * (vk - (reset ? vk : vkm1[k]));
-            y[k] = yv;
-            vkm1[k] = vk;
-            dv[k] = mask[k] ? 0.0 : yv - vn[k];
-            uo[d] = un[k] + dt * yv;
-        }
-        ux[n] = uo[0];
-        uy[n] = uo[1];
-        uz[n] = uo[2];
-    }
-}
 
-// g = (f_int + M dv / dt) - f_ext from the block partials (0 on fixed DOFs), v_next = y -
-// alpha g into v; chunk partials |g|^2, |v_next|^2, |v_k|^2, g.(v_next - v_k)
-// (SyncedNesterov.cu:32-95, 177-224).
+// The predict step of SyncedNesterov.cu:152-175 is formed where it is used: the force
+// kernel's staging builds y, dv = y - v_n and u_n + dt y per node from (v_k, v_(k-1), v_n,
+// u_n) and the instance's beta = (t_k - 1) / t_(k+1) (ForceMass), and this kernel forms
+// the same y per DOF: g = (f_int + M dv / dt) - f_ext from the block partials (0 on fixed
+// DOFs), v_next = y - alpha g into v, v_(k-1) <- v_k; chunk partials |g|^2, |v_next|^2,
+// |v_k|^2, g.(v_next - v_k) (SyncedNesterov.cu:32-95, 177-224).  An instance whose step
+// size was just halved starts again from v_k = v_(k-1) = v_n.  all[0] is zeroed for
+// k_bnes_finalize's count.
 __global__ __launch_bounds__(BT) void k_bnes_grad(Chunks ch, const int* __restrict__ g_off,
                                                   const int64_t* __restrict__ g_idx, const double* __restrict__ part,
                                                   const unsigned char* __restrict__ mask,
-                                                  const double* __restrict__ fext, const double* __restrict__ y,
-                                                  const double* __restrict__ vk, double* __restrict__ v,
+                                                  const double* __restrict__ fext, const double* __restrict__ vn,
+                                                  double* __restrict__ vkm1, double* __restrict__ v,
                                                   const double* __restrict__ ns, const int* __restrict__ I,
-                                                  double* __restrict__ cpart) {
+                                                  double* __restrict__ cpart, int* __restrict__ all) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) all[0] = 0;
     const int c = chunk_id(ch);
     const int inst = ch.inst[c];
     if (I[inst * NI + I_DONE]) return;
     const double alpha = ns[inst * NSN + N_ALPHA];
+    const double beta = (ns[inst * NSN + N_T] - 1.0) / ns[inst * NSN + N_T1];
+    const bool reset = I[inst * NI + I_RESET] != 0;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     for (int n = ch.beg[c] + threadIdx.x; n < ch.end[c]; n += BT) {
         double s3[3] = {0.0, 0.0, 0.0};
@@ -2329,10 +2302,13 @@ __global__ __launch_bounds__(BT) void k_bnes_grad(Chunks ch, const int* __restri
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             const int64_t k = 3 * (int64_t)n + d;
-            const double g = mask[k] ? 0.0 : s3[d] - fext[k];
-            const double vnext = mask[k] ? 0.0 : y[k] - alpha * g;
-            const double vo = vk[k];
+            const bool fx = mask[k] != 0;
+            const double vo = reset ? vn[k] : v[k];
+            const double yv = fx ? 0.0 : vo + beta * (vo - (reset ? vo : vkm1[k]));
+            const double g = fx ? 0.0 : s3[d] - fext[k];
+            const double vnext = fx ? 0.0 : yv - alpha * g;
             v[k] = vnext;
+            vkm1[k] = vo;
             acc[0] += g * g;
             acc[1] += vnext * vnext;
             acc[2] += vo * vo;
@@ -2355,8 +2331,8 @@ __global__ __launch_bounds__(BT) void k_bnes_grad(Chunks ch, const int* __restri
 // finite or has grown past 1e4 |g_0| means the step size exceeds 2 / L along the path (the
 // tangent stiffens under large deflection); the instance then halves its step size and
 // starts again from v_n, at most 40 times before it is given up as not converged.
-// One wave per listed instance; all[0] (zeroed by the predict kernel) counts the
-// instances still running.
+// One wave per listed instance; all[0] (zeroed by k_bnes_grad) counts the instances still
+// running.
 __global__ __launch_bounds__(64) void k_bnes_finalize(Insts in, const int* __restrict__ ico,
                                                       const double* __restrict__ cpart, double* __restrict__ ns,
                                                       int* __restrict__ I, int* __restrict__ rs, int* __restrict__ all,
@@ -2425,11 +2401,17 @@ int nesterov_iteration(tlfea_batch* b, double dt, int mass_exact, const double* 
                        const double* f_ext, double* v, double inner_tol, double rel_tol, int restart, hipStream_t s) {
     const Chunks ch = chunks(b);
     const int nc = n_chunk_launch(b);
-    k_bnes_predict<<<nc, BT, 0, s>>>(ch, b->d_i, b->d_ns, dt, b->d_mask, v, b->d_nvk, b->d_ny, v_n, u_n, b->d_ndv,
-                                     b->d_nux, b->d_nuy, b->d_nuz, b->d_all);
-    TLFEA_LAUNCH_CHECK();
     ForceMass fm;
-    fm.dv = b->d_ndv;
+    fm.nes_v = v;
+    fm.nes_vkm1 = b->d_nvk;
+    fm.nes_vn = v_n;
+    fm.nes_un = u_n;
+    fm.nes_mask = b->d_mask;
+    fm.node_inst = b->d_node_inst;
+    fm.nes_s = b->d_ns + N_T;  // t_k, then t_(k+1)
+    fm.nes_sstride = NSN;
+    fm.nes_reset = b->d_i + I_RESET;
+    fm.dt = dt;
     fm.inst_mc = b->d_nmc;
     fm.mass_exact = mass_exact;
     fm.inst_done = b->d_i + I_DONE;
@@ -2437,11 +2419,11 @@ int nesterov_iteration(tlfea_batch* b, double dt, int mass_exact, const double* 
     fm.blk_list = b->compact ? b->d_blk_list : nullptr;
     fm.n_list = b->n_blk_list;
     const tlfea_material none{0.0, 0.0, 0.0, 0.0};
-    if (int rc = launch_force_rest_mass_partials(b->plan, b->d_nux, b->d_nuy, b->d_nuz, b->d_X, b->d_Y, b->d_Z,
-                                                 b->quad, none, b->d_elem_inst, b->d_lam, b->d_mu, fm, s))
+    if (int rc = launch_force_rest_mass_partials(b->plan, nullptr, nullptr, nullptr, b->d_X, b->d_Y, b->d_Z, b->quad,
+                                                 none, b->d_elem_inst, b->d_lam, b->d_mu, fm, s))
         return rc;
-    k_bnes_grad<<<nc, BT, 0, s>>>(ch, b->plan->d_g_off, b->plan->d_g_idx, b->plan->d_part, b->d_mask, f_ext,
-                                  b->d_ny, b->d_nvk, v, b->d_ns, b->d_i, b->d_npart);
+    k_bnes_grad<<<nc, BT, 0, s>>>(ch, b->plan->d_g_off, b->plan->d_g_idx, b->plan->d_part, b->d_mask, f_ext, v_n,
+                                  b->d_nvk, v, b->d_ns, b->d_i, b->d_npart, b->d_all);
     const Insts in = insts(b);
     k_bnes_finalize<<<in.n, 64, 0, s>>>(in, b->d_inst_chunk_off, b->d_npart, b->d_ns, b->d_i, b->d_nrs, b->d_all,
                                         inner_tol, rel_tol, restart);
@@ -2475,11 +2457,16 @@ extern "C" int tlfea_batch_nesterov_solve(tlfea_batch* b, double dt, const doubl
     const int N = b->n_nodes;
     const size_t nd = 3 * (size_t)N;
     int rc;
-    if (!b->d_nvk && ((rc = dalloc(&b->d_nvk, nd)) || (rc = dalloc(&b->d_ny, nd)) || (rc = dalloc(&b->d_ndv, nd)) ||
-                      (rc = dalloc(&b->d_nux, N)) || (rc = dalloc(&b->d_nuy, N)) || (rc = dalloc(&b->d_nuz, N)) ||
-                      (rc = dalloc(&b->d_ns, (size_t)b->n_inst * NSN)) || (rc = dalloc(&b->d_nmc, b->n_inst)) ||
-                      (rc = dalloc(&b->d_nrs, b->n_inst))))
-        return rc;
+    if (!b->d_nvk) {
+        if ((rc = dalloc(&b->d_nvk, nd)) || (rc = dalloc(&b->d_node_inst, N)) ||
+            (rc = dalloc(&b->d_ns, (size_t)b->n_inst * NSN)) || (rc = dalloc(&b->d_nmc, b->n_inst)) ||
+            (rc = dalloc(&b->d_nrs, b->n_inst)))
+            return rc;
+        std::vector<int> ni(N);
+        for (int i = 0; i < b->n_inst; i++)
+            for (int n = b->h_node_off[i]; n < b->h_node_off[i + 1]; n++) ni[n] = i;
+        TLFEA_HIP_TRY(hipMemcpy(b->d_node_inst, ni.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+    }
     if (b->n_npart < b->n_chunks) {
         if (b->d_npart) (void)hipFree(b->d_npart);
         b->d_npart = nullptr;

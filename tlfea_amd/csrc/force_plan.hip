@@ -162,10 +162,13 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_force_rest(
         sR[u] = X;
         sR[max_u + u] = Y;
         sR[2 * max_u + u] = Z;
-        // x, y, z are positions, or with x_is_disp the displacement itself
-        sU[u] = x_is_disp ? x[n] : x[n] - X;
-        sU[max_u + u] = x_is_disp ? y[n] : y[n] - Y;
-        sU[2 * max_u + u] = x_is_disp ? z[n] : z[n] - Z;
+        // x, y, z are positions, or with x_is_disp the displacement itself (not read when the
+        // batched Nesterov state forms the displacement below)
+        if (MASS == 0 || !fm.nes_v) {
+            sU[u] = x_is_disp ? x[n] : x[n] - X;
+            sU[max_u + u] = x_is_disp ? y[n] : y[n] - Y;
+            sU[2 * max_u + u] = x_is_disp ? z[n] : z[n] - Z;
+        }
         sF[u] = sF[max_u + u] = sF[2 * max_u + u] = 0.0;
         if constexpr (DAMP) {
             sV[u] = v[3 * n];
@@ -173,10 +176,28 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_force_rest(
             sV[2 * max_u + u] = v[3 * n + 2];
         }
         if constexpr (MASS != 0) {
-            const double* dvn = fm.dv + (int64_t)fm.dv_stride * n;
-            sV[u] = dvn[0];
-            sV[max_u + u] = dvn[1];
-            sV[2 * max_u + u] = dvn[2];
+            if (fm.nes_v) {
+                const int inst = fm.node_inst[n];
+                const double* ts = fm.nes_s + (int64_t)inst * fm.nes_sstride;
+                const double beta = (ts[0] - 1.0) / ts[1];
+                const bool reset = fm.nes_reset[(int64_t)inst * fm.done_stride] != 0;
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    const int64_t k = 3 * (int64_t)n + d;
+                    const double vn = fm.nes_vn[k];
+                    const double vk = reset ? vn : fm.nes_v[k];
+                    const double km1 = reset ? vk : fm.nes_vkm1[k];
+                    const bool fx = fm.nes_mask[k] != 0;
+                    const double yv = fx ? 0.0 : vk + beta * (vk - km1);
+                    sV[d * max_u + u] = fx ? 0.0 : yv - vn;
+                    sU[d * max_u + u] = fm.nes_un[k] + fm.dt * yv;
+                }
+            } else {
+                const double* dvn = fm.dv + (int64_t)fm.dv_stride * n;
+                sV[u] = dvn[0];
+                sV[max_u + u] = dvn[1];
+                sV[2 * max_u + u] = dvn[2];
+            }
         }
     }
     __syncthreads();
@@ -458,7 +479,9 @@ int launch_force_rest_mass_partials(const tlfea_force_plan* p, const double* ux,
                                     const tlfea_material& mat, const int* elem_inst, const double* inst_lam,
                                     const double* inst_mu, const ForceMass& fm, hipStream_t s) {
     const bool batch = elem_inst != nullptr;
-    if ((batch && (!inst_lam || !inst_mu || !fm.inst_mc)) || !fm.dv || fm.dv_stride < 3) {
+    if ((batch && (!inst_lam || !inst_mu || !fm.inst_mc)) || (!fm.dv && !fm.nes_v) || fm.dv_stride < 3 ||
+        (fm.nes_v && (!batch || !fm.nes_vkm1 || !fm.nes_vn || !fm.nes_un || !fm.nes_mask || !fm.node_inst ||
+                      !fm.nes_s || !fm.nes_reset))) {
         set_error("the force with a mass term needs materials, dv and mass coefficients");
         return TLFEA_EINVAL;
     }

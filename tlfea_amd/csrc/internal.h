@@ -73,6 +73,21 @@ struct ForceMass {
     int done_stride = 0;
     const int* blk_list = nullptr;
     int n_list = 0;
+    // batched Nesterov staging (nes_v non-null): per staged node n of instance i =
+    // node_inst[n], y = v_k + beta_i (v_k - v_(k-1)) with beta_i = (s[0] - 1) / s[1] of the
+    // instance's scalars (nes_s + i * nes_sstride), v_k = v_(k-1) = v_n when
+    // nes_reset[i * done_stride] is set, y = 0 on masked DOFs; then dv = y - v_n and the
+    // displacement u = u_n + dt y are formed there instead of read (the u arrays unused)
+    const double* nes_v = nullptr;
+    const double* nes_vkm1 = nullptr;
+    const double* nes_vn = nullptr;
+    const double* nes_un = nullptr;
+    const unsigned char* nes_mask = nullptr;
+    const int* node_inst = nullptr;
+    const double* nes_s = nullptr;
+    int nes_sstride = 0;
+    const int* nes_reset = nullptr;
+    double dt = 0.0;
 };
 // f_int(X + u) + c M dv as block partials in plan->d_part (no gather); u = (ux, uy, uz).
 // elem_inst / inst_lam / inst_mu for a batch, else mat's (lambda, mu).
