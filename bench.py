@@ -11,8 +11,10 @@ CG to sqrt(rr/rr0) < tol).  `value` = solves/s of the FP32 + FP64-refinement pat
 flexible FP64 CG (u, r, alpha, beta and the stopping test in FP64) preconditioned by the
 FP32 two-level stage -- Jacobi-Chebyshev on the FP32 operator plus a coarse solve over the
 rigid motions of the beam's node planes (P^T K P probed with the FP64 operator, block-
-tridiagonal factorisation) -- with that setup inside every step; the Chebyshev degree is
-picked by timing one solve each of degree 4, 6 and 8 before the timed steps.  Beside it,
+tridiagonal factorisation, its dense inverse on a long chain) -- with that setup inside
+every step; the Chebyshev degree and interval ([lambda_max / ratio, lambda_max]) are picked
+by timing one solve each of (8, 500), (12, 1000) and (16, 1000) before the timed steps.
+Beside it,
 on the same problem: the plain FP64 path (block assembly + Dirichlet + CG on the assembled
 matrix, the reference's arithmetic) and the matrix-free FP64 CG (K(x) p applied element by
 element from the inverse Jacobians, in the single-reduction arrangement).  Inputs are
@@ -140,12 +142,12 @@ def batched_beams(n, tol, log, sequential=True):
     net = WarmStart.load()
     g = BatchGraph(b)
     uw = timed("fp64_learned_start", lambda: b.solve(tol=tol, u0=net.guess(b, graph=g)))
-    # FP32 two-level preconditioner inside the FP64 flexible CG: Chebyshev stage (degree 4)
-    # and the rigid-plane coarse correction, set up per batch (timed apart and included in
-    # the end-to-end rate)
+    # FP32 two-level preconditioner inside the FP64 flexible CG: Chebyshev stage (degree 6 on
+    # [lambda_max / 250, lambda_max]) and the rigid-plane coarse correction, set up per batch
+    # (timed apart and included in the end-to-end rate)
     torch.cuda.synchronize()
     t = time.perf_counter()
-    b.precond_setup(degree=4, ratio=30.0, power_iters=20)
+    b.precond_setup(degree=6, ratio=250.0, power_iters=20)
     n_coarse = b.coarse_setup()
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t
@@ -386,7 +388,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cells", type=str, default="200,20,20")
     ap.add_argument("--cheb-degree", type=int, default=0,
-                    help="Chebyshev degree of the FP32 stage (0: the fastest of 4, 6, 8 on one trial solve each)")
+                    help="Chebyshev degree of the FP32 stage (0: the fastest of (degree, interval ratio) = (8, 500), "
+                         "(12, 1000), (16, 1000) on one trial solve each)")
+    ap.add_argument("--cheb-ratio", type=float, default=1000.0,
+                    help="lambda_max / lower end of the Chebyshev interval, with --cheb-degree")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-beams measurement")
     ap.add_argument("--batch-n", type=int, default=256)
     ap.add_argument("--batch-only", action="store_true", help="only the batched-beams measurement")
@@ -511,8 +516,9 @@ def main():
     from tlfea_amd.batch import Batch
     bc = Batch([(m, dict(E=E, nu=nu))])
 
-    def step2(deg):
-        bc.precond_setup(degree=deg, ratio=30.0, power_iters=20)
+    def step2(cfg):
+        deg, ratio = cfg
+        bc.precond_setup(degree=deg, ratio=ratio, power_iters=20)
         bc.coarse_setup()
         u = bc.solve(tol=args.tol, precond=True, max_iter=20000)
         if bc.status[0] != 0:
@@ -520,16 +526,16 @@ def main():
         return u
 
     trial = {}
-    degs = [args.cheb_degree] if args.cheb_degree > 0 else [4, 6, 8]
-    for deg in degs:
-        step2(deg)
+    cfgs = [(args.cheb_degree, args.cheb_ratio)] if args.cheb_degree > 0 else [(8, 500.0), (12, 1000.0), (16, 1000.0)]
+    for cfg in cfgs:
+        step2(cfg)
         torch.cuda.synchronize()
         t = time.perf_counter()
-        step2(deg)
+        step2(cfg)
         torch.cuda.synchronize()
-        trial[deg] = time.perf_counter() - t
+        trial[cfg] = time.perf_counter() - t
     deg2 = min(trial, key=trial.get)
-    log(f"FP32 two-level trials (s per solve by Chebyshev degree): {trial}; using {deg2}")
+    log(f"FP32 two-level trials (s per solve by Chebyshev degree and interval ratio): {trial}; using {deg2}")
     for _ in range(args.warmup):
         step2(deg2)
     torch.cuda.synchronize()
@@ -735,8 +741,9 @@ def main():
                                f"the FP32 two-level preconditioner, set up per solve), cantilever {nx}x{ny}x{nz} "
                                f"cells x 6 TET10 ({m.n_elem} el, {m.n_nodes} nodes, {3 * m.n_nodes} DOF), 1e5 N "
                                f"tip load",
-                   "n_elem": m.n_elem, "n_dof": 3 * m.n_nodes, "iterations": iters2, "chebyshev_degree": deg2,
-                   "degree_trials_s": {str(k): v for k, v in trial.items()},
+                   "n_elem": m.n_elem, "n_dof": 3 * m.n_nodes, "iterations": iters2, "chebyshev_degree": deg2[0],
+                   "chebyshev_ratio": deg2[1],
+                   "degree_trials_s": {f"{k[0]}/{k[1]:g}": v for k, v in trial.items()},
                    "final_rel_residual": rel2, "true_rel_residual": true_res2,
                    "u_rel_diff_vs_matfree_fp64": u2_diff, "K_nnz": dof_nnz,
                    "plain_fp64": {"solves_per_s": args.steps / dt64, "ms_per_solve": dt64 / args.steps * 1e3,
