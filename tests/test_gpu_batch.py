@@ -57,6 +57,32 @@ def _residual_bar(tol, doff, vo, u, f):
     return tol + 20 * 2.2e-16 * _k_norm1(doff, vo) * np.linalg.norm(u) / np.linalg.norm(f)
 
 
+def test_fp32_operator_at_rest(gpu):
+    """The FP32 operator of the two-level stage at rest (the corner-value form on these
+    affine TET10 beams) equals the FP64 operator to FP32 rounding (5e-6 relative per
+    instance) and the oracle's assembled K p to the same bar, returns p on the masked rows,
+    and two applications agree to FP32 rounding (its LDS accumulation is FP64 atomics, whose
+    order follows wave scheduling)."""
+    import torch
+    from tlfea_amd.batch import Batch
+    items = _items(seed=4, n=6, big=True)
+    b = Batch(items)
+    p = np.random.default_rng(2).standard_normal(3 * b.n_nodes)
+    p32 = torch.as_tensor(p, dtype=torch.float32).cuda()
+    y32 = _np(b.tangent_apply_f32(p32))
+    y32b = _np(b.tangent_apply_f32(p32))
+    assert rel_err(y32b, y32) < 1e-6
+    pr = _np(p32).astype(np.float64)  # the rounded input both sides see
+    y64 = _np(b.tangent_apply(torch.as_tensor(pr).cuda()))
+    mask = _np(b.mask) != 0
+    assert np.array_equal(y32[mask], _np(p32)[mask])
+    for i, (m, mat) in enumerate(items):
+        sl = b.inst_slice(i)
+        assert rel_err(y32[sl], y64[sl]) < 5e-6, i
+        doff, dcol, vo, _ = _oracle_system(m, mat["E"], mat["nu"])
+        assert rel_err(y32[sl], O.spmv(doff, dcol, vo, pr[sl])) < 5e-6, i
+
+
 def test_batch_operator_matches_single_and_oracle(gpu):
     import torch
     from tlfea_amd.batch import Batch
