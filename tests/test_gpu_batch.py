@@ -61,8 +61,8 @@ def test_fp32_operator_at_rest(gpu):
     """The FP32 operator of the two-level stage at rest (the corner-value form on these
     affine TET10 beams) equals the FP64 operator to FP32 rounding (5e-6 relative per
     instance) and the oracle's assembled K p to the same bar, returns p on the masked rows,
-    and two applications agree to FP32 rounding (its LDS accumulation is FP64 atomics, whose
-    order follows wave scheduling)."""
+    and two applications give the same bits (each block sums its element outputs in the fixed
+    order of the plan's gather tables)."""
     import torch
     from tlfea_amd.batch import Batch
     items = _items(seed=4, n=6, big=True)
@@ -71,7 +71,7 @@ def test_fp32_operator_at_rest(gpu):
     p32 = torch.as_tensor(p, dtype=torch.float32).cuda()
     y32 = _np(b.tangent_apply_f32(p32))
     y32b = _np(b.tangent_apply_f32(p32))
-    assert rel_err(y32b, y32) < 1e-6
+    assert np.array_equal(y32b, y32)
     pr = _np(p32).astype(np.float64)  # the rounded input both sides see
     y64 = _np(b.tangent_apply(torch.as_tensor(pr).cuda()))
     mask = _np(b.mask) != 0

@@ -534,6 +534,10 @@ void free_plan(tlfea_force_plan* p) {
     (void)hipFree(p->d_g_idx);
     (void)hipFree(p->d_own);
     (void)hipFree(p->d_dot);
+    (void)hipFree(p->d_gperm);
+    (void)hipFree(p->d_ginc);
+    (void)hipFree(p->d_ginc_off);
+    (void)hipFree(p->d_gwinfo);
     delete p;
 }
 
@@ -617,10 +621,67 @@ extern "C" int tlfea_force_plan_create(int nn, int n_elem, int n_nodes, const in
             own[k] = 1;
         }
     std::vector<double> dot0(2 * (size_t)nb, 0.0);
+    // TET10: the tables of the in-block node sums (struct tlfea_force_plan).  Positions are
+    // sorted by their pair count, largest first, so that the 64 lanes of a wave loop over
+    // about the same number of pairs.
+    std::vector<unsigned short> gperm, ginc;
+    std::vector<int64_t> ginc_off(nb + 1, 0);
+    std::vector<unsigned> gwinfo;
+    // a node touched by 256 or more pairs of one block (not a tetrahedral mesh one meets)
+    // leaves the plan without tables: the kernels then sum with LDS atomics
+    bool tables_ok = nn == 10;
+    if (tables_ok) {
+        const int max_gw = (max_u + 63) / 64;
+        gperm.assign(unodes.size(), 0);
+        gwinfo.assign((size_t)nb * max_gw, 0);
+        std::vector<int> cnt, start, fill, pos;
+        std::vector<unsigned short> lst;
+        auto slot_of = [&](int e, int a) { return (int)slot[((size_t)(e >> 6) * nn + a) * TILE + (e & (TILE - 1))]; };
+        int max_ginc = 0;
+        for (int b = 0; b < nb && tables_ok; b++) {
+            const int e0 = b * PLAN_BLOCK, e1 = std::min(n_elem, e0 + PLAN_BLOCK);
+            const int U = (int)(blk_off[b + 1] - blk_off[b]);
+            cnt.assign(U, 0);
+            for (int e = e0; e < e1; e++)
+                for (int a = 0; a < nn; a++) cnt[slot_of(e, a)]++;
+            start.assign(U + 1, 0);
+            for (int u = 0; u < U; u++) start[u + 1] = start[u] + cnt[u];
+            lst.resize(start[U]);
+            fill.assign(start.begin(), start.end() - 1);
+            for (int e = e0; e < e1; e++)
+                for (int a = 0; a < nn; a++) lst[fill[slot_of(e, a)]++] = (unsigned short)(a * PLAN_BLOCK + (e - e0));
+            pos.resize(U);
+            for (int u = 0; u < U; u++) pos[u] = u;
+            std::stable_sort(pos.begin(), pos.end(), [&](int x, int y) { return cnt[x] > cnt[y]; });
+            const size_t base = ginc.size();
+            for (int w = 0; w * 64 < U; w++) {
+                int c = 0;
+                for (int l = 0; l < 64 && w * 64 + l < U; l++) c = std::max(c, cnt[pos[w * 64 + l]]);
+                tables_ok = tables_ok && c < 256;
+                gwinfo[(size_t)b * max_gw + w] = (unsigned)((ginc.size() - base) << 8) | (unsigned)c;
+                for (int j = 0; j < c; j++)
+                    for (int l = 0; l < 64; l++) {
+                        const int g = w * 64 + l;
+                        ginc.push_back(g < U && j < cnt[pos[g]] ? lst[start[pos[g]] + j]
+                                                                : (unsigned short)(10 * PLAN_BLOCK));
+                    }
+            }
+            for (int g = 0; g < U; g++) gperm[blk_off[b] + g] = (unsigned short)pos[g];
+            ginc_off[b + 1] = (int64_t)ginc.size();
+            max_ginc = std::max(max_ginc, (int)(ginc.size() - base));
+        }
+        P->max_gw = max_gw;
+        P->max_ginc = max_ginc;
+    }
     int rc = TLFEA_OK;
     if ((rc = upload(&P->d_blk_off, blk_off)) || (rc = upload(&P->d_unodes, unodes)) ||
         (rc = upload(&P->d_slot, slot)) || (rc = upload(&P->d_g_off, g_off)) || (rc = upload(&P->d_g_idx, g_idx)) ||
         (rc = upload(&P->d_own, own)) || (rc = upload(&P->d_dot, dot0))) {
+        free_plan(P);
+        return rc;
+    }
+    if (tables_ok && ((rc = upload(&P->d_gperm, gperm)) || (rc = upload(&P->d_ginc, ginc)) ||
+                     (rc = upload(&P->d_ginc_off, ginc_off)) || (rc = upload(&P->d_gwinfo, gwinfo)))) {
         free_plan(P);
         return rc;
     }

@@ -35,11 +35,26 @@ struct tlfea_force_plan {
     int64_t* d_g_idx = nullptr;     // partial positions, ascending per node
     unsigned char* d_own = nullptr; // [n_partials] 1 on a node's first partial (lowest block)
     double* d_dot = nullptr;        // [2][n_blocks] per-block p.Kp and p.p shares of the fused CG
+    // In-block node sums of element-major outputs (TET10 plans): a block's nodes are summed
+    // in "gather positions" g, ordered by how many of the block's (element, local node)
+    // pairs touch them, 64 positions (one wave) at a time.  For the wave of positions
+    // [64w, 64w + 64) the table holds c_w rows of 64 u16 entries (row j, lane l: the j-th
+    // pair a * 256 + e_local of position 64w + l, in ascending e_local then a; 2560 pads a
+    // position with fewer than c_w pairs).
+    unsigned short* d_gperm = nullptr;  // [n_partials] slot of gather position g of a block
+    unsigned short* d_ginc = nullptr;   // per block: its wave tables, one after the other
+    int64_t* d_ginc_off = nullptr;      // [n_blocks+1] start of a block's table (entries)
+    unsigned* d_gwinfo = nullptr;       // [n_blocks][max_gw]: (table offset in the block << 8) | c_w
+    int max_gw = 0;                     // ceil(max_u / 64)
+    int max_ginc = 0;                   // longest block table (entries)
 };
 
 namespace tlfea_hip {
 
 constexpr int PLAN_BLOCK = 256;  // elements per force-plan block (one thread each)
+
+// n rounded up to a multiple of 16 (LDS carve-outs)
+__host__ __device__ constexpr size_t align16(size_t n) { return (n + 15) & ~(size_t)15; }
 
 // Planned fused force (element.hip): partials then gather into f (accumulate or overwrite).
 int launch_force_planned(const tlfea_force_plan* p, const double* x, const double* y, const double* z,

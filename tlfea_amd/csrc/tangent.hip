@@ -228,9 +228,26 @@ __device__ __forceinline__ T grad_l_col(const T (&R)[3][3], int d, int c) {
     return c == 0 ? -(R[d][0] + R[d][1] + R[d][2]) : R[d][c - 1];
 }
 
-template <typename T>
-__device__ __forceinline__ void tangent_vertex_t10(const int (&sl)[10], const T* sP, double* sF, int max_u,
-                                                   const T (&Mn)[9], T dJ, T lam, T mu, T ks) {
+// Output of node a of the element: added to the block's FP64 LDS accumulators (EM false),
+// or stored element-major at sO[(a * 256 + el) * 3 + d] for the in-block node sums (EM).
+template <typename T, bool EM>
+__device__ __forceinline__ void vertex_out(double* sF, T* sO, int max_u, int el, int a, int slot, T v0, T v1, T v2) {
+    if constexpr (EM) {
+        T* o = sO + (a * PLAN_BLOCK + el) * 3;
+        o[0] = v0;
+        o[1] = v1;
+        o[2] = v2;
+    } else {
+        atomicAdd(&sF[slot], (double)v0);
+        atomicAdd(&sF[max_u + slot], (double)v1);
+        atomicAdd(&sF[2 * max_u + slot], (double)v2);
+    }
+}
+
+// pv(i, a): component i of the direction at local node a of the element.
+template <typename T, bool EM, typename PV>
+__device__ __forceinline__ void tangent_vertex_t10(const PV& pv, const int (&sl)[10], double* sF, T* sO, int el,
+                                                   int max_u, const T (&Mn)[9], T dJ, T lam, T mu, T ks) {
     // edge node between corners w and j
     constexpr int EDGE[4][4] = {{0, 4, 6, 7}, {4, 0, 5, 8}, {6, 5, 0, 9}, {7, 8, 9, 0}};
     constexpr int EDGES[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
@@ -241,9 +258,9 @@ __device__ __forceinline__ void tangent_vertex_t10(const int (&sl)[10], const T*
     T A[3][3];
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        const T p0 = sP[i * max_u + sl[0]];
+        const T p0 = pv(i, 0);
 #pragma unroll
-        for (int k = 0; k < 3; k++) A[i][k] = sP[i * max_u + sl[k + 1]] - p0;
+        for (int k = 0; k < 3; k++) A[i][k] = pv(i, k + 1) - p0;
     }
     T Q[4][3][3];
 #pragma unroll
@@ -253,7 +270,7 @@ __device__ __forceinline__ void tangent_vertex_t10(const int (&sl)[10], const T*
         for (int i = 0; i < 3; i++) {
             T P[4];
 #pragma unroll
-            for (int j = 0; j < 4; j++) P[j] = sP[i * max_u + sl[j == w ? w : EDGE[w][j]]];
+            for (int j = 0; j < 4; j++) P[j] = pv(i, j == w ? w : EDGE[w][j]);
 #pragma unroll
             for (int k = 0; k < 3; k++) Jp[i][k] = T(4) * (P[k + 1] - P[0]) - A[i][k];
         }
@@ -285,8 +302,7 @@ __device__ __forceinline__ void tangent_vertex_t10(const int (&sl)[10], const T*
         for (int i = 0; i < 3; i++)
 #pragma unroll
             for (int k = 0; k < 3; k++) R[i][k] = T(4) * Q[a][i][k] - Qs[i][k];
-#pragma unroll
-        for (int d = 0; d < 3; d++) atomicAdd(&sF[d * max_u + sl[a]], (double)grad_l_col(R, d, a));
+        vertex_out<T, EM>(sF, sO, max_u, el, a, sl[a], grad_l_col(R, 0, a), grad_l_col(R, 1, a), grad_l_col(R, 2, a));
     }
 #pragma unroll
     for (int m = 0; m < 6; m++) {
@@ -299,9 +315,10 @@ __device__ __forceinline__ void tangent_vertex_t10(const int (&sl)[10], const T*
                 Ri[i][k] = Q[ci][i][k] + Qs[i][k];
                 Rj[i][k] = Q[cj][i][k] + Qs[i][k];
             }
+        T o[3];
 #pragma unroll
-        for (int d = 0; d < 3; d++)
-            atomicAdd(&sF[d * max_u + sl[4 + m]], (double)(T(4) * (grad_l_col(Ri, d, cj) + grad_l_col(Rj, d, ci))));
+        for (int d = 0; d < 3; d++) o[d] = T(4) * (grad_l_col(Ri, d, cj) + grad_l_col(Rj, d, ci));
+        vertex_out<T, EM>(sF, sO, max_u, el, 4 + m, sl[4 + m], o[0], o[1], o[2]);
     }
 }
 
@@ -402,7 +419,8 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent(
     __syncthreads();
     if constexpr (NN == 10 && STD && MASS == 0 && LIN && AFF) {
         if (active)
-            tangent_vertex_t10<T>(sl, sP, sF, max_u, Mn, dJn, tan_lam(mat, inst), tan_mu(mat, inst), mat.k_scale);
+            tangent_vertex_t10<T, false>([&](int i, int a) { return sP[i * max_u + sl[a]]; }, sl, sF, nullptr, 0,
+                                         max_u, Mn, dJn, tan_lam(mat, inst), tan_mu(mat, inst), mat.k_scale);
     } else if (active) {
         const T lam = MR ? mat.mu10 : tan_lam(mat, inst), mu = MR ? mat.mu01 : tan_mu(mat, inst);
         const T mc = mat.elem_inst ? (mat.inst_mc ? mat.inst_mc[inst] : T(0)) : mat.mc;
@@ -481,6 +499,141 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent(
             }
             dot_part[b] = s;
             dot_part[gridDim.x + b] = t;
+        }
+    }
+}
+
+// The rest-state operator of an affine TET10 mesh (k_tangent's corner-value form) without
+// LDS atomics: every element stores its 30 outputs element-major in LDS, and after a
+// barrier each block node sums its (element, local node) pairs in the fixed order of the
+// plan's gather tables (struct tlfea_force_plan), so the partials are the same bits on
+// every call, and no ds_add_f64 waits (C2 FP32 pair 42.1 -> 38.0 us).  No mass term, no
+// p.Kp shares, blocks of at most 1024 nodes.  LDS: the outputs (2561 x 3 of T, the last row
+// zero for the tables' pads; the staged direction occupies their first 3 x max_u values
+// until every element holds its 30 values in registers) and the block's tables (max_ginc
+// u16): 37.5 KB on C2, four blocks per CU.
+template <typename T>
+__global__ __launch_bounds__(PLAN_BLOCK) void k_tangent_em(
+    int n_elem, const int64_t* __restrict__ blk_off, const int* __restrict__ unodes,
+    const unsigned short* __restrict__ slot_t, const T* __restrict__ p, const unsigned char* __restrict__ mask,
+    const T* __restrict__ jinv_t, const T* __restrict__ detJ_t, TanMat<T> mat, T* __restrict__ part, int max_u,
+    const int* __restrict__ flags, const int* __restrict__ blk_list, const int64_t* __restrict__ ginc_off,
+    const unsigned short* __restrict__ ginc, const unsigned* __restrict__ gwinfo, int max_gw,
+    const unsigned short* __restrict__ gperm) {
+    if (flags && flags[0]) return;
+    extern __shared__ unsigned char lds_raw[];
+    constexpr int NO = (10 * PLAN_BLOCK + 1) * 3;
+    // the staged direction shares the outputs' space: every element reads its 30 values
+    // into registers before any output is stored
+    T* sO = reinterpret_cast<T*>(lds_raw);
+    T* sP = sO;
+    unsigned short* sInc = reinterpret_cast<unsigned short*>(lds_raw + align16(sizeof(T) * NO));
+    const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+    const int b = blk_list ? blk_list[lb] : (int)lb;
+    const int64_t u0 = blk_off[b];
+    const int U = (int)(blk_off[b + 1] - u0);
+    const int e = b * PLAN_BLOCK + threadIdx.x;
+    int inst = 0;
+    bool active = e < n_elem;
+    if (active && mat.elem_inst) {
+        inst = mat.elem_inst[e];
+        if (mat.inst_done && mat.inst_done[inst * mat.done_stride]) active = false;
+    }
+    const int tile = e >> 6, lane = e & (TILE - 1);
+    int sl[10];
+    T Mn[9], dJn = T(0);
+    if (active) {
+#pragma unroll
+        for (int a = 0; a < 10; a++) sl[a] = slot_t[((int64_t)tile * 10 + a) * TILE + lane];
+        const T* Mp = jinv_t + (int64_t)tile * 5 * (9 * TILE) + lane;
+#pragma unroll
+        for (int c = 0; c < 9; c++) Mn[c] = Mp[c * TILE];
+        dJn = detJ_t[(int64_t)tile * 5 * TILE + lane];
+    }
+    // the table rows and slots of this thread's gather positions (g = threadIdx.x + 256 r)
+    const int nw = (U + 63) >> 6;
+    constexpr int GR = 4;  // positions per thread: max_u <= 1024
+    unsigned ginfo[GR];
+    int gslot[GR];
+#pragma unroll
+    for (int r = 0; r < GR; r++) {
+        const int g = threadIdx.x + r * PLAN_BLOCK;
+        ginfo[r] = (g >> 6) < nw ? gwinfo[(int64_t)b * max_gw + (g >> 6)] : 0u;
+        gslot[r] = g < U ? (int)gperm[u0 + g] : 0;
+    }
+    // the block's gather tables (a multiple of 64 u16 entries, 16-byte aligned)
+    {
+        const int64_t i0 = ginc_off[b];
+        const int n16 = (int)((ginc_off[b + 1] - i0) >> 3);
+        const uint4* src = reinterpret_cast<const uint4*>(ginc + i0);
+        uint4* dst = reinterpret_cast<uint4*>(sInc);
+        for (int k = threadIdx.x; k < n16; k += PLAN_BLOCK) dst[k] = src[k];
+    }
+    constexpr int SPT = 4;
+    for (int base = threadIdx.x; base < U; base += SPT * PLAN_BLOCK) {
+        int nid[SPT];
+#pragma unroll
+        for (int j = 0; j < SPT; j++) {
+            const int u = base + j * PLAN_BLOCK;
+            nid[j] = u < U ? unodes[u0 + u] : -1;
+        }
+        T pv[SPT][3];
+        bool mk[SPT][3];
+#pragma unroll
+        for (int j = 0; j < SPT; j++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const int n = nid[j];
+                pv[j][d] = n >= 0 ? p[3 * n + d] : T(0);
+                mk[j][d] = n >= 0 && mask && mask[3 * n + d];
+            }
+#pragma unroll
+        for (int j = 0; j < SPT; j++) {
+            const int u = base + j * PLAN_BLOCK;
+            if (u >= U) break;
+#pragma unroll
+            for (int d = 0; d < 3; d++) sP[d * max_u + u] = mk[j][d] ? T(0) : pv[j][d];
+        }
+    }
+    __syncthreads();
+    T pe[10][3];
+    if (active) {
+#pragma unroll
+        for (int a = 0; a < 10; a++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) pe[a][i] = sP[i * max_u + sl[a]];
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) sO[NO - 3 + threadIdx.x] = T(0);
+    if (active) {
+        tangent_vertex_t10<T, true>([&](int i, int a) { return pe[a][i]; }, sl, nullptr, sO, threadIdx.x, max_u, Mn,
+                                    dJn, tan_lam(mat, inst), tan_mu(mat, inst), mat.k_scale);
+    } else {
+#pragma unroll
+        for (int a = 0; a < 10; a++) vertex_out<T, true>(nullptr, sO, max_u, threadIdx.x, a, 0, T(0), T(0), T(0));
+    }
+    __syncthreads();
+    // node sums: gather position g (wave-uniform table of position wave g >> 6)
+#pragma unroll
+    for (int r = 0; r < GR; r++) {
+        const int g = threadIdx.x + r * PLAN_BLOCK;
+        if ((g >> 6) >= nw) break;
+        const unsigned info = ginfo[r];
+        const unsigned short* li = sInc + (info >> 8) + (g & 63);
+        const int c = (int)(info & 255u);
+        T s0 = T(0), s1 = T(0), s2 = T(0);
+#pragma unroll 4
+        for (int j = 0; j < c; j++) {
+            const T* o = sO + 3 * (int)li[j * 64];
+            s0 += o[0];
+            s1 += o[1];
+            s2 += o[2];
+        }
+        if (g < U) {
+            T* o = part + 3 * (u0 + gslot[r]);
+            o[0] = s0;
+            o[1] = s1;
+            o[2] = s2;
         }
     }
 }
@@ -659,7 +812,17 @@ int launch_tangent_partials(const tlfea_force_plan* pl, const double* x, const d
         // the rest-state and affine variants have the standard rule only (elsewhere the
         // general kernel computes the same values from u = 0 and the per-point M)
         const bool aff = mat.affine != 0;
-        if (std_rule && mat.at_rest && aff)
+        // FP32 without a mass term or p.Kp shares: the element-major kernel (in FP64 its 61 KB
+        // of outputs per block leave two blocks per CU, and the LDS-atomic kernel is faster)
+        if (std_rule && mat.at_rest && aff && mass == 0 && !dot_part && pl->d_ginc && sizeof(T) == 4 &&
+            pl->max_u <= 4 * PLAN_BLOCK) {
+            const size_t lds = align16(sizeof(T) * (10 * PLAN_BLOCK + 1) * 3) +
+                               align16(sizeof(unsigned short) * pl->max_ginc);
+            k_tangent_em<T><<<nb, PLAN_BLOCK, lds, s>>>(pl->n_elem, pl->d_blk_off, pl->d_unodes, pl->d_slot, p, mask,
+                                                       jinv_t, detJ_t, mat, part, pl->max_u, flags, blk_list,
+                                                       pl->d_ginc_off, pl->d_ginc, pl->d_gwinfo, pl->max_gw,
+                                                       pl->d_gperm);
+        } else if (std_rule && mat.at_rest && aff)
             TLFEA_TANGENT_BY_MASS(10, 5, true, true, true);
         else if (std_rule && mat.at_rest)
             TLFEA_TANGENT_BY_MASS(10, 5, true, true, false);
