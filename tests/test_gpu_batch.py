@@ -172,6 +172,35 @@ def test_batch_graph_matches_eager(gpu):
         assert np.all(np.abs(b.iters - it_e) <= np.maximum(3, 0.2 * it_e)), (b.iters, it_e)
 
 
+@pytest.mark.parametrize("precond", [False, True])
+def test_batch_graph_kept_across_solves(gpu, precond):
+    """The iteration graph captured by one solve is replayed by the next ones while poll,
+    tol and the set-up are unchanged: solves of f, 2 f (another output buffer) and f again
+    from the first solution as guess give u, 2 u (CG is homogeneous in f, and doubling is
+    exact) and u again, each equal to an eager solve; a new tol re-captures and still
+    agrees."""
+    import torch
+    from tlfea_amd.batch import Batch
+    items = _items(6, 4, big=precond)
+    b = Batch(items)
+    if precond:
+        b.precond_setup(degree=6)
+        b.coarse_setup()
+    f = b.f.clone()
+    ue = _np(b.solve(tol=1e-10, graph=False, poll=8, precond=precond))
+    u1 = _np(b.solve(tol=1e-10, graph=True, poll=8, precond=precond))
+    out = torch.empty_like(b.f)
+    u2 = _np(b.solve(tol=1e-10, graph=True, poll=8, precond=precond, f=2 * f, out=out))
+    u3 = _np(b.solve(tol=1e-10, graph=True, poll=8, precond=precond, u0=torch.as_tensor(u1).cuda()))
+    assert b.rc == 0 and np.all(b.status == 0)
+    assert rel_err(u1, ue) < 1e-8
+    assert rel_err(u2, 2 * ue) < 1e-8
+    assert rel_err(u3, ue) < 1e-8
+    u4 = _np(b.solve(tol=1e-12, graph=True, poll=8, precond=precond))
+    assert b.rc == 0 and rel_err(u4, ue) < 1e-8
+    b.close()
+
+
 def test_batch_guess_zero_load_and_limits(gpu):
     import torch
     from tlfea_amd import capi

@@ -73,6 +73,11 @@ struct tlfea_batch {
     hipStream_t s_own = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_iters = 0, graph_precond = -1;
+    double graph_tol = -1.0;
+    // the CG iterates on this buffer (the caller's u is copied in and out), so a captured
+    // graph stays valid from one solve to the next while the lists, poll, precond, tol and
+    // the set-up it bakes in are unchanged
+    double* d_usol = nullptr;
     // two-level coarse correction (tlfea_batch_coarse_setup): 6 rigid modes per node plane
     // across each instance's long axis; coarse DOF 6 p + j of (global, padded) plane p;
     // blocks of W planes (NB = 6 W coarse DOFs) form a block-tridiagonal matrix
@@ -1182,7 +1187,7 @@ void free_batch(tlfea_batch* b) {
                     b->d_ux, b->d_uy, b->d_uz, b->d_node_plane, b->d_node_r, b->d_plane_off,
                     b->d_plane_nodes, b->d_plane_inst, b->d_plane_local, b->d_inst_cblk, b->d_cdiag,
                     b->d_clow, b->d_cg, b->d_crhs, b->d_cx, b->d_nvk, b->d_node_inst, b->d_npart, b->d_ns, b->d_nmc, b->d_nrs, b->d_cdense_off, b->d_cdense,
-                    b->d_cd_row, b->d_cd_inst, b->d_cinst_pn};
+                    b->d_cd_row, b->d_cd_inst, b->d_cinst_pn, b->d_usol};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (b->s_own) (void)hipStreamDestroy(b->s_own);
@@ -1463,7 +1468,7 @@ extern "C" int tlfea_batch_create(int nn, int n_inst, const int* elem_off, const
         (rc = dalloc(&b->d_d32, nd)) || (rc = dalloc(&b->d_q32, nd)) || (rc = dalloc(&b->d_s, (size_t)n_inst * NS)) ||
         (rc = dalloc(&b->d_i, (size_t)n_inst * NI)) || (rc = dalloc(&b->d_all, 4)) ||
         (rc = dalloc(&b->d_cheb, (size_t)n_inst * NC)) || (rc = dalloc(&b->d_rold, nd)) ||
-        (rc = dalloc(&b->d_dinv32, nd)))
+        (rc = dalloc(&b->d_dinv32, nd)) || (rc = dalloc(&b->d_usol, nd)))
         return fail(rc);
     if (hipMemcpyAsync(b->d_conn, conn, sizeof(int) * (size_t)n_elem * nn, hipMemcpyDeviceToDevice, s) ||
         hipMemcpyAsync(b->d_X, x, sizeof(double) * n_nodes, hipMemcpyDeviceToDevice, s) ||
@@ -1555,11 +1560,11 @@ extern "C" int tlfea_batch_autotune(tlfea_batch* b, const double* f, double* u, 
     for (int chunk : chunk_set) {
         if (int rc = build_chunks(b, chunk)) return rc;
         for (int poll : poll_set) {
-            // per-iteration time from runs of n and 2n iterations: the graph capture every
-            // solve pays (and the host round trips at its ends) cancel in the difference.
-            // Each length is timed twice and the faster run kept (the first run after a
-            // chunk change pays one-time costs); a difference lost in the noise falls back
-            // to the 2n run's mean, an upper bound.
+            // per-iteration time from runs of n and 2n iterations: the host round trips at
+            // the ends of a solve cancel in the difference.  Each length is timed twice and
+            // the faster run kept (the first run after a chunk or poll change pays one-time
+            // costs, the graph capture among them); a difference lost in the noise falls
+            // back to the 2n run's mean, an upper bound.
             const int n = poll * ((trial_iters + poll - 1) / poll);
             double t1 = 1e300, t2 = 1e300;
             for (int rep = 0; rep < 2; rep++) {
@@ -2112,10 +2117,10 @@ extern "C" int tlfea_batch_set_compaction(tlfea_batch* b, double ratio) {
     return TLFEA_OK;
 }
 
-extern "C" int tlfea_batch_cg_solve(tlfea_batch* b, const double* f, double* u, int use_guess, double tol,
+extern "C" int tlfea_batch_cg_solve(tlfea_batch* b, const double* f, double* u_out, int use_guess, double tol,
                                     int max_iter, int poll, int use_graph, int precond, int* iters, double* rel_res,
                                     int* status, int* total_iters, void* stream) {
-    TLFEA_REQUIRE(b && f && u, "null argument");
+    TLFEA_REQUIRE(b && f && u_out, "null argument");
     TLFEA_REQUIRE(precond == 0 || b->cheb_degree > 0, "preconditioner requested but not set up");
     TLFEA_REQUIRE(max_iter >= 0 && tol >= 0.0, "bad tolerance or iteration limit");
     if (poll <= 0) poll = b->poll_default;
@@ -2130,7 +2135,11 @@ extern "C" int tlfea_batch_cg_solve(tlfea_batch* b, const double* f, double* u, 
     full_lists(b);
     const Chunks ch = chunks(b);
     const size_t nd = 3 * (size_t)b->n_nodes;
-    if (!use_guess) TLFEA_HIP_TRY(hipMemsetAsync(u, 0, sizeof(double) * nd, s));
+    double* u = b->d_usol;
+    if (use_guess)
+        TLFEA_HIP_TRY(hipMemcpyAsync(u, u_out, sizeof(double) * nd, hipMemcpyDeviceToDevice, s));
+    else
+        TLFEA_HIP_TRY(hipMemsetAsync(u, 0, sizeof(double) * nd, s));
     // r = f - K u0
     if (int rc = apply64(b, u, b->d_q, nullptr, s)) return rc;
     k_binit<<<b->n_chunks, BT, 0, s>>>(ch, f, b->d_q, b->d_mask, b->d_r, b->d_part, b->d_all);
@@ -2157,10 +2166,9 @@ extern "C" int tlfea_batch_cg_solve(tlfea_batch* b, const double* f, double* u, 
     auto iteration = [&](hipStream_t st, int k) {
         return precond ? pcg_iteration(b, u, tol, st) : cg_iteration(b, u, tol, st, k & 1);
     };
-    if (use_graph && (b->graph == nullptr || b->graph_iters != poll || b->graph_precond != precond)) {
-        if (b->graph) (void)hipGraphExecDestroy(b->graph);
-        b->graph = nullptr;
-    }
+    if (use_graph && (b->graph == nullptr || b->graph_iters != poll || b->graph_precond != precond ||
+                      b->graph_tol != tol))
+        drop_graph(b);
     while (active > 0 && it < max_iter) {
         const int n = std::min(poll, max_iter - it);
         if (use_graph && n == poll) {
@@ -2184,6 +2192,7 @@ extern "C" int tlfea_batch_cg_solve(tlfea_batch* b, const double* f, double* u, 
                 }
                 b->graph_iters = poll;
                 b->graph_precond = precond;
+                b->graph_tol = tol;
             }
             TLFEA_HIP_TRY(hipGraphLaunch(b->graph, s));
         } else {
@@ -2212,11 +2221,7 @@ extern "C" int tlfea_batch_cg_solve(tlfea_batch* b, const double* f, double* u, 
         k_breduce_tail<<<b->n_inst, 64, 0, s>>>(b->n_inst, b->d_inst_chunk_off, b->d_part, b->d_s, b->d_i, tol);
         TLFEA_LAUNCH_CHECK();
     }
-    // the graph bakes in u: a later solve with another output buffer must re-capture
-    if (use_graph && b->graph) {
-        (void)hipGraphExecDestroy(b->graph);
-        b->graph = nullptr;
-    }
+    TLFEA_HIP_TRY(hipMemcpyAsync(u_out, u, sizeof(double) * nd, hipMemcpyDeviceToDevice, s));
     // s is a non-blocking stream, unordered with the null stream: the copies of the
     // per-instance results go on s, after the tail kernels that write them
     std::vector<double> S((size_t)b->n_inst * NS);
