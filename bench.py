@@ -90,15 +90,15 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes of one matrix-free tangent launch pair (k_tangent + k_gather_tangent) on the
 # 200x20x20 beam: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (KiB), scales calibrated
 # on a 512 MiB copy
-PMC_TANGENT_BYTES = (51220 * 2.0 + 29066 + 29871 * 2.0 + 21776) * 1024
+PMC_TANGENT_BYTES = (51216 * 2.0 + 29066 + 27367 * 2.0 + 21638) * 1024
 # the same for the rest-state (linear-elasticity) form the linear-static step runs
-PMC_TANGENT_REST_BYTES = (35012 * 2.0 + 29066 + 29871 * 2.0 + 21776) * 1024
-# the FP32 tangent pair of the two-level stage (k_tangent<float> + k_bgather<float>)
-PMC_TANGENT32_REST_BYTES = (22553 * 2.0 + 14533 + 19216 * 2.0 + 7938) * 1024
+PMC_TANGENT_REST_BYTES = (35011 * 2.0 + 29066 + 27364 * 2.0 + 21803) * 1024
+# the FP32 tangent pair of the two-level stage (k_tangent_em<float> + k_bgather<float>)
+PMC_TANGENT32_REST_BYTES = (29260 * 2.0 + 14534 + 16766 * 2.0 + 7912) * 1024
 # the same for the force kernels with k_gather_partials: gradient-reading (k_force_planned)
 # and rest-coordinate (k_force_rest)
-PMC_FORCE_BYTES = (306420 * 2.0 + 29085 + 20867 * 2.0 + 15799) * 1024
-PMC_FORCE_REST_BYTES = (23356 * 2.0 + 29066 + 20867 * 2.0 + 15799) * 1024
+PMC_FORCE_BYTES = (306428 * 2.0 + 29085 + 18373 * 2.0 + 15799) * 1024
+PMC_FORCE_REST_BYTES = (23357 * 2.0 + 29066 + 18366 * 2.0 + 15799) * 1024
 
 
 def log(*a):
@@ -548,6 +548,17 @@ def main():
     torch.cuda.synchronize()
     dt2 = time.perf_counter() - t
     iters2, rel2 = int(bc.iters[0]), float(bc.rel[0])
+    # where a step's time goes (one more step, synchronised between its phases)
+    phase = {}
+    for tag, fn in (("chebyshev_setup", lambda: bc.precond_setup(degree=deg2[0], ratio=deg2[1], power_iters=20)),
+                    ("coarse_setup", lambda: bc.coarse_setup()),
+                    ("solve", lambda: bc.solve(tol=args.tol, precond=True, max_iter=20000, graph=False))):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        phase[tag] = (time.perf_counter() - t) * 1e3
+    log(f"FP32 two-level step phases (ms): {phase}")
     u2_diff = (torch.linalg.norm(u2 - ls.u) / torch.linalg.norm(ls.u)).item()
     true_res2 = (torch.linalg.norm((body.f_ext - body.tangent_apply(u2, x=body.X0))[free])
                  / torch.linalg.norm(body.f_ext[free])).item()
@@ -747,6 +758,7 @@ def main():
                    "n_elem": m.n_elem, "n_dof": 3 * m.n_nodes, "iterations": iters2, "chebyshev_degree": deg2[0],
                    "chebyshev_ratio": deg2[1],
                    "degree_trials_s": {f"{k[0]}/{k[1]:g}": v for k, v in trial.items()},
+                   "step_phases_ms": phase,
                    "final_rel_residual": rel2, "true_rel_residual": true_res2,
                    "u_rel_diff_vs_matfree_fp64": u2_diff, "K_nnz": dof_nnz,
                    "plain_fp64": {"solves_per_s": args.steps / dt64, "ms_per_solve": dt64 / args.steps * 1e3,
@@ -758,9 +770,9 @@ def main():
                    "speedup_vs_plain_fp64": dt64 / dt2, "speedup_vs_matfree_fp64": dt / dt2},
         "roofline": {"bound": "hbm", "achieved": t32_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": t32_gbs / HBM_PEAK_GBS, "traffic": PMC_TANGENT32_REST_BYTES if default else None,
-                     "kernel": "k_tangent<10,5,float,true,0,true,true> + k_bgather<float> (the FP32 operator of the "
-                               "two-level stage at the rest positions, the linear-elasticity form of an affine mesh "
-                               "integrated from the corner values)",
+                     "kernel": "k_tangent_em<float> + k_bgather<float> (the FP32 operator of the two-level stage "
+                               "at the rest positions: the linear-elasticity form of an affine mesh integrated from "
+                               "the corner values, element outputs summed per node in LDS in a fixed order)",
                      "avg_us": t32_s * 1e6, "bytes_per_launch": t32_bytes},
         "tangent_fp64": {"bound": "hbm", "achieved": tan_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": tan_gbs / HBM_PEAK_GBS, "traffic": traffic_tan,

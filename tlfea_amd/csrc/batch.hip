@@ -173,7 +173,7 @@ __device__ __forceinline__ void wave_inst_sums(const int* __restrict__ ico, cons
 // q = sum of partials per node (or p on masked DOFs); chunk partial of p.q
 template <typename T>
 __global__ __launch_bounds__(BT) void k_bgather(Chunks ch, const int* __restrict__ g_off,
-                                                const int64_t* __restrict__ g_idx, const T* __restrict__ part,
+                                                const int* __restrict__ g_idx, const T* __restrict__ part,
                                                 const unsigned char* __restrict__ mask, const T* __restrict__ p,
                                                 T* __restrict__ q, double* __restrict__ cpart,
                                                 const int* __restrict__ iflags) {
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(BT) void k_bdir(Chunks ch, double* __restrict__ p, 
 
 // w = K r (r on masked DOFs); chunk shares of r.w and r.r; block 0 clears the active count
 __global__ __launch_bounds__(BT) void k_bgather_rw(Chunks ch, const int* __restrict__ g_off,
-                                                   const int64_t* __restrict__ g_idx, const double* __restrict__ part,
+                                                   const int* __restrict__ g_idx, const double* __restrict__ part,
                                                    const unsigned char* __restrict__ mask,
                                                    const double* __restrict__ r, double* __restrict__ w,
                                                    double* __restrict__ cpart, const int* __restrict__ iflags,
@@ -490,7 +490,7 @@ __global__ __launch_bounds__(BT) void k_cheb_init(Chunks ch, const double* __res
 // on masked DOFs, the operator's identity rows), then res -= q, d = a d + b D^-1 res,
 // zc += d, as k_bgather<float> followed by k_cheb_step.
 __global__ __launch_bounds__(BT) void k_bgather_cheb(Chunks ch, const int* __restrict__ g_off,
-                                                     const int64_t* __restrict__ g_idx, const float* __restrict__ part,
+                                                     const int* __restrict__ g_idx, const float* __restrict__ part,
                                                      const unsigned char* __restrict__ mask, int k,
                                                      const float* __restrict__ dinv, const double* __restrict__ C,
                                                      const int* __restrict__ I, float* __restrict__ res,
@@ -2287,7 +2287,7 @@ __global__ void k_bnes_init(int n_inst, const double* __restrict__ alpha, double
 // size was just halved starts again from v_k = v_(k-1) = v_n.  all[0] is zeroed for
 // k_bnes_finalize's count.
 __global__ __launch_bounds__(BT) void k_bnes_grad(Chunks ch, const int* __restrict__ g_off,
-                                                  const int64_t* __restrict__ g_idx, const double* __restrict__ part,
+                                                  const int* __restrict__ g_idx, const double* __restrict__ part,
                                                   const unsigned char* __restrict__ mask,
                                                   const double* __restrict__ fext, const double* __restrict__ vn,
                                                   double* __restrict__ vkm1, double* __restrict__ v,

@@ -367,7 +367,7 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_force_rest(
 }
 
 // f[3i+d] (+)= sum over node i's partials, in ascending partial order (deterministic).
-__global__ void k_gather_partials(int n_nodes, const int* __restrict__ g_off, const int64_t* __restrict__ g_idx,
+__global__ void k_gather_partials(int n_nodes, const int* __restrict__ g_off, const int* __restrict__ g_idx,
                                   const double* __restrict__ part, double* __restrict__ f, int accumulate,
                                   const int* __restrict__ flags) {
     if (flags && (flags[0] | flags[1])) return;
@@ -610,9 +610,14 @@ extern "C" int tlfea_force_plan_create(int nn, int n_elem, int n_nodes, const in
     std::vector<int> g_off(n_nodes + 1, 0);
     for (int u : unodes) g_off[u + 1]++;
     for (int i = 0; i < n_nodes; i++) g_off[i + 1] += g_off[i];
-    std::vector<int64_t> g_idx(unodes.size());
+    if (unodes.size() >= (size_t)INT32_MAX) {
+        free_plan(P);
+        set_error("%zu block partials: beyond the plan's 32-bit partial index", unodes.size());
+        return TLFEA_ECAPACITY;
+    }
+    std::vector<int> g_idx(unodes.size());
     std::vector<int> cur(g_off.begin(), g_off.end() - 1);
-    for (int64_t k = 0; k < (int64_t)unodes.size(); k++) g_idx[cur[unodes[k]]++] = k;
+    for (int64_t k = 0; k < (int64_t)unodes.size(); k++) g_idx[cur[unodes[k]]++] = (int)k;
     // a node's first partial in block order: where the fused CG counts its masked DOFs
     std::vector<unsigned char> own(unodes.size(), 0), seen(n_nodes, 0);
     for (size_t k = 0; k < unodes.size(); k++)

@@ -32,7 +32,7 @@ struct tlfea_force_plan {
     unsigned short* d_slot = nullptr;  // AoSoA [tile][a][64]: slot of node a of element tile*64+lane
     double* d_part = nullptr;       // [n_partials*3] block partial forces
     int* d_g_off = nullptr;         // [n_nodes+1] node -> partial list
-    int64_t* d_g_idx = nullptr;     // partial positions, ascending per node
+    int* d_g_idx = nullptr;         // partial positions, ascending per node (n_partials < 2^31)
     unsigned char* d_own = nullptr; // [n_partials] 1 on a node's first partial (lowest block)
     double* d_dot = nullptr;        // [2][n_blocks] per-block p.Kp and p.p shares of the fused CG
     // In-block node sums of element-major outputs (TET10 plans): a block's nodes are summed
@@ -199,13 +199,13 @@ int launch_force(int nn, int n_elem, const int* conn, const double* x, const dou
 // costs two memory round trips per four entries rather than two per entry; the additions
 // run in the same order as a plain loop, so the sums are the same to the bit.
 template <typename T>
-__device__ __forceinline__ void gather_slots3(const int64_t* __restrict__ idx, const T* __restrict__ part, int k0,
+__device__ __forceinline__ void gather_slots3(const int* __restrict__ idx, const T* __restrict__ part, int k0,
                                               int k1, T& s0, T& s1, T& s2) {
     int k = k0;
     for (; k + 4 <= k1; k += 4) {
         int64_t j[4];
 #pragma unroll
-        for (int u = 0; u < 4; u++) j[u] = 3 * idx[k + u];
+        for (int u = 0; u < 4; u++) j[u] = 3 * (int64_t)idx[k + u];
         T a[4][3];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
@@ -221,7 +221,7 @@ __device__ __forceinline__ void gather_slots3(const int64_t* __restrict__ idx, c
         }
     }
     for (; k < k1; k++) {
-        const T* pp = part + 3 * idx[k];
+        const T* pp = part + 3 * (int64_t)idx[k];
         s0 += pp[0];
         s1 += pp[1];
         s2 += pp[2];

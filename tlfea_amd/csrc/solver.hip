@@ -272,7 +272,7 @@ __global__ __launch_bounds__(S_RED) void k_grad(Body b, int64_t n_slices, const 
 // node here; one thread per node, the same five partial sums.
 template <int MODE>
 __global__ __launch_bounds__(S_RED) void k_grad_mf(Body b, const int* __restrict__ g_off,
-                                                   const int64_t* __restrict__ g_idx, const double* __restrict__ fpart,
+                                                   const int* __restrict__ g_idx, const double* __restrict__ fpart,
                                                    const double* __restrict__ vg, const double* __restrict__ lam,
                                                    const int* __restrict__ cons_off, const int* __restrict__ cons_idx,
                                                    const unsigned char* __restrict__ node_fixed, double dt, double rho,

@@ -733,7 +733,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_cg_update(int64_t n, double* __
 // gamma_old / alpha_old live in scal[12 + parity] / scal[14 + parity], so no block reads
 // a slot another block of the same launch writes.  One thread per node.
 __global__ __launch_bounds__(RED_THREADS) void k_cgcg_update(
-    int n_nodes, const int* __restrict__ g_off, const int64_t* __restrict__ g_idx, const double* __restrict__ bpart,
+    int n_nodes, const int* __restrict__ g_off, const int* __restrict__ g_idx, const double* __restrict__ bpart,
     const unsigned char* __restrict__ mask, double* __restrict__ u, double* __restrict__ r, double* __restrict__ p,
     double* __restrict__ sv, const double* __restrict__ dot, int nb, double* __restrict__ scal,
     int* __restrict__ flags, int k, double tol) {

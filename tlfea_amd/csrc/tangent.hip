@@ -640,7 +640,7 @@ __global__ __launch_bounds__(PLAN_BLOCK) void k_tangent_em(
 
 // out[3i+d] = sum of node i's partials, or p for masked DOFs; optional block partials of p.out
 __global__ __launch_bounds__(256) void k_gather_tangent(int n_nodes, const int* __restrict__ g_off,
-                                                        const int64_t* __restrict__ g_idx,
+                                                        const int* __restrict__ g_idx,
                                                         const double* __restrict__ part,
                                                         const unsigned char* __restrict__ mask,
                                                         const double* __restrict__ p, double* __restrict__ out,
