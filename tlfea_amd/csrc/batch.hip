@@ -813,7 +813,8 @@ __device__ __forceinline__ void plane_restrict(int p, const int* __restrict__ pl
 
 // A(p', (p*, j)) = (P^T w)_p' for the coloured plane p* within W of p' (same instance);
 // diagonal blocks and the lower blocks A_(k, k-1) are stored (row-major NB x NB).
-__global__ __launch_bounds__(64) void k_coarse_probe_store(int W, int period, int color, int j,
+template <int NT>
+__global__ __launch_bounds__(NT) void k_coarse_probe_store(int W, int period, int color, int j,
                                                            const int* __restrict__ plane_off,
                                                            const int* __restrict__ plane_nodes,
                                                            const int* __restrict__ plane_local,
@@ -834,7 +835,7 @@ __global__ __launch_bounds__(64) void k_coarse_probe_store(int W, int period, in
     if (ls < 0 || ls >= inst_plane_n[plane_inst[pp]]) return;
     const int ps = pp + (ls - lp);
     double out[6];
-    plane_restrict(pp, plane_off, plane_nodes, node_r, w, out);
+    plane_restrict<NT>(pp, plane_off, plane_nodes, node_r, w, out);
     if (threadIdx.x != 0) return;
     const int NB = 6 * W;
     const int bp = pp / W, bs = ps / W;
@@ -910,15 +911,31 @@ __global__ __launch_bounds__(64) void k_coarse_factor(const int* __restrict__ in
     __shared__ int dead[CNB_MAX];
     const int i = blockIdx.x;
     const int k0 = inst_cblk[i], k1 = inst_cblk[i + 1];
-    const int nn = NB * NB;
+    constexpr int nn = NB * NB, PER = (nn + 63) / 64;
+    // the chain's global loads of block k + 1 are issued before block k's arithmetic, so
+    // no step waits on memory
+    double nD[PER], nA[PER];
+    auto fetch = [&](int k) {
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int e = (int)threadIdx.x + 64 * q;
+            nD[q] = e < nn ? cdiag[(int64_t)k * nn + e] : 0.0;
+            nA[q] = e < nn && k > k0 ? clow[(int64_t)k * nn + e] : 0.0;  // A_(k, k-1)
+        }
+    };
+    if (k0 < k1) fetch(k0);
     for (int k = k0; k < k1; k++) {
         double* D = cdiag + (int64_t)k * nn;
-        const double* Ag = clow + (int64_t)k * nn;  // A_(k, k-1), staged in LDS
-        for (int e = threadIdx.x; e < nn; e += 64) {
-            M[e] = D[e];
-            A[e] = k > k0 ? Ag[e] : 0.0;
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int e = (int)threadIdx.x + 64 * q;
+            if (e < nn) {
+                M[e] = nD[q];
+                A[e] = nA[q];
+            }
         }
         __syncthreads();
+        if (k + 1 < k1) fetch(k + 1);
         if (k > k0) {
             // G = A Dprev^-1 (Dprev holds the inverse), then M -= G A^T
             for (int e = threadIdx.x; e < nn; e += 64) {
@@ -1943,9 +1960,11 @@ int coarse_numeric(tlfea_batch* b, hipStream_t s) {
             k_coarse_probe_vec<<<grid_for(N, 256), 256, 0, s>>>(N, b->d_node_plane, b->d_node_r, b->d_plane_local,
                                                                 period, color, j, b->d_p);
             if ((rc = apply64(b, b->d_p, b->d_q, nullptr, s))) break;
-            k_coarse_probe_store<<<P, 64, 0, s>>>(W, period, color, j, b->d_plane_off, b->d_plane_nodes,
-                                                 b->d_plane_local, b->d_plane_inst, b->d_cinst_pn, b->d_node_r,
-                                                 b->d_q, b->d_cdiag, b->d_clow);
+            // four waves per plane on large planes (as the restriction)
+            (b->restrict_wide ? k_coarse_probe_store<256> : k_coarse_probe_store<64>)<<<P, b->restrict_wide ? 256 : 64,
+                                                                                         0, s>>>(
+                W, period, color, j, b->d_plane_off, b->d_plane_nodes, b->d_plane_local, b->d_plane_inst,
+                b->d_cinst_pn, b->d_node_r, b->d_q, b->d_cdiag, b->d_clow);
             if (hipGetLastError() != hipSuccess) {
                 set_error("coarse probe launch failed");
                 rc = TLFEA_EHIP;
