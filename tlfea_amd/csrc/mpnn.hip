@@ -8,10 +8,12 @@
 //                  pre_l = [h, m] W_l^T + b_l                     (FP32)
 //                  h = bf16(h + relu(pre_l))
 //   out = h W_out^T + b_out                                       (FP32, 3 used of 16)
-// The GEMMs run on the matrix cores (v_mfma_f32_16x16x16_bf16: 16x16 tiles, K = 16 per
-// instruction, 4 FP32 results per lane).  Backward: the same layout, with dpre rounded to
+// The GEMMs run on the matrix cores (v_mfma_f32_16x16x32_bf16, gfx950: 16x16 tiles, K = 32
+// per instruction, 8 BF16 operands and 4 FP32 results per lane; operand fragments are one
+// 16-byte load per lane).  Backward: the same layout, with dpre rounded to
 // BF16 before each GEMM; weight gradients are reduced over nodes in per-block FP32
 // partials summed in a fixed order.  Optimiser: Adam on FP32 master weights.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -20,14 +22,15 @@
 
 namespace tlfea_hip {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short bf16;
 
 constexpr int MW = 64;     // hidden width
 constexpr int MFEAT = 16;  // input features (padded)
 constexpr int MOUT = 16;   // output rows (3 used, padded to one MFMA tile)
-constexpr int WG_ROWS = 128;  // nodes per weight-gradient block
+constexpr int WG_ROWS = 128;    // nodes per weight-gradient chunk (staged in LDS)
+constexpr int WG_BLOCKS = 1024; // at most this many weight-gradient blocks (partials)
 
 // round to nearest even (finite inputs)
 __device__ __forceinline__ bf16 f2bf(float f) {
@@ -37,8 +40,20 @@ __device__ __forceinline__ bf16 f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(bf16 b) { return __uint_as_float((uint32_t)b << 16); }
 
+__device__ __forceinline__ bf16x8 zero8() {
+    bf16x8 v;
+#pragma unroll
+    for (int j = 0; j < 8; j++) v[j] = __builtin_bit_cast(__bf16, (unsigned short)0);
+    return v;
+}
+// 8 consecutive BF16 values (16-byte aligned)
+__device__ __forceinline__ bf16x8 load8(const bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
+
 // Y[M, N] = sum_p A_p[M, K_p] B_p, with B_p given transposed as BT_p[n * ldb + k] (row n of
-// B^T is column n of B).  One wave computes 16 rows x N columns; 4 waves per block.
+// B^T is column n of B).  One wave computes 16 rows x N columns; 4 waves per block.  Per
+// 32-wide k step lane (li, kg) holds k = 8 kg .. 8 kg + 7 of A's row li and of B^T's row li of
+// each 16-column tile (K a multiple of 16: the k >= K half of the last step is zero; rows and
+// ldb multiples of 8, so every fragment is one aligned 16-byte load).
 // Epilogue (mode):
 //   0: y[r * ldy + c] = acc + (bias ? bias[c] : 0)                      (FP32)
 //   1: pre = acc + bias -> pre[r * N + c] (FP32);
@@ -65,17 +80,14 @@ __global__ __launch_bounds__(256) void k_gemm(int M, const bf16* __restrict__ A0
         const int ldb = part ? ldb1 : ldb0;
         if (!A) continue;
         const int row = r0 + li;
-        for (int k0 = 0; k0 < K; k0 += 16) {
-            bf16x4 a = {0, 0, 0, 0};
-            if (row < M) {
-                const bf16* ap = A + (int64_t)row * K + k0 + 4 * kg;
-                a = bf16x4{(short)ap[0], (short)ap[1], (short)ap[2], (short)ap[3]};
-            }
+        for (int k0 = 0; k0 < K; k0 += 32) {
+            const int kk = k0 + 8 * kg;
+            const bool kin = kk < K;
+            const bf16x8 a = row < M && kin ? load8(A + (int64_t)row * K + kk) : zero8();
 #pragma unroll
             for (int t = 0; t < N / 16; t++) {
-                const bf16* bp = BT + (int64_t)(t * 16 + li) * ldb + k0 + 4 * kg;
-                const bf16x4 bv = {(short)bp[0], (short)bp[1], (short)bp[2], (short)bp[3]};
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc[t], 0, 0, 0);
+                const bf16x8 bv = kin ? load8(BT + (int64_t)(t * 16 + li) * ldb + kk) : zero8();
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc[t], 0, 0, 0);
             }
         }
     }
@@ -97,44 +109,61 @@ __global__ __launch_bounds__(256) void k_gemm(int M, const bf16* __restrict__ A0
         }
 }
 
-// Weight gradient partials: P[blk][n * K + k] = sum_{m in block} dY[m, n] A[m, k]
-// (dY: [M, N] bf16, A: [M, K] bf16); blocks of WG_ROWS nodes staged in LDS.
+// Weight gradient partials: P[blk][n * K + k] = sum_{m in the block's chunks} dY[m, n] A[m, k]
+// (dY: [M, N] bf16, A: [M, K] bf16).  Block b takes the WG_ROWS-node chunks b, b + grid,
+// b + 2 grid, ... in order, each staged in LDS, and keeps its (at most four per wave) output
+// tiles in registers across them: few partials, in a fixed summation order.
 template <int N, int K>
 __global__ __launch_bounds__(256) void k_wgrad(int M, const bf16* __restrict__ dY, const bf16* __restrict__ A,
                                                float* __restrict__ P) {
     __shared__ bf16 sY[WG_ROWS * N];
     __shared__ bf16 sA[WG_ROWS * K];
-    const int m0 = blockIdx.x * WG_ROWS;
-    for (int t = threadIdx.x; t < WG_ROWS * N; t += 256) {
-        const int m = m0 + t / N;
-        sY[t] = m < M ? dY[(int64_t)m0 * N + t] : (bf16)0;
-    }
-    for (int t = threadIdx.x; t < WG_ROWS * K; t += 256) {
-        const int m = m0 + t / K;
-        sA[t] = m < M ? A[(int64_t)m0 * K + t] : (bf16)0;
-    }
-    __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, kg = lane >> 4;
-    constexpr int TN = N / 16, TK = K / 16;
-    for (int tile = wave; tile < TN * TK; tile += 4) {
-        const int tn = tile / TK, tk = tile % TK;
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        for (int mm = 0; mm < WG_ROWS; mm += 16) {
-            // MFMA A operand: rows n, reduction m -> dY^T[tn*16 + li][mm + 4kg + j]
-            // MFMA B operand: reduction m, cols k -> A[mm + 4kg + j][tk*16 + li]
-            bf16x4 a, b;
+    constexpr int TN = N / 16, TK = K / 16, NT = (TN * TK + 3) / 4;
+    f32x4 acc[NT];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                a[j] = (short)sY[(mm + 4 * kg + j) * N + tn * 16 + li];
-                b[j] = (short)sA[(mm + 4 * kg + j) * K + tk * 16 + li];
-            }
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    for (int q = 0; q < NT; q++) acc[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int nchunks = (M + WG_ROWS - 1) / WG_ROWS;
+    for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const int m0 = c * WG_ROWS;
+        __syncthreads();  // the previous chunk's fragments have been read
+        for (int t = threadIdx.x; t < WG_ROWS * N; t += 256) {
+            const int m = m0 + t / N;
+            sY[t] = m < M ? dY[(int64_t)m0 * N + t] : (bf16)0;
         }
+        for (int t = threadIdx.x; t < WG_ROWS * K; t += 256) {
+            const int m = m0 + t / K;
+            sA[t] = m < M ? A[(int64_t)m0 * K + t] : (bf16)0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NT; q++) {
+            const int tile = wave + 4 * q;
+            if (tile >= TN * TK) break;
+            const int tn = tile / TK, tk = tile % TK;
+            for (int mm = 0; mm < WG_ROWS; mm += 32) {
+                // MFMA A operand: rows n, reduction m -> dY^T[tn*16 + li][mm + 8kg + j]
+                // MFMA B operand: reduction m, cols k -> A[mm + 8kg + j][tk*16 + li]
+                bf16x8 a, b;
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    a[j] = __builtin_bit_cast(__bf16, sY[(mm + 8 * kg + j) * N + tn * 16 + li]);
+                    b[j] = __builtin_bit_cast(__bf16, sA[(mm + 8 * kg + j) * K + tk * 16 + li]);
+                }
+                acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[q], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NT; q++) {
+        const int tile = wave + 4 * q;
+        if (tile >= TN * TK) break;
+        const int tn = tile / TK, tk = tile % TK;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int n = tn * 16 + 4 * kg + j, k = tk * 16 + li;
-            P[(int64_t)blockIdx.x * N * K + n * K + k] = acc[j];
+            P[(int64_t)blockIdx.x * N * K + n * K + k] = acc[q][j];
         }
     }
 }
@@ -145,42 +174,96 @@ __global__ void k_sum_partials(int nb, int len, int cols, int ldg, const float* 
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= len) return;
     float s = 0.f;
+#pragma unroll 8
     for (int b = 0; b < nb; b++) s += P[(int64_t)b * len + i];
     g[(i / cols) * ldg + i % cols] += s;
 }
 
-// column sums of an FP32 [M, N] matrix in blocks of WG_ROWS rows -> P[blk][N]
+// column sums of an FP32 [M, N] matrix -> P[blk][N], block b summing the WG_ROWS-row chunks
+// b, b + grid, ... in order
 __global__ void k_colsum_partials(int M, int N, const float* __restrict__ Y, float* __restrict__ P) {
     const int c = threadIdx.x;
     if (c >= N) return;
-    const int m0 = blockIdx.x * WG_ROWS;
     float s = 0.f;
-    for (int m = m0; m < min(M, m0 + WG_ROWS); m++) s += Y[(int64_t)m * N + c];
+    for (int m0 = blockIdx.x * WG_ROWS; m0 < M; m0 += gridDim.x * WG_ROWS)
+        for (int m = m0; m < min(M, m0 + WG_ROWS); m++) s += Y[(int64_t)m * N + c];
     P[(int64_t)blockIdx.x * N + c] = s;
 }
 
-// m_i = bf16(mean_{j in row i} h_j) over the node graph (CSR includes the diagonal)
+// m_i = bf16(mean_{j in row i} h_j) over the node graph (CSR includes the diagonal).  32
+// lanes per node (two nodes per wave), two features per lane (one 4-byte load per
+// neighbour row), four neighbour rows in flight; neighbours are added in CSR order.
 __global__ __launch_bounds__(256) void k_agg(int n, const int* __restrict__ off, const int* __restrict__ col,
                                              const bf16* __restrict__ h, bf16* __restrict__ m) {
-    const int node = blockIdx.x * 4 + (threadIdx.x >> 6), f = threadIdx.x & 63;
+    const int node = blockIdx.x * 8 + (threadIdx.x >> 5), f2 = threadIdx.x & 31;
     if (node >= n) return;
     const int a = off[node], b = off[node + 1];
-    float s = 0.f;
-    for (int k = a; k < b; k++) s += bf2f(h[(int64_t)col[k] * MW + f]);
-    m[(int64_t)node * MW + f] = f2bf(s / (float)(b - a));
+    const uint32_t* h2 = reinterpret_cast<const uint32_t*>(h);
+    float s0 = 0.f, s1 = 0.f;
+    int k = a;
+    for (; k + 4 <= b; k += 4) {
+        uint32_t v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) v[u] = h2[(int64_t)col[k + u] * (MW / 2) + f2];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            s0 += bf2f((bf16)(v[u] & 0xffffu));
+            s1 += bf2f((bf16)(v[u] >> 16));
+        }
+    }
+    for (; k < b; k++) {
+        const uint32_t v = h2[(int64_t)col[k] * (MW / 2) + f2];
+        s0 += bf2f((bf16)(v & 0xffffu));
+        s1 += bf2f((bf16)(v >> 16));
+    }
+    const float deg = (float)(b - a);
+    reinterpret_cast<uint32_t*>(m)[(int64_t)node * (MW / 2) + f2] =
+        (uint32_t)f2bf(s0 / deg) | ((uint32_t)f2bf(s1 / deg) << 16);
 }
 
-// dh_j += sum_{i in row j} dm_i / deg(i)  (the graph is symmetric)
+// deg(i) per node (as float)
+__global__ void k_deg(int n, const int* __restrict__ off, float* __restrict__ deg) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) deg[i] = (float)(off[i + 1] - off[i]);
+}
+
+// dh_j += sum_{i in row j} dm_i / deg(i)  (the graph is symmetric); 32 lanes per node, two
+// features per lane, four neighbour rows in flight, CSR order
 __global__ __launch_bounds__(256) void k_agg_t(int n, const int* __restrict__ off, const int* __restrict__ col,
-                                               const float* __restrict__ dm, float* __restrict__ dh) {
-    const int node = blockIdx.x * 4 + (threadIdx.x >> 6), f = threadIdx.x & 63;
+                                               const float* __restrict__ deg, const float* __restrict__ dm,
+                                               float* __restrict__ dh) {
+    const int node = blockIdx.x * 8 + (threadIdx.x >> 5), f2 = threadIdx.x & 31;
     if (node >= n) return;
-    float s = 0.f;
-    for (int k = off[node]; k < off[node + 1]; k++) {
-        const int i = col[k];
-        s += dm[(int64_t)i * MW + f] / (float)(off[i + 1] - off[i]);
+    const float2* dm2 = reinterpret_cast<const float2*>(dm);
+    float s0 = 0.f, s1 = 0.f;
+    int k = off[node];
+    const int e = off[node + 1];
+    for (; k + 4 <= e; k += 4) {
+        float2 v[4];
+        float w[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int i = col[k + u];
+            v[u] = dm2[(int64_t)i * (MW / 2) + f2];
+            w[u] = deg[i];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            s0 += v[u].x / w[u];
+            s1 += v[u].y / w[u];
+        }
     }
-    dh[(int64_t)node * MW + f] += s;
+    for (; k < e; k++) {
+        const int i = col[k];
+        const float2 v = dm2[(int64_t)i * (MW / 2) + f2];
+        s0 += v.x / deg[i];
+        s1 += v.y / deg[i];
+    }
+    float2* o = reinterpret_cast<float2*>(dh) + (int64_t)node * (MW / 2) + f2;
+    float2 cur = *o;
+    cur.x += s0;
+    cur.y += s1;
+    *o = cur;
 }
 
 // dpre = dh * (pre > 0) in FP32 and BF16
@@ -342,6 +425,7 @@ struct tlfea_mpnn {
     float *d_pre = nullptr, *d_out = nullptr, *d_dh = nullptr, *d_dm = nullptr, *d_dp = nullptr, *d_dout = nullptr;
     float* d_part = nullptr;
     double* d_lpart = nullptr;
+    float* d_deg = nullptr;  // node degrees of the last forward's graph (the backward's)
 };
 
 namespace {
@@ -372,12 +456,13 @@ int malloc_n(T** p, size_t n) {
 
 void free_ws(tlfea_mpnn* net) {
     void* ps[] = {net->d_x, net->d_h, net->d_mg, net->d_dp16, net->d_dout16, net->d_pre, net->d_out,
-                  net->d_dh, net->d_dm, net->d_dp, net->d_dout, net->d_part, net->d_lpart};
+                  net->d_dh, net->d_dm, net->d_dp, net->d_dout, net->d_part, net->d_lpart, net->d_deg};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     net->d_x = net->d_h = net->d_mg = net->d_dp16 = net->d_dout16 = nullptr;
     net->d_pre = net->d_out = net->d_dh = net->d_dm = net->d_dp = net->d_dout = nullptr;
     net->d_part = nullptr;
+    net->d_deg = nullptr;
     net->d_lpart = nullptr;
     net->cap = 0;
 }
@@ -387,7 +472,7 @@ int ensure_ws(tlfea_mpnn* net, int n) {
     free_ws(net);
     const size_t L = net->layers;
     const size_t nn = (size_t)n;
-    const size_t nblk = (nn + WG_ROWS - 1) / WG_ROWS;
+    const size_t nblk = std::min<size_t>((nn + WG_ROWS - 1) / WG_ROWS, WG_BLOCKS);
     int rc;
     if ((rc = malloc_n(&net->d_x, nn * MFEAT)) || (rc = malloc_n(&net->d_h, (L + 1) * nn * MW)) ||
         (rc = malloc_n(&net->d_mg, L * nn * MW)) || (rc = malloc_n(&net->d_dp16, nn * MW)) ||
@@ -395,7 +480,7 @@ int ensure_ws(tlfea_mpnn* net, int n) {
         (rc = malloc_n(&net->d_out, nn * MOUT)) || (rc = malloc_n(&net->d_dh, nn * MW)) ||
         (rc = malloc_n(&net->d_dm, nn * MW)) || (rc = malloc_n(&net->d_dp, nn * MW)) ||
         (rc = malloc_n(&net->d_dout, nn * MOUT)) || (rc = malloc_n(&net->d_part, nblk * MW * 2 * MW)) ||
-        (rc = malloc_n(&net->d_lpart, (nn + 255) / 256 + 1)))
+        (rc = malloc_n(&net->d_lpart, (nn + 255) / 256 + 1)) || (rc = malloc_n(&net->d_deg, nn)))
         return rc;
     net->cap = n;
     return TLFEA_OK;
@@ -427,7 +512,7 @@ int forward(tlfea_mpnn* net, int n, const int* off, const int* col, hipStream_t 
     for (int l = 1; l <= L; l++) {
         const bf16* h = H + (l - 1) * nn * MW;
         bf16* mg = net->d_mg + (l - 1) * nn * MW;
-        k_agg<<<(n + 3) / 4, 256, 0, s>>>(n, off, col, h, mg);
+        k_agg<<<(n + 7) / 8, 256, 0, s>>>(n, off, col, h, mg);
         const bf16* W = net->d_w16 + net->off_W[l];  // [64 x 128]
         k_gemm<MW><<<gemm_blocks(n), 256, 0, s>>>(n, h, MW, W, 2 * MW, mg, MW, W + MW, 2 * MW,
                                                   net->d_w + net->off_b[l], 1, h, PRE + l * nn * MW,
@@ -444,13 +529,13 @@ int forward(tlfea_mpnn* net, int n, const int* off, const int* col, hipStream_t 
 // g (an N x K block with row stride ldg) += dY^T A
 template <int N, int K>
 void wgrad(tlfea_mpnn* net, int n, const bf16* dY, const bf16* A, float* g, int ldg, hipStream_t s) {
-    const int nb = (n + WG_ROWS - 1) / WG_ROWS;
+    const int nb = std::min((n + WG_ROWS - 1) / WG_ROWS, WG_BLOCKS);
     k_wgrad<N, K><<<nb, 256, 0, s>>>(n, dY, A, net->d_part);
     k_sum_partials<<<grid_for(N * K, 256), 256, 0, s>>>(nb, N * K, K, ldg, net->d_part, g);
 }
 
 void bgrad(tlfea_mpnn* net, int n, int N, const float* dY, float* g, hipStream_t s) {
-    const int nb = (n + WG_ROWS - 1) / WG_ROWS;
+    const int nb = std::min((n + WG_ROWS - 1) / WG_ROWS, WG_BLOCKS);
     k_colsum_partials<<<nb, 64, 0, s>>>(n, N, dY, net->d_part);
     k_sum_partials<<<1, 64, 0, s>>>(nb, N, N, N, net->d_part, g);
 }
@@ -461,6 +546,7 @@ int backward(tlfea_mpnn* net, int n, const int* off, const int* col, hipStream_t
     const size_t nn = (size_t)n;
     const int lo = L + 1;
     const int64_t len = (int64_t)nn * MW;
+    k_deg<<<grid_for(n, 256), 256, 0, s>>>(n, off, net->d_deg);
     // decoder: dW_out += dout^T h_L, db_out += colsum(dout), dh = dout W_out
     wgrad<MOUT, MW>(net, n, net->d_dout16, net->d_h + L * nn * MW, net->d_g + net->off_W[lo], MW, s);
     bgrad(net, n, MOUT, net->d_dout, net->d_g + net->off_b[lo], s);
@@ -484,7 +570,7 @@ int backward(tlfea_mpnn* net, int n, const int* off, const int* col, hipStream_t
                                                   nullptr, nullptr, nullptr, net->d_dp, MW);
         // dh_prev = dh (residual path) + dh_direct + agg^T(dm)
         k_add_f32<<<grid_for(len, 256), 256, 0, s>>>(len, net->d_dp, net->d_dh);
-        k_agg_t<<<(n + 3) / 4, 256, 0, s>>>(n, off, col, net->d_dm, net->d_dh);
+        k_agg_t<<<(n + 7) / 8, 256, 0, s>>>(n, off, col, net->d_deg, net->d_dm, net->d_dh);
     }
     // encoder
     k_relu_bwd<<<grid_for(len, 256), 256, 0, s>>>(len, net->d_dh, net->d_pre, net->d_dp, net->d_dp16);
