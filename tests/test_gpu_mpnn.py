@@ -97,13 +97,15 @@ def test_adam_training_reduces_loss(gpu):
 
 
 def test_warm_start_on_held_out_beams(gpu):
-    """The committed weights (trained on converged solves of the beam sampler) on held-out
-    beams: the raw guess is within 15% (median L2) of the FP64 solution; CG from the
-    energy-scaled guess reaches the same solution; the zero-weight network falls back on
-    every instance.  The iteration counts of both starts are printed: on these slender
-    beams the guess does not shorten CG (its node-level BF16 noise carries more energy
-    than the solution, see the learned module's notes), and the energy scaling keeps it
-    from costing more than a few per cent."""
+    """The committed weights (trained on 20,000 converged solves of the beam sampler) on
+    held-out beams: what the network gains over the zero guess is the prediction itself --
+    the raw guess is within 5% (median L2; 2.5% measured on 256 held-out beams) of the FP64
+    solution where the zero guess is 100% off; CG from the energy-scaled guess reaches the
+    same solution; the zero-weight network falls back on every instance.  The iteration
+    counts of both starts are printed: on these slender beams the guess does not shorten CG
+    (its error, small in L2, sits in modes 1e3-1e4 times stiffer than the beam's bending
+    and carries more energy than the solution, see the learned module's notes), and the
+    energy scaling keeps it from costing more than a few per cent."""
     import torch
     from tlfea_amd import mesh as M
     from tlfea_amd.batch import Batch
@@ -117,7 +119,7 @@ def test_warm_start_on_held_out_beams(gpu):
     raw = net.guess(b, graph=g, policy=None, smooth=0)
     err = [torch.linalg.norm(raw[b.inst_slice(i)] - u_zero[b.inst_slice(i)]).item()
            / torch.linalg.norm(u_zero[b.inst_slice(i)]).item() for i in range(b.n_inst)]
-    assert np.median(err) < 0.15, err
+    assert np.median(err) < 0.05, err
     u0 = net.guess(b, graph=g)
     assert net.fell_back.sum() <= 1
     u_warm = b.solve(tol=1e-8, u0=u0)

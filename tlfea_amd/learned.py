@@ -19,6 +19,16 @@ applies it instead): K amplifies the BF16-level node-to-node noise of any networ
 by up to its condition number, so that test rejects guesses that still save CG work.
 The CG always finishes from u0 with the plain method's own stopping test, so the guess
 changes the iteration count, never the answer's tolerance.
+
+Measured on 256 held-out sampler beams with the weights in ``weights/`` (20,000 training
+solves, 8 epochs): the raw guess is 2.5% off the solution in L2 (median), yet its energy
+error ||u0 - u*||_K^2 is 156 times the solution's energy ||u*||_K^2.  Chebyshev smoothing
+of the guess with the operator itself (degree 16 on [lambda_max / 30, lambda_max], up to
+three sweeps) brings that only to 6-28: the error is not node-level noise alone but sits
+in cross-section modes 1e3-1e4 times stiffer than the beam's bending, where a 2% error
+outweighs the whole solution.  Energy-scaled, every variant stays at 0.7-0.9 of the zero
+guess's energy error, and neither the plain CG (3,075 vs 3,006 median iterations) nor the
+two-level preconditioned CG (54-55 vs 53) gets shorter from it.
 """
 from __future__ import annotations
 
@@ -223,8 +233,10 @@ def _sub_csr(off, col, n0, n1):
     return (o - o[0]).contiguous(), c.contiguous()
 
 
-def generate(n_inst, seed, group=256, tol=1e-8, log=print):
-    """Converged FP64 solves of the beam sampler, returned as training groups."""
+def generate(n_inst, seed, group=256, tol=1e-8, log=print, two_level=True):
+    """Converged FP64 solves of the beam sampler, returned as training groups.  two_level:
+    the flexible FP64 CG with the FP32 two-level preconditioner (the same solutions to the
+    same tolerance as the plain CG, about ten times faster on these batches)."""
     from . import mesh as M
     from .batch import Batch
     data = []
@@ -233,7 +245,10 @@ def generate(n_inst, seed, group=256, tol=1e-8, log=print):
         k = min(group, n_inst - done)
         items = M.random_beam_batch(k, seed + done)
         b = Batch(items)
-        u = b.solve(tol=tol)
+        if two_level:
+            b.precond_setup(degree=6, ratio=250.0, power_iters=20)
+            b.coarse_setup()
+        u = b.solve(tol=tol, precond=two_level)
         g = BatchGraph(b)
         t, w = g.targets(u)
         ok = b.status == 0
