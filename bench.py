@@ -151,8 +151,11 @@ def batched_beams(n, tol, log, sequential=True):
     n_coarse = b.coarse_setup()
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t
-    u2 = timed("fp32_twolevel", lambda: b.solve(tol=tol, precond=True))
-    u2w = timed("fp32_twolevel_learned_start", lambda: b.solve(tol=tol, precond=True, u0=net.guess(b, graph=g)))
+    # the autotuner's replay length is chosen for the plain CG's thousands of iterations; the
+    # two-level solves finish in 50-140, so they poll every 16
+    u2 = timed("fp32_twolevel", lambda: b.solve(tol=tol, precond=True, poll=16))
+    u2w = timed("fp32_twolevel_learned_start",
+                lambda: b.solve(tol=tol, precond=True, poll=16, u0=net.guess(b, graph=g)))
     for k in ("fp32_twolevel", "fp32_twolevel_learned_start"):
         ok = res[k]["converged"]
         res[k]["setup_s"] = setup_s
