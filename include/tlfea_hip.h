@@ -364,8 +364,12 @@ int tlfea_batch_precond_setup(tlfea_batch* b, int degree, double ratio, int powe
 /* Two-level coarse correction for the FP32 stage, not in the reference: the nodes of each
  * instance are grouped into planes across its longest axis (the mesh's lattice planes, or
  * equal-width bins of an unstructured mesh); the coarse space holds the 6 rigid motions of
- * each plane's free nodes, and A_c = P^T A P (A = K, or K + c M with a mass term) is
- * assembled by probing with the FP64 operator
+ * each plane's free nodes and, on request where W <= 2 (tlfea_batch_set_coarse_modes), five
+ * in-plane fields of the plane's cross-section coordinates (s, t): the quadratic warping
+ * (s t, (t^2 - s^2)/2) and ((s^2 - t^2)/2, s t) that bending about either axis brings with
+ * the Poisson effect, the uniform stretch (s, t) and the two shears (t, s), (s, -t).
+ * A_c = P^T A P (A = K, or K + c M with a mass term) is assembled by probing with the FP64
+ * operator
  * and factored per instance as a block-tridiagonal matrix (blocks of W planes, W = the
  * widest plane span of an element, at most 3: TLFEA_ECAPACITY beyond).  tlfea_batch_cg_solve
  * (precond = 1) then applies the additive two-level preconditioner z = P A_c^-1 P^T r +
@@ -373,6 +377,10 @@ int tlfea_batch_precond_setup(tlfea_batch* b, int degree, double ratio, int powe
  * stage cannot reach, go to the coarse solve.  Needs tlfea_batch_precond_setup first;
  * enable = 0 removes it.  n_coarse (may be NULL) receives the coarse dimension. */
 int tlfea_batch_coarse_setup(tlfea_batch* b, int enable, int* n_coarse, void* stream);
+/* Coarse modes per plane for the next tlfea_batch_coarse_setup: 6 (the default: the rigid
+ * motions) or 11 (the rigid motions and the five in-plane fields, used where elements span
+ * at most 2 planes, else the rigid motions alone).  Not in the reference. */
+int tlfea_batch_set_coarse_modes(tlfea_batch* b, int modes);
 /* z = P A_c^-1 P^T r for every instance (FP64; r and z [3 n_nodes] on the device): the
  * coarse part of the preconditioner on its own. */
 int tlfea_batch_coarse_apply(tlfea_batch* b, const double* r, double* z, void* stream);

@@ -320,9 +320,12 @@ def test_batch_autotune(gpu):
         assert rel_err(u[sl], u_ref[sl]) < 1e-8, i
 
 
-def _plane_basis(m):
+def _plane_basis(m, modes=6):
     """P of the coarse space in numpy: translations and rotations about the centroid of the
-    free nodes of each lattice plane x = const (fixed nodes: zero rows)."""
+    free nodes of each lattice plane x = const (fixed nodes: zero rows); with modes = 11 also
+    the in-plane fields of the plane's coordinates (s, t) = (y, z) - centroid: the bending
+    warping (s t, (t^2 - s^2)/2) and ((s^2 - t^2)/2, s t), the stretch (s, t) and the shears
+    (t, s) and (s, -t) in the (y, z) components."""
     X = m.xyz()
     key = np.unique(np.round(X[:, 0], 9), return_inverse=True)[1]
     free = np.ones(m.n_nodes, bool)
@@ -341,16 +344,25 @@ def _plane_basis(m):
                 e = np.eye(3)[j - 3]
                 v[nodes] = np.cross(e, r)
             cols.append(v.reshape(-1))
+        if modes == 11:
+            s_, t_ = r[:, 1], r[:, 2]
+            for fy, fz in ((s_ * t_, 0.5 * (t_ * t_ - s_ * s_)), (0.5 * (s_ * s_ - t_ * t_), s_ * t_), (s_, t_),
+                           (t_, s_), (s_, -t_)):
+                v = np.zeros((m.n_nodes, 3))
+                v[nodes, 1], v[nodes, 2] = fy, fz
+                cols.append(v.reshape(-1))
     return np.stack(cols, 1)
 
 
+@pytest.mark.parametrize("modes", [6, 11])
 @pytest.mark.parametrize("tet10", [True, False])
-def test_coarse_solve_equals_galerkin_projection(gpu, tet10):
+def test_coarse_solve_equals_galerkin_projection(gpu, tet10, modes):
     """tlfea_batch_coarse_apply against P (P^T K P)^-1 P^T r built in numpy from the oracle's
-    Dirichlet-eliminated K and the lattice planes (any basis of the same rigid motions gives
-    the same product): for a random r, and for r = K u_c with u_c in the coarse space, where
-    it returns u_c.  This pins the probing assembly of P^T K P, its block factorisation
-    (TET10: two planes per block; TET4: one), the restriction and the prolongation."""
+    Dirichlet-eliminated K and the lattice planes (any basis of the same space gives the same
+    product), with the rigid motions alone and with the five in-plane fields: for a random
+    r, and for r = K u_c with u_c in the coarse space, where it returns u_c.  This pins the
+    probing assembly of P^T K P, its block factorisation (TET10: two planes per block; TET4:
+    one), the restriction and the prolongation of every mode."""
     import torch
     from tlfea_amd import mesh as M
     from tlfea_amd.batch import Batch
@@ -362,34 +374,36 @@ def test_coarse_solve_equals_galerkin_projection(gpu, tet10):
                                  -100.0)
         items.append((m, dict(E=float(10 ** rng.uniform(6, 10)), nu=0.3)))
     b = Batch(items)
+    b.set_coarse_modes(modes)
     b.precond_setup(degree=4)
     nc = b.coarse_setup()
     r = rng.standard_normal(3 * b.n_nodes)
     z = _np(b.coarse_apply(torch.as_tensor(r).cuda()))
-    uc = np.concatenate([_plane_basis(m) @ rng.standard_normal(_plane_basis(m).shape[1]) for m, _ in items])
+    uc = np.concatenate([_plane_basis(m, modes) @ rng.standard_normal(_plane_basis(m, modes).shape[1])
+                         for m, _ in items])
     zc = _np(b.coarse_apply(b.apply(torch.as_tensor(uc).cuda())))
     for i, (m, mat) in enumerate(items):
         sl = b.inst_slice(i)
         doff, dcol, vo, _ = _oracle_system(m, mat["E"], mat["nu"])
         import scipy.sparse as sp
         K = sp.csr_matrix((vo, dcol, doff))
-        P = _plane_basis(m)
-        want = P @ np.linalg.solve(P.T @ (K @ P), P.T @ r[sl])
-        print(f"[coarse] tet10={tet10} inst {i}: coarse dim {P.shape[1]}, err {rel_err(z[sl], want):.2e}, "
-              f"u_c err {rel_err(zc[sl], uc[sl]):.2e}", flush=True)
+        P = _plane_basis(m, modes)
+        want = P @ np.linalg.lstsq(P.T @ (K @ P), P.T @ r[sl], rcond=None)[0]
+        print(f"[coarse] tet10={tet10} modes={modes} inst {i}: coarse dim {P.shape[1]}, "
+              f"err {rel_err(z[sl], want):.2e}, u_c err {rel_err(zc[sl], uc[sl]):.2e}", flush=True)
         assert rel_err(z[sl], want) < 1e-9
         assert rel_err(zc[sl], uc[sl]) < 1e-9
-    assert nc >= sum(_plane_basis(m).shape[1] for m, _ in items)
+    assert nc >= sum(_plane_basis(m, modes).shape[1] for m, _ in items)
     b.close()
 
 
 @pytest.mark.parametrize("seed", [3, 7])
 def test_two_level_solve_against_direct_and_plain(gpu, seed):
-    """The FP32 two-level preconditioned flexible CG on uneven beams: every instance reaches
-    the FP64 recursive residual 1e-10, agrees with the oracle's banded Cholesky solve within
-    the bar of the plain batched CG (1e-6, these beams' kappa allows no tighter), with the
-    plain batched FP64 CG to 1e-6, and needs far fewer iterations than it (at least 5x on the
-    batch's slowest instance)."""
+    """The FP32 two-level preconditioned flexible CG on uneven beams, with the 6 rigid
+    coarse modes per plane and with 11: every instance reaches the FP64 recursive residual
+    1e-10, agrees with the oracle's banded Cholesky solve within the bar of the plain batched
+    CG (1e-6, these beams' kappa allows no tighter), with the plain batched FP64 CG to 1e-6,
+    and needs far fewer iterations than it (at least 5x on the batch's slowest instance)."""
     from tlfea_amd.batch import Batch
     items = _items(seed, 6, big=True)
     b = Batch(items)
@@ -399,14 +413,20 @@ def test_two_level_solve_against_direct_and_plain(gpu, seed):
     b.coarse_setup()
     u2 = _np(b.solve(tol=1e-10, precond=True))
     it2 = b.iters.copy()
-    print(f"[two-level] plain {list(it64)} -> two-level {list(it2)}", flush=True)
     assert np.all(b.status == 0)
-    assert it2.max() * 5 <= it64.max()
+    b.set_coarse_modes(11)
+    b.coarse_setup()
+    u11 = _np(b.solve(tol=1e-10, precond=True))
+    it11 = b.iters.copy()
+    print(f"[two-level] plain {list(it64)} -> two-level {list(it2)}, 11 modes {list(it11)}",
+          flush=True)
+    assert np.all(b.status == 0)
+    assert it2.max() * 5 <= it64.max() and it11.max() * 5 <= it64.max()
     for i, (m, mat) in enumerate(items):
         sl = b.inst_slice(i)
         doff, dcol, vo, fo = _oracle_system(m, mat["E"], mat["nu"])
         ud = O.band_cholesky_solve(doff, dcol, vo, fo)
-        assert rel_err(u2[sl], ud) < 1e-6
+        assert rel_err(u2[sl], ud) < 1e-6 and rel_err(u11[sl], ud) < 1e-6
         assert rel_err(u2[sl], u64[sl]) < 1e-6
         assert np.linalg.norm(fo - O.spmv(doff, dcol, vo, u2[sl])) / np.linalg.norm(fo) < _residual_bar(
             1e-10, doff, vo, u2[sl], fo)

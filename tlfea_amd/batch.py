@@ -164,6 +164,12 @@ class Batch:
         self.precond_degree = degree
         return lm
 
+    def set_coarse_modes(self, modes):
+        """Coarse modes per node plane for the next coarse_setup: 6 (rigid motions, the
+        default) or 11 (rigid motions and five in-plane fields; fewer iterations, but a
+        set-up about three times longer, which the beams measured here do not win back)."""
+        call("tlfea_batch_set_coarse_modes", self.h, int(modes))
+
     def coarse_setup(self, enable=True):
         """Two-level coarse correction (rigid motions of node planes) inside the FP32 stage;
         returns the coarse dimension (needs precond_setup first)."""

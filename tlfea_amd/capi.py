@@ -141,6 +141,7 @@ _SIGS = {
     "tlfea_batch_autotune": (i32, [vp, vp, vp, i32, f64, i32, C.POINTER(BatchTuning), vp]),
     "tlfea_batch_coarse_setup": (i32, [vp, i32, C.POINTER(i32), vp]),
     "tlfea_batch_coarse_apply": (i32, [vp, vp, vp, vp]),
+    "tlfea_batch_set_coarse_modes": (i32, [vp, i32]),
     "tlfea_batch_nesterov_solve": (i32, [vp, f64, vp, i32, vp, vp, vp, vp, vp, i32, f64, f64, i32, i32, i32, i32,
                                          vp, vp, vp, vp, vp, C.POINTER(i32), vp]),
     "tlfea_batch_smooth": (i32, [vp, vp, vp, i32, vp]),

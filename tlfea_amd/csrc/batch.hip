@@ -24,6 +24,11 @@
 
 #include "internal.h"
 
+// coarse-space modes per node plane: the 6 rigid motions, and with 11 also five in-plane
+// fields (the quadratic cross-section warping of bending about either axis, uniform lateral
+// stretch and the two in-plane shears)
+constexpr int CM_MAX = 11;
+
 struct tlfea_batch {
     int nn = 0, n_inst = 0, n_elem = 0, n_nodes = 0;
     tlfea_quad quad{};
@@ -83,6 +88,9 @@ struct tlfea_batch {
     // blocks of W planes (NB = 6 W coarse DOFs) form a block-tridiagonal matrix
     bool coarse_on = false;
     int cw = 0, cnb = 0, n_planes = 0, n_cblocks = 0;
+    int cm_req = 6;  // modes per plane requested (tlfea_batch_set_coarse_modes)
+    int cm = 6;      // modes per plane in use: cm_req, or the 6 rigid ones when W > 2
+    int* d_plane_axis = nullptr;   // [n_planes]: the plane's normal (its instance's long axis)
     int* d_node_plane = nullptr;   // [n_nodes]: global plane, -1 for fixed nodes
     double* d_node_r = nullptr;    // [3 n_nodes]: (X - centroid) / size of the node's plane
     int *d_plane_off = nullptr, *d_plane_nodes = nullptr;  // plane -> free nodes (CSR)
@@ -90,7 +98,7 @@ struct tlfea_batch {
     int* d_plane_local = nullptr;  // [n_planes]: plane index within its instance
     int* d_inst_cblk = nullptr;    // [n_inst + 1]: first coarse block of each instance
     double *d_cdiag = nullptr, *d_clow = nullptr, *d_cg = nullptr;  // [n_cblocks][NB][NB]
-    double *d_crhs = nullptr, *d_cx = nullptr;                       // [6 n_planes]
+    double *d_crhs = nullptr, *d_cx = nullptr;                       // [cm n_planes]
     // instances whose coarse matrix is applied as a dense inverse (long block chains, where
     // the sweep's latency exceeds reading the inverse): offsets into d_cdense (-1 = sweep),
     // and their coarse rows (global coarse DOF, instance)
@@ -520,23 +528,46 @@ __global__ __launch_bounds__(BT) void k_bgather_cheb(Chunks ch, const int* __res
     }
 }
 
+// In-plane fields of a node plane with normal ax at the node's scaled offset r (coarse modes
+// 6..10): with (s, t) the offset's in-plane coordinates along (e_s, e_t), the two lowest
+// in-plane axes,
+//     6: s t e_s + (t^2 - s^2)/2 e_t,  7: (s^2 - t^2)/2 e_s + s t e_t   (bending warping)
+//     8: s e_s + t e_t,  9: t e_s + s e_t,  10: s e_s - t e_t            (stretch, shears)
+// Returns the contribution sum_j c[j] mode_j(r) . e_d for component d (c[0..4] = modes 6..10).
+__device__ __forceinline__ double inplane_value(const double* c, const double* r, int ax, int d) {
+    const int es = ax == 0 ? 1 : 0, et = ax == 2 ? 1 : 2;
+    const double s = r[es], t = r[et];
+    if (d == es) return c[0] * s * t + c[1] * 0.5 * (s * s - t * t) + c[2] * s + c[3] * t + c[4] * s;
+    if (d == et) return c[0] * 0.5 * (t * t - s * s) + c[1] * s * t + c[2] * t + c[3] * s - c[4] * t;
+    return 0.0;
+}
+
 // The coarse correction's prolongation (node_plane non-null): (P x)_k = t_p + theta_p x r_n
-// at DOF k = 3 n + d of a free node n of plane p (0 on fixed nodes).
+// (+ the in-plane fields with cm = 11) at DOF k = 3 n + d of a free node n of plane p (0 on
+// fixed nodes); x holds cm values per plane.
 struct CoarseProlong {
     const int* node_plane;
     const double* node_r;
     const double* x;
+    int cm = 6;
+    const int* plane_axis = nullptr;
 };
 __device__ __forceinline__ double coarse_value(const CoarseProlong& cp, int64_t k) {
     const int64_t n = k / 3;
     const int d = (int)(k - 3 * n);
     const int pl = cp.node_plane[n];
     if (pl < 0) return 0.0;
-    const double* t = cp.x + 6 * (int64_t)pl;
+    const double* t = cp.x + cp.cm * (int64_t)pl;
     const double* r = cp.node_r + 3 * n;
-    if (d == 0) return t[0] + t[4] * r[2] - t[5] * r[1];
-    if (d == 1) return t[1] + t[5] * r[0] - t[3] * r[2];
-    return t[2] + t[3] * r[1] - t[4] * r[0];
+    double v;
+    if (d == 0)
+        v = t[0] + t[4] * r[2] - t[5] * r[1];
+    else if (d == 1)
+        v = t[1] + t[5] * r[0] - t[3] * r[2];
+    else
+        v = t[2] + t[3] * r[1] - t[4] * r[0];
+    if (cp.cm > 6) v += inplane_value(t + 6, r, cp.plane_axis[pl], d);
+    return v;
 }
 
 // z = P x on every DOF (FP64)
@@ -735,48 +766,57 @@ __global__ void k_d2f(int64_t n, const double* __restrict__ a, float* __restrict
 // ---- two-level coarse correction --------------------------------------------------------
 // Coarse space: for each plane p of nodes across an instance's long axis, the 6 rigid
 // motions u_n = t_p + theta_p x r_n of its free nodes (r_n = (X_n - c_p) / size_p), so
-// P^T v = [sum_n v_n ; sum_n r_n x v_n] per plane.  Elements span at most W consecutive
-// planes, so P^T K P couples planes at most W apart: grouping W planes per block makes it
-// block-tridiagonal (blocks of NB = 6 W).  It is assembled by probing with the batched
-// operator (planes coloured mod 2W+1), factored as D_k = A_kk - G_k A_(k-1,k)^T with
-// G_k = A_(k,k-1) D_(k-1)^-1, and solved by a block forward and backward sweep.
-constexpr int CNB_MAX = 18;  // W <= 3
+// P^T v = [sum_n v_n ; sum_n r_n x v_n] per plane, and with cm = 11 the five in-plane
+// fields of inplane_value.  Elements span at most W consecutive planes, so P^T K P couples
+// planes at most W apart: grouping W planes per block makes it block-tridiagonal (blocks
+// of NB = cm W).  It is assembled by probing with the batched operator (planes coloured
+// mod 2W+1), factored as D_k = A_kk - G_k A_(k-1,k)^T with G_k = A_(k,k-1) D_(k-1)^-1, and
+// solved by a block forward and backward sweep.
+constexpr int CNB_MAX = 22;  // 11 modes with W <= 2, 6 with W = 3
 
-__device__ __forceinline__ void rigid_mode(int j, const double* r, double v[3]) {
+// mode j < cm of a plane with normal ax at the scaled offset r
+__device__ __forceinline__ void coarse_mode(int j, const double* r, int ax, double v[3]) {
     v[0] = v[1] = v[2] = 0.0;
     if (j < 3) {
         v[j] = 1.0;
-    } else {  // e_(j-3) x r
+    } else if (j < 6) {  // e_(j-3) x r
         const int a = j - 3;
         const double e[3] = {a == 0 ? 1.0 : 0.0, a == 1 ? 1.0 : 0.0, a == 2 ? 1.0 : 0.0};
         v[0] = e[1] * r[2] - e[2] * r[1];
         v[1] = e[2] * r[0] - e[0] * r[2];
         v[2] = e[0] * r[1] - e[1] * r[0];
+    } else {
+        double c[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        c[j - 6] = 1.0;
+        for (int d = 0; d < 3; d++) v[d] = inplane_value(c, r, ax, d);
     }
 }
 
 // q = P e_(p, j) summed over the planes p with local index = color mod (2W + 1)
 __global__ void k_coarse_probe_vec(int n_nodes, const int* __restrict__ node_plane, const double* __restrict__ node_r,
-                                   const int* __restrict__ plane_local, int period, int color, int j,
-                                   double* __restrict__ q) {
+                                   const int* __restrict__ plane_local, const int* __restrict__ plane_axis, int period,
+                                   int color, int j, double* __restrict__ q) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= n_nodes) return;
     const int pl = node_plane[n];
     double v[3] = {0.0, 0.0, 0.0};
-    if (pl >= 0 && plane_local[pl] % period == color) rigid_mode(j, node_r + 3 * (int64_t)n, v);
+    if (pl >= 0 && plane_local[pl] % period == color) coarse_mode(j, node_r + 3 * (int64_t)n, plane_axis[pl], v);
     q[3 * (int64_t)n] = v[0];
     q[3 * (int64_t)n + 1] = v[1];
     q[3 * (int64_t)n + 2] = v[2];
 }
 
-// the 6 sums of P^T v over plane p, in lane order then by wave reduction (64 threads)
-// (NT threads: a workgroup of NT; with NT > 64 the waves' sums combine through LDS and
-// every thread returns the total)
+// the cm sums of P^T v over plane p (normal ax), in lane order then by wave reduction (64
+// threads) (NT threads: a workgroup of NT; with NT > 64 the waves' sums combine through LDS
+// and every thread returns the total); out[cm..CM_MAX) are zero
 template <int NT = 64>
 __device__ __forceinline__ void plane_restrict(int p, const int* __restrict__ plane_off,
                                                const int* __restrict__ plane_nodes, const double* __restrict__ node_r,
-                                               const double* __restrict__ v, double out[6]) {
-    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+                                               const double* __restrict__ v, int cm, int ax, double out[CM_MAX]) {
+    double acc[CM_MAX];
+#pragma unroll
+    for (int i = 0; i < CM_MAX; i++) acc[i] = 0.0;
+    const int es = ax == 0 ? 1 : 0, et = ax == 2 ? 1 : 2;
     for (int k = plane_off[p] + (int)threadIdx.x; k < plane_off[p + 1]; k += NT) {
         const int n = plane_nodes[k];
         const double* r = node_r + 3 * (int64_t)n;
@@ -787,22 +827,31 @@ __device__ __forceinline__ void plane_restrict(int p, const int* __restrict__ pl
         acc[3] += r[1] * c - r[2] * b;
         acc[4] += r[2] * a - r[0] * c;
         acc[5] += r[0] * b - r[1] * a;
+        if (cm > 6) {
+            const double s = r[es], t = r[et];
+            const double vs = v[3 * (int64_t)n + es], vt = v[3 * (int64_t)n + et];
+            acc[6] += s * t * vs + 0.5 * (t * t - s * s) * vt;
+            acc[7] += 0.5 * (s * s - t * t) * vs + s * t * vt;
+            acc[8] += s * vs + t * vt;
+            acc[9] += t * vs + s * vt;
+            acc[10] += s * vs - t * vt;
+        }
     }
 #pragma unroll
-    for (int i = 0; i < 6; i++) {
+    for (int i = 0; i < CM_MAX; i++) {
         double x = acc[i];
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
         out[i] = x;
     }
     if constexpr (NT > 64) {
-        __shared__ double red[NT / 64][6];
+        __shared__ double red[NT / 64][CM_MAX];
         if ((threadIdx.x & 63) == 0)
 #pragma unroll
-            for (int i = 0; i < 6; i++) red[threadIdx.x >> 6][i] = out[i];
+            for (int i = 0; i < CM_MAX; i++) red[threadIdx.x >> 6][i] = out[i];
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < 6; i++) {
+        for (int i = 0; i < CM_MAX; i++) {
             double x = 0.0;
 #pragma unroll
             for (int w = 0; w < NT / 64; w++) x += red[w][i];
@@ -814,11 +863,12 @@ __device__ __forceinline__ void plane_restrict(int p, const int* __restrict__ pl
 // A(p', (p*, j)) = (P^T w)_p' for the coloured plane p* within W of p' (same instance);
 // diagonal blocks and the lower blocks A_(k, k-1) are stored (row-major NB x NB).
 template <int NT>
-__global__ __launch_bounds__(NT) void k_coarse_probe_store(int W, int period, int color, int j,
+__global__ __launch_bounds__(NT) void k_coarse_probe_store(int W, int cm, int period, int color, int j,
                                                            const int* __restrict__ plane_off,
                                                            const int* __restrict__ plane_nodes,
                                                            const int* __restrict__ plane_local,
                                                            const int* __restrict__ plane_inst,
+                                                           const int* __restrict__ plane_axis,
                                                            const int* __restrict__ inst_plane_n,
                                                            const double* __restrict__ node_r,
                                                            const double* __restrict__ w, double* __restrict__ cdiag,
@@ -834,15 +884,15 @@ __global__ __launch_bounds__(NT) void k_coarse_probe_store(int W, int period, in
         ls = lp + (period - dm);
     if (ls < 0 || ls >= inst_plane_n[plane_inst[pp]]) return;
     const int ps = pp + (ls - lp);
-    double out[6];
-    plane_restrict<NT>(pp, plane_off, plane_nodes, node_r, w, out);
+    double out[CM_MAX];
+    plane_restrict<NT>(pp, plane_off, plane_nodes, node_r, w, cm, plane_axis[pp], out);
     if (threadIdx.x != 0) return;
-    const int NB = 6 * W;
+    const int NB = cm * W;
     const int bp = pp / W, bs = ps / W;
-    const int row0 = 6 * (pp % W), col = 6 * (ps % W) + j;
+    const int row0 = cm * (pp % W), col = cm * (ps % W) + j;
     double* dst = bs == bp ? cdiag + (int64_t)bp * NB * NB : (bs == bp - 1 ? clow + (int64_t)bp * NB * NB : nullptr);
     if (!dst) return;  // the upper block is the transpose of the next lower one
-    for (int i = 0; i < 6; i++) dst[(row0 + i) * NB + col] = out[i];
+    for (int i = 0; i < cm; i++) dst[(row0 + i) * NB + col] = out[i];
 }
 
 // In-place inverse of the SPD NB x NB matrix m (LDS) by Cholesky: the workgroup factors
@@ -984,14 +1034,21 @@ template <int NT>
 __global__ __launch_bounds__(NT) void k_coarse_restrict(const int* __restrict__ plane_off,
                                                         const int* __restrict__ plane_nodes,
                                                         const int* __restrict__ plane_inst,
+                                                        const int* __restrict__ plane_axis, int cm,
                                                         const double* __restrict__ node_r,
                                                         const int* __restrict__ I, const double* __restrict__ r,
                                                         double* __restrict__ rhs) {
     const int p = blockIdx.x;
     if (I[(int64_t)plane_inst[p] * NI + I_DONE]) return;
-    double out[6];
-    plane_restrict<NT>(p, plane_off, plane_nodes, node_r, r, out);
-    if (threadIdx.x < 6) rhs[6 * (int64_t)p + threadIdx.x] = out[threadIdx.x];
+    double out[CM_MAX];
+    plane_restrict<NT>(p, plane_off, plane_nodes, node_r, r, cm, plane_axis[p], out);
+    if (threadIdx.x < cm) {
+        double o = out[0];
+#pragma unroll
+        for (int i = 1; i < CM_MAX; i++)
+            if (threadIdx.x == i) o = out[i];
+        rhs[cm * (int64_t)p + threadIdx.x] = o;
+    }
 }
 
 // block forward / backward sweep per instance: x = A_c^-1 rhs.  The sweep is a chain of
@@ -1154,8 +1211,12 @@ int launch_coarse_solve(int NB, int n_inst, const int* inst_cblk, const int* I, 
     const size_t lds = 2 * CS_WORDS * sizeof(double);
     if (NB == 6)
         k_coarse_solve<6><<<n_inst, 256, lds, s>>>(inst_cblk, I, dense_off, dinv, clow, cg, rhs, x);
+    else if (NB == 11)
+        k_coarse_solve<11><<<n_inst, 256, lds, s>>>(inst_cblk, I, dense_off, dinv, clow, cg, rhs, x);
     else if (NB == 12)
         k_coarse_solve<12><<<n_inst, 256, lds, s>>>(inst_cblk, I, dense_off, dinv, clow, cg, rhs, x);
+    else if (NB == 22)
+        k_coarse_solve<22><<<n_inst, 256, lds, s>>>(inst_cblk, I, dense_off, dinv, clow, cg, rhs, x);
     else
         k_coarse_solve<18><<<n_inst, 256, lds, s>>>(inst_cblk, I, dense_off, dinv, clow, cg, rhs, x);
     TLFEA_LAUNCH_CHECK();
@@ -1168,8 +1229,12 @@ int launch_coarse_inverse(int NB, int k0, int k1, const double* dinv, const doub
     const unsigned cols = (unsigned)((k1 - k0) * NB);
     if (NB == 6)
         k_coarse_inverse<6><<<cols, 256, lds, s>>>(k0, k1, dinv, clow, cg, Ainv);
+    else if (NB == 11)
+        k_coarse_inverse<11><<<cols, 256, lds, s>>>(k0, k1, dinv, clow, cg, Ainv);
     else if (NB == 12)
         k_coarse_inverse<12><<<cols, 256, lds, s>>>(k0, k1, dinv, clow, cg, Ainv);
+    else if (NB == 22)
+        k_coarse_inverse<22><<<cols, 256, lds, s>>>(k0, k1, dinv, clow, cg, Ainv);
     else
         k_coarse_inverse<18><<<cols, 256, lds, s>>>(k0, k1, dinv, clow, cg, Ainv);
     TLFEA_LAUNCH_CHECK();
@@ -1204,7 +1269,7 @@ void free_batch(tlfea_batch* b) {
                     b->d_ux, b->d_uy, b->d_uz, b->d_node_plane, b->d_node_r, b->d_plane_off,
                     b->d_plane_nodes, b->d_plane_inst, b->d_plane_local, b->d_inst_cblk, b->d_cdiag,
                     b->d_clow, b->d_cg, b->d_crhs, b->d_cx, b->d_nvk, b->d_node_inst, b->d_npart, b->d_ns, b->d_nmc, b->d_nrs, b->d_cdense_off, b->d_cdense,
-                    b->d_cd_row, b->d_cd_inst, b->d_cinst_pn, b->d_usol};
+                    b->d_cd_row, b->d_cd_inst, b->d_cinst_pn, b->d_usol, b->d_plane_axis};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (b->s_own) (void)hipStreamDestroy(b->s_own);
@@ -1361,9 +1426,11 @@ int apply32(tlfea_batch* b, const float* p, float* q, const int* iflags, hipStre
 void launch_restrict(tlfea_batch* b, const int* plane_off, const int* plane_nodes, const int* plane_inst,
                      const double* node_r, const int* I, const double* r, double* rhs, hipStream_t s) {
     if (b->restrict_wide)
-        k_coarse_restrict<256><<<b->n_planes, 256, 0, s>>>(plane_off, plane_nodes, plane_inst, node_r, I, r, rhs);
+        k_coarse_restrict<256><<<b->n_planes, 256, 0, s>>>(plane_off, plane_nodes, plane_inst, b->d_plane_axis,
+                                                           b->cm, node_r, I, r, rhs);
     else
-        k_coarse_restrict<64><<<b->n_planes, 64, 0, s>>>(plane_off, plane_nodes, plane_inst, node_r, I, r, rhs);
+        k_coarse_restrict<64><<<b->n_planes, 64, 0, s>>>(plane_off, plane_nodes, plane_inst, b->d_plane_axis, b->cm,
+                                                         node_r, I, r, rhs);
 }
 
 // x_c = A_c^-1 rhs_c for every active instance: the block sweep, or the dense inverse
@@ -1388,7 +1455,7 @@ int precondition(tlfea_batch* b, hipStream_t s) {
         launch_restrict(b, b->d_plane_off, b->d_plane_nodes, b->d_plane_inst, b->d_node_r, b->d_i, b->d_r,
                         b->d_crhs, s);
         if (int rc = coarse_solve_all(b, s)) return rc;
-        cp = CoarseProlong{b->d_node_plane, b->d_node_r, b->d_cx};
+        cp = CoarseProlong{b->d_node_plane, b->d_node_r, b->d_cx, b->cm, b->d_plane_axis};
     }
     k_cheb_init<<<n_chunk_launch(b), BT, 0, s>>>(ch, b->d_r, b->d_dinv32, b->d_cheb, b->d_i, b->d_r32, b->d_d32,
                                                 b->d_z32);
@@ -1675,7 +1742,8 @@ namespace {
 // Node planes of one instance across its long axis: the distinct coordinates along that
 // axis when the nodes lie on few of them (a structured mesh), else equal-width bins.
 // Returns the plane count; plane[n - n0] receives the local plane of node n.
-int instance_planes(const double* X, const double* Y, const double* Z, int n0, int n1, std::vector<int>& plane) {
+int instance_planes(const double* X, const double* Y, const double* Z, int n0, int n1, std::vector<int>& plane,
+                    int* axis) {
     const double* c[3] = {X, Y, Z};
     double lo[3], hi[3];
     for (int d = 0; d < 3; d++) {
@@ -1689,6 +1757,7 @@ int instance_planes(const double* X, const double* Y, const double* Z, int n0, i
     int ax = 0;
     for (int d = 1; d < 3; d++)
         if (hi[d] - lo[d] > hi[ax] - lo[ax]) ax = d;
+    *axis = ax;
     const double ext = hi[ax] - lo[ax];
     const int nn = n1 - n0;
     plane.assign(nn, 0);
@@ -1724,7 +1793,7 @@ namespace {
 int coarse_dense_select(tlfea_batch* b, hipStream_t s) {
     const int NB = b->cnb, n = b->n_inst;
     const std::vector<int>& inst_cblk = b->h_inst_cblk;
-    constexpr double T_BLOCK = 1.0e-6, BW = 4.0e12;
+    const double T_BLOCK = 1.0e-6 * std::max(1.0, NB / 12.0), BW = 4.0e12;
     constexpr double BUDGET = 4.0 * (1 << 30);
     b->h_cdense_off.assign(n, -1);
     std::vector<int> order(n);
@@ -1815,11 +1884,11 @@ int coarse_geometry(tlfea_batch* b, hipStream_t s) {
     };
     // planes per instance and the widest element span W
     std::unique_ptr<int[]> local(new int[N]);
-    std::vector<int> inst_np(b->n_inst), inst_w(b->n_inst, 1);
+    std::vector<int> inst_np(b->n_inst), inst_w(b->n_inst, 1), inst_ax(b->n_inst, 0);
     parallel_instances([&](int i) {
         const int n0 = b->h_node_off[i], n1 = b->h_node_off[i + 1];
         std::vector<int> pl;
-        inst_np[i] = instance_planes(X.get(), Y.get(), Z.get(), n0, n1, pl);
+        inst_np[i] = instance_planes(X.get(), Y.get(), Z.get(), n0, n1, pl, &inst_ax[i]);
         for (int n = n0; n < n1; n++) local[n] = pl[n - n0];
         for (int e = b->h_elem_off[i]; e < b->h_elem_off[i + 1]; e++) {
             int mn = 1 << 30, mx = -1;
@@ -1836,7 +1905,9 @@ int coarse_geometry(tlfea_batch* b, hipStream_t s) {
         set_error("coarse correction: elements span %d node planes (at most 3 supported)", W);
         return TLFEA_ECAPACITY;
     }
-    const int NB = 6 * W;
+    // the in-plane modes where the blocks stay within CNB_MAX
+    const int cm = (b->cm_req == CM_MAX && CM_MAX * W <= CNB_MAX) ? CM_MAX : 6;
+    const int NB = cm * W;
     // global planes, padded per instance to a multiple of W
     std::vector<int> inst_poff(b->n_inst + 1, 0), inst_pn(b->n_inst);
     for (int i = 0; i < b->n_inst; i++) {
@@ -1844,7 +1915,7 @@ int coarse_geometry(tlfea_batch* b, hipStream_t s) {
         inst_poff[i + 1] = inst_poff[i] + inst_pn[i];
     }
     const int P = inst_poff[b->n_inst];
-    std::vector<int> plane_inst(P), plane_local(P), inst_cblk(b->n_inst + 1), cnt(P + 1, 0);
+    std::vector<int> plane_inst(P), plane_local(P), plane_axis(P), inst_cblk(b->n_inst + 1), cnt(P + 1, 0);
     inst_cblk[b->n_inst] = P / W;
     // free nodes per plane, centroid and size, node offsets r = (X - c) / size
     std::unique_ptr<int[]> node_plane(new int[N]);
@@ -1855,6 +1926,7 @@ int coarse_geometry(tlfea_batch* b, hipStream_t s) {
         for (int q = 0; q < inst_pn[i]; q++) {
             plane_inst[inst_poff[i] + q] = i;
             plane_local[inst_poff[i] + q] = q;
+            plane_axis[inst_poff[i] + q] = inst_ax[i];
         }
         const int n0 = b->h_node_off[i], n1 = b->h_node_off[i + 1];
         for (int n = n0; n < n1; n++) {
@@ -1900,8 +1972,10 @@ int coarse_geometry(tlfea_batch* b, hipStream_t s) {
     for (void* q : {(void*)b->d_node_plane, (void*)b->d_node_r, (void*)b->d_plane_off, (void*)b->d_plane_nodes,
                     (void*)b->d_plane_inst, (void*)b->d_plane_local, (void*)b->d_inst_cblk, (void*)b->d_cdiag,
                     (void*)b->d_clow, (void*)b->d_cg, (void*)b->d_crhs, (void*)b->d_cx, (void*)b->d_cdense_off,
-                    (void*)b->d_cdense, (void*)b->d_cd_row, (void*)b->d_cd_inst, (void*)b->d_cinst_pn})
+                    (void*)b->d_cdense, (void*)b->d_cd_row, (void*)b->d_cd_inst, (void*)b->d_cinst_pn,
+                    (void*)b->d_plane_axis})
         if (q) (void)hipFree(q);
+    b->d_plane_axis = nullptr;
     b->d_cinst_pn = nullptr;
     b->d_cdense_off = nullptr;
     b->d_cdense = nullptr;
@@ -1918,7 +1992,8 @@ int coarse_geometry(tlfea_batch* b, hipStream_t s) {
         (rc = dalloc(&b->d_plane_inst, P)) || (rc = dalloc(&b->d_plane_local, P)) ||
         (rc = dalloc(&b->d_inst_cblk, b->n_inst + 1)) || (rc = dalloc(&b->d_cdiag, nblk * bsz)) ||
         (rc = dalloc(&b->d_clow, nblk * bsz)) || (rc = dalloc(&b->d_cg, nblk * bsz)) ||
-        (rc = dalloc(&b->d_crhs, 6 * (size_t)P)) || (rc = dalloc(&b->d_cx, 6 * (size_t)P)) ||
+        (rc = dalloc(&b->d_crhs, cm * (size_t)P)) || (rc = dalloc(&b->d_cx, cm * (size_t)P)) ||
+        (rc = dalloc(&b->d_plane_axis, P)) ||
         (rc = dalloc(&d_inst_pn, b->n_inst)))
         return rc;
     TLFEA_HIP_TRY(hipMemcpy(b->d_node_plane, node_plane.get(), sizeof(int) * N, hipMemcpyHostToDevice));
@@ -1928,9 +2003,11 @@ int coarse_geometry(tlfea_batch* b, hipStream_t s) {
                             hipMemcpyHostToDevice));
     TLFEA_HIP_TRY(hipMemcpy(b->d_plane_inst, plane_inst.data(), sizeof(int) * P, hipMemcpyHostToDevice));
     TLFEA_HIP_TRY(hipMemcpy(b->d_plane_local, plane_local.data(), sizeof(int) * P, hipMemcpyHostToDevice));
+    TLFEA_HIP_TRY(hipMemcpy(b->d_plane_axis, plane_axis.data(), sizeof(int) * P, hipMemcpyHostToDevice));
     TLFEA_HIP_TRY(hipMemcpy(b->d_inst_cblk, inst_cblk.data(), sizeof(int) * (b->n_inst + 1), hipMemcpyHostToDevice));
     TLFEA_HIP_TRY(hipMemcpy(d_inst_pn, inst_pn.data(), sizeof(int) * b->n_inst, hipMemcpyHostToDevice));
     b->cw = W;
+    b->cm = cm;
     b->cnb = NB;
     b->n_planes = P;
     b->n_cblocks = (int)nblk;
@@ -1950,21 +2027,21 @@ int coarse_numeric(tlfea_batch* b, hipStream_t s) {
     int rc = TLFEA_OK;
     TLFEA_HIP_TRY(hipMemsetAsync(b->d_cdiag, 0, sizeof(double) * nblk * bsz, s));
     TLFEA_HIP_TRY(hipMemsetAsync(b->d_clow, 0, sizeof(double) * nblk * bsz, s));
-    TLFEA_HIP_TRY(hipMemsetAsync(b->d_cx, 0, sizeof(double) * 6 * (size_t)P, s));
+    TLFEA_HIP_TRY(hipMemsetAsync(b->d_cx, 0, sizeof(double) * b->cm * (size_t)P, s));
     // P^T A P (A = K + c M when a mass term is set) by probing: planes coloured by local
     // index mod 2W+1 are further apart than any element reaches, so each coloured rigid
     // mode's image restricted to a plane belongs to the one coloured plane within W of it
     const int period = 2 * W + 1;
     for (int color = 0; color < period && rc == TLFEA_OK; color++)
-        for (int j = 0; j < 6 && rc == TLFEA_OK; j++) {
+        for (int j = 0; j < b->cm && rc == TLFEA_OK; j++) {
             k_coarse_probe_vec<<<grid_for(N, 256), 256, 0, s>>>(N, b->d_node_plane, b->d_node_r, b->d_plane_local,
-                                                                period, color, j, b->d_p);
+                                                                b->d_plane_axis, period, color, j, b->d_p);
             if ((rc = apply64(b, b->d_p, b->d_q, nullptr, s))) break;
             // four waves per plane on large planes (as the restriction)
             (b->restrict_wide ? k_coarse_probe_store<256> : k_coarse_probe_store<64>)<<<P, b->restrict_wide ? 256 : 64,
                                                                                          0, s>>>(
-                W, period, color, j, b->d_plane_off, b->d_plane_nodes, b->d_plane_local, b->d_plane_inst,
-                b->d_cinst_pn, b->d_node_r, b->d_q, b->d_cdiag, b->d_clow);
+                W, b->cm, period, color, j, b->d_plane_off, b->d_plane_nodes, b->d_plane_local, b->d_plane_inst,
+                b->d_plane_axis, b->d_cinst_pn, b->d_node_r, b->d_q, b->d_cdiag, b->d_clow);
             if (hipGetLastError() != hipSuccess) {
                 set_error("coarse probe launch failed");
                 rc = TLFEA_EHIP;
@@ -1973,8 +2050,12 @@ int coarse_numeric(tlfea_batch* b, hipStream_t s) {
     if (rc == TLFEA_OK) {
         if (NB == 6)
             k_coarse_factor<6><<<b->n_inst, 64, 0, s>>>(b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
+        else if (NB == 11)
+            k_coarse_factor<11><<<b->n_inst, 64, 0, s>>>(b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
         else if (NB == 12)
             k_coarse_factor<12><<<b->n_inst, 64, 0, s>>>(b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
+        else if (NB == 22)
+            k_coarse_factor<22><<<b->n_inst, 64, 0, s>>>(b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
         else
             k_coarse_factor<18><<<b->n_inst, 64, 0, s>>>(b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
         if (hipGetLastError() != hipSuccess) {
@@ -2003,8 +2084,18 @@ extern "C" int tlfea_batch_coarse_setup(tlfea_batch* b, int enable, int* n_coars
         if (int rc = coarse_geometry(b, s)) return rc;
     }
     if (int rc = coarse_numeric(b, s)) return rc;
-    if (n_coarse) *n_coarse = 6 * b->n_planes;
+    if (n_coarse) *n_coarse = b->cm * b->n_planes;
     b->coarse_on = true;
+    return TLFEA_OK;
+}
+
+extern "C" int tlfea_batch_set_coarse_modes(tlfea_batch* b, int modes) {
+    TLFEA_REQUIRE(b != nullptr, "null batch");
+    TLFEA_REQUIRE(modes == 6 || modes == CM_MAX, "coarse modes per plane must be 6 or %d", CM_MAX);
+    drop_graph(b);
+    b->cm_req = modes;
+    b->coarse_on = false;
+    b->coarse_geom = false;  // rebuilt (with the new block size) by the next set-up
     return TLFEA_OK;
 }
 
@@ -2016,7 +2107,8 @@ extern "C" int tlfea_batch_coarse_apply(tlfea_batch* b, const double* r, double*
     launch_restrict(b, b->d_plane_off, b->d_plane_nodes, b->d_plane_inst, b->d_node_r, b->d_i, r, b->d_crhs, s);
     if (int rc = coarse_solve_all(b, s)) return rc;
     const int64_t n3 = 3 * (int64_t)b->n_nodes;
-    k_coarse_prolong64<<<grid_for(n3, 256), 256, 0, s>>>(n3, CoarseProlong{b->d_node_plane, b->d_node_r, b->d_cx},
+    k_coarse_prolong64<<<grid_for(n3, 256), 256, 0, s>>>(n3, CoarseProlong{b->d_node_plane, b->d_node_r, b->d_cx,
+                                                                            b->cm, b->d_plane_axis},
                                                          z);
     TLFEA_LAUNCH_CHECK();
     return TLFEA_OK;
