@@ -951,9 +951,11 @@ __device__ void spd_inverse(double* m, double* L, int* dead) {
     __syncthreads();
 }
 
-// Block factorisation per instance (one workgroup of 64): cdiag[k] becomes D_k^-1, cg[k] = G_k.
+// Block factorisation per instance (one workgroup of FACTOR_NT threads, one per block entry
+// for NB <= 16): cdiag[k] becomes D_k^-1, cg[k] = G_k.
+constexpr int FACTOR_NT = 256;
 template <int NB>
-__global__ __launch_bounds__(64) void k_coarse_factor(const int* __restrict__ inst_cblk,
+__global__ __launch_bounds__(FACTOR_NT) void k_coarse_factor(const int* __restrict__ inst_cblk,
                                                       double* __restrict__ cdiag, const double* __restrict__ clow,
                                                       double* __restrict__ cg) {
     __shared__ double Dprev[CNB_MAX * CNB_MAX], M[CNB_MAX * CNB_MAX], T[CNB_MAX * CNB_MAX],
@@ -961,14 +963,14 @@ __global__ __launch_bounds__(64) void k_coarse_factor(const int* __restrict__ in
     __shared__ int dead[CNB_MAX];
     const int i = blockIdx.x;
     const int k0 = inst_cblk[i], k1 = inst_cblk[i + 1];
-    constexpr int nn = NB * NB, PER = (nn + 63) / 64;
+    constexpr int nn = NB * NB, PER = (nn + FACTOR_NT - 1) / FACTOR_NT;
     // the chain's global loads of block k + 1 are issued before block k's arithmetic, so
     // no step waits on memory
     double nD[PER], nA[PER];
     auto fetch = [&](int k) {
 #pragma unroll
         for (int q = 0; q < PER; q++) {
-            const int e = (int)threadIdx.x + 64 * q;
+            const int e = (int)threadIdx.x + FACTOR_NT * q;
             nD[q] = e < nn ? cdiag[(int64_t)k * nn + e] : 0.0;
             nA[q] = e < nn && k > k0 ? clow[(int64_t)k * nn + e] : 0.0;  // A_(k, k-1)
         }
@@ -978,7 +980,7 @@ __global__ __launch_bounds__(64) void k_coarse_factor(const int* __restrict__ in
         double* D = cdiag + (int64_t)k * nn;
 #pragma unroll
         for (int q = 0; q < PER; q++) {
-            const int e = (int)threadIdx.x + 64 * q;
+            const int e = (int)threadIdx.x + FACTOR_NT * q;
             if (e < nn) {
                 M[e] = nD[q];
                 A[e] = nA[q];
@@ -988,7 +990,7 @@ __global__ __launch_bounds__(64) void k_coarse_factor(const int* __restrict__ in
         if (k + 1 < k1) fetch(k + 1);
         if (k > k0) {
             // G = A Dprev^-1 (Dprev holds the inverse), then M -= G A^T
-            for (int e = threadIdx.x; e < nn; e += 64) {
+            for (int e = threadIdx.x; e < nn; e += FACTOR_NT) {
                 const int r = e / NB, c = e % NB;
                 double v = 0.0;
                 for (int t = 0; t < NB; t++) v += A[r * NB + t] * Dprev[t * NB + c];
@@ -996,7 +998,7 @@ __global__ __launch_bounds__(64) void k_coarse_factor(const int* __restrict__ in
                 cg[(int64_t)k * nn + e] = v;
             }
             __syncthreads();
-            for (int e = threadIdx.x; e < nn; e += 64) {
+            for (int e = threadIdx.x; e < nn; e += FACTOR_NT) {
                 const int r = e / NB, c = e % NB;
                 double v = 0.0;
                 for (int t = 0; t < NB; t++) v += T[r * NB + t] * A[c * NB + t];
@@ -1004,10 +1006,10 @@ __global__ __launch_bounds__(64) void k_coarse_factor(const int* __restrict__ in
             }
             __syncthreads();
         } else {
-            for (int e = threadIdx.x; e < nn; e += 64) cg[(int64_t)k * nn + e] = 0.0;
+            for (int e = threadIdx.x; e < nn; e += FACTOR_NT) cg[(int64_t)k * nn + e] = 0.0;
         }
         // symmetrise (the probed entries agree to rounding) and invert
-        for (int e = threadIdx.x; e < nn; e += 64) {
+        for (int e = threadIdx.x; e < nn; e += FACTOR_NT) {
             const int r = e / NB, c = e % NB;
             if (c > r) {
                 const double v = 0.5 * (M[r * NB + c] + M[c * NB + r]);
@@ -1015,13 +1017,13 @@ __global__ __launch_bounds__(64) void k_coarse_factor(const int* __restrict__ in
             }
         }
         __syncthreads();
-        for (int e = threadIdx.x; e < nn; e += 64) {
+        for (int e = threadIdx.x; e < nn; e += FACTOR_NT) {
             const int r = e / NB, c = e % NB;
             if (c > r) M[e] = M[c * NB + r] = T[e];
         }
         __syncthreads();
         spd_inverse<NB>(M, L, dead);
-        for (int e = threadIdx.x; e < nn; e += 64) {
+        for (int e = threadIdx.x; e < nn; e += FACTOR_NT) {
             D[e] = M[e];
             Dprev[e] = M[e];
         }
@@ -2049,15 +2051,15 @@ int coarse_numeric(tlfea_batch* b, hipStream_t s) {
         }
     if (rc == TLFEA_OK) {
         if (NB == 6)
-            k_coarse_factor<6><<<b->n_inst, 64, 0, s>>>(b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
+            k_coarse_factor<6><<<b->n_inst, FACTOR_NT, 0, s>>>(b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
         else if (NB == 11)
-            k_coarse_factor<11><<<b->n_inst, 64, 0, s>>>(b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
+            k_coarse_factor<11><<<b->n_inst, FACTOR_NT, 0, s>>>(b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
         else if (NB == 12)
-            k_coarse_factor<12><<<b->n_inst, 64, 0, s>>>(b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
+            k_coarse_factor<12><<<b->n_inst, FACTOR_NT, 0, s>>>(b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
         else if (NB == 22)
-            k_coarse_factor<22><<<b->n_inst, 64, 0, s>>>(b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
+            k_coarse_factor<22><<<b->n_inst, FACTOR_NT, 0, s>>>(b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
         else
-            k_coarse_factor<18><<<b->n_inst, 64, 0, s>>>(b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
+            k_coarse_factor<18><<<b->n_inst, FACTOR_NT, 0, s>>>(b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
         if (hipGetLastError() != hipSuccess) {
             set_error("coarse factor launch failed");
             rc = TLFEA_EHIP;
