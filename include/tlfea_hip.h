@@ -356,8 +356,8 @@ int tlfea_batch_dot(tlfea_batch* b, const double* a, const double* c, double* ou
 /* FP32 Jacobi-Chebyshev preconditioner for tlfea_batch_cg_solve(precond = 1) of the batch's
  * operator A (K, or K + c M with a mass term, at the current positions): the diagonal D of
  * A (FP64, matrix-free), lambda_max(D^-1 A) per instance by `power_iters`
- * FP32 power iterations (scaled by `safety` >= 1), and `degree` Chebyshev steps on
- * [lambda_max / ratio, lambda_max] applied with the FP32 operator.  degree = 0 removes it.
+ * FP32 power iterations (scaled by `safety` >= 1), and `degree` (at most 64) Chebyshev steps
+ * on [lambda_max / ratio, lambda_max] applied with the FP32 operator.  degree = 0 removes it.
  * lambda_max (host [n_inst], may be NULL) receives the unscaled estimates. */
 int tlfea_batch_precond_setup(tlfea_batch* b, int degree, double ratio, int power_iters, double safety,
                               double* lambda_max, void* stream);

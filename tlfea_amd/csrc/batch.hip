@@ -1698,7 +1698,7 @@ extern "C" int tlfea_batch_autotune(tlfea_batch* b, const double* f, double* u, 
 extern "C" int tlfea_batch_precond_setup(tlfea_batch* b, int degree, double ratio, int power_iters,
                                          double safety, double* lambda_max, void* stream) {
     TLFEA_REQUIRE(b != nullptr, "null batch");
-    TLFEA_REQUIRE(degree >= 0 && degree <= 16, "Chebyshev degree out of range [0, 16]");
+    TLFEA_REQUIRE(degree >= 0 && degree <= 64, "Chebyshev degree out of range [0, 64]");
     TLFEA_REQUIRE(degree == 0 || (ratio > 1.0 && power_iters >= 1 && safety >= 1.0), "bad Chebyshev parameters");
     b->cheb_degree = 0;
     drop_graph(b);
