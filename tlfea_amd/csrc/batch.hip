@@ -895,6 +895,136 @@ __global__ __launch_bounds__(NT) void k_coarse_probe_store(int W, int cm, int pe
     for (int i = 0; i < cm; i++) dst[(row0 + i) * NB + col] = out[i];
 }
 
+// Block factorisation per instance, one wave: cdiag[k] becomes D_k^-1, cg[k] = G_k, with
+// D_k = A_kk - G_k A_(k-1,k) and G_k = A_(k,k-1) D_(k-1)^-1.  The chain over k is serial, so
+// a step is kept short: lane r holds row r of the step's blocks in registers (NB a
+// compile-time constant), the products read the other operand from LDS as broadcasts, and
+// the Cholesky factorisation of D_k runs in registers, the columns passed between lanes by
+// shuffles (no barrier inside it), each pivot's reciprocal formed once so that no division
+// sits on the serial path.  A pivot at or below 1e-10 of its diagonal entry drops that
+// coarse DOF (its row and column of the inverse are zero: no rigid mode of a plane without
+// the free nodes to carry it).  Lane c then solves L y = e_c, L^T x = y for column c of
+// D_k^-1.
+template <int NB>
+__global__ __launch_bounds__(64) void k_coarse_factor(const int* __restrict__ inst_cblk,
+                                                      double* __restrict__ cdiag, const double* __restrict__ clow,
+                                                      double* __restrict__ cg) {
+    static_assert(NB <= 12, "rows of at most 12 doubles per lane (larger blocks: k_coarse_factor_lds)");
+    constexpr int nn = NB * NB;
+    __shared__ double sDinv[nn], sA[nn], sM[nn], sL[nn];
+    const int i = blockIdx.x;
+    const int k0 = inst_cblk[i], k1 = inst_cblk[i + 1];
+    const int lane = (int)threadIdx.x;
+    const int r = lane < NB ? lane : NB - 1;  // idle lanes shadow the last row, store nothing
+    // row r of the next step's diagonal and lower blocks, fetched a step ahead
+    double nD[NB], nA[NB];
+    auto fetch = [&](int k) {
+#pragma unroll
+        for (int c = 0; c < NB; c++) {
+            nD[c] = cdiag[(int64_t)k * nn + r * NB + c];
+            nA[c] = k > k0 ? clow[(int64_t)k * nn + r * NB + c] : 0.0;  // A_(k, k-1)
+        }
+    };
+    if (k0 < k1) fetch(k0);
+    for (int k = k0; k < k1; k++) {
+        double m[NB], a[NB];
+#pragma unroll
+        for (int c = 0; c < NB; c++) {
+            m[c] = nD[c];
+            a[c] = nA[c];
+        }
+        if (k + 1 < k1) fetch(k + 1);
+        double* G = cg + (int64_t)k * nn;
+        if (k > k0) {
+            if (lane < NB)
+#pragma unroll
+                for (int c = 0; c < NB; c++) sA[r * NB + c] = a[c];
+            __syncthreads();
+            // row r of G = A Dprev^-1 (Dprev^-1 read row by row), then of M -= G A^T
+            double g[NB];
+#pragma unroll
+            for (int c = 0; c < NB; c++) g[c] = 0.0;
+#pragma unroll
+            for (int t = 0; t < NB; t++)
+#pragma unroll
+                for (int c = 0; c < NB; c++) g[c] += a[t] * sDinv[t * NB + c];
+            if (lane < NB)
+#pragma unroll
+                for (int c = 0; c < NB; c++) G[r * NB + c] = g[c];
+#pragma unroll
+            for (int c = 0; c < NB; c++) {
+                double v = 0.0;
+#pragma unroll
+                for (int t = 0; t < NB; t++) v += g[t] * sA[c * NB + t];
+                m[c] -= v;
+            }
+        } else if (lane < NB) {
+#pragma unroll
+            for (int c = 0; c < NB; c++) G[r * NB + c] = 0.0;
+        }
+        // symmetrise (the probed entries agree to rounding): the lower triangle takes the
+        // mean of the two
+        if (lane < NB)
+#pragma unroll
+            for (int c = 0; c < NB; c++) sM[r * NB + c] = m[c];
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NB; c++)
+            if (c < r) m[c] = 0.5 * (sM[c * NB + r] + m[c]);
+        // Cholesky in registers: after column c, m[c] of lane r > c holds L[r][c]
+        double dg = m[0];
+#pragma unroll
+        for (int c = 1; c < NB; c++) dg = lane == c ? m[c] : dg;
+        bool dead[NB];
+        double rl[NB];  // 1 / L[c][c] (0 on dead columns)
+#pragma unroll
+        for (int c = 0; c < NB; c++) {
+            const double d = __shfl(m[c], c, 64), d0 = __shfl(dg, c, 64);
+            const bool dd = !(d > 1e-10 * fabs(d0)) || d0 <= 0.0;
+            dead[c] = dd;
+            const double lc = dd ? 0.0 : sqrt(d);
+            rl[c] = dd ? 0.0 : 1.0 / lc;
+            m[c] = r > c ? m[c] * rl[c] : (r == c ? lc : m[c]);
+#pragma unroll
+            for (int j = c + 1; j < NB; j++) {
+                const double ljc = __shfl(m[c], j, 64);
+                if (r >= j) m[j] -= m[c] * ljc;
+            }
+        }
+        if (lane < NB)
+#pragma unroll
+            for (int c = 0; c < NB; c++)
+                if (c <= r) sL[r * NB + c] = m[c];
+        __syncthreads();
+        // column c of D_k^-1: L y = e_c, L^T x = y (dead rows skipped)
+        double* D = cdiag + (int64_t)k * nn;
+        if (lane < NB) {
+            const int c = lane;
+            double y[NB], x[NB];
+#pragma unroll
+            for (int q = 0; q < NB; q++) {
+                double v = (q == c) ? 1.0 : 0.0;
+#pragma unroll
+                for (int t = 0; t < q; t++) v -= sL[q * NB + t] * y[t];
+                y[q] = v * rl[q];
+            }
+#pragma unroll
+            for (int q = NB - 1; q >= 0; q--) {
+                double v = y[q];
+#pragma unroll
+                for (int t = q + 1; t < NB; t++) v -= sL[t * NB + q] * x[t];
+                x[q] = v * rl[q];
+            }
+#pragma unroll
+            for (int q = 0; q < NB; q++) {
+                sDinv[q * NB + c] = x[q];
+                D[q * NB + c] = x[q];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // In-place inverse of the SPD NB x NB matrix m (LDS) by Cholesky: the workgroup factors
 // together, then lanes 0..NB-1 solve one column each.  A pivot at or below 1e-10 of its diagonal entry
 // drops that coarse DOF (its row and column of the inverse are zero: no rigid mode of a
@@ -951,11 +1081,11 @@ __device__ void spd_inverse(double* m, double* L, int* dead) {
     __syncthreads();
 }
 
-// Block factorisation per instance (one workgroup of FACTOR_NT threads, one per block entry
-// for NB <= 16): cdiag[k] becomes D_k^-1, cg[k] = G_k.
+// The same block factorisation for larger blocks (NB > 12, whose rows do not fit a lane's
+// registers), one workgroup of FACTOR_NT threads per instance over LDS copies of the blocks.
 constexpr int FACTOR_NT = 256;
 template <int NB>
-__global__ __launch_bounds__(FACTOR_NT) void k_coarse_factor(const int* __restrict__ inst_cblk,
+__global__ __launch_bounds__(FACTOR_NT) void k_coarse_factor_lds(const int* __restrict__ inst_cblk,
                                                       double* __restrict__ cdiag, const double* __restrict__ clow,
                                                       double* __restrict__ cg) {
     __shared__ double Dprev[CNB_MAX * CNB_MAX], M[CNB_MAX * CNB_MAX], T[CNB_MAX * CNB_MAX],
@@ -2051,15 +2181,15 @@ int coarse_numeric(tlfea_batch* b, hipStream_t s) {
         }
     if (rc == TLFEA_OK) {
         if (NB == 6)
-            k_coarse_factor<6><<<b->n_inst, FACTOR_NT, 0, s>>>(b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
+            k_coarse_factor<6><<<b->n_inst, 64, 0, s>>>(b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
         else if (NB == 11)
-            k_coarse_factor<11><<<b->n_inst, FACTOR_NT, 0, s>>>(b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
+            k_coarse_factor<11><<<b->n_inst, 64, 0, s>>>(b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
         else if (NB == 12)
-            k_coarse_factor<12><<<b->n_inst, FACTOR_NT, 0, s>>>(b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
+            k_coarse_factor<12><<<b->n_inst, 64, 0, s>>>(b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
         else if (NB == 22)
-            k_coarse_factor<22><<<b->n_inst, FACTOR_NT, 0, s>>>(b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
+            k_coarse_factor_lds<22><<<b->n_inst, FACTOR_NT, 0, s>>>(b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
         else
-            k_coarse_factor<18><<<b->n_inst, FACTOR_NT, 0, s>>>(b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
+            k_coarse_factor_lds<18><<<b->n_inst, FACTOR_NT, 0, s>>>(b->d_inst_cblk, b->d_cdiag, b->d_clow, b->d_cg);
         if (hipGetLastError() != hipSuccess) {
             set_error("coarse factor launch failed");
             rc = TLFEA_EHIP;
