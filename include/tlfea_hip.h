@@ -363,7 +363,8 @@ int tlfea_batch_precond_setup(tlfea_batch* b, int degree, double ratio, int powe
                               double* lambda_max, void* stream);
 /* Two-level coarse correction for the FP32 stage, not in the reference: the nodes of each
  * instance are grouped into planes across its longest axis (the mesh's lattice planes, or
- * equal-width bins of an unstructured mesh); the coarse space holds the 6 rigid motions of
+ * for other meshes equal-width slabs half as thick as the longest element along that axis,
+ * so that no element spans more than three); the coarse space holds the 6 rigid motions of
  * each plane's free nodes and, on request where W <= 2 (tlfea_batch_set_coarse_modes), five
  * in-plane fields of the plane's cross-section coordinates (s, t): the quadratic warping
  * (s t, (t^2 - s^2)/2) and ((s^2 - t^2)/2, s t) that bending about either axis brings with

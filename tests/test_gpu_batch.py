@@ -433,6 +433,65 @@ def test_two_level_solve_against_direct_and_plain(gpu, seed):
     b.close()
 
 
+def _jittered_beam(nx, ny, nz, tet10, amp, seed):
+    """A cantilever (1 x 0.1 x 0.1) whose free vertices are moved by up to amp of the cell
+    size along each axis (edge nodes kept at their edge midpoints, so the elements stay
+    affine): its node coordinates along the beam are no lattice."""
+    from tlfea_amd import mesh as M
+    m = M.clamp_and_tip_load(M.kuhn_beam(nx, ny, nz, tet10=tet10), -1.0e3)
+    rng = np.random.default_rng(seed)
+    X = m.xyz().copy()
+    verts = np.unique(m.elems[:, :4])
+    move = np.setdiff1d(verts, m.fixed)
+    h = np.array([1.0 / nx, 0.1 / ny, 0.1 / nz])
+    X[move] += amp * h * rng.uniform(-1.0, 1.0, (len(move), 3))
+    if tet10:
+        for k, (a, b) in enumerate([(0, 1), (1, 2), (0, 2), (0, 3), (1, 3), (2, 3)]):
+            X[m.elems[:, 4 + k]] = 0.5 * (X[m.elems[:, a]] + X[m.elems[:, b]])
+    jm = M.TetMesh(X[:, 0].copy(), X[:, 1].copy(), X[:, 2].copy(), m.elems)
+    return M.clamp_and_tip_load(jm, -1.0e3)
+
+
+@pytest.mark.parametrize("tet10", [True, False])
+def test_two_level_on_unstructured_meshes(gpu, tet10):
+    """Meshes whose nodes lie on no lattice planes: the reference's own beam_highres mesh
+    (tests/golden/meshes, TET10 or TET4) and a jittered cantilever, one batch.  The coarse
+    space bins the nodes along the beam in slabs half as thick as the longest element, so
+    every element spans at most three of them; the FP32 two-level solve reaches 1e-10 on
+    each instance, agrees with a sparse direct solve of the oracle's assembled system and
+    with the plain batched FP64 CG to 1e-6, in at most a fifth of its iterations."""
+    import os
+    import scipy.sparse as sps
+    import scipy.sparse.linalg as spla
+    from tlfea_amd import mesh as M
+    from tlfea_amd.batch import Batch
+    here = os.path.dirname(os.path.abspath(__file__))
+    ref = M.read_vtu_ascii(os.path.join(here, "golden", "meshes",
+                                        "beam_highres_t10.vtu" if tet10 else "beam_highres_t4.vtu"))
+    items = [(M.clamp_and_tip_load(ref, -1.0e3), dict(E=2.1e11, nu=0.3)),
+             (_jittered_beam(16, 3, 3, tet10, 0.2, 7), dict(E=7.0e10, nu=0.33))]
+    b = Batch(items)
+    u64 = _np(b.solve(tol=1e-10))
+    it64 = b.iters.copy()
+    assert np.all(b.status == 0)
+    b.precond_setup(degree=8, ratio=300.0)
+    nc = b.coarse_setup()
+    u2 = _np(b.solve(tol=1e-10, precond=True))
+    it2 = b.iters.copy()
+    print(f"[unstructured tet10={tet10}] coarse dim {nc}; plain {list(it64)} -> two-level {list(it2)}",
+          flush=True)
+    assert np.all(b.status == 0)
+    assert it2.max() * 5 <= it64.max()
+    for i, (m, mat) in enumerate(items):
+        sl = b.inst_slice(i)
+        doff, dcol, vo, fo = _oracle_system(m, mat["E"], mat["nu"])
+        n = len(fo)
+        ud = spla.spsolve(sps.csr_matrix((vo, dcol, doff), shape=(n, n)).tocsc(), fo)
+        assert rel_err(u2[sl], ud) < 1e-6
+        assert rel_err(u2[sl], u64[sl]) < 1e-6
+    b.close()
+
+
 def test_two_level_on_the_256_beam_sampler_batch(gpu):
     """The batched configuration at full size: the sampler's 256 uneven cantilevers (seed
     2026, 1.68M TET10, 8.2M DOF) solved at once by the FP32 two-level preconditioned FP64

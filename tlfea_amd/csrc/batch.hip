@@ -1872,10 +1872,12 @@ extern "C" int tlfea_batch_precond_setup(tlfea_batch* b, int degree, double rati
 
 namespace {
 // Node planes of one instance across its long axis: the distinct coordinates along that
-// axis when the nodes lie on few of them (a structured mesh), else equal-width bins.
-// Returns the plane count; plane[n - n0] receives the local plane of node n.
-int instance_planes(const double* X, const double* Y, const double* Z, int n0, int n1, std::vector<int>& plane,
-                    int* axis) {
+// axis when the nodes lie on few of them (a structured mesh), else equal-width bins half as
+// wide as the longest element along the axis, so that no element spans more than three
+// consecutive bins (W <= 2).  Elements e0..e1 of nn nodes, node a of element e at
+// conn[e + a NE].  Returns the plane count; plane[n - n0] receives the local plane of node n.
+int instance_planes(const double* X, const double* Y, const double* Z, int n0, int n1, const int* conn, int e0,
+                    int e1, int nn_e, size_t NE, std::vector<int>& plane, int* axis) {
     const double* c[3] = {X, Y, Z};
     double lo[3], hi[3];
     for (int d = 0; d < 3; d++) {
@@ -1905,12 +1907,21 @@ int instance_planes(const double* X, const double* Y, const double* Z, int n0, i
     }
     np++;
     if (np * 4 <= nn) return np;  // lattice planes
-    // unstructured: bins of equal width, about as thick as the mean node spacing
-    const double t1 = hi[(ax + 1) % 3] - lo[(ax + 1) % 3], t2 = hi[(ax + 2) % 3] - lo[(ax + 2) % 3];
-    const double h = std::cbrt(std::max(ext * std::max(t1, 1e-300) * std::max(t2, 1e-300), 1e-300) / nn);
-    np = std::max(2, std::min(nn / 4, (int)std::lround(ext / std::max(h, 1e-300))));
-    for (int n = n0; n < n1; n++)
-        plane[n - n0] = std::min(np - 1, (int)(((c[ax][n]) - lo[ax]) / ext * np));
+    // unstructured: bins of equal width w >= (longest element extent along the axis) / 2: an
+    // element's interval of length <= 2w meets at most three consecutive bins
+    double emax = 0.0;
+    for (int e = e0; e < e1; e++) {
+        double a = 1e300, z = -1e300;
+        for (int k = 0; k < nn_e; k++) {
+            const double v = c[ax][conn[e + (size_t)k * NE]];
+            a = std::min(a, v);
+            z = std::max(z, v);
+        }
+        emax = std::max(emax, z - a);
+    }
+    const double w = std::max(0.5 * emax * (1.0 + 1e-9), ext * 1e-9);
+    np = std::max(1, std::min(nn, (int)std::ceil(ext / w)));
+    for (int n = n0; n < n1; n++) plane[n - n0] = std::min(np - 1, (int)((c[ax][n] - lo[ax]) / w));
     return np;
 }
 }  // namespace
@@ -2020,7 +2031,8 @@ int coarse_geometry(tlfea_batch* b, hipStream_t s) {
     parallel_instances([&](int i) {
         const int n0 = b->h_node_off[i], n1 = b->h_node_off[i + 1];
         std::vector<int> pl;
-        inst_np[i] = instance_planes(X.get(), Y.get(), Z.get(), n0, n1, pl, &inst_ax[i]);
+        inst_np[i] = instance_planes(X.get(), Y.get(), Z.get(), n0, n1, conn.get(), b->h_elem_off[i],
+                                     b->h_elem_off[i + 1], nn, (size_t)NE, pl, &inst_ax[i]);
         for (int n = n0; n < n1; n++) local[n] = pl[n - n0];
         for (int e = b->h_elem_off[i]; e < b->h_elem_off[i + 1]; e++) {
             int mn = 1 << 30, mx = -1;
