@@ -6,7 +6,8 @@
 //
 //   tlfea_driver linear_static  --mesh kuhn:20,2,2 | kuhn4:12,3,2 | vtu:PATH | tetgen:PREFIX
 //                 [--E 2.1e11 --nu 0.3 --load -1000 --tol 1e-10 --max-iter 50000]
-//                 [--assembled 1: assemble K as the reference does] [--solves N] --out FILE
+//                 [--assembled 1: assemble K as the reference does] [--solves N]
+//                 [--two-level DEG: FP32 two-level preconditioned CG] --out FILE
 //   tlfea_driver nesterov|adamw|adamw_nocoop --mesh tetgen:PREFIX|kuhn:.. [--steps 3] [--E ..]
 //                 [--rho0 ..] [--fix z0|x0] [--node-load NODE,FX] --out FILE
 //                 [--update-fixed-after K --fix2 SPEC]: before step K, UpdateNodalFixed to the
@@ -201,6 +202,8 @@ int run(const std::string& mode, std::map<std::string, std::string>& a, MatrixXR
     if (mode == "linear_static") {
         LinearStaticSolver<TData> solver(b, num(a, "--tol", 1e-10), (int)num(a, "--max-iter", 50000));
         solver.UseAssembledOperator(num(a, "--assembled", 0) != 0);
+        // --two-level DEG: the FP32 two-level preconditioned CG with DEG Chebyshev steps
+        if (num(a, "--two-level", 0) > 0) solver.UseTwoLevelPreconditioner(true, (int)num(a, "--two-level", 0));
         fea.AddSolver("linear_static", &solver);
         // --solves N: N Solve() calls in a row (each re-linearises at the moved positions)
         const int n_solves = std::max(1, (int)num(a, "--solves", 1));
@@ -212,6 +215,7 @@ int run(const std::string& mode, std::map<std::string, std::string>& a, MatrixXR
         extra = ", \"iters\": " + std::to_string(solver.GetLastIterCount()) +
                 ", \"residual\": " + sci(solver.GetLastResidual()) +
                 ", \"matrix_free\": " + (solver.LastSolveWasMatrixFree() ? "true" : "false") +
+                ", \"two_level\": " + (solver.LastSolveWasTwoLevel() ? "true" : "false") +
                 ", \"matrix_free_each\": [" + each + "]";
     } else {
         b->CalcMassMatrix();

@@ -22,6 +22,17 @@ detail::TetElementData* as_tet(ElementBase* data) {
     return static_cast<detail::TetElementData*>(data);
 }
 
+// u = x - X0 per coordinate (the body's displacement from its rest positions)
+__global__ void k_displacement(int n, const double* __restrict__ x, const double* __restrict__ y,
+                               const double* __restrict__ z, const double* __restrict__ X0,
+                               const double* __restrict__ Y0, const double* __restrict__ Z0, double* __restrict__ u) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    u[i] = x[i] - X0[i];
+    u[n + i] = y[i] - Y0[i];
+    u[2 * n + i] = z[i] - Z0[i];
+}
+
 static_assert(sizeof(SyncedNesterovParams) == sizeof(tlfea_nesterov_params), "param layout");
 static_assert(sizeof(SyncedAdamWParams) == sizeof(tlfea_adamw_params), "param layout");
 }  // namespace
@@ -115,6 +126,8 @@ LinearStaticSolver<TData>::LinearStaticSolver(TData* data, Real tol, int max_ite
 
 template <typename TData>
 LinearStaticSolver<TData>::~LinearStaticSolver() {
+    if (batch_) tlfea_batch_destroy(batch_);
+    if (disp_) hipFree(disp_);
     if (cg_) tlfea_cg_destroy(cg_);
     if (k_) hipFree(k_);
     if (f_) hipFree(f_);
@@ -143,7 +156,9 @@ void LinearStaticSolver<TData>::Solve() {
     // K(x) applied without assembly when the rest positions are known (with Kelvin-Voigt
     // damping the operator is K + C); a rejected call falls back to the assembled tangent
     const bool damp = b.mat.eta_damp != 0.0 || b.mat.lambda_damp != 0.0;
-    bool matfree = !assembled_only_ && b.force_plan && b.X0;
+    last_two_level_ = two_level_ && !assembled_only_ && b.X0 && !damp && b.mat.model == 0 &&
+                      SolveTwoLevel(b, &iters, &res, &rc);
+    bool matfree = !last_two_level_ && !assembled_only_ && b.force_plan && b.X0;
     if (matfree) {
         if (!jinv_) {
             const size_t nq = (size_t)b.quad.n;
@@ -164,8 +179,8 @@ void LinearStaticSolver<TData>::Solve() {
             rc = TLFEA_OK;
         }
     }
-    last_matfree_ = matfree;
-    if (!matfree) {
+    last_matfree_ = matfree || last_two_level_;
+    if (!matfree && !last_two_level_) {
         // assemble the tangent at the current positions (h = 1), eliminate fixed DOFs
         lib_check(tlfea_stiffness_assemble(b.pattern, b.nn, b.n_elem, b.conn, b.x, b.y, b.z, b.grad_t, b.detJ_t,
                                            &b.quad, &b.mat, 1.0, k_, nullptr),
@@ -185,6 +200,58 @@ void LinearStaticSolver<TData>::Solve() {
     last_residual_ = res;
     lib_check(tlfea_add_displacement(b.n_nodes, u_, b.x, b.y, b.z, nullptr), "UpdatePositions");
     if (hipDeviceSynchronize() != hipSuccess) throw std::runtime_error("LinearStaticSolver: device error");
+}
+
+// The two-level path through a one-instance batch: the batch copies the connectivity, rest
+// positions, fixed-DOF mask and Lame constants at creation, so it is rebuilt when the fixed
+// set (UpdateNodalFixed) or the material changes; the operator follows the current
+// positions through the displacement x - X0 (K at rest while x = X0).  Returns false when
+// the batch cannot be built (the plain CG then runs).
+template <typename TData>
+bool LinearStaticSolver<TData>::SolveTwoLevel(const tlfea_body& b, int* iters, double* res, int* rc) {
+    const int N = b.n_nodes;
+    std::vector<int> fixed(b.n_fixed);
+    if (b.n_fixed > 0 &&
+        hipMemcpy(fixed.data(), b.fixed_nodes, sizeof(int) * b.n_fixed, hipMemcpyDeviceToHost) != hipSuccess)
+        throw std::runtime_error("LinearStaticSolver: copy of the fixed nodes failed");
+    if (!batch_ || fixed != batch_fixed_ || b.mat.lambda != batch_lam_ || b.mat.mu != batch_mu_) {
+        if (batch_) {
+            tlfea_batch_destroy(batch_);
+            batch_ = nullptr;
+        }
+        if (!mask_ && hipMalloc((void**)&mask_, (size_t)n_dof_) != hipSuccess)
+            throw std::runtime_error("LinearStaticSolver: device allocation failed");
+        lib_check(tlfea_fixed_dof_mask(N, b.n_fixed, b.n_fixed > 0 ? b.fixed_nodes : nullptr, mask_, nullptr),
+                  "LinearStaticSolver");
+        const int eo[2] = {0, b.n_elem}, no[2] = {0, N};
+        double lam = b.mat.lambda, mu = b.mat.mu;
+        if (tlfea_batch_create(b.nn, 1, eo, no, b.conn, b.X0, b.Y0, b.Z0, &lam, &mu, mask_, &batch_, nullptr) !=
+            TLFEA_OK) {
+            batch_ = nullptr;
+            return false;
+        }
+        batch_fixed_ = fixed;
+        batch_lam_ = b.mat.lambda;
+        batch_mu_ = b.mat.mu;
+    }
+    if (!disp_ && hipMalloc((void**)&disp_, sizeof(double) * 3 * (size_t)N) != hipSuccess)
+        throw std::runtime_error("LinearStaticSolver: device allocation failed");
+    k_displacement<<<(N + 255) / 256, 256>>>(N, b.x, b.y, b.z, b.X0, b.Y0, b.Z0, disp_);
+    if (hipGetLastError() != hipSuccess) throw std::runtime_error("LinearStaticSolver: displacement kernel failed");
+    lib_check(tlfea_batch_set_displacement(batch_, disp_, disp_ + N, disp_ + 2 * (size_t)N, nullptr),
+              "LinearStaticSolver");
+    lib_check(tlfea_batch_precond_setup(batch_, tl_degree_, tl_ratio_, 20, 1.1, nullptr, nullptr),
+              "LinearStaticSolver");
+    lib_check(tlfea_batch_coarse_setup(batch_, 1, nullptr, nullptr), "LinearStaticSolver");
+    int it = 0, st = 0, total = 0;
+    double rr = 0.0;
+    const int r = tlfea_batch_cg_solve(batch_, f_, u_, 0, cg_tol_, cg_max_iter_, 16, 0, 1, &it, &rr, &st, &total,
+                                       nullptr);
+    if (r != TLFEA_OK && r != TLFEA_ENOCONV) lib_check(r, "SolveLinearSystemCG");
+    *iters = it;
+    *res = rr;
+    *rc = st;
+    return true;
 }
 
 template class LinearStaticSolver<GPU_FEAT10_Data>;
