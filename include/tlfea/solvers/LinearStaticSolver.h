@@ -35,9 +35,11 @@ class LinearStaticSolver : public SolverBase {
     // FP32 two-level stage (tlfea_batch_precond_setup with `degree` Chebyshev steps on
     // [lambda_max / ratio, lambda_max], and tlfea_batch_coarse_setup) on K(x), set up afresh
     // for every Solve, with the same stopping rule sqrt(r.r / r0.r0) < tol: the same solution
-    // in a small fraction of the iterations on slender bodies (17.9 vs 1.95 solves/s on a
-    // 2M-DOF beam).  SVK bodies without Kelvin-Voigt damping whose rest positions are known
-    // (CalcDnDuPre); others keep the plain CG.
+    // in a small fraction of the iterations on slender bodies (80 vs 1,226 on the 20x2x2-cell
+    // beam, 102 vs 3,740 on the reference's beam_highres_t10 mesh).  The first Solve also
+    // builds the one-instance batch (its force plan and coarse geometry).  SVK bodies
+    // without Kelvin-Voigt damping whose rest positions are known (CalcDnDuPre); others keep
+    // the plain CG.
     void UseTwoLevelPreconditioner(bool on, int degree = 16, Real ratio = 1000.0) {
         two_level_ = on;
         tl_degree_ = degree;
