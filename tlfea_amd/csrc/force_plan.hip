@@ -10,6 +10,7 @@
 // the gradient stream plus ~3 x 8 B per (block, node) written and read back, instead of
 // 30 global FP64 atomics per element.
 #include <algorithm>
+#include <thread>
 #include <vector>
 
 #include <cstring>
@@ -573,30 +574,46 @@ extern "C" int tlfea_force_plan_create(int nn, int n_elem, int n_nodes, const in
     const int nb = (n_elem + PLAN_BLOCK - 1) / PLAN_BLOCK;
     P->n_blocks = nb;
     const int64_t n_tiles = padded(n_elem) / TILE;
+    // blocks are independent: host threads take contiguous ranges of them, and the results
+    // are joined in block order (the plan does not depend on the thread count)
+    const int nthreads = std::max(1, std::min<int>(nb, std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
+    auto parallel_blocks = [&](auto&& fn) {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nthreads; t++)
+            pool.emplace_back([&, t] {
+                for (int b = (int)((int64_t)nb * t / nthreads); b < (int)((int64_t)nb * (t + 1) / nthreads); b++) fn(b);
+            });
+        for (auto& th : pool) th.join();
+    };
     std::vector<int64_t> blk_off(nb + 1, 0);
-    std::vector<int> unodes;
-    unodes.reserve((size_t)n_elem * 3);
+    std::vector<std::vector<int>> blk_nodes(nb);
     std::vector<unsigned short> slot((size_t)n_tiles * nn * TILE, 0);
-    std::vector<int> buf;
-    int max_u = 0;
-    for (int b = 0; b < nb; b++) {
+    parallel_blocks([&](int b) {
         const int e0 = b * PLAN_BLOCK, e1 = std::min(n_elem, e0 + PLAN_BLOCK);
-        buf.clear();
+        std::vector<int>& buf = blk_nodes[b];
+        buf.reserve((size_t)(e1 - e0) * nn);
         for (int e = e0; e < e1; e++)
             for (int a = 0; a < nn; a++) buf.push_back(h_conn[e + (size_t)a * n_elem]);
         std::sort(buf.begin(), buf.end());
         buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
-        const int U = (int)buf.size();
-        max_u = std::max(max_u, U);
         for (int e = e0; e < e1; e++)
             for (int a = 0; a < nn; a++) {
                 const int node = h_conn[e + (size_t)a * n_elem];
                 const int sidx = (int)(std::lower_bound(buf.begin(), buf.end(), node) - buf.begin());
                 slot[((size_t)(e >> 6) * nn + a) * TILE + (e & (TILE - 1))] = (unsigned short)sidx;
             }
-        unodes.insert(unodes.end(), buf.begin(), buf.end());
+    });
+    int max_u = 0;
+    for (int b = 0; b < nb; b++) {
+        const int U = (int)blk_nodes[b].size();
+        max_u = std::max(max_u, U);
         blk_off[b + 1] = blk_off[b] + U;
     }
+    std::vector<int> unodes((size_t)blk_off[nb]);
+    parallel_blocks([&](int b) {
+        std::copy(blk_nodes[b].begin(), blk_nodes[b].end(), unodes.begin() + blk_off[b]);
+        std::vector<int>().swap(blk_nodes[b]);
+    });
     P->max_u = max_u;
     P->n_partials = blk_off[nb];
     // LDS: up to 9 arrays of max_u doubles (positions, forces, velocities) within the
@@ -639,11 +656,14 @@ extern "C" int tlfea_force_plan_create(int nn, int n_elem, int n_nodes, const in
         const int max_gw = (max_u + 63) / 64;
         gperm.assign(unodes.size(), 0);
         gwinfo.assign((size_t)nb * max_gw, 0);
-        std::vector<int> cnt, start, fill, pos;
-        std::vector<unsigned short> lst;
         auto slot_of = [&](int e, int a) { return (int)slot[((size_t)(e >> 6) * nn + a) * TILE + (e & (TILE - 1))]; };
-        int max_ginc = 0;
-        for (int b = 0; b < nb && tables_ok; b++) {
+        // each block's table built by its thread (offsets within the block), then joined
+        std::vector<std::vector<unsigned short>> blk_tab(nb);
+        std::vector<unsigned char> blk_ok(nb, 1);
+        parallel_blocks([&](int b) {
+            std::vector<int> cnt, start, fill, pos;
+            std::vector<unsigned short> lst;
+            std::vector<unsigned short>& tab = blk_tab[b];
             const int e0 = b * PLAN_BLOCK, e1 = std::min(n_elem, e0 + PLAN_BLOCK);
             const int U = (int)(blk_off[b + 1] - blk_off[b]);
             cnt.assign(U, 0);
@@ -658,22 +678,29 @@ extern "C" int tlfea_force_plan_create(int nn, int n_elem, int n_nodes, const in
             pos.resize(U);
             for (int u = 0; u < U; u++) pos[u] = u;
             std::stable_sort(pos.begin(), pos.end(), [&](int x, int y) { return cnt[x] > cnt[y]; });
-            const size_t base = ginc.size();
             for (int w = 0; w * 64 < U; w++) {
                 int c = 0;
                 for (int l = 0; l < 64 && w * 64 + l < U; l++) c = std::max(c, cnt[pos[w * 64 + l]]);
-                tables_ok = tables_ok && c < 256;
-                gwinfo[(size_t)b * max_gw + w] = (unsigned)((ginc.size() - base) << 8) | (unsigned)c;
+                if (c >= 256) blk_ok[b] = 0;
+                gwinfo[(size_t)b * max_gw + w] = (unsigned)(tab.size() << 8) | (unsigned)(c & 255);
                 for (int j = 0; j < c; j++)
                     for (int l = 0; l < 64; l++) {
                         const int g = w * 64 + l;
-                        ginc.push_back(g < U && j < cnt[pos[g]] ? lst[start[pos[g]] + j]
-                                                                : (unsigned short)(10 * PLAN_BLOCK));
+                        tab.push_back(g < U && j < cnt[pos[g]] ? lst[start[pos[g]] + j]
+                                                               : (unsigned short)(10 * PLAN_BLOCK));
                     }
             }
             for (int g = 0; g < U; g++) gperm[blk_off[b] + g] = (unsigned short)pos[g];
-            ginc_off[b + 1] = (int64_t)ginc.size();
-            max_ginc = std::max(max_ginc, (int)(ginc.size() - base));
+        });
+        int max_ginc = 0;
+        for (int b = 0; b < nb; b++) {
+            tables_ok = tables_ok && blk_ok[b];
+            ginc_off[b + 1] = ginc_off[b] + (int64_t)blk_tab[b].size();
+            max_ginc = std::max(max_ginc, (int)blk_tab[b].size());
+        }
+        if (tables_ok) {
+            ginc.resize((size_t)ginc_off[nb]);
+            parallel_blocks([&](int b) { std::copy(blk_tab[b].begin(), blk_tab[b].end(), ginc.begin() + ginc_off[b]); });
         }
         P->max_gw = max_gw;
         P->max_ginc = max_ginc;
