@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <memory>
 #include <thread>
 #include <vector>
@@ -755,6 +756,141 @@ __global__ void k_elem_inst(int n_inst, const int* __restrict__ elem_off, int* _
     const int i = blockIdx.x;
     if (i >= n_inst) return;
     for (int e = elem_off[i] + threadIdx.x; e < elem_off[i + 1]; e += blockDim.x) elem_inst[e] = i;
+}
+
+// ---- coarse-space geometry on the device (the host sorts one coordinate per node) -------
+// min and max of X, Y, Z over each instance's nodes, one workgroup of 256 per instance:
+// bbox[6 i ..] = (lo_x, lo_y, lo_z, hi_x, hi_y, hi_z)
+__global__ __launch_bounds__(256) void k_geom_bbox(const int* __restrict__ node_off, const double* __restrict__ X,
+                                                   const double* __restrict__ Y, const double* __restrict__ Z,
+                                                   double* __restrict__ bbox) {
+    const int i = blockIdx.x;
+    const double* c[3] = {X, Y, Z};
+    double v[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
+    for (int n = node_off[i] + (int)threadIdx.x; n < node_off[i + 1]; n += 256)
+        for (int d = 0; d < 3; d++) {
+            v[d] = fmin(v[d], c[d][n]);
+            v[3 + d] = fmax(v[3 + d], c[d][n]);
+        }
+    __shared__ double red[4][6];
+    for (int k = 0; k < 6; k++) {
+        double x = v[k];
+        for (int o = 32; o > 0; o >>= 1) {
+            const double y = __shfl_xor(x, o, 64);
+            x = k < 3 ? fmin(x, y) : fmax(x, y);
+        }
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][k] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        double x = red[0][k];
+        for (int w = 1; w < 4; w++) x = k < 3 ? fmin(x, red[w][k]) : fmax(x, red[w][k]);
+        bbox[6 * (int64_t)i + k] = x;
+    }
+}
+
+// per node: its coordinate along its instance's long axis, and 1 where a DOF is fixed
+__global__ void k_geom_axis(int n_nodes, const int* __restrict__ node_inst, const int* __restrict__ axis,
+                            const double* __restrict__ X, const double* __restrict__ Y, const double* __restrict__ Z,
+                            const unsigned char* __restrict__ mask, double* __restrict__ cax,
+                            unsigned char* __restrict__ fixed) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= n_nodes) return;
+    const int a = axis[node_inst[n]];
+    cax[n] = a == 0 ? X[n] : (a == 1 ? Y[n] : Z[n]);
+    fixed[n] = (mask[3 * (int64_t)n] | mask[3 * (int64_t)n + 1] | mask[3 * (int64_t)n + 2]) ? 1 : 0;
+}
+
+// per element: its extent along its instance's axis; the maximum per instance by an atomic
+// max of the bit patterns (non-negative doubles order as their patterns)
+__global__ void k_geom_elem_extent(int n_elem, int nn, const int* __restrict__ conn, const int* __restrict__ elem_inst,
+                                   const double* __restrict__ cax, unsigned long long* __restrict__ emax) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_elem) return;
+    double lo = 1e300, hi = -1e300;
+    for (int a = 0; a < nn; a++) {
+        const double v = cax[conn[e + (int64_t)a * n_elem]];
+        lo = fmin(lo, v);
+        hi = fmax(hi, v);
+    }
+    atomicMax(&emax[elem_inst[e]], (unsigned long long)__double_as_longlong(fmax(hi - lo, 0.0)));
+}
+
+// per element: the span of its nodes' local planes; the maximum per instance
+__global__ void k_geom_elem_span(int n_elem, int nn, const int* __restrict__ conn, const int* __restrict__ elem_inst,
+                                 const int* __restrict__ local, int* __restrict__ span) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_elem) return;
+    int lo = 1 << 30, hi = -1;
+    for (int a = 0; a < nn; a++) {
+        const int q = local[conn[e + (int64_t)a * n_elem]];
+        lo = min(lo, q);
+        hi = max(hi, q);
+    }
+    atomicMax(&span[elem_inst[e]], hi - lo);
+}
+
+// per plane, one workgroup of 256: the centroid c of its free nodes and their largest
+// distance from it (cen[3 p ..], size[p]; zero for a plane without free nodes)
+__global__ __launch_bounds__(256) void k_geom_plane_frame(const int* __restrict__ plane_off,
+                                                          const int* __restrict__ plane_nodes,
+                                                          const double* __restrict__ X, const double* __restrict__ Y,
+                                                          const double* __restrict__ Z, double* __restrict__ cen,
+                                                          double* __restrict__ size) {
+    const int p = blockIdx.x;
+    const int k0 = plane_off[p], k1 = plane_off[p + 1];
+    __shared__ double red[4][3];
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int k = k0 + (int)threadIdx.x; k < k1; k += 256) {
+        const int n = plane_nodes[k];
+        s[0] += X[n];
+        s[1] += Y[n];
+        s[2] += Z[n];
+    }
+    for (int d = 0; d < 3; d++) {
+        for (int o = 32; o > 0; o >>= 1) s[d] += __shfl_xor(s[d], o, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][d] = s[d];
+    }
+    __syncthreads();
+    double c[3];
+    for (int d = 0; d < 3; d++) c[d] = k1 > k0 ? (red[0][d] + red[1][d] + red[2][d] + red[3][d]) / (k1 - k0) : 0.0;
+    __syncthreads();
+    double r2 = 0.0;
+    for (int k = k0 + (int)threadIdx.x; k < k1; k += 256) {
+        const int n = plane_nodes[k];
+        const double a = X[n] - c[0], b = Y[n] - c[1], e = Z[n] - c[2];
+        r2 = fmax(r2, a * a + b * b + e * e);
+    }
+    for (int o = 32; o > 0; o >>= 1) r2 = fmax(r2, __shfl_xor(r2, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][0] = r2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        cen[3 * (int64_t)p] = c[0];
+        cen[3 * (int64_t)p + 1] = c[1];
+        cen[3 * (int64_t)p + 2] = c[2];
+        size[p] = sqrt(fmax(fmax(red[0][0], red[1][0]), fmax(red[2][0], red[3][0])));
+    }
+}
+
+// per node: r = (X - c) / size of its plane (zero on fixed nodes)
+__global__ void k_geom_node_r(int n_nodes, const int* __restrict__ node_plane, const double* __restrict__ X,
+                              const double* __restrict__ Y, const double* __restrict__ Z,
+                              const double* __restrict__ cen, const double* __restrict__ size,
+                              double* __restrict__ node_r) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= n_nodes) return;
+    const int g = node_plane[n];
+    double r[3] = {0.0, 0.0, 0.0};
+    if (g >= 0) {
+        const double sc = size[g] > 0.0 ? 1.0 / size[g] : 1.0;
+        r[0] = (X[n] - cen[3 * (int64_t)g]) * sc;
+        r[1] = (Y[n] - cen[3 * (int64_t)g + 1]) * sc;
+        r[2] = (Z[n] - cen[3 * (int64_t)g + 2]) * sc;
+    }
+    node_r[3 * (int64_t)n] = r[0];
+    node_r[3 * (int64_t)n + 1] = r[1];
+    node_r[3 * (int64_t)n + 2] = r[2];
 }
 
 __global__ void k_d2f(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
@@ -1871,33 +2007,18 @@ extern "C" int tlfea_batch_precond_setup(tlfea_batch* b, int degree, double rati
 }
 
 namespace {
-// Node planes of one instance across its long axis: the distinct coordinates along that
-// axis when the nodes lie on few of them (a structured mesh), else equal-width bins half as
-// wide as the longest element along the axis, so that no element spans more than three
-// consecutive bins (W <= 2).  Elements e0..e1 of nn nodes, node a of element e at
-// conn[e + a NE].  Returns the plane count; plane[n - n0] receives the local plane of node n.
-int instance_planes(const double* X, const double* Y, const double* Z, int n0, int n1, const int* conn, int e0,
-                    int e1, int nn_e, size_t NE, std::vector<int>& plane, int* axis) {
-    const double* c[3] = {X, Y, Z};
-    double lo[3], hi[3];
-    for (int d = 0; d < 3; d++) {
-        lo[d] = 1e300;
-        hi[d] = -1e300;
-        for (int n = n0; n < n1; n++) {
-            lo[d] = std::min(lo[d], c[d][n]);
-            hi[d] = std::max(hi[d], c[d][n]);
-        }
-    }
-    int ax = 0;
-    for (int d = 1; d < 3; d++)
-        if (hi[d] - lo[d] > hi[ax] - lo[ax]) ax = d;
-    *axis = ax;
-    const double ext = hi[ax] - lo[ax];
+// Node planes of one instance across its long axis, from the nodes' coordinates c along it
+// (nodes n0..n1-1; lo and ext the axis's minimum and extent over them, emax the longest
+// element along it): the distinct coordinates when the nodes lie on few of them (a
+// structured mesh), else equal-width bins of width w >= emax / 2, so that an element's
+// interval of length <= 2w meets at most three consecutive bins (W <= 2).  Returns the plane
+// count; plane[n - n0] receives the local plane of node n.
+int instance_planes(const double* c, double lo, double ext, double emax, int n0, int n1, std::vector<int>& plane) {
     const int nn = n1 - n0;
     plane.assign(nn, 0);
     if (!(ext > 0.0)) return 1;
     std::vector<std::pair<double, int>> v(nn);
-    for (int n = n0; n < n1; n++) v[n - n0] = {c[ax][n], n - n0};
+    for (int n = n0; n < n1; n++) v[n - n0] = {c[n], n - n0};
     std::sort(v.begin(), v.end());
     const double tol = 1e-9 * ext;
     int np = 0;
@@ -1907,21 +2028,9 @@ int instance_planes(const double* X, const double* Y, const double* Z, int n0, i
     }
     np++;
     if (np * 4 <= nn) return np;  // lattice planes
-    // unstructured: bins of equal width w >= (longest element extent along the axis) / 2: an
-    // element's interval of length <= 2w meets at most three consecutive bins
-    double emax = 0.0;
-    for (int e = e0; e < e1; e++) {
-        double a = 1e300, z = -1e300;
-        for (int k = 0; k < nn_e; k++) {
-            const double v = c[ax][conn[e + (size_t)k * NE]];
-            a = std::min(a, v);
-            z = std::max(z, v);
-        }
-        emax = std::max(emax, z - a);
-    }
     const double w = std::max(0.5 * emax * (1.0 + 1e-9), ext * 1e-9);
     np = std::max(1, std::min(nn, (int)std::ceil(ext / w)));
-    for (int n = n0; n < n1; n++) plane[n - n0] = std::min(np - 1, (int)((c[ax][n] - lo[ax]) / w));
+    for (int n = n0; n < n1; n++) plane[n - n0] = std::min(np - 1, (int)((c[n] - lo) / w));
     return np;
 }
 }  // namespace
@@ -2001,49 +2110,87 @@ namespace {
 // fixed DOFs do not change after creation): node planes, rigid-mode offsets, plane tables,
 // the block layout and the choice of dense inverses, with the device arrays they need.
 int coarse_geometry(tlfea_batch* b, hipStream_t s) {
-    const int N = b->n_nodes, NE = b->n_elem, nn = b->nn;
-    // host staging without zero fill (every entry is written before it is read)
-    std::unique_ptr<double[]> X(new double[N]), Y(new double[N]), Z(new double[N]);
-    std::unique_ptr<unsigned char[]> mask(new unsigned char[3 * (size_t)N]);
-    std::unique_ptr<int[]> conn(new int[(size_t)NE * nn]);
+    const int N = b->n_nodes, NE = b->n_elem, nn = b->nn, n_inst = b->n_inst;
     TLFEA_HIP_TRY(hipStreamSynchronize(s));
-    TLFEA_HIP_TRY(hipMemcpy(X.get(), b->d_X, sizeof(double) * N, hipMemcpyDeviceToHost));
-    TLFEA_HIP_TRY(hipMemcpy(Y.get(), b->d_Y, sizeof(double) * N, hipMemcpyDeviceToHost));
-    TLFEA_HIP_TRY(hipMemcpy(Z.get(), b->d_Z, sizeof(double) * N, hipMemcpyDeviceToHost));
-    TLFEA_HIP_TRY(hipMemcpy(mask.get(), b->d_mask, 3 * (size_t)N, hipMemcpyDeviceToHost));
-    TLFEA_HIP_TRY(hipMemcpy(conn.get(), b->d_conn, sizeof(int) * (size_t)NE * nn, hipMemcpyDeviceToHost));
+    // device scratch: the CG workspace (free between solves) for the per-node arrays, and a
+    // few per-instance arrays freed on return
+    double* d_cax = b->d_r;
+    unsigned char* d_fixed = reinterpret_cast<unsigned char*>(b->d_p);
+    int* d_ninst = reinterpret_cast<int*>(b->d_q);
+    int* d_local = reinterpret_cast<int*>(b->d_z);
+    struct Scratch {
+        double* bbox = nullptr;
+        unsigned long long* emax = nullptr;
+        int *axis = nullptr, *span = nullptr, *noff = nullptr;
+        double *cen = nullptr, *size = nullptr;
+        ~Scratch() {
+            for (void* q : {(void*)bbox, (void*)emax, (void*)axis, (void*)span, (void*)noff, (void*)cen, (void*)size})
+                if (q) (void)hipFree(q);
+        }
+    } t;
+    int rc;
+    if ((rc = dalloc(&t.bbox, 6 * (size_t)n_inst)) || (rc = dalloc(&t.emax, n_inst)) ||
+        (rc = dalloc(&t.axis, n_inst)) || (rc = dalloc(&t.span, n_inst)) || (rc = dalloc(&t.noff, n_inst + 1)))
+        return rc;
+    // per-instance bounding boxes -> long axis, then the nodes' coordinate along it, their
+    // fixed flags and the longest element along it
+    TLFEA_HIP_TRY(hipMemcpyAsync(t.noff, b->h_node_off.data(), sizeof(int) * (n_inst + 1), hipMemcpyHostToDevice, s));
+    k_elem_inst<<<n_inst, 256, 0, s>>>(n_inst, t.noff, d_ninst);
+    k_geom_bbox<<<n_inst, 256, 0, s>>>(t.noff, b->d_X, b->d_Y, b->d_Z, t.bbox);
+    std::vector<double> bbox(6 * (size_t)n_inst);
+    TLFEA_HIP_TRY(hipMemcpyAsync(bbox.data(), t.bbox, sizeof(double) * bbox.size(), hipMemcpyDeviceToHost, s));
+    TLFEA_HIP_TRY(hipStreamSynchronize(s));
+    std::vector<int> inst_ax(n_inst, 0);
+    for (int i = 0; i < n_inst; i++)
+        for (int d = 1; d < 3; d++)
+            if (bbox[6 * i + 3 + d] - bbox[6 * i + d] > bbox[6 * i + 3 + inst_ax[i]] - bbox[6 * i + inst_ax[i]])
+                inst_ax[i] = d;
+    TLFEA_HIP_TRY(hipMemcpyAsync(t.axis, inst_ax.data(), sizeof(int) * n_inst, hipMemcpyHostToDevice, s));
+    k_geom_axis<<<grid_for(N, 256), 256, 0, s>>>(N, d_ninst, t.axis, b->d_X, b->d_Y, b->d_Z, b->d_mask, d_cax,
+                                                 d_fixed);
+    TLFEA_HIP_TRY(hipMemsetAsync(t.emax, 0, sizeof(unsigned long long) * n_inst, s));
+    k_geom_elem_extent<<<grid_for(NE, 256), 256, 0, s>>>(NE, nn, b->d_conn, b->d_elem_inst, d_cax, t.emax);
+    TLFEA_LAUNCH_CHECK();
+    std::unique_ptr<double[]> cax(new double[N]);
+    std::unique_ptr<unsigned char[]> fixed(new unsigned char[N]);
+    std::vector<unsigned long long> emax_bits(n_inst);
+    TLFEA_HIP_TRY(hipMemcpyAsync(cax.get(), d_cax, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(fixed.get(), d_fixed, N, hipMemcpyDeviceToHost, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(emax_bits.data(), t.emax, sizeof(unsigned long long) * n_inst, hipMemcpyDeviceToHost,
+                                 s));
+    TLFEA_HIP_TRY(hipStreamSynchronize(s));
     // instances on host threads: each owns its node, element and plane ranges
-    const int nthreads =
-        std::max(1, std::min<int>(b->n_inst, std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
+    const int nthreads = std::max(1, std::min<int>(n_inst, std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
     auto parallel_instances = [&](auto&& fn) {
         std::vector<std::thread> pool;
-        for (int t = 0; t < nthreads; t++)
-            pool.emplace_back([&, t] {
-                for (int i = (int)((int64_t)b->n_inst * t / nthreads); i < (int)((int64_t)b->n_inst * (t + 1) / nthreads);
-                     i++)
+        for (int th = 0; th < nthreads; th++)
+            pool.emplace_back([&, th] {
+                for (int i = (int)((int64_t)n_inst * th / nthreads); i < (int)((int64_t)n_inst * (th + 1) / nthreads); i++)
                     fn(i);
             });
         for (auto& th : pool) th.join();
     };
-    // planes per instance and the widest element span W
+    // planes per instance (host: one sort of the axis coordinate), then the widest element
+    // span W over them (device)
     std::unique_ptr<int[]> local(new int[N]);
-    std::vector<int> inst_np(b->n_inst), inst_w(b->n_inst, 1), inst_ax(b->n_inst, 0);
+    std::vector<int> inst_np(n_inst), inst_w(n_inst, 1);
     parallel_instances([&](int i) {
-        const int n0 = b->h_node_off[i], n1 = b->h_node_off[i + 1];
+        const int n0 = b->h_node_off[i], n1 = b->h_node_off[i + 1], ax = inst_ax[i];
+        double emax;
+        std::memcpy(&emax, &emax_bits[i], sizeof(double));
         std::vector<int> pl;
-        inst_np[i] = instance_planes(X.get(), Y.get(), Z.get(), n0, n1, conn.get(), b->h_elem_off[i],
-                                     b->h_elem_off[i + 1], nn, (size_t)NE, pl, &inst_ax[i]);
+        inst_np[i] = instance_planes(cax.get(), bbox[6 * i + ax], bbox[6 * i + 3 + ax] - bbox[6 * i + ax], emax, n0, n1,
+                                     pl);
         for (int n = n0; n < n1; n++) local[n] = pl[n - n0];
-        for (int e = b->h_elem_off[i]; e < b->h_elem_off[i + 1]; e++) {
-            int mn = 1 << 30, mx = -1;
-            for (int a = 0; a < nn; a++) {
-                const int q = local[conn[e + (size_t)a * NE]];
-                mn = std::min(mn, q);
-                mx = std::max(mx, q);
-            }
-            inst_w[i] = std::max(inst_w[i], mx - mn);
-        }
     });
+    TLFEA_HIP_TRY(hipMemcpyAsync(d_local, local.get(), sizeof(int) * N, hipMemcpyHostToDevice, s));
+    TLFEA_HIP_TRY(hipMemsetAsync(t.span, 0, sizeof(int) * n_inst, s));
+    k_geom_elem_span<<<grid_for(NE, 256), 256, 0, s>>>(NE, nn, b->d_conn, b->d_elem_inst, d_local, t.span);
+    TLFEA_LAUNCH_CHECK();
+    std::vector<int> span(n_inst);
+    TLFEA_HIP_TRY(hipMemcpyAsync(span.data(), t.span, sizeof(int) * n_inst, hipMemcpyDeviceToHost, s));
+    TLFEA_HIP_TRY(hipStreamSynchronize(s));
+    for (int i = 0; i < n_inst; i++) inst_w[i] = std::max(1, span[i]);
     const int W = *std::max_element(inst_w.begin(), inst_w.end());
     if (W > 3) {
         set_error("coarse correction: elements span %d node planes (at most 3 supported)", W);
@@ -2053,18 +2200,16 @@ int coarse_geometry(tlfea_batch* b, hipStream_t s) {
     const int cm = (b->cm_req == CM_MAX && CM_MAX * W <= CNB_MAX) ? CM_MAX : 6;
     const int NB = cm * W;
     // global planes, padded per instance to a multiple of W
-    std::vector<int> inst_poff(b->n_inst + 1, 0), inst_pn(b->n_inst);
-    for (int i = 0; i < b->n_inst; i++) {
+    std::vector<int> inst_poff(n_inst + 1, 0), inst_pn(n_inst);
+    for (int i = 0; i < n_inst; i++) {
         inst_pn[i] = (inst_np[i] + W - 1) / W * W;
         inst_poff[i + 1] = inst_poff[i] + inst_pn[i];
     }
-    const int P = inst_poff[b->n_inst];
-    std::vector<int> plane_inst(P), plane_local(P), plane_axis(P), inst_cblk(b->n_inst + 1), cnt(P + 1, 0);
-    inst_cblk[b->n_inst] = P / W;
-    // free nodes per plane, centroid and size, node offsets r = (X - c) / size
+    const int P = inst_poff[n_inst];
+    std::vector<int> plane_inst(P), plane_local(P), plane_axis(P), inst_cblk(n_inst + 1), cnt(P + 1, 0);
+    inst_cblk[n_inst] = P / W;
+    // the free nodes of each plane (CSR)
     std::unique_ptr<int[]> node_plane(new int[N]);
-    std::unique_ptr<double[]> node_r(new double[3 * (size_t)N]);
-    std::vector<double> cen(3 * (size_t)P, 0.0), size(P, 0.0);
     parallel_instances([&](int i) {
         inst_cblk[i] = inst_poff[i] / W;
         for (int q = 0; q < inst_pn[i]; q++) {
@@ -2072,37 +2217,9 @@ int coarse_geometry(tlfea_batch* b, hipStream_t s) {
             plane_local[inst_poff[i] + q] = q;
             plane_axis[inst_poff[i] + q] = inst_ax[i];
         }
-        const int n0 = b->h_node_off[i], n1 = b->h_node_off[i + 1];
-        for (int n = n0; n < n1; n++) {
-            const bool fixed = mask[3 * (size_t)n] || mask[3 * (size_t)n + 1] || mask[3 * (size_t)n + 2];
-            node_plane[n] = fixed ? -1 : inst_poff[i] + local[n];
-            if (fixed) continue;
-            const int g = node_plane[n];
-            cnt[g + 1]++;
-            cen[3 * (size_t)g] += X[n];
-            cen[3 * (size_t)g + 1] += Y[n];
-            cen[3 * (size_t)g + 2] += Z[n];
-        }
-        for (int g = inst_poff[i]; g < inst_poff[i + 1]; g++)
-            if (cnt[g + 1] > 0)
-                for (int d = 0; d < 3; d++) cen[3 * (size_t)g + d] /= cnt[g + 1];
-        for (int n = n0; n < n1; n++) {
-            const int g = node_plane[n];
-            if (g < 0) continue;
-            const double r[3] = {X[n] - cen[3 * (size_t)g], Y[n] - cen[3 * (size_t)g + 1],
-                                 Z[n] - cen[3 * (size_t)g + 2]};
-            size[g] = std::max(size[g], std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]));
-        }
-        for (int n = n0; n < n1; n++) {
-            const int g = node_plane[n];
-            if (g < 0) {
-                node_r[3 * (size_t)n] = node_r[3 * (size_t)n + 1] = node_r[3 * (size_t)n + 2] = 0.0;
-                continue;
-            }
-            const double sc = size[g] > 0.0 ? 1.0 / size[g] : 1.0;
-            node_r[3 * (size_t)n] = (X[n] - cen[3 * (size_t)g]) * sc;
-            node_r[3 * (size_t)n + 1] = (Y[n] - cen[3 * (size_t)g + 1]) * sc;
-            node_r[3 * (size_t)n + 2] = (Z[n] - cen[3 * (size_t)g + 2]) * sc;
+        for (int n = b->h_node_off[i]; n < b->h_node_off[i + 1]; n++) {
+            node_plane[n] = fixed[n] ? -1 : inst_poff[i] + local[n];
+            if (node_plane[n] >= 0) cnt[node_plane[n] + 1]++;
         }
     });
     for (int g = 0; g < P; g++) cnt[g + 1] += cnt[g];
@@ -2129,27 +2246,32 @@ int coarse_geometry(tlfea_batch* b, hipStream_t s) {
         nullptr;
     b->d_node_r = b->d_cdiag = b->d_clow = b->d_cg = b->d_crhs = b->d_cx = nullptr;
     const size_t nblk = (size_t)P / W, bsz = (size_t)NB * NB;
-    int rc;
     int* d_inst_pn = nullptr;
     if ((rc = dalloc(&b->d_node_plane, N)) || (rc = dalloc(&b->d_node_r, 3 * (size_t)N)) ||
         (rc = dalloc(&b->d_plane_off, P + 1)) || (rc = dalloc(&b->d_plane_nodes, plane_nodes.size())) ||
         (rc = dalloc(&b->d_plane_inst, P)) || (rc = dalloc(&b->d_plane_local, P)) ||
-        (rc = dalloc(&b->d_inst_cblk, b->n_inst + 1)) || (rc = dalloc(&b->d_cdiag, nblk * bsz)) ||
+        (rc = dalloc(&b->d_inst_cblk, n_inst + 1)) || (rc = dalloc(&b->d_cdiag, nblk * bsz)) ||
         (rc = dalloc(&b->d_clow, nblk * bsz)) || (rc = dalloc(&b->d_cg, nblk * bsz)) ||
         (rc = dalloc(&b->d_crhs, cm * (size_t)P)) || (rc = dalloc(&b->d_cx, cm * (size_t)P)) ||
-        (rc = dalloc(&b->d_plane_axis, P)) ||
-        (rc = dalloc(&d_inst_pn, b->n_inst)))
+        (rc = dalloc(&b->d_plane_axis, P)) || (rc = dalloc(&d_inst_pn, n_inst)) || (rc = dalloc(&t.cen, 3 * (size_t)P)) ||
+        (rc = dalloc(&t.size, P)))
         return rc;
-    TLFEA_HIP_TRY(hipMemcpy(b->d_node_plane, node_plane.get(), sizeof(int) * N, hipMemcpyHostToDevice));
-    TLFEA_HIP_TRY(hipMemcpy(b->d_node_r, node_r.get(), sizeof(double) * 3 * (size_t)N, hipMemcpyHostToDevice));
-    TLFEA_HIP_TRY(hipMemcpy(b->d_plane_off, cnt.data(), sizeof(int) * (P + 1), hipMemcpyHostToDevice));
-    TLFEA_HIP_TRY(hipMemcpy(b->d_plane_nodes, plane_nodes.data(), sizeof(int) * plane_nodes.size(),
-                            hipMemcpyHostToDevice));
-    TLFEA_HIP_TRY(hipMemcpy(b->d_plane_inst, plane_inst.data(), sizeof(int) * P, hipMemcpyHostToDevice));
-    TLFEA_HIP_TRY(hipMemcpy(b->d_plane_local, plane_local.data(), sizeof(int) * P, hipMemcpyHostToDevice));
-    TLFEA_HIP_TRY(hipMemcpy(b->d_plane_axis, plane_axis.data(), sizeof(int) * P, hipMemcpyHostToDevice));
-    TLFEA_HIP_TRY(hipMemcpy(b->d_inst_cblk, inst_cblk.data(), sizeof(int) * (b->n_inst + 1), hipMemcpyHostToDevice));
-    TLFEA_HIP_TRY(hipMemcpy(d_inst_pn, inst_pn.data(), sizeof(int) * b->n_inst, hipMemcpyHostToDevice));
+    TLFEA_HIP_TRY(hipMemcpyAsync(b->d_node_plane, node_plane.get(), sizeof(int) * N, hipMemcpyHostToDevice, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(b->d_plane_off, cnt.data(), sizeof(int) * (P + 1), hipMemcpyHostToDevice, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(b->d_plane_nodes, plane_nodes.data(), sizeof(int) * plane_nodes.size(),
+                                 hipMemcpyHostToDevice, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(b->d_plane_inst, plane_inst.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(b->d_plane_local, plane_local.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(b->d_plane_axis, plane_axis.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+    TLFEA_HIP_TRY(
+        hipMemcpyAsync(b->d_inst_cblk, inst_cblk.data(), sizeof(int) * (n_inst + 1), hipMemcpyHostToDevice, s));
+    TLFEA_HIP_TRY(hipMemcpyAsync(d_inst_pn, inst_pn.data(), sizeof(int) * n_inst, hipMemcpyHostToDevice, s));
+    // plane centroids and sizes, then every node's scaled offset r = (X - c) / size
+    k_geom_plane_frame<<<P, 256, 0, s>>>(b->d_plane_off, b->d_plane_nodes, b->d_X, b->d_Y, b->d_Z, t.cen, t.size);
+    k_geom_node_r<<<grid_for(N, 256), 256, 0, s>>>(N, b->d_node_plane, b->d_X, b->d_Y, b->d_Z, t.cen, t.size,
+                                                   b->d_node_r);
+    TLFEA_LAUNCH_CHECK();
+    TLFEA_HIP_TRY(hipStreamSynchronize(s));
     b->cw = W;
     b->cm = cm;
     b->cnb = NB;
