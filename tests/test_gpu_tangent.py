@@ -315,6 +315,42 @@ def test_matfree_cg_against_direct(gpu, name):
     assert rel_err(_np(ls8.u), uo) < 1e-6
 
 
+@pytest.mark.parametrize("name", ["C1", "kuhn_t4"])
+def test_two_level_linear_static(gpu, name):
+    """LinearStatic(two_level=True), the Python mirror of the drop-in's
+    UseTwoLevelPreconditioner: at tol 1e-12 it matches the oracle's direct solve within
+    1e-10 (kappa * eps allows it on these beams) in a fifth of the matrix-free CG's
+    iterations or fewer, and a second solve from the moved positions (the operator K(x)
+    there) matches the matrix-free second solve."""
+    import torch
+    from tlfea_amd.fea import LinearStatic
+    m = load_mesh(name)
+    b = _body(m)
+    ls = LinearStatic(b, tol=1e-12, max_iter=200000, two_level=True, cheb_degree=8, cheb_ratio=300.0)
+    assert ls.solve() == 0, (ls.iters, ls.res)
+    go, djo = O.grad(m)
+    lam, mu = lame(E_STEEL, NU_STEEL)
+    doff, dcol = O.dof_pattern(m)
+    vo, fo = O.apply_dirichlet(doff, dcol, O.stiffness_values(m, go, djo, lam, mu, doff, dcol), m.f_ext, m.fixed)
+    assert rel_err(_np(ls.u), O.band_cholesky_solve(doff, dcol, vo, fo)) < 1e-10
+    it2 = ls.iters
+    assert ls.solve() == 0
+    u_tl = _np(ls.u).copy()
+    b2 = _body(m)
+    lm = LinearStatic(b2, tol=1e-12, max_iter=200000, matfree=True)
+    assert lm.solve() == 0
+    it_plain = lm.iters
+    assert lm.solve() == 0
+    print(f"[two-level LinearStatic] {name}: iterations {it2} vs {it_plain}", flush=True)
+    assert it2 * 5 <= it_plain
+    assert rel_err(u_tl, _np(lm.u)) < 1e-9
+    X = m.xyz()
+    moved = np.stack([_np(b.x[d]) - X[:, d] for d in range(3)], 1)
+    moved_mf = np.stack([_np(b2.x[d]) - X[:, d] for d in range(3)], 1)
+    assert rel_err(moved, moved_mf) < 1e-9
+    ls.close()
+
+
 def test_matfree_cg_iteration_limits(gpu):
     """Out of iterations the matrix-free CG reports ENOCONV with the relative residual of the
     returned u (tested after the loop); a tolerance just above that residual with the same

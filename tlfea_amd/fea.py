@@ -293,17 +293,26 @@ class LinearStatic:
     (`src/solvers/LinearStaticSolver.cu:469-481`)."""
 
     def __init__(self, body: Body, tol=1e-10, max_iter=10000, check_every=32, mixed=False, refresh=1e-2,
-                 matfree=False):
+                 matfree=False, two_level=False, cheb_degree=16, cheb_ratio=1000.0):
         """mixed=True: FP32 Krylov recursion with FP64 accumulation and reliable updates
         (tlfea_cg_solve_mixed); tol is then the true FP64 relative residual.
         matfree=True: FP64 CG with K(x) p applied element by element
         (tlfea_cg_solve_matfree); nothing is assembled.  The operator is the assembled
-        path's: K + C with Kelvin-Voigt damping."""
-        if mixed and matfree:
-            raise ValueError("mixed and matfree are exclusive")
+        path's: K + C with Kelvin-Voigt damping.
+        two_level=True (not in the reference): the flexible FP64 CG preconditioned by the
+        FP32 two-level stage (cheb_degree Chebyshev steps on [lambda_max / cheb_ratio,
+        lambda_max] plus the node-plane coarse correction, set up afresh every solve) on
+        K(x), through a one-instance batch of the body that follows its positions; the same
+        stopping rule.  SVK bodies without Kelvin-Voigt damping."""
+        if sum((mixed, matfree, two_level)) > 1:
+            raise ValueError("mixed, matfree and two_level are exclusive")
+        if two_level and (body.mat.model != 0 or body.mat.eta_damp != 0.0 or body.mat.lambda_damp != 0.0):
+            raise ValueError("the two-level path needs an SVK body without damping")
         self.b = body
         self.tol, self.max_iter, self.check_every = tol, max_iter, check_every
         self.mixed, self.refresh, self.matfree = mixed, refresh, matfree
+        self.two_level, self.cheb_degree, self.cheb_ratio = two_level, cheb_degree, cheb_ratio
+        self.batch, self._batch_key = None, None
         self.k32 = None
         self.refreshes = 0
         h = C.c_void_p()
@@ -315,7 +324,26 @@ class LinearStatic:
         self.iters = 0
         self.res = 0.0
 
+    def _batch(self):
+        """The body as a one-instance batch, rebuilt when its fixed set or material changes
+        (the batch copies both at creation)."""
+        from .batch import Batch
+        b = self.b
+        fixed = b.fixed.cpu().numpy()
+        key = (fixed.tobytes(), b.mat.lam, b.mat.mu)
+        if self.batch is None or key != self._batch_key:
+            if self.batch is not None:
+                self.batch.close()
+            m = b.mesh
+            mesh = type(m)(m.x, m.y, m.z, m.elems, fixed=fixed.astype(np.int32), f_ext=None)
+            self.batch = Batch([(mesh, dict(E=b.E, nu=b.nu))], stream=b.stream)
+            self._batch_key = key
+        return self.batch
+
     def assemble(self):
+        if self.two_level:
+            self.f = self.b.f_ext  # fixed DOFs are masked inside the solve
+            return
         if self.matfree:
             if self.b.plan is None:
                 raise RuntimeError("matrix-free CG needs a force plan")
@@ -341,6 +369,17 @@ class LinearStatic:
                                                  int(self.check_every), C.byref(it), C.byref(res), C.byref(nref),
                                                  self.b.s)
             self.refreshes = nref.value
+        elif self.two_level:
+            b = self.b
+            bt = self._batch()
+            u = torch.stack([b.x[d] - b.X0[d] for d in range(3)], 1).reshape(-1)
+            bt.set_displacement(u if bool(torch.any(u != 0)) else None)
+            bt.precond_setup(degree=self.cheb_degree, ratio=self.cheb_ratio)
+            bt.coarse_setup()
+            bt.solve(tol=float(self.tol), max_iter=int(self.max_iter), f=self.f, out=self.u, precond=True,
+                     graph=False)
+            self.iters, self.res = int(bt.iters[0]), float(bt.rel[0])
+            return int(bt.status[0])
         elif self.matfree:
             b = self.b
             mat = b.mat
@@ -371,6 +410,9 @@ class LinearStatic:
         if self.cg is not None:
             capi.lib().tlfea_cg_destroy(self.cg)
             self.cg = None
+        if getattr(self, "batch", None) is not None:
+            self.batch.close()
+            self.batch = None
 
     def __del__(self):
         try:
