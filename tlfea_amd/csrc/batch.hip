@@ -802,33 +802,63 @@ __global__ void k_geom_axis(int n_nodes, const int* __restrict__ node_inst, cons
     fixed[n] = (mask[3 * (int64_t)n] | mask[3 * (int64_t)n + 1] | mask[3 * (int64_t)n + 2]) ? 1 : 0;
 }
 
+// The per-instance maximum of a per-element value v (>= 0): a wave whose elements all
+// belong to one instance reduces first and adds one atomic (a large instance would
+// otherwise serialise all its elements' atomics on one address); other waves, at an
+// instance boundary or the end, take one atomic per element.
+template <typename T>
+__device__ __forceinline__ void inst_max(T* __restrict__ out, int inst, T v) {
+    const int i0 = __shfl(inst, 0, 64);
+    if (__all(inst == i0)) {
+        for (int o = 32; o > 0; o >>= 1) {
+            const T w = __shfl_xor(v, o, 64);
+            v = w > v ? w : v;
+        }
+        if ((threadIdx.x & 63) == 0) atomicMax(&out[i0], v);
+    } else if (inst >= 0) {
+        atomicMax(&out[inst], v);
+    }
+}
+
 // per element: its extent along its instance's axis; the maximum per instance by an atomic
 // max of the bit patterns (non-negative doubles order as their patterns)
-__global__ void k_geom_elem_extent(int n_elem, int nn, const int* __restrict__ conn, const int* __restrict__ elem_inst,
-                                   const double* __restrict__ cax, unsigned long long* __restrict__ emax) {
+__global__ __launch_bounds__(256) void k_geom_elem_extent(int n_elem, int nn, const int* __restrict__ conn,
+                                                          const int* __restrict__ elem_inst,
+                                                          const double* __restrict__ cax,
+                                                          unsigned long long* __restrict__ emax) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n_elem) return;
-    double lo = 1e300, hi = -1e300;
-    for (int a = 0; a < nn; a++) {
-        const double v = cax[conn[e + (int64_t)a * n_elem]];
-        lo = fmin(lo, v);
-        hi = fmax(hi, v);
+    unsigned long long bits = 0;
+    int inst = -1;
+    if (e < n_elem) {
+        double lo = 1e300, hi = -1e300;
+        for (int a = 0; a < nn; a++) {
+            const double v = cax[conn[e + (int64_t)a * n_elem]];
+            lo = fmin(lo, v);
+            hi = fmax(hi, v);
+        }
+        bits = (unsigned long long)__double_as_longlong(fmax(hi - lo, 0.0));
+        inst = elem_inst[e];
     }
-    atomicMax(&emax[elem_inst[e]], (unsigned long long)__double_as_longlong(fmax(hi - lo, 0.0)));
+    inst_max(emax, inst, bits);
 }
 
 // per element: the span of its nodes' local planes; the maximum per instance
-__global__ void k_geom_elem_span(int n_elem, int nn, const int* __restrict__ conn, const int* __restrict__ elem_inst,
-                                 const int* __restrict__ local, int* __restrict__ span) {
+__global__ __launch_bounds__(256) void k_geom_elem_span(int n_elem, int nn, const int* __restrict__ conn,
+                                                        const int* __restrict__ elem_inst,
+                                                        const int* __restrict__ local, int* __restrict__ span) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n_elem) return;
-    int lo = 1 << 30, hi = -1;
-    for (int a = 0; a < nn; a++) {
-        const int q = local[conn[e + (int64_t)a * n_elem]];
-        lo = min(lo, q);
-        hi = max(hi, q);
+    int d = 0, inst = -1;
+    if (e < n_elem) {
+        int lo = 1 << 30, hi = -1;
+        for (int a = 0; a < nn; a++) {
+            const int q = local[conn[e + (int64_t)a * n_elem]];
+            lo = min(lo, q);
+            hi = max(hi, q);
+        }
+        d = hi - lo;
+        inst = elem_inst[e];
     }
-    atomicMax(&span[elem_inst[e]], hi - lo);
+    inst_max(span, inst, d);
 }
 
 // per plane, one workgroup of 256: the centroid c of its free nodes and their largest
