@@ -152,10 +152,11 @@ def batched_beams(n, tol, log, sequential=True):
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t
     # the autotuner's replay length is chosen for the plain CG's thousands of iterations; the
-    # two-level solves finish in 50-140, so they poll every 16
-    u2 = timed("fp32_twolevel", lambda: b.solve(tol=tol, precond=True, poll=16))
+    # two-level solves finish in 50-140, so they poll every 8 (99 ms against 102 at 16 and
+    # 108 at 32 on these beams)
+    u2 = timed("fp32_twolevel", lambda: b.solve(tol=tol, precond=True, poll=8))
     u2w = timed("fp32_twolevel_learned_start",
-                lambda: b.solve(tol=tol, precond=True, poll=16, u0=net.guess(b, graph=g)))
+                lambda: b.solve(tol=tol, precond=True, poll=8, u0=net.guess(b, graph=g)))
     for k in ("fp32_twolevel", "fp32_twolevel_learned_start"):
         ok = res[k]["converged"]
         res[k]["setup_s"] = setup_s
