@@ -36,8 +36,8 @@ class LinearStaticSolver : public SolverBase {
     // [lambda_max / ratio, lambda_max], and tlfea_batch_coarse_setup) on K(x), set up afresh
     // for every Solve, with the same stopping rule sqrt(r.r / r0.r0) < tol: the same solution
     // in a small fraction of the iterations on slender bodies (80 vs 1,226 on the 20x2x2-cell
-    // beam, 102 vs 3,740 on the reference's beam_highres_t10 mesh; 0.29 s then 0.09 s per
-    // Solve on a 2M-DOF beam against 0.64 and 0.85 s).  The first Solve also builds the
+    // beam, 102 vs 3,740 on the reference's beam_highres_t10 mesh; 0.26 s then 0.09 s per
+    // Solve on a 2M-DOF beam against 0.63 and 0.85 s).  The first Solve also builds the
     // one-instance batch (its force plan and coarse geometry).  SVK bodies
     // without Kelvin-Voigt damping whose rest positions are known (CalcDnDuPre); others keep
     // the plain CG.
